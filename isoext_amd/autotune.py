@@ -1,0 +1,94 @@
+"""Tile-shape selection for the fused marching-cubes kernel.
+
+The library compiles a handful of tile configurations (TX x TY x 64*NZC
+cells per workgroup).  ``mc_config(shape)`` returns the configuration to use
+for a grid shape: a measured choice from the tuning table if one exists for
+the shape's size class, else a static rule.  ``tune(shape)`` times every
+configuration on the current GPU with HIP events and records the fastest in
+``autotune.json`` next to this file.
+"""
+
+from __future__ import annotations
+
+import json
+import threading
+from pathlib import Path
+
+_HERE = Path(__file__).resolve().parent
+TABLE = _HERE / "autotune.json"
+_lock = threading.Lock()
+_table = None
+
+# (TX, TY, NZC) of each compiled configuration, in library order
+CONFIGS = [(16, 16, 4), (32, 16, 4), (16, 16, 2), (8, 8, 4), (32, 32, 4), (16, 8, 4)]
+
+
+def size_class(shape) -> str:
+    X, Y, Z = shape
+    n = X * Y * Z
+    zc = "z64" if Z <= 65 else ("z128" if Z <= 129 else "zbig")
+    if n <= 1 << 18:
+        nc = "tiny"
+    elif n <= 1 << 21:
+        nc = "small"
+    elif n <= 1 << 24:
+        nc = "medium"
+    elif n <= 1 << 27:
+        nc = "large"
+    else:
+        nc = "huge"
+    return f"{nc}-{zc}"
+
+
+def _load():
+    global _table
+    if _table is None:
+        try:
+            _table = json.loads(TABLE.read_text())
+        except (OSError, ValueError):
+            _table = {}
+    return _table
+
+
+def mc_config(shape) -> int:
+    with _lock:
+        t = _load()
+        hit = t.get(size_class(shape))
+    if hit is not None:
+        return int(hit)
+    X, Y, Z = shape
+    if Z <= 129:
+        return 2 if Z > 65 else 3
+    return 0
+
+
+def tune(shape, values=None, level=0.0, reps=5, write=True):
+    """Time each configuration on `values` (or a synthetic sphere field)."""
+    import torch
+
+    from .mc import mc_dense_raw
+
+    if values is None:
+        X, Y, Z = shape
+        axes = [torch.linspace(-1, 1, n, device="cuda") for n in shape]
+        gx, gy, gz = torch.meshgrid(*axes, indexing="ij")
+        values = (gx * gx + gy * gy + gz * gz).sqrt() - 0.6
+    results = {}
+    for cfg in range(len(CONFIGS)):
+        mc_dense_raw(values, shape, (-1, -1, -1), (1, 1, 1), level, 0, cfg)
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        times = []
+        for _ in range(reps):
+            t0.record()
+            mc_dense_raw(values, shape, (-1, -1, -1), (1, 1, 1), level, 0, cfg)
+            t1.record()
+            t1.synchronize()
+            times.append(t0.elapsed_time(t1))
+        results[cfg] = sorted(times)[len(times) // 2]
+    best = min(results, key=results.get)
+    if write:
+        with _lock:
+            t = _load()
+            t[size_class(shape)] = best
+            TABLE.write_text(json.dumps(t, indent=1, sort_keys=True) + "\n")
+    return best, results

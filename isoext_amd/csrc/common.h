@@ -1,0 +1,116 @@
+// Shared device/host helpers for the iso-surface kernels (gfx950).
+//
+// Float semantics follow the reference exactly, with every operation
+// uncontracted (the whole library is built with -ffp-contract=off):
+//   grid coordinate  get_vtx_pos_op   include/utils.cuh:62-80
+//   case bits        get_case_num_op  include/utils.cuh:91-101
+//   edge parameter   process_cube_op  src/mc/nagae.cu:55-57
+//   lerp                              include/math.cuh:77-81
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ISO_HD __host__ __device__ __forceinline__
+
+namespace iso {
+
+// Corner k of a cell sits at (k >> 2, (k >> 1) & 1, k & 1); edge e joins
+// corners kEdgeA[e] < kEdgeB[e] (include/shared_luts.cuh:34-38).
+// Edge axis: 0 = x, 1 = y, 2 = z.  Owner corner = the lower endpoint.
+static __constant__ const uint8_t kEdgeA[12] = {0, 1, 2, 0, 4, 5, 6, 4, 0, 1, 3, 2};
+static __constant__ const uint8_t kEdgeB[12] = {1, 3, 3, 2, 5, 7, 7, 6, 4, 5, 7, 6};
+static __constant__ const uint8_t kEdgeAxis[12] = {2, 1, 2, 1, 2, 1, 2, 1, 0, 0, 0, 0};
+
+ISO_HD int32_t f2i_bits(float f) {
+    union {
+        float f;
+        int32_t i;
+    } u;
+    u.f = f;
+    return u.i;
+}
+
+ISO_HD float grid_coord(int64_t i, int64_t res, float amin, float amax) {
+    float size = amax - amin;
+    float p = (float) i / (float) (res - 1);
+    float prod = p * size;
+    return amin + prod;
+}
+
+ISO_HD float lerp1(float t, float a, float b) {
+    float s = 1.0f - t;
+    float u = s * a;
+    float v = t * b;
+    return u + v;
+}
+
+// Interpolation parameter of a crossing edge.  NaN (only from NaN/inf
+// inputs) is mapped to 0 so a vertex never carries NaN.
+ISO_HD float edge_t(float v0, float v1, float level) {
+    float denom = v1 - v0;
+    float t = (denom != 0.0f) ? (level - v0) / denom : 0.0f;
+    if (t != t) t = 0.0f;
+    return t;
+}
+
+ISO_HD bool inside(float v, float level) { return (v - level) < 0.0f; }
+
+// Order-preserving map of a float to an integer (ulp steps between two
+// floats = difference of their keys; -0 and +0 map to the same key).
+ISO_HD int32_t ordered_key(float f) {
+    int32_t b = f2i_bits(f);
+    return b >= 0 ? b : (int32_t) (0x80000000u - (uint32_t) b);
+}
+
+// A vertex on an edge may coincide bitwise with a vertex of another edge only
+// when its along-axis coordinate r is within a few ulps of an endpoint
+// coordinate (the off-axis coordinates of a lerp between equal values stay
+// within 3 ulps of the grid line).  16 ulp steps is a safe margin.
+ISO_HD bool near_endpoint(float r, float a, float b) {
+    int64_t kr = ordered_key(r), ka = ordered_key(a), kb = ordered_key(b);
+    int64_t da = kr - ka, db = kr - kb;
+    da = da < 0 ? -da : da;
+    db = db < 0 ? -db : db;
+    return da <= 16 || db <= 16;
+}
+
+ISO_HD uint32_t f3_hash(float x, float y, float z) {
+    // -0 and +0 compare equal, so hash their shared ordered key.
+    uint32_t h = 2166136261u;
+    uint32_t k[3] = {(uint32_t) ordered_key(x), (uint32_t) ordered_key(y),
+                     (uint32_t) ordered_key(z)};
+    for (int i = 0; i < 3; ++i) {
+        h ^= k[i];
+        h *= 16777619u;
+        h ^= h >> 15;
+    }
+    return h;
+}
+
+ISO_HD uint64_t mix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+// Agent-scope relaxed atomics: every cross-workgroup word carries its own
+// payload, so no fences are needed for it to be read consistently.
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t ld_agent32(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded spin: after this many polls a wait gives up and raises an error
+// flag instead of hanging the device.
+constexpr int kSpinLimit = 1 << 22;
+
+}   // namespace iso

@@ -1,0 +1,122 @@
+"""marching_cubes -- drop-in for isoext.marching_cubes.
+
+Reference: the nanobind binding src/isoext_ext.cu:95-109 over
+mc::marching_cubes (src/mc/mc.cu:17-68) and the variant registry
+(include/mc/base.cuh:12-41, src/mc/base.cu:19-27: an unknown name raises
+RuntimeError("Unknown method: <name>")).  Returns (vertices (V, 3) float32,
+faces (F, 3) int32) on the grid's device, or (None, None) for an empty mesh
+(src/isoext_ext.cu:47-49).
+
+Vertex order differs from the reference (which sorts positions
+lexicographically): vertices come in grid-edge order, faces in tile/cell
+order.  The vertex set, the face set and every face's corner order are the
+reference's (compare after canonical ordering).
+"""
+
+from __future__ import annotations
+
+import threading
+
+import torch
+
+from . import _lib, autotune, workspace
+from .grid import Grid, UniformGrid
+
+METHODS = {"nagae": 0, "lorensen": 1}
+
+# provisional output sizes seen last time for a grid shape (sizing hint)
+_size_hint: dict = {}
+_hint_lock = threading.Lock()
+DEFAULT_SPECIAL_CAP = 1 << 16
+
+
+def _method_id(method: str) -> int:
+    if method not in METHODS:
+        raise RuntimeError(f"Unknown method: {method}")
+    return METHODS[method]
+
+
+def _initial_caps(shape):
+    X, Y, Z = shape
+    with _hint_lock:
+        hint = _size_hint.get(tuple(shape))
+    if hint is not None:
+        return hint
+    # surface-like estimate: a few times the largest cross-section, at least 64K
+    cross = max(X * Y, Y * Z, X * Z)
+    nv = max(1 << 16, 4 * cross)
+    return nv, 2 * nv + 1024
+
+
+def mc_dense_raw(values: torch.Tensor, shape, aabb_min, aabb_max, level: float, method: int,
+                 cfg: int | None = None):
+    """Run dense marching cubes on a (X, Y, Z) float32 CUDA tensor.
+
+    Returns (v, f, info) with v (V, 3) float32 and f (F, 3) int32 tensors
+    (possibly zero rows) and an info dict (provisional counts, tile config).
+    """
+    dev = values.device
+    shape = [int(s) for s in shape]
+    if cfg is None:
+        cfg = autotune.mc_config(shape)
+    L = _lib.lib()
+    special_cap = DEFAULT_SPECIAL_CAP
+    cap_v, cap_f = _initial_caps(shape)
+    counts = _lib.I64x4()
+    for attempt in range(4):
+        ws_bytes = L.isoext_mc_dense_workspace_bytes(_lib.i64x3(shape), cfg, special_cap)
+        if ws_bytes < 0:
+            raise ValueError(f"invalid tile config {cfg}")
+        ws = workspace.get("mc_dense", ws_bytes, dev)
+        out_v = torch.empty((cap_v, 3), dtype=torch.float32, device=dev)
+        out_f = torch.empty((cap_f, 3), dtype=torch.int32, device=dev)
+        with torch.cuda.device(dev):
+            rc = L.isoext_mc_dense(values.data_ptr(), _lib.i64x3(shape), _lib.f32x3(aabb_min),
+                                   _lib.f32x3(aabb_max), float(level), int(method), int(cfg),
+                                   ws.data_ptr(), ws.numel(), special_cap, out_v.data_ptr(), cap_v,
+                                   out_f.data_ptr(), cap_f, counts, _lib.stream_ptr(dev))
+        if rc == _lib.E_CAPACITY:
+            cap_v, cap_f = int(counts[0]) + 1, int(counts[1]) + 1
+            continue
+        if rc == _lib.E_WORKSPACE and counts[2] > special_cap:
+            special_cap = int(counts[2]) + 1024
+            continue
+        _lib.check(rc, "marching_cubes")
+        break
+    else:
+        raise _lib.IsoextError("marching_cubes: output sizing did not converge")
+    with _hint_lock:
+        _size_hint[tuple(shape)] = (max(int(counts[0]), 1), max(int(counts[1]), 1))
+    nv, nf = int(counts[0]), int(counts[1])
+    v, f = out_v[:nv], out_f[:nf]
+    # do not pin a much larger allocation behind a small mesh
+    if cap_v > 2 * nv + (1 << 16):
+        v = v.clone()
+    if cap_f > 2 * nf + (1 << 16):
+        f = f.clone()
+    return v, f, {"cfg": cfg, "special": int(counts[2]), "err": int(counts[3])}
+
+
+def marching_cubes(grid: Grid, level: float = 0.0, method: str = "nagae"):
+    """Extract an iso-surface with Marching Cubes.
+
+    Args:
+        grid: a UniformGrid or SparseGrid.
+        level: iso-value (default 0.0).
+        method: "nagae" (default) or "lorensen".
+
+    Returns:
+        (vertices (N, 3) float32, faces (M, 3) int32), or (None, None) when
+        no surface crosses the grid.
+    """
+    m = _method_id(method)
+    if isinstance(grid, UniformGrid):
+        v, f, _ = mc_dense_raw(grid.values, grid.shape, grid.aabb_min, grid.aabb_max, level, m)
+    else:
+        from .sparse import SparseGrid, mc_sparse
+        if not isinstance(grid, SparseGrid):
+            raise TypeError("grid must be a UniformGrid or SparseGrid")
+        v, f = mc_sparse(grid, level, m)
+    if v is None or f is None or f.shape[0] == 0:
+        return None, None
+    return v, f
