@@ -1,0 +1,158 @@
+"""Benchmark: plain marching cubes on a dense 1024^3 fp32 UniformGrid.
+
+Workload (the reference metric's configuration): SDF = min over 64 spheres
+(centres U[-0.8, 0.8]^3, radii U[0.05, 0.25], seed 0) on [-1, 1]^3, iso 0,
+method "nagae".  Values are generated on the GPU before timing; a step is one
+isoext_amd.marching_cubes call (kernel + output sizing + host sync).
+
+Prints one JSON line: Gvoxels/s (grid points / step time), the roofline of
+the fused kernel (algorithmic bytes 4*N_points + 12*V + 12*F over its
+HIP-event duration, against 8 TB/s), and the CPU oracle timed on a bounded
+slab of the same field on this host.
+
+    python bench.py [--steps K] [--warmup W] [--size N] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "MC Gvoxels/s & % HBM peak at 1024³ fp32; serving p50/p99 vs per-call plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--spheres", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--method", default="nagae")
+    ap.add_argument("--cfg", type=int, default=None, help="tile configuration (default: autotuned)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-planes", type=int, default=129, help="x-planes in the CPU sample")
+    return ap.parse_args()
+
+
+def cpu_baseline(values_gpu, planes: int, level: float, method: str):
+    """The CPU oracle (C++ restatement of the reference, OpenMP) on the first
+    `planes` x-planes of the same field; returns Gvoxels/s and details."""
+    import numpy as np
+
+    import oracle
+    sample = values_gpu[:planes].contiguous().cpu().numpy()
+    threads = min(16, os.cpu_count() or 1)
+    oracle.mc_dense(sample[:9], level, method, threads=threads)   # warm-up (load, page-in)
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.mc_dense(sample, level, method, threads=threads)
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[1]
+    pts = int(np.prod(sample.shape))
+    return {"value": pts / t / 1e9, "unit": "Gvoxels/s", "cores": threads, "kind": "port",
+            "sample": f"oracle marching cubes ({method}) on the first {planes} x-planes "
+                      f"({sample.shape[0]}x{sample.shape[1]}x{sample.shape[2]} points) of the same "
+                      f"field, median of 3 runs, {t:.2f} s each"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(every box has one MI355X).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import isoext_amd as iso
+    from isoext_amd import _lib, workloads
+    from isoext_amd.mc import METHODS, mc_dense_raw
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    N = args.size
+    shape = (N, N, N)
+    grid = iso.UniformGrid(list(shape))
+    workloads.multi_sphere(shape, args.spheres, args.seed, out=grid.values)
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    m = METHODS[args.method]
+
+    L.isoext_set_timing(1)
+    kernel_ms = []
+    nv = nf = 0
+    cfg = None
+    for _ in range(args.warmup):
+        v, f, info = mc_dense_raw(grid.values, shape, grid.aabb_min, grid.aabb_max, 0.0, m, args.cfg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        v, f, info = mc_dense_raw(grid.values, shape, grid.aabb_min, grid.aabb_max, 0.0, m, args.cfg)
+        kernel_ms.append(L.isoext_last_kernel_ms())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.isoext_set_timing(0)
+    nv, nf, cfg = v.shape[0], f.shape[0], info["cfg"]
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    npts = N * N * N
+    kms = sorted(kernel_ms)[len(kernel_ms) // 2]
+    kmean = sum(kernel_ms) / len(kernel_ms)
+    alg_bytes = 4 * npts + 12 * nv + 12 * nf
+    achieved = alg_bytes / (kmean * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "mc_dense_1024_traffic.json")
+    if os.path.exists(tfile) and N == 1024:
+        try:
+            traffic = json.load(open(tfile)).get("traffic_bytes")
+        except (OSError, ValueError):
+            traffic = None
+    out = {
+        "metric": METRIC,
+        "value": npts / (ms_step * 1e-3) / 1e9,
+        "unit": "Gvoxels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain marching cubes ({args.method}) on a dense {N}^3 fp32 UniformGrid over "
+                        f"[-1,1]^3, SDF = min of {args.spheres} seeded spheres (seed {args.seed}), iso 0",
+            "grid_points": [N, N, N],
+            "vertices": nv,
+            "faces": nf,
+            "tile_config": cfg,
+            "kernel_ms_median": kms,
+            "kernel_ms_mean": kmean,
+            "cell_rate_gcells_s": (N - 1) ** 3 / (ms_step * 1e-3) / 1e9,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(grid.values, min(args.cpu_planes, N), 0.0, args.method)
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -38,6 +38,12 @@ enum {
 const char *isoext_last_error(void);
 int isoext_version(void);
 
+/* Instrumentation (no reference counterpart): when enabled, extraction calls
+ * bracket their main kernel with HIP events on the caller's stream and
+ * isoext_last_kernel_ms() returns its duration for the calling thread. */
+void isoext_set_timing(int on);
+float isoext_last_kernel_ms(void);
+
 /* ---- grids ---------------------------------------------------------------- */
 
 /* UniformGrid.get_points (src/grid/uniform.cu:22-30, bound at
@@ -73,6 +79,125 @@ int isoext_mc_dense(const float *values, const int64_t shape[3], const float aab
                     void *workspace, int64_t workspace_bytes, int64_t special_cap,
                     float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f,
                     int64_t counts[4], void *stream);
+
+/* ---- grid descriptor used by intersections, dual contouring, sparse MC ---- */
+
+/* A UniformGrid (sparse_cell_ids == NULL: values is (X, Y, Z)) or a
+ * SparseGrid (values is (n_sparse, 8) per-cell corner values and
+ * sparse_cell_ids the n_sparse sorted, unique linear cell ids over
+ * (X-1, Y-1, Z-1); src/grid/sparse.cu:21-69). */
+typedef struct isoext_grid {
+    const float *values;
+    const int64_t *sparse_cell_ids;
+    int64_t n_sparse;
+    int64_t shape[3];
+    float aabb_min[3];
+    float aabb_max[3];
+} isoext_grid;
+
+/* An intersection set in the reference's entry order (src/its.cu:93-159,
+ * include/its.cuh:6-26): active cells ascending; per cell its crossing edges
+ * in edge-index order.  cell_indices is the linear cell id (UniformGrid) or
+ * the sparse slot (SparseGrid); cell_dense is always the linear cell id. */
+typedef struct isoext_its {
+    int64_t ncells, npts;
+    const int64_t *cell_indices;
+    const int64_t *cell_dense;
+    const int64_t *cell_offsets; /* ncells + 1 */
+    const float *points;         /* npts x 3 */
+    const float *normals;        /* npts x 3 */
+    const uint8_t *edge_local;   /* edge index 0..11 inside its cell */
+    const uint8_t *is_out;       /* value at the lower corner <= value at the upper */
+    const int32_t *entry_cell;   /* active-cell index of each entry */
+} isoext_its;
+
+/* Scratch bytes shared by isoext_its_count / isoext_its_emit for `g`. */
+int64_t isoext_its_scratch_bytes(const isoext_grid *g);
+
+/* get_intersection pass 1 (src/its.cu:105-135): counts[0] = active cells,
+ * counts[1] = intersection points.  Synchronises `stream`. */
+int isoext_its_count(const isoext_grid *g, float level, void *scratch, int64_t scratch_bytes,
+                     int64_t counts[2], void *stream);
+
+/* get_intersection pass 2 (get_its_op src/its.cu:37-90, plus
+ * compute_normals_op src/its.cu:186-268 when normals != NULL), using the
+ * scratch of the preceding isoext_its_count on the same grid and level. */
+int isoext_its_emit(const isoext_grid *g, float level, const void *scratch, int64_t ncells,
+                    int64_t npts, int64_t *cell_indices, int64_t *cell_dense, int64_t *cell_offsets,
+                    float *points, float *normals, uint8_t *edge_local, uint8_t *is_out,
+                    int32_t *entry_cell, void *stream);
+
+/* compute_intersection_normals (src/its.cu:271-284) for an existing set. */
+int isoext_its_normals(const isoext_grid *g, const isoext_its *its, float *normals, void *stream);
+
+/* Workspace bytes for isoext_dc on an intersection set of this size. */
+int64_t isoext_dc_workspace_bytes(int64_t ncells, int64_t npts);
+
+/* dual_contouring(grid, level, intersection, reg, svd_tol) -- replaces
+ * ::dual_contouring (include/dc.cuh:9-11, src/dc.cu:161-218) with the
+ * BatchedLASolver (src/batched_la.cu:151-178) and get_dual_quads
+ * (src/grid/uniform.cu:60-83, src/grid/sparse.cu:181-246), bound at
+ * src/isoext_ext.cu:345-378.  Needs cap_v >= ncells and cap_f >= 2*npts.
+ * counts = {vertices, faces, cells re-solved in FP64, quads}.  Synchronises. */
+int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_tol, void *workspace,
+              int64_t workspace_bytes, float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f,
+              float *dual_v, int64_t counts[4], void *stream);
+
+/* Same as isoext_mc_dense, also writing each vertex's grid edge to out_key
+ * (owner point linear id * 3 + axis, 0 = x) for the backward pass.  No
+ * reference counterpart: the reference has no autograd. */
+int isoext_mc_dense_keyed(const float *values, const int64_t shape[3], const float aabb_min[3],
+                          const float aabb_max[3], float level, int method, int cfg,
+                          void *workspace, int64_t workspace_bytes, int64_t special_cap,
+                          float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f,
+                          int64_t *out_key, int64_t counts[4], void *stream);
+
+/* Backward of marching cubes (no reference counterpart): accumulates
+ * dL/dvalues += dL/dv * dv/dvalues through each vertex's edge interpolation.
+ * Dense: keys from isoext_mc_dense_keyed, grad_values is (X, Y, Z).
+ * Sparse: vsrc from isoext_mc_sparse, grad_values is (n_sparse, 8). */
+int isoext_mc_backward(const isoext_grid *g, float level, const float *grad_v, const int64_t *vertex_src,
+                       int64_t nv, float *grad_values, void *stream);
+
+/* Benchmark input generation (no reference counterpart; the reference's SDFs
+ * are torch code, src/isoext/sdf.py): out[x,y,z] = min_k |p - c_k| - r_k over
+ * k <= 256 spheres given as device float4 (cx, cy, cz, r). */
+int isoext_sdf_spheres(const int64_t shape[3], const float aabb_min[3], const float aabb_max[3],
+                       const float *spheres, int k, float *out, void *stream);
+
+/* ---- SparseGrid ------------------------------------------------------------ */
+
+/* SparseGrid.get_points / get_points_by_cell_indices
+ * (src/grid/sparse.cu:21-36,144-148): (n, 8, 3) corner positions of cells
+ * given by linear cell id over (X-1, Y-1, Z-1). */
+int isoext_sparse_corner_points(const int64_t *cell_ids, int64_t n, const int64_t shape[3],
+                                const float aabb_min[3], const float aabb_max[3], float *out,
+                                void *stream);
+
+/* SparseGrid.filter_cell_indices (src/grid/sparse.cu:150-179): flags[i] = 1
+ * when cell i's 8 values (values is (n, 8)) straddle `level`. */
+int isoext_sparse_crossing_flags(const float *values, int64_t n, float level, uint8_t *flags,
+                                 void *stream);
+
+/* Workspace bytes for isoext_mc_sparse on n cells with `ninst` crossing-edge
+ * instances (from isoext_mc_sparse_count). */
+int64_t isoext_mc_sparse_workspace_bytes(int64_t n, int64_t ninst);
+
+/* Pass 1 of marching_cubes on a SparseGrid: number of crossing-edge
+ * instances.  scratch needs 8 * (n / 256 + 16) bytes.  Synchronises. */
+int isoext_mc_sparse_count(const isoext_grid *g, float level, int64_t *scratch, int64_t *ninst,
+                           void *stream);
+
+/* marching_cubes(grid: SparseGrid, level, method) -- replaces
+ * mc::marching_cubes (src/mc/mc.cu:17-68) over SparseGrid::get_cells
+ * (src/grid/sparse.cu:61-69), bound at src/isoext_ext.cu:95-109.  Faces in
+ * cell then LUT-slot order; vertices welded by exact position.  vsrc
+ * (optional) receives slot * 12 + edge of each vertex.  counts = {vertices,
+ * faces}; ISOEXT_E_CAPACITY when cap_v/cap_f are too small (counts hold the
+ * need).  Synchronises. */
+int isoext_mc_sparse(const isoext_grid *g, float level, int method, int64_t ninst, void *workspace,
+                     int64_t workspace_bytes, float *out_v, int64_t cap_v, int32_t *out_f,
+                     int64_t cap_f, int64_t *vsrc, int64_t counts[2], void *stream);
 
 #ifdef __cplusplus
 }

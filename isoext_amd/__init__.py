@@ -28,13 +28,7 @@ __all__ = [
     "write_obj",
 ]
 
-try:  # components built on top of the plain path
-    from .sparse import SparseGrid  # noqa: F401
-    __all__.append("SparseGrid")
-except ImportError:  # pragma: no cover - during incremental builds
-    pass
-try:
-    from .dc import Intersection, dual_contouring, get_intersection  # noqa: F401
-    __all__ += ["Intersection", "dual_contouring", "get_intersection"]
-except ImportError:  # pragma: no cover
-    pass
+from .dc import Intersection, dual_contouring, get_intersection  # noqa: E402
+from .sparse import SparseGrid  # noqa: E402
+
+__all__ += ["Intersection", "SparseGrid", "dual_contouring", "get_intersection"]

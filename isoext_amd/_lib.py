@@ -42,11 +42,18 @@ def _declare(L):
     L.isoext_last_error.argtypes = []
     L.isoext_version.restype = c_int
     L.isoext_mc_config_count.restype = c_int
+    L.isoext_set_timing.restype = None
+    L.isoext_set_timing.argtypes = [c_int]
+    L.isoext_last_kernel_ms.restype = c_f
+    L.isoext_last_kernel_ms.argtypes = []
     L.isoext_mc_dense_workspace_bytes.restype = c_i64
     L.isoext_mc_dense_workspace_bytes.argtypes = [I64x3, c_int, c_i64]
     L.isoext_mc_dense.restype = c_int
     L.isoext_mc_dense.argtypes = [vp, I64x3, F32x3, F32x3, c_f, c_int, c_int, vp, c_i64, c_i64,
                                   vp, c_i64, vp, c_i64, I64x4, vp]
+    L.isoext_mc_dense_keyed.restype = c_int
+    L.isoext_mc_dense_keyed.argtypes = [vp, I64x3, F32x3, F32x3, c_f, c_int, c_int, vp, c_i64, c_i64,
+                                        vp, c_i64, vp, c_i64, vp, I64x4, vp]
     L.isoext_uniform_points.restype = c_int
     L.isoext_uniform_points.argtypes = [I64x3, F32x3, F32x3, vp, vp]
     L.isoext_fill_f32.restype = c_int

@@ -22,7 +22,7 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libisoext_amd.so"
 ARCH = os.environ.get("ISOEXT_ARCH", "gfx950")
 
-SOURCES = ["capi.hip", "mc_dense.hip", "fixup.hip"]
+SOURCES = ["capi.hip", "mc_dense.hip", "fixup.hip", "its_dc.hip", "sparse.hip", "backward.hip", "sdf.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

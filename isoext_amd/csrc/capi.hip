@@ -13,11 +13,18 @@
 #include "../../include/isoext_amd.h"
 #include "common.h"
 #include "fixup.h"
+#include "its_dc.h"
+#include "sparse.h"
+#include "backward.h"
 #include "mc_dense.h"
 
 namespace {
 
 thread_local std::string g_err;
+
+// Optional HIP-event timing of the main kernel of the last call.
+bool g_timing = false;
+thread_local float g_kernel_ms = -1.0f;
 
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char *fmt, ...) {
@@ -92,11 +99,176 @@ bool grid_well_separated(const int64_t shape[3], const float *amin, const float 
     return true;
 }
 
+iso::GridDesc to_desc(const isoext_grid *g) {
+    iso::GridDesc d;
+    d.values = g->values;
+    d.sparse_ids = g->sparse_cell_ids;
+    d.n_sparse = g->n_sparse;
+    d.X = g->shape[0];
+    d.Y = g->shape[1];
+    d.Z = g->shape[2];
+    for (int a = 0; a < 3; ++a) {
+        d.amin[a] = g->aabb_min[a];
+        d.amax[a] = g->aabb_max[a];
+    }
+    return d;
+}
+
+iso::ItsIn to_in(const isoext_its *t) {
+    iso::ItsIn in;
+    in.ncells = t->ncells;
+    in.npts = t->npts;
+    in.cell_indices = t->cell_indices;
+    in.cell_dense = t->cell_dense;
+    in.cell_offsets = t->cell_offsets;
+    in.points = t->points;
+    in.normals = t->normals;
+    in.edge_local = t->edge_local;
+    in.is_out = t->is_out;
+    in.entry_cell = t->entry_cell;
+    return in;
+}
+
+bool grid_has_cells(const isoext_grid *g) {
+    if (g->shape[0] < 2 || g->shape[1] < 2 || g->shape[2] < 2) return false;
+    if (g->sparse_cell_ids) return g->n_sparse > 0;
+    return true;
+}
+
+struct DcLayout {
+    size_t o_dual, o_cond, o_cnt, o_used, o_bsum, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, total;
+    int64_t hslots;
+};
+
+DcLayout dc_layout(int64_t ncells, int64_t npts) {
+    DcLayout L;
+    int64_t nb = (std::max(ncells, npts) + 255) / 256 + 1;
+    int64_t hs = 1;
+    while (hs < 2 * ncells + 16) hs <<= 1;
+    L.hslots = hs;
+    size_t o = 0;
+    auto take = [&](size_t n) {
+        size_t r = o;
+        o = align256(o + n);
+        return r;
+    };
+    L.o_dual = take(sizeof(float) * 3 * std::max<int64_t>(ncells, 1));
+    L.o_cond = take(sizeof(float) * std::max<int64_t>(ncells, 1));
+    L.o_cnt = take(64);
+    L.o_used = take(std::max<int64_t>(ncells, 1));
+    L.o_bsum = take(sizeof(int64_t) * 2 * nb);
+    L.o_tot = take(64);
+    L.o_tri = take(sizeof(int32_t) * 6 * std::max<int64_t>(npts, 1));
+    L.o_hs = take(sizeof(int32_t) * hs);
+    L.o_hm = take(sizeof(int32_t) * hs);
+    L.o_rep = take(sizeof(int32_t) * std::max<int64_t>(ncells, 1));
+    L.o_fin = take(sizeof(int32_t) * std::max<int64_t>(ncells, 1));
+    L.total = o;
+    return L;
+}
+
 }   // namespace
 
 extern "C" {
 
+int64_t isoext_its_scratch_bytes(const isoext_grid *g) {
+    if (!grid_has_cells(g)) return 64;
+    iso::GridDesc d = to_desc(g);
+    return (int64_t) (sizeof(int64_t) * 2 * (iso::its_scratch_pairs(d) + 4));
+}
+
+int isoext_its_count(const isoext_grid *g, float level, void *scratch, int64_t scratch_bytes, int64_t counts[2],
+                     void *stream) {
+    counts[0] = counts[1] = 0;
+    if (!grid_has_cells(g)) return ISOEXT_OK;
+    if (scratch_bytes < isoext_its_scratch_bytes(g)) return fail(ISOEXT_E_WORKSPACE, "intersection scratch too small");
+    iso::GridDesc d = to_desc(g);
+    hipStream_t st = (hipStream_t) stream;
+    int64_t nb = iso::its_scratch_pairs(d);
+    int64_t *bsum = (int64_t *) scratch;
+    int64_t *tot = bsum + 2 * nb;
+    HIP_CHECK(iso::its_count(d, level, bsum, tot, st), "intersection count");
+    int64_t h[2];
+    HIP_CHECK(hipMemcpyAsync(h, tot, sizeof(h), hipMemcpyDeviceToHost, st), "read counts");
+    HIP_CHECK(hipStreamSynchronize(st), "intersection count");
+    counts[0] = h[0];
+    counts[1] = h[1];
+    return ISOEXT_OK;
+}
+
+int isoext_its_emit(const isoext_grid *g, float level, const void *scratch, int64_t ncells, int64_t npts,
+                    int64_t *cell_indices, int64_t *cell_dense, int64_t *cell_offsets, float *points, float *normals,
+                    uint8_t *edge_local, uint8_t *is_out, int32_t *entry_cell, void *stream) {
+    hipStream_t st = (hipStream_t) stream;
+    if (grid_has_cells(g) && ncells > 0) {
+        iso::GridDesc d = to_desc(g);
+        iso::ItsOut o{cell_indices, cell_dense, cell_offsets, points, normals, edge_local, is_out, entry_cell};
+        HIP_CHECK(iso::its_emit(d, level, (const int64_t *) scratch, o, st), "intersection emit");
+    }
+    HIP_CHECK(hipMemcpyAsync(cell_offsets + ncells, &npts, sizeof(int64_t), hipMemcpyHostToDevice, st),
+              "write offsets");
+    HIP_CHECK(hipStreamSynchronize(st), "intersection emit");
+    return ISOEXT_OK;
+}
+
+int isoext_its_normals(const isoext_grid *g, const isoext_its *its, float *normals, void *stream) {
+    if (!grid_has_cells(g) || its->npts == 0) return ISOEXT_OK;
+    HIP_CHECK(iso::its_normals(to_desc(g), to_in(its), normals, (hipStream_t) stream), "normals");
+    return ISOEXT_OK;
+}
+
+int64_t isoext_dc_workspace_bytes(int64_t ncells, int64_t npts) { return (int64_t) dc_layout(ncells, npts).total; }
+
+int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_tol, void *workspace,
+              int64_t workspace_bytes, float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f, float *dual_v,
+              int64_t counts[4], void *stream) {
+    counts[0] = counts[1] = counts[2] = counts[3] = 0;
+    if (!grid_has_cells(g) || its->ncells == 0) return ISOEXT_OK;
+    if (!its->normals) return fail(ISOEXT_E_ARG, "intersection has no normals");
+    DcLayout L = dc_layout(its->ncells, its->npts);
+    if ((int64_t) L.total > workspace_bytes) return fail(ISOEXT_E_WORKSPACE, "dual contouring workspace too small");
+    if (cap_v < its->ncells || cap_f < 2 * its->npts) return fail(ISOEXT_E_CAPACITY, "output capacity too small");
+    hipStream_t st = (hipStream_t) stream;
+    char *ws = (char *) workspace;
+    iso::DcScratch w;
+    w.dual_v = dual_v ? dual_v : (float *) (ws + L.o_dual);
+    w.cond = (float *) (ws + L.o_cond);
+    w.n_fp64 = (unsigned long long *) (ws + L.o_cnt);
+    w.used = (uint8_t *) (ws + L.o_used);
+    w.bsum = (int64_t *) (ws + L.o_bsum);
+    w.tot = (int64_t *) (ws + L.o_tot);
+    w.tri_cells = (int32_t *) (ws + L.o_tri);
+    w.cap_tri = 6 * std::max<int64_t>(its->npts, 1);
+    w.hslots = (int32_t *) (ws + L.o_hs);
+    w.slot_min = (int32_t *) (ws + L.o_hm);
+    w.hmask = L.hslots - 1;
+    w.rep_slot = (int32_t *) (ws + L.o_rep);
+    w.final_id = (int32_t *) (ws + L.o_fin);
+    w.out_v = out_v;
+    w.out_f = out_f;
+    unsigned long long zero = 0;
+    HIP_CHECK(hipMemcpyAsync(w.n_fp64, &zero, sizeof(zero), hipMemcpyHostToDevice, st), "reset counter");
+    // re-solve in FP64 where the FP32 QEF is ill-conditioned, and always in the
+    // regime where the reference truncates singular values (cond >= 1/svd_tol)
+    float cond_fp64 = std::min(1e6f, 0.5f / std::max(svd_tol, 1e-30f));
+    int64_t nv = 0, nf = 0;
+    HIP_CHECK(iso::dc_run(to_desc(g), to_in(its), reg, svd_tol, cond_fp64, w, nv, nf, st), "dual contouring");
+    unsigned long long nfp = 0;
+    HIP_CHECK(hipMemcpyAsync(&nfp, w.n_fp64, sizeof(nfp), hipMemcpyDeviceToHost, st), "read counter");
+    HIP_CHECK(hipStreamSynchronize(st), "dual contouring");
+    counts[0] = nv;
+    counts[1] = nf;
+    counts[2] = (int64_t) nfp;
+    counts[3] = w.nquads;
+    return ISOEXT_OK;
+}
+
+
 const char *isoext_last_error(void) { return g_err.c_str(); }
+
+void isoext_set_timing(int on) { g_timing = on != 0; }
+
+float isoext_last_kernel_ms(void) { return g_kernel_ms; }
 
 int isoext_version(void) { return ISOEXT_ABI_VERSION; }
 
@@ -108,10 +280,31 @@ int64_t isoext_mc_dense_workspace_bytes(const int64_t shape[3], int cfg, int64_t
     return (int64_t) L.total;
 }
 
+static int mc_dense_impl(const float *values, const int64_t shape[3], const float aabb_min[3],
+                         const float aabb_max[3], float level, int method, int cfg, void *workspace,
+                         int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
+                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[4], void *stream);
+
 int isoext_mc_dense(const float *values, const int64_t shape[3], const float aabb_min[3],
                     const float aabb_max[3], float level, int method, int cfg, void *workspace,
                     int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
                     int32_t *out_f, int64_t cap_f, int64_t counts[4], void *stream) {
+    return mc_dense_impl(values, shape, aabb_min, aabb_max, level, method, cfg, workspace, workspace_bytes,
+                         special_cap, out_v, cap_v, out_f, cap_f, nullptr, counts, stream);
+}
+
+int isoext_mc_dense_keyed(const float *values, const int64_t shape[3], const float aabb_min[3],
+                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
+                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
+                          int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[4], void *stream) {
+    return mc_dense_impl(values, shape, aabb_min, aabb_max, level, method, cfg, workspace, workspace_bytes,
+                         special_cap, out_v, cap_v, out_f, cap_f, out_key, counts, stream);
+}
+
+static int mc_dense_impl(const float *values, const int64_t shape[3], const float aabb_min[3],
+                         const float aabb_max[3], float level, int method, int cfg, void *workspace,
+                         int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
+                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[4], void *stream) {
     counts[0] = counts[1] = counts[2] = counts[3] = 0;
     if (method != 0 && method != 1) return fail(ISOEXT_E_ARG, "Unknown method: %d", method);
     for (int a = 0; a < 3; ++a)
@@ -154,6 +347,7 @@ int isoext_mc_dense(const float *values, const int64_t shape[3], const float aab
     p.ntiles = L.ntiles;
     p.out_v = out_v;
     p.cap_v = out_v ? cap_v : 0;
+    p.out_key = out_key;
     p.out_f = out_f;
     p.cap_f = out_f ? cap_f : 0;
     p.status = (uint64_t *) (ws + L.o_status);
@@ -163,7 +357,12 @@ int isoext_mc_dense(const float *values, const int64_t shape[3], const float aab
     p.err = (int *) (ws + kOffErr);
     p.totals = (int64_t *) (ws + kOffTotals);
     p.sp = (iso::SpecialLists *) (ws + kOffSp);
-    HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st), "launch marching cubes");
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (g_timing) {
+        HIP_CHECK(hipEventCreate(&ev0), "event");
+        HIP_CHECK(hipEventCreate(&ev1), "event");
+    }
+    HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, ev0, ev1), "launch marching cubes");
 
     // read back counts and the special lists' descriptor
     struct {
@@ -176,6 +375,12 @@ int isoext_mc_dense(const float *values, const int64_t shape[3], const float aab
     HIP_CHECK(hipMemcpyAsync(&sp_dev, ws + kOffSp, sizeof(sp_dev), hipMemcpyDeviceToHost, st),
               "read special lists");
     HIP_CHECK(hipStreamSynchronize(st), "marching cubes");
+    if (g_timing) {
+        float ms = -1.0f;
+        if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) g_kernel_ms = ms;
+        (void) hipEventDestroy(ev0);
+        (void) hipEventDestroy(ev1);
+    }
     int64_t nv = hdr.totals[0], nf = hdr.totals[1];
     counts[0] = nv;
     counts[1] = nf;
@@ -194,7 +399,7 @@ int isoext_mc_dense(const float *values, const int64_t shape[3], const float aab
         return fail(ISOEXT_E_WORKSPACE, "special-vertex lists need capacity %lld", (long long) need);
     if (sp_dev.n_cand > 0 || sp_dev.n_hole > 0) {
         iso::FixupGrid g{shape[0], shape[1], shape[2]};
-        HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st), "weld fix-up");
+        HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key), "weld fix-up");
         counts[0] = nv;
         counts[1] = nf;
     }
@@ -237,6 +442,100 @@ int isoext_fill_f32(float *ptr, int64_t n, float value, void *stream) {
     unsigned grid = (unsigned) std::min<int64_t>(65536, (n + 255) / 256);
     hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, (hipStream_t) stream, ptr, n, value);
     HIP_CHECK(hipGetLastError(), "launch fill");
+    return ISOEXT_OK;
+}
+
+
+static iso::SparseGeom geom_of(const int64_t shape[3], const float amin[3], const float amax[3]) {
+    iso::SparseGeom g;
+    g.X = shape[0];
+    g.Y = shape[1];
+    g.Z = shape[2];
+    for (int a = 0; a < 3; ++a) {
+        g.amin[a] = amin[a];
+        g.amax[a] = amax[a];
+    }
+    return g;
+}
+
+int isoext_sparse_corner_points(const int64_t *cell_ids, int64_t n, const int64_t shape[3], const float aabb_min[3],
+                                const float aabb_max[3], float *out, void *stream) {
+    if (n <= 0) return ISOEXT_OK;
+    if (shape[1] < 2 || shape[2] < 2) return fail(ISOEXT_E_ARG, "sparse grid shape must be >= 2 along y and z");
+    HIP_CHECK(iso::sparse_corner_points(cell_ids, n, geom_of(shape, aabb_min, aabb_max), out, (hipStream_t) stream),
+              "corner points");
+    return ISOEXT_OK;
+}
+
+int isoext_sparse_crossing_flags(const float *values, int64_t n, float level, uint8_t *flags, void *stream) {
+    if (n <= 0) return ISOEXT_OK;
+    HIP_CHECK(iso::sparse_crossing_flags(values, n, level, flags, (hipStream_t) stream), "crossing flags");
+    return ISOEXT_OK;
+}
+
+int64_t isoext_mc_sparse_workspace_bytes(int64_t n, int64_t ninst) {
+    return (int64_t) iso::sparse_mc_work_bytes(n, ninst);
+}
+
+static iso::SparseMcIn smc_in(const isoext_grid *g, float level, int method) {
+    iso::SparseMcIn in;
+    in.values = g->values;
+    in.ids = g->sparse_cell_ids;
+    in.n = g->n_sparse;
+    in.g = geom_of(g->shape, g->aabb_min, g->aabb_max);
+    in.level = level;
+    in.method = method;
+    return in;
+}
+
+int isoext_mc_sparse_count(const isoext_grid *g, float level, int64_t *scratch, int64_t *ninst, void *stream) {
+    *ninst = 0;
+    if (!g->sparse_cell_ids || g->n_sparse == 0) return ISOEXT_OK;
+    HIP_CHECK(iso::sparse_mc_count(smc_in(g, level, 0), scratch, *ninst, (hipStream_t) stream), "sparse count");
+    return ISOEXT_OK;
+}
+
+int isoext_mc_sparse(const isoext_grid *g, float level, int method, int64_t ninst, void *workspace,
+                     int64_t workspace_bytes, float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f,
+                     int64_t *vsrc, int64_t counts[2], void *stream) {
+    counts[0] = counts[1] = 0;
+    if (method != 0 && method != 1) return fail(ISOEXT_E_ARG, "Unknown method: %d", method);
+    if (!g->sparse_cell_ids || g->n_sparse == 0 || ninst == 0) return ISOEXT_OK;
+    if (workspace_bytes < isoext_mc_sparse_workspace_bytes(g->n_sparse, ninst))
+        return fail(ISOEXT_E_WORKSPACE, "sparse marching cubes workspace too small");
+    iso::SparseMcOut o{out_v, out_v ? cap_v : 0, out_f, out_f ? cap_f : 0, vsrc};
+    int64_t nv = 0, nf = 0;
+    HIP_CHECK(iso::sparse_mc(smc_in(g, level, method), ninst, workspace, (size_t) workspace_bytes, o, nv, nf,
+                             (hipStream_t) stream),
+              "sparse marching cubes");
+    counts[0] = nv;
+    counts[1] = nf;
+    if (nv > cap_v || nf > cap_f)
+        return fail(ISOEXT_E_CAPACITY, "output capacity too small: need %lld vertices, %lld faces", (long long) nv,
+                    (long long) nf);
+    return ISOEXT_OK;
+}
+
+
+int isoext_mc_backward(const isoext_grid *g, float level, const float *grad_v, const int64_t *vertex_src, int64_t nv,
+                       float *grad_values, void *stream) {
+    if (nv <= 0) return ISOEXT_OK;
+    iso::DenseGeom d;
+    d.X = g->shape[0];
+    d.Y = g->shape[1];
+    d.Z = g->shape[2];
+    for (int a = 0; a < 3; ++a) {
+        d.amin[a] = g->aabb_min[a];
+        d.amax[a] = g->aabb_max[a];
+    }
+    hipStream_t st = (hipStream_t) stream;
+    if (g->sparse_cell_ids)
+        HIP_CHECK(iso::mc_backward_sparse(grad_v, vertex_src, nv, g->values, g->sparse_cell_ids, d, level, grad_values,
+                                          st),
+                  "marching cubes backward");
+    else
+        HIP_CHECK(iso::mc_backward_dense(grad_v, vertex_src, nv, g->values, d, level, grad_values, st),
+                  "marching cubes backward");
     return ISOEXT_OK;
 }
 

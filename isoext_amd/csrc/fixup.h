@@ -12,6 +12,6 @@ struct FixupGrid {
 };
 
 hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
-                      const FixupGrid &g, hipStream_t st);
+                      const FixupGrid &g, hipStream_t st, int64_t *keys = nullptr);
 
 }   // namespace iso

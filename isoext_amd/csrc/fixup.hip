@@ -193,7 +193,7 @@ __device__ __forceinline__ int64_t bit_rank(const uint32_t *bits, const int64_t 
 }
 
 __global__ void k_compact_v(const float *in, float *out, int64_t n, const uint32_t *vkeep,
-                            const int64_t *vpre) {
+                            const int64_t *vpre, const int64_t *kin, int64_t *kout) {
     for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t) gridDim.x * blockDim.x) {
         if (!((vkeep[i >> 5] >> (i & 31)) & 1)) continue;
@@ -201,6 +201,7 @@ __global__ void k_compact_v(const float *in, float *out, int64_t n, const uint32
         out[3 * j + 0] = in[3 * i + 0];
         out[3 * j + 1] = in[3 * i + 1];
         out[3 * j + 2] = in[3 * i + 2];
+        if (kin) kout[j] = kin[i];
     }
 }
 
@@ -319,7 +320,7 @@ hipError_t bitmap_prefix(const uint32_t *bits, int64_t nw, int64_t *pre, int64_t
 // Resolves the special lists in place.  nv / nf hold the provisional counts on
 // entry and the final counts on return.  Synchronises the stream.
 hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
-                      const FixupGrid &g, hipStream_t st) {
+                      const FixupGrid &g, hipStream_t st, int64_t *vkeys) {
     const int64_t ncand = (int64_t) sp.n_cand, nscell = (int64_t) sp.n_scell;
     const int64_t nkept = (int64_t) sp.n_kept, nhole = (int64_t) sp.n_hole;
     if (ncand == 0 && nhole == 0) return hipSuccess;
@@ -342,6 +343,7 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     size_t o_misc = take(sizeof(int64_t) * 8);
     size_t o_v2 = take(sizeof(float) * 3 * std::max<int64_t>(nv, 1));
     size_t o_f2 = take(sizeof(int32_t) * 3 * std::max<int64_t>(nf, 1));
+    size_t o_k2 = take(sizeof(int64_t) * std::max<int64_t>(vkeys ? nv : 1, 1));
     char *base = nullptr;
     ISO_TRY(hipMallocAsync((void **) &base, bytes, st));
     int64_t *keys = (int64_t *) (base + o_keys);
@@ -354,6 +356,7 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     int64_t *scr = (int64_t *) (base + o_scr), *misc = (int64_t *) (base + o_misc);
     float *v2 = (float *) (base + o_v2);
     int32_t *f2 = (int32_t *) (base + o_f2);
+    int64_t *k2 = (int64_t *) (base + o_k2);
 
     hipLaunchKernelGGL(k_clear64, dim3(grid_for(nslots)), dim3(256), 0, st, keys, nslots, kEmpty);
     hipLaunchKernelGGL(k_init_info, dim3(grid_for(nslots)), dim3(256), 0, st, info, nslots);
@@ -392,7 +395,7 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     ISO_TRY(hipGetLastError());
     ISO_TRY(bitmap_prefix(vkeep, vw, vpre, scr, misc + 2, st));
     ISO_TRY(bitmap_prefix(fkeep, fw, fpre, scr, misc + 3, st));
-    hipLaunchKernelGGL(k_compact_v, dim3(grid_for(nv)), dim3(256), 0, st, v, v2, nv, vkeep, vpre);
+    hipLaunchKernelGGL(k_compact_v, dim3(grid_for(nv)), dim3(256), 0, st, v, v2, nv, vkeep, vpre, vkeys, k2);
     hipLaunchKernelGGL(k_compact_f, dim3(grid_for(nf)), dim3(256), 0, st, f, f2, nf, fkeep, fpre, vkeep,
                        vpre, sfb, keys, info, mask, groups, nv);
     ISO_TRY(hipGetLastError());
@@ -402,6 +405,7 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     const int64_t nv2 = h[2], nf2 = h[3];
     ISO_TRY(hipMemcpyAsync(v, v2, sizeof(float) * 3 * nv2, hipMemcpyDeviceToDevice, st));
     ISO_TRY(hipMemcpyAsync(f, f2, sizeof(int32_t) * 3 * nf2, hipMemcpyDeviceToDevice, st));
+    if (vkeys) ISO_TRY(hipMemcpyAsync(vkeys, k2, sizeof(int64_t) * nv2, hipMemcpyDeviceToDevice, st));
     ISO_TRY(hipFreeAsync(base, st));
     ISO_TRY(hipStreamSynchronize(st));
     nv = nv2;

@@ -1,0 +1,65 @@
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+namespace iso {
+
+struct GridDesc {
+    const float *values;         // dense (X, Y, Z) or sparse (n, 8)
+    const int64_t *sparse_ids;   // sorted dense cell ids of the sparse slots, or nullptr
+    int64_t n_sparse;
+    int64_t X, Y, Z;             // points per axis
+    float amin[3], amax[3];
+};
+
+struct ItsOut {
+    int64_t *cell_indices;   // dense: cell id; sparse: slot
+    int64_t *cell_dense;     // dense cell id
+    int64_t *cell_offsets;   // ncells (+1 written by the host)
+    float *points;
+    float *normals;          // may be null
+    uint8_t *edge_local;
+    uint8_t *is_out;
+    int32_t *entry_cell;
+};
+
+struct ItsIn {
+    int64_t ncells, npts;
+    const int64_t *cell_indices;
+    const int64_t *cell_dense;
+    const int64_t *cell_offsets;
+    const float *points;
+    const float *normals;
+    const uint8_t *edge_local;
+    const uint8_t *is_out;
+    const int32_t *entry_cell;
+};
+
+struct DcScratch {
+    float *dual_v;
+    float *cond;
+    unsigned long long *n_fp64;
+    uint8_t *used;
+    int64_t *bsum;
+    int64_t *tot;
+    int32_t *tri_cells;
+    int64_t cap_tri;
+    int32_t *hslots, *slot_min;
+    int64_t hmask;
+    int32_t *rep_slot, *final_id;
+    float *out_v;
+    int32_t *out_f;
+    int64_t nquads;
+};
+
+int64_t its_scratch_pairs(const GridDesc &g);
+hipError_t its_count(const GridDesc &g, float level, int64_t *bsum, int64_t *tot, hipStream_t st);
+hipError_t its_emit(const GridDesc &g, float level, const int64_t *bsum_pre, const ItsOut &o, hipStream_t st);
+hipError_t its_normals(const GridDesc &g, const ItsIn &in, float *normals, hipStream_t st);
+hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float cond_fp64, DcScratch &w,
+                  int64_t &nv, int64_t &nf, hipStream_t st);
+
+}   // namespace iso

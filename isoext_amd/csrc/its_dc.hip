@@ -1,0 +1,714 @@
+// Edge intersections, trilinear normals and dual contouring (gfx950).
+//
+// Reference path replaced:
+//   get_intersection            src/its.cu:93-159   (edge status, compaction,
+//                                                   popcount offsets, get_its_op)
+//   compute_intersection_normals src/its.cu:170-284 (trilinear central differences)
+//   dual_contouring              src/dc.cu:161-218  (QEF, batched SVD solve,
+//                                                   clip, quads, weld)
+//   get_dual_quads               src/grid/uniform.cu:60-83, src/grid/sparse.cu:181-246
+//
+// Intersection layout (same entry order as the reference): active cells in
+// ascending cell order; inside a cell its crossing edges in edge-index order.
+// Per entry: point, normal, owning active cell, local edge index, is_out.
+//
+// Cells come from a dense grid (cell id = linear (x, y, z) over (X-1, Y-1,
+// Z-1)) or from a sparse grid (slot s: dense id ids[s], 8 values at
+// values[8 s ..]); `sparse` selects the source.  Counting and emission use a
+// reduce-then-scan over fixed blocks of cells so the output is ordered.
+
+#include "common.h"
+#include "its_dc.h"
+
+namespace iso {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct CellSrc {
+    const float *values;
+    const int64_t *ids;   // sparse: sorted dense cell ids (nullptr for dense)
+    int64_t n;            // number of cells (dense: (X-1)(Y-1)(Z-1); sparse: slots)
+    int64_t X, Y, Z;
+    float amin[3], amax[3];
+};
+
+__device__ __forceinline__ int64_t cell_dense_id(const CellSrc &s, int64_t i) {
+    return s.ids ? s.ids[i] : i;
+}
+
+__device__ __forceinline__ void cell_xyz(const CellSrc &s, int64_t id, int64_t &x, int64_t &y, int64_t &z) {
+    int64_t cz = s.Z - 1, cy = s.Y - 1;
+    z = id % cz;
+    int64_t r = id / cz;
+    y = r % cy;
+    x = r / cy;
+}
+
+__device__ __forceinline__ void cell_values(const CellSrc &s, int64_t i, int64_t x, int64_t y, int64_t z,
+                                            float cv[8]) {
+    if (s.ids) {
+        const float *v = s.values + 8 * i;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) cv[k] = v[k];
+    } else {
+        const float *r00 = s.values + (x * s.Y + y) * s.Z + z;
+        const float *r01 = r00 + s.Z;
+        const float *r10 = r00 + s.Y * s.Z;
+        const float *r11 = r10 + s.Z;
+        cv[0] = r00[0];
+        cv[1] = r00[1];
+        cv[2] = r01[0];
+        cv[3] = r01[1];
+        cv[4] = r10[0];
+        cv[5] = r10[1];
+        cv[6] = r11[0];
+        cv[7] = r11[1];
+    }
+}
+
+__device__ __forceinline__ int case_bits(const float cv[8], float level) {
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c |= inside(cv[k], level) << k;
+    return c;
+}
+
+__device__ __forceinline__ int crossing12(int cs) {
+    int st = 0;
+#pragma unroll
+    for (int e = 0; e < 12; ++e)
+        if (((cs >> kEdgeA[e]) & 1) != ((cs >> kEdgeB[e]) & 1)) st |= 1 << e;
+    return st;
+}
+
+// Block-wide exclusive scan of two int64 counters (kBlock threads).
+__device__ __forceinline__ void block_scan2(int64_t a, int64_t b, int64_t &ea, int64_t &eb, int64_t &ta,
+                                            int64_t &tb) {
+    __shared__ int64_t sa[kBlock / 64], sb[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t ia = a, ib = b;
+    for (int o = 1; o < 64; o <<= 1) {
+        int64_t ya = __shfl_up(ia, o), yb = __shfl_up(ib, o);
+        if (lane >= o) {
+            ia += ya;
+            ib += yb;
+        }
+    }
+    if (lane == 63) {
+        sa[wave] = ia;
+        sb[wave] = ib;
+    }
+    __syncthreads();
+    int64_t wa = 0, wb = 0;
+    ta = tb = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wave) {
+            wa += sa[w];
+            wb += sb[w];
+        }
+        ta += sa[w];
+        tb += sb[w];
+    }
+    ea = wa + ia - a;
+    eb = wb + ib - b;
+    __syncthreads();
+}
+
+// ---- intersections ----------------------------------------------------------
+
+__global__ void k_its_count(CellSrc s, float level, int64_t *bsum) {
+    int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    int64_t na = 0, ni = 0;
+    if (i < s.n) {
+        int64_t id = cell_dense_id(s, i), x, y, z;
+        cell_xyz(s, id, x, y, z);
+        float cv[8];
+        cell_values(s, i, x, y, z, cv);
+        int cs = case_bits(cv, level);
+        if (cs != 0 && cs != 255) {
+            na = 1;
+            ni = __popc(crossing12(cs));
+        }
+    }
+    int64_t ea, ei, ta, ti;
+    block_scan2(na, ni, ea, ei, ta, ti);
+    if (threadIdx.x == 0) {
+        bsum[2 * blockIdx.x] = ta;
+        bsum[2 * blockIdx.x + 1] = ti;
+    }
+}
+
+// trilinear_interp (src/its.cu:170-184), restated op for op
+__device__ __forceinline__ float trilinear(float tx, float ty, float tz, const float *c) {
+    float a = 1.0f - tz;
+    float c00 = c[0] * a + c[1] * tz;
+    float c01 = c[2] * a + c[3] * tz;
+    float c10 = c[4] * a + c[5] * tz;
+    float c11 = c[6] * a + c[7] * tz;
+    float b = 1.0f - ty;
+    float c0 = c00 * b + c01 * ty;
+    float c1 = c10 * b + c11 * ty;
+    float d = 1.0f - tx;
+    return c0 * d + c1 * tx;
+}
+
+// compute_normals_op (src/its.cu:186-268) for one point in a cell.
+__device__ void cell_normal(const float lo[3], const float hi[3], const float *cv, float px, float py,
+                            float pz, float &nx, float &ny, float &nz) {
+    float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
+    float tx = (px - lo[0]) / sx, ty = (py - lo[1]) / sy, tz = (pz - lo[2]) / sz;
+    tx = fmaxf(0.01f, fminf(0.99f, tx));
+    ty = fmaxf(0.01f, fminf(0.99f, ty));
+    tz = fmaxf(0.01f, fminf(0.99f, tz));
+    const float eps = 0.02f;
+    float pxx = fminf(tx + eps, 0.99f), mxx = fmaxf(tx - eps, 0.01f);
+    float pyy = fminf(ty + eps, 0.99f), myy = fmaxf(ty - eps, 0.01f);
+    float pzz = fminf(tz + eps, 0.99f), mzz = fmaxf(tz - eps, 0.01f);
+    float dx = (trilinear(pxx, ty, tz, cv) - trilinear(mxx, ty, tz, cv)) / ((pxx - mxx) * sx);
+    float dy = (trilinear(tx, pyy, tz, cv) - trilinear(tx, myy, tz, cv)) / ((pyy - myy) * sy);
+    float dz = (trilinear(tx, ty, pzz, cv) - trilinear(tx, ty, mzz, cv)) / ((pzz - mzz) * sz);
+    float d2 = dx * dx + dy * dy;
+    d2 = d2 + dz * dz;
+    float len = sqrtf(d2);
+    if (len > 1e-8f) {
+        nx = dx / len;
+        ny = dy / len;
+        nz = dz / len;
+    } else {
+        nx = 0.0f;
+        ny = 0.0f;
+        nz = 1.0f;
+    }
+}
+
+__global__ void k_its_emit(CellSrc s, float level, const int64_t *bsum_pre, ItsOut o) {
+    int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    int64_t na = 0, ni = 0;
+    float cv[8];
+    int64_t x = 0, y = 0, z = 0, id = 0;
+    int cs = 0;
+    if (i < s.n) {
+        id = cell_dense_id(s, i);
+        cell_xyz(s, id, x, y, z);
+        cell_values(s, i, x, y, z, cv);
+        cs = case_bits(cv, level);
+        if (cs != 0 && cs != 255) {
+            na = 1;
+            ni = __popc(crossing12(cs));
+        }
+    }
+    int64_t ea, ei, ta, ti;
+    block_scan2(na, ni, ea, ei, ta, ti);
+    if (!na) return;
+    const int64_t a = bsum_pre[2 * blockIdx.x] + ea;
+    int64_t e = bsum_pre[2 * blockIdx.x + 1] + ei;
+    o.cell_indices[a] = s.ids ? i : id;
+    o.cell_dense[a] = id;
+    o.cell_offsets[a] = e;
+    float lo[3] = {grid_coord(x, s.X, s.amin[0], s.amax[0]), grid_coord(y, s.Y, s.amin[1], s.amax[1]),
+                   grid_coord(z, s.Z, s.amin[2], s.amax[2])};
+    float hi[3] = {grid_coord(x + 1, s.X, s.amin[0], s.amax[0]),
+                   grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]),
+                   grid_coord(z + 1, s.Z, s.amin[2], s.amax[2])};
+    const int st = crossing12(cs);
+    for (int k = 0; k < 12; ++k) {
+        if (!((st >> k) & 1)) continue;
+        int ca = kEdgeA[k], cb = kEdgeB[k];
+        float t = edge_t(cv[ca], cv[cb], level);
+        float ax = (ca >> 2) ? hi[0] : lo[0], ay = ((ca >> 1) & 1) ? hi[1] : lo[1], az = (ca & 1) ? hi[2] : lo[2];
+        float bx = (cb >> 2) ? hi[0] : lo[0], by = ((cb >> 1) & 1) ? hi[1] : lo[1], bz = (cb & 1) ? hi[2] : lo[2];
+        float px = lerp1(t, ax, bx), py = lerp1(t, ay, by), pz = lerp1(t, az, bz);
+        o.points[3 * e + 0] = px;
+        o.points[3 * e + 1] = py;
+        o.points[3 * e + 2] = pz;
+        o.edge_local[e] = (uint8_t) k;
+        o.is_out[e] = cv[ca] <= cv[cb] ? 1 : 0;
+        o.entry_cell[e] = (int32_t) a;
+        if (o.normals) {
+            float nx, ny, nz;
+            cell_normal(lo, hi, cv, px, py, pz, nx, ny, nz);
+            o.normals[3 * e + 0] = nx;
+            o.normals[3 * e + 1] = ny;
+            o.normals[3 * e + 2] = nz;
+        }
+        ++e;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        // nothing: the host writes cell_offsets[ncells]
+    }
+}
+
+// Normals for an existing intersection set (compute_intersection_normals,
+// src/its.cu:271-284): one thread per entry.
+__global__ void k_its_normals(CellSrc s, ItsIn in, float *normals) {
+    int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    if (e >= in.npts) return;
+    int32_t a = in.entry_cell[e];
+    int64_t id = in.cell_dense[a];
+    int64_t x, y, z;
+    cell_xyz(s, id, x, y, z);
+    float cv[8];
+    cell_values(s, s.ids ? in.cell_indices[a] : id, x, y, z, cv);
+    float lo[3] = {grid_coord(x, s.X, s.amin[0], s.amax[0]), grid_coord(y, s.Y, s.amin[1], s.amax[1]),
+                   grid_coord(z, s.Z, s.amin[2], s.amax[2])};
+    float hi[3] = {grid_coord(x + 1, s.X, s.amin[0], s.amax[0]),
+                   grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]),
+                   grid_coord(z + 1, s.Z, s.amin[2], s.amax[2])};
+    float nx, ny, nz;
+    cell_normal(lo, hi, cv, in.points[3 * e], in.points[3 * e + 1], in.points[3 * e + 2], nx, ny, nz);
+    normals[3 * e + 0] = nx;
+    normals[3 * e + 1] = ny;
+    normals[3 * e + 2] = nz;
+}
+
+// ---- QEF solve ----------------------------------------------------------------
+
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (in registers).
+template <typename T>
+__device__ __forceinline__ void jacobi_sym3(T a[3][3], T w[3], T v[3][3], int sweeps) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v[i][j] = (i == j) ? T(1) : T(0);
+    for (int sw = 0; sw < sweeps; ++sw) {
+        T off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        if (off == T(0)) break;
+#pragma unroll
+        for (int pq = 0; pq < 3; ++pq) {
+            const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+            T apq = a[p][q];
+            if (apq == T(0)) continue;
+            T theta = (a[q][q] - a[p][p]) / (T(2) * apq);
+            T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
+            T c = T(1) / sqrt(t * t + T(1)), s = t * c;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                T akp = a[k][p], akq = a[k][q];
+                a[k][p] = c * akp - s * akq;
+                a[k][q] = s * akp + c * akq;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                T apk = a[p][k], aqk = a[q][k];
+                a[p][k] = c * apk - s * aqk;
+                a[q][k] = s * apk + c * aqk;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                T vkp = v[k][p], vkq = v[k][q];
+                v[k][p] = c * vkp - s * vkq;
+                v[k][q] = s * vkp + c * vkq;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) w[i] = a[i][i];
+}
+
+// Truncated pseudo-inverse x = sum_j (v_j . b / w_j) v_j over |w_j| > tol*max|w|
+// (the reference's rule, src/batched_la.cu:160-175).  Returns cond = max/min |w|.
+template <typename T>
+__device__ __forceinline__ T pinv3(T A[3][3], const T b[3], T tol, T x[3], int sweeps) {
+    T w[3], v[3][3];
+    jacobi_sym3(A, w, v, sweeps);
+    T wmax = fmax(fabs(w[0]), fmax(fabs(w[1]), fabs(w[2])));
+    T wmin = fmin(fabs(w[0]), fmin(fabs(w[1]), fabs(w[2])));
+    T thr = tol * wmax;
+    x[0] = x[1] = x[2] = T(0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        if (!(fabs(w[j]) > thr)) continue;
+        T proj = v[0][j] * b[0] + v[1][j] * b[1] + v[2][j] * b[2];
+        T c = proj / w[j];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) x[i] += c * v[i][j];
+    }
+    return wmin > T(0) ? wmax / wmin : T(INFINITY);
+}
+
+__global__ void k_dc_qef(const ItsIn in, const CellSrc s, float reg, float svd_tol, float cond_fp64,
+                         float *dual_v, float *cond_out, unsigned long long *n_fp64) {
+    int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    if (a >= in.ncells) return;
+    const int64_t lo = in.cell_offsets[a], hi = in.cell_offsets[a + 1];
+    const int64_t id = in.cell_dense[a];
+    int64_t x, y, z;
+    cell_xyz(s, id, x, y, z);
+    float blo[3] = {grid_coord(x, s.X, s.amin[0], s.amax[0]), grid_coord(y, s.Y, s.amin[1], s.amax[1]),
+                    grid_coord(z, s.Z, s.amin[2], s.amax[2])};
+    float bhi[3] = {grid_coord(x + 1, s.X, s.amin[0], s.amax[0]),
+                    grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]),
+                    grid_coord(z + 1, s.Z, s.amin[2], s.amax[2])};
+    // mass point and cell scale
+    float c[3] = {0.f, 0.f, 0.f};
+    for (int64_t j = lo; j < hi; ++j)
+        for (int k = 0; k < 3; ++k) c[k] += in.points[3 * j + k];
+    const float inv_n = 1.0f / (float) (hi - lo);
+    for (int k = 0; k < 3; ++k) c[k] *= inv_n;
+    const float h = fmaxf(bhi[0] - blo[0], fmaxf(bhi[1] - blo[1], bhi[2] - blo[2]));
+    const float ih = 1.0f / h;
+    // A = sum n n^T + reg I ; r = sum n (n . (p - c)) / h   (centred, scaled)
+    float A[3][3] = {{reg, 0.f, 0.f}, {0.f, reg, 0.f}, {0.f, 0.f, reg}};
+    float r[3] = {0.f, 0.f, 0.f};
+    for (int64_t j = lo; j < hi; ++j) {
+        float n[3] = {in.normals[3 * j], in.normals[3 * j + 1], in.normals[3 * j + 2]};
+        float d = (n[0] * ((in.points[3 * j] - c[0]) * ih) + n[1] * ((in.points[3 * j + 1] - c[1]) * ih)) +
+                  n[2] * ((in.points[3 * j + 2] - c[2]) * ih);
+        for (int p = 0; p < 3; ++p) {
+            for (int q = 0; q < 3; ++q) A[p][q] += n[p] * n[q];
+            r[p] += n[p] * d;
+        }
+    }
+    float Ac[3][3];
+    for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) Ac[p][q] = A[p][q];
+    float xs[3];
+    float cond = pinv3<float>(Ac, r, svd_tol, xs, 6);
+    // one step of iterative refinement: residual r - A x in FP64 (removes the
+    // eigen-solver's error), correction solved in FP32
+    {
+        float res[3];
+        for (int p = 0; p < 3; ++p)
+            res[p] = (float) ((double) r[p] - ((double) A[p][0] * xs[0] + (double) A[p][1] * xs[1] +
+                                               (double) A[p][2] * xs[2]));
+        float A2[3][3];
+        for (int p = 0; p < 3; ++p)
+            for (int q = 0; q < 3; ++q) A2[p][q] = A[p][q];
+        float dx[3];
+        pinv3<float>(A2, res, svd_tol, dx, 6);
+        for (int p = 0; p < 3; ++p) xs[p] += dx[p];
+    }
+    float out[3];
+    for (int k = 0; k < 3; ++k) out[k] = c[k] + xs[k] * h;
+    if (!(cond <= cond_fp64)) {
+        // FP64 re-solve in absolute coordinates exactly as the reference
+        // poses it (minimum-norm when singular values are truncated).
+        double Ad[3][3] = {{reg, 0, 0}, {0, reg, 0}, {0, 0, reg}}, bd[3] = {0, 0, 0}, pa[3] = {0, 0, 0};
+        for (int64_t j = lo; j < hi; ++j) {
+            double n[3] = {in.normals[3 * j], in.normals[3 * j + 1], in.normals[3 * j + 2]};
+            double p[3] = {in.points[3 * j], in.points[3 * j + 1], in.points[3 * j + 2]};
+            double d = n[0] * p[0] + n[1] * p[1] + n[2] * p[2];
+            for (int u = 0; u < 3; ++u) {
+                for (int v = 0; v < 3; ++v) Ad[u][v] += n[u] * n[v];
+                bd[u] += n[u] * d;
+                pa[u] += p[u];
+            }
+        }
+        for (int u = 0; u < 3; ++u) bd[u] += (double) reg * (pa[u] / (double) (hi - lo));
+        double xd[3];
+        pinv3<double>(Ad, bd, (double) svd_tol, xd, 12);
+        for (int k = 0; k < 3; ++k) out[k] = (float) xd[k];
+        atomicAdd(n_fp64, 1ull);
+    }
+    // clip to the cell (fix_dual_v_op, src/dc.cu:82-99)
+    for (int k = 0; k < 3; ++k) dual_v[3 * a + k] = fminf(fmaxf(out[k], blo[k]), bhi[k]);
+    if (cond_out) cond_out[a] = cond;
+}
+
+// ---- quads -------------------------------------------------------------------
+
+// edge_neighbors_table (src/shared_luts.cu:78-82): offsets subtracted from an
+// edge's lower point to reach its 4 cells, per axis.
+__constant__ const int8_t kNb[3][4][3] = {{{0, 0, 0}, {0, 1, 0}, {0, 1, 1}, {0, 0, 1}},
+                                          {{0, 0, 0}, {0, 0, 1}, {1, 0, 1}, {1, 0, 0}},
+                                          {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}}};
+
+__device__ __forceinline__ int64_t lower_bound64(const int64_t *a, int64_t n, int64_t key) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Active index of the dense cell id `cid`, or -1.  Sparse grids first map the
+// dense id to its slot (slots are sorted by dense id).
+__device__ __forceinline__ int64_t active_of(const ItsIn &in, const CellSrc &s, int64_t cid) {
+    int64_t key = cid;
+    if (s.ids) {
+        int64_t slot = lower_bound64(s.ids, s.n, cid);
+        if (slot >= s.n || s.ids[slot] != cid) return -1;
+        key = slot;
+    }
+    int64_t a = lower_bound64(in.cell_indices, in.ncells, key);
+    if (a >= in.ncells || in.cell_indices[a] != key) return -1;
+    return a;
+}
+
+// Quad of entry e, if e is the canonical copy of an interior crossing edge:
+// the copy held by the cell whose min corner owns the edge (local edges 0,
+// 3, 8).  These copies are in (p0, p1) order, the reference's quad order.
+__device__ bool entry_quad(const ItsIn &in, const CellSrc &s, int64_t e, int64_t q[4]) {
+    int k = in.edge_local[e];
+    if (k != 0 && k != 3 && k != 8) return false;
+    const int axis = kEdgeAxis[k];
+    int64_t x, y, z;
+    cell_xyz(s, in.cell_dense[in.entry_cell[e]], x, y, z);
+    for (int j = 0; j < 4; ++j) {
+        int64_t cx = x - kNb[axis][j][0], cy = y - kNb[axis][j][1], cz = z - kNb[axis][j][2];
+        if (cx < 0 || cy < 0 || cz < 0 || cx >= s.X - 1 || cy >= s.Y - 1 || cz >= s.Z - 1) return false;
+        q[j] = active_of(in, s, (cx * (s.Y - 1) + cy) * (s.Z - 1) + cz);
+        if (q[j] < 0) return false;
+    }
+    if (!in.is_out[e]) {
+        int64_t t0 = q[0], t1 = q[1];
+        q[0] = q[3];
+        q[1] = q[2];
+        q[2] = t1;
+        q[3] = t0;
+    }
+    return true;
+}
+
+__global__ void k_quad_count(ItsIn in, CellSrc s, int64_t *bsum) {
+    int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    int64_t nq = 0;
+    int64_t q[4];
+    if (e < in.npts && entry_quad(in, s, e, q)) nq = 1;
+    int64_t eq, dummy, tq, td;
+    block_scan2(nq, 0, eq, dummy, tq, td);
+    if (threadIdx.x == 0) {
+        bsum[2 * blockIdx.x] = tq;
+        bsum[2 * blockIdx.x + 1] = 0;
+    }
+}
+
+__device__ __forceinline__ float dist3(const float *a, const float *b) {
+    float x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+    float s = x * x + y * y;
+    s = s + z * z;
+    return sqrtf(s);
+}
+
+// get_triangles_op (src/dc.cu:101-157): split along the shorter diagonal.
+__global__ void k_quad_emit(ItsIn in, CellSrc s, const int64_t *bsum_pre, const float *dual_v,
+                            int32_t *tri_cells, uint8_t *used) {
+    int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    int64_t nq = 0;
+    int64_t q[4] = {0, 0, 0, 0};
+    if (e < in.npts && entry_quad(in, s, e, q)) nq = 1;
+    int64_t eq, dummy, tq, td;
+    block_scan2(nq, 0, eq, dummy, tq, td);
+    if (!nq) return;
+    const int64_t slot = bsum_pre[2 * blockIdx.x] + eq;
+    const float *v0 = dual_v + 3 * q[0], *v1 = dual_v + 3 * q[1], *v2 = dual_v + 3 * q[2], *v3 = dual_v + 3 * q[3];
+    int32_t t[6];
+    if (dist3(v0, v2) > dist3(v1, v3)) {
+        int32_t u[6] = {(int32_t) q[1], (int32_t) q[3], (int32_t) q[0], (int32_t) q[3], (int32_t) q[1], (int32_t) q[2]};
+        for (int k = 0; k < 6; ++k) t[k] = u[k];
+    } else {
+        int32_t u[6] = {(int32_t) q[2], (int32_t) q[0], (int32_t) q[1], (int32_t) q[0], (int32_t) q[2], (int32_t) q[3]};
+        for (int k = 0; k < 6; ++k) t[k] = u[k];
+    }
+    for (int k = 0; k < 6; ++k) tri_cells[6 * slot + k] = t[k];
+    for (int k = 0; k < 4; ++k) used[q[k]] = 1;
+}
+
+// ---- weld of dual vertices by position ------------------------------------------
+
+__global__ void k_pos_insert(const float *dv, const uint8_t *used, int64_t n, int32_t *slots, int64_t mask,
+                             int32_t *rep) {
+    int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    if (a >= n || !used[a]) return;
+    float x = dv[3 * a], y = dv[3 * a + 1], z = dv[3 * a + 2];
+    uint64_t h = f3_hash(x, y, z) & (uint64_t) mask;
+    for (int64_t probe = 0; probe <= mask; ++probe) {
+        int64_t sl = (int64_t) ((h + probe) & (uint64_t) mask);
+        int32_t cur = slots[sl];
+        if (cur < 0) {
+            int32_t prev = atomicCAS(&slots[sl], -1, (int32_t) a);
+            if (prev < 0) {
+                rep[a] = (int32_t) sl;
+                return;
+            }
+            cur = prev;
+        }
+        if (dv[3 * cur] == x && dv[3 * cur + 1] == y && dv[3 * cur + 2] == z) {
+            rep[a] = (int32_t) sl;
+            return;
+        }
+    }
+}
+
+__global__ void k_slot_min(const uint8_t *used, int64_t n, const int32_t *rep_slot, int32_t *slot_min) {
+    int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    if (a >= n || !used[a]) return;
+    atomicMin(&slot_min[rep_slot[a]], (int32_t) a);
+}
+
+__global__ void k_keep_flags(const uint8_t *used, int64_t n, const int32_t *rep_slot, const int32_t *slot_min,
+                             int64_t *bsum) {
+    int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    int64_t k = (a < n && used[a] && slot_min[rep_slot[a]] == (int32_t) a) ? 1 : 0;
+    int64_t ek, d, tk, td;
+    block_scan2(k, 0, ek, d, tk, td);
+    if (threadIdx.x == 0) {
+        bsum[2 * blockIdx.x] = tk;
+        bsum[2 * blockIdx.x + 1] = 0;
+    }
+}
+
+__global__ void k_keep_emit(const float *dv, const uint8_t *used, int64_t n, const int32_t *rep_slot,
+                            const int32_t *slot_min, const int64_t *bsum_pre, int32_t *final_id, float *out_v) {
+    int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    int64_t k = (a < n && used[a] && slot_min[rep_slot[a]] == (int32_t) a) ? 1 : 0;
+    int64_t ek, d, tk, td;
+    block_scan2(k, 0, ek, d, tk, td);
+    if (!k) return;
+    int64_t id = bsum_pre[2 * blockIdx.x] + ek;
+    final_id[a] = (int32_t) id;
+    out_v[3 * id + 0] = dv[3 * a + 0];
+    out_v[3 * id + 1] = dv[3 * a + 1];
+    out_v[3 * id + 2] = dv[3 * a + 2];
+}
+
+__global__ void k_remap_faces(const int32_t *tri_cells, int64_t n, const int32_t *rep_slot, const int32_t *slot_min,
+                              const int32_t *final_id, int32_t *out_f) {
+    int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int32_t a = tri_cells[i];
+    out_f[i] = final_id[slot_min[rep_slot[a]]];
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t) gridDim.x * kBlock) p[i] = v;
+}
+__global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
+    for (int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t) gridDim.x * kBlock) p[i] = v;
+}
+
+// exclusive scan of block sums (pairs), single block, in place; totals out
+__global__ void k_scan_pairs(int64_t *bsum, int64_t nb, int64_t *tot) {
+    __shared__ int64_t sa[1024], sb[1024];
+    int64_t per = (nb + 1023) / 1024;
+    int64_t lo = threadIdx.x * per, hi = min(lo + per, nb);
+    int64_t a = 0, b = 0;
+    for (int64_t i = lo; i < hi; ++i) {
+        a += bsum[2 * i];
+        b += bsum[2 * i + 1];
+    }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int64_t ya = threadIdx.x >= (unsigned) o ? sa[threadIdx.x - o] : 0;
+        int64_t yb = threadIdx.x >= (unsigned) o ? sb[threadIdx.x - o] : 0;
+        __syncthreads();
+        sa[threadIdx.x] += ya;
+        sb[threadIdx.x] += yb;
+        __syncthreads();
+    }
+    int64_t ra = sa[threadIdx.x] - a, rb = sb[threadIdx.x] - b;
+    for (int64_t i = lo; i < hi; ++i) {
+        int64_t x = bsum[2 * i], y = bsum[2 * i + 1];
+        bsum[2 * i] = ra;
+        bsum[2 * i + 1] = rb;
+        ra += x;
+        rb += y;
+    }
+    if (threadIdx.x == 1023) {
+        tot[0] = sa[1023];
+        tot[1] = sb[1023];
+    }
+}
+
+unsigned nblocks(int64_t n) { return (unsigned) std::max<int64_t>(1, (n + kBlock - 1) / kBlock); }
+
+CellSrc make_src(const GridDesc &g) {
+    CellSrc s;
+    s.values = g.values;
+    s.ids = g.sparse_ids;
+    s.X = g.X;
+    s.Y = g.Y;
+    s.Z = g.Z;
+    s.n = g.sparse_ids ? g.n_sparse : (g.X - 1) * (g.Y - 1) * (g.Z - 1);
+    for (int a = 0; a < 3; ++a) {
+        s.amin[a] = g.amin[a];
+        s.amax[a] = g.amax[a];
+    }
+    return s;
+}
+
+}   // namespace
+
+#define ISO_TRY(x)                       \
+    do {                                 \
+        hipError_t e_ = (x);             \
+        if (e_ != hipSuccess) return e_; \
+    } while (0)
+
+hipError_t its_count(const GridDesc &g, float level, int64_t *bsum, int64_t *tot, hipStream_t st) {
+    CellSrc s = make_src(g);
+    unsigned nb = nblocks(s.n);
+    hipLaunchKernelGGL(k_its_count, dim3(nb), dim3(kBlock), 0, st, s, level, bsum);
+    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, bsum, (int64_t) nb, tot);
+    return hipGetLastError();
+}
+
+int64_t its_scratch_pairs(const GridDesc &g) {
+    CellSrc s = make_src(g);
+    return nblocks(s.n);
+}
+
+hipError_t its_emit(const GridDesc &g, float level, const int64_t *bsum_pre, const ItsOut &o, hipStream_t st) {
+    CellSrc s = make_src(g);
+    hipLaunchKernelGGL(k_its_emit, dim3(nblocks(s.n)), dim3(kBlock), 0, st, s, level, bsum_pre, o);
+    return hipGetLastError();
+}
+
+hipError_t its_normals(const GridDesc &g, const ItsIn &in, float *normals, hipStream_t st) {
+    CellSrc s = make_src(g);
+    if (in.npts == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_its_normals, dim3(nblocks(in.npts)), dim3(kBlock), 0, st, s, in, normals);
+    return hipGetLastError();
+}
+
+hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float cond_fp64, DcScratch &w,
+                  int64_t &nv, int64_t &nf, hipStream_t st) {
+    CellSrc s = make_src(g);
+    nv = nf = 0;
+    if (in.ncells == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u8, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, in.ncells, (uint8_t) 0);
+    hipLaunchKernelGGL(k_dc_qef, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in, s, reg, svd_tol, cond_fp64,
+                       w.dual_v, w.cond, w.n_fp64);
+    unsigned nbq = nblocks(in.npts);
+    hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, w.bsum);
+    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbq, w.tot);
+    ISO_TRY(hipGetLastError());
+    int64_t tot[2];
+    ISO_TRY(hipMemcpyAsync(tot, w.tot, sizeof(tot), hipMemcpyDeviceToHost, st));
+    ISO_TRY(hipStreamSynchronize(st));
+    const int64_t nq = tot[0];
+    w.nquads = nq;
+    if (nq == 0) return hipSuccess;
+    if (6 * nq > w.cap_tri) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_quad_emit, dim3(nbq), dim3(kBlock), 0, st, in, s, w.bsum, w.dual_v, w.tri_cells, w.used);
+    // weld dual vertices by position
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.hslots, w.hmask + 1, -1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.slot_min, w.hmask + 1,
+                       INT32_MAX);
+    hipLaunchKernelGGL(k_pos_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.dual_v, w.used, in.ncells,
+                       w.hslots, w.hmask, w.rep_slot);
+    hipLaunchKernelGGL(k_slot_min, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, in.ncells, w.rep_slot,
+                       w.slot_min);
+    unsigned nbv = nblocks(in.ncells);
+    hipLaunchKernelGGL(k_keep_flags, dim3(nbv), dim3(kBlock), 0, st, w.used, in.ncells, w.rep_slot, w.slot_min,
+                       w.bsum);
+    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbv, w.tot);
+    hipLaunchKernelGGL(k_keep_emit, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, in.ncells, w.rep_slot,
+                       w.slot_min, w.bsum, w.final_id, w.out_v);
+    hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * nq)), dim3(kBlock), 0, st, w.tri_cells, 6 * nq, w.rep_slot,
+                       w.slot_min, w.final_id, w.out_f);
+    ISO_TRY(hipGetLastError());
+    ISO_TRY(hipMemcpyAsync(tot, w.tot, sizeof(tot), hipMemcpyDeviceToHost, st));
+    ISO_TRY(hipStreamSynchronize(st));
+    nv = tot[0];
+    nf = 2 * nq;
+    return hipSuccess;
+}
+
+}   // namespace iso

@@ -48,6 +48,7 @@ struct McParams {
     int64_t ntiles;
     float *out_v;
     int64_t cap_v;
+    int64_t *out_key;   // optional: owner point linear id * 3 + axis per vertex
     int32_t *out_f;
     int64_t cap_f;
     uint64_t *status;   // per ticket: look-back state
@@ -61,6 +62,7 @@ struct McParams {
 
 int mc_config_count();
 bool mc_config_tiles(int cfg, const int64_t shape[3], int &ni, int &nj, int &nk, int64_t &pub);
-hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream);
+hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
+                           hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 
 }   // namespace iso

@@ -88,7 +88,8 @@ struct Lds {
     uint64_t w[G::NWORDS];
     uint32_t fo[G::NUNITS + 1];   // face count, then exclusive offset
     uint32_t vo[G::NUNITS + 1];   // vertex count, then exclusive offset
-    uint32_t tsum_f[256], tsum_v[256];
+    uint32_t na[G::NUNITS + 1];   // active-cell count, then exclusive offset
+    uint32_t tsum_f[16], tsum_v[16], tsum_a[16];
     int64_t fbase, vbase;
     int64_t ticket;
 };
@@ -301,7 +302,7 @@ __global__ void __launch_bounds__(NT) mc_dense_kernel(McParams p) {
     const int ncz = (t.nzo + 63) >> 6;
     for (int u = tid; u < G::NUNITS; u += NT) {
         int c = u % G::PC, r = u / G::PC, py = r % G::PY, px = r / G::PY;
-        uint32_t fc = 0, vc = 0;
+        uint32_t fc = 0, vc = 0, ac = 0;
         if (px < t.nxo && py < t.nyo && c < ncz) {
             uint64_t xc, yc, zc;
             unit_masks(s, p, t, px, py, c, xc, yc, zc);
@@ -319,6 +320,7 @@ __global__ void __launch_bounds__(NT) mc_dense_kernel(McParams p) {
                 uint64_t any = a | as | b | bs | d0 | d0s | d1 | d1s;
                 uint64_t all = a & as & b & bs & d0 & d0s & d1 & d1s;
                 uint64_t act = any & ~all & cm;
+                ac = (uint32_t) __popcll(act);
                 while (act) {
                     int k = __ffsll((unsigned long long) act) - 1;
                     act &= act - 1;
@@ -332,6 +334,7 @@ __global__ void __launch_bounds__(NT) mc_dense_kernel(McParams p) {
         }
         s.fo[u] = fc;
         s.vo[u] = vc;
+        s.na[u] = ac;
     }
     __syncthreads();
 
@@ -339,47 +342,54 @@ __global__ void __launch_bounds__(NT) mc_dense_kernel(McParams p) {
     {
         constexpr int PER = (G::NUNITS + NT - 1) / NT;
         const int lo = tid * PER, hi = min(lo + PER, G::NUNITS);
-        uint32_t sf = 0, sv = 0;
+        uint32_t sf = 0, sv = 0, sa = 0;
         for (int u = lo; u < hi; ++u) {
             sf += s.fo[u];
             sv += s.vo[u];
+            sa += s.na[u];
         }
         // inclusive scan of thread sums across the block
-        uint32_t incf = sf, incv = sv;
+        uint32_t incf = sf, incv = sv, inca = sa;
         for (int o = 1; o < 64; o <<= 1) {
-            uint32_t yf = __shfl_up(incf, o), yv = __shfl_up(incv, o);
+            uint32_t yf = __shfl_up(incf, o), yv = __shfl_up(incv, o), ya = __shfl_up(inca, o);
             if (lane >= o) {
                 incf += yf;
                 incv += yv;
+                inca += ya;
             }
         }
         if (lane == 63) {
             s.tsum_f[wave] = incf;
             s.tsum_v[wave] = incv;
+            s.tsum_a[wave] = inca;
         }
         __syncthreads();
-        uint32_t wf = 0, wv = 0;
-        for (int w = 0; w < wave; ++w) {
-            wf += s.tsum_f[w];
-            wv += s.tsum_v[w];
-        }
-        uint32_t tot_f = 0, tot_v = 0;
+        uint32_t wf = 0, wv = 0, wa = 0, tot_f = 0, tot_v = 0, tot_a = 0;
         for (int w = 0; w < NW; ++w) {
+            if (w < wave) {
+                wf += s.tsum_f[w];
+                wv += s.tsum_v[w];
+                wa += s.tsum_a[w];
+            }
             tot_f += s.tsum_f[w];
             tot_v += s.tsum_v[w];
+            tot_a += s.tsum_a[w];
         }
-        uint32_t ef = wf + incf - sf, ev = wv + incv - sv;
+        uint32_t ef = wf + incf - sf, ev = wv + incv - sv, ea = wa + inca - sa;
         for (int u = lo; u < hi; ++u) {
-            uint32_t f = s.fo[u], v = s.vo[u];
+            uint32_t f = s.fo[u], v = s.vo[u], a = s.na[u];
             s.fo[u] = ef;
             s.vo[u] = ev;
+            s.na[u] = ea;
             ef += f;
             ev += v;
+            ea += a;
         }
         __syncthreads();
         if (tid == 0) {
             s.fo[G::NUNITS] = tot_f;
             s.vo[G::NUNITS] = tot_v;
+            s.na[G::NUNITS] = tot_a;
         }
     }
     __syncthreads();
@@ -496,36 +506,50 @@ __global__ void __launch_bounds__(NT) mc_dense_kernel(McParams p) {
         }
     }
 
-    // ---- emit: one lane per unit with faces, looping over its active cells ---
-    const int ncellc = (t.nz + 63) >> 6;
+    // ---- emit: one lane per active cell (cells compacted through the scan) ---
     const float lvl = p.level;
-    for (int u = tid; u < G::NUNITS; u += NT) {
-        uint32_t f0 = s.fo[u];
-        uint32_t nf = s.fo[u + 1] - f0;
-        // units are laid out (px, py, c) with the count array padded, so the
-        // next unit's offset bounds this unit's faces
-        if (nf == 0) continue;
-        int c = u % G::PC, r = u / G::PC, py = r % G::PY, px = r / G::PY;
-        if (!(px < t.nx && py < t.ny && c < ncellc)) continue;
-        int ncell = t.nz - 64 * c;
-        uint64_t cm = ncell >= 64 ? ~0ull : ((1ull << ncell) - 1);
-        uint64_t a = W(s, px, py, c), b = W(s, px, py + 1, c);
-        uint64_t d0 = W(s, px + 1, py, c), d1 = W(s, px + 1, py + 1, c);
-        bool hn = c + 1 < t.nchunk;
-        uint64_t as = (a >> 1) | (hn ? W(s, px, py, c + 1) << 63 : 0ull);
-        uint64_t bs = (b >> 1) | (hn ? W(s, px, py + 1, c + 1) << 63 : 0ull);
-        uint64_t d0s = (d0 >> 1) | (hn ? W(s, px + 1, py, c + 1) << 63 : 0ull);
-        uint64_t d1s = (d1 >> 1) | (hn ? W(s, px + 1, py + 1, c + 1) << 63 : 0ull);
+    const int nactive = (int) s.na[G::NUNITS];
+    for (int j0 = 0; j0 < nactive; j0 += NT) {
+        const int j = j0 + tid;
+        if (j >= nactive) break;
+        // unit holding the j-th active cell: last u with na[u] <= j
+        int lo = 0, hi = G::NUNITS;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (s.na[mid] <= (uint32_t) j) lo = mid;
+            else hi = mid;
+        }
+        const int u = lo;
+        int rank = j - (int) s.na[u];
+        const int c = u % G::PC, r = u / G::PC, py = r % G::PY, px = r / G::PY;
+        const int ncell = t.nz - 64 * c;
+        const uint64_t cm = ncell >= 64 ? ~0ull : ((1ull << ncell) - 1);
+        const uint64_t a = W(s, px, py, c), b = W(s, px, py + 1, c);
+        const uint64_t d0 = W(s, px + 1, py, c), d1 = W(s, px + 1, py + 1, c);
+        const bool hn = c + 1 < t.nchunk;
+        const uint64_t as = (a >> 1) | (hn ? W(s, px, py, c + 1) << 63 : 0ull);
+        const uint64_t bs = (b >> 1) | (hn ? W(s, px, py + 1, c + 1) << 63 : 0ull);
+        const uint64_t d0s = (d0 >> 1) | (hn ? W(s, px + 1, py, c + 1) << 63 : 0ull);
+        const uint64_t d1s = (d1 >> 1) | (hn ? W(s, px + 1, py + 1, c + 1) << 63 : 0ull);
         uint64_t act = (a | as | b | bs | d0 | d0s | d1 | d1s) & ~(a & as & b & bs & d0 & d0s & d1 & d1s) & cm;
-        int64_t fslot = fbase + f0;
+        // skip the active cells before this one, adding up their faces
+        int64_t fslot = fbase + s.fo[u];
+        for (; rank > 0; --rank) {
+            int k0 = __ffsll((unsigned long long) act) - 1;
+            act &= act - 1;
+            int cs0 = (int) ((a >> k0) & 1) | (int) ((as >> k0) & 1) << 1 | (int) ((b >> k0) & 1) << 2 |
+                      (int) ((bs >> k0) & 1) << 3 | (int) ((d0 >> k0) & 1) << 4 |
+                      (int) ((d0s >> k0) & 1) << 5 | (int) ((d1 >> k0) & 1) << 6 |
+                      (int) ((d1s >> k0) & 1) << 7;
+            fslot += tri_count(p.method, cs0);
+        }
         const int64_t gx = t.x0 + px, gy = t.y0 + py;
         const float cx0 = grid_coord(gx, p.X, p.amin[0], p.amax[0]);
         const float cx1 = grid_coord(gx + 1, p.X, p.amin[0], p.amax[0]);
         const float cy0 = grid_coord(gy, p.Y, p.amin[1], p.amax[1]);
         const float cy1 = grid_coord(gy + 1, p.Y, p.amin[1], p.amax[1]);
-        while (act) {
-            int k = __ffsll((unsigned long long) act) - 1;
-            act &= act - 1;
+        {
+            const int k = __ffsll((unsigned long long) act) - 1;
             const int zl = 64 * c + k;
             const int64_t gz = t.z0 + zl;
             int cs = (int) ((a >> k) & 1) | (int) ((as >> k) & 1) << 1 | (int) ((b >> k) & 1) << 2 |
@@ -594,6 +618,7 @@ __global__ void __launch_bounds__(NT) mc_dense_kernel(McParams p) {
                         p.out_v[3 * v + 0] = x;
                         p.out_v[3 * v + 1] = y;
                         p.out_v[3 * v + 2] = z;
+                        if (p.out_key) p.out_key[v] = edge_key(p, gx, gy, gz, e);
                     } else {
                         set_err(p, ISO_ERR_CAPACITY);
                     }
@@ -740,7 +765,8 @@ hipError_t mc_upload_luts() {
     return e;
 }
 
-hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream) {
+hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
+                           hipEvent_t ev0, hipEvent_t ev1) {
     hipError_t e = mc_upload_luts();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(mc_dense_reset, dim3((unsigned) std::min<int64_t>(1024, (p.ntiles + 255) / 256 + 1)),
@@ -749,15 +775,22 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     // the kernel reads the epoch from device memory after the reset bumps it
+    if (ev0) {
+        e = hipEventRecord(ev0, stream);
+        if (e != hipSuccess) return e;
+    }
     switch (cfg) {
-#define X(id, tx, ty, nzc, nt) \
-    case id:                   \
-        return Cfg<tx, ty, nzc, nt>::launch(p, stream);
+#define X(id, tx, ty, nzc, nt)                     \
+    case id:                                       \
+        e = Cfg<tx, ty, nzc, nt>::launch(p, stream); \
+        break;
         ISO_MC_CONFIGS(X)
 #undef X
     default:
         return hipErrorInvalidValue;
     }
+    if (e == hipSuccess && ev1) e = hipEventRecord(ev1, stream);
+    return e;
 }
 
 }   // namespace iso

@@ -49,11 +49,12 @@ def _initial_caps(shape):
 
 
 def mc_dense_raw(values: torch.Tensor, shape, aabb_min, aabb_max, level: float, method: int,
-                 cfg: int | None = None):
+                 cfg: int | None = None, want_keys: bool = False):
     """Run dense marching cubes on a (X, Y, Z) float32 CUDA tensor.
 
     Returns (v, f, info) with v (V, 3) float32 and f (F, 3) int32 tensors
-    (possibly zero rows) and an info dict (provisional counts, tile config).
+    (possibly zero rows) and an info dict (provisional counts, tile config;
+    with want_keys, info["keys"] holds each vertex's grid edge).
     """
     dev = values.device
     shape = [int(s) for s in shape]
@@ -70,11 +71,19 @@ def mc_dense_raw(values: torch.Tensor, shape, aabb_min, aabb_max, level: float, 
         ws = workspace.get("mc_dense", ws_bytes, dev)
         out_v = torch.empty((cap_v, 3), dtype=torch.float32, device=dev)
         out_f = torch.empty((cap_f, 3), dtype=torch.int32, device=dev)
+        keys = torch.empty(cap_v, dtype=torch.int64, device=dev) if want_keys else None
         with torch.cuda.device(dev):
-            rc = L.isoext_mc_dense(values.data_ptr(), _lib.i64x3(shape), _lib.f32x3(aabb_min),
-                                   _lib.f32x3(aabb_max), float(level), int(method), int(cfg),
-                                   ws.data_ptr(), ws.numel(), special_cap, out_v.data_ptr(), cap_v,
-                                   out_f.data_ptr(), cap_f, counts, _lib.stream_ptr(dev))
+            if want_keys:
+                rc = L.isoext_mc_dense_keyed(values.data_ptr(), _lib.i64x3(shape), _lib.f32x3(aabb_min),
+                                             _lib.f32x3(aabb_max), float(level), int(method), int(cfg),
+                                             ws.data_ptr(), ws.numel(), special_cap, out_v.data_ptr(), cap_v,
+                                             out_f.data_ptr(), cap_f, keys.data_ptr(), counts,
+                                             _lib.stream_ptr(dev))
+            else:
+                rc = L.isoext_mc_dense(values.data_ptr(), _lib.i64x3(shape), _lib.f32x3(aabb_min),
+                                       _lib.f32x3(aabb_max), float(level), int(method), int(cfg),
+                                       ws.data_ptr(), ws.numel(), special_cap, out_v.data_ptr(), cap_v,
+                                       out_f.data_ptr(), cap_f, counts, _lib.stream_ptr(dev))
         if rc == _lib.E_CAPACITY:
             cap_v, cap_f = int(counts[0]) + 1, int(counts[1]) + 1
             continue
@@ -94,7 +103,10 @@ def mc_dense_raw(values: torch.Tensor, shape, aabb_min, aabb_max, level: float, 
         v = v.clone()
     if cap_f > 2 * nf + (1 << 16):
         f = f.clone()
-    return v, f, {"cfg": cfg, "special": int(counts[2]), "err": int(counts[3])}
+    info = {"cfg": cfg, "special": int(counts[2]), "err": int(counts[3])}
+    if want_keys:
+        info["keys"] = keys[:nv]
+    return v, f, info
 
 
 def marching_cubes(grid: Grid, level: float = 0.0, method: str = "nagae"):
