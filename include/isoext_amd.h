@@ -72,13 +72,15 @@ int64_t isoext_mc_dense_workspace_bytes(const int64_t shape[3], int cfg, int64_t
  * (src/mc/base.cu:19-27).  Writes up to cap_v vertices (3 floats each) and
  * cap_f faces (3 int32 each).  counts[0..1] = vertices, faces of the final
  * mesh; counts[2] = special-list capacity needed; counts[3] = device error
- * bits.  On ISOEXT_E_CAPACITY counts[0..1] hold the provisional sizes to
- * allocate before retrying.  Synchronises `stream`. */
+ * bits; counts[4..5] = provisional vertices, faces (before the weld fix-up
+ * removes duplicates): the capacities a call on the same input needs.  On
+ * ISOEXT_E_CAPACITY counts[0..1] also hold the provisional sizes to allocate
+ * before retrying.  Synchronises `stream`. */
 int isoext_mc_dense(const float *values, const int64_t shape[3], const float aabb_min[3],
                     const float aabb_max[3], float level, int method, int cfg,
                     void *workspace, int64_t workspace_bytes, int64_t special_cap,
                     float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f,
-                    int64_t counts[4], void *stream);
+                    int64_t counts[6], void *stream);
 
 /* ---- grid descriptor used by intersections, dual contouring, sparse MC ---- */
 
@@ -150,7 +152,7 @@ int isoext_mc_dense_keyed(const float *values, const int64_t shape[3], const flo
                           const float aabb_max[3], float level, int method, int cfg,
                           void *workspace, int64_t workspace_bytes, int64_t special_cap,
                           float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f,
-                          int64_t *out_key, int64_t counts[4], void *stream);
+                          int64_t *out_key, int64_t counts[6], void *stream);
 
 /* Backward of marching cubes (no reference counterpart): accumulates
  * dL/dvalues += dL/dv * dv/dvalues through each vertex's edge interpolation.

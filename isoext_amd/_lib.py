@@ -31,6 +31,7 @@ vp = ctypes.c_void_p
 I64x3 = c_i64 * 3
 F32x3 = c_f * 3
 I64x4 = c_i64 * 4
+I64x6 = c_i64 * 6
 
 
 class IsoextError(RuntimeError):
@@ -50,10 +51,10 @@ def _declare(L):
     L.isoext_mc_dense_workspace_bytes.argtypes = [I64x3, c_int, c_i64]
     L.isoext_mc_dense.restype = c_int
     L.isoext_mc_dense.argtypes = [vp, I64x3, F32x3, F32x3, c_f, c_int, c_int, vp, c_i64, c_i64,
-                                  vp, c_i64, vp, c_i64, I64x4, vp]
+                                  vp, c_i64, vp, c_i64, I64x6, vp]
     L.isoext_mc_dense_keyed.restype = c_int
     L.isoext_mc_dense_keyed.argtypes = [vp, I64x3, F32x3, F32x3, c_f, c_int, c_int, vp, c_i64, c_i64,
-                                        vp, c_i64, vp, c_i64, vp, I64x4, vp]
+                                        vp, c_i64, vp, c_i64, vp, I64x6, vp]
     L.isoext_uniform_points.restype = c_int
     L.isoext_uniform_points.argtypes = [I64x3, F32x3, F32x3, vp, vp]
     L.isoext_fill_f32.restype = c_int

@@ -283,12 +283,12 @@ int64_t isoext_mc_dense_workspace_bytes(const int64_t shape[3], int cfg, int64_t
 static int mc_dense_impl(const float *values, const int64_t shape[3], const float aabb_min[3],
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
-                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[4], void *stream);
+                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream);
 
 int isoext_mc_dense(const float *values, const int64_t shape[3], const float aabb_min[3],
                     const float aabb_max[3], float level, int method, int cfg, void *workspace,
                     int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
-                    int32_t *out_f, int64_t cap_f, int64_t counts[4], void *stream) {
+                    int32_t *out_f, int64_t cap_f, int64_t counts[6], void *stream) {
     return mc_dense_impl(values, shape, aabb_min, aabb_max, level, method, cfg, workspace, workspace_bytes,
                          special_cap, out_v, cap_v, out_f, cap_f, nullptr, counts, stream);
 }
@@ -296,7 +296,7 @@ int isoext_mc_dense(const float *values, const int64_t shape[3], const float aab
 int isoext_mc_dense_keyed(const float *values, const int64_t shape[3], const float aabb_min[3],
                           const float aabb_max[3], float level, int method, int cfg, void *workspace,
                           int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
-                          int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[4], void *stream) {
+                          int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream) {
     return mc_dense_impl(values, shape, aabb_min, aabb_max, level, method, cfg, workspace, workspace_bytes,
                          special_cap, out_v, cap_v, out_f, cap_f, out_key, counts, stream);
 }
@@ -304,8 +304,8 @@ int isoext_mc_dense_keyed(const float *values, const int64_t shape[3], const flo
 static int mc_dense_impl(const float *values, const int64_t shape[3], const float aabb_min[3],
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
-                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[4], void *stream) {
-    counts[0] = counts[1] = counts[2] = counts[3] = 0;
+                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream) {
+    for (int i = 0; i < 6; ++i) counts[i] = 0;
     if (method != 0 && method != 1) return fail(ISOEXT_E_ARG, "Unknown method: %d", method);
     for (int a = 0; a < 3; ++a)
         if (shape[a] < 1) return fail(ISOEXT_E_ARG, "grid shape must be positive");
@@ -341,6 +341,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     p.level = level;
     p.method = method;
     p.all_special = grid_well_separated(shape, aabb_min, aabb_max) ? 0 : 1;
+    p.vec4 = (shape[2] % 4 == 0 && ((uintptr_t) values & 15) == 0) ? 1 : 0;
     p.NI = L.ni;
     p.NJ = L.nj;
     p.NK = L.nk;
@@ -382,8 +383,8 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
         (void) hipEventDestroy(ev1);
     }
     int64_t nv = hdr.totals[0], nf = hdr.totals[1];
-    counts[0] = nv;
-    counts[1] = nf;
+    counts[0] = counts[4] = nv;
+    counts[1] = counts[5] = nf;
     int64_t need = (int64_t) std::max<unsigned long long>(
         sp_dev.n_cand, std::max<unsigned long long>((sp_dev.n_scell + 11) / 12,
                                                     std::max((sp_dev.n_kept + 4) / 5, (sp_dev.n_hole + 4) / 5)));

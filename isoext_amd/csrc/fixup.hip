@@ -192,36 +192,12 @@ __device__ __forceinline__ int64_t bit_rank(const uint32_t *bits, const int64_t 
     return prefix[i >> 5] + __popc(below);
 }
 
-__global__ void k_compact_v(const float *in, float *out, int64_t n, const uint32_t *vkeep,
-                            const int64_t *vpre, const int64_t *kin, int64_t *kout) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x) {
-        if (!((vkeep[i >> 5] >> (i & 31)) & 1)) continue;
-        int64_t j = bit_rank(vkeep, vpre, i);
-        out[3 * j + 0] = in[3 * i + 0];
-        out[3 * j + 1] = in[3 * i + 1];
-        out[3 * j + 2] = in[3 * i + 2];
-        if (kin) kout[j] = kin[i];
-    }
-}
-
-__global__ void k_compact_f(const int32_t *in, int32_t *out, int64_t n, const uint32_t *fkeep,
-                            const int64_t *fpre, const uint32_t *vkeep, const int64_t *vpre,
-                            const uint32_t *special_face_bits, const int64_t *keys, const VInfo *info,
-                            int64_t mask, const PGroup *groups, int64_t nv) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x) {
-        if (!((fkeep[i >> 5] >> (i & 31)) & 1)) continue;
-        int64_t j = bit_rank(fkeep, fpre, i);
-        bool sp = (special_face_bits[i >> 5] >> (i & 31)) & 1;
-        for (int k = 0; k < 3; ++k) {
-            int64_t v = in[3 * i + k];
-            if (sp) {
-                int64_t s = h1_find_or_insert(const_cast<int64_t *>(keys), mask, v, false);
-                if (s >= 0 && info[s].is_cand && info[s].group >= 0) v = groups[info[s].group].rep;
-            }
-            out[3 * j + k] = (v >= 0 && v < nv) ? (int32_t) bit_rank(vkeep, vpre, v) : -1;
-        }
+__global__ void k_tail_bits(uint32_t *vkeep, int64_t nv, uint32_t *fkeep, int64_t nf) {
+    if (threadIdx.x == 0) {
+        vkeep[nv >> 5] = (nv & 31) ? ((1u << (nv & 31)) - 1) : 0u;
+        vkeep[(nv >> 5) + 1] = 0u;
+        fkeep[nf >> 5] = (nf & 31) ? ((1u << (nf & 31)) - 1) : 0u;
+        fkeep[(nf >> 5) + 1] = 0u;
     }
 }
 
@@ -291,6 +267,96 @@ __global__ void k_scan_add(int64_t *pre, int64_t nw, const int64_t *bsum) {
     if (i < nw) pre[i] += bsum[blockIdx.x];
 }
 
+// i-th set bit (ones = true) or i-th clear bit (ones = false) of a bitmap
+// whose per-word exclusive popcount prefix is pre (nw words).
+__device__ int64_t bit_select(const uint32_t *bits, const int64_t *pre, int64_t nw, int64_t i, bool ones) {
+    int64_t lo = 0, hi = nw;   // last word w with count_before(w) <= i
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        int64_t before = ones ? pre[mid] : 32 * mid - pre[mid];
+        if (before <= i) lo = mid;
+        else hi = mid;
+    }
+    int64_t before = ones ? pre[lo] : 32 * lo - pre[lo];
+    uint32_t w = ones ? bits[lo] : ~bits[lo];
+    int64_t r = i - before;
+    for (int b = 0; b < 32; ++b) {
+        if ((w >> b) & 1) {
+            if (r == 0) return 32 * lo + b;
+            --r;
+        }
+    }
+    return -1;
+}
+
+// Relocation pairs: the i-th removed slot below n_final takes the i-th kept
+// element at or above n_final (disjoint ranges, so moves do not race).
+__global__ void k_reloc_pairs(const uint32_t *keep, const int64_t *pre, int64_t nw, int64_t n_final, int64_t k,
+                              int64_t *src, int64_t *dst) {
+    // removed slots below n_final (the rest of the removed ones are in the tail)
+    const uint32_t wd = keep[n_final >> 5];
+    const uint32_t below = (n_final & 31) ? (wd & ((1u << (n_final & 31)) - 1)) : 0u;
+    const int64_t holes_low = n_final - (pre[n_final >> 5] + __popc(below));
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t) gridDim.x * blockDim.x) {
+        if (i < holes_low) {
+            dst[i] = bit_select(keep, pre, nw, i, false);
+            src[i] = bit_select(keep, pre, nw, (n_final - holes_low) + i, true);
+        } else {
+            dst[i] = -1;
+            src[i] = INT64_MAX;   // keeps src sorted for the remap's binary search
+        }
+    }
+}
+
+__global__ void k_move_v(float *v, int64_t *vkeys, const int64_t *src, const int64_t *dst, int64_t k) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t) gridDim.x * blockDim.x) {
+        int64_t s = src[i], d = dst[i];
+        if (d < 0 || s == INT64_MAX) continue;
+        v[3 * d + 0] = v[3 * s + 0];
+        v[3 * d + 1] = v[3 * s + 1];
+        v[3 * d + 2] = v[3 * s + 2];
+        if (vkeys) vkeys[d] = vkeys[s];
+    }
+}
+
+__global__ void k_move_f(int32_t *f, const int64_t *src, const int64_t *dst, int64_t k) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t) gridDim.x * blockDim.x) {
+        int64_t s = src[i], d = dst[i];
+        if (d < 0 || s == INT64_MAX) continue;
+        f[3 * d + 0] = f[3 * s + 0];
+        f[3 * d + 1] = f[3 * s + 1];
+        f[3 * d + 2] = f[3 * s + 2];
+    }
+}
+
+// Rewrites face ids in place: non-representatives -> representative (only
+// special faces can reference them), then tail vertices -> relocated slot.
+__global__ void k_remap_inplace(int32_t *f, int64_t nf, const uint32_t *fkeep, const uint32_t *special_face_bits,
+                                const int64_t *keys, const VInfo *info, int64_t mask, const PGroup *groups,
+                                int64_t nv_final, const int64_t *rsrc, const int64_t *rdst, int64_t k) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += (int64_t) gridDim.x * blockDim.x) {
+        if (!((fkeep[i >> 5] >> (i & 31)) & 1)) continue;
+        const bool sp = (special_face_bits[i >> 5] >> (i & 31)) & 1;
+        for (int j = 0; j < 3; ++j) {
+            int64_t v0 = f[3 * i + j], v = v0;
+            if (sp) {
+                int64_t s = h1_find_or_insert(const_cast<int64_t *>(keys), mask, v, false);
+                if (s >= 0 && info[s].is_cand && info[s].group >= 0) v = groups[info[s].group].rep;
+            }
+            if (v >= nv_final) {
+                int64_t lo = 0, hi = k;
+                while (lo < hi) {
+                    int64_t mid = (lo + hi) >> 1;
+                    if (rsrc[mid] < v) lo = mid + 1;
+                    else hi = mid;
+                }
+                v = (lo < k && rsrc[lo] == v) ? rdst[lo] : -1;
+            }
+            if (v != v0) f[3 * i + j] = (int32_t) v;
+        }
+    }
+}
+
 unsigned grid_for(int64_t n) { return (unsigned) std::max<int64_t>(1, std::min<int64_t>(4096, (n + 255) / 256)); }
 
 int64_t pow2_at_least(int64_t n) {
@@ -326,14 +392,15 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     if (ncand == 0 && nhole == 0) return hipSuccess;
     const int64_t nslots = pow2_at_least(2 * (ncand + nscell) + 16), mask = nslots - 1;
     const int64_t npslots = pow2_at_least(2 * ncand + 16), pmask = npslots - 1;
-    const int64_t vw = (nv + 31) / 32 + 1, fw = (nf + 31) / 32 + 1;
-    // scratch layout
+    const int64_t vw = nv / 32 + 2, fw = nf / 32 + 2;
+    // scratch layout (a cached per-thread buffer, grown on demand)
     size_t bytes = 0;
     auto take = [&](size_t n) {
         size_t o = bytes;
         bytes += (n + 255) & ~size_t(255);
         return o;
     };
+    const int64_t nrel = ncand + nscell + 16;
     size_t o_keys = take(sizeof(int64_t) * nslots), o_info = take(sizeof(VInfo) * nslots);
     size_t o_pslot = take(sizeof(int32_t) * npslots), o_groups = take(sizeof(PGroup) * npslots);
     size_t o_vkeep = take(sizeof(uint32_t) * vw), o_fkeep = take(sizeof(uint32_t) * fw);
@@ -341,11 +408,22 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     size_t o_vpre = take(sizeof(int64_t) * vw), o_fpre = take(sizeof(int64_t) * fw);
     size_t o_scr = take(sizeof(int64_t) * ((std::max(vw, fw) + 1023) / 1024 + 1));
     size_t o_misc = take(sizeof(int64_t) * 8);
-    size_t o_v2 = take(sizeof(float) * 3 * std::max<int64_t>(nv, 1));
-    size_t o_f2 = take(sizeof(int32_t) * 3 * std::max<int64_t>(nf, 1));
-    size_t o_k2 = take(sizeof(int64_t) * std::max<int64_t>(vkeys ? nv : 1, 1));
-    char *base = nullptr;
-    ISO_TRY(hipMallocAsync((void **) &base, bytes, st));
+    size_t o_rv = take(sizeof(int64_t) * 2 * nrel);
+    size_t o_rf = take(sizeof(int64_t) * 2 * (nhole + 16));
+    static thread_local char *cache = nullptr;
+    static thread_local size_t cache_bytes = 0;
+    if (cache_bytes < bytes) {
+        if (cache) {
+            ISO_TRY(hipStreamSynchronize(st));
+            ISO_TRY(hipFree(cache));
+            cache = nullptr;
+            cache_bytes = 0;
+        }
+        size_t want = bytes + bytes / 2;
+        ISO_TRY(hipMalloc((void **) &cache, want));
+        cache_bytes = want;
+    }
+    char *base = cache;
     int64_t *keys = (int64_t *) (base + o_keys);
     VInfo *info = (VInfo *) (base + o_info);
     int32_t *pslot = (int32_t *) (base + o_pslot);
@@ -354,9 +432,8 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     uint32_t *sfb = (uint32_t *) (base + o_sfb);
     int64_t *vpre = (int64_t *) (base + o_vpre), *fpre = (int64_t *) (base + o_fpre);
     int64_t *scr = (int64_t *) (base + o_scr), *misc = (int64_t *) (base + o_misc);
-    float *v2 = (float *) (base + o_v2);
-    int32_t *f2 = (int32_t *) (base + o_f2);
-    int64_t *k2 = (int64_t *) (base + o_k2);
+    int64_t *rv_src = (int64_t *) (base + o_rv), *rv_dst = rv_src + nrel;
+    int64_t *rf_src = (int64_t *) (base + o_rf), *rf_dst = rf_src + (nhole + 16);
 
     hipLaunchKernelGGL(k_clear64, dim3(grid_for(nslots)), dim3(256), 0, st, keys, nslots, kEmpty);
     hipLaunchKernelGGL(k_init_info, dim3(grid_for(nslots)), dim3(256), 0, st, info, nslots);
@@ -368,17 +445,8 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     hipLaunchKernelGGL(k_clear32, dim3(grid_for(fw)), dim3(256), 0, st, fkeep, fw, 0xffffffffu);
     hipLaunchKernelGGL(k_clear32, dim3(grid_for(fw)), dim3(256), 0, st, sfb, fw, 0u);
     hipLaunchKernelGGL(k_clear64, dim3(1), dim3(64), 0, st, misc, 8, 0);
+    hipLaunchKernelGGL(k_tail_bits, dim3(1), dim3(64), 0, st, vkeep, nv, fkeep, nf);
     ISO_TRY(hipGetLastError());
-    // tail bits beyond nv / nf
-    {
-        uint32_t tailv[2] = {nv % 32 ? ((1u << (nv % 32)) - 1) : 0xffffffffu, 0u};
-        uint32_t tailf[2] = {nf % 32 ? ((1u << (nf % 32)) - 1) : 0xffffffffu, 0u};
-        if (nv % 32) ISO_TRY(hipMemcpyAsync(vkeep + nv / 32, tailv, 8, hipMemcpyHostToDevice, st));
-        else ISO_TRY(hipMemcpyAsync(vkeep + nv / 32, tailv + 1, 4, hipMemcpyHostToDevice, st));
-        if (nf % 32) ISO_TRY(hipMemcpyAsync(fkeep + nf / 32, tailf, 8, hipMemcpyHostToDevice, st));
-        else ISO_TRY(hipMemcpyAsync(fkeep + nf / 32, tailf + 1, 4, hipMemcpyHostToDevice, st));
-        ISO_TRY(hipStreamSynchronize(st));
-    }
     if (nscell)
         hipLaunchKernelGGL(k_scell, dim3(grid_for(nscell)), dim3(256), 0, st, sp.scell, nscell, keys, info,
                            mask);
@@ -395,18 +463,28 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     ISO_TRY(hipGetLastError());
     ISO_TRY(bitmap_prefix(vkeep, vw, vpre, scr, misc + 2, st));
     ISO_TRY(bitmap_prefix(fkeep, fw, fpre, scr, misc + 3, st));
-    hipLaunchKernelGGL(k_compact_v, dim3(grid_for(nv)), dim3(256), 0, st, v, v2, nv, vkeep, vpre, vkeys, k2);
-    hipLaunchKernelGGL(k_compact_f, dim3(grid_for(nf)), dim3(256), 0, st, f, f2, nf, fkeep, fpre, vkeep,
-                       vpre, sfb, keys, info, mask, groups, nv);
-    ISO_TRY(hipGetLastError());
     int64_t h[4];
     ISO_TRY(hipMemcpyAsync(h, misc, sizeof(h), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipStreamSynchronize(st));
     const int64_t nv2 = h[2], nf2 = h[3];
-    ISO_TRY(hipMemcpyAsync(v, v2, sizeof(float) * 3 * nv2, hipMemcpyDeviceToDevice, st));
-    ISO_TRY(hipMemcpyAsync(f, f2, sizeof(int32_t) * 3 * nf2, hipMemcpyDeviceToDevice, st));
-    if (vkeys) ISO_TRY(hipMemcpyAsync(vkeys, k2, sizeof(int64_t) * nv2, hipMemcpyDeviceToDevice, st));
-    ISO_TRY(hipFreeAsync(base, st));
+    const int64_t kv = nv - nv2, kf = nf - nf2;   // removed vertices / dropped faces
+    if (kv > nrel || kf > nhole + 16) return hipErrorInvalidValue;
+    // vertex relocation: tail survivors fill the removed slots
+    if (kv > 0) {
+        hipLaunchKernelGGL(k_reloc_pairs, dim3(grid_for(kv)), dim3(256), 0, st, vkeep, vpre, vw, nv2, kv, rv_src,
+                           rv_dst);
+        hipLaunchKernelGGL(k_move_v, dim3(grid_for(kv)), dim3(256), 0, st, v, vkeys, rv_src, rv_dst, kv);
+    }
+    // face ids: representatives, then relocated tail vertices
+    hipLaunchKernelGGL(k_remap_inplace, dim3(grid_for(nf)), dim3(256), 0, st, f, nf, fkeep, sfb, keys, info, mask,
+                       groups, nv2, rv_src, rv_dst, kv);
+    // face relocation: tail faces fill the dropped slots
+    if (kf > 0) {
+        hipLaunchKernelGGL(k_reloc_pairs, dim3(grid_for(kf)), dim3(256), 0, st, fkeep, fpre, fw, nf2, kf, rf_src,
+                           rf_dst);
+        hipLaunchKernelGGL(k_move_f, dim3(grid_for(kf)), dim3(256), 0, st, f, rf_src, rf_dst, kf);
+    }
+    ISO_TRY(hipGetLastError());
     ISO_TRY(hipStreamSynchronize(st));
     nv = nv2;
     nf = nf2;

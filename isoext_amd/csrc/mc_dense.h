@@ -44,6 +44,7 @@ struct McParams {
     float level;
     int method;        // 0 = nagae, 1 = lorensen
     int all_special;   // treat every vertex as a weld candidate
+    int vec4;          // Z % 4 == 0 and values 16-byte aligned: rows load as float4
     int NI, NJ, NK;    // tiles per axis
     int64_t ntiles;
     float *out_v;

@@ -260,39 +260,69 @@ __global__ void __launch_bounds__(NT) mc_dense_kernel(McParams p) {
     t.nzo = t.nz + (t.tk == p.NK - 1 ? 1 : 0);
     t.nchunk = (t.nz + 1 + 63) >> 6;
 
-    // ---- sweep: stream values once, one ballot per 64-point row chunk ------
+    // ---- sweep: stream values once --------------------------------------------
+    // The first 64*NZC points of each point row are the row's body: 16*NZC
+    // lanes each load 4 consecutive values (one dwordx4), turn them into a
+    // 4-bit inside mask and the 16 lanes of each 64-point chunk pack their
+    // nibbles into the chunk's 64-bit word.  The one-point z halo is loaded
+    // separately.
     {
+        constexpr int LPR = 16 * NZC;        // lanes per row body
+        constexpr int RPW = 64 / LPR;        // rows per wave instruction
+        constexpr int U = 8;                 // row loads in flight per wave
         const int nrow = (t.nx + 1) * (t.ny + 1);
-        const int nitem = nrow * t.nchunk;
-        constexpr int U = 8;
-        for (int base = wave; base < nitem; base += NW * U) {
-            float v[U];
-            bool ok[U];
+        const int body = min(t.nz + 1, G::TZ);
+        const bool vec4 = p.vec4 != 0;
+        const int lrow = lane / LPR, lq = lane % LPR;
+        const int zq = 4 * lq;
+        for (int base = wave * RPW; base < nrow; base += NW * RPW * U) {
+            float v[U][4];
+            int nvalid[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                int it = base + u * NW;
-                ok[u] = false;
-                v[u] = 0.0f;
-                if (it < nitem) {
-                    int c = it % t.nchunk, row = it / t.nchunk;
-                    int py = row % (t.ny + 1), px = row / (t.ny + 1);
-                    int zl = 64 * c + lane;
-                    if (zl <= t.nz) {
-                        const float *src = p.values + ((t.x0 + px) * p.Y + (t.y0 + py)) * p.Z + t.z0 + zl;
-                        v[u] = *src;
-                        ok[u] = true;
+                const int row = base + u * NW * RPW + lrow;
+                nvalid[u] = 0;
+                v[u][0] = v[u][1] = v[u][2] = v[u][3] = 0.0f;
+                if (row < nrow) {
+                    const int py = row % (t.ny + 1), px = row / (t.ny + 1);
+                    const float *src = p.values + ((t.x0 + px) * p.Y + (t.y0 + py)) * p.Z + t.z0 + zq;
+                    const int nv4 = min(4, body - zq);
+                    nvalid[u] = nv4 < 0 ? 0 : nv4;
+                    if (nv4 == 4 && vec4) {
+                        const float4 q = *reinterpret_cast<const float4 *>(src);
+                        v[u][0] = q.x;
+                        v[u][1] = q.y;
+                        v[u][2] = q.z;
+                        v[u][3] = q.w;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (j < nv4) v[u][j] = src[j];
                     }
                 }
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                int it = base + u * NW;
-                uint64_t m = __ballot(ok[u] && inside(v[u], p.level));
-                if (it < nitem && lane == 0) {
-                    int c = it % t.nchunk, row = it / t.nchunk;
-                    int py = row % (t.ny + 1), px = row / (t.ny + 1);
-                    s.w[(px * G::PY + py) * G::PC + c] = m;
+                const int row = base + u * NW * RPW + lrow;
+                uint32_t x = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) x |= (uint32_t) (j < nvalid[u] && inside(v[u][j], p.level)) << j;
+                x |= __shfl_down(x, 1, 16) << 4;
+                x |= __shfl_down(x, 2, 16) << 8;
+                x |= __shfl_down(x, 4, 16) << 16;
+                const uint32_t hi = __shfl_down(x, 8, 16);
+                if (row < nrow && (lq & 15) == 0) {
+                    const int py = row % (t.ny + 1), px = row / (t.ny + 1);
+                    s.w[(px * G::PY + py) * G::PC + (lq >> 4)] = (uint64_t) x | ((uint64_t) hi << 32);
                 }
+            }
+        }
+        // halo point z = 64*NZC (present when the tile has a full z extent)
+        if (t.nz + 1 > G::TZ) {
+            for (int row = tid; row < nrow; row += NT) {
+                const int py = row % (t.ny + 1), px = row / (t.ny + 1);
+                const float hv = p.values[((t.x0 + px) * p.Y + (t.y0 + py)) * p.Z + t.z0 + G::TZ];
+                s.w[(px * G::PY + py) * G::PC + NZC] = inside(hv, p.level) ? 1ull : 0ull;
             }
         }
     }

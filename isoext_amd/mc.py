@@ -63,7 +63,7 @@ def mc_dense_raw(values: torch.Tensor, shape, aabb_min, aabb_max, level: float, 
     L = _lib.lib()
     special_cap = DEFAULT_SPECIAL_CAP
     cap_v, cap_f = _initial_caps(shape)
-    counts = _lib.I64x4()
+    counts = _lib.I64x6()
     for attempt in range(4):
         ws_bytes = L.isoext_mc_dense_workspace_bytes(_lib.i64x3(shape), cfg, special_cap)
         if ws_bytes < 0:
@@ -95,7 +95,7 @@ def mc_dense_raw(values: torch.Tensor, shape, aabb_min, aabb_max, level: float, 
     else:
         raise _lib.IsoextError("marching_cubes: output sizing did not converge")
     with _hint_lock:
-        _size_hint[tuple(shape)] = (max(int(counts[0]), 1), max(int(counts[1]), 1))
+        _size_hint[tuple(shape)] = (max(int(counts[4]), 1), max(int(counts[5]), 1))
     nv, nf = int(counts[0]), int(counts[1])
     v, f = out_v[:nv], out_f[:nf]
     # do not pin a much larger allocation behind a small mesh
