@@ -5,10 +5,12 @@ Workload (the reference metric's configuration): SDF = min over 64 spheres
 method "nagae".  Values are generated on the GPU before timing; a step is one
 isoext_amd.marching_cubes call (kernel + output sizing + host sync).
 
-Prints one JSON line: Gvoxels/s (grid points / step time), the roofline of
-the fused kernel (algorithmic bytes 4*N_points + 12*V + 12*F over its
-HIP-event duration, against 8 TB/s), and the CPU oracle timed on a bounded
-slab of the same field on this host.
+Prints one JSON line: Gvoxels/s (grid points / step time); the roofline of
+the dominant kernel, classify (it streams every value once: algorithmic
+bytes 4*N_points over its mean HIP-event duration, against 8 TB/s); the
+whole extraction's algorithmic bytes 4*N_points + 12*V + 12*F over its device
+time; and the CPU oracle timed on a bounded slab of the same field on this
+host, on all cores and on one.
 
     python bench.py [--steps K] [--warmup W] [--size N] [--no-cpu]
 """
@@ -47,19 +49,26 @@ def cpu_baseline(values_gpu, planes: int, level: float, method: str):
 
     import oracle
     sample = values_gpu[:planes].contiguous().cpu().numpy()
-    threads = min(16, os.cpu_count() or 1)
+    threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1)
     oracle.mc_dense(sample[:9], level, method, threads=threads)   # warm-up (load, page-in)
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        oracle.mc_dense(sample, level, method, threads=threads)
-        times.append(time.perf_counter() - t0)
-    t = sorted(times)[1]
+
+    def timed(arr, nthreads, reps):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            oracle.mc_dense(arr, level, method, threads=nthreads)
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2]
+    t = timed(sample, threads, 5)
     pts = int(np.prod(sample.shape))
+    one = sample[: max(2, planes // 8)]
+    t1 = timed(one, 1, 3)
     return {"value": pts / t / 1e9, "unit": "Gvoxels/s", "cores": threads, "kind": "port",
             "sample": f"oracle marching cubes ({method}) on the first {planes} x-planes "
                       f"({sample.shape[0]}x{sample.shape[1]}x{sample.shape[2]} points) of the same "
-                      f"field, median of 3 runs, {t:.2f} s each"}
+                      f"field, {threads} threads, median of 5 runs, {t:.2f} s each",
+            "single_core": {"value": int(np.prod(one.shape)) / t1 / 1e9, "unit": "Gvoxels/s", "cores": 1,
+                            "sample": f"first {one.shape[0]} x-planes, median of 3 runs, {t1:.2f} s each"}}
 
 
 def main():
@@ -84,8 +93,11 @@ def main():
     L = _lib.lib()
     m = METHODS[args.method]
 
+    import ctypes
     L.isoext_set_timing(1)
     kernel_ms = []
+    stages = []
+    st = (ctypes.c_float * 5)()
     nv = nf = 0
     cfg = None
     for _ in range(args.warmup):
@@ -95,6 +107,8 @@ def main():
     for _ in range(args.steps):
         v, f, info = mc_dense_raw(grid.values, shape, grid.aabb_min, grid.aabb_max, 0.0, m, args.cfg)
         kernel_ms.append(L.isoext_last_kernel_ms())
+        L.isoext_mc_last_stage_ms(st)
+        stages.append(list(st))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     L.isoext_set_timing(0)
@@ -103,8 +117,12 @@ def main():
     npts = N * N * N
     kms = sorted(kernel_ms)[len(kernel_ms) // 2]
     kmean = sum(kernel_ms) / len(kernel_ms)
+    names = ["classify", "count", "scan", "emit", "total"]
+    stage_mean = {k: sum(s[i] for s in stages) / len(stages) for i, k in enumerate(names)}
     alg_bytes = 4 * npts + 12 * nv + 12 * nf
-    achieved = alg_bytes / (kmean * 1e-3) / 1e9
+    pipe_gbs = alg_bytes / (stage_mean["total"] * 1e-3) / 1e9
+    # dominant kernel: classify reads every value once (4 bytes per point)
+    achieved = 4 * npts / (stage_mean["classify"] * 1e-3) / 1e9
     traffic = None
     tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "mc_dense_1024_traffic.json")
     if os.path.exists(tfile) and N == 1024:
@@ -134,9 +152,13 @@ def main():
             "tile_config": cfg,
             "kernel_ms_median": kms,
             "kernel_ms_mean": kmean,
+            "stage_ms_mean": stage_mean,
+            "pipeline": {"alg_bytes": alg_bytes, "device_ms": stage_mean["total"], "achieved_GBs": pipe_gbs,
+                         "frac": pipe_gbs / HBM_PEAK_GBS},
             "cell_rate_gcells_s": (N - 1) ** 3 / (ms_step * 1e-3) / 1e9,
         },
         "roofline": {
+            "kernel": "k_classify (4 B per grid point read)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -38,11 +38,14 @@ enum {
 const char *isoext_last_error(void);
 int isoext_version(void);
 
-/* Instrumentation (no reference counterpart): when enabled, extraction calls
- * bracket their main kernel with HIP events on the caller's stream and
- * isoext_last_kernel_ms() returns its duration for the calling thread. */
+/* Instrumentation (no reference counterpart): when enabled, dense marching
+ * cubes brackets its kernels with HIP events on the caller's stream;
+ * isoext_last_kernel_ms() returns the extraction's device time (first to
+ * last kernel, weld fix-up excluded) and isoext_mc_last_stage_ms() the
+ * stages: classify, count, scan, emit, total.  Per calling thread. */
 void isoext_set_timing(int on);
 float isoext_last_kernel_ms(void);
+void isoext_mc_last_stage_ms(float out[5]);
 
 /* ---- grids ---------------------------------------------------------------- */
 

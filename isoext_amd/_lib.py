@@ -47,6 +47,8 @@ def _declare(L):
     L.isoext_set_timing.argtypes = [c_int]
     L.isoext_last_kernel_ms.restype = c_f
     L.isoext_last_kernel_ms.argtypes = []
+    L.isoext_mc_last_stage_ms.restype = None
+    L.isoext_mc_last_stage_ms.argtypes = [c_f * 5]
     L.isoext_mc_dense_workspace_bytes.restype = c_i64
     L.isoext_mc_dense_workspace_bytes.argtypes = [I64x3, c_int, c_i64]
     L.isoext_mc_dense.restype = c_int

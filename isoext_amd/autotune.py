@@ -1,7 +1,8 @@
-"""Tile-shape selection for the fused marching-cubes kernel.
+"""Tile-shape selection for the marching-cubes emit kernel.
 
-The library compiles a handful of tile configurations (TX x TY x 64*NZC
-cells per workgroup).  ``mc_config(shape)`` returns the configuration to use
+The library compiles a handful of emit tile configurations (TX x TY cell
+rows x NZC 64-cell chunks per workgroup; NZC is also the z-segment length of
+the count/scan pass).  ``mc_config(shape)`` returns the configuration to use
 for a grid shape: a measured choice from the tuning table if one exists for
 the shape's size class, else a static rule.  ``tune(shape)`` times every
 configuration on the current GPU with HIP events and records the fastest in
@@ -20,7 +21,7 @@ _lock = threading.Lock()
 _table = None
 
 # (TX, TY, NZC) of each compiled configuration, in library order
-CONFIGS = [(16, 16, 4), (32, 16, 4), (16, 16, 2), (8, 8, 4), (32, 32, 4), (16, 8, 4)]
+CONFIGS = [(16, 16, 4), (32, 16, 4), (16, 16, 2), (16, 16, 1), (16, 16, 8), (32, 16, 2)]
 
 
 def size_class(shape) -> str:

@@ -1,0 +1,110 @@
+"""Ragged batches of dense marching-cubes extractions.
+
+``mc_batch(grids)`` extracts every grid of a list of (values, shape, aabb,
+level, method) items with one launch per stage for the whole batch
+(classify, count, scan, emit), in two phases with a single host
+synchronisation each: phase 1 sizes every mesh, the outputs are allocated,
+phase 2 writes them.  The launch sequence of each phase is captured once in
+a HIP graph per (device, stream, workspace, tile config) and replayed; the
+batch content travels in a pinned descriptor buffer that the graph copies to
+the device.  Results are identical to one ``marching_cubes`` call per grid.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib, autotune, workspace
+
+
+class McItem(ctypes.Structure):
+    _fields_ = [("values", ctypes.c_void_p), ("shape", ctypes.c_int64 * 3), ("aabb_min", ctypes.c_float * 3),
+                ("aabb_max", ctypes.c_float * 3), ("level", ctypes.c_float), ("method", ctypes.c_int),
+                ("out_v", ctypes.c_void_p), ("cap_v", ctypes.c_int64), ("out_f", ctypes.c_void_p),
+                ("cap_f", ctypes.c_int64)]
+
+
+_declared = False
+_decl_lock = threading.Lock()
+
+
+def _decl():
+    global _declared
+    with _decl_lock:
+        if _declared:
+            return _lib.lib()
+        L = _lib.lib()
+        vp, i64, c_int = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+        L.isoext_mc_batch_workspace_bytes.restype = i64
+        L.isoext_mc_batch_workspace_bytes.argtypes = [c_int, ctypes.POINTER(McItem), c_int, i64]
+        L.isoext_mc_batch_count.restype = c_int
+        L.isoext_mc_batch_count.argtypes = [c_int, ctypes.POINTER(McItem), c_int, vp, i64, i64,
+                                            ctypes.POINTER(i64), vp]
+        L.isoext_mc_batch_emit.restype = c_int
+        L.isoext_mc_batch_emit.argtypes = [c_int, ctypes.POINTER(McItem), c_int, vp, i64, i64,
+                                           ctypes.POINTER(i64), vp]
+        _declared = True
+        return L
+
+
+@dataclass
+class GridJob:
+    values: torch.Tensor
+    level: float = 0.0
+    method: int = 0
+    aabb_min: tuple = (-1.0, -1.0, -1.0)
+    aabb_max: tuple = (1.0, 1.0, 1.0)
+
+
+BATCH_SPECIAL_CAP = 4096
+
+
+def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_batch"):
+    """Marching cubes over a ragged list of dense grids.  Returns a list of
+    (v, f) pairs (zero-row tensors for empty meshes)."""
+    if not jobs:
+        return []
+    L = _decl()
+    dev = jobs[0].values.device
+    n = len(jobs)
+    items = (McItem * n)()
+    for it, j in zip(items, jobs):
+        if j.values.device != dev or j.values.dtype != torch.float32 or not j.values.is_contiguous():
+            raise TypeError("mc_batch: values must be contiguous float32 tensors on one device")
+        it.values = j.values.data_ptr()
+        for a in range(3):
+            it.shape[a] = int(j.values.shape[a])
+            it.aabb_min[a] = float(j.aabb_min[a])
+            it.aabb_max[a] = float(j.aabb_max[a])
+        it.level = float(j.level)
+        it.method = int(j.method)
+    if cfg is None:
+        cfg = autotune.mc_config(max((tuple(j.values.shape) for j in jobs), key=lambda s: s[0] * s[1] * s[2]))
+    special = BATCH_SPECIAL_CAP
+    counts = (ctypes.c_int64 * (2 * n))()
+    stream = _lib.stream_ptr(dev)
+    with torch.cuda.device(dev):
+        ws_bytes = L.isoext_mc_batch_workspace_bytes(n, items, cfg, special)
+        if ws_bytes < 0:
+            raise ValueError(_lib.last_error())
+        ws = workspace.get(ws_kind, ws_bytes, dev)
+        _lib.check(L.isoext_mc_batch_count(n, items, cfg, ws.data_ptr(), ws.numel(), special, counts, stream),
+                   "mc_batch count")
+        outs = []
+        for i, it in enumerate(items):
+            nv, nf = int(counts[2 * i]), int(counts[2 * i + 1])
+            v = torch.empty((max(nv, 1), 3), dtype=torch.float32, device=dev)
+            f = torch.empty((max(nf, 1), 3), dtype=torch.int32, device=dev)
+            it.out_v, it.cap_v, it.out_f, it.cap_f = v.data_ptr(), nv, f.data_ptr(), nf
+            outs.append((v, f))
+        rc = L.isoext_mc_batch_emit(n, items, cfg, ws.data_ptr(), ws.numel(), special, counts, stream)
+    _lib.check(rc, "mc_batch emit")
+    res = []
+    for i, (v, f) in enumerate(outs):
+        nv, nf = int(counts[2 * i]), int(counts[2 * i + 1])
+        res.append((v[:nv], f[:nf]))
+    return res

@@ -52,8 +52,6 @@ __global__ void k_bwd_dense(const float *grad_v, const int64_t *keys, int64_t nv
     if (gb != 0.0f) atomicAdd(&grad_values[q], gb);
 }
 
-__constant__ const uint8_t kA[12] = {0, 1, 2, 0, 4, 5, 6, 4, 0, 1, 3, 2};
-__constant__ const uint8_t kBc[12] = {1, 3, 3, 2, 5, 7, 7, 6, 4, 5, 7, 6};
 
 __global__ void k_bwd_sparse(const float *grad_v, const int64_t *vsrc, int64_t nv, const float *values,
                              const int64_t *ids, DenseGeom g, float level, float *grad_values) {
@@ -65,7 +63,7 @@ __global__ void k_bwd_sparse(const float *grad_v, const int64_t *vsrc, int64_t n
     int64_t id = ids[slot];
     int64_t cz = g.Z - 1, cy = g.Y - 1;
     int64_t z = id % cz, r = id / cz, y = r % cy, x = r / cy;
-    int ca = kA[e], cb = kBc[e];
+    int ca = edge_a(e), cb = edge_b(e);
     float pa[3] = {grid_coord(x + (ca >> 2), g.X, g.amin[0], g.amax[0]),
                    grid_coord(y + ((ca >> 1) & 1), g.Y, g.amin[1], g.amax[1]),
                    grid_coord(z + (ca & 1), g.Z, g.amin[2], g.amax[2])};

@@ -25,6 +25,7 @@ thread_local std::string g_err;
 // Optional HIP-event timing of the main kernel of the last call.
 bool g_timing = false;
 thread_local float g_kernel_ms = -1.0f;
+thread_local iso::McTimes g_times = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
 
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char *fmt, ...) {
@@ -49,34 +50,54 @@ int hip_fail(hipError_t e, const char *what) {
 
 // Workspace layout for dense marching cubes.
 struct McLayout {
-    int ni, nj, nk;
-    int64_t ntiles, pub_per_tile;
-    size_t o_status, o_pub, o_cand, o_scell, o_kept, o_hole, total;
-    int64_t cap;
+    int tx, ty, nzc;
+    int64_t W, S, NI, NJ, NK, nblk;
+    size_t o_bits, o_cnt_v, o_cnt_f, o_bsum_v, o_bsum_f, o_blk_v, o_blk_f;
+    size_t o_ctr, o_scand, o_sscell, o_skept, o_shole;
+    size_t o_cand, o_scell, o_kept, o_hole, total;
+    int64_t cap, seg_cap[4];
 };
 
-constexpr size_t kHeader = 4096;   // ticket, epoch, err, totals, SpecialLists
-constexpr size_t kOffTicket = 0, kOffEpoch = 4, kOffErr = 8, kOffTotals = 16, kOffSp = 64;
+constexpr size_t kHeader = 4096;   // err, totals, SpecialLists
+constexpr size_t kOffErr = 8, kOffTotals = 16, kOffSp = 64;
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 bool mc_layout(const int64_t shape[3], int cfg, int64_t cap, McLayout &L) {
-    if (!iso::mc_config_tiles(cfg, shape, L.ni, L.nj, L.nk, L.pub_per_tile)) return false;
-    L.ntiles = (int64_t) L.ni * L.nj * L.nk;
+    if (!iso::mc_config_geom(cfg, L.tx, L.ty, L.nzc)) return false;
+    const int64_t X = shape[0], Y = shape[1], Z = shape[2];
+    const int64_t R = X * Y;
+    L.W = (Z + 63) / 64;
+    L.S = (L.W + L.nzc - 1) / L.nzc;
+    const int64_t cell_chunks = (Z - 1 + 63) / 64;
+    L.NI = (X - 1 + L.tx - 1) / L.tx;
+    L.NJ = (Y - 1 + L.ty - 1) / L.ty;
+    L.NK = (cell_chunks + L.nzc - 1) / L.nzc;
+    L.nblk = (R * L.S + iso::kMcScanBlock - 1) / iso::kMcScanBlock;
     L.cap = cap;
     size_t o = kHeader;
-    L.o_status = o;
-    o = align256(o + sizeof(uint64_t) * L.ntiles);
-    L.o_pub = o;
-    o = align256(o + sizeof(uint64_t) * L.ntiles * L.pub_per_tile);
-    L.o_cand = o;
-    o = align256(o + sizeof(iso::SpecialVertex) * cap);
-    L.o_scell = o;
-    o = align256(o + sizeof(iso::SpecialEdge) * cap * 12);
-    L.o_kept = o;
-    o = align256(o + sizeof(int64_t) * cap * 5);
-    L.o_hole = o;
-    o = align256(o + sizeof(int64_t) * cap * 5);
+    auto take = [&](size_t n) {
+        size_t r = o;
+        o = align256(o + n);
+        return r;
+    };
+    L.o_bits = take(sizeof(uint64_t) * R * L.W);
+    L.o_cnt_v = take(sizeof(uint32_t) * R * L.S);
+    L.o_cnt_f = take(sizeof(uint32_t) * R * L.S);
+    L.o_bsum_v = take(sizeof(uint32_t) * L.nblk);
+    L.o_bsum_f = take(sizeof(uint32_t) * L.nblk);
+    L.o_blk_v = take(sizeof(int64_t) * L.nblk);
+    L.o_blk_f = take(sizeof(int64_t) * L.nblk);
+    for (int k = 0; k < 4; ++k) L.seg_cap[k] = iso::sp_seg_cap(cap, k);
+    L.o_ctr = take(sizeof(unsigned long long) * 4 * iso::kSpSeg * iso::kSpStride);
+    L.o_scand = take(sizeof(iso::SpecialVertex) * iso::kSpSeg * L.seg_cap[0]);
+    L.o_sscell = take(sizeof(iso::SpecialEdge) * iso::kSpSeg * L.seg_cap[1]);
+    L.o_skept = take(sizeof(int64_t) * iso::kSpSeg * L.seg_cap[2]);
+    L.o_shole = take(sizeof(int64_t) * iso::kSpSeg * L.seg_cap[3]);
+    L.o_cand = take(sizeof(iso::SpecialVertex) * iso::kSpSeg * L.seg_cap[0]);
+    L.o_scell = take(sizeof(iso::SpecialEdge) * iso::kSpSeg * L.seg_cap[1]);
+    L.o_kept = take(sizeof(int64_t) * iso::kSpSeg * L.seg_cap[2]);
+    L.o_hole = take(sizeof(int64_t) * iso::kSpSeg * L.seg_cap[3]);
     L.total = o;
     return true;
 }
@@ -270,6 +291,14 @@ void isoext_set_timing(int on) { g_timing = on != 0; }
 
 float isoext_last_kernel_ms(void) { return g_kernel_ms; }
 
+void isoext_mc_last_stage_ms(float out[5]) {
+    out[0] = g_times.classify;
+    out[1] = g_times.count;
+    out[2] = g_times.scan;
+    out[3] = g_times.emit;
+    out[4] = g_times.total;
+}
+
 int isoext_version(void) { return ISOEXT_ABI_VERSION; }
 
 int isoext_mc_config_count(void) { return iso::mc_config_count(); }
@@ -320,10 +349,10 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     char *ws = (char *) workspace;
 
     iso::SpecialLists sp_host{};
-    sp_host.cap_cand = special_cap;
-    sp_host.cap_scell = special_cap * 12;
-    sp_host.cap_kept = special_cap * 5;
-    sp_host.cap_hole = special_cap * 5;
+    sp_host.cap_cand = iso::kSpSeg * L.seg_cap[0];
+    sp_host.cap_scell = iso::kSpSeg * L.seg_cap[1];
+    sp_host.cap_kept = iso::kSpSeg * L.seg_cap[2];
+    sp_host.cap_hole = iso::kSpSeg * L.seg_cap[3];
     sp_host.cand = (iso::SpecialVertex *) (ws + L.o_cand);
     sp_host.scell = (iso::SpecialEdge *) (ws + L.o_scell);
     sp_host.kept = (int64_t *) (ws + L.o_kept);
@@ -342,28 +371,38 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     p.method = method;
     p.all_special = grid_well_separated(shape, aabb_min, aabb_max) ? 0 : 1;
     p.vec4 = (shape[2] % 4 == 0 && ((uintptr_t) values & 15) == 0) ? 1 : 0;
-    p.NI = L.ni;
-    p.NJ = L.nj;
-    p.NK = L.nk;
-    p.ntiles = L.ntiles;
+    p.W = L.W;
+    p.nzc = L.nzc;
+    p.S = L.S;
+    p.NI = L.NI;
+    p.NJ = L.NJ;
+    p.NK = L.NK;
+    p.bits = (uint64_t *) (ws + L.o_bits);
+    p.cnt_v = (uint32_t *) (ws + L.o_cnt_v);
+    p.cnt_f = (uint32_t *) (ws + L.o_cnt_f);
+    p.bsum_v = (uint32_t *) (ws + L.o_bsum_v);
+    p.bsum_f = (uint32_t *) (ws + L.o_bsum_f);
+    p.blk_v = (int64_t *) (ws + L.o_blk_v);
+    p.blk_f = (int64_t *) (ws + L.o_blk_f);
+    p.nblk = L.nblk;
     p.out_v = out_v;
     p.cap_v = out_v ? cap_v : 0;
     p.out_key = out_key;
     p.out_f = out_f;
     p.cap_f = out_f ? cap_f : 0;
-    p.status = (uint64_t *) (ws + L.o_status);
-    p.ticket = (uint32_t *) (ws + kOffTicket);
-    p.epoch_ptr = (uint32_t *) (ws + kOffEpoch);
-    p.pub = (uint64_t *) (ws + L.o_pub);
     p.err = (int *) (ws + kOffErr);
     p.totals = (int64_t *) (ws + kOffTotals);
     p.sp = (iso::SpecialLists *) (ws + kOffSp);
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (g_timing) {
-        HIP_CHECK(hipEventCreate(&ev0), "event");
-        HIP_CHECK(hipEventCreate(&ev1), "event");
-    }
-    HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, ev0, ev1), "launch marching cubes");
+    p.seg.ctr = (unsigned long long *) (ws + L.o_ctr);
+    for (int k = 0; k < 4; ++k) p.seg.cap[k] = L.seg_cap[k];
+    p.seg.cand = (iso::SpecialVertex *) (ws + L.o_scand);
+    p.seg.scell = (iso::SpecialEdge *) (ws + L.o_sscell);
+    p.seg.kept = (int64_t *) (ws + L.o_skept);
+    p.seg.hole = (int64_t *) (ws + L.o_shole);
+    hipEvent_t ev[5] = {};
+    if (g_timing)
+        for (auto &e : ev) HIP_CHECK(hipEventCreate(&e), "event");
+    HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, g_timing ? ev : nullptr), "launch marching cubes");
 
     // read back counts and the special lists' descriptor
     struct {
@@ -377,20 +416,23 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
               "read special lists");
     HIP_CHECK(hipStreamSynchronize(st), "marching cubes");
     if (g_timing) {
-        float ms = -1.0f;
-        if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) g_kernel_ms = ms;
-        (void) hipEventDestroy(ev0);
-        (void) hipEventDestroy(ev1);
+        float ms[5] = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
+        for (int i = 0; i < 4; ++i) (void) hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
+        (void) hipEventElapsedTime(&ms[4], ev[0], ev[4]);
+        g_times = {ms[0], ms[1], ms[2], ms[3], ms[4]};
+        g_kernel_ms = ms[4];
+        for (auto &e : ev) (void) hipEventDestroy(e);
     }
     int64_t nv = hdr.totals[0], nf = hdr.totals[1];
     counts[0] = counts[4] = nv;
     counts[1] = counts[5] = nf;
-    int64_t need = (int64_t) std::max<unsigned long long>(
-        sp_dev.n_cand, std::max<unsigned long long>((sp_dev.n_scell + 11) / 12,
-                                                    std::max((sp_dev.n_kept + 4) / 5, (sp_dev.n_hole + 4) / 5)));
+    int64_t need = std::max<int64_t>(
+        sp_dev.need,
+        (int64_t) std::max<unsigned long long>(
+            sp_dev.n_cand, std::max<unsigned long long>((sp_dev.n_scell + 11) / 12,
+                                                        std::max((sp_dev.n_kept + 4) / 5, (sp_dev.n_hole + 4) / 5))));
     counts[2] = need;
     counts[3] = hdr.err;
-    if (hdr.err & iso::ISO_ERR_SPIN) return fail(ISOEXT_E_DEVICE, "inter-workgroup wait timed out");
     if (hdr.err & iso::ISO_ERR_TOO_LARGE)
         return fail(ISOEXT_E_TOO_LARGE, "mesh exceeds 2^31-1 vertices or faces");
     if (nv > cap_v || nf > cap_f || (hdr.err & iso::ISO_ERR_CAPACITY))

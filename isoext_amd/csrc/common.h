@@ -16,11 +16,16 @@
 namespace iso {
 
 // Corner k of a cell sits at (k >> 2, (k >> 1) & 1, k & 1); edge e joins
-// corners kEdgeA[e] < kEdgeB[e] (include/shared_luts.cuh:34-38).
+// corners edge_a(e) < edge_b(e) (include/shared_luts.cuh:34-38).
 // Edge axis: 0 = x, 1 = y, 2 = z.  Owner corner = the lower endpoint.
-static __constant__ const uint8_t kEdgeA[12] = {0, 1, 2, 0, 4, 5, 6, 4, 0, 1, 3, 2};
-static __constant__ const uint8_t kEdgeB[12] = {1, 3, 3, 2, 5, 7, 7, 6, 4, 5, 7, 6};
-static __constant__ const uint8_t kEdgeAxis[12] = {2, 1, 2, 1, 2, 1, 2, 1, 0, 0, 0, 0};
+// Packed in immediates (4-bit / 2-bit fields) so that a lookup with a
+// per-lane edge index is ALU work rather than a memory load.
+constexpr uint64_t kEdgeAPacked = 0x231046540210ull;   // 0, 1, 2, 0, 4, 5, 6, 4, 0, 1, 3, 2
+constexpr uint64_t kEdgeBPacked = 0x675467752331ull;   // 1, 3, 3, 2, 5, 7, 7, 6, 4, 5, 7, 6
+constexpr uint32_t kEdgeAxisPacked = 0x6666u;          // 2, 1, 2, 1, 2, 1, 2, 1, 0, 0, 0, 0
+ISO_HD constexpr int edge_a(int e) { return (int) ((kEdgeAPacked >> (4 * e)) & 15); }
+ISO_HD constexpr int edge_b(int e) { return (int) ((kEdgeBPacked >> (4 * e)) & 15); }
+ISO_HD constexpr int edge_axis(int e) { return (int) ((kEdgeAxisPacked >> (2 * e)) & 3); }
 
 ISO_HD int32_t f2i_bits(float f) {
     union {
@@ -96,21 +101,5 @@ ISO_HD uint64_t mix64(uint64_t x) {
     x ^= x >> 33;
     return x;
 }
-
-// Agent-scope relaxed atomics: every cross-workgroup word carries its own
-// payload, so no fences are needed for it to be read consistently.
-__device__ __forceinline__ uint64_t ld_agent(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t ld_agent32(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bounded spin: after this many polls a wait gives up and raises an error
-// flag instead of hanging the device.
-constexpr int kSpinLimit = 1 << 22;
 
 }   // namespace iso

@@ -79,7 +79,7 @@ __device__ __forceinline__ int crossing12(int cs) {
     int st = 0;
 #pragma unroll
     for (int e = 0; e < 12; ++e)
-        if (((cs >> kEdgeA[e]) & 1) != ((cs >> kEdgeB[e]) & 1)) st |= 1 << e;
+        if (((cs >> edge_a(e)) & 1) != ((cs >> edge_b(e)) & 1)) st |= 1 << e;
     return st;
 }
 
@@ -215,7 +215,7 @@ __global__ void k_its_emit(CellSrc s, float level, const int64_t *bsum_pre, ItsO
     const int st = crossing12(cs);
     for (int k = 0; k < 12; ++k) {
         if (!((st >> k) & 1)) continue;
-        int ca = kEdgeA[k], cb = kEdgeB[k];
+        int ca = edge_a(k), cb = edge_b(k);
         float t = edge_t(cv[ca], cv[cb], level);
         float ax = (ca >> 2) ? hi[0] : lo[0], ay = ((ca >> 1) & 1) ? hi[1] : lo[1], az = (ca & 1) ? hi[2] : lo[2];
         float bx = (cb >> 2) ? hi[0] : lo[0], by = ((cb >> 1) & 1) ? hi[1] : lo[1], bz = (cb & 1) ? hi[2] : lo[2];
@@ -445,7 +445,7 @@ __device__ __forceinline__ int64_t active_of(const ItsIn &in, const CellSrc &s, 
 __device__ bool entry_quad(const ItsIn &in, const CellSrc &s, int64_t e, int64_t q[4]) {
     int k = in.edge_local[e];
     if (k != 0 && k != 3 && k != 8) return false;
-    const int axis = kEdgeAxis[k];
+    const int axis = edge_axis(k);
     int64_t x, y, z;
     cell_xyz(s, in.cell_dense[in.entry_cell[e]], x, y, z);
     for (int j = 0; j < 4; ++j) {

@@ -35,7 +35,32 @@ struct SpecialLists {
     SpecialEdge *scell;    // crossing edges of special cells (one entry per cell)
     int64_t *kept;         // face slots kept in special cells
     int64_t *hole;         // face slots dropped as degenerate
+    int64_t need;          // special capacity a call needs (segment overflow included)
 };
+
+// The emit kernel appends special entries through kSpSeg independent
+// counters per list (workgroup b uses segment b % kSpSeg, each counter on its
+// own 256-byte line), so the atomics of different workgroups do not
+// serialise on one address; a gather pass concatenates the segments into
+// the contiguous SpecialLists.
+constexpr int kSpSeg = 256;
+constexpr int kSpStride = 32;
+struct SpecialSegs {
+    unsigned long long *ctr;   // [4][kSpSeg * kSpStride]: cand, scell, kept, hole
+    int64_t cap[4];            // entries per segment
+    SpecialVertex *cand;       // [kSpSeg * cap[0]]
+    SpecialEdge *scell;        // [kSpSeg * cap[1]]
+    int64_t *kept, *hole;      // [kSpSeg * cap[2]], [kSpSeg * cap[3]]
+};
+// Entries of each list per unit of special capacity: cand 1, scell 12,
+// kept 5, hole 5.
+inline int64_t sp_seg_cap(int64_t special_cap, int k) {
+    const int64_t mult = k == 1 ? 12 : (k >= 2 ? 5 : 1);
+    return (special_cap * mult + kSpSeg - 1) / kSpSeg;
+}
+
+// Entries of (point row, z-segment) counted and scanned per workgroup.
+constexpr int kMcScanBlock = 1024;
 
 struct McParams {
     const float *values;
@@ -45,25 +70,36 @@ struct McParams {
     int method;        // 0 = nagae, 1 = lorensen
     int all_special;   // treat every vertex as a weld candidate
     int vec4;          // Z % 4 == 0 and values 16-byte aligned: rows load as float4
-    int NI, NJ, NK;    // tiles per axis
-    int64_t ntiles;
+    int64_t W;         // 64-point chunks per point row, ceil(Z / 64)
+    int nzc;           // chunks per z-segment (= emit tile depth)
+    int64_t S;         // z-segments per point row, ceil(W / nzc)
+    int64_t NI, NJ, NK;   // emit tiles per axis
+    uint64_t *bits;    // (X*Y, W) inside masks: bit z%64 of word z/64
+    uint32_t *cnt_v;   // (X*Y*S) vertex counts -> block-local exclusive offsets
+    uint32_t *cnt_f;   // (X*Y*S) face counts   -> block-local exclusive offsets
+    uint32_t *bsum_v, *bsum_f;   // per scan block totals
+    int64_t *blk_v, *blk_f;      // per scan block global exclusive bases
+    int64_t nblk;
     float *out_v;
     int64_t cap_v;
     int64_t *out_key;   // optional: owner point linear id * 3 + axis per vertex
     int32_t *out_f;
     int64_t cap_f;
-    uint64_t *status;   // per ticket: look-back state
-    uint32_t *ticket;
-    uint32_t *epoch_ptr;
-    uint64_t *pub;      // per tile: published boundary vertex ids
     int *err;
     int64_t *totals;    // [0] vertices, [1] faces (provisional)
-    SpecialLists *sp;
+    SpecialLists *sp;   // contiguous lists (written by the gather pass)
+    SpecialSegs seg;    // segmented lists (written by emit)
+};
+
+// Stage timings (ms) of the last timed launch: classify, count, scan, emit,
+// and the whole extraction (first to last kernel).
+struct McTimes {
+    float classify, count, scan, emit, total;
 };
 
 int mc_config_count();
-bool mc_config_tiles(int cfg, const int64_t shape[3], int &ni, int &nj, int &nk, int64_t &pub);
+bool mc_config_geom(int cfg, int &tx, int &ty, int &nzc);
 hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
-                           hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                           hipEvent_t *ev = nullptr);
 
 }   // namespace iso

@@ -1,4 +1,4 @@
-// Dense marching cubes: one fused, read-once pass over the value grid.
+// Dense marching cubes: classify -> count -> scan -> emit.
 //
 // Reference path replaced: mc::marching_cubes on a UniformGrid
 // (src/mc/mc.cu:17-68 -> get_case_num_op include/utils.cuh:82-102,
@@ -7,29 +7,34 @@
 // src/utils.cu:32-59).
 //
 // Design (gfx950, wave64):
-//   * The grid of cells is cut into tiles of TX x TY x (64*NZC) cells.  A
-//     workgroup takes a ticket (processing order = ticket order, highest
-//     linear tile first) and streams its tile's values plus a one-point
-//     upper halo from HBM exactly once, turning each 64-point row chunk into a
-//     64-bit inside/outside mask with one ballot.  The masks live in LDS.
-//   * Every vertex belongs to one grid edge, every edge to its lower endpoint
-//     ("owner point"), every owner point to one tile.  Vertices are numbered
-//     per tile in (x, y, z-chunk) unit order, then (z, axis) inside a unit, so
-//     a vertex's index is a popcount over the LDS masks.  No hashing and no
-//     sort: welding is by construction.
-//   * Per-unit face and vertex counts are scanned in LDS; tile bases come from
-//     a single-pass decoupled look-back over the tickets.  A tile then
-//     publishes the ids of the vertices on its lower boundary planes, which
-//     its lower neighbours (processed later) reference.
-//   * Active cells gather their 8 corner values (L2/MALL hits), interpolate
-//     with the reference's float op order and write faces and the vertices
-//     they own.  Output is deterministic: faces in tile order, then cell
-//     order, then LUT slot.
+//   * classify streams the value grid once with dwordx4 loads.  16 lanes
+//     turn 64 consecutive z-values of a point row into one 64-bit inside mask
+//     (4-bit nibbles merged with three in-group shuffles); only the masks,
+//     1 bit per point, are written.
+//   * Vertices are numbered in point-major order: a vertex belongs to the
+//     lower endpoint of its grid edge (its owner point) and vertices are
+//     ordered by (owner x, y, z, axis x < y < z).  Faces are ordered by
+//     (cell x, y, z, LUT slot).  Both numberings are a prefix sum over
+//     (point row, z-segment) entries plus a popcount inside the segment, so
+//     every vertex id is a pure function of the masks: welding without
+//     hashing, sorting or any wait between workgroups.
+//   * count derives per-entry vertex and face counts from the masks and
+//     scans them per workgroup; scan_top turns the block totals into global
+//     bases and the mesh size.
+//   * emit tiles the cells (TX x TY cell rows x one z-segment of NZC 64-cell
+//     chunks), stages the mask patch in LDS, rebuilds per-chunk vertex and
+//     face bases, compacts the active cells through an LDS scan and gives
+//     each one lane: it gathers the 8 corner values, interpolates with the
+//     reference's float op order and writes the vertices its cell owns and
+//     its faces.  Output is deterministic.
 //   * Vertices whose position could coincide bitwise with another edge's
 //     vertex (within 16 ulps of a grid corner) are recorded, together with the
 //     faces of their cells, for the exact position-based weld of the fix-up
 //     pass (the reference welds by position equality and drops degenerate
 //     triangles, src/utils.cu:49-55, src/mc/nagae.cu:71).
+
+#include <type_traits>
+#include <utility>
 
 #include "common.h"
 #include "mc_dense.h"
@@ -39,664 +44,707 @@ namespace iso {
 
 static __constant__ uint64_t cTris[2][256];
 
-// Number of triangles of each case, per method (low nibble of the packed LUT).
-__device__ __forceinline__ int tri_count(int method, int c) { return (int) (cTris[method][c] & 15); }
 
-__device__ __forceinline__ uint64_t pack_status(uint64_t state, uint64_t f, uint64_t v) {
-    return (state << 62) | ((f & 0x7fffffffull) << 31) | (v & 0x7fffffffull);
-}
-
-__device__ __forceinline__ int64_t wave_sum64(int64_t x) {
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    return x;
-}
-
-// Append to a device list; returns the slot, or -1 when full (the counter
+// Append to segment `seg` of special list k (0 cand, 1 scell, 2 kept, 3
+// hole); returns the entry index, or -1 when the segment is full (the counter
 // still counts so the host can size a retry).
-__device__ __forceinline__ int64_t list_push(unsigned long long *counter, int64_t cap) {
-    unsigned long long i = atomicAdd(counter, 1ull);
-    return (int64_t) i < cap ? (int64_t) i : -1;
+__device__ __forceinline__ int64_t seg_push(const McParams &p, int k, int seg) {
+    const unsigned long long i = atomicAdd(&p.seg.ctr[(k * kSpSeg + seg) * kSpStride], 1ull);
+    return (int64_t) i < p.seg.cap[k] ? (int64_t) seg * p.seg.cap[k] + (int64_t) i : -1;
 }
 
-template <int TX, int TY, int NZC>
-struct TileGeom {
-    static constexpr int PX = TX + 1;            // point planes incl. halo
-    static constexpr int PY = TY + 1;            // point rows incl. halo
-    static constexpr int PC = NZC + 1;           // 64-point chunks incl. halo
-    static constexpr int TZ = 64 * NZC;          // cells along z
-    static constexpr int NWORDS = PX * PY * PC;  // LDS mask words
-    static constexpr int NUNITS = PX * PY * PC;  // count units (point rows x chunks)
-    // publish buffer layout per tile (entries of 64 bits)
-    static constexpr int PUB_X = PY * (TZ + 1) * 2;   // plane px=0: (py, z) x {y, z}
-    static constexpr int PUB_Y = PX * (TZ + 1) * 2;   // plane py=0: (px, z) x {x, z}
-    static constexpr int PUB_Z = PX * PY * 2;         // plane z=0: (px, py) x {x, y}
-    static constexpr int PUB = PUB_X + PUB_Y + PUB_Z;
+__device__ __forceinline__ void set_err(const McParams &p, int code) { atomicOr(p.err, code); }
+
+// Bits [0, n), n clamped to [0, 64].
+__device__ __forceinline__ uint64_t low_bits(int64_t n) {
+    return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1));
+}
+
+// Vertices owned by the points of one 64-point chunk of a point row: b the
+// chunk's mask, bn0 bit 0 of the next chunk, bx / by the masks of the rows at
+// x+1 / y+1 (0 when absent), zlo the chunk's first z.
+__device__ __forceinline__ int chunk_vertices(uint64_t b, uint64_t bn0, uint64_t bx, uint64_t by, bool hasx,
+                                              bool hasy, int64_t zlo, int64_t Z) {
+    const uint64_t pv = low_bits(Z - zlo), ev = low_bits(Z - 1 - zlo);
+    const uint64_t zs = (b >> 1) | (bn0 << 63);
+    int n = __popcll((b ^ zs) & ev);
+    if (hasx) n += __popcll((b ^ bx) & pv);
+    if (hasy) n += __popcll((b ^ by) & pv);
+    return n;
+}
+
+// Corner masks of the 64 cells of a chunk of a cell row: m[k] bit j = corner
+// k of cell z = zlo + j is inside.  Corner k sits at (k>>2, (k>>1)&1, k&1)
+// (include/utils.cuh:32-60).
+struct CellMasks {
+    uint64_t m[8];
 };
 
-struct TileCtx {
-    int64_t x0, y0, z0;    // global origin (cells == owner points)
-    int nx, ny, nz;        // cells in this tile
-    int nxo, nyo, nzo;     // owned points (cells, +1 on the grid's upper face)
-    int nchunk;            // loaded 64-point chunks per row (covers nz+1)
-    int64_t tile;          // linear tile index
-    int ti, tj, tk;
-};
-
-template <int TX, int TY, int NZC>
-struct Lds {
-    using G = TileGeom<TX, TY, NZC>;
-    uint64_t w[G::NWORDS];
-    uint32_t fo[G::NUNITS + 1];   // face count, then exclusive offset
-    uint32_t vo[G::NUNITS + 1];   // vertex count, then exclusive offset
-    uint32_t na[G::NUNITS + 1];   // active-cell count, then exclusive offset
-    uint32_t tsum_f[16], tsum_v[16], tsum_a[16];
-    int64_t fbase, vbase;
-    int64_t ticket;
-};
-
-template <int TX, int TY, int NZC>
-__device__ __forceinline__ uint64_t W(const Lds<TX, TY, NZC> &s, int px, int py, int c) {
-    using G = TileGeom<TX, TY, NZC>;
-    return s.w[(px * G::PY + py) * G::PC + c];
+__device__ __forceinline__ CellMasks cell_masks(uint64_t a, uint64_t an, uint64_t b, uint64_t bn, uint64_t d0,
+                                                uint64_t d0n, uint64_t d1, uint64_t d1n) {
+    CellMasks c;
+    c.m[0] = a;
+    c.m[1] = (a >> 1) | ((an & 1) << 63);
+    c.m[2] = b;
+    c.m[3] = (b >> 1) | ((bn & 1) << 63);
+    c.m[4] = d0;
+    c.m[5] = (d0 >> 1) | ((d0n & 1) << 63);
+    c.m[6] = d1;
+    c.m[7] = (d1 >> 1) | ((d1n & 1) << 63);
+    return c;
 }
 
-// Crossing masks of the three edges owned by the points of unit (px, py, c):
-// bit b = point z = 64c + b (tile-local).  Only owned points with existing
-// edges are set.
-template <int TX, int TY, int NZC>
-__device__ __forceinline__ void unit_masks(const Lds<TX, TY, NZC> &s, const McParams &p,
-                                           const TileCtx &t, int px, int py, int c, uint64_t &xc,
-                                           uint64_t &yc, uint64_t &zc) {
-    uint64_t a = W(s, px, py, c);
-    uint64_t an = (c + 1 < t.nchunk) ? W(s, px, py, c + 1) : 0ull;
-    uint64_t as = (a >> 1) | (an << 63);
-    int zlo = 64 * c;
-    int nown = t.nzo - zlo;   // owned points in this chunk
-    uint64_t own = nown >= 64 ? ~0ull : (nown <= 0 ? 0ull : ((1ull << nown) - 1));
-    // z-edge exists for global z < Z-1  <=>  local z < nz (cells along z)
-    int nze = t.nz - zlo;
-    uint64_t zexist = nze >= 64 ? ~0ull : (nze <= 0 ? 0ull : ((1ull << nze) - 1));
-    zc = (a ^ as) & own & zexist;
-    yc = 0;
-    xc = 0;
-    if (t.y0 + py < p.Y - 1) yc = (a ^ W(s, px, py + 1, c)) & own;
-    if (t.x0 + px < p.X - 1) xc = (a ^ W(s, px + 1, py, c)) & own;
+__device__ __forceinline__ uint64_t cell_active(const CellMasks &c) {
+    uint64_t o = 0, a = ~0ull;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        o |= c.m[k];
+        a &= c.m[k];
+    }
+    return o & ~a;
 }
 
-template <int TX, int TY, int NZC>
-__device__ __forceinline__ int unit_index(int px, int py, int c) {
-    using G = TileGeom<TX, TY, NZC>;
-    return (px * G::PY + py) * G::PC + c;
-}
-
-// Tile-local vertex id of the edge (point (px,py,z), axis) owned by this tile.
-template <int TX, int TY, int NZC>
-__device__ __forceinline__ int64_t local_vid(const Lds<TX, TY, NZC> &s, const McParams &p,
-                                             const TileCtx &t, int px, int py, int z, int axis) {
-    int c = z >> 6, b = z & 63;
-    uint64_t xc, yc, zc;
-    unit_masks(s, p, t, px, py, c, xc, yc, zc);
-    uint64_t below = (b == 0) ? 0ull : (~0ull >> (64 - b));
-    int64_t r = __popcll(xc & below) + __popcll(yc & below) + __popcll(zc & below);
-    if (axis >= 1) r += (xc >> b) & 1;
-    if (axis >= 2) r += (yc >> b) & 1;
-    return (int64_t) s.vo[unit_index<TX, TY, NZC>(px, py, c)] + r;
-}
-
-// Publish-buffer slot of an in-plane edge on a tile's lower boundary.
-// plane 0: px == 0 (edges y, z); 1: py == 0 (x, z); 2: z == 0 (x, y).
-template <int TX, int TY, int NZC>
-__device__ __forceinline__ int64_t pub_slot(int plane, int lx, int ly, int lz, int axis) {
-    using G = TileGeom<TX, TY, NZC>;
-    if (plane == 0) return (int64_t) (ly * (G::TZ + 1) + lz) * 2 + (axis == 1 ? 0 : 1);
-    if (plane == 1) return G::PUB_X + (int64_t) (lx * (G::TZ + 1) + lz) * 2 + (axis == 0 ? 0 : 1);
-    return G::PUB_X + G::PUB_Y + (int64_t) (lx * G::PY + ly) * 2 + (axis == 0 ? 0 : 1);
-}
-
-__device__ __forceinline__ void set_err(const McParams &p, int code) {
-    atomicOr(p.err, code);
+__device__ __forceinline__ int cell_case(const CellMasks &c, int j) {
+    int cs = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cs |= (int) ((c.m[k] >> j) & 1) << k;
+    return cs;
 }
 
 // Per-cell interpolation context: corner values and the two grid
-// coordinates of the cell along each axis.
+// coordinates of the cell along each axis.  Only ever indexed with
+// compile-time constants (dynamic indexing would put it in scratch).
 struct CellCtx {
-    float cv[8];
+    float v[8];
     float c[3][2];
     float level;
 };
+
+// Calls f(std::integral_constant<int, E>) for E = 0..11 (unrolled).
+template <typename F, int... E>
+__device__ __forceinline__ void for_edges_impl(F &&f, std::integer_sequence<int, E...>) {
+    (f(std::integral_constant<int, E>{}), ...);
+}
+template <typename F>
+__device__ __forceinline__ void for_edges(F &&f) {
+    for_edges_impl(f, std::make_integer_sequence<int, 12>{});
+}
 
 __device__ __forceinline__ int crossing_status(int cs) {
     int st = 0;
 #pragma unroll
     for (int e = 0; e < 12; ++e)
-        if (((cs >> kEdgeA[e]) & 1) != ((cs >> kEdgeB[e]) & 1)) st |= 1 << e;
+        if (((cs >> edge_a(e)) & 1) != ((cs >> edge_b(e)) & 1)) st |= 1 << e;
     return st;
 }
 
-// Vertex position on crossing edge e, with the reference's op order.
-__device__ __forceinline__ void edge_pos(const CellCtx &cc, int e, float &x, float &y, float &z) {
-    int ca = kEdgeA[e], cb = kEdgeB[e];
-    float tt = edge_t(cc.cv[ca], cc.cv[cb], cc.level);
+// Vertex position on crossing edge E, with the reference's op order.
+template <int E>
+__device__ __forceinline__ void edge_pos_c(const CellCtx &cc, float &x, float &y, float &z) {
+    constexpr int ca = edge_a(E), cb = edge_b(E);
+    const float tt = edge_t(cc.v[ca], cc.v[cb], cc.level);
     x = lerp1(tt, cc.c[0][ca >> 2], cc.c[0][cb >> 2]);
     y = lerp1(tt, cc.c[1][(ca >> 1) & 1], cc.c[1][(cb >> 1) & 1]);
     z = lerp1(tt, cc.c[2][ca & 1], cc.c[2][cb & 1]);
 }
 
-__device__ __forceinline__ bool edge_is_candidate(const CellCtx &cc, int e) {
-    int ca = kEdgeA[e], cb = kEdgeB[e], axis = kEdgeAxis[e];
-    float tt = edge_t(cc.cv[ca], cc.cv[cb], cc.level);
-    int ia = axis == 0 ? (ca >> 2) : (axis == 1 ? ((ca >> 1) & 1) : (ca & 1));
-    int ib = axis == 0 ? (cb >> 2) : (axis == 1 ? ((cb >> 1) & 1) : (cb & 1));
-    float a = cc.c[axis][ia], b = cc.c[axis][ib];
+// Same, for a per-lane edge index (only used in the rare special cells).
+__device__ __forceinline__ void edge_pos_dyn(const CellCtx &cc, int e, float &x, float &y, float &z) {
+    switch (e) {
+    case 0: edge_pos_c<0>(cc, x, y, z); break;
+    case 1: edge_pos_c<1>(cc, x, y, z); break;
+    case 2: edge_pos_c<2>(cc, x, y, z); break;
+    case 3: edge_pos_c<3>(cc, x, y, z); break;
+    case 4: edge_pos_c<4>(cc, x, y, z); break;
+    case 5: edge_pos_c<5>(cc, x, y, z); break;
+    case 6: edge_pos_c<6>(cc, x, y, z); break;
+    case 7: edge_pos_c<7>(cc, x, y, z); break;
+    case 8: edge_pos_c<8>(cc, x, y, z); break;
+    case 9: edge_pos_c<9>(cc, x, y, z); break;
+    case 10: edge_pos_c<10>(cc, x, y, z); break;
+    default: edge_pos_c<11>(cc, x, y, z); break;
+    }
+}
+
+template <int E>
+__device__ __forceinline__ bool edge_is_candidate(const CellCtx &cc) {
+    constexpr int ca = edge_a(E), cb = edge_b(E), axis = edge_axis(E);
+    constexpr int ia = axis == 0 ? (ca >> 2) : (axis == 1 ? ((ca >> 1) & 1) : (ca & 1));
+    constexpr int ib = axis == 0 ? (cb >> 2) : (axis == 1 ? ((cb >> 1) & 1) : (cb & 1));
+    const float tt = edge_t(cc.v[ca], cc.v[cb], cc.level);
+    const float a = cc.c[axis][ia], b = cc.c[axis][ib];
     return near_endpoint(lerp1(tt, a, b), a, b);
 }
 
 // Global key of edge e of cell (gx, gy, gz): owner point linear id * 3 + axis.
-__device__ __forceinline__ int64_t edge_key(const McParams &p, int64_t gx, int64_t gy, int64_t gz,
-                                            int e) {
-    int ca = kEdgeA[e];
+__device__ __forceinline__ int64_t edge_key(const McParams &p, int64_t gx, int64_t gy, int64_t gz, int e) {
+    int ca = edge_a(e);
     int64_t ox = gx + (ca >> 2), oy = gy + ((ca >> 1) & 1), oz = gz + (ca & 1);
-    return ((ox * p.Y + oy) * p.Z + oz) * 3 + kEdgeAxis[e];
+    return ((ox * p.Y + oy) * p.Z + oz) * 3 + edge_axis(e);
 }
 
-// Global vertex id of edge e of tile-local cell (px, py, zl): from the LDS
-// ranks when this tile owns the edge, else from the owner's published entry.
-template <int TX, int TY, int NZC>
-__device__ __noinline__ int64_t edge_vid(const Lds<TX, TY, NZC> &s, const McParams &p,
-                                         const TileCtx &t, int px, int py, int zl, int e,
-                                         int64_t vbase, uint64_t epoch) {
-    using G = TileGeom<TX, TY, NZC>;
-    int ca = kEdgeA[e], axis = kEdgeAxis[e];
-    int ox = px + (ca >> 2), oy = py + ((ca >> 1) & 1), oz = zl + (ca & 1);
-    if (ox < t.nxo && oy < t.nyo && oz < t.nzo) return vbase + local_vid(s, p, t, ox, oy, oz, axis);
-    int64_t gox = t.x0 + ox, goy = t.y0 + oy, goz = t.z0 + oz;
-    int oi = (int) min<int64_t>(gox / TX, p.NI - 1);
-    int oj = (int) min<int64_t>(goy / TY, p.NJ - 1);
-    int ok = (int) min<int64_t>(goz / G::TZ, p.NK - 1);
-    int lx = (int) (gox - (int64_t) oi * TX), ly = (int) (goy - (int64_t) oj * TY),
-        lz = (int) (goz - (int64_t) ok * G::TZ);
-    // the owner lies above this tile along an axis other than the edge's;
-    // that axis names the boundary plane it published
-    int plane = (oi != t.ti) ? 0 : ((oj != t.tj) ? 1 : 2);
-    int64_t otile = ((int64_t) oi * p.NJ + oj) * p.NK + ok;
-    const uint64_t *src = p.pub + otile * (int64_t) G::PUB + pub_slot<TX, TY, NZC>(plane, lx, ly, lz, axis);
-    uint64_t wv = ld_agent(src);
-    int spins = 0;
-    while ((wv & 0xffffffff00000000ull) != epoch) {
-        if (++spins > kSpinLimit || ((spins & 1023) == 0 && (ld_agent32((const uint32_t *) p.err) & ISO_ERR_SPIN))) {
-            set_err(p, ISO_ERR_SPIN);
-            return -1;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        wv = ld_agent(src);
-    }
-    return (int64_t) (wv & 0xffffffffull);
-}
-
-template <int TX, int TY, int NZC, int NT>
-__global__ void __launch_bounds__(NT) mc_dense_kernel(McParams p) {
-    using G = TileGeom<TX, TY, NZC>;
-    __shared__ Lds<TX, TY, NZC> s;
-    constexpr int NW = NT / 64;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-
-    // ---- ticket -> tile (descending linear order: upper neighbours first) ---
-    if (tid == 0) s.ticket = (int64_t) atomicAdd(p.ticket, 1u);
-    __syncthreads();
-    const int64_t ticket = s.ticket;
-    TileCtx t;
-    t.tile = p.ntiles - 1 - ticket;
-    t.tk = (int) (t.tile % p.NK);
-    t.tj = (int) ((t.tile / p.NK) % p.NJ);
-    t.ti = (int) (t.tile / ((int64_t) p.NK * p.NJ));
-    t.x0 = (int64_t) t.ti * TX;
-    t.y0 = (int64_t) t.tj * TY;
-    t.z0 = (int64_t) t.tk * G::TZ;
-    t.nx = (int) min<int64_t>(TX, p.X - 1 - t.x0);
-    t.ny = (int) min<int64_t>(TY, p.Y - 1 - t.y0);
-    t.nz = (int) min<int64_t>(G::TZ, p.Z - 1 - t.z0);
-    t.nxo = t.nx + (t.ti == p.NI - 1 ? 1 : 0);
-    t.nyo = t.ny + (t.tj == p.NJ - 1 ? 1 : 0);
-    t.nzo = t.nz + (t.tk == p.NK - 1 ? 1 : 0);
-    t.nchunk = (t.nz + 1 + 63) >> 6;
-
-    // ---- sweep: stream values once --------------------------------------------
-    // The first 64*NZC points of each point row are the row's body: 16*NZC
-    // lanes each load 4 consecutive values (one dwordx4), turn them into a
-    // 4-bit inside mask and the 16 lanes of each 64-point chunk pack their
-    // nibbles into the chunk's 64-bit word.  The one-point z halo is loaded
-    // separately.
-    {
-        constexpr int LPR = 16 * NZC;        // lanes per row body
-        constexpr int RPW = 64 / LPR;        // rows per wave instruction
-        constexpr int U = 8;                 // row loads in flight per wave
-        const int nrow = (t.nx + 1) * (t.ny + 1);
-        const int body = min(t.nz + 1, G::TZ);
-        const bool vec4 = p.vec4 != 0;
-        const int lrow = lane / LPR, lq = lane % LPR;
-        const int zq = 4 * lq;
-        for (int base = wave * RPW; base < nrow; base += NW * RPW * U) {
-            float v[U][4];
-            int nvalid[U];
+// ---- classify ------------------------------------------------------------------
+// MODE 0: Z % 64 == 0 and float4-aligned rows (word w covers values
+// [64w, 64w + 64)); 1: Z % 4 == 0 and float4-aligned; 2: scalar loads.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_classify(const float *__restrict__ values, int64_t Z, int64_t W,
+                                                  int64_t nwords, float level, uint64_t *__restrict__ bits) {
+    constexpr int U = 4;   // words in flight per 16-lane group
+    const int lane = threadIdx.x & 63;
+    const int grp = lane >> 4, q = lane & 15;
+    const int64_t wave = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t nwave = (int64_t) gridDim.x * 4;
+    for (int64_t base = wave * (4 * U); base < nwords; base += nwave * (4 * U)) {
+        float v[U][4];
+        uint32_t valid[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int row = base + u * NW * RPW + lrow;
-                nvalid[u] = 0;
-                v[u][0] = v[u][1] = v[u][2] = v[u][3] = 0.0f;
-                if (row < nrow) {
-                    const int py = row % (t.ny + 1), px = row / (t.ny + 1);
-                    const float *src = p.values + ((t.x0 + px) * p.Y + (t.y0 + py)) * p.Z + t.z0 + zq;
-                    const int nv4 = min(4, body - zq);
-                    nvalid[u] = nv4 < 0 ? 0 : nv4;
-                    if (nv4 == 4 && vec4) {
-                        const float4 q = *reinterpret_cast<const float4 *>(src);
-                        v[u][0] = q.x;
-                        v[u][1] = q.y;
-                        v[u][2] = q.z;
-                        v[u][3] = q.w;
-                    } else {
+        for (int u = 0; u < U; ++u) {
+            const int64_t w = base + 4 * u + grp;
+            v[u][0] = v[u][1] = v[u][2] = v[u][3] = 0.0f;
+            valid[u] = 0;
+            if (w < nwords) {
+                if (MODE == 0) {
+                    const float4 f = *reinterpret_cast<const float4 *>(values + 64 * w + 4 * q);
+                    v[u][0] = f.x;
+                    v[u][1] = f.y;
+                    v[u][2] = f.z;
+                    v[u][3] = f.w;
+                    valid[u] = 15;
+                } else {
+                    const int64_t row = w / W, c = w - row * W;
+                    const int64_t z = 64 * c + 4 * q;
+                    const int64_t left = Z - z;
+                    const float *src = values + row * Z + z;
+                    if (left > 0) {
+                        if (MODE == 1) {
+                            const float4 f = *reinterpret_cast<const float4 *>(src);
+                            v[u][0] = f.x;
+                            v[u][1] = f.y;
+                            v[u][2] = f.z;
+                            v[u][3] = f.w;
+                            valid[u] = 15;
+                        } else {
 #pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if (j < nv4) v[u][j] = src[j];
+                            for (int j = 0; j < 4; ++j)
+                                if (j < left) v[u][j] = src[j];
+                            valid[u] = left >= 4 ? 15u : ((1u << left) - 1);
+                        }
                     }
                 }
             }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int row = base + u * NW * RPW + lrow;
-                uint32_t x = 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) x |= (uint32_t) (j < nvalid[u] && inside(v[u][j], p.level)) << j;
-                x |= __shfl_down(x, 1, 16) << 4;
-                x |= __shfl_down(x, 2, 16) << 8;
-                x |= __shfl_down(x, 4, 16) << 16;
-                const uint32_t hi = __shfl_down(x, 8, 16);
-                if (row < nrow && (lq & 15) == 0) {
-                    const int py = row % (t.ny + 1), px = row / (t.ny + 1);
-                    s.w[(px * G::PY + py) * G::PC + (lq >> 4)] = (uint64_t) x | ((uint64_t) hi << 32);
-                }
-            }
         }
-        // halo point z = 64*NZC (present when the tile has a full z extent)
-        if (t.nz + 1 > G::TZ) {
-            for (int row = tid; row < nrow; row += NT) {
-                const int py = row % (t.ny + 1), px = row / (t.ny + 1);
-                const float hv = p.values[((t.x0 + px) * p.Y + (t.y0 + py)) * p.Z + t.z0 + G::TZ];
-                s.w[(px * G::PY + py) * G::PC + NZC] = inside(hv, p.level) ? 1ull : 0ull;
-            }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x |= (uint32_t) (((valid[u] >> j) & 1) && inside(v[u][j], level)) << j;
+            x |= __shfl_down(x, 1, 16) << 4;
+            x |= __shfl_down(x, 2, 16) << 8;
+            x |= __shfl_down(x, 4, 16) << 16;
+            const uint32_t hi = __shfl_down(x, 8, 16);
+            const int64_t w = base + 4 * u + grp;
+            if (q == 0 && w < nwords) bits[w] = (uint64_t) x | ((uint64_t) hi << 32);
         }
     }
-    __syncthreads();
+}
 
-    // ---- per-unit face / vertex counts (one lane per unit) ------------------
-    const int ncz = (t.nzo + 63) >> 6;
-    for (int u = tid; u < G::NUNITS; u += NT) {
-        int c = u % G::PC, r = u / G::PC, py = r % G::PY, px = r / G::PY;
-        uint32_t fc = 0, vc = 0, ac = 0;
-        if (px < t.nxo && py < t.nyo && c < ncz) {
-            uint64_t xc, yc, zc;
-            unit_masks(s, p, t, px, py, c, xc, yc, zc);
-            vc = __popcll(xc) + __popcll(yc) + __popcll(zc);
-            if (px < t.nx && py < t.ny) {
-                int ncell = t.nz - 64 * c;
-                uint64_t cm = ncell >= 64 ? ~0ull : (ncell <= 0 ? 0ull : ((1ull << ncell) - 1));
-                uint64_t a = W(s, px, py, c), b = W(s, px, py + 1, c);
-                uint64_t d0 = W(s, px + 1, py, c), d1 = W(s, px + 1, py + 1, c);
-                bool hn = c + 1 < t.nchunk;
-                uint64_t as = (a >> 1) | (hn ? W(s, px, py, c + 1) << 63 : 0ull);
-                uint64_t bs = (b >> 1) | (hn ? W(s, px, py + 1, c + 1) << 63 : 0ull);
-                uint64_t d0s = (d0 >> 1) | (hn ? W(s, px + 1, py, c + 1) << 63 : 0ull);
-                uint64_t d1s = (d1 >> 1) | (hn ? W(s, px + 1, py + 1, c + 1) << 63 : 0ull);
-                uint64_t any = a | as | b | bs | d0 | d0s | d1 | d1s;
-                uint64_t all = a & as & b & bs & d0 & d0s & d1 & d1s;
-                uint64_t act = any & ~all & cm;
-                ac = (uint32_t) __popcll(act);
-                while (act) {
-                    int k = __ffsll((unsigned long long) act) - 1;
-                    act &= act - 1;
-                    int cs = (int) ((a >> k) & 1) | (int) ((as >> k) & 1) << 1 |
-                             (int) ((b >> k) & 1) << 2 | (int) ((bs >> k) & 1) << 3 |
-                             (int) ((d0 >> k) & 1) << 4 | (int) ((d0s >> k) & 1) << 5 |
-                             (int) ((d1 >> k) & 1) << 6 | (int) ((d1s >> k) & 1) << 7;
-                    fc += tri_count(p.method, cs);
-                }
-            }
+// ---- count + block scan ----------------------------------------------------------
+// Entry i = (point row r, z-segment s), r = x * Y + y.  Vertex count: edges
+// owned by the row's points in the segment; face count: triangles of the
+// cell row (x, y) in the segment (0 when x = X-1 or y = Y-1).
+template <int NT>
+__device__ __forceinline__ void block_exscan2(uint32_t &a, uint32_t &b, uint32_t &tot_a, uint32_t &tot_b,
+                                              uint32_t *sa, uint32_t *sb) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t ia = a, ib = b;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t ya = __shfl_up(ia, o), yb = __shfl_up(ib, o);
+        if (lane >= o) {
+            ia += ya;
+            ib += yb;
         }
-        s.fo[u] = fc;
-        s.vo[u] = vc;
-        s.na[u] = ac;
+    }
+    if (lane == 63) {
+        sa[wave] = ia;
+        sb[wave] = ib;
     }
     __syncthreads();
-
-    // ---- block exclusive scan of unit counts ---------------------------------
-    {
-        constexpr int PER = (G::NUNITS + NT - 1) / NT;
-        const int lo = tid * PER, hi = min(lo + PER, G::NUNITS);
-        uint32_t sf = 0, sv = 0, sa = 0;
-        for (int u = lo; u < hi; ++u) {
-            sf += s.fo[u];
-            sv += s.vo[u];
-            sa += s.na[u];
+    uint32_t pa = 0, pb = 0;
+    tot_a = tot_b = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if (w < wave) {
+            pa += sa[w];
+            pb += sb[w];
         }
-        // inclusive scan of thread sums across the block
-        uint32_t incf = sf, incv = sv, inca = sa;
+        tot_a += sa[w];
+        tot_b += sb[w];
+    }
+    a = pa + ia - a;
+    b = pb + ib - b;
+}
+
+// First chunk (point row r, chunk c) of entry i, as the global chunk index r * W + c.
+__device__ __forceinline__ int64_t entry_chunk(const McParams &p, int64_t i) {
+    const int64_t r = i / p.S, s = i - r * p.S;
+    return r * p.W + s * p.nzc;
+}
+
+// One workgroup per kMcScanBlock entries.  Lanes walk the block's chunks in
+// memory order (coalesced mask loads), add their counts into the chunk's
+// entry in LDS, then the entries are scanned.
+__global__ void __launch_bounds__(256) k_count(McParams p) {
+    constexpr int NT = 256, PER = kMcScanBlock / NT;
+    __shared__ uint32_t ent_v[kMcScanBlock], ent_f[kMcScanBlock];
+    __shared__ uint32_t sa[NT / 64], sb[NT / 64];
+    __shared__ uint8_t ntri[256];
+    const int tid = threadIdx.x;
+    for (int c = tid; c < 256; c += NT) ntri[c] = (uint8_t) (cTris[p.method][c] & 15);
+    const int64_t n = p.X * p.Y * p.S;
+    const int64_t i0 = (int64_t) blockIdx.x * kMcScanBlock;
+    const int64_t i1 = min(i0 + (int64_t) kMcScanBlock, n);
+    for (int k = tid; k < kMcScanBlock; k += NT) ent_v[k] = ent_f[k] = 0;
+    __syncthreads();
+    const int64_t g0 = entry_chunk(p, i0);
+    const int64_t g1 = i1 < n ? entry_chunk(p, i1) : p.X * p.Y * p.W;
+    const int64_t YW = p.Y * p.W;
+    for (int64_t g = g0 + tid; g < g1; g += NT) {
+        const int64_t r = g / p.W, c = g - r * p.W;
+        const int64_t x = r / p.Y, y = r - x * p.Y;
+        const bool hasx = x + 1 < p.X, hasy = y + 1 < p.Y, hn = c + 1 < p.W;
+        const int64_t zlo = 64 * c;
+        const uint64_t *q = p.bits + g;
+        const uint64_t b = q[0], bn = hn ? q[1] : 0ull;
+        const uint64_t bx = hasx ? q[YW] : 0ull, by = hasy ? q[p.W] : 0ull;
+        const uint32_t cv = (uint32_t) chunk_vertices(b, bn & 1, bx, by, hasx, hasy, zlo, p.Z);
+        uint32_t cf = 0;
+        if (hasx && hasy) {
+            const uint64_t bxy = q[YW + p.W];
+            const uint64_t byn = hn ? q[p.W + 1] : 0ull, bxn = hn ? q[YW + 1] : 0ull,
+                           bxyn = hn ? q[YW + p.W + 1] : 0ull;
+            const CellMasks m = cell_masks(b, bn, by, byn, bx, bxn, bxy, bxyn);
+            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - zlo);
+            while (act) {
+                const int j = __ffsll((unsigned long long) act) - 1;
+                act &= act - 1;
+                cf += ntri[cell_case(m, j)];
+            }
+        }
+        const int e = (int) (r * p.S + c / p.nzc - i0);
+        if (cv) atomicAdd(&ent_v[e], cv);
+        if (cf) atomicAdd(&ent_f[e], cf);
+    }
+    __syncthreads();
+    uint32_t cv[PER], cf[PER], tv = 0, tf = 0;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        cv[e] = ent_v[tid * PER + e];
+        cf[e] = ent_f[tid * PER + e];
+        tv += cv[e];
+        tf += cf[e];
+    }
+    uint32_t bv, bf;
+    block_exscan2<NT>(tv, tf, bv, bf, sa, sb);
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int64_t i = i0 + tid * PER + e;
+        if (i < n) {
+            p.cnt_v[i] = tv;
+            p.cnt_f[i] = tf;
+        }
+        tv += cv[e];
+        tf += cf[e];
+    }
+    if (tid == 0) {
+        p.bsum_v[blockIdx.x] = bv;
+        p.bsum_f[blockIdx.x] = bf;
+    }
+}
+
+// Exclusive scan of the block totals (one workgroup, carried across rounds);
+// writes the mesh size and flags meshes beyond int32 indices.
+__global__ void __launch_bounds__(1024) k_scan_top(McParams p) {
+    constexpr int NT = 1024, NW = NT / 64;
+    __shared__ int64_t sv[NW], sf[NW];
+    __shared__ int64_t carry_v, carry_f;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry_v = carry_f = 0;
+    __syncthreads();
+    for (int64_t b0 = 0; b0 < p.nblk; b0 += NT) {
+        const int64_t i = b0 + tid;
+        const int64_t v = i < p.nblk ? (int64_t) p.bsum_v[i] : 0, f = i < p.nblk ? (int64_t) p.bsum_f[i] : 0;
+        int64_t iv = v, iff = f;
+#pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            uint32_t yf = __shfl_up(incf, o), yv = __shfl_up(incv, o), ya = __shfl_up(inca, o);
+            int64_t yv = __shfl_up(iv, o), yf = __shfl_up(iff, o);
             if (lane >= o) {
-                incf += yf;
-                incv += yv;
-                inca += ya;
+                iv += yv;
+                iff += yf;
             }
         }
         if (lane == 63) {
-            s.tsum_f[wave] = incf;
-            s.tsum_v[wave] = incv;
-            s.tsum_a[wave] = inca;
+            sv[wave] = iv;
+            sf[wave] = iff;
         }
         __syncthreads();
-        uint32_t wf = 0, wv = 0, wa = 0, tot_f = 0, tot_v = 0, tot_a = 0;
+        int64_t pv = carry_v, pf = carry_f, tv = 0, tf = 0;
         for (int w = 0; w < NW; ++w) {
             if (w < wave) {
-                wf += s.tsum_f[w];
-                wv += s.tsum_v[w];
-                wa += s.tsum_a[w];
+                pv += sv[w];
+                pf += sf[w];
             }
-            tot_f += s.tsum_f[w];
-            tot_v += s.tsum_v[w];
-            tot_a += s.tsum_a[w];
+            tv += sv[w];
+            tf += sf[w];
         }
-        uint32_t ef = wf + incf - sf, ev = wv + incv - sv, ea = wa + inca - sa;
-        for (int u = lo; u < hi; ++u) {
-            uint32_t f = s.fo[u], v = s.vo[u], a = s.na[u];
-            s.fo[u] = ef;
-            s.vo[u] = ev;
-            s.na[u] = ea;
-            ef += f;
-            ev += v;
-            ea += a;
+        if (i < p.nblk) {
+            p.blk_v[i] = pv + iv - v;
+            p.blk_f[i] = pf + iff - f;
         }
         __syncthreads();
         if (tid == 0) {
-            s.fo[G::NUNITS] = tot_f;
-            s.vo[G::NUNITS] = tot_v;
-            s.na[G::NUNITS] = tot_a;
+            carry_v += tv;
+            carry_f += tf;
         }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        p.totals[0] = carry_v;
+        p.totals[1] = carry_f;
+        if (carry_v > 0x7fffffffll || carry_f > 0x7fffffffll) set_err(p, ISO_ERR_TOO_LARGE);
+    }
+}
+
+// ---- emit ------------------------------------------------------------------------
+template <int TX, int TY, int NZC, int NT>
+struct EmitLds {
+    static constexpr int PX = TX + 2, PY = TY + 2, PC = NZC + 1;   // mask patch
+    static constexpr int QY = TY + 1, QC = NZC + 1;                // vertex bases
+    static constexpr int NUNITS = TX * TY * NZC;                   // (cell row, chunk)
+    uint64_t w[PX * PY * PC];
+    int32_t vb[(TX + 1) * QY * QC];   // first vertex id per (point row, chunk)
+    int32_t fb[NUNITS];               // first face id per unit
+    uint32_t na[NUNITS + 1];          // active cells per unit -> exclusive offsets
+    uint32_t list[NT];                // batch: unit << 15 | z-bit << 9 | face offset in unit
+    uint32_t tsum[NT / 64];
+    uint64_t tris[256];               // the method's packed triangle table
+};
+
+template <int TX, int TY, int NZC, int NT>
+__device__ __forceinline__ uint64_t PW(const EmitLds<TX, TY, NZC, NT> &s, int px, int py, int j) {
+    using L = EmitLds<TX, TY, NZC, NT>;
+    return s.w[(px * L::PY + py) * L::PC + j];
+}
+
+template <int TX, int TY, int NZC, int NT>
+__device__ __forceinline__ CellMasks unit_cells(const EmitLds<TX, TY, NZC, NT> &s, int cx, int cy, int j) {
+    return cell_masks(PW(s, cx, cy, j), PW(s, cx, cy, j + 1), PW(s, cx, cy + 1, j), PW(s, cx, cy + 1, j + 1),
+                      PW(s, cx + 1, cy, j), PW(s, cx + 1, cy, j + 1), PW(s, cx + 1, cy + 1, j),
+                      PW(s, cx + 1, cy + 1, j + 1));
+}
+
+// Global id of the vertex on edge (owner point (opx, opy, oz) tile-local,
+// axis): the (point row, chunk) base plus the rank among the row's crossing
+// edges in (z, axis) order.
+template <int TX, int TY, int NZC, int NT>
+__device__ __forceinline__ int64_t vertex_id(const EmitLds<TX, TY, NZC, NT> &s, const McParams &p, int64_t x0,
+                                             int64_t y0, int opx, int opy, int oz, int axis) {
+    using L = EmitLds<TX, TY, NZC, NT>;
+    const int j = oz >> 6, k = oz & 63;
+    const uint64_t b = PW(s, opx, opy, j);
+    const uint64_t xm = (x0 + opx + 1 < p.X) ? (b ^ PW(s, opx + 1, opy, j)) : 0ull;
+    const uint64_t ym = (y0 + opy + 1 < p.Y) ? (b ^ PW(s, opx, opy + 1, j)) : 0ull;
+    const uint64_t zm = b ^ (b >> 1);   // only bits below k are used
+    const uint64_t low = k ? (~0ull >> (64 - k)) : 0ull;
+    int r = __popcll(xm & low) + __popcll(ym & low) + __popcll(zm & low);
+    if (axis >= 1) r += (int) ((xm >> k) & 1);
+    if (axis == 2) r += (int) ((ym >> k) & 1);
+    return (int64_t) s.vb[(opx * L::QY + opy) * L::QC + j] + r;
+}
+
+template <int N, int NT>
+__device__ __forceinline__ void block_exscan_lds(uint32_t *a, uint32_t *tsum) {
+    constexpr int NW = NT / 64, PER = (N + NT - 1) / NT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lo = tid * PER, hi = min(lo + PER, N);
+    uint32_t sum = 0;
+    for (int i = lo; i < hi; ++i) sum += a[i];
+    uint32_t inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) tsum[wave] = inc;
+    __syncthreads();
+    uint32_t run = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if (w < wave) run += tsum[w];
+        tot += tsum[w];
+    }
+    run += inc - sum;
+    for (int i = lo; i < hi; ++i) {
+        uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    if (tid == 0) a[N] = tot;
+    __syncthreads();
+}
+
+template <int TX, int TY, int NZC, int NT>
+__global__ void __launch_bounds__(NT) k_emit(McParams p) {
+    using L = EmitLds<TX, TY, NZC, NT>;
+    __shared__ L s;
+    const int tid = threadIdx.x;
+    if (*(volatile int *) p.err & ISO_ERR_TOO_LARGE) return;
+
+    const int64_t tile = blockIdx.x;
+    const int64_t tk = tile % p.NK, tj = (tile / p.NK) % p.NJ, ti = tile / (p.NK * p.NJ);
+    const int64_t x0 = ti * TX, y0 = tj * TY, c0 = tk * NZC, z0 = 64 * c0;
+    const int nx = (int) min<int64_t>(TX, p.X - 1 - x0);
+    const int ny = (int) min<int64_t>(TY, p.Y - 1 - y0);
+
+    // ---- mask patch: point rows [x0, x0+TX+2) x [y0, y0+TY+2), chunks [c0, c0+NZC];
+    // all loads are issued before the first use, together with the global
+    // bases the tile needs if it has active cells
+    constexpr int NP = L::PX * L::PY * L::PC, IT = (NP + NT - 1) / NT;
+    constexpr int CR = (TX * TY + NT - 1) / NT, PR = ((TX + 1) * (TY + 1) + NT - 1) / NT;
+    uint64_t wv[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = tid + k * NT;
+        const int j = i % L::PC, r = i / L::PC, py = r % L::PY, px = r / L::PY;
+        const int64_t gx = x0 + px, gy = y0 + py, c = c0 + j;
+        wv[k] = (i < NP && gx < p.X && gy < p.Y && c < p.W) ? p.bits[(gx * p.Y + gy) * p.W + c] : 0ull;
+    }
+    int64_t fbase_pf[CR], vbase_pf[PR], vnext_pf[PR];
+#pragma unroll
+    for (int k = 0; k < CR; ++k) {
+        const int i = tid + k * NT, cx = i / TY, cy = i % TY;
+        fbase_pf[k] = 0;
+        if (i < TX * TY && cx < nx && cy < ny) {
+            const int64_t e = ((x0 + cx) * p.Y + (y0 + cy)) * p.S + tk;
+            fbase_pf[k] = p.blk_f[e / kMcScanBlock] + (int64_t) p.cnt_f[e];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PR; ++k) {
+        const int i = tid + k * NT, px = i / (ny + 1), py = i % (ny + 1);
+        vbase_pf[k] = vnext_pf[k] = 0;
+        if (i < (nx + 1) * (ny + 1)) {
+            const int64_t e = ((x0 + px) * p.Y + (y0 + py)) * p.S + tk;
+            vbase_pf[k] = p.blk_v[e / kMcScanBlock] + (int64_t) p.cnt_v[e];
+            if (tk + 1 < p.S) vnext_pf[k] = p.blk_v[(e + 1) / kMcScanBlock] + (int64_t) p.cnt_v[e + 1];
+        }
+    }
+    for (int c = tid; c < 256; c += NT) s.tris[c] = cTris[p.method][c];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = tid + k * NT;
+        if (i < NP) s.w[i] = wv[k];
     }
     __syncthreads();
 
-    // ---- decoupled look-back over tickets for the tile's global bases --------
-    if (wave == 0) {
-        const uint64_t tf = s.fo[G::NUNITS], tv = s.vo[G::NUNITS];
-        int64_t exf = 0, exv = 0;
-        if (ticket == 0) {
-            if (lane == 0) st_agent(&p.status[0], pack_status(2, tf, tv));
-        } else {
-            if (lane == 0) st_agent(&p.status[ticket], pack_status(1, tf, tv));
-            int64_t look = ticket - 1;
-            bool done = false;
-            while (!done) {
-                int64_t idx = look - lane;
-                uint64_t st = 0;
-                if (idx >= 0) {
-                    int spins = 0;
-                    st = ld_agent(&p.status[idx]);
-                    while ((st >> 62) == 0) {
-                        if (++spins > kSpinLimit ||
-                            ((spins & 1023) == 0 && (ld_agent32((const uint32_t *) p.err) & ISO_ERR_SPIN))) {
-                            set_err(p, ISO_ERR_SPIN);
-                            st = pack_status(2, 0, 0);
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(2);
-                        st = ld_agent(&p.status[idx]);
-                    }
-                }
-                uint64_t incl = __ballot(idx >= 0 && (st >> 62) == 2);
-                int kstop = incl ? (__ffsll((unsigned long long) incl) - 1) : 63;
-                bool take = idx >= 0 && lane <= kstop;
-                int64_t f = take ? (int64_t) ((st >> 31) & 0x7fffffffull) : 0;
-                int64_t v = take ? (int64_t) (st & 0x7fffffffull) : 0;
-                exf += wave_sum64(f);
-                exv += wave_sum64(v);
-                if (incl || look - 63 <= 0) done = true;
-                look -= 64;
+    // ---- active cells and faces per unit (cell row, chunk) ----------------------
+    for (int i = tid; i < TX * TY; i += NT) {
+        const int cx = i / TY, cy = i % TY;
+        uint32_t *na = &s.na[i * NZC];
+        int32_t *fc = &s.fb[i * NZC];
+        for (int j = 0; j < NZC; ++j) {
+            na[j] = 0;
+            fc[j] = 0;
+            const int64_t zlo = z0 + 64 * j;
+            if (cx >= nx || cy >= ny || zlo >= p.Z - 1) continue;
+            const CellMasks m = unit_cells(s, cx, cy, j);
+            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - zlo);
+            na[j] = (uint32_t) __popcll(act);
+            int f = 0;
+            while (act) {
+                const int k = __ffsll((unsigned long long) act) - 1;
+                act &= act - 1;
+                f += (int) (s.tris[cell_case(m, k)] & 15);
             }
-            if (exf + (int64_t) tf > 0x7fffffffll || exv + (int64_t) tv > 0x7fffffffll) {
-                if (lane == 0) set_err(p, ISO_ERR_TOO_LARGE);
-            }
-            if (lane == 0) st_agent(&p.status[ticket], pack_status(2, exf + tf, exv + tv));
-        }
-        if (lane == 0) {
-            s.fbase = exf;
-            s.vbase = exv;
-            if (ticket == p.ntiles - 1) {
-                p.totals[0] = exv + (int64_t) tv;
-                p.totals[1] = exf + (int64_t) tf;
-            }
+            fc[j] = f;
         }
     }
     __syncthreads();
-    const int64_t fbase = s.fbase, vbase = s.vbase;
-    const uint64_t epoch = (uint64_t) (*p.epoch_ptr) << 32;
-    uint64_t *pub = p.pub + t.tile * (int64_t) G::PUB;
+    block_exscan_lds<L::NUNITS, NT>(s.na, s.tsum);
+    const int nactive = (int) s.na[L::NUNITS];
+    if (nactive == 0) return;
 
-    // ---- publish ids of crossing in-plane edges on the lower boundary planes -
-    // plane px == 0 (read by tile ti-1): edges y, z of points (0, py, z)
-    if (t.ti > 0) {
-        for (int u = tid; u < t.nyo * ncz; u += NT) {
-            int py = u / ncz, c = u % ncz;
-            uint64_t xc, yc, zc;
-            unit_masks(s, p, t, 0, py, c, xc, yc, zc);
-            uint64_t m = yc | zc;
-            while (m) {
-                int b = __ffsll((unsigned long long) m) - 1;
-                m &= m - 1;
-                int z = 64 * c + b;
-                if ((yc >> b) & 1)
-                    st_agent(&pub[pub_slot<TX, TY, NZC>(0, 0, py, z, 1)],
-                             epoch | (uint64_t) (vbase + local_vid(s, p, t, 0, py, z, 1)));
-                if ((zc >> b) & 1)
-                    st_agent(&pub[pub_slot<TX, TY, NZC>(0, 0, py, z, 2)],
-                             epoch | (uint64_t) (vbase + local_vid(s, p, t, 0, py, z, 2)));
-            }
+    // ---- global bases: faces per unit, vertices per (point row, chunk) ---------
+#pragma unroll
+    for (int k = 0; k < CR; ++k) {
+        const int i = tid + k * NT, cx = i / TY, cy = i % TY;
+        if (i >= TX * TY || cx >= nx || cy >= ny) continue;
+        int64_t base = fbase_pf[k];
+        int32_t *fc = &s.fb[i * NZC];
+        for (int j = 0; j < NZC; ++j) {
+            const int32_t f = fc[j];
+            fc[j] = (int32_t) base;
+            base += f;
         }
     }
-    // plane py == 0 (read by tile tj-1): edges x, z of points (px, 0, z)
-    if (t.tj > 0) {
-        for (int u = tid; u < t.nxo * ncz; u += NT) {
-            int px = u / ncz, c = u % ncz;
-            uint64_t xc, yc, zc;
-            unit_masks(s, p, t, px, 0, c, xc, yc, zc);
-            uint64_t m = xc | zc;
-            while (m) {
-                int b = __ffsll((unsigned long long) m) - 1;
-                m &= m - 1;
-                int z = 64 * c + b;
-                if ((xc >> b) & 1)
-                    st_agent(&pub[pub_slot<TX, TY, NZC>(1, px, 0, z, 0)],
-                             epoch | (uint64_t) (vbase + local_vid(s, p, t, px, 0, z, 0)));
-                if ((zc >> b) & 1)
-                    st_agent(&pub[pub_slot<TX, TY, NZC>(1, px, 0, z, 2)],
-                             epoch | (uint64_t) (vbase + local_vid(s, p, t, px, 0, z, 2)));
-            }
+#pragma unroll
+    for (int k = 0; k < PR; ++k) {
+        const int i = tid + k * NT;
+        if (i >= (nx + 1) * (ny + 1)) continue;
+        const int px = i / (ny + 1), py = i % (ny + 1);
+        const int64_t gx = x0 + px, gy = y0 + py;
+        const bool hasx = gx + 1 < p.X, hasy = gy + 1 < p.Y;
+        int64_t base = vbase_pf[k];
+        int32_t *vb = &s.vb[(px * L::QY + py) * L::QC];
+        for (int j = 0; j < NZC; ++j) {
+            vb[j] = (int32_t) base;
+            if (c0 + j < p.W)
+                base += chunk_vertices(PW(s, px, py, j), PW(s, px, py, j + 1) & 1, PW(s, px + 1, py, j),
+                                       PW(s, px, py + 1, j), hasx, hasy, 64 * (c0 + j), p.Z);
         }
+        vb[NZC] = (tk + 1 < p.S) ? (int32_t) vnext_pf[k] : (int32_t) base;
     }
-    // plane z == 0 (read by tile tk-1): edges x, y of points (px, py, 0)
-    if (t.tk > 0) {
-        for (int u = tid; u < t.nxo * t.nyo; u += NT) {
-            int px = u / t.nyo, py = u % t.nyo;
-            uint64_t xc, yc, zc;
-            unit_masks(s, p, t, px, py, 0, xc, yc, zc);
-            if (xc & 1)
-                st_agent(&pub[pub_slot<TX, TY, NZC>(2, px, py, 0, 0)],
-                         epoch | (uint64_t) (vbase + local_vid(s, p, t, px, py, 0, 0)));
-            if (yc & 1)
-                st_agent(&pub[pub_slot<TX, TY, NZC>(2, px, py, 0, 1)],
-                         epoch | (uint64_t) (vbase + local_vid(s, p, t, px, py, 0, 1)));
-        }
-    }
+    __syncthreads();
 
-    // ---- emit: one lane per active cell (cells compacted through the scan) ---
+    // ---- emit, NT active cells per batch ----------------------------------------
     const float lvl = p.level;
-    const int nactive = (int) s.na[G::NUNITS];
+    const int spseg = (int) (blockIdx.x % kSpSeg);
     for (int j0 = 0; j0 < nactive; j0 += NT) {
-        const int j = j0 + tid;
-        if (j >= nactive) break;
-        // unit holding the j-th active cell: last u with na[u] <= j
-        int lo = 0, hi = G::NUNITS;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (s.na[mid] <= (uint32_t) j) lo = mid;
-            else hi = mid;
+        // fill the batch list: unit, z-bit and face offset inside the unit
+        for (int u = tid; u < L::NUNITS; u += NT) {
+            const uint32_t a0 = s.na[u], a1 = s.na[u + 1];
+            // empty units include those outside the tile, whose patch cells
+            // may look active: they must not write the list
+            if (a1 == a0 || a1 <= (uint32_t) j0 || a0 >= (uint32_t) (j0 + NT)) continue;
+            const int j = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
+            const CellMasks m = unit_cells(s, cx, cy, j);
+            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - (z0 + 64 * j));
+            const uint32_t end = min(a1, (uint32_t) (j0 + NT));
+            uint32_t idx = a0, fo = 0;
+            while (act && idx < end) {
+                const int k = __ffsll((unsigned long long) act) - 1;
+                act &= act - 1;
+                if (idx >= (uint32_t) j0) s.list[idx - j0] = ((uint32_t) u << 15) | ((uint32_t) k << 9) | fo;
+                fo += (uint32_t) (s.tris[cell_case(m, k)] & 15);
+                ++idx;
+            }
         }
-        const int u = lo;
-        int rank = j - (int) s.na[u];
-        const int c = u % G::PC, r = u / G::PC, py = r % G::PY, px = r / G::PY;
-        const int ncell = t.nz - 64 * c;
-        const uint64_t cm = ncell >= 64 ? ~0ull : ((1ull << ncell) - 1);
-        const uint64_t a = W(s, px, py, c), b = W(s, px, py + 1, c);
-        const uint64_t d0 = W(s, px + 1, py, c), d1 = W(s, px + 1, py + 1, c);
-        const bool hn = c + 1 < t.nchunk;
-        const uint64_t as = (a >> 1) | (hn ? W(s, px, py, c + 1) << 63 : 0ull);
-        const uint64_t bs = (b >> 1) | (hn ? W(s, px, py + 1, c + 1) << 63 : 0ull);
-        const uint64_t d0s = (d0 >> 1) | (hn ? W(s, px + 1, py, c + 1) << 63 : 0ull);
-        const uint64_t d1s = (d1 >> 1) | (hn ? W(s, px + 1, py + 1, c + 1) << 63 : 0ull);
-        uint64_t act = (a | as | b | bs | d0 | d0s | d1 | d1s) & ~(a & as & b & bs & d0 & d0s & d1 & d1s) & cm;
-        // skip the active cells before this one, adding up their faces
-        int64_t fslot = fbase + s.fo[u];
-        for (; rank > 0; --rank) {
-            int k0 = __ffsll((unsigned long long) act) - 1;
-            act &= act - 1;
-            int cs0 = (int) ((a >> k0) & 1) | (int) ((as >> k0) & 1) << 1 | (int) ((b >> k0) & 1) << 2 |
-                      (int) ((bs >> k0) & 1) << 3 | (int) ((d0 >> k0) & 1) << 4 |
-                      (int) ((d0s >> k0) & 1) << 5 | (int) ((d1 >> k0) & 1) << 6 |
-                      (int) ((d1s >> k0) & 1) << 7;
-            fslot += tri_count(p.method, cs0);
-        }
-        const int64_t gx = t.x0 + px, gy = t.y0 + py;
-        const float cx0 = grid_coord(gx, p.X, p.amin[0], p.amax[0]);
-        const float cx1 = grid_coord(gx + 1, p.X, p.amin[0], p.amax[0]);
-        const float cy0 = grid_coord(gy, p.Y, p.amin[1], p.amax[1]);
-        const float cy1 = grid_coord(gy + 1, p.Y, p.amin[1], p.amax[1]);
-        {
-            const int k = __ffsll((unsigned long long) act) - 1;
-            const int zl = 64 * c + k;
-            const int64_t gz = t.z0 + zl;
-            int cs = (int) ((a >> k) & 1) | (int) ((as >> k) & 1) << 1 | (int) ((b >> k) & 1) << 2 |
-                     (int) ((bs >> k) & 1) << 3 | (int) ((d0 >> k) & 1) << 4 |
-                     (int) ((d0s >> k) & 1) << 5 | (int) ((d1 >> k) & 1) << 6 |
-                     (int) ((d1s >> k) & 1) << 7;
-            // corner values (gathered; the tile was just streamed)
+        __syncthreads();
+        if (j0 + tid < nactive) {
+            const uint32_t ent = s.list[tid];
+            const int u = (int) (ent >> 15), k = (int) ((ent >> 9) & 63), fo = (int) (ent & 511);
+            const int jc = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
+            const int zl = 64 * jc + k;
+            const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + zl;
+            const int cs = cell_case(unit_cells(s, cx, cy, jc), k);
             CellCtx cc;
             {
                 const float *r00 = p.values + (gx * p.Y + gy) * p.Z + gz;
                 const float *r01 = r00 + p.Z;
                 const float *r10 = r00 + p.Y * p.Z;
                 const float *r11 = r10 + p.Z;
-                cc.cv[0] = r00[0];
-                cc.cv[1] = r00[1];
-                cc.cv[2] = r01[0];
-                cc.cv[3] = r01[1];
-                cc.cv[4] = r10[0];
-                cc.cv[5] = r10[1];
-                cc.cv[6] = r11[0];
-                cc.cv[7] = r11[1];
+                cc.v[0] = r00[0];
+                cc.v[1] = r00[1];
+                cc.v[2] = r01[0];
+                cc.v[3] = r01[1];
+                cc.v[4] = r10[0];
+                cc.v[5] = r10[1];
+                cc.v[6] = r11[0];
+                cc.v[7] = r11[1];
             }
-            cc.c[0][0] = cx0;
-            cc.c[0][1] = cx1;
-            cc.c[1][0] = cy0;
-            cc.c[1][1] = cy1;
+            cc.c[0][0] = grid_coord(gx, p.X, p.amin[0], p.amax[0]);
+            cc.c[0][1] = grid_coord(gx + 1, p.X, p.amin[0], p.amax[0]);
+            cc.c[1][0] = grid_coord(gy, p.Y, p.amin[1], p.amax[1]);
+            cc.c[1][1] = grid_coord(gy + 1, p.Y, p.amin[1], p.amax[1]);
             cc.c[2][0] = grid_coord(gz, p.Z, p.amin[2], p.amax[2]);
             cc.c[2][1] = grid_coord(gz + 1, p.Z, p.amin[2], p.amax[2]);
             cc.level = lvl;
-            const int status = crossing_status(cs);
+            const uint32_t status = (uint32_t) crossing_status(cs);
             // weld candidates: a vertex within a few ulps of a grid corner
             uint32_t candm = 0;
-            {
-                uint32_t m = (uint32_t) status;
-                while (m) {
-                    int e = __ffs(m) - 1;
-                    m &= m - 1;
-                    if (p.all_special || edge_is_candidate(cc, e)) candm |= 1u << e;
-                }
+            if (p.all_special) {
+                candm = status;
+            } else {
+                for_edges([&](auto E) {
+                    if (((status >> E) & 1) && edge_is_candidate<E>(cc)) candm |= 1u << E;
+                });
             }
             const bool special = candm != 0;
-            // vertices emitted by this cell: edges of owned corners that have
-            // no lower-indexed cell in the tile to emit them
+            // vertices owned by this cell: the edges of its lowest corner, plus
+            // those of upper corners on the grid's upper faces (no cell above)
             {
-                bool lastx = (px == t.nx - 1) && (t.nxo == t.nx + 1);
-                bool lasty = (py == t.ny - 1) && (t.nyo == t.ny + 1);
-                bool lastz = (zl == t.nz - 1) && (t.nzo == t.nz + 1);
-                uint32_t emit = (1u << 0) | (1u << 3) | (1u << 8);          // v0: z, y, x
-                if (lastx) emit |= (1u << 7) | (1u << 4);                    // v4: y, z
-                if (lasty) emit |= (1u << 11) | (1u << 2);                   // v2: x, z
-                if (lastz) emit |= (1u << 9) | (1u << 1);                    // v1: x, y
-                if (lastx && lasty) emit |= 1u << 6;                         // v6: z
-                if (lastx && lastz) emit |= 1u << 5;                         // v5: y
-                if (lasty && lastz) emit |= 1u << 10;                        // v3: x
-                uint32_t m = emit & (uint32_t) status;
-                while (m) {
-                    int e = __ffs(m) - 1;
-                    m &= m - 1;
-                    // emitted edges are owned by this tile
-                    int ca = kEdgeA[e];
-                    int64_t v = vbase + local_vid(s, p, t, px + (ca >> 2), py + ((ca >> 1) & 1),
-                                                  zl + (ca & 1), kEdgeAxis[e]);
+                const bool lastx = gx == p.X - 2, lasty = gy == p.Y - 2, lastz = gz == p.Z - 2;
+                uint32_t emit = (1u << 0) | (1u << 3) | (1u << 8);   // corner 0: z, y, x
+                if (lastx) emit |= (1u << 7) | (1u << 4);             // corner 4: y, z
+                if (lasty) emit |= (1u << 11) | (1u << 2);            // corner 2: x, z
+                if (lastz) emit |= (1u << 9) | (1u << 1);             // corner 1: x, y
+                if (lastx && lasty) emit |= 1u << 6;                  // corner 6: z
+                if (lastx && lastz) emit |= 1u << 5;                  // corner 5: y
+                if (lasty && lastz) emit |= 1u << 10;                 // corner 3: x
+                const uint32_t m = emit & status;
+                for_edges([&](auto E) {
+                    if (!((m >> E) & 1)) return;
+                    constexpr int ca = edge_a(E);
+                    const int64_t v = vertex_id(s, p, x0, y0, cx + (ca >> 2), cy + ((ca >> 1) & 1), zl + (ca & 1),
+                                                edge_axis(E));
                     float x, y, z;
-                    edge_pos(cc, e, x, y, z);
+                    edge_pos_c<E>(cc, x, y, z);
                     if (v >= 0 && v < p.cap_v) {
-                        p.out_v[3 * v + 0] = x;
-                        p.out_v[3 * v + 1] = y;
-                        p.out_v[3 * v + 2] = z;
-                        if (p.out_key) p.out_key[v] = edge_key(p, gx, gy, gz, e);
+                        *reinterpret_cast<float3 *>(p.out_v + 3 * v) = make_float3(x, y, z);
+                        if (p.out_key) p.out_key[v] = edge_key(p, gx, gy, gz, E);
                     } else {
                         set_err(p, ISO_ERR_CAPACITY);
                     }
-                    if ((candm >> e) & 1) {
-                        int64_t i = list_push(&p.sp->n_cand, p.sp->cap_cand);
+                    if ((candm >> E) & 1) {
+                        const int64_t i = seg_push(p, 0, spseg);
                         if (i >= 0) {
                             SpecialVertex sv;
                             sv.vid = v;
-                            sv.edge = edge_key(p, gx, gy, gz, e);
+                            sv.edge = edge_key(p, gx, gy, gz, E);
                             sv.x = x;
                             sv.y = y;
                             sv.z = z;
                             sv.pad = 0.0f;
-                            p.sp->cand[i] = sv;
+                            p.seg.cand[i] = sv;
                         }
                     }
-                }
+                });
             }
             // faces
-            const uint64_t w = cTris[p.method][cs];
-            const int ntri = (int) (w & 15);
-            for (int q = 0; q < ntri; ++q) {
-                int e0 = (int) ((w >> (4 + 12 * q)) & 15);
-                int e1 = (int) ((w >> (8 + 12 * q)) & 15);
-                int e2 = (int) ((w >> (12 + 12 * q)) & 15);
-                int64_t slot = fslot++;
+            const uint64_t wl = s.tris[cs];
+            const int ntri = (int) (wl & 15);
+            int64_t slot = (int64_t) s.fb[u] + fo;
+            for (int q = 0; q < ntri; ++q, ++slot) {
+                const int e0 = (int) ((wl >> (4 + 12 * q)) & 15);
+                const int e1 = (int) ((wl >> (8 + 12 * q)) & 15);
+                const int e2 = (int) ((wl >> (12 + 12 * q)) & 15);
                 bool keep = true;
                 if (special) {
-                    float x0, y0, z0, x1, y1, z1, x2, y2, z2;
-                    edge_pos(cc, e0, x0, y0, z0);
-                    edge_pos(cc, e1, x1, y1, z1);
-                    edge_pos(cc, e2, x2, y2, z2);
-                    bool d01 = x0 == x1 && y0 == y1 && z0 == z1;
-                    bool d02 = x0 == x2 && y0 == y2 && z0 == z2;
-                    bool d12 = x1 == x2 && y1 == y2 && z1 == z2;
+                    float xa, ya, za, xb, yb, zb, xc, yc, zc;
+                    edge_pos_dyn(cc, e0, xa, ya, za);
+                    edge_pos_dyn(cc, e1, xb, yb, zb);
+                    edge_pos_dyn(cc, e2, xc, yc, zc);
+                    const bool d01 = xa == xb && ya == yb && za == zb;
+                    const bool d02 = xa == xc && ya == yc && za == zc;
+                    const bool d12 = xb == xc && yb == yc && zb == zc;
                     keep = !(d01 || d02 || d12);
-                    int64_t i = list_push(keep ? &p.sp->n_kept : &p.sp->n_hole,
-                                          keep ? p.sp->cap_kept : p.sp->cap_hole);
-                    if (i >= 0) (keep ? p.sp->kept : p.sp->hole)[i] = slot;
+                    const int64_t i = seg_push(p, keep ? 2 : 3, spseg);
+                    if (i >= 0) (keep ? p.seg.kept : p.seg.hole)[i] = slot;
                 }
-                if (slot < p.cap_f) {
+                if (slot >= 0 && slot < p.cap_f) {
                     int32_t *dst = p.out_f + 3 * slot;
                     if (keep) {
-                        dst[0] = (int32_t) edge_vid<TX, TY, NZC>(s, p, t, px, py, zl, e0, vbase, epoch);
-                        dst[1] = (int32_t) edge_vid<TX, TY, NZC>(s, p, t, px, py, zl, e1, vbase, epoch);
-                        dst[2] = (int32_t) edge_vid<TX, TY, NZC>(s, p, t, px, py, zl, e2, vbase, epoch);
+                        const int a0 = edge_a(e0), a1 = edge_a(e1), a2 = edge_a(e2);
+                        const int i0 = (int) vertex_id(s, p, x0, y0, cx + (a0 >> 2), cy + ((a0 >> 1) & 1),
+                                                       zl + (a0 & 1), edge_axis(e0));
+                        const int i1 = (int) vertex_id(s, p, x0, y0, cx + (a1 >> 2), cy + ((a1 >> 1) & 1),
+                                                       zl + (a1 & 1), edge_axis(e1));
+                        const int i2 = (int) vertex_id(s, p, x0, y0, cx + (a2 >> 2), cy + ((a2 >> 1) & 1),
+                                                       zl + (a2 & 1), edge_axis(e2));
+                        *reinterpret_cast<int3 *>(dst) = make_int3(i0, i1, i2);
                     } else {
-                        dst[0] = dst[1] = dst[2] = -1;
+                        *reinterpret_cast<int3 *>(dst) = make_int3(-1, -1, -1);
                     }
                 } else {
                     set_err(p, ISO_ERR_CAPACITY);
@@ -705,60 +753,94 @@ __global__ void __launch_bounds__(NT) mc_dense_kernel(McParams p) {
             if (special) {
                 uint32_t m = (uint32_t) status;
                 while (m) {
-                    int e = __ffs(m) - 1;
+                    const int e = __ffs(m) - 1;
                     m &= m - 1;
-                    int64_t i = list_push(&p.sp->n_scell, p.sp->cap_scell);
+                    const int64_t i = seg_push(p, 1, spseg);
                     if (i >= 0) {
+                        const int ca = edge_a(e);
                         SpecialEdge se;
-                        se.vid = edge_vid<TX, TY, NZC>(s, p, t, px, py, zl, e, vbase, epoch);
+                        se.vid = vertex_id(s, p, x0, y0, cx + (ca >> 2), cy + ((ca >> 1) & 1), zl + (ca & 1),
+                                           edge_axis(e));
                         se.edge = edge_key(p, gx, gy, gz, e);
-                        p.sp->scell[i] = se;
+                        p.seg.scell[i] = se;
                     }
                 }
             }
         }
+        __syncthreads();
     }
 }
 
-__global__ void mc_dense_reset(uint64_t *status, int64_t n, uint32_t *ticket, uint32_t *epoch,
-                               int *err, int64_t *totals, SpecialLists *sp, SpecialLists init) {
-    int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < n; i += (int64_t) gridDim.x * blockDim.x) status[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *ticket = 0;
-        *epoch = *epoch + 1;
-        *err = 0;
-        totals[0] = totals[1] = 0;
+__global__ void mc_dense_reset(McParams p, SpecialLists init) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 4 * kSpSeg) p.seg.ctr[t * kSpStride] = 0;
+    if (t == 0) {
+        *p.err = 0;
+        p.totals[0] = p.totals[1] = 0;
         init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
-        *sp = init;
+        init.need = 0;
+        *p.sp = init;
+    }
+}
+
+// Concatenate the special-list segments (in segment order) into the
+// contiguous lists and record the counts and the capacity a call needs.
+__global__ void __launch_bounds__(1024) k_sp_gather(McParams p) {
+    __shared__ int64_t cnt[4][kSpSeg], off[4][kSpSeg];
+    __shared__ unsigned long long raw[4];
+    __shared__ int64_t need;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < 4) raw[tid] = 0;
+    if (tid == 0) need = 0;
+    __syncthreads();
+    {
+        const int k = tid / kSpSeg, sg = tid % kSpSeg;   // 1024 threads = 4 lists x 256 segments
+        const unsigned long long n = p.seg.ctr[(k * kSpSeg + sg) * kSpStride];
+        cnt[k][sg] = min((int64_t) n, p.seg.cap[k]);
+        atomicAdd(&raw[k], n);
+        const int mult = k == 1 ? 12 : (k >= 2 ? 5 : 1);
+        const int64_t want = (int64_t) ((n + mult - 1) / mult) * kSpSeg;
+        if (n > 0) atomicMax((unsigned long long *) &need, (unsigned long long) want);
+    }
+    __syncthreads();
+    if (tid < 4) {
+        int64_t run = 0;
+        for (int sg = 0; sg < kSpSeg; ++sg) {
+            off[tid][sg] = run;
+            run += cnt[tid][sg];
+        }
+    }
+    __syncthreads();
+    for (int q = wave; q < 4 * kSpSeg; q += 16) {
+        const int k = q / kSpSeg, sg = q % kSpSeg;
+        const int64_t n = cnt[k][sg], src = (int64_t) sg * p.seg.cap[k], dst = off[k][sg];
+        for (int64_t i = lane; i < n; i += 64) {
+            if (k == 0) p.sp->cand[dst + i] = p.seg.cand[src + i];
+            else if (k == 1) p.sp->scell[dst + i] = p.seg.scell[src + i];
+            else if (k == 2) p.sp->kept[dst + i] = p.seg.kept[src + i];
+            else p.sp->hole[dst + i] = p.seg.hole[src + i];
+        }
+    }
+    if (tid == 0) {
+        p.sp->n_cand = raw[0];
+        p.sp->n_scell = raw[1];
+        p.sp->n_kept = raw[2];
+        p.sp->n_hole = raw[3];
+        p.sp->need = need;
     }
 }
 
 // ---- host side ---------------------------------------------------------------
 
-template <int TX, int TY, int NZC, int NT>
-struct Cfg {
-    using G = TileGeom<TX, TY, NZC>;
-    static void tiles(const int64_t shape[3], int &ni, int &nj, int &nk) {
-        ni = (int) ((shape[0] - 1 + TX - 1) / TX);
-        nj = (int) ((shape[1] - 1 + TY - 1) / TY);
-        nk = (int) ((shape[2] - 1 + G::TZ - 1) / G::TZ);
-    }
-    static hipError_t launch(McParams &p, hipStream_t stream) {
-        hipLaunchKernelGGL((mc_dense_kernel<TX, TY, NZC, NT>), dim3((unsigned) p.ntiles), dim3(NT), 0,
-                           stream, p);
-        return hipGetLastError();
-    }
-};
-
-// Tile configurations the autotuner may choose from.
-#define ISO_MC_CONFIGS(X)       \
-    X(0, 16, 16, 4, 256)        \
-    X(1, 32, 16, 4, 256)        \
-    X(2, 16, 16, 2, 256)        \
-    X(3, 8, 8, 4, 256)          \
-    X(4, 32, 32, 4, 512)        \
-    X(5, 16, 8, 4, 256)
+// Emit tile configurations the autotuner may choose from: (cell rows along
+// x, along y, 64-cell chunks along z, threads).
+#define ISO_MC_CONFIGS(X)   \
+    X(0, 16, 16, 4, 256)    \
+    X(1, 32, 16, 4, 256)    \
+    X(2, 16, 16, 2, 256)    \
+    X(3, 16, 16, 1, 256)    \
+    X(4, 16, 16, 8, 256)    \
+    X(5, 32, 16, 2, 256)
 
 int mc_config_count() {
     int n = 0;
@@ -768,12 +850,13 @@ int mc_config_count() {
     return n;
 }
 
-bool mc_config_tiles(int cfg, const int64_t shape[3], int &ni, int &nj, int &nk, int64_t &pub) {
+bool mc_config_geom(int cfg, int &tx, int &ty, int &nzc) {
     switch (cfg) {
-#define X(id, tx, ty, nzc, nt)                          \
-    case id:                                            \
-        Cfg<tx, ty, nzc, nt>::tiles(shape, ni, nj, nk); \
-        pub = TileGeom<tx, ty, nzc>::PUB;               \
+#define X(id, tx_, ty_, nzc_, nt) \
+    case id:                      \
+        tx = tx_;                 \
+        ty = ty_;                 \
+        nzc = nzc_;               \
         return true;
         ISO_MC_CONFIGS(X)
 #undef X
@@ -796,31 +879,50 @@ hipError_t mc_upload_luts() {
 }
 
 hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
-                           hipEvent_t ev0, hipEvent_t ev1) {
+                           hipEvent_t *ev) {
     hipError_t e = mc_upload_luts();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(mc_dense_reset, dim3((unsigned) std::min<int64_t>(1024, (p.ntiles + 255) / 256 + 1)),
-                       dim3(256), 0, stream, p.status, p.ntiles, p.ticket, p.epoch_ptr, p.err,
-                       p.totals, p.sp, sp_init);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    // the kernel reads the epoch from device memory after the reset bumps it
-    if (ev0) {
-        e = hipEventRecord(ev0, stream);
-        if (e != hipSuccess) return e;
-    }
+    auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], stream) : hipSuccess; };
+    if ((e = mark(0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(mc_dense_reset, dim3((4 * kSpSeg + 255) / 256), dim3(256), 0, stream, p, sp_init);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+
+    const int64_t nwords = p.X * p.Y * p.W;
+    const unsigned cblocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nwords + 63) / 64, 8192));
+    if (p.vec4 && p.Z % 64 == 0)
+        hipLaunchKernelGGL(k_classify<0>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
+                           p.bits);
+    else if (p.vec4)
+        hipLaunchKernelGGL(k_classify<1>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
+                           p.bits);
+    else
+        hipLaunchKernelGGL(k_classify<2>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
+                           p.bits);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = mark(1)) != hipSuccess) return e;
+
+    hipLaunchKernelGGL(k_count, dim3((unsigned) p.nblk), dim3(256), 0, stream, p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = mark(2)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = mark(3)) != hipSuccess) return e;
+
+    const unsigned tiles = (unsigned) (p.NI * p.NJ * p.NK);
     switch (cfg) {
-#define X(id, tx, ty, nzc, nt)                     \
-    case id:                                       \
-        e = Cfg<tx, ty, nzc, nt>::launch(p, stream); \
+#define X(id, tx, ty, nzc, nt)                                                                     \
+    case id:                                                                                       \
+        hipLaunchKernelGGL((k_emit<tx, ty, nzc, nt>), dim3(tiles), dim3(nt), 0, stream, p);        \
         break;
         ISO_MC_CONFIGS(X)
 #undef X
     default:
         return hipErrorInvalidValue;
     }
-    if (e == hipSuccess && ev1) e = hipEventRecord(ev1, stream);
-    return e;
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = mark(4)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sp_gather, dim3(1), dim3(1024), 0, stream, p);
+    return hipGetLastError();
 }
 
 }   // namespace iso

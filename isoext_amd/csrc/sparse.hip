@@ -91,7 +91,7 @@ __device__ __forceinline__ bool load_cell(const SparseMcIn &in, int64_t i, SCell
 }
 
 __device__ __forceinline__ void spos(const SCell &s, float level, int e, float &x, float &y, float &z) {
-    int ca = kEdgeA[e], cb = kEdgeB[e];
+    int ca = edge_a(e), cb = edge_b(e);
     float t = edge_t(s.cv[ca], s.cv[cb], level);
     x = lerp1(t, s.c[0][ca >> 2], s.c[0][cb >> 2]);
     y = lerp1(t, s.c[1][(ca >> 1) & 1], s.c[1][(cb >> 1) & 1]);
@@ -102,7 +102,7 @@ __device__ __forceinline__ int scross(int cs) {
     int st = 0;
 #pragma unroll
     for (int e = 0; e < 12; ++e)
-        if (((cs >> kEdgeA[e]) & 1) != ((cs >> kEdgeB[e]) & 1)) st |= 1 << e;
+        if (((cs >> edge_a(e)) & 1) != ((cs >> edge_b(e)) & 1)) st |= 1 << e;
     return st;
 }
 
