@@ -204,6 +204,41 @@ int isoext_mc_sparse(const isoext_grid *g, float level, int method, int64_t nins
                      int64_t workspace_bytes, float *out_v, int64_t cap_v, int32_t *out_f,
                      int64_t cap_f, int64_t *vsrc, int64_t counts[2], void *stream);
 
+/* ---- ragged batches of dense marching cubes (no reference counterpart:
+ * the reference extracts one grid per call, src/mc/mc.cu:17-68) ---------- */
+
+/* One grid of a batch.  out_v / cap_v / out_f / cap_f are only read by
+ * isoext_mc_batch_emit. */
+typedef struct isoext_mc_item {
+    const float *values;      /* (X, Y, Z) float32, C order */
+    int64_t shape[3];
+    float aabb_min[3], aabb_max[3];
+    float level;
+    int method;               /* 0 = "nagae", 1 = "lorensen" */
+    float *out_v;
+    int64_t cap_v;
+    int32_t *out_f;
+    int64_t cap_f;
+} isoext_mc_item;
+
+/* Workspace for a batch (zero-filled before first use); special_cap is the
+ * weld-candidate capacity per item. */
+int64_t isoext_mc_batch_workspace_bytes(int n, const isoext_mc_item *items, int cfg, int64_t special_cap);
+
+/* Phase 1: classify, count and scan every item (one launch per stage for the
+ * whole batch).  counts[2i], counts[2i+1] = provisional vertices, faces of
+ * item i: the capacities its outputs need.  Synchronises `stream`. */
+int isoext_mc_batch_count(int n, const isoext_mc_item *items, int cfg, void *workspace, int64_t workspace_bytes,
+                          int64_t special_cap, int64_t *counts, void *stream);
+
+/* Phase 2 (same items and workspace, outputs set): emit every item, then the
+ * weld fix-up of the items that need it.  counts[2i], counts[2i+1] = final
+ * vertices, faces; -1, -1 when item i overflowed its special capacity (run it
+ * alone with isoext_mc_dense).  Results equal isoext_mc_dense's per item.
+ * Synchronises `stream`. */
+int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *workspace, int64_t workspace_bytes,
+                         int64_t special_cap, int64_t *counts, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

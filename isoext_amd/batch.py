@@ -1,13 +1,14 @@
 """Ragged batches of dense marching-cubes extractions.
 
-``mc_batch(grids)`` extracts every grid of a list of (values, shape, aabb,
-level, method) items with one launch per stage for the whole batch
-(classify, count, scan, emit), in two phases with a single host
+``mc_batch(jobs)`` extracts every grid of a list of (values, level, method,
+aabb) jobs with one launch per stage for the whole batch (classify, count,
+scan, emit, special-list gather), in two phases with a single host
 synchronisation each: phase 1 sizes every mesh, the outputs are allocated,
-phase 2 writes them.  The launch sequence of each phase is captured once in
-a HIP graph per (device, stream, workspace, tile config) and replayed; the
-batch content travels in a pinned descriptor buffer that the graph copies to
-the device.  Results are identical to one ``marching_cubes`` call per grid.
+phase 2 writes them, then the weld fix-up runs for the grids that need it.
+The kernels walk batch-wide index spaces (words, count blocks, tiles) and
+find their grid by binary search over a descriptor array uploaded from
+pinned memory.  Results are identical to one ``marching_cubes`` call per
+grid (tests/test_batch_gpu.py).
 """
 
 from __future__ import annotations
@@ -106,5 +107,11 @@ def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_bat
     res = []
     for i, (v, f) in enumerate(outs):
         nv, nf = int(counts[2 * i]), int(counts[2 * i + 1])
-        res.append((v[:nv], f[:nf]))
+        if nv < 0:   # special lists overflowed the batch capacity: run this grid alone
+            from .mc import mc_dense_raw
+            j = jobs[i]
+            v, f, _ = mc_dense_raw(j.values, tuple(j.values.shape), j.aabb_min, j.aabb_max, j.level, j.method, cfg)
+            res.append((v, f))
+        else:
+            res.append((v[:nv], f[:nf]))
     return res

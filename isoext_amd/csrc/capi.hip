@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/isoext_amd.h"
 #include "common.h"
@@ -88,7 +89,7 @@ bool mc_layout(const int64_t shape[3], int cfg, int64_t cap, McLayout &L) {
     L.o_bsum_f = take(sizeof(uint32_t) * L.nblk);
     L.o_blk_v = take(sizeof(int64_t) * L.nblk);
     L.o_blk_f = take(sizeof(int64_t) * L.nblk);
-    for (int k = 0; k < 4; ++k) L.seg_cap[k] = iso::sp_seg_cap(cap, k);
+    for (int k = 0; k < 4; ++k) L.seg_cap[k] = iso::sp_seg_cap(cap, k, iso::kSpSeg);
     L.o_ctr = take(sizeof(unsigned long long) * 4 * iso::kSpSeg * iso::kSpStride);
     L.o_scand = take(sizeof(iso::SpecialVertex) * iso::kSpSeg * L.seg_cap[0]);
     L.o_sscell = take(sizeof(iso::SpecialEdge) * iso::kSpSeg * L.seg_cap[1]);
@@ -186,6 +187,201 @@ DcLayout dc_layout(int64_t ncells, int64_t npts) {
     L.o_fin = take(sizeof(int32_t) * std::max<int64_t>(ncells, 1));
     L.total = o;
     return L;
+}
+
+
+// ---- ragged batches of dense marching cubes --------------------------------------
+
+constexpr int kBatchSeg = 16;   // special-list segments per batch item
+
+struct ItemLayout {
+    bool cells;
+    int64_t W, S, NI, NJ, NK, nblk, nwords, ntiles, seg_cap[4];
+    size_t o_hdr, o_bits, o_cnt_v, o_cnt_f, o_bsum_v, o_bsum_f, o_blk_v, o_blk_f;
+    size_t o_ctr, o_scand, o_sscell, o_skept, o_shole, o_cand, o_scell, o_kept, o_hole;
+};
+
+struct BatchLayout {
+    size_t o_hdr, o_items, o_summary, total;
+    std::vector<ItemLayout> it;
+    iso::McBatchHdr h;
+};
+
+bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_cap, BatchLayout &B) {
+    int tx, ty, nzc;
+    if (!iso::mc_config_geom(cfg, tx, ty, nzc)) return false;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t r = o;
+        o = align256(o + bytes);
+        return r;
+    };
+    B.o_hdr = take(sizeof(iso::McBatchHdr));
+    B.o_items = take(sizeof(iso::McBatchItem) * (size_t) n);
+    B.o_summary = take(sizeof(iso::McBatchSummary) * (size_t) n);
+    B.it.assign(n, ItemLayout{});
+    B.h = iso::McBatchHdr{n, 0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+        ItemLayout &L = B.it[i];
+        const int64_t X = items[i].shape[0], Y = items[i].shape[1], Z = items[i].shape[2];
+        L.cells = X >= 2 && Y >= 2 && Z >= 2;
+        L.W = L.cells ? (Z + 63) / 64 : 0;
+        L.S = L.cells ? (L.W + nzc - 1) / nzc : 0;
+        L.NI = L.cells ? (X - 1 + tx - 1) / tx : 0;
+        L.NJ = L.cells ? (Y - 1 + ty - 1) / ty : 0;
+        L.NK = L.cells ? ((Z - 1 + 63) / 64 + nzc - 1) / nzc : 0;
+        L.nblk = L.cells ? (X * Y * L.S + iso::kMcScanBlock - 1) / iso::kMcScanBlock : 0;
+        L.nwords = L.cells ? X * Y * L.W : 0;
+        L.ntiles = L.NI * L.NJ * L.NK;
+        B.h.nwords += L.nwords;
+        B.h.nblk += L.nblk;
+        B.h.ntiles += L.ntiles;
+        const int64_t R = L.cells ? X * Y : 0;
+        for (int k = 0; k < 4; ++k) L.seg_cap[k] = iso::sp_seg_cap(special_cap, k, kBatchSeg);
+        L.o_hdr = take(kHeader);
+        L.o_bits = take(sizeof(uint64_t) * R * L.W);
+        L.o_cnt_v = take(sizeof(uint32_t) * R * L.S);
+        L.o_cnt_f = take(sizeof(uint32_t) * R * L.S);
+        L.o_bsum_v = take(sizeof(uint32_t) * std::max<int64_t>(L.nblk, 1));
+        L.o_bsum_f = take(sizeof(uint32_t) * std::max<int64_t>(L.nblk, 1));
+        L.o_blk_v = take(sizeof(int64_t) * std::max<int64_t>(L.nblk, 1));
+        L.o_blk_f = take(sizeof(int64_t) * std::max<int64_t>(L.nblk, 1));
+        L.o_ctr = take(sizeof(unsigned long long) * 4 * kBatchSeg * iso::kSpStride);
+        L.o_scand = take(sizeof(iso::SpecialVertex) * kBatchSeg * L.seg_cap[0]);
+        L.o_sscell = take(sizeof(iso::SpecialEdge) * kBatchSeg * L.seg_cap[1]);
+        L.o_skept = take(sizeof(int64_t) * kBatchSeg * L.seg_cap[2]);
+        L.o_shole = take(sizeof(int64_t) * kBatchSeg * L.seg_cap[3]);
+        L.o_cand = take(sizeof(iso::SpecialVertex) * kBatchSeg * L.seg_cap[0]);
+        L.o_scell = take(sizeof(iso::SpecialEdge) * kBatchSeg * L.seg_cap[1]);
+        L.o_kept = take(sizeof(int64_t) * kBatchSeg * L.seg_cap[2]);
+        L.o_hole = take(sizeof(int64_t) * kBatchSeg * L.seg_cap[3]);
+    }
+    B.total = o;
+    return true;
+}
+
+// Pinned staging for the descriptor upload and the summary read-back,
+// grown on demand, one per host thread (reused after each phase's sync).
+struct Pinned {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    ~Pinned() {
+        if (ptr) (void) hipHostFree(ptr);
+    }
+    void *get(size_t want) {
+        if (want > bytes) {
+            if (ptr) (void) hipHostFree(ptr);
+            ptr = nullptr;
+            bytes = 0;
+            if (hipHostMalloc(&ptr, want, 0) != hipSuccess) return nullptr;
+            bytes = want;
+        }
+        return ptr;
+    }
+};
+thread_local Pinned g_pinned;
+
+void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t word0, int64_t blk0,
+               int64_t tile0, iso::McBatchItem &out) {
+    iso::McParams &p = out.p;
+    p = iso::McParams{};
+    p.values = in.values;
+    p.X = in.shape[0];
+    p.Y = in.shape[1];
+    p.Z = in.shape[2];
+    for (int a = 0; a < 3; ++a) {
+        p.amin[a] = in.aabb_min[a];
+        p.amax[a] = in.aabb_max[a];
+    }
+    p.level = in.level;
+    p.method = in.method;
+    p.all_special = L.cells && !grid_well_separated(in.shape, in.aabb_min, in.aabb_max) ? 1 : 0;
+    p.vec4 = (in.shape[2] % 4 == 0 && ((uintptr_t) in.values & 15) == 0) ? 1 : 0;
+    p.W = L.W;
+    p.S = L.S;
+    p.NI = L.NI;
+    p.NJ = L.NJ;
+    p.NK = L.NK;
+    p.nzc = 1;   // set by the caller from the tile config
+    p.bits = (uint64_t *) (ws + L.o_bits);
+    p.cnt_v = (uint32_t *) (ws + L.o_cnt_v);
+    p.cnt_f = (uint32_t *) (ws + L.o_cnt_f);
+    p.bsum_v = (uint32_t *) (ws + L.o_bsum_v);
+    p.bsum_f = (uint32_t *) (ws + L.o_bsum_f);
+    p.blk_v = (int64_t *) (ws + L.o_blk_v);
+    p.blk_f = (int64_t *) (ws + L.o_blk_f);
+    p.nblk = L.nblk;
+    p.out_v = in.out_v;
+    p.cap_v = in.out_v ? in.cap_v : 0;
+    p.out_key = nullptr;
+    p.out_f = in.out_f;
+    p.cap_f = in.out_f ? in.cap_f : 0;
+    p.err = (int *) (ws + L.o_hdr + kOffErr);
+    p.totals = (int64_t *) (ws + L.o_hdr + kOffTotals);
+    p.sp = (iso::SpecialLists *) (ws + L.o_hdr + kOffSp);
+    p.seg.ctr = (unsigned long long *) (ws + L.o_ctr);
+    for (int k = 0; k < 4; ++k) p.seg.cap[k] = L.seg_cap[k];
+    p.seg.nseg = kBatchSeg;
+    p.seg.cand = (iso::SpecialVertex *) (ws + L.o_scand);
+    p.seg.scell = (iso::SpecialEdge *) (ws + L.o_sscell);
+    p.seg.kept = (int64_t *) (ws + L.o_skept);
+    p.seg.hole = (int64_t *) (ws + L.o_shole);
+    iso::SpecialLists &sp = out.sp_init;
+    sp = iso::SpecialLists{};
+    sp.cap_cand = kBatchSeg * L.seg_cap[0];
+    sp.cap_scell = kBatchSeg * L.seg_cap[1];
+    sp.cap_kept = kBatchSeg * L.seg_cap[2];
+    sp.cap_hole = kBatchSeg * L.seg_cap[3];
+    sp.cand = (iso::SpecialVertex *) (ws + L.o_cand);
+    sp.scell = (iso::SpecialEdge *) (ws + L.o_scell);
+    sp.kept = (int64_t *) (ws + L.o_kept);
+    sp.hole = (int64_t *) (ws + L.o_hole);
+    out.word0 = word0;
+    out.blk0 = blk0;
+    out.tile0 = tile0;
+    out.mode = p.vec4 ? (p.Z % 64 == 0 ? 0 : 1) : 2;
+}
+
+// Upload the descriptors of a batch phase (pinned staging) and return the
+// staging area for the summaries.
+int batch_upload(int n, const isoext_mc_item *items, int cfg, char *ws, int64_t special_cap, const BatchLayout &B,
+                 hipStream_t st, iso::McBatchSummary **summary_host) {
+    int tx, ty, nzc;
+    iso::mc_config_geom(cfg, tx, ty, nzc);
+    const size_t desc = sizeof(iso::McBatchHdr) + sizeof(iso::McBatchItem) * (size_t) n;
+    const size_t sum = sizeof(iso::McBatchSummary) * (size_t) n;
+    char *host = (char *) g_pinned.get(desc + sum + 64);
+    if (!host) return fail(ISOEXT_E_DEVICE, "pinned staging allocation failed");
+    iso::McBatchHdr *h = (iso::McBatchHdr *) host;
+    *h = B.h;
+    iso::McBatchItem *it = (iso::McBatchItem *) (host + sizeof(iso::McBatchHdr));
+    int64_t w0 = 0, b0 = 0, t0 = 0;
+    for (int i = 0; i < n; ++i) {
+        fill_item(items[i], B.it[i], ws, w0, b0, t0, it[i]);
+        it[i].p.nzc = nzc;
+        w0 += B.it[i].nwords;
+        b0 += B.it[i].nblk;
+        t0 += B.it[i].ntiles;
+    }
+    HIP_CHECK(hipMemcpyAsync(ws + B.o_hdr, h, sizeof(iso::McBatchHdr), hipMemcpyHostToDevice, st), "upload batch");
+    HIP_CHECK(hipMemcpyAsync(ws + B.o_items, it, sizeof(iso::McBatchItem) * (size_t) n, hipMemcpyHostToDevice, st),
+              "upload batch");
+    *summary_host = (iso::McBatchSummary *) (host + ((desc + 63) & ~size_t(63)));
+    return ISOEXT_OK;
+}
+
+int batch_check(int n, const isoext_mc_item *items, int cfg) {
+    int tx, ty, nzc;
+    if (n < 1) return fail(ISOEXT_E_ARG, "empty batch");
+    if (!iso::mc_config_geom(cfg, tx, ty, nzc)) return fail(ISOEXT_E_ARG, "unknown tile config %d", cfg);
+    for (int i = 0; i < n; ++i) {
+        if (items[i].method != 0 && items[i].method != 1)
+            return fail(ISOEXT_E_ARG, "Unknown method: %d", items[i].method);
+        for (int a = 0; a < 3; ++a)
+            if (items[i].shape[a] < 1) return fail(ISOEXT_E_ARG, "grid shape must be positive");
+        if (!items[i].values) return fail(ISOEXT_E_ARG, "values is null");
+    }
+    return ISOEXT_OK;
 }
 
 }   // namespace
@@ -395,6 +591,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     p.sp = (iso::SpecialLists *) (ws + kOffSp);
     p.seg.ctr = (unsigned long long *) (ws + L.o_ctr);
     for (int k = 0; k < 4; ++k) p.seg.cap[k] = L.seg_cap[k];
+    p.seg.nseg = iso::kSpSeg;
     p.seg.cand = (iso::SpecialVertex *) (ws + L.o_scand);
     p.seg.scell = (iso::SpecialEdge *) (ws + L.o_sscell);
     p.seg.kept = (int64_t *) (ws + L.o_skept);
@@ -579,6 +776,88 @@ int isoext_mc_backward(const isoext_grid *g, float level, const float *grad_v, c
     else
         HIP_CHECK(iso::mc_backward_dense(grad_v, vertex_src, nv, g->values, d, level, grad_values, st),
                   "marching cubes backward");
+    return ISOEXT_OK;
+}
+
+int64_t isoext_mc_batch_workspace_bytes(int n, const isoext_mc_item *items, int cfg, int64_t special_cap) {
+    if (batch_check(n, items, cfg) != ISOEXT_OK) return -1;
+    BatchLayout B;
+    if (!batch_layout(n, items, cfg, special_cap, B)) return -1;
+    return (int64_t) B.total;
+}
+
+int isoext_mc_batch_count(int n, const isoext_mc_item *items, int cfg, void *workspace, int64_t workspace_bytes,
+                          int64_t special_cap, int64_t *counts, void *stream) {
+    int rc = batch_check(n, items, cfg);
+    if (rc != ISOEXT_OK) return rc;
+    for (int i = 0; i < 2 * n; ++i) counts[i] = 0;
+    BatchLayout B;
+    batch_layout(n, items, cfg, special_cap, B);
+    if ((int64_t) B.total > workspace_bytes)
+        return fail(ISOEXT_E_WORKSPACE, "workspace too small: %lld < %lld bytes", (long long) workspace_bytes,
+                    (long long) B.total);
+    hipStream_t st = (hipStream_t) stream;
+    char *ws = (char *) workspace;
+    iso::McBatchSummary *sum = nullptr;
+    if ((rc = batch_upload(n, items, cfg, ws, special_cap, B, st, &sum)) != ISOEXT_OK) return rc;
+    HIP_CHECK(iso::mc_batch_count_launch((const iso::McBatchItem *) (ws + B.o_items),
+                                         (const iso::McBatchHdr *) (ws + B.o_hdr), B.h,
+                                         (iso::McBatchSummary *) (ws + B.o_summary), st),
+              "launch batch count");
+    HIP_CHECK(hipMemcpyAsync(sum, ws + B.o_summary, sizeof(iso::McBatchSummary) * (size_t) n, hipMemcpyDeviceToHost,
+                             st),
+              "read batch counts");
+    HIP_CHECK(hipStreamSynchronize(st), "batch count");
+    for (int i = 0; i < n; ++i) {
+        if (sum[i].err & iso::ISO_ERR_TOO_LARGE)
+            return fail(ISOEXT_E_TOO_LARGE, "item %d: mesh exceeds 2^31-1 vertices or faces", i);
+        counts[2 * i] = sum[i].nv;
+        counts[2 * i + 1] = sum[i].nf;
+    }
+    return ISOEXT_OK;
+}
+
+int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *workspace, int64_t workspace_bytes,
+                         int64_t special_cap, int64_t *counts, void *stream) {
+    int rc = batch_check(n, items, cfg);
+    if (rc != ISOEXT_OK) return rc;
+    BatchLayout B;
+    batch_layout(n, items, cfg, special_cap, B);
+    if ((int64_t) B.total > workspace_bytes)
+        return fail(ISOEXT_E_WORKSPACE, "workspace too small: %lld < %lld bytes", (long long) workspace_bytes,
+                    (long long) B.total);
+    hipStream_t st = (hipStream_t) stream;
+    char *ws = (char *) workspace;
+    iso::McBatchSummary *sum = nullptr;
+    if ((rc = batch_upload(n, items, cfg, ws, special_cap, B, st, &sum)) != ISOEXT_OK) return rc;
+    HIP_CHECK(iso::mc_batch_emit_launch(cfg, (const iso::McBatchItem *) (ws + B.o_items),
+                                        (const iso::McBatchHdr *) (ws + B.o_hdr), B.h,
+                                        (iso::McBatchSummary *) (ws + B.o_summary), st),
+              "launch batch emit");
+    HIP_CHECK(hipMemcpyAsync(sum, ws + B.o_summary, sizeof(iso::McBatchSummary) * (size_t) n, hipMemcpyDeviceToHost,
+                             st),
+              "read batch counts");
+    HIP_CHECK(hipStreamSynchronize(st), "batch emit");
+    for (int i = 0; i < n; ++i) {
+        int64_t nv = sum[i].nv, nf = sum[i].nf;
+        counts[2 * i] = nv;
+        counts[2 * i + 1] = nf;
+        if (nv > items[i].cap_v || nf > items[i].cap_f)
+            return fail(ISOEXT_E_CAPACITY, "item %d: output capacity too small", i);
+        if (sum[i].need > special_cap) {   // special lists overflowed: the caller re-runs this item alone
+            counts[2 * i] = counts[2 * i + 1] = -1;
+            continue;
+        }
+        if (sum[i].n_cand > 0 || sum[i].n_hole > 0) {
+            iso::SpecialLists sp_dev;
+            HIP_CHECK(hipMemcpy(&sp_dev, ws + B.it[i].o_hdr + kOffSp, sizeof(sp_dev), hipMemcpyDeviceToHost),
+                      "read special lists");
+            iso::FixupGrid g{items[i].shape[0], items[i].shape[1], items[i].shape[2]};
+            HIP_CHECK(iso::weld_fixup(sp_dev, items[i].out_v, items[i].out_f, nv, nf, g, st, nullptr), "weld fix-up");
+            counts[2 * i] = nv;
+            counts[2 * i + 1] = nf;
+        }
+    }
     return ISOEXT_OK;
 }
 

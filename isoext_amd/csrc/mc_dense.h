@@ -38,25 +38,26 @@ struct SpecialLists {
     int64_t need;          // special capacity a call needs (segment overflow included)
 };
 
-// The emit kernel appends special entries through kSpSeg independent
-// counters per list (workgroup b uses segment b % kSpSeg, each counter on its
-// own 256-byte line), so the atomics of different workgroups do not
-// serialise on one address; a gather pass concatenates the segments into
-// the contiguous SpecialLists.
+// The emit kernel appends special entries through nseg (<= kSpSeg)
+// independent counters per list (workgroup b uses segment b % nseg, each
+// counter on its own 256-byte line), so the atomics of different workgroups
+// do not serialise on one address; a gather pass concatenates the segments
+// into the contiguous SpecialLists.
 constexpr int kSpSeg = 256;
 constexpr int kSpStride = 32;
 struct SpecialSegs {
-    unsigned long long *ctr;   // [4][kSpSeg * kSpStride]: cand, scell, kept, hole
+    unsigned long long *ctr;   // [4][nseg * kSpStride]: cand, scell, kept, hole
     int64_t cap[4];            // entries per segment
-    SpecialVertex *cand;       // [kSpSeg * cap[0]]
-    SpecialEdge *scell;        // [kSpSeg * cap[1]]
-    int64_t *kept, *hole;      // [kSpSeg * cap[2]], [kSpSeg * cap[3]]
+    int nseg;
+    SpecialVertex *cand;       // [nseg * cap[0]]
+    SpecialEdge *scell;        // [nseg * cap[1]]
+    int64_t *kept, *hole;      // [nseg * cap[2]], [nseg * cap[3]]
 };
 // Entries of each list per unit of special capacity: cand 1, scell 12,
 // kept 5, hole 5.
-inline int64_t sp_seg_cap(int64_t special_cap, int k) {
+inline int64_t sp_seg_cap(int64_t special_cap, int k, int nseg) {
     const int64_t mult = k == 1 ? 12 : (k >= 2 ? 5 : 1);
-    return (special_cap * mult + kSpSeg - 1) / kSpSeg;
+    return (special_cap * mult + nseg - 1) / nseg;
 }
 
 // Entries of (point row, z-segment) counted and scanned per workgroup.
@@ -97,9 +98,36 @@ struct McTimes {
     float classify, count, scan, emit, total;
 };
 
+// One grid of a ragged batch: its parameters (pointers into the batch
+// workspace) and where its work starts in the batch-wide index spaces.
+struct McBatchItem {
+    McParams p;
+    SpecialLists sp_init;   // contiguous special lists (counters zeroed by reset)
+    int64_t word0;          // first classify word
+    int64_t blk0;           // first count block
+    int64_t tile0;          // first emit tile
+    int mode;               // classify loads: 0 Z%64 == 0 float4, 1 float4, 2 scalar
+};
+
+struct McBatchHdr {
+    int64_t n, nwords, nblk, ntiles;
+};
+
+// Per-item result of a batch phase, gathered for one read-back.
+struct McBatchSummary {
+    int64_t err, nv, nf, need;
+    unsigned long long n_cand, n_hole;
+};
+
 int mc_config_count();
 bool mc_config_geom(int cfg, int &tx, int &ty, int &nzc);
 hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
                            hipEvent_t *ev = nullptr);
+// Batch phases: classify + count + scan of every item, then emit + gather.
+// Both end with a kernel that writes one McBatchSummary per item.
+hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr, const McBatchHdr &h,
+                                 McBatchSummary *summary, hipStream_t stream);
+hipError_t mc_batch_emit_launch(int cfg, const McBatchItem *items, const McBatchHdr *hdr, const McBatchHdr &h,
+                                McBatchSummary *summary, hipStream_t stream);
 
 }   // namespace iso

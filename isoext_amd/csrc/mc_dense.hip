@@ -49,7 +49,7 @@ static __constant__ uint64_t cTris[2][256];
 // hole); returns the entry index, or -1 when the segment is full (the counter
 // still counts so the host can size a retry).
 __device__ __forceinline__ int64_t seg_push(const McParams &p, int k, int seg) {
-    const unsigned long long i = atomicAdd(&p.seg.ctr[(k * kSpSeg + seg) * kSpStride], 1ull);
+    const unsigned long long i = atomicAdd(&p.seg.ctr[(k * p.seg.nseg + seg) * kSpStride], 1ull);
     return (int64_t) i < p.seg.cap[k] ? (int64_t) seg * p.seg.cap[k] + (int64_t) i : -1;
 }
 
@@ -295,7 +295,7 @@ __device__ __forceinline__ int64_t entry_chunk(const McParams &p, int64_t i) {
 // One workgroup per kMcScanBlock entries.  Lanes walk the block's chunks in
 // memory order (coalesced mask loads), add their counts into the chunk's
 // entry in LDS, then the entries are scanned.
-__global__ void __launch_bounds__(256) k_count(McParams p) {
+__device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
     constexpr int NT = 256, PER = kMcScanBlock / NT;
     __shared__ uint32_t ent_v[kMcScanBlock], ent_f[kMcScanBlock];
     __shared__ uint32_t sa[NT / 64], sb[NT / 64];
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(256) k_count(McParams p) {
     const int tid = threadIdx.x;
     for (int c = tid; c < 256; c += NT) ntri[c] = (uint8_t) (cTris[p.method][c] & 15);
     const int64_t n = p.X * p.Y * p.S;
-    const int64_t i0 = (int64_t) blockIdx.x * kMcScanBlock;
+    const int64_t i0 = blk * kMcScanBlock;
     const int64_t i1 = min(i0 + (int64_t) kMcScanBlock, n);
     for (int k = tid; k < kMcScanBlock; k += NT) ent_v[k] = ent_f[k] = 0;
     __syncthreads();
@@ -358,14 +358,16 @@ __global__ void __launch_bounds__(256) k_count(McParams p) {
         tf += cf[e];
     }
     if (tid == 0) {
-        p.bsum_v[blockIdx.x] = bv;
-        p.bsum_f[blockIdx.x] = bf;
+        p.bsum_v[blk] = bv;
+        p.bsum_f[blk] = bf;
     }
 }
 
+__global__ void __launch_bounds__(256) k_count(McParams p) { count_block(p, blockIdx.x); }
+
 // Exclusive scan of the block totals (one workgroup, carried across rounds);
 // writes the mesh size and flags meshes beyond int32 indices.
-__global__ void __launch_bounds__(1024) k_scan_top(McParams p) {
+__device__ __forceinline__ void scan_top(const McParams &p) {
     constexpr int NT = 1024, NW = NT / 64;
     __shared__ int64_t sv[NW], sf[NW];
     __shared__ int64_t carry_v, carry_f;
@@ -415,6 +417,8 @@ __global__ void __launch_bounds__(1024) k_scan_top(McParams p) {
         if (carry_v > 0x7fffffffll || carry_f > 0x7fffffffll) set_err(p, ISO_ERR_TOO_LARGE);
     }
 }
+
+__global__ void __launch_bounds__(1024) k_scan_top(McParams p) { scan_top(p); }
 
 // ---- emit ------------------------------------------------------------------------
 template <int TX, int TY, int NZC, int NT>
@@ -495,13 +499,12 @@ __device__ __forceinline__ void block_exscan_lds(uint32_t *a, uint32_t *tsum) {
 }
 
 template <int TX, int TY, int NZC, int NT>
-__global__ void __launch_bounds__(NT) k_emit(McParams p) {
+__device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int spseg) {
     using L = EmitLds<TX, TY, NZC, NT>;
     __shared__ L s;
     const int tid = threadIdx.x;
     if (*(volatile int *) p.err & ISO_ERR_TOO_LARGE) return;
 
-    const int64_t tile = blockIdx.x;
     const int64_t tk = tile % p.NK, tj = (tile / p.NK) % p.NJ, ti = tile / (p.NK * p.NJ);
     const int64_t x0 = ti * TX, y0 = tj * TY, c0 = tk * NZC, z0 = 64 * c0;
     const int nx = (int) min<int64_t>(TX, p.X - 1 - x0);
@@ -609,7 +612,6 @@ __global__ void __launch_bounds__(NT) k_emit(McParams p) {
 
     // ---- emit, NT active cells per batch ----------------------------------------
     const float lvl = p.level;
-    const int spseg = (int) (blockIdx.x % kSpSeg);
     for (int j0 = 0; j0 < nactive; j0 += NT) {
         // fill the batch list: unit, z-bit and face offset inside the unit
         for (int u = tid; u < L::NUNITS; u += NT) {
@@ -693,9 +695,7 @@ __global__ void __launch_bounds__(NT) k_emit(McParams p) {
                     if (v >= 0 && v < p.cap_v) {
                         *reinterpret_cast<float3 *>(p.out_v + 3 * v) = make_float3(x, y, z);
                         if (p.out_key) p.out_key[v] = edge_key(p, gx, gy, gz, E);
-                    } else {
-                        set_err(p, ISO_ERR_CAPACITY);
-                    }
+                    }   // else: the host sees totals beyond the capacity and retries
                     if ((candm >> E) & 1) {
                         const int64_t i = seg_push(p, 0, spseg);
                         if (i >= 0) {
@@ -746,8 +746,6 @@ __global__ void __launch_bounds__(NT) k_emit(McParams p) {
                     } else {
                         *reinterpret_cast<int3 *>(dst) = make_int3(-1, -1, -1);
                     }
-                } else {
-                    set_err(p, ISO_ERR_CAPACITY);
                 }
             }
             if (special) {
@@ -771,9 +769,14 @@ __global__ void __launch_bounds__(NT) k_emit(McParams p) {
     }
 }
 
+template <int TX, int TY, int NZC, int NT>
+__global__ void __launch_bounds__(NT) k_emit(McParams p) {
+    emit_tile<TX, TY, NZC, NT>(p, blockIdx.x, (int) (blockIdx.x % p.seg.nseg));
+}
+
 __global__ void mc_dense_reset(McParams p, SpecialLists init) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < 4 * kSpSeg) p.seg.ctr[t * kSpStride] = 0;
+    if (t < 4 * p.seg.nseg) p.seg.ctr[t * kSpStride] = 0;
     if (t == 0) {
         *p.err = 0;
         p.totals[0] = p.totals[1] = 0;
@@ -785,48 +788,213 @@ __global__ void mc_dense_reset(McParams p, SpecialLists init) {
 
 // Concatenate the special-list segments (in segment order) into the
 // contiguous lists and record the counts and the capacity a call needs.
-__global__ void __launch_bounds__(1024) k_sp_gather(McParams p) {
-    __shared__ int64_t cnt[4][kSpSeg], off[4][kSpSeg];
-    __shared__ unsigned long long raw[4];
-    __shared__ int64_t need;
+// Workgroup b copies segment b of every list; its offset is the sum of the
+// (clamped) counts of the segments before it.
+static_assert(kSpSeg == 256, "sp_gather_seg maps one thread to each segment");
+__device__ __forceinline__ void sp_gather_seg(const McParams &p, int b) {
+    __shared__ int64_t part[4][4];
+    __shared__ int64_t off[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid < 4) raw[tid] = 0;
-    if (tid == 0) need = 0;
-    __syncthreads();
-    {
-        const int k = tid / kSpSeg, sg = tid % kSpSeg;   // 1024 threads = 4 lists x 256 segments
-        const unsigned long long n = p.seg.ctr[(k * kSpSeg + sg) * kSpStride];
-        cnt[k][sg] = min((int64_t) n, p.seg.cap[k]);
-        atomicAdd(&raw[k], n);
-        const int mult = k == 1 ? 12 : (k >= 2 ? 5 : 1);
-        const int64_t want = (int64_t) ((n + mult - 1) / mult) * kSpSeg;
-        if (n > 0) atomicMax((unsigned long long *) &need, (unsigned long long) want);
+    // thread t reads segment t of each list
+    int64_t below[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t n = tid < b ? (int64_t) p.seg.ctr[(k * p.seg.nseg + tid) * kSpStride] : 0;
+        below[k] = min(n, p.seg.cap[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        int64_t v = below[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) part[k][wave] = v;
     }
     __syncthreads();
-    if (tid < 4) {
-        int64_t run = 0;
-        for (int sg = 0; sg < kSpSeg; ++sg) {
-            off[tid][sg] = run;
-            run += cnt[tid][sg];
-        }
-    }
+    if (tid < 4) off[tid] = part[tid][0] + part[tid][1] + part[tid][2] + part[tid][3];
     __syncthreads();
-    for (int q = wave; q < 4 * kSpSeg; q += 16) {
-        const int k = q / kSpSeg, sg = q % kSpSeg;
-        const int64_t n = cnt[k][sg], src = (int64_t) sg * p.seg.cap[k], dst = off[k][sg];
-        for (int64_t i = lane; i < n; i += 64) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t n = min((int64_t) p.seg.ctr[(k * p.seg.nseg + b) * kSpStride], p.seg.cap[k]);
+        const int64_t src = (int64_t) b * p.seg.cap[k], dst = off[k];
+        for (int64_t i = tid; i < n; i += 256) {
             if (k == 0) p.sp->cand[dst + i] = p.seg.cand[src + i];
             else if (k == 1) p.sp->scell[dst + i] = p.seg.scell[src + i];
             else if (k == 2) p.sp->kept[dst + i] = p.seg.kept[src + i];
             else p.sp->hole[dst + i] = p.seg.hole[src + i];
         }
     }
-    if (tid == 0) {
-        p.sp->n_cand = raw[0];
-        p.sp->n_scell = raw[1];
-        p.sp->n_kept = raw[2];
-        p.sp->n_hole = raw[3];
-        p.sp->need = need;
+    if (b == 0) {
+        // totals (unclamped) and the capacity a call needs
+        __shared__ unsigned long long raw[4], need;
+        if (tid < 4) raw[tid] = 0;
+        if (tid == 0) need = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const unsigned long long n = tid < p.seg.nseg ? p.seg.ctr[(k * p.seg.nseg + tid) * kSpStride] : 0ull;
+            atomicAdd(&raw[k], n);
+            const int mult = k == 1 ? 12 : (k >= 2 ? 5 : 1);
+            if (n > 0) atomicMax(&need, ((n + mult - 1) / mult) * (unsigned long long) p.seg.nseg);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            p.sp->n_cand = raw[0];
+            p.sp->n_scell = raw[1];
+            p.sp->n_kept = raw[2];
+            p.sp->n_hole = raw[3];
+            p.sp->need = (int64_t) need;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sp_gather(McParams p) { sp_gather_seg(p, blockIdx.x); }
+
+// ---- ragged batches ----------------------------------------------------------------
+// Every kernel walks the batch-wide index space of its stage (classify words,
+// count blocks, emit tiles) and finds the owning item by binary search over
+// the items' start offsets; the per-item work is the single-grid code.
+
+// Last item whose start (field 0 word0, 1 blk0, 2 tile0) is <= x.
+__device__ __forceinline__ int find_item(const McBatchItem *it, int n, int64_t x, int which) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        const int64_t st = which == 0 ? it[mid].word0 : (which == 1 ? it[mid].blk0 : it[mid].tile0);
+        if (st <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+    const int n = (int) hdr->n;
+    for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < (int64_t) n * 4 * kSpSeg;
+         t += (int64_t) gridDim.x * 256) {
+        const int i = (int) (t / (4 * kSpSeg)), c = (int) (t % (4 * kSpSeg));
+        const McParams &p = items[i].p;
+        if (c < 4 * p.seg.nseg) p.seg.ctr[c * kSpStride] = 0;
+        if (c == 0) {
+            *p.err = 0;
+            p.totals[0] = p.totals[1] = 0;
+            SpecialLists init = items[i].sp_init;
+            init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
+            init.need = 0;
+            *p.sp = init;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+    constexpr int U = 4;
+    const int n = (int) hdr->n;
+    const int64_t nwords = hdr->nwords;
+    const int lane = threadIdx.x & 63;
+    const int grp = lane >> 4, q = lane & 15;
+    const int64_t wave = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t nwave = (int64_t) gridDim.x * 4;
+    for (int64_t base = wave * (4 * U); base < nwords; base += nwave * (4 * U)) {
+        float v[U][4];
+        uint32_t valid[U];
+        int item[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t gw = base + 4 * u + grp;
+            v[u][0] = v[u][1] = v[u][2] = v[u][3] = 0.0f;
+            valid[u] = 0;
+            item[u] = 0;
+            if (gw < nwords) {
+                const int i = find_item(items, n, gw, 0);
+                item[u] = i;
+                const McBatchItem &it = items[i];
+                const int64_t w = gw - it.word0;
+                const int64_t Z = it.p.Z, W = it.p.W;
+                const int64_t row = w / W, c = w - row * W;
+                const int64_t z = 64 * c + 4 * q;
+                const int64_t left = Z - z;
+                const float *src = it.p.values + row * Z + z;
+                if (left > 0) {
+                    if (it.mode <= 1) {
+                        const float4 f = *reinterpret_cast<const float4 *>(src);
+                        v[u][0] = f.x;
+                        v[u][1] = f.y;
+                        v[u][2] = f.z;
+                        v[u][3] = f.w;
+                        valid[u] = 15;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if (j < left) v[u][j] = src[j];
+                        valid[u] = left >= 4 ? 15u : ((1u << left) - 1);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t gw = base + 4 * u + grp;
+            const float level = gw < nwords ? items[item[u]].p.level : 0.0f;
+            uint32_t x = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x |= (uint32_t) (((valid[u] >> j) & 1) && inside(v[u][j], level)) << j;
+            x |= __shfl_down(x, 1, 16) << 4;
+            x |= __shfl_down(x, 2, 16) << 8;
+            x |= __shfl_down(x, 4, 16) << 16;
+            const uint32_t hi = __shfl_down(x, 8, 16);
+            if (q == 0 && gw < nwords) {
+                const McBatchItem &it = items[item[u]];
+                it.p.bits[gw - it.word0] = (uint64_t) x | ((uint64_t) hi << 32);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_count_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+    const int n = (int) hdr->n;
+    for (int64_t gb = blockIdx.x; gb < hdr->nblk; gb += gridDim.x) {
+        const int i = find_item(items, n, gb, 1);
+        __syncthreads();
+        count_block(items[i].p, gb - items[i].blk0);
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_top_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+    for (int64_t i = blockIdx.x; i < hdr->n; i += gridDim.x) {
+        __syncthreads();
+        scan_top(items[i].p);
+    }
+}
+
+template <int TX, int TY, int NZC, int NT>
+__global__ void __launch_bounds__(NT) k_emit_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+    const int n = (int) hdr->n;
+    for (int64_t gt = blockIdx.x; gt < hdr->ntiles; gt += gridDim.x) {
+        const int i = find_item(items, n, gt, 2);
+        const int64_t t = gt - items[i].tile0;
+        __syncthreads();
+        emit_tile<TX, TY, NZC, NT>(items[i].p, t, (int) (t % items[i].p.seg.nseg));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sp_gather_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+    for (int64_t i = 0; i < hdr->n; ++i) {
+        const int nseg = items[i].p.seg.nseg;
+        for (int b = blockIdx.x; b < nseg; b += gridDim.x) {
+            __syncthreads();
+            sp_gather_seg(items[i].p, b);
+        }
+    }
+}
+
+__global__ void k_batch_summary(const McBatchItem *items, const McBatchHdr *hdr, McBatchSummary *out) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < hdr->n; i += (int64_t) gridDim.x * blockDim.x) {
+        const McParams &p = items[i].p;
+        McBatchSummary s;
+        s.err = *p.err;
+        s.nv = p.totals[0];
+        s.nf = p.totals[1];
+        s.need = p.sp->need;
+        s.n_cand = p.sp->n_cand;
+        s.n_hole = p.sp->n_hole;
+        out[i] = s;
     }
 }
 
@@ -921,7 +1089,50 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(4)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sp_gather, dim3(1), dim3(1024), 0, stream, p);
+    hipLaunchKernelGGL(k_sp_gather, dim3(p.seg.nseg), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr, const McBatchHdr &h,
+                                 McBatchSummary *summary, hipStream_t stream) {
+    hipError_t e = mc_upload_luts();
+    if (e != hipSuccess) return e;
+    const unsigned rb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.n * 4 * kSpSeg + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_reset_batch, dim3(rb), dim3(256), 0, stream, items, hdr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const unsigned cb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.nwords + 63) / 64, 8192));
+    hipLaunchKernelGGL(k_classify_batch, dim3(cb), dim3(256), 0, stream, items, hdr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const unsigned nb = (unsigned) std::max<int64_t>(1, std::min<int64_t>(h.nblk, 8192));
+    hipLaunchKernelGGL(k_count_batch, dim3(nb), dim3(256), 0, stream, items, hdr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const unsigned sb = (unsigned) std::max<int64_t>(1, std::min<int64_t>(h.n, 256));
+    hipLaunchKernelGGL(k_scan_top_batch, dim3(sb), dim3(1024), 0, stream, items, hdr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch_summary, dim3((unsigned) ((h.n + 255) / 256)), dim3(256), 0, stream, items, hdr,
+                       summary);
+    return hipGetLastError();
+}
+
+hipError_t mc_batch_emit_launch(int cfg, const McBatchItem *items, const McBatchHdr *hdr, const McBatchHdr &h,
+                                McBatchSummary *summary, hipStream_t stream) {
+    hipError_t e = hipSuccess;
+    const unsigned tb = (unsigned) std::max<int64_t>(1, std::min<int64_t>(h.ntiles, 4096));
+    switch (cfg) {
+#define X(id, tx, ty, nzc, nt)                                                                           \
+    case id:                                                                                             \
+        hipLaunchKernelGGL((k_emit_batch<tx, ty, nzc, nt>), dim3(tb), dim3(nt), 0, stream, items, hdr);  \
+        break;
+        ISO_MC_CONFIGS(X)
+#undef X
+    default:
+        return hipErrorInvalidValue;
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sp_gather_batch, dim3(kSpSeg), dim3(256), 0, stream, items, hdr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch_summary, dim3((unsigned) ((h.n + 255) / 256)), dim3(256), 0, stream, items, hdr,
+                       summary);
     return hipGetLastError();
 }
 
