@@ -105,6 +105,20 @@ class UniformGrid(Grid):
                                                   self.default_value, _lib.stream_ptr()),
                        "UniformGrid")
 
+    @classmethod
+    def wrap(cls, values: torch.Tensor, aabb_min=(-1.0, -1.0, -1.0), aabb_max=(1.0, 1.0, 1.0)):
+        """A grid over an existing (X, Y, Z) float32 CUDA tensor, without a
+        copy (the grid borrows the tensor)."""
+        check_cuda_f32(values, ndim=3, name="values")
+        g = cls.__new__(cls)
+        g.shape = [int(s) for s in values.shape]
+        g.aabb_min = _vec3(aabb_min, "aabb_min")
+        g.aabb_max = _vec3(aabb_max, "aabb_max")
+        g.default_value = float(np.float32(FLT_MAX))
+        g.device = values.device
+        g.values = values if values.is_contiguous() else values.contiguous()
+        return g
+
     # -- counts -----------------------------------------------------------
     def get_num_cells(self) -> int:
         X, Y, Z = self.shape

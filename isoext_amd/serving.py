@@ -1,0 +1,265 @@
+"""Continuous-batching extraction server on one GPU.
+
+Requests (marching cubes or dual contouring on a dense grid already in HBM)
+arrive at any time.  A request is *admitted* when its estimated device
+footprint (mask workspace, count arrays, outputs) fits in the HBM budget
+that in-flight requests leave free; admitted marching-cubes requests are
+grouped by whichever worker becomes free next into one ragged batch
+(``isoext_amd.batch.mc_batch``: one launch per stage for the whole batch),
+dual-contouring requests run one by one.  Workers own separate HIP streams,
+so batches overlap on the device; a request that arrives while others run is
+picked up by the next free worker (continuous batching, no fixed batch
+boundaries).  Each result is identical to the plain per-call path on the
+same request (tests/test_serving_gpu.py).
+"""
+
+from __future__ import annotations
+
+import collections
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+
+@dataclass
+class Request:
+    rid: int
+    values: torch.Tensor            # (X, Y, Z) float32 on the server's device
+    kind: str = "mc"                # "mc" or "dc"
+    level: float = 0.0
+    method: int = 0                 # marching cubes: 0 nagae, 1 lorensen
+    arrival: float = 0.0            # seconds after the stream start
+    # filled in by the server
+    t_admit: float = -1.0
+    t_done: float = -1.0
+    result: tuple | None = None
+    error: str | None = None
+    done: threading.Event = field(default_factory=threading.Event)
+
+    @property
+    def points(self) -> int:
+        return int(self.values.numel())
+
+
+def footprint(req: Request) -> int:
+    """Estimated device bytes a request holds while in flight: the inside
+    masks and count arrays (about N/8 + N/16 bytes), and outputs sized for a
+    surface crossing every x-plane (12 B per vertex + 24 B per face, at most
+    3 vertices per cross-section cell)."""
+    X, Y, Z = (int(s) for s in req.values.shape)
+    n = X * Y * Z
+    surface = 3 * max(X * Y, Y * Z, X * Z)
+    out = surface * (12 + 24)
+    if req.kind == "dc":
+        out *= 2
+    return n // 8 + n // 16 + out + (1 << 20)
+
+
+class Server:
+    def __init__(self, device=0, hbm_budget: float = 200e9, max_batch: int = 32,
+                 max_batch_points: int = 1 << 28, workers: int = 4, cfg: int | None = None):
+        self.device = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
+        self.budget = int(hbm_budget)
+        self.max_batch = max_batch
+        self.max_batch_points = max_batch_points
+        self.cfg = cfg
+        self._lock = threading.Condition()
+        self._pending: collections.deque[Request] = collections.deque()   # arrived, not admitted
+        self._mc: collections.deque[Request] = collections.deque()         # admitted, not started
+        self._dc: collections.deque[Request] = collections.deque()
+        self._in_use = 0
+        self._seq = 0
+        self._stop = False
+        self._t0 = time.perf_counter()
+        self.stats = {"batches": 0, "batched_requests": 0, "dc_requests": 0, "max_in_use": 0}
+        self._workers = [threading.Thread(target=self._work, args=(i,), daemon=True) for i in range(workers)]
+        for w in self._workers:
+            w.start()
+
+    # ---- client side -----------------------------------------------------------
+    def now(self) -> float:
+        return time.perf_counter() - self._t0
+
+    def reset_clock(self):
+        self._t0 = time.perf_counter()
+
+    def submit(self, req: Request):
+        with self._lock:
+            self._pending.append(req)
+            self._admit_locked()
+            self._lock.notify_all()
+
+    def close(self):
+        with self._lock:
+            self._stop = True
+            self._lock.notify_all()
+        for w in self._workers:
+            w.join()
+
+    # ---- admission ---------------------------------------------------------------
+    def _admit_locked(self):
+        while self._pending:
+            r = self._pending[0]
+            need = footprint(r)
+            if self._in_use + need > self.budget and self._in_use > 0:
+                break   # wait for in-flight requests to free memory (FIFO, no overtaking)
+            self._pending.popleft()
+            r._bytes = need
+            self._in_use += need
+            self.stats["max_in_use"] = max(self.stats["max_in_use"], self._in_use)
+            r.t_admit = self.now()
+            r._seq = self._seq
+            self._seq += 1
+            (self._dc if r.kind == "dc" else self._mc).append(r)
+
+    def _release(self, reqs):
+        with self._lock:
+            for r in reqs:
+                self._in_use -= r._bytes
+            self._admit_locked()
+            self._lock.notify_all()
+
+    # ---- workers -----------------------------------------------------------------
+    def _take_locked(self):
+        """Next unit of work: the oldest admitted request decides its kind;
+        a DC request runs alone, MC requests go as one batch of every
+        admitted MC request (in admission order, up to the size limits)."""
+        if not self._mc and not self._dc:
+            return None
+        if self._dc and (not self._mc or self._dc[0]._seq < self._mc[0]._seq):
+            return [self._dc.popleft()]
+        batch, pts = [], 0
+        while self._mc and len(batch) < self.max_batch:
+            r = self._mc[0]
+            if batch and pts + r.points > self.max_batch_points:
+                break
+            batch.append(self._mc.popleft())
+            pts += r.points
+        return batch
+
+    def _work(self, wid: int):
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(device=self.device)
+        while True:
+            with self._lock:
+                while not self._mc and not self._dc and not self._stop:
+                    self._lock.wait()
+                if self._stop and not self._mc and not self._dc:
+                    return
+                work = self._take_locked()
+            if not work:
+                continue
+            try:
+                with torch.cuda.stream(stream):
+                    if work[0].kind == "dc":
+                        self._run_dc(work[0])
+                    else:
+                        self._run_mc(work)
+                stream.synchronize()
+            except Exception as e:  # reported per request, the server keeps running
+                for r in work:
+                    r.error = repr(e)
+            t = self.now()
+            for r in work:
+                r.t_done = t
+                r.done.set()
+            self._release(work)
+
+    def _run_mc(self, reqs):
+        from .batch import GridJob, mc_batch
+        jobs = [GridJob(r.values, r.level, r.method) for r in reqs]
+        out = mc_batch(jobs, cfg=self.cfg, ws_kind="serving")
+        for r, (v, f) in zip(reqs, out):
+            r.result = (v, f) if f.shape[0] > 0 else (None, None)
+        with self._lock:
+            self.stats["batches"] += 1
+            self.stats["batched_requests"] += len(reqs)
+
+    def _run_dc(self, r):
+        from .dc import dual_contouring
+        from .grid import UniformGrid
+        r.result = dual_contouring(UniformGrid.wrap(r.values), level=r.level)
+        with self._lock:
+            self.stats["dc_requests"] += 1
+
+
+# ---- plain per-call path (the baseline) --------------------------------------------
+
+def run_plain(r: Request):
+    """One request through the plain API (one extraction per call)."""
+    from .dc import dual_contouring
+    from .grid import UniformGrid
+    from .mc import marching_cubes
+    g = UniformGrid.wrap(r.values)
+    if r.kind == "dc":
+        return dual_contouring(g, level=r.level)
+    return marching_cubes(g, r.level, ["nagae", "lorensen"][r.method])
+
+
+# ---- request streams ---------------------------------------------------------------
+
+def make_pool(n_grids: int, seed: int = 0, side_lo: int = 32, side_hi: int = 512, device="cuda",
+              max_points: int | None = None):
+    """Distinct synthetic SDF grids: cubic sides log-uniform in [side_lo,
+    side_hi], each the union (min) of 1-64 random spheres (centres
+    U[-0.8, 0.8]^3, radii U[0.05, 0.25], as in the 1024^3 benchmark field)."""
+    from .workloads import spheres_field
+    rng = np.random.default_rng(seed)
+    pool = []
+    for _ in range(n_grids):
+        side = int(round(np.exp(rng.uniform(np.log(side_lo), np.log(side_hi)))))
+        if max_points is not None:
+            side = min(side, int(max_points ** (1 / 3)))
+        k = int(rng.integers(1, 65))
+        c = rng.uniform(-0.8, 0.8, size=(k, 3)).astype(np.float32)
+        r = rng.uniform(0.05, 0.25, size=k).astype(np.float32)
+        pool.append(spheres_field((side, side, side), c, r, device=device))
+    return pool
+
+
+def make_stream(pool, n: int, rate: float, seed: int = 0, dc_frac: float = 0.2):
+    """n requests, Poisson arrivals at `rate` per second (rate <= 0: all at
+    t = 0), grids drawn from the pool, dc_frac of them dual contouring."""
+    rng = np.random.default_rng(seed + 1)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n)) if rate > 0 else np.zeros(n)
+    reqs = []
+    for i in range(n):
+        kind = "dc" if rng.random() < dc_frac else "mc"
+        reqs.append(Request(rid=i, values=pool[int(rng.integers(len(pool)))], kind=kind,
+                            method=int(rng.integers(0, 2)) if kind == "mc" else 0, arrival=float(t[i])))
+    return reqs
+
+
+def drive(server: Server, reqs):
+    """Submit requests at their arrival times, wait for all; returns
+    (wall seconds, latencies in seconds)."""
+    server.reset_clock()
+    t0 = time.perf_counter()
+    for r in reqs:
+        dt = r.arrival - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        server.submit(r)
+    for r in reqs:
+        r.done.wait()
+    wall = time.perf_counter() - t0
+    lat = np.array([r.t_done - r.arrival for r in reqs])
+    return wall, lat
+
+
+def drive_plain(reqs):
+    """The per-call baseline on the same stream: one request at a time, in
+    arrival order, each started no earlier than its arrival."""
+    t0 = time.perf_counter()
+    lat = []
+    for r in reqs:
+        dt = r.arrival - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        r.result = run_plain(r)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0 - r.arrival)
+    return time.perf_counter() - t0, np.array(lat)

@@ -1,10 +1,10 @@
 """Device workspaces reused across calls.
 
 Each (device, stream, kind) gets one zero-initialised byte buffer that grows
-on demand; kernels keep their cross-workgroup state (tickets, look-back
-status, published boundary ids tagged with a per-call epoch) in it, so a
-buffer must be zero-filled once when it is (re)allocated and then reused.
-Calls sharing one buffer are serialised by their stream.
+on demand; the kernels keep their intermediate state in it (inside masks,
+count arrays, special-vertex lists, scan partials) and reset what must start
+at zero themselves.  Calls sharing one buffer are serialised by their
+stream.
 """
 
 from __future__ import annotations

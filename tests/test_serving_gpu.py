@@ -1,0 +1,57 @@
+"""The continuous-batching server: every request's result equals the plain
+per-call path on the same request (alone vs in the mix), under HBM
+admission pressure too."""
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def canon(res):
+    v, f = res
+    if v is None:
+        return None
+    return oracle.canonical(v.cpu().numpy(), f.cpu().numpy())
+
+
+def same(a, b):
+    if a is None or b is None:
+        return a is None and b is None
+    return np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def run(budget, workers, n=48, seed=4):
+    import torch
+
+    from isoext_amd import serving
+    pool = serving.make_pool(10, seed=seed, side_lo=16, side_hi=96)
+    torch.cuda.synchronize()
+    reqs = serving.make_stream(pool, n, rate=2000.0, seed=seed, dc_frac=0.25)
+    srv = serving.Server(hbm_budget=budget, max_batch=8, workers=workers)
+    try:
+        wall, lat = serving.drive(srv, reqs)
+    finally:
+        srv.close()
+    assert all(r.error is None for r in reqs), [r.error for r in reqs if r.error][:3]
+    for r in reqs:
+        assert r.t_done >= r.t_admit >= 0
+        assert same(canon(r.result), canon(serving.run_plain(r))), r.rid
+    return srv, reqs, lat
+
+
+def test_mix_equals_alone(gpu):
+    srv, reqs, lat = run(200e9, workers=2)
+    assert srv.stats["batches"] >= 1 and srv.stats["dc_requests"] >= 1
+    assert srv.stats["batched_requests"] + srv.stats["dc_requests"] == len(reqs)
+    assert np.all(lat >= 0)
+
+
+def test_admission_under_pressure(gpu):
+    from isoext_amd import serving
+    # a budget that holds about one large request at a time
+    srv, reqs, lat = run(48e6, workers=2, n=24, seed=9)
+    biggest = max(serving.footprint(r) for r in reqs)
+    assert srv.stats["max_in_use"] <= max(48e6, biggest)
