@@ -121,6 +121,44 @@ bool grid_well_separated(const int64_t shape[3], const float *amin, const float 
     return true;
 }
 
+// Mask-pipeline parameters of the intersection pass on a dense grid (the
+// scratch holds the masks, count arrays and totals; cap_* bound the writes).
+iso::McParams its_params(const isoext_grid *g, float level, char *ws, int64_t cap_cells, int64_t cap_pts) {
+    McLayout L;
+    mc_layout(g->shape, iso::kItsCfg, 1, L);
+    iso::McParams p{};
+    p.values = g->values;
+    p.X = g->shape[0];
+    p.Y = g->shape[1];
+    p.Z = g->shape[2];
+    for (int a = 0; a < 3; ++a) {
+        p.amin[a] = g->aabb_min[a];
+        p.amax[a] = g->aabb_max[a];
+    }
+    p.level = level;
+    p.its = 1;
+    p.vec4 = (p.Z % 4 == 0 && ((uintptr_t) g->values & 15) == 0) ? 1 : 0;
+    p.W = L.W;
+    p.nzc = L.nzc;
+    p.S = L.S;
+    p.NI = L.NI;
+    p.NJ = L.NJ;
+    p.NK = L.NK;
+    p.bits = (uint64_t *) (ws + L.o_bits);
+    p.cnt_v = (uint32_t *) (ws + L.o_cnt_v);
+    p.cnt_f = (uint32_t *) (ws + L.o_cnt_f);
+    p.bsum_v = (uint32_t *) (ws + L.o_bsum_v);
+    p.bsum_f = (uint32_t *) (ws + L.o_bsum_f);
+    p.blk_v = (int64_t *) (ws + L.o_blk_v);
+    p.blk_f = (int64_t *) (ws + L.o_blk_f);
+    p.nblk = L.nblk;
+    p.cap_v = cap_cells;
+    p.cap_f = cap_pts;
+    p.err = (int *) (ws + kOffErr);
+    p.totals = (int64_t *) (ws + kOffTotals);
+    return p;
+}
+
 iso::GridDesc to_desc(const isoext_grid *g) {
     iso::GridDesc d;
     d.values = g->values;
@@ -158,7 +196,7 @@ bool grid_has_cells(const isoext_grid *g) {
 }
 
 struct DcLayout {
-    size_t o_dual, o_cond, o_cnt, o_used, o_bsum, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, total;
+    size_t o_dual, o_cond, o_cnt, o_used, o_bsum, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, o_ak, o_av, total;
     int64_t hslots;
 };
 
@@ -185,6 +223,8 @@ DcLayout dc_layout(int64_t ncells, int64_t npts) {
     L.o_hm = take(sizeof(int32_t) * hs);
     L.o_rep = take(sizeof(int32_t) * std::max<int64_t>(ncells, 1));
     L.o_fin = take(sizeof(int32_t) * std::max<int64_t>(ncells, 1));
+    L.o_ak = take(sizeof(int64_t) * hs);
+    L.o_av = take(sizeof(int32_t) * hs);
     L.total = o;
     return L;
 }
@@ -390,6 +430,11 @@ extern "C" {
 
 int64_t isoext_its_scratch_bytes(const isoext_grid *g) {
     if (!grid_has_cells(g)) return 64;
+    if (!g->sparse_cell_ids) {
+        McLayout L;
+        mc_layout(g->shape, iso::kItsCfg, 1, L);
+        return (int64_t) L.total;
+    }
     iso::GridDesc d = to_desc(g);
     return (int64_t) (sizeof(int64_t) * 2 * (iso::its_scratch_pairs(d) + 4));
 }
@@ -399,14 +444,20 @@ int isoext_its_count(const isoext_grid *g, float level, void *scratch, int64_t s
     counts[0] = counts[1] = 0;
     if (!grid_has_cells(g)) return ISOEXT_OK;
     if (scratch_bytes < isoext_its_scratch_bytes(g)) return fail(ISOEXT_E_WORKSPACE, "intersection scratch too small");
-    iso::GridDesc d = to_desc(g);
     hipStream_t st = (hipStream_t) stream;
-    int64_t nb = iso::its_scratch_pairs(d);
-    int64_t *bsum = (int64_t *) scratch;
-    int64_t *tot = bsum + 2 * nb;
-    HIP_CHECK(iso::its_count(d, level, bsum, tot, st), "intersection count");
     int64_t h[2];
-    HIP_CHECK(hipMemcpyAsync(h, tot, sizeof(h), hipMemcpyDeviceToHost, st), "read counts");
+    if (!g->sparse_cell_ids) {
+        iso::McParams p = its_params(g, level, (char *) scratch, 0, 0);
+        HIP_CHECK(iso::its_dense_count_launch(p, st), "intersection count");
+        HIP_CHECK(hipMemcpyAsync(h, p.totals, sizeof(h), hipMemcpyDeviceToHost, st), "read counts");
+    } else {
+        iso::GridDesc d = to_desc(g);
+        int64_t nb = iso::its_scratch_pairs(d);
+        int64_t *bsum = (int64_t *) scratch;
+        int64_t *tot = bsum + 2 * nb;
+        HIP_CHECK(iso::its_count(d, level, bsum, tot, st), "intersection count");
+        HIP_CHECK(hipMemcpyAsync(h, tot, sizeof(h), hipMemcpyDeviceToHost, st), "read counts");
+    }
     HIP_CHECK(hipStreamSynchronize(st), "intersection count");
     counts[0] = h[0];
     counts[1] = h[1];
@@ -418,9 +469,13 @@ int isoext_its_emit(const isoext_grid *g, float level, const void *scratch, int6
                     uint8_t *edge_local, uint8_t *is_out, int32_t *entry_cell, void *stream) {
     hipStream_t st = (hipStream_t) stream;
     if (grid_has_cells(g) && ncells > 0) {
-        iso::GridDesc d = to_desc(g);
         iso::ItsOut o{cell_indices, cell_dense, cell_offsets, points, normals, edge_local, is_out, entry_cell};
-        HIP_CHECK(iso::its_emit(d, level, (const int64_t *) scratch, o, st), "intersection emit");
+        if (!g->sparse_cell_ids) {
+            iso::McParams p = its_params(g, level, (char *) scratch, ncells, npts);
+            HIP_CHECK(iso::its_dense_emit_launch(p, o, st), "intersection emit");
+        } else {
+            HIP_CHECK(iso::its_emit(to_desc(g), level, (const int64_t *) scratch, o, st), "intersection emit");
+        }
     }
     HIP_CHECK(hipMemcpyAsync(cell_offsets + ncells, &npts, sizeof(int64_t), hipMemcpyHostToDevice, st),
               "write offsets");
@@ -459,6 +514,9 @@ int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_
     w.hslots = (int32_t *) (ws + L.o_hs);
     w.slot_min = (int32_t *) (ws + L.o_hm);
     w.hmask = L.hslots - 1;
+    w.akey = (int64_t *) (ws + L.o_ak);
+    w.aval = (int32_t *) (ws + L.o_av);
+    w.amask = L.hslots - 1;
     w.rep_slot = (int32_t *) (ws + L.o_rep);
     w.final_id = (int32_t *) (ws + L.o_fin);
     w.out_v = out_v;

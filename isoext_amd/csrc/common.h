@@ -102,4 +102,47 @@ ISO_HD uint64_t mix64(uint64_t x) {
     return x;
 }
 
+// trilinear_interp (src/its.cu:170-184), restated op for op
+__device__ __forceinline__ float trilinear(float tx, float ty, float tz, const float *c) {
+    float a = 1.0f - tz;
+    float c00 = c[0] * a + c[1] * tz;
+    float c01 = c[2] * a + c[3] * tz;
+    float c10 = c[4] * a + c[5] * tz;
+    float c11 = c[6] * a + c[7] * tz;
+    float b = 1.0f - ty;
+    float c0 = c00 * b + c01 * ty;
+    float c1 = c10 * b + c11 * ty;
+    float d = 1.0f - tx;
+    return c0 * d + c1 * tx;
+}
+
+// compute_normals_op (src/its.cu:186-268) for one point in a cell.
+__device__ inline void cell_normal(const float lo[3], const float hi[3], const float *cv, float px, float py,
+                            float pz, float &nx, float &ny, float &nz) {
+    float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
+    float tx = (px - lo[0]) / sx, ty = (py - lo[1]) / sy, tz = (pz - lo[2]) / sz;
+    tx = fmaxf(0.01f, fminf(0.99f, tx));
+    ty = fmaxf(0.01f, fminf(0.99f, ty));
+    tz = fmaxf(0.01f, fminf(0.99f, tz));
+    const float eps = 0.02f;
+    float pxx = fminf(tx + eps, 0.99f), mxx = fmaxf(tx - eps, 0.01f);
+    float pyy = fminf(ty + eps, 0.99f), myy = fmaxf(ty - eps, 0.01f);
+    float pzz = fminf(tz + eps, 0.99f), mzz = fmaxf(tz - eps, 0.01f);
+    float dx = (trilinear(pxx, ty, tz, cv) - trilinear(mxx, ty, tz, cv)) / ((pxx - mxx) * sx);
+    float dy = (trilinear(tx, pyy, tz, cv) - trilinear(tx, myy, tz, cv)) / ((pyy - myy) * sy);
+    float dz = (trilinear(tx, ty, pzz, cv) - trilinear(tx, ty, mzz, cv)) / ((pzz - mzz) * sz);
+    float d2 = dx * dx + dy * dy;
+    d2 = d2 + dz * dz;
+    float len = sqrtf(d2);
+    if (len > 1e-8f) {
+        nx = dx / len;
+        ny = dy / len;
+        nz = dz / len;
+    } else {
+        nx = 0.0f;
+        ny = 0.0f;
+        nz = 1.0f;
+    }
+}
+
 }   // namespace iso

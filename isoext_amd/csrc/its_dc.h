@@ -50,6 +50,9 @@ struct DcScratch {
     int32_t *hslots, *slot_min;
     int64_t hmask;
     int32_t *rep_slot, *final_id;
+    int64_t *akey;      // active-cell hash: dense cell id -> active index
+    int32_t *aval;
+    int64_t amask;
     float *out_v;
     int32_t *out_f;
     int64_t nquads;

@@ -40,6 +40,14 @@ __device__ __forceinline__ int64_t cell_dense_id(const CellSrc &s, int64_t i) {
 
 __device__ __forceinline__ void cell_xyz(const CellSrc &s, int64_t id, int64_t &x, int64_t &y, int64_t &z) {
     int64_t cz = s.Z - 1, cy = s.Y - 1;
+    if ((uint64_t) id <= 0xffffffffull) {   // 32-bit division when the id fits
+        const uint32_t i32 = (uint32_t) id, cz32 = (uint32_t) cz, cy32 = (uint32_t) cy;
+        const uint32_t r = i32 / cz32;
+        z = i32 - r * cz32;
+        y = r % cy32;
+        x = r / cy32;
+        return;
+    }
     z = id % cz;
     int64_t r = id / cz;
     y = r % cy;
@@ -118,20 +126,52 @@ __device__ __forceinline__ void block_scan2(int64_t a, int64_t b, int64_t &ea, i
 
 // ---- intersections ----------------------------------------------------------
 
-__global__ void k_its_count(CellSrc s, float level, int64_t *bsum) {
-    int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x;
-    int64_t na = 0, ni = 0;
-    if (i < s.n) {
-        int64_t id = cell_dense_id(s, i), x, y, z;
-        cell_xyz(s, id, x, y, z);
-        float cv[8];
-        cell_values(s, i, x, y, z, cv);
-        int cs = case_bits(cv, level);
-        if (cs != 0 && cs != 255) {
-            na = 1;
-            ni = __popc(crossing12(cs));
+// Cells per thread in the intersection passes (consecutive ids, so a
+// thread's entries are contiguous in the reference's entry order).
+constexpr int kCpt = 8;
+
+// Next cell in linear id order (dense grids): z fastest, then y, then x.
+__device__ __forceinline__ void next_cell(const CellSrc &s, int64_t &x, int64_t &y, int64_t &z) {
+    if (++z == s.Z - 1) {
+        z = 0;
+        if (++y == s.Y - 1) {
+            y = 0;
+            ++x;
         }
     }
+}
+
+// Cases of a thread's kCpt consecutive cells (8 bits each; 0 past the end),
+// with the active-cell and intersection counts.  Dense grids decode the
+// first cell's coordinates once and step; sparse grids decode each id.
+__device__ __forceinline__ uint64_t thread_cases(const CellSrc &s, int64_t i0, float level, int64_t &na,
+                                                 int64_t &ni) {
+    uint64_t cases = 0;
+    na = ni = 0;
+    if (i0 >= s.n) return 0;
+    int64_t x, y, z;
+    cell_xyz(s, cell_dense_id(s, i0), x, y, z);
+    for (int j = 0; j < kCpt; ++j) {
+        const int64_t i = i0 + j;
+        if (i >= s.n) break;
+        if (s.ids && j > 0) cell_xyz(s, s.ids[i], x, y, z);
+        float cv[8];
+        cell_values(s, i, x, y, z, cv);
+        const int cs = case_bits(cv, level);
+        cases |= (uint64_t) cs << (8 * j);
+        if (cs != 0 && cs != 255) {
+            na += 1;
+            ni += __popc(crossing12(cs));
+        }
+        if (!s.ids) next_cell(s, x, y, z);
+    }
+    return cases;
+}
+
+__global__ void k_its_count(CellSrc s, float level, int64_t *bsum) {
+    const int64_t i0 = ((int64_t) blockIdx.x * kBlock + threadIdx.x) * kCpt;
+    int64_t na, ni;
+    (void) thread_cases(s, i0, level, na, ni);
     int64_t ea, ei, ta, ti;
     block_scan2(na, ni, ea, ei, ta, ti);
     if (threadIdx.x == 0) {
@@ -140,103 +180,56 @@ __global__ void k_its_count(CellSrc s, float level, int64_t *bsum) {
     }
 }
 
-// trilinear_interp (src/its.cu:170-184), restated op for op
-__device__ __forceinline__ float trilinear(float tx, float ty, float tz, const float *c) {
-    float a = 1.0f - tz;
-    float c00 = c[0] * a + c[1] * tz;
-    float c01 = c[2] * a + c[3] * tz;
-    float c10 = c[4] * a + c[5] * tz;
-    float c11 = c[6] * a + c[7] * tz;
-    float b = 1.0f - ty;
-    float c0 = c00 * b + c01 * ty;
-    float c1 = c10 * b + c11 * ty;
-    float d = 1.0f - tx;
-    return c0 * d + c1 * tx;
-}
-
-// compute_normals_op (src/its.cu:186-268) for one point in a cell.
-__device__ void cell_normal(const float lo[3], const float hi[3], const float *cv, float px, float py,
-                            float pz, float &nx, float &ny, float &nz) {
-    float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
-    float tx = (px - lo[0]) / sx, ty = (py - lo[1]) / sy, tz = (pz - lo[2]) / sz;
-    tx = fmaxf(0.01f, fminf(0.99f, tx));
-    ty = fmaxf(0.01f, fminf(0.99f, ty));
-    tz = fmaxf(0.01f, fminf(0.99f, tz));
-    const float eps = 0.02f;
-    float pxx = fminf(tx + eps, 0.99f), mxx = fmaxf(tx - eps, 0.01f);
-    float pyy = fminf(ty + eps, 0.99f), myy = fmaxf(ty - eps, 0.01f);
-    float pzz = fminf(tz + eps, 0.99f), mzz = fmaxf(tz - eps, 0.01f);
-    float dx = (trilinear(pxx, ty, tz, cv) - trilinear(mxx, ty, tz, cv)) / ((pxx - mxx) * sx);
-    float dy = (trilinear(tx, pyy, tz, cv) - trilinear(tx, myy, tz, cv)) / ((pyy - myy) * sy);
-    float dz = (trilinear(tx, ty, pzz, cv) - trilinear(tx, ty, mzz, cv)) / ((pzz - mzz) * sz);
-    float d2 = dx * dx + dy * dy;
-    d2 = d2 + dz * dz;
-    float len = sqrtf(d2);
-    if (len > 1e-8f) {
-        nx = dx / len;
-        ny = dy / len;
-        nz = dz / len;
-    } else {
-        nx = 0.0f;
-        ny = 0.0f;
-        nz = 1.0f;
-    }
-}
-
 __global__ void k_its_emit(CellSrc s, float level, const int64_t *bsum_pre, ItsOut o) {
-    int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x;
-    int64_t na = 0, ni = 0;
-    float cv[8];
-    int64_t x = 0, y = 0, z = 0, id = 0;
-    int cs = 0;
-    if (i < s.n) {
-        id = cell_dense_id(s, i);
-        cell_xyz(s, id, x, y, z);
-        cell_values(s, i, x, y, z, cv);
-        cs = case_bits(cv, level);
-        if (cs != 0 && cs != 255) {
-            na = 1;
-            ni = __popc(crossing12(cs));
-        }
-    }
+    const int64_t i0 = ((int64_t) blockIdx.x * kBlock + threadIdx.x) * kCpt;
+    int64_t na, ni;
+    const uint64_t cases = thread_cases(s, i0, level, na, ni);
     int64_t ea, ei, ta, ti;
     block_scan2(na, ni, ea, ei, ta, ti);
     if (!na) return;
-    const int64_t a = bsum_pre[2 * blockIdx.x] + ea;
+    int64_t a = bsum_pre[2 * blockIdx.x] + ea;
     int64_t e = bsum_pre[2 * blockIdx.x + 1] + ei;
-    o.cell_indices[a] = s.ids ? i : id;
-    o.cell_dense[a] = id;
-    o.cell_offsets[a] = e;
-    float lo[3] = {grid_coord(x, s.X, s.amin[0], s.amax[0]), grid_coord(y, s.Y, s.amin[1], s.amax[1]),
-                   grid_coord(z, s.Z, s.amin[2], s.amax[2])};
-    float hi[3] = {grid_coord(x + 1, s.X, s.amin[0], s.amax[0]),
-                   grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]),
-                   grid_coord(z + 1, s.Z, s.amin[2], s.amax[2])};
-    const int st = crossing12(cs);
-    for (int k = 0; k < 12; ++k) {
-        if (!((st >> k) & 1)) continue;
-        int ca = edge_a(k), cb = edge_b(k);
-        float t = edge_t(cv[ca], cv[cb], level);
-        float ax = (ca >> 2) ? hi[0] : lo[0], ay = ((ca >> 1) & 1) ? hi[1] : lo[1], az = (ca & 1) ? hi[2] : lo[2];
-        float bx = (cb >> 2) ? hi[0] : lo[0], by = ((cb >> 1) & 1) ? hi[1] : lo[1], bz = (cb & 1) ? hi[2] : lo[2];
-        float px = lerp1(t, ax, bx), py = lerp1(t, ay, by), pz = lerp1(t, az, bz);
-        o.points[3 * e + 0] = px;
-        o.points[3 * e + 1] = py;
-        o.points[3 * e + 2] = pz;
-        o.edge_local[e] = (uint8_t) k;
-        o.is_out[e] = cv[ca] <= cv[cb] ? 1 : 0;
-        o.entry_cell[e] = (int32_t) a;
-        if (o.normals) {
-            float nx, ny, nz;
-            cell_normal(lo, hi, cv, px, py, pz, nx, ny, nz);
-            o.normals[3 * e + 0] = nx;
-            o.normals[3 * e + 1] = ny;
-            o.normals[3 * e + 2] = nz;
+    for (int j = 0; j < kCpt; ++j) {
+        const int64_t i = i0 + j;
+        if (i >= s.n) break;
+        const int cs = (int) ((cases >> (8 * j)) & 255);
+        if (cs == 0 || cs == 255) continue;
+        int64_t id = cell_dense_id(s, i), x, y, z;
+        cell_xyz(s, id, x, y, z);
+        float cv[8];
+        cell_values(s, i, x, y, z, cv);
+        o.cell_indices[a] = s.ids ? i : id;
+        o.cell_dense[a] = id;
+        o.cell_offsets[a] = e;
+        float lo[3] = {grid_coord(x, s.X, s.amin[0], s.amax[0]), grid_coord(y, s.Y, s.amin[1], s.amax[1]),
+                       grid_coord(z, s.Z, s.amin[2], s.amax[2])};
+        float hi[3] = {grid_coord(x + 1, s.X, s.amin[0], s.amax[0]),
+                       grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]),
+                       grid_coord(z + 1, s.Z, s.amin[2], s.amax[2])};
+        const int st = crossing12(cs);
+        for (int k = 0; k < 12; ++k) {
+            if (!((st >> k) & 1)) continue;
+            int ca = edge_a(k), cb = edge_b(k);
+            float t = edge_t(cv[ca], cv[cb], level);
+            float ax = (ca >> 2) ? hi[0] : lo[0], ay = ((ca >> 1) & 1) ? hi[1] : lo[1], az = (ca & 1) ? hi[2] : lo[2];
+            float bx = (cb >> 2) ? hi[0] : lo[0], by = ((cb >> 1) & 1) ? hi[1] : lo[1], bz = (cb & 1) ? hi[2] : lo[2];
+            float px = lerp1(t, ax, bx), py = lerp1(t, ay, by), pz = lerp1(t, az, bz);
+            o.points[3 * e + 0] = px;
+            o.points[3 * e + 1] = py;
+            o.points[3 * e + 2] = pz;
+            o.edge_local[e] = (uint8_t) k;
+            o.is_out[e] = cv[ca] <= cv[cb] ? 1 : 0;
+            o.entry_cell[e] = (int32_t) a;
+            if (o.normals) {
+                float nx, ny, nz;
+                cell_normal(lo, hi, cv, px, py, pz, nx, ny, nz);
+                o.normals[3 * e + 0] = nx;
+                o.normals[3 * e + 1] = ny;
+                o.normals[3 * e + 2] = nz;
+            }
+            ++e;
         }
-        ++e;
-    }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-        // nothing: the host writes cell_offsets[ncells]
+        ++a;
     }
 }
 
@@ -425,24 +418,46 @@ __device__ __forceinline__ int64_t lower_bound64(const int64_t *a, int64_t n, in
     return lo;
 }
 
-// Active index of the dense cell id `cid`, or -1.  Sparse grids first map the
-// dense id to its slot (slots are sorted by dense id).
-__device__ __forceinline__ int64_t active_of(const ItsIn &in, const CellSrc &s, int64_t cid) {
-    int64_t key = cid;
-    if (s.ids) {
-        int64_t slot = lower_bound64(s.ids, s.n, cid);
-        if (slot >= s.n || s.ids[slot] != cid) return -1;
-        key = slot;
+// Active-cell hash: dense cell id -> active index (open addressing, linear
+// probing, at most half full).  Replaces a binary search per neighbour.
+struct ActHash {
+    const int64_t *key;
+    const int32_t *val;
+    int64_t mask;
+};
+
+__global__ void k_act_insert(const int64_t *cell_dense, int64_t ncells, int64_t *akey, int32_t *aval,
+                             int64_t amask) {
+    const int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    if (a >= ncells) return;
+    const int64_t key = cell_dense[a];
+    int64_t h = (int64_t) (mix64((uint64_t) key) & (uint64_t) amask);
+    while (true) {
+        const unsigned long long prev =
+            atomicCAS((unsigned long long *) &akey[h], ~0ull, (unsigned long long) key);
+        if (prev == ~0ull) {
+            aval[h] = (int32_t) a;
+            return;
+        }
+        h = (h + 1) & amask;
     }
-    int64_t a = lower_bound64(in.cell_indices, in.ncells, key);
-    if (a >= in.ncells || in.cell_indices[a] != key) return -1;
-    return a;
+}
+
+// Active index of the dense cell id `cid`, or -1.
+__device__ __forceinline__ int64_t active_of(const ActHash &t, int64_t cid) {
+    int64_t h = (int64_t) (mix64((uint64_t) cid) & (uint64_t) t.mask);
+    while (true) {
+        const int64_t k = t.key[h];
+        if (k == cid) return t.val[h];
+        if (k == -1) return -1;
+        h = (h + 1) & t.mask;
+    }
 }
 
 // Quad of entry e, if e is the canonical copy of an interior crossing edge:
 // the copy held by the cell whose min corner owns the edge (local edges 0,
 // 3, 8).  These copies are in (p0, p1) order, the reference's quad order.
-__device__ bool entry_quad(const ItsIn &in, const CellSrc &s, int64_t e, int64_t q[4]) {
+__device__ bool entry_quad(const ItsIn &in, const CellSrc &s, const ActHash &ah, int64_t e, int64_t q[4]) {
     int k = in.edge_local[e];
     if (k != 0 && k != 3 && k != 8) return false;
     const int axis = edge_axis(k);
@@ -451,7 +466,7 @@ __device__ bool entry_quad(const ItsIn &in, const CellSrc &s, int64_t e, int64_t
     for (int j = 0; j < 4; ++j) {
         int64_t cx = x - kNb[axis][j][0], cy = y - kNb[axis][j][1], cz = z - kNb[axis][j][2];
         if (cx < 0 || cy < 0 || cz < 0 || cx >= s.X - 1 || cy >= s.Y - 1 || cz >= s.Z - 1) return false;
-        q[j] = active_of(in, s, (cx * (s.Y - 1) + cy) * (s.Z - 1) + cz);
+        q[j] = active_of(ah, (cx * (s.Y - 1) + cy) * (s.Z - 1) + cz);
         if (q[j] < 0) return false;
     }
     if (!in.is_out[e]) {
@@ -464,11 +479,11 @@ __device__ bool entry_quad(const ItsIn &in, const CellSrc &s, int64_t e, int64_t
     return true;
 }
 
-__global__ void k_quad_count(ItsIn in, CellSrc s, int64_t *bsum) {
+__global__ void k_quad_count(ItsIn in, CellSrc s, ActHash ah, int64_t *bsum) {
     int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t nq = 0;
     int64_t q[4];
-    if (e < in.npts && entry_quad(in, s, e, q)) nq = 1;
+    if (e < in.npts && entry_quad(in, s, ah, e, q)) nq = 1;
     int64_t eq, dummy, tq, td;
     block_scan2(nq, 0, eq, dummy, tq, td);
     if (threadIdx.x == 0) {
@@ -485,12 +500,12 @@ __device__ __forceinline__ float dist3(const float *a, const float *b) {
 }
 
 // get_triangles_op (src/dc.cu:101-157): split along the shorter diagonal.
-__global__ void k_quad_emit(ItsIn in, CellSrc s, const int64_t *bsum_pre, const float *dual_v,
+__global__ void k_quad_emit(ItsIn in, CellSrc s, ActHash ah, const int64_t *bsum_pre, const float *dual_v,
                             int32_t *tri_cells, uint8_t *used) {
     int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t nq = 0;
     int64_t q[4] = {0, 0, 0, 0};
-    if (e < in.npts && entry_quad(in, s, e, q)) nq = 1;
+    if (e < in.npts && entry_quad(in, s, ah, e, q)) nq = 1;
     int64_t eq, dummy, tq, td;
     block_scan2(nq, 0, eq, dummy, tq, td);
     if (!nq) return;
@@ -566,15 +581,19 @@ __global__ void k_keep_emit(const float *dv, const uint8_t *used, int64_t n, con
     out_v[3 * id + 2] = dv[3 * a + 2];
 }
 
-__global__ void k_remap_faces(const int32_t *tri_cells, int64_t n, const int32_t *rep_slot, const int32_t *slot_min,
-                              const int32_t *final_id, int32_t *out_f) {
+// Face corners: 6 per quad; the quad count is read from the device.
+__global__ void k_remap_faces(const int32_t *tri_cells, const int64_t *nquads, const int32_t *rep_slot,
+                              const int32_t *slot_min, const int32_t *final_id, int32_t *out_f) {
     int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    if (i >= 6 * *nquads) return;
     int32_t a = tri_cells[i];
     out_f[i] = final_id[slot_min[rep_slot[a]]];
 }
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t) gridDim.x * kBlock) p[i] = v;
+}
+__global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
     for (int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t) gridDim.x * kBlock) p[i] = v;
 }
 __global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
@@ -641,9 +660,11 @@ CellSrc make_src(const GridDesc &g) {
         if (e_ != hipSuccess) return e_; \
     } while (0)
 
+unsigned its_blocks(int64_t n) { return (unsigned) std::max<int64_t>(1, (n + kBlock * kCpt - 1) / (kBlock * kCpt)); }
+
 hipError_t its_count(const GridDesc &g, float level, int64_t *bsum, int64_t *tot, hipStream_t st) {
     CellSrc s = make_src(g);
-    unsigned nb = nblocks(s.n);
+    unsigned nb = its_blocks(s.n);
     hipLaunchKernelGGL(k_its_count, dim3(nb), dim3(kBlock), 0, st, s, level, bsum);
     hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, bsum, (int64_t) nb, tot);
     return hipGetLastError();
@@ -651,12 +672,12 @@ hipError_t its_count(const GridDesc &g, float level, int64_t *bsum, int64_t *tot
 
 int64_t its_scratch_pairs(const GridDesc &g) {
     CellSrc s = make_src(g);
-    return nblocks(s.n);
+    return its_blocks(s.n);
 }
 
 hipError_t its_emit(const GridDesc &g, float level, const int64_t *bsum_pre, const ItsOut &o, hipStream_t st) {
     CellSrc s = make_src(g);
-    hipLaunchKernelGGL(k_its_emit, dim3(nblocks(s.n)), dim3(kBlock), 0, st, s, level, bsum_pre, o);
+    hipLaunchKernelGGL(k_its_emit, dim3(its_blocks(s.n)), dim3(kBlock), 0, st, s, level, bsum_pre, o);
     return hipGetLastError();
 }
 
@@ -675,18 +696,17 @@ hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, 
     hipLaunchKernelGGL(k_fill_u8, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, in.ncells, (uint8_t) 0);
     hipLaunchKernelGGL(k_dc_qef, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in, s, reg, svd_tol, cond_fp64,
                        w.dual_v, w.cond, w.n_fp64);
+    // active-cell hash for the quads' neighbour lookups
+    const unsigned nbh = (unsigned) std::min<int64_t>(nblocks(w.amask + 1), 4096);
+    hipLaunchKernelGGL(k_fill_i64, dim3(nbh), dim3(kBlock), 0, st, w.akey, w.amask + 1, (int64_t) -1);
+    hipLaunchKernelGGL(k_act_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in.cell_dense, in.ncells,
+                       w.akey, w.aval, w.amask);
+    const ActHash ah{w.akey, w.aval, w.amask};
     unsigned nbq = nblocks(in.npts);
-    hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, w.bsum);
+    hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum);
     hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbq, w.tot);
-    ISO_TRY(hipGetLastError());
-    int64_t tot[2];
-    ISO_TRY(hipMemcpyAsync(tot, w.tot, sizeof(tot), hipMemcpyDeviceToHost, st));
-    ISO_TRY(hipStreamSynchronize(st));
-    const int64_t nq = tot[0];
-    w.nquads = nq;
-    if (nq == 0) return hipSuccess;
-    if (6 * nq > w.cap_tri) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_quad_emit, dim3(nbq), dim3(kBlock), 0, st, in, s, w.bsum, w.dual_v, w.tri_cells, w.used);
+    hipLaunchKernelGGL(k_quad_emit, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, w.dual_v, w.tri_cells,
+                       w.used);
     // weld dual vertices by position
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.hslots, w.hmask + 1, -1);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.slot_min, w.hmask + 1,
@@ -698,15 +718,18 @@ hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, 
     unsigned nbv = nblocks(in.ncells);
     hipLaunchKernelGGL(k_keep_flags, dim3(nbv), dim3(kBlock), 0, st, w.used, in.ncells, w.rep_slot, w.slot_min,
                        w.bsum);
-    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbv, w.tot);
+    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbv, w.tot + 2);
     hipLaunchKernelGGL(k_keep_emit, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, in.ncells, w.rep_slot,
                        w.slot_min, w.bsum, w.final_id, w.out_v);
-    hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * nq)), dim3(kBlock), 0, st, w.tri_cells, 6 * nq, w.rep_slot,
-                       w.slot_min, w.final_id, w.out_f);
+    hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * in.npts)), dim3(kBlock), 0, st, w.tri_cells, w.tot,
+                       w.rep_slot, w.slot_min, w.final_id, w.out_f);
     ISO_TRY(hipGetLastError());
+    int64_t tot[4];
     ISO_TRY(hipMemcpyAsync(tot, w.tot, sizeof(tot), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipStreamSynchronize(st));
-    nv = tot[0];
+    const int64_t nq = tot[0];
+    w.nquads = nq;
+    nv = tot[2];
     nf = 2 * nq;
     return hipSuccess;
 }

@@ -71,6 +71,8 @@ struct McParams {
     int method;        // 0 = nagae, 1 = lorensen
     int all_special;   // treat every vertex as a weld candidate
     int vec4;          // Z % 4 == 0 and values 16-byte aligned: rows load as float4
+    int its;           // count pass: 0 marching cubes (vertices, faces), 1 intersections
+                       // (active cells, crossing edges)
     int64_t W;         // 64-point chunks per point row, ceil(Z / 64)
     int nzc;           // chunks per z-segment (= emit tile depth)
     int64_t S;         // z-segments per point row, ceil(W / nzc)
@@ -118,6 +120,14 @@ struct McBatchSummary {
     int64_t err, nv, nf, need;
     unsigned long long n_cand, n_hole;
 };
+
+struct ItsOut;
+// Intersections of a dense grid through the mask pipeline (p.its = 1):
+// classify + count + scan (totals: active cells, intersection points), then
+// the emit pass writing the reference's entry order.
+hipError_t its_dense_count_launch(McParams &p, hipStream_t stream);
+hipError_t its_dense_emit_launch(McParams &p, const ItsOut &o, hipStream_t stream);
+constexpr int kItsCfg = 0;   // tile configuration of the intersection pass
 
 int mc_config_count();
 bool mc_config_geom(int cfg, int &tx, int &ty, int &nzc);

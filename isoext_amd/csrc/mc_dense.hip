@@ -37,6 +37,7 @@
 #include <utility>
 
 #include "common.h"
+#include "its_dc.h"
 #include "mc_dense.h"
 #include "mc_luts.h"
 
@@ -301,7 +302,8 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
     __shared__ uint32_t sa[NT / 64], sb[NT / 64];
     __shared__ uint8_t ntri[256];
     const int tid = threadIdx.x;
-    for (int c = tid; c < 256; c += NT) ntri[c] = (uint8_t) (cTris[p.method][c] & 15);
+    for (int c = tid; c < 256; c += NT)
+        ntri[c] = p.its ? (uint8_t) __popc(crossing_status(c)) : (uint8_t) (cTris[p.method][c] & 15);
     const int64_t n = p.X * p.Y * p.S;
     const int64_t i0 = blk * kMcScanBlock;
     const int64_t i1 = min(i0 + (int64_t) kMcScanBlock, n);
@@ -318,7 +320,7 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
         const uint64_t *q = p.bits + g;
         const uint64_t b = q[0], bn = hn ? q[1] : 0ull;
         const uint64_t bx = hasx ? q[YW] : 0ull, by = hasy ? q[p.W] : 0ull;
-        const uint32_t cv = (uint32_t) chunk_vertices(b, bn & 1, bx, by, hasx, hasy, zlo, p.Z);
+        uint32_t cv = p.its ? 0u : (uint32_t) chunk_vertices(b, bn & 1, bx, by, hasx, hasy, zlo, p.Z);
         uint32_t cf = 0;
         if (hasx && hasy) {
             const uint64_t bxy = q[YW + p.W];
@@ -326,6 +328,7 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
                            bxyn = hn ? q[YW + p.W + 1] : 0ull;
             const CellMasks m = cell_masks(b, bn, by, byn, bx, bxn, bxy, bxyn);
             uint64_t act = cell_active(m) & low_bits(p.Z - 1 - zlo);
+            if (p.its) cv = (uint32_t) __popcll(act);
             while (act) {
                 const int j = __ffsll((unsigned long long) act) - 1;
                 act &= act - 1;
@@ -947,30 +950,36 @@ __global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *items
     }
 }
 
-__global__ void __launch_bounds__(256) k_count_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+__global__ void __launch_bounds__(256) k_count_batch(const McBatchItem *__restrict__ items,
+                                                     const McBatchHdr *__restrict__ hdr) {
     const int n = (int) hdr->n;
     for (int64_t gb = blockIdx.x; gb < hdr->nblk; gb += gridDim.x) {
-        const int i = find_item(items, n, gb, 1);
+        const int i = __builtin_amdgcn_readfirstlane(find_item(items, n, gb, 1));
+        const McParams p = items[i].p;
         __syncthreads();
-        count_block(items[i].p, gb - items[i].blk0);
+        count_block(p, gb - items[i].blk0);
     }
 }
 
-__global__ void __launch_bounds__(1024) k_scan_top_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+__global__ void __launch_bounds__(1024) k_scan_top_batch(const McBatchItem *__restrict__ items,
+                                                         const McBatchHdr *__restrict__ hdr) {
     for (int64_t i = blockIdx.x; i < hdr->n; i += gridDim.x) {
+        const McParams p = items[i].p;
         __syncthreads();
-        scan_top(items[i].p);
+        scan_top(p);
     }
 }
 
 template <int TX, int TY, int NZC, int NT>
-__global__ void __launch_bounds__(NT) k_emit_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+__global__ void __launch_bounds__(NT) k_emit_batch(const McBatchItem *__restrict__ items,
+                                                   const McBatchHdr *__restrict__ hdr) {
     const int n = (int) hdr->n;
     for (int64_t gt = blockIdx.x; gt < hdr->ntiles; gt += gridDim.x) {
-        const int i = find_item(items, n, gt, 2);
+        const int i = __builtin_amdgcn_readfirstlane(find_item(items, n, gt, 2));
+        const McParams p = items[i].p;   // uniform copy: scalar registers
         const int64_t t = gt - items[i].tile0;
         __syncthreads();
-        emit_tile<TX, TY, NZC, NT>(items[i].p, t, (int) (t % items[i].p.seg.nseg));
+        emit_tile<TX, TY, NZC, NT>(p, t, (int) (t % p.seg.nseg));
     }
 }
 
@@ -995,6 +1004,179 @@ __global__ void k_batch_summary(const McBatchItem *items, const McBatchHdr *hdr,
         s.n_cand = p.sp->n_cand;
         s.n_hole = p.sp->n_hole;
         out[i] = s;
+    }
+}
+
+// ---- intersections of a dense grid -------------------------------------------------
+// Entry order of get_intersection (src/its.cu:93-159): active cells by
+// ascending cell id, each cell's crossing edges by edge index.  Cell ids run
+// over cell rows (x, y) then z, so the (point row, z-segment) entries of the
+// count pass (p.its = 1: active cells, crossing edges) give every active
+// cell's index and first entry by a prefix sum plus in-segment popcounts.
+
+template <int TX, int TY, int NZC, int NT>
+struct ItsLds {
+    static constexpr int PX = TX + 1, PY = TY + 1, PC = NZC + 1;
+    static constexpr int NUNITS = TX * TY * NZC;
+    uint64_t w[PX * PY * PC];
+    int64_t abase[NUNITS];   // active index of the unit's first active cell
+    int64_t ebase[NUNITS];   // first entry of the unit (crossing count before the scan)
+    uint32_t na[NUNITS + 1];
+    uint32_t list[NT];       // batch: unit << 16 | z-bit << 10 | entry offset in unit
+    uint32_t tsum[NT / 64];
+    uint8_t ncross[256];
+};
+
+template <int TX, int TY, int NZC, int NT>
+__device__ __forceinline__ CellMasks its_unit_cells(const ItsLds<TX, TY, NZC, NT> &s, int cx, int cy, int j) {
+    using L = ItsLds<TX, TY, NZC, NT>;
+    auto W = [&](int px, int py, int jj) { return s.w[(px * L::PY + py) * L::PC + jj]; };
+    return cell_masks(W(cx, cy, j), W(cx, cy, j + 1), W(cx, cy + 1, j), W(cx, cy + 1, j + 1), W(cx + 1, cy, j),
+                      W(cx + 1, cy, j + 1), W(cx + 1, cy + 1, j), W(cx + 1, cy + 1, j + 1));
+}
+
+template <int TX, int TY, int NZC, int NT>
+__global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
+    using L = ItsLds<TX, TY, NZC, NT>;
+    __shared__ L s;
+    const int tid = threadIdx.x;
+    const int64_t tile = blockIdx.x;
+    const int64_t tk = tile % p.NK, tj = (tile / p.NK) % p.NJ, ti = tile / (p.NK * p.NJ);
+    const int64_t x0 = ti * TX, y0 = tj * TY, c0 = tk * NZC, z0 = 64 * c0;
+    const int nx = (int) min<int64_t>(TX, p.X - 1 - x0);
+    const int ny = (int) min<int64_t>(TY, p.Y - 1 - y0);
+
+    for (int c = tid; c < 256; c += NT) s.ncross[c] = (uint8_t) __popc(crossing_status(c));
+    for (int i = tid; i < L::PX * L::PY * L::PC; i += NT) {
+        const int j = i % L::PC, r = i / L::PC, py = r % L::PY, px = r / L::PY;
+        const int64_t gx = x0 + px, gy = y0 + py, c = c0 + j;
+        s.w[i] = (gx < p.X && gy < p.Y && c < p.W) ? p.bits[(gx * p.Y + gy) * p.W + c] : 0ull;
+    }
+    __syncthreads();
+    for (int i = tid; i < TX * TY; i += NT) {
+        const int cx = i / TY, cy = i % TY;
+        for (int j = 0; j < NZC; ++j) {
+            const int u = i * NZC + j;
+            s.na[u] = 0;
+            s.ebase[u] = 0;
+            const int64_t zlo = z0 + 64 * j;
+            if (cx >= nx || cy >= ny || zlo >= p.Z - 1) continue;
+            const CellMasks m = its_unit_cells(s, cx, cy, j);
+            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - zlo);
+            s.na[u] = (uint32_t) __popcll(act);
+            int64_t nc = 0;
+            while (act) {
+                const int k = __ffsll((unsigned long long) act) - 1;
+                act &= act - 1;
+                nc += s.ncross[cell_case(m, k)];
+            }
+            s.ebase[u] = nc;
+        }
+    }
+    __syncthreads();
+    // global bases per unit (before the scan turns na into offsets)
+    for (int i = tid; i < TX * TY; i += NT) {
+        const int cx = i / TY, cy = i % TY;
+        if (cx >= nx || cy >= ny) continue;
+        const int64_t e = ((x0 + cx) * p.Y + (y0 + cy)) * p.S + tk;
+        int64_t ab = p.blk_v[e / kMcScanBlock] + (int64_t) p.cnt_v[e];
+        int64_t eb = p.blk_f[e / kMcScanBlock] + (int64_t) p.cnt_f[e];
+        for (int j = 0; j < NZC; ++j) {
+            const int u = i * NZC + j;
+            const int64_t n_a = s.na[u], n_e = s.ebase[u];
+            s.abase[u] = ab;
+            s.ebase[u] = eb;
+            ab += n_a;
+            eb += n_e;
+        }
+    }
+    __syncthreads();
+    block_exscan_lds<L::NUNITS, NT>(s.na, s.tsum);
+    const int nactive = (int) s.na[L::NUNITS];
+    for (int j0 = 0; j0 < nactive; j0 += NT) {
+        for (int u = tid; u < L::NUNITS; u += NT) {
+            const uint32_t a0 = s.na[u], a1 = s.na[u + 1];
+            if (a1 == a0 || a1 <= (uint32_t) j0 || a0 >= (uint32_t) (j0 + NT)) continue;
+            const int j = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
+            const CellMasks m = its_unit_cells(s, cx, cy, j);
+            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - (z0 + 64 * j));
+            const uint32_t end = min(a1, (uint32_t) (j0 + NT));
+            uint32_t idx = a0, eo = 0;
+            while (act && idx < end) {
+                const int k = __ffsll((unsigned long long) act) - 1;
+                act &= act - 1;
+                if (idx >= (uint32_t) j0) s.list[idx - j0] = ((uint32_t) u << 16) | ((uint32_t) k << 10) | eo;
+                eo += s.ncross[cell_case(m, k)];
+                ++idx;
+            }
+        }
+        __syncthreads();
+        if (j0 + tid < nactive) {
+            const uint32_t ent = s.list[tid];
+            const int u = (int) (ent >> 16), k = (int) ((ent >> 10) & 63), eo = (int) (ent & 1023);
+            const int jc = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
+            const CellMasks m = its_unit_cells(s, cx, cy, jc);
+            const uint64_t act = cell_active(m);
+            const int cs = cell_case(m, k);
+            const int64_t a = s.abase[u] + __popcll(act & (k ? (~0ull >> (64 - k)) : 0ull));
+            int64_t e = s.ebase[u] + eo;
+            const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + 64 * jc + k;
+            const int64_t id = (gx * (p.Y - 1) + gy) * (p.Z - 1) + gz;
+            float cv[8];
+            {
+                const float *r00 = p.values + (gx * p.Y + gy) * p.Z + gz;
+                const float *r01 = r00 + p.Z;
+                const float *r10 = r00 + p.Y * p.Z;
+                const float *r11 = r10 + p.Z;
+                cv[0] = r00[0];
+                cv[1] = r00[1];
+                cv[2] = r01[0];
+                cv[3] = r01[1];
+                cv[4] = r10[0];
+                cv[5] = r10[1];
+                cv[6] = r11[0];
+                cv[7] = r11[1];
+            }
+            if (a >= p.cap_v) goto next;   // capacities from the count pass (defensive)
+            o.cell_indices[a] = id;
+            o.cell_dense[a] = id;
+            o.cell_offsets[a] = e;
+            {
+            const float lo[3] = {grid_coord(gx, p.X, p.amin[0], p.amax[0]), grid_coord(gy, p.Y, p.amin[1], p.amax[1]),
+                                 grid_coord(gz, p.Z, p.amin[2], p.amax[2])};
+            const float hi[3] = {grid_coord(gx + 1, p.X, p.amin[0], p.amax[0]),
+                                 grid_coord(gy + 1, p.Y, p.amin[1], p.amax[1]),
+                                 grid_coord(gz + 1, p.Z, p.amin[2], p.amax[2])};
+            const int st = crossing_status(cs);
+            for_edges([&](auto E) {
+                if (!((st >> E) & 1) || e >= p.cap_f) return;
+                constexpr int ca = edge_a(E), cb = edge_b(E);
+                const float t = edge_t(cv[ca], cv[cb], p.level);
+                const float px = lerp1(t, (ca >> 2) ? hi[0] : lo[0], (cb >> 2) ? hi[0] : lo[0]);
+                const float py = lerp1(t, ((ca >> 1) & 1) ? hi[1] : lo[1], ((cb >> 1) & 1) ? hi[1] : lo[1]);
+                const float pz = lerp1(t, (ca & 1) ? hi[2] : lo[2], (cb & 1) ? hi[2] : lo[2]);
+                *reinterpret_cast<float3 *>(o.points + 3 * e) = make_float3(px, py, pz);
+                o.edge_local[e] = (uint8_t) E;
+                o.is_out[e] = cv[ca] <= cv[cb] ? 1 : 0;
+                o.entry_cell[e] = (int32_t) a;
+                if (o.normals) {
+                    float nxv, nyv, nzv;
+                    cell_normal(lo, hi, cv, px, py, pz, nxv, nyv, nzv);
+                    *reinterpret_cast<float3 *>(o.normals + 3 * e) = make_float3(nxv, nyv, nzv);
+                }
+                ++e;
+            });
+            }
+        next:;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_its_reset(int *err, int64_t *totals) {
+    if (threadIdx.x == 0) {
+        *err = 0;
+        totals[0] = totals[1] = 0;
     }
 }
 
@@ -1090,6 +1272,33 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(4)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sp_gather, dim3(p.seg.nseg), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t its_dense_count_launch(McParams &p, hipStream_t stream) {
+    hipError_t e = mc_upload_luts();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_its_reset, dim3(1), dim3(64), 0, stream, p.err, p.totals);
+    const int64_t nwords = p.X * p.Y * p.W;
+    const unsigned cblocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nwords + 63) / 64, 8192));
+    if (p.vec4 && p.Z % 64 == 0)
+        hipLaunchKernelGGL(k_classify<0>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
+                           p.bits);
+    else if (p.vec4)
+        hipLaunchKernelGGL(k_classify<1>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
+                           p.bits);
+    else
+        hipLaunchKernelGGL(k_classify<2>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
+                           p.bits);
+    hipLaunchKernelGGL(k_count, dim3((unsigned) p.nblk), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t its_dense_emit_launch(McParams &p, const ItsOut &o, hipStream_t stream) {
+    static_assert(kItsCfg == 0, "the intersection pass is compiled for tile configuration 0");
+    const unsigned tiles = (unsigned) (p.NI * p.NJ * p.NK);
+    hipLaunchKernelGGL((k_its_tiles<16, 16, 4, 256>), dim3(tiles), dim3(256), 0, stream, p, o);
     return hipGetLastError();
 }
 
