@@ -63,6 +63,10 @@ int isoext_fill_f32(float *ptr, int64_t n, float value, void *stream);
 /* Number of tile configurations compiled into the library (autotuner range). */
 int isoext_mc_config_count(void);
 
+/* Default tile configuration for a grid shape: the largest tiles that still
+ * give every CU several workgroups (no reference counterpart). */
+int isoext_mc_auto_config(const int64_t shape[3]);
+
 /* Workspace bytes for isoext_mc_dense with tile config `cfg` and room for
  * `special_cap` weld-candidate vertices.  The workspace must be zero-filled
  * before its first use and may be reused by later calls on the same stream

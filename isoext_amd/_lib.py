@@ -43,6 +43,8 @@ def _declare(L):
     L.isoext_last_error.argtypes = []
     L.isoext_version.restype = c_int
     L.isoext_mc_config_count.restype = c_int
+    L.isoext_mc_auto_config.restype = c_int
+    L.isoext_mc_auto_config.argtypes = [I64x3]
     L.isoext_set_timing.restype = None
     L.isoext_set_timing.argtypes = [c_int]
     L.isoext_last_kernel_ms.restype = c_f

@@ -21,7 +21,7 @@ _lock = threading.Lock()
 _table = None
 
 # (TX, TY, NZC) of each compiled configuration, in library order
-CONFIGS = [(16, 16, 4), (32, 16, 4), (16, 16, 2), (16, 16, 1), (16, 16, 8), (32, 16, 2)]
+CONFIGS = [(16, 16, 4), (8, 8, 2), (16, 16, 2), (16, 16, 1), (16, 16, 8), (8, 8, 1)]
 
 
 def size_class(shape) -> str:
@@ -57,10 +57,9 @@ def mc_config(shape) -> int:
         hit = t.get(size_class(shape))
     if hit is not None:
         return int(hit)
-    X, Y, Z = shape
-    if Z <= 129:
-        return 2 if Z > 65 else 3
-    return 0
+    from . import _lib
+    L = _lib.lib()
+    return int(L.isoext_mc_auto_config(_lib.i64x3([int(s) for s in shape])))
 
 
 def tune(shape, values=None, level=0.0, reps=5, write=True):

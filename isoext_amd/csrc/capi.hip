@@ -125,7 +125,7 @@ bool grid_well_separated(const int64_t shape[3], const float *amin, const float 
 // scratch holds the masks, count arrays and totals; cap_* bound the writes).
 iso::McParams its_params(const isoext_grid *g, float level, char *ws, int64_t cap_cells, int64_t cap_pts) {
     McLayout L;
-    mc_layout(g->shape, iso::kItsCfg, 1, L);
+    mc_layout(g->shape, iso::mc_auto_config(g->shape), 1, L);
     iso::McParams p{};
     p.values = g->values;
     p.X = g->shape[0];
@@ -232,7 +232,7 @@ DcLayout dc_layout(int64_t ncells, int64_t npts) {
 
 // ---- ragged batches of dense marching cubes --------------------------------------
 
-constexpr int kBatchSeg = 16;   // special-list segments per batch item
+constexpr int kBatchSeg = iso::kMaxSpSegBatch;   // special-list segments per batch item
 
 struct ItemLayout {
     bool cells;
@@ -413,6 +413,7 @@ int batch_upload(int n, const isoext_mc_item *items, int cfg, char *ws, int64_t 
 int batch_check(int n, const isoext_mc_item *items, int cfg) {
     int tx, ty, nzc;
     if (n < 1) return fail(ISOEXT_E_ARG, "empty batch");
+    if (n > iso::kMaxBatch) return fail(ISOEXT_E_ARG, "batch of %d grids exceeds %d", n, iso::kMaxBatch);
     if (!iso::mc_config_geom(cfg, tx, ty, nzc)) return fail(ISOEXT_E_ARG, "unknown tile config %d", cfg);
     for (int i = 0; i < n; ++i) {
         if (items[i].method != 0 && items[i].method != 1)
@@ -432,7 +433,7 @@ int64_t isoext_its_scratch_bytes(const isoext_grid *g) {
     if (!grid_has_cells(g)) return 64;
     if (!g->sparse_cell_ids) {
         McLayout L;
-        mc_layout(g->shape, iso::kItsCfg, 1, L);
+        mc_layout(g->shape, iso::mc_auto_config(g->shape), 1, L);
         return (int64_t) L.total;
     }
     iso::GridDesc d = to_desc(g);
@@ -472,7 +473,7 @@ int isoext_its_emit(const isoext_grid *g, float level, const void *scratch, int6
         iso::ItsOut o{cell_indices, cell_dense, cell_offsets, points, normals, edge_local, is_out, entry_cell};
         if (!g->sparse_cell_ids) {
             iso::McParams p = its_params(g, level, (char *) scratch, ncells, npts);
-            HIP_CHECK(iso::its_dense_emit_launch(p, o, st), "intersection emit");
+            HIP_CHECK(iso::its_dense_emit_launch(iso::mc_auto_config(g->shape), p, o, st), "intersection emit");
         } else {
             HIP_CHECK(iso::its_emit(to_desc(g), level, (const int64_t *) scratch, o, st), "intersection emit");
         }
@@ -556,6 +557,8 @@ void isoext_mc_last_stage_ms(float out[5]) {
 int isoext_version(void) { return ISOEXT_ABI_VERSION; }
 
 int isoext_mc_config_count(void) { return iso::mc_config_count(); }
+
+int isoext_mc_auto_config(const int64_t shape[3]) { return iso::mc_auto_config(shape); }
 
 int64_t isoext_mc_dense_workspace_bytes(const int64_t shape[3], int cfg, int64_t special_cap) {
     McLayout L;

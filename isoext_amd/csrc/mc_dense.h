@@ -126,10 +126,14 @@ struct ItsOut;
 // classify + count + scan (totals: active cells, intersection points), then
 // the emit pass writing the reference's entry order.
 hipError_t its_dense_count_launch(McParams &p, hipStream_t stream);
-hipError_t its_dense_emit_launch(McParams &p, const ItsOut &o, hipStream_t stream);
-constexpr int kItsCfg = 0;   // tile configuration of the intersection pass
+hipError_t its_dense_emit_launch(int cfg, McParams &p, const ItsOut &o, hipStream_t stream);
+
+constexpr int kMaxBatch = 256;        // items per batch
+constexpr int kMaxSpSegBatch = 16;    // special-list segments per batch item
 
 int mc_config_count();
+int64_t mc_config_tiles(int cfg, const int64_t shape[3]);
+int mc_auto_config(const int64_t shape[3]);
 bool mc_config_geom(int cfg, int &tx, int &ty, int &nzc);
 hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
                            hipEvent_t *ev = nullptr);

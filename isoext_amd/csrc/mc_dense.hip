@@ -868,6 +868,17 @@ __device__ __forceinline__ int find_item(const McBatchItem *it, int n, int64_t x
     return lo;
 }
 
+// The same over item starts staged in LDS (batches hold at most kMaxBatch items).
+__device__ __forceinline__ int find_start(const int64_t *st, int n, int64_t x) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (st[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, const McBatchHdr *hdr) {
     const int n = (int) hdr->n;
     for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < (int64_t) n * 4 * kSpSeg;
@@ -888,8 +899,11 @@ __global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, c
 
 __global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *items, const McBatchHdr *hdr) {
     constexpr int U = 4;
+    __shared__ int64_t starts[kMaxBatch];
     const int n = (int) hdr->n;
     const int64_t nwords = hdr->nwords;
+    for (int i = threadIdx.x; i < n; i += 256) starts[i] = items[i].word0;
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int grp = lane >> 4, q = lane & 15;
     const int64_t wave = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -905,7 +919,7 @@ __global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *items
             valid[u] = 0;
             item[u] = 0;
             if (gw < nwords) {
-                const int i = find_item(items, n, gw, 0);
+                const int i = find_start(starts, n, gw);
                 item[u] = i;
                 const McBatchItem &it = items[i];
                 const int64_t w = gw - it.word0;
@@ -983,14 +997,11 @@ __global__ void __launch_bounds__(NT) k_emit_batch(const McBatchItem *__restrict
     }
 }
 
+// grid (segments, items)
 __global__ void __launch_bounds__(256) k_sp_gather_batch(const McBatchItem *items, const McBatchHdr *hdr) {
-    for (int64_t i = 0; i < hdr->n; ++i) {
-        const int nseg = items[i].p.seg.nseg;
-        for (int b = blockIdx.x; b < nseg; b += gridDim.x) {
-            __syncthreads();
-            sp_gather_seg(items[i].p, b);
-        }
-    }
+    const int64_t i = blockIdx.y;
+    if (i >= hdr->n || (int) blockIdx.x >= items[i].p.seg.nseg) return;
+    sp_gather_seg(items[i].p, blockIdx.x);
 }
 
 __global__ void k_batch_summary(const McBatchItem *items, const McBatchHdr *hdr, McBatchSummary *out) {
@@ -1186,11 +1197,11 @@ __global__ void k_its_reset(int *err, int64_t *totals) {
 // x, along y, 64-cell chunks along z, threads).
 #define ISO_MC_CONFIGS(X)   \
     X(0, 16, 16, 4, 256)    \
-    X(1, 32, 16, 4, 256)    \
+    X(1, 8, 8, 2, 256)      \
     X(2, 16, 16, 2, 256)    \
     X(3, 16, 16, 1, 256)    \
     X(4, 16, 16, 8, 256)    \
-    X(5, 32, 16, 2, 256)
+    X(5, 8, 8, 1, 256)
 
 int mc_config_count() {
     int n = 0;
@@ -1213,6 +1224,32 @@ bool mc_config_geom(int cfg, int &tx, int &ty, int &nzc) {
     default:
         return false;
     }
+}
+
+int64_t mc_config_tiles(int cfg, const int64_t shape[3]) {
+    int tx, ty, nzc;
+    if (!mc_config_geom(cfg, tx, ty, nzc) || shape[0] < 2 || shape[1] < 2 || shape[2] < 2) return 0;
+    const int64_t cell_chunks = (shape[2] - 1 + 63) / 64;
+    return ((shape[0] - 1 + tx - 1) / tx) * ((shape[1] - 1 + ty - 1) / ty) * ((cell_chunks + nzc - 1) / nzc);
+}
+
+// Largest tiles that still give every CU several workgroups (tiles per
+// workgroup shrink for small grids: 16x16x256 cells at 1024^3, 8x8x64 at
+// 256^3 and below).
+int mc_auto_config(const int64_t shape[3]) {
+    static const int order[] = {0, 2, 3, 1, 5};
+    constexpr int64_t kWant = 2048;
+    int best = 0;
+    int64_t best_tiles = -1;
+    for (int cfg : order) {
+        const int64_t t = mc_config_tiles(cfg, shape);
+        if (t >= kWant) return cfg;
+        if (t > best_tiles) {
+            best_tiles = t;
+            best = cfg;
+        }
+    }
+    return best;
 }
 
 hipError_t mc_upload_luts() {
@@ -1295,10 +1332,18 @@ hipError_t its_dense_count_launch(McParams &p, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t its_dense_emit_launch(McParams &p, const ItsOut &o, hipStream_t stream) {
-    static_assert(kItsCfg == 0, "the intersection pass is compiled for tile configuration 0");
+hipError_t its_dense_emit_launch(int cfg, McParams &p, const ItsOut &o, hipStream_t stream) {
     const unsigned tiles = (unsigned) (p.NI * p.NJ * p.NK);
-    hipLaunchKernelGGL((k_its_tiles<16, 16, 4, 256>), dim3(tiles), dim3(256), 0, stream, p, o);
+    switch (cfg) {
+#define X(id, tx, ty, nzc, nt)                                                                        \
+    case id:                                                                                          \
+        hipLaunchKernelGGL((k_its_tiles<tx, ty, nzc, nt>), dim3(tiles), dim3(nt), 0, stream, p, o);   \
+        break;
+        ISO_MC_CONFIGS(X)
+#undef X
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -1338,7 +1383,7 @@ hipError_t mc_batch_emit_launch(int cfg, const McBatchItem *items, const McBatch
         return hipErrorInvalidValue;
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sp_gather_batch, dim3(kSpSeg), dim3(256), 0, stream, items, hdr);
+    hipLaunchKernelGGL(k_sp_gather_batch, dim3(kMaxSpSegBatch, (unsigned) h.n), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_batch_summary, dim3((unsigned) ((h.n + 255) / 256)), dim3(256), 0, stream, items, hdr,
                        summary);
