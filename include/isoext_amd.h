@@ -150,7 +150,17 @@ int64_t isoext_dc_workspace_bytes(int64_t ncells, int64_t npts);
  * counts = {vertices, faces, cells re-solved in FP64, quads}.  Synchronises. */
 int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_tol, void *workspace,
               int64_t workspace_bytes, float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f,
-              float *dual_v, int64_t counts[4], void *stream);
+              float *dual_v, int32_t *out_vcell, int64_t counts[4], void *stream);
+
+/* Backward of dual contouring (no reference counterpart): accumulates
+ * dL/dvalues += dL/dv * dv/dvalues through the regularised QEF solve of each
+ * output vertex's cell and the edge interpolation of its intersection
+ * points (out_vcell of isoext_dc maps vertices to cells; clipped components
+ * pass no gradient); writes dL/dnormals per intersection when grad_normals
+ * is not NULL.  Normals are treated as inputs. */
+int isoext_dc_backward(const isoext_grid *g, const isoext_its *its, float reg, float svd_tol, float level,
+                       int64_t nv, const int32_t *vcell, const float *grad_v, float *grad_values,
+                       float *grad_normals, void *stream);
 
 /* Same as isoext_mc_dense, also writing each vertex's grid edge to out_key
  * (owner point linear id * 3 + axis, 0 = x) for the backward pass.  No

@@ -241,6 +241,12 @@ struct ItemLayout {
     size_t o_ctr, o_scand, o_sscell, o_skept, o_shole, o_cand, o_scell, o_kept, o_hole;
 };
 
+// Words an item occupies in the batch word space (padded so that every item
+// starts at a multiple of the classify task size).
+inline int64_t word_span(int64_t nwords) {
+    return (nwords + iso::kClassifyWords - 1) / iso::kClassifyWords * iso::kClassifyWords;
+}
+
 struct BatchLayout {
     size_t o_hdr, o_items, o_summary, total;
     std::vector<ItemLayout> it;
@@ -273,7 +279,7 @@ bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_c
         L.nblk = L.cells ? (X * Y * L.S + iso::kMcScanBlock - 1) / iso::kMcScanBlock : 0;
         L.nwords = L.cells ? X * Y * L.W : 0;
         L.ntiles = L.NI * L.NJ * L.NK;
-        B.h.nwords += L.nwords;
+        B.h.nwords += word_span(L.nwords);
         B.h.nblk += L.nblk;
         B.h.ntiles += L.ntiles;
         const int64_t R = L.cells ? X * Y : 0;
@@ -399,7 +405,7 @@ int batch_upload(int n, const isoext_mc_item *items, int cfg, char *ws, int64_t 
     for (int i = 0; i < n; ++i) {
         fill_item(items[i], B.it[i], ws, w0, b0, t0, it[i]);
         it[i].p.nzc = nzc;
-        w0 += B.it[i].nwords;
+        w0 += word_span(B.it[i].nwords);
         b0 += B.it[i].nblk;
         t0 += B.it[i].ntiles;
     }
@@ -484,6 +490,17 @@ int isoext_its_emit(const isoext_grid *g, float level, const void *scratch, int6
     return ISOEXT_OK;
 }
 
+int isoext_dc_backward(const isoext_grid *g, const isoext_its *its, float reg, float svd_tol, float level,
+                       int64_t nv, const int32_t *vcell, const float *grad_v, float *grad_values,
+                       float *grad_normals, void *stream) {
+    if (!grid_has_cells(g) || nv == 0) return ISOEXT_OK;
+    if (!its->normals || !vcell || !grad_v || !grad_values) return fail(ISOEXT_E_ARG, "missing argument");
+    HIP_CHECK(iso::dc_backward(to_desc(g), to_in(its), reg, svd_tol, level, nv, vcell, grad_v, grad_values,
+                               grad_normals, (hipStream_t) stream),
+              "dual contouring backward");
+    return ISOEXT_OK;
+}
+
 int isoext_its_normals(const isoext_grid *g, const isoext_its *its, float *normals, void *stream) {
     if (!grid_has_cells(g) || its->npts == 0) return ISOEXT_OK;
     HIP_CHECK(iso::its_normals(to_desc(g), to_in(its), normals, (hipStream_t) stream), "normals");
@@ -494,7 +511,7 @@ int64_t isoext_dc_workspace_bytes(int64_t ncells, int64_t npts) { return (int64_
 
 int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_tol, void *workspace,
               int64_t workspace_bytes, float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f, float *dual_v,
-              int64_t counts[4], void *stream) {
+              int32_t *out_vcell, int64_t counts[4], void *stream) {
     counts[0] = counts[1] = counts[2] = counts[3] = 0;
     if (!grid_has_cells(g) || its->ncells == 0) return ISOEXT_OK;
     if (!its->normals) return fail(ISOEXT_E_ARG, "intersection has no normals");
@@ -522,6 +539,7 @@ int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_
     w.final_id = (int32_t *) (ws + L.o_fin);
     w.out_v = out_v;
     w.out_f = out_f;
+    w.out_vcell = out_vcell;
     unsigned long long zero = 0;
     HIP_CHECK(hipMemcpyAsync(w.n_fp64, &zero, sizeof(zero), hipMemcpyHostToDevice, st), "reset counter");
     // re-solve in FP64 where the FP32 QEF is ill-conditioned, and always in the

@@ -55,6 +55,7 @@ struct DcScratch {
     int64_t amask;
     float *out_v;
     int32_t *out_f;
+    int32_t *out_vcell;   // optional: active cell of each output vertex
     int64_t nquads;
 };
 
@@ -64,5 +65,11 @@ hipError_t its_emit(const GridDesc &g, float level, const int64_t *bsum_pre, con
 hipError_t its_normals(const GridDesc &g, const ItsIn &in, float *normals, hipStream_t st);
 hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float cond_fp64, DcScratch &w,
                   int64_t &nv, int64_t &nf, hipStream_t st);
+// Backward of dual contouring: dL/d(output vertex) -> dL/d(grid values)
+// (through the QEF solve and the edge interpolation of the intersection
+// points) and, when grad_normals != nullptr, dL/d(normals).
+hipError_t dc_backward(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float level, int64_t nv,
+                       const int32_t *vcell, const float *grad_v, float *grad_values, float *grad_normals,
+                       hipStream_t st);
 
 }   // namespace iso

@@ -568,7 +568,8 @@ __global__ void k_keep_flags(const uint8_t *used, int64_t n, const int32_t *rep_
 }
 
 __global__ void k_keep_emit(const float *dv, const uint8_t *used, int64_t n, const int32_t *rep_slot,
-                            const int32_t *slot_min, const int64_t *bsum_pre, int32_t *final_id, float *out_v) {
+                            const int32_t *slot_min, const int64_t *bsum_pre, int32_t *final_id, float *out_v,
+                            int32_t *out_vcell) {
     int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t k = (a < n && used[a] && slot_min[rep_slot[a]] == (int32_t) a) ? 1 : 0;
     int64_t ek, d, tk, td;
@@ -579,6 +580,7 @@ __global__ void k_keep_emit(const float *dv, const uint8_t *used, int64_t n, con
     out_v[3 * id + 0] = dv[3 * a + 0];
     out_v[3 * id + 1] = dv[3 * a + 1];
     out_v[3 * id + 2] = dv[3 * a + 2];
+    if (out_vcell) out_vcell[id] = (int32_t) a;
 }
 
 // Face corners: 6 per quad; the quad count is read from the device.
@@ -632,6 +634,89 @@ __global__ void k_scan_pairs(int64_t *bsum, int64_t nb, int64_t *tot) {
     if (threadIdx.x == 1023) {
         tot[0] = sa[1023];
         tot[1] = sb[1023];
+    }
+}
+
+// ---- backward -------------------------------------------------------------------
+// x = M^+ (sum_i n_i (n_i . p_i) + reg c), M = sum_i n_i n_i^T + reg I,
+// c = mean p_i (the reference's regularised QEF, src/dc.cu:14-65, solved by
+// src/batched_la.cu:151-178).  With y = M^+ g:
+//   dL/dp_i = n_i (n_i . y) + (reg / k) y
+//   dL/dn_i = (n_i . (p_i - x)) y + (p_i - x) (n_i . y)
+// components of x clipped to the cell (src/dc.cu:82-99) pass no gradient;
+// p_i = (1 - t) Pa + t Pb with t = (L - va) / (vb - va) carries dL/dp_i to va, vb.
+__global__ void k_dc_backward(const ItsIn in, const CellSrc s, float reg, float svd_tol, float level, int64_t nv,
+                              const int32_t *vcell, const float *grad_v, float *grad_values, float *grad_normals) {
+    const int64_t v = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    if (v >= nv) return;
+    const int64_t a = vcell[v];
+    const int64_t lo = in.cell_offsets[a], hi = in.cell_offsets[a + 1];
+    const int64_t id = in.cell_dense[a];
+    int64_t cx, cy, cz;
+    cell_xyz(s, id, cx, cy, cz);
+    const double blo[3] = {grid_coord(cx, s.X, s.amin[0], s.amax[0]), grid_coord(cy, s.Y, s.amin[1], s.amax[1]),
+                           grid_coord(cz, s.Z, s.amin[2], s.amax[2])};
+    const double bhi[3] = {grid_coord(cx + 1, s.X, s.amin[0], s.amax[0]),
+                           grid_coord(cy + 1, s.Y, s.amin[1], s.amax[1]),
+                           grid_coord(cz + 1, s.Z, s.amin[2], s.amax[2])};
+    double M[3][3] = {{reg, 0, 0}, {0, reg, 0}, {0, 0, reg}}, b[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+    const double k = (double) (hi - lo);
+    for (int64_t j = lo; j < hi; ++j) {
+        const double n[3] = {in.normals[3 * j], in.normals[3 * j + 1], in.normals[3 * j + 2]};
+        const double p[3] = {in.points[3 * j], in.points[3 * j + 1], in.points[3 * j + 2]};
+        const double d = n[0] * p[0] + n[1] * p[1] + n[2] * p[2];
+        for (int u = 0; u < 3; ++u) {
+            for (int w = 0; w < 3; ++w) M[u][w] += n[u] * n[w];
+            b[u] += n[u] * d;
+            c[u] += p[u] / k;
+        }
+    }
+    for (int u = 0; u < 3; ++u) b[u] += reg * c[u];
+    double Mc[3][3], x[3], y[3];
+    for (int u = 0; u < 3; ++u)
+        for (int w = 0; w < 3; ++w) Mc[u][w] = M[u][w];
+    pinv3<double>(Mc, b, (double) svd_tol, x, 12);
+    double g[3] = {grad_v[3 * v], grad_v[3 * v + 1], grad_v[3 * v + 2]};
+    for (int u = 0; u < 3; ++u)
+        if (x[u] < blo[u] || x[u] > bhi[u]) g[u] = 0.0;
+    for (int u = 0; u < 3; ++u)
+        for (int w = 0; w < 3; ++w) Mc[u][w] = M[u][w];
+    pinv3<double>(Mc, g, (double) svd_tol, y, 12);
+    for (int64_t j = lo; j < hi; ++j) {
+        const double n[3] = {in.normals[3 * j], in.normals[3 * j + 1], in.normals[3 * j + 2]};
+        const double p[3] = {in.points[3 * j], in.points[3 * j + 1], in.points[3 * j + 2]};
+        const double ny = n[0] * y[0] + n[1] * y[1] + n[2] * y[2];
+        double gp[3];
+        for (int u = 0; u < 3; ++u) gp[u] = n[u] * ny + (reg / k) * y[u];
+        if (grad_normals) {
+            const double r[3] = {p[0] - x[0], p[1] - x[1], p[2] - x[2]};
+            const double nr = n[0] * r[0] + n[1] * r[1] + n[2] * r[2];
+            for (int u = 0; u < 3; ++u) grad_normals[3 * j + u] = (float) (nr * y[u] + r[u] * ny);
+        }
+        const int e = in.edge_local[j];
+        const int ca = edge_a(e), cb = edge_b(e);
+        const int64_t ax = cx + (ca >> 2), ay = cy + ((ca >> 1) & 1), az = cz + (ca & 1);
+        const int64_t bx = cx + (cb >> 2), by = cy + ((cb >> 1) & 1), bz = cz + (cb & 1);
+        int64_t ia, ib;
+        if (s.ids) {
+            const int64_t slot = in.cell_indices[a];
+            ia = 8 * slot + ca;
+            ib = 8 * slot + cb;
+        } else {
+            ia = (ax * s.Y + ay) * s.Z + az;
+            ib = (bx * s.Y + by) * s.Z + bz;
+        }
+        const double va = s.values[ia], vb = s.values[ib];
+        const double dv = vb - va;
+        if (!(dv != 0.0)) continue;
+        const double Pa[3] = {grid_coord(ax, s.X, s.amin[0], s.amax[0]), grid_coord(ay, s.Y, s.amin[1], s.amax[1]),
+                              grid_coord(az, s.Z, s.amin[2], s.amax[2])};
+        const double Pb[3] = {grid_coord(bx, s.X, s.amin[0], s.amax[0]), grid_coord(by, s.Y, s.amin[1], s.amax[1]),
+                              grid_coord(bz, s.Z, s.amin[2], s.amax[2])};
+        const double sd = gp[0] * (Pb[0] - Pa[0]) + gp[1] * (Pb[1] - Pa[1]) + gp[2] * (Pb[2] - Pa[2]);
+        const double ga = sd * ((double) level - vb) / (dv * dv), gb = -sd * ((double) level - va) / (dv * dv);
+        if (isfinite(ga) && ga != 0.0) atomicAdd(&grad_values[ia], (float) ga);
+        if (isfinite(gb) && gb != 0.0) atomicAdd(&grad_values[ib], (float) gb);
     }
 }
 
@@ -720,7 +805,7 @@ hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, 
                        w.bsum);
     hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbv, w.tot + 2);
     hipLaunchKernelGGL(k_keep_emit, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, in.ncells, w.rep_slot,
-                       w.slot_min, w.bsum, w.final_id, w.out_v);
+                       w.slot_min, w.bsum, w.final_id, w.out_v, w.out_vcell);
     hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * in.npts)), dim3(kBlock), 0, st, w.tri_cells, w.tot,
                        w.rep_slot, w.slot_min, w.final_id, w.out_f);
     ISO_TRY(hipGetLastError());
@@ -732,6 +817,16 @@ hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, 
     nv = tot[2];
     nf = 2 * nq;
     return hipSuccess;
+}
+
+hipError_t dc_backward(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float level, int64_t nv,
+                       const int32_t *vcell, const float *grad_v, float *grad_values, float *grad_normals,
+                       hipStream_t st) {
+    if (nv == 0) return hipSuccess;
+    CellSrc s = make_src(g);
+    hipLaunchKernelGGL(k_dc_backward, dim3(nblocks(nv)), dim3(kBlock), 0, st, in, s, reg, svd_tol, level, nv, vcell,
+                       grad_v, grad_values, grad_normals);
+    return hipGetLastError();
 }
 
 }   // namespace iso

@@ -63,6 +63,11 @@ inline int64_t sp_seg_cap(int64_t special_cap, int k, int nseg) {
 // Entries of (point row, z-segment) counted and scanned per workgroup.
 constexpr int kMcScanBlock = 1024;
 
+// 64-point words classified per wave task by the lane-per-point classify;
+// batch items start their words at multiples of it, so that no task spans
+// two grids.
+constexpr int kClassifyWords = 16;
+
 struct McParams {
     const float *values;
     int64_t X, Y, Z;   // points per axis

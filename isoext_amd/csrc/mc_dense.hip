@@ -185,11 +185,11 @@ __device__ __forceinline__ int64_t edge_key(const McParams &p, int64_t gx, int64
 }
 
 // ---- classify ------------------------------------------------------------------
-// MODE 0: Z % 64 == 0 and float4-aligned rows (word w covers values
-// [64w, 64w + 64)); 1: Z % 4 == 0 and float4-aligned; 2: scalar loads.
-template <int MODE>
-__global__ void __launch_bounds__(256) k_classify(const float *__restrict__ values, int64_t Z, int64_t W,
-                                                  int64_t nwords, float level, uint64_t *__restrict__ bits) {
+// k_classify_v4 (Z % 64 == 0, float4-aligned rows): word w covers values
+// [64w, 64w + 64); 16 lanes load it with dwordx4 loads and merge their
+// nibbles with three in-group shuffles.
+__global__ void __launch_bounds__(256) k_classify_v4(const float *__restrict__ values, int64_t nwords, float level,
+                                                     uint64_t *__restrict__ bits) {
     constexpr int U = 4;   // words in flight per 16-lane group
     const int lane = threadIdx.x & 63;
     const int grp = lane >> 4, q = lane & 15;
@@ -197,48 +197,23 @@ __global__ void __launch_bounds__(256) k_classify(const float *__restrict__ valu
     const int64_t nwave = (int64_t) gridDim.x * 4;
     for (int64_t base = wave * (4 * U); base < nwords; base += nwave * (4 * U)) {
         float v[U][4];
-        uint32_t valid[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t w = base + 4 * u + grp;
             v[u][0] = v[u][1] = v[u][2] = v[u][3] = 0.0f;
-            valid[u] = 0;
             if (w < nwords) {
-                if (MODE == 0) {
-                    const float4 f = *reinterpret_cast<const float4 *>(values + 64 * w + 4 * q);
-                    v[u][0] = f.x;
-                    v[u][1] = f.y;
-                    v[u][2] = f.z;
-                    v[u][3] = f.w;
-                    valid[u] = 15;
-                } else {
-                    const int64_t row = w / W, c = w - row * W;
-                    const int64_t z = 64 * c + 4 * q;
-                    const int64_t left = Z - z;
-                    const float *src = values + row * Z + z;
-                    if (left > 0) {
-                        if (MODE == 1) {
-                            const float4 f = *reinterpret_cast<const float4 *>(src);
-                            v[u][0] = f.x;
-                            v[u][1] = f.y;
-                            v[u][2] = f.z;
-                            v[u][3] = f.w;
-                            valid[u] = 15;
-                        } else {
-#pragma unroll
-                            for (int j = 0; j < 4; ++j)
-                                if (j < left) v[u][j] = src[j];
-                            valid[u] = left >= 4 ? 15u : ((1u << left) - 1);
-                        }
-                    }
-                }
+                const float4 f = *reinterpret_cast<const float4 *>(values + 64 * w + 4 * q);
+                v[u][0] = f.x;
+                v[u][1] = f.y;
+                v[u][2] = f.z;
+                v[u][3] = f.w;
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             uint32_t x = 0;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) x |= (uint32_t) (((valid[u] >> j) & 1) && inside(v[u][j], level)) << j;
+            for (int j = 0; j < 4; ++j) x |= (uint32_t) inside(v[u][j], level) << j;
             x |= __shfl_down(x, 1, 16) << 4;
             x |= __shfl_down(x, 2, 16) << 8;
             x |= __shfl_down(x, 4, 16) << 16;
@@ -247,6 +222,51 @@ __global__ void __launch_bounds__(256) k_classify(const float *__restrict__ valu
             if (q == 0 && w < nwords) bits[w] = (uint64_t) x | ((uint64_t) hi << 32);
         }
     }
+}
+
+// a / b for non-negative operands, in 32 bits when both fit.
+__device__ __forceinline__ int64_t div_nonneg(int64_t a, int64_t b) {
+    if (((uint64_t) a | (uint64_t) b) >> 32 == 0) return (int64_t) ((uint32_t) a / (uint32_t) b);
+    return a / b;
+}
+
+// Lane-per-point classify of kClassifyWords consecutive words [w0, w0 + 16)
+// of one grid (w0 wave-uniform): lane l loads value z = 64c + l of word
+// (row, c), one 256-byte coalesced load per word whatever the row alignment,
+// and the wave's compare mask is the word.  Lane u stores word u.
+__device__ __forceinline__ void classify_task(const float *__restrict__ values, int64_t Z, int64_t W,
+                                              int64_t nwords, float level, int64_t w0, uint64_t *__restrict__ bits) {
+    constexpr int U = kClassifyWords;
+    const int lane = threadIdx.x & 63;
+    int64_t row = div_nonneg(w0, W);
+    int64_t c = w0 - row * W;
+    float v[U];
+    bool ok[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int64_t z = 64 * c + lane;
+        ok[u] = w0 + u < nwords && z < Z;
+        v[u] = values[ok[u] ? row * Z + z : 0];   // lanes past the row read a valid dummy
+        if (++c == W) {
+            c = 0;
+            ++row;
+        }
+    }
+    uint64_t mine = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint64_t m = __ballot(ok[u] && inside(v[u], level));
+        if (lane == u) mine = m;
+    }
+    if (lane < U && w0 + lane < nwords) bits[w0 + lane] = mine;
+}
+
+__global__ void __launch_bounds__(256) k_classify(const float *__restrict__ values, int64_t Z, int64_t W,
+                                                  int64_t nwords, float level, uint64_t *__restrict__ bits) {
+    const int64_t wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int64_t nwave = (int64_t) gridDim.x * 4;
+    const int64_t ntask = (nwords + kClassifyWords - 1) / kClassifyWords;
+    for (int64_t t = wave; t < ntask; t += nwave) classify_task(values, Z, W, nwords, level, t * kClassifyWords, bits);
 }
 
 // ---- count + block scan ----------------------------------------------------------
@@ -433,9 +453,10 @@ struct EmitLds {
     int32_t vb[(TX + 1) * QY * QC];   // first vertex id per (point row, chunk)
     int32_t fb[NUNITS];               // first face id per unit
     uint32_t na[NUNITS + 1];          // active cells per unit -> exclusive offsets
-    uint32_t list[NT];                // batch: unit << 15 | z-bit << 9 | face offset in unit
+    uint64_t list[NT];                // batch: unit << 32 | case << 24 | z-bit << 16 | face offset in unit
     uint32_t tsum[NT / 64];
     uint64_t tris[256];               // the method's packed triangle table
+    float cx[TX + 1], cy[TY + 1], cz[64 * NZC + 1];   // grid coordinates of the tile's points
 };
 
 template <int TX, int TY, int NZC, int NT>
@@ -547,6 +568,15 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
         }
     }
     for (int c = tid; c < 256; c += NT) s.tris[c] = cTris[p.method][c];
+    for (int c = tid; c < TX + TY + 64 * NZC + 3; c += NT) {
+        if (c <= TX) {
+            s.cx[c] = grid_coord(x0 + c, p.X, p.amin[0], p.amax[0]);
+        } else if (c <= TX + TY + 1) {
+            s.cy[c - TX - 1] = grid_coord(y0 + (c - TX - 1), p.Y, p.amin[1], p.amax[1]);
+        } else {
+            s.cz[c - TX - TY - 2] = grid_coord(z0 + (c - TX - TY - 2), p.Z, p.amin[2], p.amax[2]);
+        }
+    }
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
         const int i = tid + k * NT;
@@ -630,19 +660,21 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             while (act && idx < end) {
                 const int k = __ffsll((unsigned long long) act) - 1;
                 act &= act - 1;
-                if (idx >= (uint32_t) j0) s.list[idx - j0] = ((uint32_t) u << 15) | ((uint32_t) k << 9) | fo;
-                fo += (uint32_t) (s.tris[cell_case(m, k)] & 15);
+                const int cs = cell_case(m, k);
+                if (idx >= (uint32_t) j0)
+                    s.list[idx - j0] = ((uint64_t) u << 32) | ((uint32_t) cs << 24) | ((uint32_t) k << 16) | fo;
+                fo += (uint32_t) (s.tris[cs] & 15);
                 ++idx;
             }
         }
         __syncthreads();
         if (j0 + tid < nactive) {
-            const uint32_t ent = s.list[tid];
-            const int u = (int) (ent >> 15), k = (int) ((ent >> 9) & 63), fo = (int) (ent & 511);
+            const uint64_t ent = s.list[tid];
+            const int u = (int) (ent >> 32), cs = (int) ((ent >> 24) & 255), k = (int) ((ent >> 16) & 63);
+            const int fo = (int) (ent & 0xffff);
             const int jc = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
             const int zl = 64 * jc + k;
             const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + zl;
-            const int cs = cell_case(unit_cells(s, cx, cy, jc), k);
             CellCtx cc;
             {
                 const float *r00 = p.values + (gx * p.Y + gy) * p.Z + gz;
@@ -658,12 +690,12 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                 cc.v[6] = r11[0];
                 cc.v[7] = r11[1];
             }
-            cc.c[0][0] = grid_coord(gx, p.X, p.amin[0], p.amax[0]);
-            cc.c[0][1] = grid_coord(gx + 1, p.X, p.amin[0], p.amax[0]);
-            cc.c[1][0] = grid_coord(gy, p.Y, p.amin[1], p.amax[1]);
-            cc.c[1][1] = grid_coord(gy + 1, p.Y, p.amin[1], p.amax[1]);
-            cc.c[2][0] = grid_coord(gz, p.Z, p.amin[2], p.amax[2]);
-            cc.c[2][1] = grid_coord(gz + 1, p.Z, p.amin[2], p.amax[2]);
+            cc.c[0][0] = s.cx[cx];
+            cc.c[0][1] = s.cx[cx + 1];
+            cc.c[1][0] = s.cy[cy];
+            cc.c[1][1] = s.cy[cy + 1];
+            cc.c[2][0] = s.cz[zl];
+            cc.c[2][1] = s.cz[zl + 1];
             cc.level = lvl;
             const uint32_t status = (uint32_t) crossing_status(cs);
             // weld candidates: a vertex within a few ulps of a grid corner
@@ -854,21 +886,10 @@ __global__ void __launch_bounds__(256) k_sp_gather(McParams p) { sp_gather_seg(p
 // ---- ragged batches ----------------------------------------------------------------
 // Every kernel walks the batch-wide index space of its stage (classify words,
 // count blocks, emit tiles) and finds the owning item by binary search over
-// the items' start offsets; the per-item work is the single-grid code.
+// the items' start offsets, staged in LDS; the per-item work is the
+// single-grid code.
 
-// Last item whose start (field 0 word0, 1 blk0, 2 tile0) is <= x.
-__device__ __forceinline__ int find_item(const McBatchItem *it, int n, int64_t x, int which) {
-    int lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        const int64_t st = which == 0 ? it[mid].word0 : (which == 1 ? it[mid].blk0 : it[mid].tile0);
-        if (st <= x) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// The same over item starts staged in LDS (batches hold at most kMaxBatch items).
+// Last item whose start is <= x (starts staged in LDS; at most kMaxBatch items).
 __device__ __forceinline__ int find_start(const int64_t *st, int n, int64_t x) {
     int lo = 0, hi = n;
     while (hi - lo > 1) {
@@ -897,82 +918,39 @@ __global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, c
     }
 }
 
-__global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *items, const McBatchHdr *hdr) {
-    constexpr int U = 4;
+// Tasks of kClassifyWords words over the batch word space; item starts are
+// multiples of kClassifyWords, so each task lies in one grid.
+__global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *__restrict__ items,
+                                                        const McBatchHdr *__restrict__ hdr) {
     __shared__ int64_t starts[kMaxBatch];
     const int n = (int) hdr->n;
-    const int64_t nwords = hdr->nwords;
+    const int64_t ntask = hdr->nwords / kClassifyWords;
     for (int i = threadIdx.x; i < n; i += 256) starts[i] = items[i].word0;
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int grp = lane >> 4, q = lane & 15;
-    const int64_t wave = (int64_t) blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x * 4 + (threadIdx.x >> 6)));
     const int64_t nwave = (int64_t) gridDim.x * 4;
-    for (int64_t base = wave * (4 * U); base < nwords; base += nwave * (4 * U)) {
-        float v[U][4];
-        uint32_t valid[U];
-        int item[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t gw = base + 4 * u + grp;
-            v[u][0] = v[u][1] = v[u][2] = v[u][3] = 0.0f;
-            valid[u] = 0;
-            item[u] = 0;
-            if (gw < nwords) {
-                const int i = find_start(starts, n, gw);
-                item[u] = i;
-                const McBatchItem &it = items[i];
-                const int64_t w = gw - it.word0;
-                const int64_t Z = it.p.Z, W = it.p.W;
-                const int64_t row = w / W, c = w - row * W;
-                const int64_t z = 64 * c + 4 * q;
-                const int64_t left = Z - z;
-                const float *src = it.p.values + row * Z + z;
-                if (left > 0) {
-                    if (it.mode <= 1) {
-                        const float4 f = *reinterpret_cast<const float4 *>(src);
-                        v[u][0] = f.x;
-                        v[u][1] = f.y;
-                        v[u][2] = f.z;
-                        v[u][3] = f.w;
-                        valid[u] = 15;
-                    } else {
-#pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            if (j < left) v[u][j] = src[j];
-                        valid[u] = left >= 4 ? 15u : ((1u << left) - 1);
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int64_t gw = base + 4 * u + grp;
-            const float level = gw < nwords ? items[item[u]].p.level : 0.0f;
-            uint32_t x = 0;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) x |= (uint32_t) (((valid[u] >> j) & 1) && inside(v[u][j], level)) << j;
-            x |= __shfl_down(x, 1, 16) << 4;
-            x |= __shfl_down(x, 2, 16) << 8;
-            x |= __shfl_down(x, 4, 16) << 16;
-            const uint32_t hi = __shfl_down(x, 8, 16);
-            if (q == 0 && gw < nwords) {
-                const McBatchItem &it = items[item[u]];
-                it.p.bits[gw - it.word0] = (uint64_t) x | ((uint64_t) hi << 32);
-            }
-        }
+    for (int64_t t = wave; t < ntask; t += nwave) {
+        const int64_t gw = t * kClassifyWords;
+        const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, gw));
+        const McParams &p = items[i].p;
+        const int64_t w0 = gw - items[i].word0;
+        const int64_t nw = p.X * p.Y * p.W;
+        if (w0 < nw) classify_task(p.values, p.Z, p.W, nw, p.level, w0, p.bits);
     }
 }
 
+// One workgroup per count block (grid = the batch's block count); the
+// item comes from a search over the item starts staged in LDS.
 __global__ void __launch_bounds__(256) k_count_batch(const McBatchItem *__restrict__ items,
                                                      const McBatchHdr *__restrict__ hdr) {
+    __shared__ int64_t starts[kMaxBatch];
     const int n = (int) hdr->n;
-    for (int64_t gb = blockIdx.x; gb < hdr->nblk; gb += gridDim.x) {
-        const int i = __builtin_amdgcn_readfirstlane(find_item(items, n, gb, 1));
-        const McParams p = items[i].p;
-        __syncthreads();
-        count_block(p, gb - items[i].blk0);
-    }
+    const int64_t gb = blockIdx.x;
+    if (gb >= hdr->nblk) return;
+    for (int i = threadIdx.x; i < n; i += 256) starts[i] = items[i].blk0;
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, gb));
+    count_block(items[i].p, gb - items[i].blk0);
 }
 
 __global__ void __launch_bounds__(1024) k_scan_top_batch(const McBatchItem *__restrict__ items,
@@ -984,17 +962,19 @@ __global__ void __launch_bounds__(1024) k_scan_top_batch(const McBatchItem *__re
     }
 }
 
+// One workgroup per emit tile (grid = the batch's tile count).
 template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_emit_batch(const McBatchItem *__restrict__ items,
                                                    const McBatchHdr *__restrict__ hdr) {
+    __shared__ int64_t starts[kMaxBatch];
     const int n = (int) hdr->n;
-    for (int64_t gt = blockIdx.x; gt < hdr->ntiles; gt += gridDim.x) {
-        const int i = __builtin_amdgcn_readfirstlane(find_item(items, n, gt, 2));
-        const McParams p = items[i].p;   // uniform copy: scalar registers
-        const int64_t t = gt - items[i].tile0;
-        __syncthreads();
-        emit_tile<TX, TY, NZC, NT>(p, t, (int) (t % p.seg.nseg));
-    }
+    const int64_t gt = blockIdx.x;
+    if (gt >= hdr->ntiles) return;
+    for (int i = threadIdx.x; i < n; i += NT) starts[i] = items[i].tile0;
+    __syncthreads();
+    const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, gt));
+    const int64_t t = gt - items[i].tile0;
+    emit_tile<TX, TY, NZC, NT>(items[i].p, t, (int) (t % items[i].p.seg.nseg));
 }
 
 // grid (segments, items)
@@ -1265,6 +1245,21 @@ hipError_t mc_upload_luts() {
     return e;
 }
 
+// Inside masks of a whole grid: dwordx4 loads when rows are whole aligned
+// words, the lane-per-point kernel otherwise.
+static void launch_classify(const McParams &p, hipStream_t stream) {
+    const int64_t nwords = p.X * p.Y * p.W;
+    if (p.vec4 && p.Z % 64 == 0) {
+        const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nwords + 63) / 64, 8192));
+        hipLaunchKernelGGL(k_classify_v4, dim3(blocks), dim3(256), 0, stream, p.values, nwords, p.level, p.bits);
+    } else {
+        const int64_t ntask = (nwords + kClassifyWords - 1) / kClassifyWords;
+        const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((ntask + 3) / 4, 8192));
+        hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
+                           p.bits);
+    }
+}
+
 hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
                            hipEvent_t *ev) {
     hipError_t e = mc_upload_luts();
@@ -1274,17 +1269,7 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     hipLaunchKernelGGL(mc_dense_reset, dim3((4 * kSpSeg + 255) / 256), dim3(256), 0, stream, p, sp_init);
     if ((e = hipGetLastError()) != hipSuccess) return e;
 
-    const int64_t nwords = p.X * p.Y * p.W;
-    const unsigned cblocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nwords + 63) / 64, 8192));
-    if (p.vec4 && p.Z % 64 == 0)
-        hipLaunchKernelGGL(k_classify<0>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
-                           p.bits);
-    else if (p.vec4)
-        hipLaunchKernelGGL(k_classify<1>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
-                           p.bits);
-    else
-        hipLaunchKernelGGL(k_classify<2>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
-                           p.bits);
+    launch_classify(p, stream);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(1)) != hipSuccess) return e;
 
@@ -1316,17 +1301,7 @@ hipError_t its_dense_count_launch(McParams &p, hipStream_t stream) {
     hipError_t e = mc_upload_luts();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_its_reset, dim3(1), dim3(64), 0, stream, p.err, p.totals);
-    const int64_t nwords = p.X * p.Y * p.W;
-    const unsigned cblocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nwords + 63) / 64, 8192));
-    if (p.vec4 && p.Z % 64 == 0)
-        hipLaunchKernelGGL(k_classify<0>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
-                           p.bits);
-    else if (p.vec4)
-        hipLaunchKernelGGL(k_classify<1>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
-                           p.bits);
-    else
-        hipLaunchKernelGGL(k_classify<2>, dim3(cblocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
-                           p.bits);
+    launch_classify(p, stream);
     hipLaunchKernelGGL(k_count, dim3((unsigned) p.nblk), dim3(256), 0, stream, p);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, p);
     return hipGetLastError();
@@ -1354,10 +1329,10 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     const unsigned rb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.n * 4 * kSpSeg + 255) / 256, 1024));
     hipLaunchKernelGGL(k_reset_batch, dim3(rb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const unsigned cb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.nwords + 63) / 64, 8192));
+    const unsigned cb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.nwords / kClassifyWords + 3) / 4, 8192));
     hipLaunchKernelGGL(k_classify_batch, dim3(cb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const unsigned nb = (unsigned) std::max<int64_t>(1, std::min<int64_t>(h.nblk, 8192));
+    const unsigned nb = (unsigned) std::max<int64_t>(1, h.nblk);
     hipLaunchKernelGGL(k_count_batch, dim3(nb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const unsigned sb = (unsigned) std::max<int64_t>(1, std::min<int64_t>(h.n, 256));
@@ -1371,7 +1346,7 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
 hipError_t mc_batch_emit_launch(int cfg, const McBatchItem *items, const McBatchHdr *hdr, const McBatchHdr &h,
                                 McBatchSummary *summary, hipStream_t stream) {
     hipError_t e = hipSuccess;
-    const unsigned tb = (unsigned) std::max<int64_t>(1, std::min<int64_t>(h.ntiles, 4096));
+    const unsigned tb = (unsigned) std::max<int64_t>(1, h.ntiles);
     switch (cfg) {
 #define X(id, tx, ty, nzc, nt)                                                                           \
     case id:                                                                                             \

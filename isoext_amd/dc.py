@@ -49,8 +49,9 @@ _lib.declare("isoext_its_count", ctypes.c_int, [P(IsoGrid), f32, vp, i64, i64 * 
 _lib.declare("isoext_its_emit", ctypes.c_int, [P(IsoGrid), f32, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp])
 _lib.declare("isoext_its_normals", ctypes.c_int, [P(IsoGrid), P(IsoIts), vp, vp])
 _lib.declare("isoext_dc_workspace_bytes", i64, [i64, i64])
-_lib.declare("isoext_dc", ctypes.c_int, [P(IsoGrid), P(IsoIts), f32, f32, vp, i64, vp, i64, vp, i64, vp,
+_lib.declare("isoext_dc", ctypes.c_int, [P(IsoGrid), P(IsoIts), f32, f32, vp, i64, vp, i64, vp, i64, vp, vp,
                                          _lib.I64x4, vp])
+_lib.declare("isoext_dc_backward", ctypes.c_int, [P(IsoGrid), P(IsoIts), f32, f32, f32, i64, vp, vp, vp, vp, vp])
 _lib.declare("isoext_mc_sparse_count", ctypes.c_int, [P(IsoGrid), f32, vp, P(i64), vp])
 _lib.declare("isoext_mc_sparse", ctypes.c_int, [P(IsoGrid), f32, ctypes.c_int, i64, vp, i64, vp, i64, vp, i64, vp,
                                                 i64 * 2, vp])
@@ -180,8 +181,10 @@ def compute_normals(grid, its: Intersection) -> torch.Tensor:
     return out
 
 
-def dc_raw(grid, its: Intersection, normals: torch.Tensor, reg: float, svd_tol: float, want_dual: bool = False):
-    """Run the DC kernels; returns (v, f, info[, dual_v])."""
+def dc_raw(grid, its: Intersection, normals: torch.Tensor, reg: float, svd_tol: float, want_dual: bool = False,
+           want_vcell: bool = False):
+    """Run the DC kernels; returns (v, f, info[, dual_v]); with want_vcell,
+    info["vcell"] holds the active cell of each output vertex."""
     dev = _device(grid)
     L = _lib.lib()
     g = grid_struct(grid)
@@ -191,14 +194,18 @@ def dc_raw(grid, its: Intersection, normals: torch.Tensor, reg: float, svd_tol: 
     out_v = torch.empty((max(nc, 1), 3), dtype=torch.float32, device=dev)
     out_f = torch.empty((max(2 * npt, 1), 3), dtype=torch.int32, device=dev)
     dual = torch.empty((max(nc, 1), 3), dtype=torch.float32, device=dev) if want_dual else None
+    vcell = torch.empty(max(nc, 1), dtype=torch.int32, device=dev) if want_vcell else None
     counts = _lib.I64x4()
     with torch.cuda.device(dev):
         _lib.check(L.isoext_dc(ctypes.byref(g), ctypes.byref(s), float(reg), float(svd_tol), ws.data_ptr(),
                                ws.numel(), out_v.data_ptr(), out_v.shape[0], out_f.data_ptr(), out_f.shape[0],
-                               dual.data_ptr() if dual is not None else None, counts, _lib.stream_ptr(dev)),
+                               dual.data_ptr() if dual is not None else None,
+                               vcell.data_ptr() if vcell is not None else None, counts, _lib.stream_ptr(dev)),
                    "dual_contouring")
     nv, nf = int(counts[0]), int(counts[1])
     info = {"fp64_resolves": int(counts[2]), "quads": int(counts[3])}
+    if want_vcell:
+        info["vcell"] = vcell[:nv]
     v, f = out_v[:nv], out_f[:nf]
     if out_v.shape[0] > 2 * nv + 65536:
         v = v.clone()
