@@ -121,6 +121,31 @@ bool grid_well_separated(const int64_t shape[3], const float *amin, const float 
     return true;
 }
 
+// Bound on the interpolation parameter t of a vertex that lies within 16
+// ulps of an edge endpoint, with a 2x margin: |r - a| <= 32 ulp(M) near an
+// endpoint, |r - (a + t h)| <= 4 ulp(M) from the rounding of the lerp (M the
+// largest coordinate magnitude of the axis, h its smallest spacing), so
+// t <= 36 ulp(M) / h, and 1 - t is known to within 2^-22 from the operands
+// the kernel compares.  Infinite when an axis has no positive spacing.
+float weld_cand_eps(const int64_t shape[3], const float *amin, const float *amax) {
+    float eps = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        if (shape[a] < 2) continue;
+        float m = std::max(std::fabs(amin[a]), std::fabs(amax[a])), hmin = INFINITY;
+        float prev = iso::grid_coord(0, shape[a], amin[a], amax[a]);
+        for (int64_t i = 1; i < shape[a]; ++i) {
+            const float cur = iso::grid_coord(i, shape[a], amin[a], amax[a]);
+            hmin = std::min(hmin, cur - prev);
+            m = std::max(m, std::fabs(cur));
+            prev = cur;
+        }
+        if (!(hmin > 0.0f) || !std::isfinite(m)) return INFINITY;
+        const float ulp = std::nextafter(m, INFINITY) - m;
+        eps = std::max(eps, 72.0f * ulp / hmin + 0x1p-20f);
+    }
+    return eps;
+}
+
 // Mask-pipeline parameters of the intersection pass on a dense grid (the
 // scratch holds the masks, count arrays and totals; cap_* bound the writes).
 iso::McParams its_params(const isoext_grid *g, float level, char *ws, int64_t cap_cells, int64_t cap_pts) {
@@ -342,6 +367,7 @@ void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t 
     p.level = in.level;
     p.method = in.method;
     p.all_special = L.cells && !grid_well_separated(in.shape, in.aabb_min, in.aabb_max) ? 1 : 0;
+    p.cand_eps = weld_cand_eps(in.shape, in.aabb_min, in.aabb_max);
     p.vec4 = (in.shape[2] % 4 == 0 && ((uintptr_t) in.values & 15) == 0) ? 1 : 0;
     p.W = L.W;
     p.S = L.S;
@@ -645,6 +671,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     p.level = level;
     p.method = method;
     p.all_special = grid_well_separated(shape, aabb_min, aabb_max) ? 0 : 1;
+    p.cand_eps = weld_cand_eps(shape, aabb_min, aabb_max);
     p.vec4 = (shape[2] % 4 == 0 && ((uintptr_t) values & 15) == 0) ? 1 : 0;
     p.W = L.W;
     p.nzc = L.nzc;

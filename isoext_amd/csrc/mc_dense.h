@@ -75,6 +75,7 @@ struct McParams {
     float level;
     int method;        // 0 = nagae, 1 = lorensen
     int all_special;   // treat every vertex as a weld candidate
+    float cand_eps;    // weld-candidate prefilter: t within cand_eps of 0 or 1
     int vec4;          // Z % 4 == 0 and values 16-byte aligned: rows load as float4
     int its;           // count pass: 0 marching cubes (vertices, faces), 1 intersections
                        // (active cells, crossing edges)

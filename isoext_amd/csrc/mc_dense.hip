@@ -448,13 +448,16 @@ template <int TX, int TY, int NZC, int NT>
 struct EmitLds {
     static constexpr int PX = TX + 2, PY = TY + 2, PC = NZC + 1;   // mask patch
     static constexpr int QY = TY + 1, QC = NZC + 1;                // vertex bases
-    static constexpr int NUNITS = TX * TY * NZC;                   // (cell row, chunk)
+    static constexpr int NUNITS = TX * TY * NZC;                   // (cell row, chunk), chunk fastest
     uint64_t w[PX * PY * PC];
+    uint64_t amask[NUNITS];           // active cells of each unit
     int32_t vb[(TX + 1) * QY * QC];   // first vertex id per (point row, chunk)
-    int32_t fb[NUNITS];               // first face id per unit
+    int32_t fb[TX * TY];              // first face id per cell row (this z-segment)
     uint32_t na[NUNITS + 1];          // active cells per unit -> exclusive offsets
-    uint64_t list[NT];                // batch: unit << 32 | case << 24 | z-bit << 16 | face offset in unit
+    int32_t unit[NT];                 // batch: unit of each slot
+    uint32_t fsc[NT + 1];             // batch: faces per slot -> exclusive offsets
     uint32_t tsum[NT / 64];
+    int32_t carry[2];                 // faces of the straddling row's cells before a batch
     uint64_t tris[256];               // the method's packed triangle table
     float cx[TX + 1], cy[TY + 1], cz[64 * NZC + 1];   // grid coordinates of the tile's points
 };
@@ -474,21 +477,22 @@ __device__ __forceinline__ CellMasks unit_cells(const EmitLds<TX, TY, NZC, NT> &
 
 // Global id of the vertex on edge (owner point (opx, opy, oz) tile-local,
 // axis): the (point row, chunk) base plus the rank among the row's crossing
-// edges in (z, axis) order.
+// edges in (z, axis) order.  hx / hy: tile-local point rows below which the
+// row at x+1 / y+1 exists.
 template <int TX, int TY, int NZC, int NT>
-__device__ __forceinline__ int64_t vertex_id(const EmitLds<TX, TY, NZC, NT> &s, const McParams &p, int64_t x0,
-                                             int64_t y0, int opx, int opy, int oz, int axis) {
+__device__ __forceinline__ int vertex_id(const EmitLds<TX, TY, NZC, NT> &s, int hx, int hy, int opx, int opy, int oz,
+                                         int axis) {
     using L = EmitLds<TX, TY, NZC, NT>;
     const int j = oz >> 6, k = oz & 63;
     const uint64_t b = PW(s, opx, opy, j);
-    const uint64_t xm = (x0 + opx + 1 < p.X) ? (b ^ PW(s, opx + 1, opy, j)) : 0ull;
-    const uint64_t ym = (y0 + opy + 1 < p.Y) ? (b ^ PW(s, opx, opy + 1, j)) : 0ull;
+    const uint64_t xm = opx < hx ? (b ^ PW(s, opx + 1, opy, j)) : 0ull;
+    const uint64_t ym = opy < hy ? (b ^ PW(s, opx, opy + 1, j)) : 0ull;
     const uint64_t zm = b ^ (b >> 1);   // only bits below k are used
-    const uint64_t low = k ? (~0ull >> (64 - k)) : 0ull;
+    const uint64_t low = (1ull << k) - 1;
     int r = __popcll(xm & low) + __popcll(ym & low) + __popcll(zm & low);
     if (axis >= 1) r += (int) ((xm >> k) & 1);
     if (axis == 2) r += (int) ((ym >> k) & 1);
-    return (int64_t) s.vb[(opx * L::QY + opy) * L::QC + j] + r;
+    return s.vb[(opx * L::QY + opy) * L::QC + j] + r;
 }
 
 template <int N, int NT>
@@ -522,6 +526,61 @@ __device__ __forceinline__ void block_exscan_lds(uint32_t *a, uint32_t *tsum) {
     __syncthreads();
 }
 
+// Inclusive running maximum over a[0, NT) (one element per thread).
+template <int NT>
+__device__ __forceinline__ void block_maxscan_lds(int32_t *a, uint32_t *tsum) {
+    constexpr int NW = NT / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int32_t m = a[tid];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(m, o);
+        if (lane >= o) m = max(m, y);
+    }
+    if (lane == 63) tsum[wave] = (uint32_t) m;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+        if (w < wave) m = max(m, (int32_t) tsum[w]);
+    a[tid] = m;
+    __syncthreads();
+}
+
+// Position of the r-th set bit (r < popcount(m)).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
+    int pos = 0;
+    uint32_t x = (uint32_t) m;
+    int c = __popc(x);
+    if (r >= c) {
+        r -= c;
+        x = (uint32_t) (m >> 32);
+        pos = 32;
+    }
+#pragma unroll
+    for (int half = 16; half >= 1; half >>= 1) {
+        c = __popc(x & ((1u << half) - 1));
+        if (r >= c) {
+            r -= c;
+            x >>= half;
+            pos += half;
+        }
+    }
+    return pos;
+}
+
+// Weld-candidate test with a division-free prefilter: an edge's vertex can
+// lie within 16 ulps of an endpoint coordinate only when its parameter t is
+// within cand_eps of 0 or 1, i.e. |level - va| or |vb - level| is at most
+// cand_eps * |vb - va| (cand_eps holds a 2x margin over the rounding of
+// both sides); the exact test runs for the edges that pass.
+template <int E>
+__device__ __forceinline__ bool edge_maybe_candidate(const CellCtx &cc, float eps) {
+    constexpr int ca = edge_a(E), cb = edge_b(E);
+    const float d = fabsf(cc.v[cb] - cc.v[ca]) * eps;
+    // written so that NaN or infinite operands pass to the exact test
+    return !(fabsf(cc.level - cc.v[ca]) > d && fabsf(cc.v[cb] - cc.level) > d);
+}
+
 template <int TX, int TY, int NZC, int NT>
 __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int spseg) {
     using L = EmitLds<TX, TY, NZC, NT>;
@@ -533,6 +592,8 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
     const int64_t x0 = ti * TX, y0 = tj * TY, c0 = tk * NZC, z0 = 64 * c0;
     const int nx = (int) min<int64_t>(TX, p.X - 1 - x0);
     const int ny = (int) min<int64_t>(TY, p.Y - 1 - y0);
+    const int hx = (int) min<int64_t>(TX + 2, p.X - 1 - x0);   // point rows px < hx have a row at px + 1
+    const int hy = (int) min<int64_t>(TY + 2, p.Y - 1 - y0);
 
     // ---- mask patch: point rows [x0, x0+TX+2) x [y0, y0+TY+2), chunks [c0, c0+NZC];
     // all loads are issued before the first use, together with the global
@@ -584,45 +645,26 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
     }
     __syncthreads();
 
-    // ---- active cells and faces per unit (cell row, chunk) ----------------------
-    for (int i = tid; i < TX * TY; i += NT) {
-        const int cx = i / TY, cy = i % TY;
-        uint32_t *na = &s.na[i * NZC];
-        int32_t *fc = &s.fb[i * NZC];
-        for (int j = 0; j < NZC; ++j) {
-            na[j] = 0;
-            fc[j] = 0;
-            const int64_t zlo = z0 + 64 * j;
-            if (cx >= nx || cy >= ny || zlo >= p.Z - 1) continue;
-            const CellMasks m = unit_cells(s, cx, cy, j);
-            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - zlo);
-            na[j] = (uint32_t) __popcll(act);
-            int f = 0;
-            while (act) {
-                const int k = __ffsll((unsigned long long) act) - 1;
-                act &= act - 1;
-                f += (int) (s.tris[cell_case(m, k)] & 15);
-            }
-            fc[j] = f;
-        }
+    // ---- active cells per unit (cell row, chunk) --------------------------------
+    for (int u = tid; u < L::NUNITS; u += NT) {
+        const int j = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
+        const int64_t zlo = z0 + 64 * j;
+        uint64_t act = 0;
+        if (cx < nx && cy < ny && zlo < p.Z - 1)
+            act = cell_active(unit_cells(s, cx, cy, j)) & low_bits(p.Z - 1 - zlo);
+        s.amask[u] = act;
+        s.na[u] = (uint32_t) __popcll(act);
     }
     __syncthreads();
     block_exscan_lds<L::NUNITS, NT>(s.na, s.tsum);
     const int nactive = (int) s.na[L::NUNITS];
     if (nactive == 0) return;
 
-    // ---- global bases: faces per unit, vertices per (point row, chunk) ---------
+    // ---- global bases: faces per cell row, vertices per (point row, chunk) ------
 #pragma unroll
     for (int k = 0; k < CR; ++k) {
-        const int i = tid + k * NT, cx = i / TY, cy = i % TY;
-        if (i >= TX * TY || cx >= nx || cy >= ny) continue;
-        int64_t base = fbase_pf[k];
-        int32_t *fc = &s.fb[i * NZC];
-        for (int j = 0; j < NZC; ++j) {
-            const int32_t f = fc[j];
-            fc[j] = (int32_t) base;
-            base += f;
-        }
+        const int i = tid + k * NT;
+        if (i < TX * TY) s.fb[i] = (int32_t) fbase_pf[k];
     }
 #pragma unroll
     for (int k = 0; k < PR; ++k) {
@@ -641,38 +683,59 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
         }
         vb[NZC] = (tk + 1 < p.S) ? (int32_t) vnext_pf[k] : (int32_t) base;
     }
-    __syncthreads();
+    if (tid == 0) s.carry[0] = 0;
 
     // ---- emit, NT active cells per batch ----------------------------------------
     const float lvl = p.level;
-    for (int j0 = 0; j0 < nactive; j0 += NT) {
-        // fill the batch list: unit, z-bit and face offset inside the unit
+    for (int j0 = 0, bt = 0; j0 < nactive; j0 += NT, bt ^= 1) {
+        // slot -> unit: each unit overlapping the batch marks its first slot,
+        // a running maximum carries it over the unit's other slots
+        s.unit[tid] = -1;
+        __syncthreads();
         for (int u = tid; u < L::NUNITS; u += NT) {
-            const uint32_t a0 = s.na[u], a1 = s.na[u + 1];
-            // empty units include those outside the tile, whose patch cells
-            // may look active: they must not write the list
-            if (a1 == a0 || a1 <= (uint32_t) j0 || a0 >= (uint32_t) (j0 + NT)) continue;
-            const int j = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
-            const CellMasks m = unit_cells(s, cx, cy, j);
-            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - (z0 + 64 * j));
-            const uint32_t end = min(a1, (uint32_t) (j0 + NT));
-            uint32_t idx = a0, fo = 0;
-            while (act && idx < end) {
-                const int k = __ffsll((unsigned long long) act) - 1;
-                act &= act - 1;
-                const int cs = cell_case(m, k);
-                if (idx >= (uint32_t) j0)
-                    s.list[idx - j0] = ((uint64_t) u << 32) | ((uint32_t) cs << 24) | ((uint32_t) k << 16) | fo;
-                fo += (uint32_t) (s.tris[cs] & 15);
-                ++idx;
-            }
+            const int a0 = (int) s.na[u], a1 = (int) s.na[u + 1];
+            if (a1 > a0 && a0 < j0 + NT && a1 > j0) s.unit[max(a0, j0) - j0] = u;
         }
         __syncthreads();
-        if (j0 + tid < nactive) {
-            const uint64_t ent = s.list[tid];
-            const int u = (int) (ent >> 32), cs = (int) ((ent >> 24) & 255), k = (int) ((ent >> 16) & 63);
-            const int fo = (int) (ent & 0xffff);
+        block_maxscan_lds<NT>(s.unit, s.tsum);
+
+        const int jj = j0 + tid;
+        const bool live = jj < nactive;
+        int u = 0, cs = 0, k = 0;
+        if (live) {
+            u = s.unit[tid];
+            k = nth_set_bit(s.amask[u], jj - (int) s.na[u]);
+            const int jc = u % NZC, r = u / NZC, cyy = r % TY, cxx = r / TY;
+            // corner k of the cell sits at (k>>2, (k>>1)&1, k&1)
+            const uint64_t w00 = PW(s, cxx, cyy, jc), w01 = PW(s, cxx, cyy + 1, jc);
+            const uint64_t w10 = PW(s, cxx + 1, cyy, jc), w11 = PW(s, cxx + 1, cyy + 1, jc);
+            uint32_t up;   // bit z+1 of the four rows (bit 0 of the next chunk when z is the chunk's last)
+            if (k < 63) {
+                up = (uint32_t) ((w00 >> (k + 1)) & 1) | (uint32_t) ((w01 >> (k + 1)) & 1) << 1 |
+                     (uint32_t) ((w10 >> (k + 1)) & 1) << 2 | (uint32_t) ((w11 >> (k + 1)) & 1) << 3;
+            } else {
+                up = (uint32_t) (PW(s, cxx, cyy, jc + 1) & 1) | (uint32_t) (PW(s, cxx, cyy + 1, jc + 1) & 1) << 1 |
+                     (uint32_t) (PW(s, cxx + 1, cyy, jc + 1) & 1) << 2 |
+                     (uint32_t) (PW(s, cxx + 1, cyy + 1, jc + 1) & 1) << 3;
+            }
+            cs = (int) (((w00 >> k) & 1) | ((up & 1) << 1) | (((w01 >> k) & 1) << 2) | (((up >> 1) & 1) << 3) |
+                        (((w10 >> k) & 1) << 4) | (((up >> 2) & 1) << 5) | (((w11 >> k) & 1) << 6) |
+                        (((up >> 3) & 1) << 7));
+        }
+        s.fsc[tid] = live ? (uint32_t) (s.tris[cs] & 15) : 0u;
+        __syncthreads();
+        block_exscan_lds<NT, NT>(s.fsc, s.tsum);
+        // faces of the straddling row's cells before the next batch
+        if (tid == NT - 1 && j0 + NT < nactive) {
+            const int rl = s.unit[NT - 1] / NZC;
+            const int rs = (int) s.na[rl * NZC], re = (int) s.na[(rl + 1) * NZC];
+            s.carry[bt ^ 1] = re > j0 + NT ? (rs < j0 ? s.carry[bt] : 0) + (int) (s.fsc[NT] - s.fsc[max(rs - j0, 0)])
+                                           : 0;
+        }
+        if (live) {
             const int jc = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
+            const int rs = (int) s.na[r * NZC];   // the row's first active cell
+            const int fo = (int) (s.fsc[tid] - s.fsc[max(rs - j0, 0)]) + (rs < j0 ? s.carry[bt] : 0);
             const int zl = 64 * jc + k;
             const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + zl;
             CellCtx cc;
@@ -703,9 +766,15 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             if (p.all_special) {
                 candm = status;
             } else {
+                uint32_t maybe = 0;
                 for_edges([&](auto E) {
-                    if (((status >> E) & 1) && edge_is_candidate<E>(cc)) candm |= 1u << E;
+                    if (((status >> E) & 1) && edge_maybe_candidate<E>(cc, p.cand_eps)) maybe |= 1u << E;
                 });
+                if (maybe) {
+                    for_edges([&](auto E) {
+                        if (((maybe >> E) & 1) && edge_is_candidate<E>(cc)) candm |= 1u << E;
+                    });
+                }
             }
             const bool special = candm != 0;
             // vertices owned by this cell: the edges of its lowest corner, plus
@@ -723,7 +792,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                 for_edges([&](auto E) {
                     if (!((m >> E) & 1)) return;
                     constexpr int ca = edge_a(E);
-                    const int64_t v = vertex_id(s, p, x0, y0, cx + (ca >> 2), cy + ((ca >> 1) & 1), zl + (ca & 1),
+                    const int64_t v = vertex_id(s, hx, hy, cx + (ca >> 2), cy + ((ca >> 1) & 1), zl + (ca & 1),
                                                 edge_axis(E));
                     float x, y, z;
                     edge_pos_c<E>(cc, x, y, z);
@@ -749,7 +818,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             // faces
             const uint64_t wl = s.tris[cs];
             const int ntri = (int) (wl & 15);
-            int64_t slot = (int64_t) s.fb[u] + fo;
+            int64_t slot = (int64_t) s.fb[r] + fo;
             for (int q = 0; q < ntri; ++q, ++slot) {
                 const int e0 = (int) ((wl >> (4 + 12 * q)) & 15);
                 const int e1 = (int) ((wl >> (8 + 12 * q)) & 15);
@@ -771,12 +840,12 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                     int32_t *dst = p.out_f + 3 * slot;
                     if (keep) {
                         const int a0 = edge_a(e0), a1 = edge_a(e1), a2 = edge_a(e2);
-                        const int i0 = (int) vertex_id(s, p, x0, y0, cx + (a0 >> 2), cy + ((a0 >> 1) & 1),
-                                                       zl + (a0 & 1), edge_axis(e0));
-                        const int i1 = (int) vertex_id(s, p, x0, y0, cx + (a1 >> 2), cy + ((a1 >> 1) & 1),
-                                                       zl + (a1 & 1), edge_axis(e1));
-                        const int i2 = (int) vertex_id(s, p, x0, y0, cx + (a2 >> 2), cy + ((a2 >> 1) & 1),
-                                                       zl + (a2 & 1), edge_axis(e2));
+                        const int i0 = vertex_id(s, hx, hy, cx + (a0 >> 2), cy + ((a0 >> 1) & 1), zl + (a0 & 1),
+                                                 edge_axis(e0));
+                        const int i1 = vertex_id(s, hx, hy, cx + (a1 >> 2), cy + ((a1 >> 1) & 1), zl + (a1 & 1),
+                                                 edge_axis(e1));
+                        const int i2 = vertex_id(s, hx, hy, cx + (a2 >> 2), cy + ((a2 >> 1) & 1), zl + (a2 & 1),
+                                                 edge_axis(e2));
                         *reinterpret_cast<int3 *>(dst) = make_int3(i0, i1, i2);
                     } else {
                         *reinterpret_cast<int3 *>(dst) = make_int3(-1, -1, -1);
@@ -792,7 +861,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                     if (i >= 0) {
                         const int ca = edge_a(e);
                         SpecialEdge se;
-                        se.vid = vertex_id(s, p, x0, y0, cx + (ca >> 2), cy + ((ca >> 1) & 1), zl + (ca & 1),
+                        se.vid = vertex_id(s, hx, hy, cx + (ca >> 2), cy + ((ca >> 1) & 1), zl + (ca & 1),
                                            edge_axis(e));
                         se.edge = edge_key(p, gx, gy, gz, e);
                         p.seg.scell[i] = se;
