@@ -744,7 +744,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     if (need > special_cap)
         return fail(ISOEXT_E_WORKSPACE, "special-vertex lists need capacity %lld", (long long) need);
     if (sp_dev.n_cand > 0 || sp_dev.n_hole > 0) {
-        iso::FixupGrid g{shape[0], shape[1], shape[2]};
+        iso::FixupGrid g{shape[0], shape[1], shape[2], p.blk_v, p.blk_f, p.cnt_v, p.cnt_f, p.S};
         HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key), "weld fix-up");
         counts[0] = nv;
         counts[1] = nf;
@@ -956,9 +956,13 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
         }
         if (sum[i].n_cand > 0 || sum[i].n_hole > 0) {
             iso::SpecialLists sp_dev;
-            HIP_CHECK(hipMemcpy(&sp_dev, ws + B.it[i].o_hdr + kOffSp, sizeof(sp_dev), hipMemcpyDeviceToHost),
+            HIP_CHECK(hipMemcpyAsync(&sp_dev, ws + B.it[i].o_hdr + kOffSp, sizeof(sp_dev), hipMemcpyDeviceToHost, st),
                       "read special lists");
-            iso::FixupGrid g{items[i].shape[0], items[i].shape[1], items[i].shape[2]};
+            HIP_CHECK(hipStreamSynchronize(st), "read special lists");
+            const ItemLayout &IL = B.it[i];
+            iso::FixupGrid g{items[i].shape[0], items[i].shape[1], items[i].shape[2],
+                             (const int64_t *) (ws + IL.o_blk_v), (const int64_t *) (ws + IL.o_blk_f),
+                             (const uint32_t *) (ws + IL.o_cnt_v), (const uint32_t *) (ws + IL.o_cnt_f), IL.S};
             HIP_CHECK(iso::weld_fixup(sp_dev, items[i].out_v, items[i].out_f, nv, nf, g, st, nullptr), "weld fix-up");
             counts[2 * i] = nv;
             counts[2 * i + 1] = nf;

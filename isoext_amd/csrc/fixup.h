@@ -9,6 +9,13 @@ namespace iso {
 
 struct FixupGrid {
     int64_t X, Y, Z;   // points per axis
+    // Optional, dense point-major numbering: exclusive vertex / face prefix of
+    // (point row, z-segment) entry e = blk[e / kMcScanBlock] + cnt[e], S
+    // segments per row.  Lets the face remap skip the faces that cannot
+    // reference a relocated tail vertex.
+    const int64_t *blk_v = nullptr, *blk_f = nullptr;
+    const uint32_t *cnt_v = nullptr, *cnt_f = nullptr;
+    int64_t S = 0;
 };
 
 hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,

@@ -73,72 +73,75 @@ __device__ int edge_cell_count(int64_t edge, const FixupGrid &g) {
     return cnt;
 }
 
-__global__ void k_clear64(int64_t *p, int64_t n, int64_t v) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x)
-        p[i] = v;
-}
-__global__ void k_clear32(uint32_t *p, int64_t n, uint32_t v) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x)
-        p[i] = v;
+
+__device__ void insert_scell(const SpecialEdge &e, int64_t *keys, VInfo *info, int64_t mask) {
+    int64_t s = h1_find_or_insert(keys, mask, e.vid, true);
+    if (s < 0) return;
+    atomicAdd(&info[s].count_s, 1);
+    info[s].edge = e.edge;
 }
 
-__global__ void k_scell(const SpecialEdge *se, int64_t n, int64_t *keys, VInfo *info, int64_t mask) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x) {
-        int64_t s = h1_find_or_insert(keys, mask, se[i].vid, true);
-        if (s < 0) continue;
-        atomicAdd(&info[s].count_s, 1);
-        info[s].edge = se[i].edge;
-    }
-}
-
-__global__ void k_cand(const SpecialVertex *cv, int64_t n, int64_t *keys, VInfo *info, int64_t mask,
-                       int32_t *pslot, PGroup *groups, int64_t pmask) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x) {
-        const SpecialVertex v = cv[i];
-        int64_t s = h1_find_or_insert(keys, mask, v.vid, true);
-        // position group: slots hold the index of the first candidate seen
-        uint64_t h = f3_hash(v.x, v.y, v.z) & (uint64_t) pmask;
-        int64_t g = -1;
-        for (int64_t probe = 0; probe <= pmask; ++probe) {
-            int64_t slot = (int64_t) ((h + probe) & (uint64_t) pmask);
-            int32_t cur = pslot[slot];
-            if (cur < 0) {
-                int32_t prev = atomicCAS(&pslot[slot], -1, (int32_t) i);
-                if (prev < 0) {
-                    g = slot;
-                    break;
-                }
-                cur = prev;
-            }
-            const SpecialVertex o = cv[cur];
-            if (o.x == v.x && o.y == v.y && o.z == v.z) {
+__device__ void insert_cand(const SpecialVertex *cv, int64_t i, int64_t *keys, VInfo *info, int64_t mask,
+                            int32_t *pslot, PGroup *groups, int64_t pmask) {
+    const SpecialVertex v = cv[i];
+    int64_t s = h1_find_or_insert(keys, mask, v.vid, true);
+    // position group: slots hold the index of the first candidate seen
+    uint64_t h = f3_hash(v.x, v.y, v.z) & (uint64_t) pmask;
+    int64_t g = -1;
+    for (int64_t probe = 0; probe <= pmask; ++probe) {
+        int64_t slot = (int64_t) ((h + probe) & (uint64_t) pmask);
+        int32_t cur = pslot[slot];
+        if (cur < 0) {
+            int32_t prev = atomicCAS(&pslot[slot], -1, (int32_t) i);
+            if (prev < 0) {
                 g = slot;
                 break;
             }
+            cur = prev;
         }
-        if (g >= 0) atomicMin((long long *) &groups[g].rep, (long long) v.vid);
-        if (s >= 0) {
-            info[s].is_cand = 1;
-            info[s].group = (int) g;
-            info[s].edge = v.edge;
+        const SpecialVertex o = cv[cur];
+        if (o.x == v.x && o.y == v.y && o.z == v.z) {
+            g = slot;
+            break;
         }
+    }
+    if (g >= 0) atomicMin((long long *) &groups[g].rep, (long long) v.vid);
+    if (s >= 0) {
+        info[s].is_cand = 1;
+        info[s].group = (int) g;
+        info[s].edge = v.edge;
     }
 }
 
-__global__ void k_kept_refs(const int64_t *kept, int64_t n, const int32_t *faces, int64_t nf,
-                            int64_t *keys, VInfo *info, int64_t mask, uint32_t *special_face_bits) {
+// Crossing edges of special cells, then candidate vertices (one index space).
+__global__ void k_fix_insert(const SpecialEdge *se, int64_t nscell, const SpecialVertex *cv, int64_t ncand,
+                             int64_t *keys, VInfo *info, int64_t mask, int32_t *pslot, PGroup *groups,
+                             int64_t pmask) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nscell + ncand;
+         i += (int64_t) gridDim.x * blockDim.x) {
+        if (i < nscell) insert_scell(se[i], keys, info, mask);
+        else insert_cand(cv, i - nscell, keys, info, mask, pslot, groups, pmask);
+    }
+}
+
+// Kept faces of special cells: mark the vertices they reference, then point
+// their corners at the position group's representative (only these faces
+// can reference a non-representative).
+__global__ void k_kept_refs(const int64_t *kept, int64_t n, int32_t *faces, int64_t nf, int64_t *keys, VInfo *info,
+                            int64_t mask, const PGroup *groups) {
     for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t) gridDim.x * blockDim.x) {
         int64_t f = kept[i];
         if (f < 0 || f >= nf) continue;
-        atomicOr(&special_face_bits[f >> 5], 1u << (f & 31));
         for (int j = 0; j < 3; ++j) {
-            int64_t s = h1_find_or_insert(keys, mask, (int64_t) faces[3 * f + j], false);
-            if (s >= 0) info[s].referenced = 1;
+            const int64_t v = faces[3 * f + j];
+            int64_t s = h1_find_or_insert(keys, mask, v, false);
+            if (s < 0) continue;
+            info[s].referenced = 1;
+            if (info[s].is_cand && info[s].group >= 0) {
+                const int64_t rep = groups[info[s].group].rep;
+                if (rep != v) faces[3 * f + j] = (int32_t) rep;
+            }
         }
     }
 }
@@ -158,10 +161,18 @@ __global__ void k_used(const int64_t *keys, VInfo *info, int64_t nslots, PGroup 
     }
 }
 
+// Clears the keep bits of removed vertices (counted in *nremoved) and of
+// dropped faces.
 __global__ void k_removed(const int64_t *keys, const VInfo *info, int64_t nslots, const PGroup *groups,
-                          uint32_t *vkeep, int64_t nv, unsigned long long *nremoved) {
-    for (int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; s < nslots;
+                          uint32_t *vkeep, int64_t nv, unsigned long long *nremoved, const int64_t *hole,
+                          int64_t nhole, uint32_t *fkeep, int64_t nf) {
+    for (int64_t s = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; s < max(nslots, nhole);
          s += (int64_t) gridDim.x * blockDim.x) {
+        if (s < nhole) {
+            const int64_t f = hole[s];
+            if (f >= 0 && f < nf) atomicAnd(&fkeep[f >> 5], ~(1u << (f & 31)));
+        }
+        if (s >= nslots) continue;
         int64_t vid = keys[s];
         if (vid < 0 || vid >= nv) continue;
         const VInfo &v = info[s];
@@ -177,47 +188,40 @@ __global__ void k_removed(const int64_t *keys, const VInfo *info, int64_t nslots
     }
 }
 
-__global__ void k_holes(const int64_t *hole, int64_t n, uint32_t *fkeep, int64_t nf) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x) {
-        int64_t f = hole[i];
-        if (f < 0 || f >= nf) continue;
-        atomicAnd(&fkeep[f >> 5], ~(1u << (f & 31)));
-    }
-}
-
 __device__ __forceinline__ int64_t bit_rank(const uint32_t *bits, const int64_t *prefix, int64_t i) {
     uint32_t w = bits[i >> 5];
     uint32_t below = (i & 31) ? (w & ((1u << (i & 31)) - 1)) : 0u;
     return prefix[i >> 5] + __popc(below);
 }
 
-__global__ void k_tail_bits(uint32_t *vkeep, int64_t nv, uint32_t *fkeep, int64_t nf) {
-    if (threadIdx.x == 0) {
-        vkeep[nv >> 5] = (nv & 31) ? ((1u << (nv & 31)) - 1) : 0u;
-        vkeep[(nv >> 5) + 1] = 0u;
-        fkeep[nf >> 5] = (nf & 31) ? ((1u << (nf & 31)) - 1) : 0u;
-        fkeep[(nf >> 5) + 1] = 0u;
-    }
-}
 
-__global__ void k_init_groups(PGroup *g, int64_t n) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x) {
-        g[i].rep = INT64_MAX;
-        g[i].used = 0;
-        g[i].pad = 0;
-    }
-}
 
-__global__ void k_init_info(VInfo *v, int64_t n) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t) gridDim.x * blockDim.x) {
-        v[i].count_s = 0;
-        v[i].referenced = 0;
-        v[i].is_cand = 0;
-        v[i].group = -1;
-        v[i].edge = 0;
+
+// Resets the fix-up state in one launch: the vertex hash, the position
+// groups, the keep bitmaps (bits set for [0, n), clear beyond) and the
+// counters.
+__global__ void k_fix_init(int64_t *keys, VInfo *info, int64_t nslots, int32_t *pslot, PGroup *groups,
+                           int64_t npslots, uint32_t *vkeep, int64_t vw, int64_t nv, uint32_t *fkeep, int64_t fw,
+                           int64_t nf, int64_t *misc) {
+    const int64_t n = max(max(nslots, npslots), max(vw, fw));
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        if (i < nslots) {
+            keys[i] = kEmpty;
+            info[i].count_s = 0;
+            info[i].referenced = 0;
+            info[i].is_cand = 0;
+            info[i].group = -1;
+            info[i].edge = 0;
+        }
+        if (i < npslots) {
+            pslot[i] = -1;
+            groups[i].rep = INT64_MAX;
+            groups[i].used = 0;
+            groups[i].pad = 0;
+        }
+        if (i < vw) vkeep[i] = (i + 1) * 32 <= nv ? ~0u : (i * 32 < nv ? ((1u << (nv & 31)) - 1) : 0u);
+        if (i < fw) fkeep[i] = (i + 1) * 32 <= nf ? ~0u : (i * 32 < nf ? ((1u << (nf & 31)) - 1) : 0u);
+        if (i < 8) misc[i] = 0;
     }
 }
 
@@ -289,70 +293,86 @@ __device__ int64_t bit_select(const uint32_t *bits, const int64_t *pre, int64_t 
     return -1;
 }
 
-// Relocation pairs: the i-th removed slot below n_final takes the i-th kept
-// element at or above n_final (disjoint ranges, so moves do not race).
-__global__ void k_reloc_pairs(const uint32_t *keep, const int64_t *pre, int64_t nw, int64_t n_final, int64_t k,
-                              int64_t *src, int64_t *dst) {
-    // removed slots below n_final (the rest of the removed ones are in the tail)
+// Tail relocation: the i-th removed slot below n_final takes the i-th kept
+// element at or above n_final (disjoint ranges, so the moves do not race);
+// n_final is read from the device.  The pairs are recorded sorted by source
+// (src = INT64_MAX past the last pair) for the face remap.  Moves vertex
+// triples (and keys) when v is given, face triples when f is given.
+__global__ void k_reloc(const uint32_t *keep, const int64_t *pre, int64_t nw, int64_t n, const int64_t *n_final_p,
+                        int64_t kmax, int64_t *src, int64_t *dst, float *v, int64_t *vkeys, int32_t *f,
+                        int64_t *overflow) {
+    const int64_t n_final = *n_final_p, k = n - n_final;
+    if (k > kmax) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *overflow = 1;
+        return;
+    }
     const uint32_t wd = keep[n_final >> 5];
     const uint32_t below = (n_final & 31) ? (wd & ((1u << (n_final & 31)) - 1)) : 0u;
     const int64_t holes_low = n_final - (pre[n_final >> 5] + __popc(below));
     for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t) gridDim.x * blockDim.x) {
         if (i < holes_low) {
-            dst[i] = bit_select(keep, pre, nw, i, false);
-            src[i] = bit_select(keep, pre, nw, (n_final - holes_low) + i, true);
+            const int64_t d = bit_select(keep, pre, nw, i, false);
+            const int64_t sidx = bit_select(keep, pre, nw, (n_final - holes_low) + i, true);
+            dst[i] = d;
+            src[i] = sidx;
+            if (v) {
+                v[3 * d + 0] = v[3 * sidx + 0];
+                v[3 * d + 1] = v[3 * sidx + 1];
+                v[3 * d + 2] = v[3 * sidx + 2];
+                if (vkeys) vkeys[d] = vkeys[sidx];
+            }
+            if (f) {
+                f[3 * d + 0] = f[3 * sidx + 0];
+                f[3 * d + 1] = f[3 * sidx + 1];
+                f[3 * d + 2] = f[3 * sidx + 2];
+            }
         } else {
             dst[i] = -1;
-            src[i] = INT64_MAX;   // keeps src sorted for the remap's binary search
+            src[i] = INT64_MAX;
         }
     }
 }
 
-__global__ void k_move_v(float *v, int64_t *vkeys, const int64_t *src, const int64_t *dst, int64_t k) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t) gridDim.x * blockDim.x) {
-        int64_t s = src[i], d = dst[i];
-        if (d < 0 || s == INT64_MAX) continue;
-        v[3 * d + 0] = v[3 * s + 0];
-        v[3 * d + 1] = v[3 * s + 1];
-        v[3 * d + 2] = v[3 * s + 2];
-        if (vkeys) vkeys[d] = vkeys[s];
+// First face that can reference a vertex with id >= v under dense
+// point-major numbering: vertices owned by points of x-plane x are
+// referenced only by cells of x-planes x - 1 and x.
+__device__ int64_t first_face_for_tail(const FixupGrid &g, int64_t v) {
+    if (!g.blk_v) return 0;
+    auto prefix = [&](const int64_t *blk, const uint32_t *cnt, int64_t x) {
+        const int64_t e = x * g.Y * g.S;
+        return blk[e / kMcScanBlock] + (int64_t) cnt[e];
+    };
+    int64_t lo = 0, hi = g.X;   // prefix(lo) <= v
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (prefix(g.blk_v, g.cnt_v, mid) <= v) lo = mid;
+        else hi = mid;
     }
+    return lo >= 1 ? prefix(g.blk_f, g.cnt_f, lo - 1) : 0;
 }
 
-__global__ void k_move_f(int32_t *f, const int64_t *src, const int64_t *dst, int64_t k) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t) gridDim.x * blockDim.x) {
-        int64_t s = src[i], d = dst[i];
-        if (d < 0 || s == INT64_MAX) continue;
-        f[3 * d + 0] = f[3 * s + 0];
-        f[3 * d + 1] = f[3 * s + 1];
-        f[3 * d + 2] = f[3 * s + 2];
-    }
-}
-
-// Rewrites face ids in place: non-representatives -> representative (only
-// special faces can reference them), then tail vertices -> relocated slot.
-__global__ void k_remap_inplace(int32_t *f, int64_t nf, const uint32_t *fkeep, const uint32_t *special_face_bits,
-                                const int64_t *keys, const VInfo *info, int64_t mask, const PGroup *groups,
-                                int64_t nv_final, const int64_t *rsrc, const int64_t *rdst, int64_t k) {
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += (int64_t) gridDim.x * blockDim.x) {
-        if (!((fkeep[i >> 5] >> (i & 31)) & 1)) continue;
-        const bool sp = (special_face_bits[i >> 5] >> (i & 31)) & 1;
+// Face ids of relocated tail vertices -> their new slot (faces from the
+// first one that can reference the tail).
+__global__ void k_remap_tail(int32_t *f, int64_t nf, const int64_t *nv_final_p, int64_t nv, int64_t kmax,
+                             const int64_t *rsrc, const int64_t *rdst, FixupGrid g) {
+    const int64_t nv_final = *nv_final_p, k = nv - nv_final;
+    if (k <= 0 || k > kmax) return;   // k > kmax: flagged by k_reloc
+    __shared__ int64_t f0s;
+    if (threadIdx.x == 0) f0s = first_face_for_tail(g, nv_final);
+    __syncthreads();
+    const int64_t f0 = f0s;
+    for (int64_t i = f0 + (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < nf;
+         i += (int64_t) gridDim.x * blockDim.x) {
         for (int j = 0; j < 3; ++j) {
-            int64_t v0 = f[3 * i + j], v = v0;
-            if (sp) {
-                int64_t s = h1_find_or_insert(const_cast<int64_t *>(keys), mask, v, false);
-                if (s >= 0 && info[s].is_cand && info[s].group >= 0) v = groups[info[s].group].rep;
+            const int64_t v0 = f[3 * i + j];
+            if (v0 < nv_final) continue;
+            int64_t lo = 0, hi = k;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (rsrc[mid] < v0) lo = mid + 1;
+                else hi = mid;
             }
-            if (v >= nv_final) {
-                int64_t lo = 0, hi = k;
-                while (lo < hi) {
-                    int64_t mid = (lo + hi) >> 1;
-                    if (rsrc[mid] < v) lo = mid + 1;
-                    else hi = mid;
-                }
-                v = (lo < k && rsrc[lo] == v) ? rdst[lo] : -1;
-            }
-            if (v != v0) f[3 * i + j] = (int32_t) v;
+            f[3 * i + j] = (lo < k && rsrc[lo] == v0) ? (int32_t) rdst[lo] : -1;
         }
     }
 }
@@ -384,7 +404,8 @@ hipError_t bitmap_prefix(const uint32_t *bits, int64_t nw, int64_t *pre, int64_t
     } while (0)
 
 // Resolves the special lists in place.  nv / nf hold the provisional counts on
-// entry and the final counts on return.  Synchronises the stream.
+// entry and the final counts on return.  Synchronises the stream once, at
+// the end.
 hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
                       const FixupGrid &g, hipStream_t st, int64_t *vkeys) {
     const int64_t ncand = (int64_t) sp.n_cand, nscell = (int64_t) sp.n_scell;
@@ -400,16 +421,17 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
         bytes += (n + 255) & ~size_t(255);
         return o;
     };
-    const int64_t nrel = ncand + nscell + 16;
+    const int64_t nrel = ncand + nscell + 16;   // removed vertices are hashed vertices
+    const int64_t nfrel = nhole + 16;           // dropped faces are listed holes
     size_t o_keys = take(sizeof(int64_t) * nslots), o_info = take(sizeof(VInfo) * nslots);
     size_t o_pslot = take(sizeof(int32_t) * npslots), o_groups = take(sizeof(PGroup) * npslots);
     size_t o_vkeep = take(sizeof(uint32_t) * vw), o_fkeep = take(sizeof(uint32_t) * fw);
-    size_t o_sfb = take(sizeof(uint32_t) * fw);
     size_t o_vpre = take(sizeof(int64_t) * vw), o_fpre = take(sizeof(int64_t) * fw);
-    size_t o_scr = take(sizeof(int64_t) * ((std::max(vw, fw) + 1023) / 1024 + 1));
+    size_t o_vscr = take(sizeof(int64_t) * ((vw + 1023) / 1024 + 1));
+    size_t o_fscr = take(sizeof(int64_t) * ((fw + 1023) / 1024 + 1));
     size_t o_misc = take(sizeof(int64_t) * 8);
     size_t o_rv = take(sizeof(int64_t) * 2 * nrel);
-    size_t o_rf = take(sizeof(int64_t) * 2 * (nhole + 16));
+    size_t o_rf = take(sizeof(int64_t) * 2 * nfrel);
     static thread_local char *cache = nullptr;
     static thread_local size_t cache_bytes = 0;
     if (cache_bytes < bytes) {
@@ -429,65 +451,42 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     int32_t *pslot = (int32_t *) (base + o_pslot);
     PGroup *groups = (PGroup *) (base + o_groups);
     uint32_t *vkeep = (uint32_t *) (base + o_vkeep), *fkeep = (uint32_t *) (base + o_fkeep);
-    uint32_t *sfb = (uint32_t *) (base + o_sfb);
     int64_t *vpre = (int64_t *) (base + o_vpre), *fpre = (int64_t *) (base + o_fpre);
-    int64_t *scr = (int64_t *) (base + o_scr), *misc = (int64_t *) (base + o_misc);
+    int64_t *vscr = (int64_t *) (base + o_vscr), *fscr = (int64_t *) (base + o_fscr);
+    int64_t *misc = (int64_t *) (base + o_misc);   // 0 removed, 2 nv final, 3 nf final, 4 overflow
     int64_t *rv_src = (int64_t *) (base + o_rv), *rv_dst = rv_src + nrel;
-    int64_t *rf_src = (int64_t *) (base + o_rf), *rf_dst = rf_src + (nhole + 16);
+    int64_t *rf_src = (int64_t *) (base + o_rf), *rf_dst = rf_src + nfrel;
 
-    hipLaunchKernelGGL(k_clear64, dim3(grid_for(nslots)), dim3(256), 0, st, keys, nslots, kEmpty);
-    hipLaunchKernelGGL(k_init_info, dim3(grid_for(nslots)), dim3(256), 0, st, info, nslots);
-    hipLaunchKernelGGL(k_clear32, dim3(grid_for(npslots)), dim3(256), 0, st, (uint32_t *) pslot, npslots,
-                       0xffffffffu);
-    hipLaunchKernelGGL(k_init_groups, dim3(grid_for(npslots)), dim3(256), 0, st, groups, npslots);
-    // keep bitmaps: bits set for [0, n), clear beyond
-    hipLaunchKernelGGL(k_clear32, dim3(grid_for(vw)), dim3(256), 0, st, vkeep, vw, 0xffffffffu);
-    hipLaunchKernelGGL(k_clear32, dim3(grid_for(fw)), dim3(256), 0, st, fkeep, fw, 0xffffffffu);
-    hipLaunchKernelGGL(k_clear32, dim3(grid_for(fw)), dim3(256), 0, st, sfb, fw, 0u);
-    hipLaunchKernelGGL(k_clear64, dim3(1), dim3(64), 0, st, misc, 8, 0);
-    hipLaunchKernelGGL(k_tail_bits, dim3(1), dim3(64), 0, st, vkeep, nv, fkeep, nf);
-    ISO_TRY(hipGetLastError());
-    if (nscell)
-        hipLaunchKernelGGL(k_scell, dim3(grid_for(nscell)), dim3(256), 0, st, sp.scell, nscell, keys, info,
-                           mask);
-    if (ncand)
-        hipLaunchKernelGGL(k_cand, dim3(grid_for(ncand)), dim3(256), 0, st, sp.cand, ncand, keys, info, mask,
-                           pslot, groups, pmask);
+    hipLaunchKernelGGL(k_fix_init, dim3(grid_for(std::max(std::max(nslots, npslots), std::max(vw, fw)))), dim3(256),
+                       0, st, keys, info, nslots, pslot, groups, npslots, vkeep, vw, nv, fkeep, fw, nf, misc);
+    if (nscell + ncand)
+        hipLaunchKernelGGL(k_fix_insert, dim3(grid_for(nscell + ncand)), dim3(256), 0, st, sp.scell, nscell, sp.cand,
+                           ncand, keys, info, mask, pslot, groups, pmask);
     if (nkept)
-        hipLaunchKernelGGL(k_kept_refs, dim3(grid_for(nkept)), dim3(256), 0, st, sp.kept, nkept, f, nf,
-                           keys, info, mask, sfb);
+        hipLaunchKernelGGL(k_kept_refs, dim3(grid_for(nkept)), dim3(256), 0, st, sp.kept, nkept, f, nf, keys, info,
+                           mask, groups);
     hipLaunchKernelGGL(k_used, dim3(grid_for(nslots)), dim3(256), 0, st, keys, info, nslots, groups, g);
-    hipLaunchKernelGGL(k_removed, dim3(grid_for(nslots)), dim3(256), 0, st, keys, info, nslots, groups,
-                       vkeep, nv, (unsigned long long *) misc);
-    if (nhole) hipLaunchKernelGGL(k_holes, dim3(grid_for(nhole)), dim3(256), 0, st, sp.hole, nhole, fkeep, nf);
+    hipLaunchKernelGGL(k_removed, dim3(grid_for(std::max(nslots, nhole))), dim3(256), 0, st, keys, info, nslots,
+                       groups, vkeep, nv, (unsigned long long *) misc, sp.hole, nhole, fkeep, nf);
     ISO_TRY(hipGetLastError());
-    ISO_TRY(bitmap_prefix(vkeep, vw, vpre, scr, misc + 2, st));
-    ISO_TRY(bitmap_prefix(fkeep, fw, fpre, scr, misc + 3, st));
-    int64_t h[4];
+    ISO_TRY(bitmap_prefix(vkeep, vw, vpre, vscr, misc + 2, st));
+    ISO_TRY(bitmap_prefix(fkeep, fw, fpre, fscr, misc + 3, st));
+    // vertices: tail survivors fill the removed slots; faces: representatives
+    // were applied above, relocated tail vertices now; then tail faces fill
+    // the dropped slots
+    hipLaunchKernelGGL(k_reloc, dim3(grid_for(nrel)), dim3(256), 0, st, vkeep, vpre, vw, nv, misc + 2, nrel, rv_src,
+                       rv_dst, v, vkeys, (int32_t *) nullptr, misc + 4);
+    hipLaunchKernelGGL(k_remap_tail, dim3(grid_for(nf)), dim3(256), 0, st, f, nf, misc + 2, nv, nrel, rv_src, rv_dst,
+                       g);
+    hipLaunchKernelGGL(k_reloc, dim3(grid_for(nfrel)), dim3(256), 0, st, fkeep, fpre, fw, nf, misc + 3, nfrel,
+                       rf_src, rf_dst, (float *) nullptr, (int64_t *) nullptr, f, misc + 4);
+    ISO_TRY(hipGetLastError());
+    int64_t h[5];
     ISO_TRY(hipMemcpyAsync(h, misc, sizeof(h), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipStreamSynchronize(st));
-    const int64_t nv2 = h[2], nf2 = h[3];
-    const int64_t kv = nv - nv2, kf = nf - nf2;   // removed vertices / dropped faces
-    if (kv > nrel || kf > nhole + 16) return hipErrorInvalidValue;
-    // vertex relocation: tail survivors fill the removed slots
-    if (kv > 0) {
-        hipLaunchKernelGGL(k_reloc_pairs, dim3(grid_for(kv)), dim3(256), 0, st, vkeep, vpre, vw, nv2, kv, rv_src,
-                           rv_dst);
-        hipLaunchKernelGGL(k_move_v, dim3(grid_for(kv)), dim3(256), 0, st, v, vkeys, rv_src, rv_dst, kv);
-    }
-    // face ids: representatives, then relocated tail vertices
-    hipLaunchKernelGGL(k_remap_inplace, dim3(grid_for(nf)), dim3(256), 0, st, f, nf, fkeep, sfb, keys, info, mask,
-                       groups, nv2, rv_src, rv_dst, kv);
-    // face relocation: tail faces fill the dropped slots
-    if (kf > 0) {
-        hipLaunchKernelGGL(k_reloc_pairs, dim3(grid_for(kf)), dim3(256), 0, st, fkeep, fpre, fw, nf2, kf, rf_src,
-                           rf_dst);
-        hipLaunchKernelGGL(k_move_f, dim3(grid_for(kf)), dim3(256), 0, st, f, rf_src, rf_dst, kf);
-    }
-    ISO_TRY(hipGetLastError());
-    ISO_TRY(hipStreamSynchronize(st));
-    nv = nv2;
-    nf = nf2;
+    if (h[4]) return hipErrorInvalidValue;   // more relocations than the lists allow: inconsistent lists
+    nv = h[2];
+    nf = h[3];
     return hipSuccess;
 }
 
