@@ -110,6 +110,14 @@ def check(rc: int, what: str = "") -> None:
     raise IsoextError(f"{what}: {msg}" if what else msg)
 
 
+def on_device(dev):
+    """Context that makes `dev` current (a no-op when it already is)."""
+    import contextlib
+    if dev.index is None or dev.index == torch.cuda.current_device():
+        return contextlib.nullcontext()
+    return torch.cuda.device(dev)
+
+
 def stream_ptr(device=None) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 

@@ -146,6 +146,33 @@ float weld_cand_eps(const int64_t shape[3], const float *amin, const float *amax
     return eps;
 }
 
+// Weld parameters of a grid geometry (corner separation, candidate bound),
+// cached for the last geometry seen by the calling thread.
+struct WeldGeom {
+    int all_special;
+    float cand_eps;
+};
+WeldGeom weld_geom(const int64_t shape[3], const float *amin, const float *amax) {
+    thread_local bool valid = false;
+    thread_local int64_t c_shape[3];
+    thread_local float c_min[3], c_max[3];
+    thread_local WeldGeom c_geom;
+    bool hit = valid;
+    for (int a = 0; a < 3 && hit; ++a)
+        hit = c_shape[a] == shape[a] && c_min[a] == amin[a] && c_max[a] == amax[a];
+    if (!hit) {
+        c_geom.all_special = grid_well_separated(shape, amin, amax) ? 0 : 1;
+        c_geom.cand_eps = weld_cand_eps(shape, amin, amax);
+        for (int a = 0; a < 3; ++a) {
+            c_shape[a] = shape[a];
+            c_min[a] = amin[a];
+            c_max[a] = amax[a];
+        }
+        valid = true;
+    }
+    return c_geom;
+}
+
 // Mask-pipeline parameters of the intersection pass on a dense grid (the
 // scratch holds the masks, count arrays and totals; cap_* bound the writes).
 iso::McParams its_params(const isoext_grid *g, float level, char *ws, int64_t cap_cells, int64_t cap_pts) {
@@ -351,6 +378,25 @@ struct Pinned {
     }
 };
 thread_local Pinned g_pinned;
+thread_local Pinned g_pinned_rb;   // single-call read-back (counts, list descriptors)
+
+// Stage timing events of the calling thread, created once per device.
+hipEvent_t *timing_events() {
+    thread_local hipEvent_t ev[5] = {};
+    thread_local int dev = -1;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return nullptr;
+    if (dev != cur) {
+        for (auto &e : ev) {
+            if (e) (void) hipEventDestroy(e);
+            e = nullptr;
+        }
+        for (auto &e : ev)
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        dev = cur;
+    }
+    return ev;
+}
 
 void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t word0, int64_t blk0,
                int64_t tile0, iso::McBatchItem &out) {
@@ -366,8 +412,9 @@ void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t 
     }
     p.level = in.level;
     p.method = in.method;
-    p.all_special = L.cells && !grid_well_separated(in.shape, in.aabb_min, in.aabb_max) ? 1 : 0;
-    p.cand_eps = weld_cand_eps(in.shape, in.aabb_min, in.aabb_max);
+    const WeldGeom wg = weld_geom(in.shape, in.aabb_min, in.aabb_max);
+    p.all_special = L.cells ? wg.all_special : 0;
+    p.cand_eps = wg.cand_eps;
     p.vec4 = (in.shape[2] % 4 == 0 && ((uintptr_t) in.values & 15) == 0) ? 1 : 0;
     p.W = L.W;
     p.S = L.S;
@@ -670,8 +717,9 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     }
     p.level = level;
     p.method = method;
-    p.all_special = grid_well_separated(shape, aabb_min, aabb_max) ? 0 : 1;
-    p.cand_eps = weld_cand_eps(shape, aabb_min, aabb_max);
+    const WeldGeom wg = weld_geom(shape, aabb_min, aabb_max);
+    p.all_special = wg.all_special;
+    p.cand_eps = wg.cand_eps;
     p.vec4 = (shape[2] % 4 == 0 && ((uintptr_t) values & 15) == 0) ? 1 : 0;
     p.W = L.W;
     p.nzc = L.nzc;
@@ -702,29 +750,33 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     p.seg.scell = (iso::SpecialEdge *) (ws + L.o_sscell);
     p.seg.kept = (int64_t *) (ws + L.o_skept);
     p.seg.hole = (int64_t *) (ws + L.o_shole);
-    hipEvent_t ev[5] = {};
-    if (g_timing)
-        for (auto &e : ev) HIP_CHECK(hipEventCreate(&e), "event");
-    HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, g_timing ? ev : nullptr), "launch marching cubes");
+    hipEvent_t *ev = g_timing ? timing_events() : nullptr;
+    if (g_timing && !ev) return fail(ISOEXT_E_DEVICE, "timing events unavailable");
+    HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, ev), "launch marching cubes");
 
-    // read back counts and the special lists' descriptor
-    struct {
+    // read back counts and the special lists' descriptor (one pinned copy of
+    // the workspace header: err, totals, SpecialLists)
+    struct Header {
         int err;
         int pad;
         int64_t totals[2];
-    } hdr;
-    iso::SpecialLists sp_dev;
-    HIP_CHECK(hipMemcpyAsync(&hdr, ws + kOffErr, sizeof(hdr), hipMemcpyDeviceToHost, st), "read counts");
-    HIP_CHECK(hipMemcpyAsync(&sp_dev, ws + kOffSp, sizeof(sp_dev), hipMemcpyDeviceToHost, st),
-              "read special lists");
+    };
+    static_assert(kOffTotals == kOffErr + 8 && kOffSp >= kOffErr + (int) sizeof(Header), "header layout");
+    const size_t rb_bytes = kOffSp - kOffErr + sizeof(iso::SpecialLists);
+    char *rb = (char *) g_pinned_rb.get(rb_bytes);
+    if (!rb) return fail(ISOEXT_E_DEVICE, "pinned read-back allocation failed");
+    HIP_CHECK(hipMemcpyAsync(rb, ws + kOffErr, rb_bytes, hipMemcpyDeviceToHost, st), "read counts");
     HIP_CHECK(hipStreamSynchronize(st), "marching cubes");
-    if (g_timing) {
+    Header hdr;
+    iso::SpecialLists sp_dev;
+    std::memcpy(&hdr, rb, sizeof(hdr));
+    std::memcpy(&sp_dev, rb + (kOffSp - kOffErr), sizeof(sp_dev));
+    if (ev) {
         float ms[5] = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
         for (int i = 0; i < 4; ++i) (void) hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]);
         (void) hipEventElapsedTime(&ms[4], ev[0], ev[4]);
         g_times = {ms[0], ms[1], ms[2], ms[3], ms[4]};
         g_kernel_ms = ms[4];
-        for (auto &e : ev) (void) hipEventDestroy(e);
     }
     int64_t nv = hdr.totals[0], nf = hdr.totals[1];
     counts[0] = counts[4] = nv;

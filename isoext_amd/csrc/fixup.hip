@@ -481,9 +481,19 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     hipLaunchKernelGGL(k_reloc, dim3(grid_for(nfrel)), dim3(256), 0, st, fkeep, fpre, fw, nf, misc + 3, nfrel,
                        rf_src, rf_dst, (float *) nullptr, (int64_t *) nullptr, f, misc + 4);
     ISO_TRY(hipGetLastError());
-    int64_t h[5];
-    ISO_TRY(hipMemcpyAsync(h, misc, sizeof(h), hipMemcpyDeviceToHost, st));
+    // final counts through a small pinned buffer of the calling thread
+    struct PinnedCounts {
+        int64_t *p = nullptr;
+        ~PinnedCounts() {
+            if (p) (void) hipHostFree(p);
+        }
+    };
+    thread_local PinnedCounts pc;
+    if (!pc.p) ISO_TRY(hipHostMalloc((void **) &pc.p, 8 * sizeof(int64_t), 0));
+    ISO_TRY(hipMemcpyAsync(pc.p, misc, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipStreamSynchronize(st));
+    int64_t h[5];
+    for (int i = 0; i < 5; ++i) h[i] = pc.p[i];
     if (h[4]) return hipErrorInvalidValue;   // more relocations than the lists allow: inconsistent lists
     nv = h[2];
     nf = h[3];

@@ -458,6 +458,7 @@ struct EmitLds {
     uint32_t fsc[NT + 1];             // batch: faces per slot -> exclusive offsets
     uint32_t tsum[NT / 64];
     int32_t carry[2];                 // faces of the straddling row's cells before a batch
+    int32_t ids[12][NT];              // batch: vertex id of each edge of each slot's cell
     uint64_t tris[256];               // the method's packed triangle table
     float cx[TX + 1], cy[TY + 1], cz[64 * NZC + 1];   // grid coordinates of the tile's points
 };
@@ -777,6 +778,62 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                 }
             }
             const bool special = candm != 0;
+            // vertex ids of the cell's 12 edges (meaningful where crossing):
+            // ranks in the owner rows' words from the masks of the cell's
+            // four point rows and their x+1 / y+1 neighbours
+            int32_t id[12];
+            {
+                const uint64_t b00 = PW(s, cx, cy, jc), b01 = PW(s, cx, cy + 1, jc);
+                const uint64_t b10 = PW(s, cx + 1, cy, jc), b11 = PW(s, cx + 1, cy + 1, jc);
+                const uint64_t b20 = PW(s, cx + 2, cy, jc), b21 = PW(s, cx + 2, cy + 1, jc);
+                const uint64_t b02 = PW(s, cx, cy + 2, jc), b12 = PW(s, cx + 1, cy + 2, jc);
+                const bool hx0 = cx < hx, hx1 = cx + 1 < hx, hy0 = cy < hy, hy1 = cy + 1 < hy;
+                const uint64_t xm00 = hx0 ? b00 ^ b10 : 0ull, xm01 = hx0 ? b01 ^ b11 : 0ull;
+                const uint64_t xm10 = hx1 ? b10 ^ b20 : 0ull, xm11 = hx1 ? b11 ^ b21 : 0ull;
+                const uint64_t ym00 = hy0 ? b00 ^ b01 : 0ull, ym10 = hy0 ? b10 ^ b11 : 0ull;
+                const uint64_t ym01 = hy1 ? b01 ^ b02 : 0ull, ym11 = hy1 ? b11 ^ b12 : 0ull;
+                const uint64_t zm00 = b00 ^ (b00 >> 1), zm01 = b01 ^ (b01 >> 1), zm10 = b10 ^ (b10 >> 1);
+                const uint64_t zm11 = b11 ^ (b11 >> 1);
+                const uint64_t low = (1ull << k) - 1;
+                auto rank = [&](uint64_t xm, uint64_t ym, uint64_t zm) {
+                    return __popcll(xm & low) + __popcll(ym & low) + __popcll(zm & low);
+                };
+                auto bit = [&](uint64_t m, int q) { return (int) ((m >> q) & 1); };
+                constexpr int QC = L::QC, QY = L::QY;
+                const int v00 = s.vb[(cx * QY + cy) * QC + jc], v01 = s.vb[(cx * QY + cy + 1) * QC + jc];
+                const int v10 = s.vb[((cx + 1) * QY + cy) * QC + jc];
+                const int v11 = s.vb[((cx + 1) * QY + cy + 1) * QC + jc];
+                const int r00 = v00 + rank(xm00, ym00, zm00), r01 = v01 + rank(xm01, ym01, zm01);
+                const int r10 = v10 + rank(xm10, ym10, zm10), r11 = v11 + rank(xm11, ym11, zm11);
+                // point (row, k): x-edge, y-edge, z-edge in that order; the
+                // point at k + 1 follows the three edges of k
+                const int x00 = bit(xm00, k), y00 = bit(ym00, k), z00 = bit(zm00, k);
+                const int x01 = bit(xm01, k), y01 = bit(ym01, k), z01 = bit(zm01, k);
+                const int x10 = bit(xm10, k), y10 = bit(ym10, k), z10 = bit(zm10, k);
+                id[8] = r00;                                  // corner 0 x
+                id[3] = r00 + x00;                            // corner 0 y
+                id[0] = r00 + x00 + y00;                      // corner 0 z
+                id[11] = r01;                                 // corner 2 x
+                id[2] = r01 + x01 + y01;                      // corner 2 z
+                id[7] = r10 + x10;                            // corner 4 y
+                id[4] = r10 + x10 + y10;                      // corner 4 z
+                id[6] = r11 + bit(xm11, k) + bit(ym11, k);    // corner 6 z
+                if (k < 63) {
+                    const int s00 = r00 + x00 + y00 + z00, s01 = r01 + x01 + y01 + z01;
+                    const int s10 = r10 + x10 + y10 + z10;
+                    id[9] = s00;                              // corner 1 x
+                    id[1] = s00 + bit(xm00, k + 1);           // corner 1 y
+                    id[10] = s01;                             // corner 3 x
+                    id[5] = s10 + bit(xm10, k + 1);           // corner 5 y
+                } else {   // corners at z + 1 sit in the next word
+                    id[9] = vertex_id(s, hx, hy, cx, cy, zl + 1, 0);
+                    id[1] = vertex_id(s, hx, hy, cx, cy, zl + 1, 1);
+                    id[10] = vertex_id(s, hx, hy, cx, cy + 1, zl + 1, 0);
+                    id[5] = vertex_id(s, hx, hy, cx + 1, cy, zl + 1, 1);
+                }
+#pragma unroll
+                for (int e = 0; e < 12; ++e) s.ids[e][tid] = id[e];
+            }
             // vertices owned by this cell: the edges of its lowest corner, plus
             // those of upper corners on the grid's upper faces (no cell above)
             {
@@ -791,9 +848,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                 const uint32_t m = emit & status;
                 for_edges([&](auto E) {
                     if (!((m >> E) & 1)) return;
-                    constexpr int ca = edge_a(E);
-                    const int64_t v = vertex_id(s, hx, hy, cx + (ca >> 2), cy + ((ca >> 1) & 1), zl + (ca & 1),
-                                                edge_axis(E));
+                    const int64_t v = id[E];
                     float x, y, z;
                     edge_pos_c<E>(cc, x, y, z);
                     if (v >= 0 && v < p.cap_v) {
@@ -839,14 +894,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                 if (slot >= 0 && slot < p.cap_f) {
                     int32_t *dst = p.out_f + 3 * slot;
                     if (keep) {
-                        const int a0 = edge_a(e0), a1 = edge_a(e1), a2 = edge_a(e2);
-                        const int i0 = vertex_id(s, hx, hy, cx + (a0 >> 2), cy + ((a0 >> 1) & 1), zl + (a0 & 1),
-                                                 edge_axis(e0));
-                        const int i1 = vertex_id(s, hx, hy, cx + (a1 >> 2), cy + ((a1 >> 1) & 1), zl + (a1 & 1),
-                                                 edge_axis(e1));
-                        const int i2 = vertex_id(s, hx, hy, cx + (a2 >> 2), cy + ((a2 >> 1) & 1), zl + (a2 & 1),
-                                                 edge_axis(e2));
-                        *reinterpret_cast<int3 *>(dst) = make_int3(i0, i1, i2);
+                        *reinterpret_cast<int3 *>(dst) = make_int3(s.ids[e0][tid], s.ids[e1][tid], s.ids[e2][tid]);
                     } else {
                         *reinterpret_cast<int3 *>(dst) = make_int3(-1, -1, -1);
                     }
@@ -859,10 +907,8 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                     m &= m - 1;
                     const int64_t i = seg_push(p, 1, spseg);
                     if (i >= 0) {
-                        const int ca = edge_a(e);
                         SpecialEdge se;
-                        se.vid = vertex_id(s, hx, hy, cx + (ca >> 2), cy + ((ca >> 1) & 1), zl + (ca & 1),
-                                           edge_axis(e));
+                        se.vid = s.ids[e][tid];
                         se.edge = edge_key(p, gx, gy, gz, e);
                         p.seg.scell[i] = se;
                     }
@@ -1077,14 +1123,18 @@ __global__ void k_batch_summary(const McBatchItem *items, const McBatchHdr *hdr,
 template <int TX, int TY, int NZC, int NT>
 struct ItsLds {
     static constexpr int PX = TX + 1, PY = TY + 1, PC = NZC + 1;
-    static constexpr int NUNITS = TX * TY * NZC;
+    static constexpr int NUNITS = TX * TY * NZC;   // (cell row, chunk), chunk fastest
     uint64_t w[PX * PY * PC];
-    int64_t abase[NUNITS];   // active index of the unit's first active cell
-    int64_t ebase[NUNITS];   // first entry of the unit (crossing count before the scan)
-    uint32_t na[NUNITS + 1];
-    uint32_t list[NT];       // batch: unit << 16 | z-bit << 10 | entry offset in unit
+    uint64_t amask[NUNITS];   // active cells of each unit
+    int64_t abase[TX * TY];   // active index of the row's first active cell (this z-segment)
+    int64_t ebase[TX * TY];   // first entry of the row's cells
+    uint32_t na[NUNITS + 1];  // active cells per unit -> exclusive offsets
+    int32_t unit[NT];         // batch: unit of each slot
+    uint32_t esc[NT + 1];     // batch: entries per slot -> exclusive offsets
     uint32_t tsum[NT / 64];
+    int32_t carry[2];         // entries of the straddling row's cells before a batch
     uint8_t ncross[256];
+    float cx[TX + 1], cy[TY + 1], cz[64 * NZC + 1];
 };
 
 template <int TX, int TY, int NZC, int NT>
@@ -1095,6 +1145,10 @@ __device__ __forceinline__ CellMasks its_unit_cells(const ItsLds<TX, TY, NZC, NT
                       W(cx + 1, cy, j + 1), W(cx + 1, cy + 1, j), W(cx + 1, cy + 1, j + 1));
 }
 
+// Same tiling and slot machinery as the marching-cubes emit: active index of
+// a cell = its row's base + its rank in the row, first entry = the row's base
+// + a block scan of crossing counts (with a carry for the row that straddles
+// two batches).
 template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
     using L = ItsLds<TX, TY, NZC, NT>;
@@ -1112,75 +1166,71 @@ __global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
         const int64_t gx = x0 + px, gy = y0 + py, c = c0 + j;
         s.w[i] = (gx < p.X && gy < p.Y && c < p.W) ? p.bits[(gx * p.Y + gy) * p.W + c] : 0ull;
     }
-    __syncthreads();
-    for (int i = tid; i < TX * TY; i += NT) {
-        const int cx = i / TY, cy = i % TY;
-        for (int j = 0; j < NZC; ++j) {
-            const int u = i * NZC + j;
-            s.na[u] = 0;
-            s.ebase[u] = 0;
-            const int64_t zlo = z0 + 64 * j;
-            if (cx >= nx || cy >= ny || zlo >= p.Z - 1) continue;
-            const CellMasks m = its_unit_cells(s, cx, cy, j);
-            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - zlo);
-            s.na[u] = (uint32_t) __popcll(act);
-            int64_t nc = 0;
-            while (act) {
-                const int k = __ffsll((unsigned long long) act) - 1;
-                act &= act - 1;
-                nc += s.ncross[cell_case(m, k)];
-            }
-            s.ebase[u] = nc;
+    for (int c = tid; c < TX + TY + 64 * NZC + 3; c += NT) {
+        if (c <= TX) {
+            s.cx[c] = grid_coord(x0 + c, p.X, p.amin[0], p.amax[0]);
+        } else if (c <= TX + TY + 1) {
+            s.cy[c - TX - 1] = grid_coord(y0 + (c - TX - 1), p.Y, p.amin[1], p.amax[1]);
+        } else {
+            s.cz[c - TX - TY - 2] = grid_coord(z0 + (c - TX - TY - 2), p.Z, p.amin[2], p.amax[2]);
         }
     }
-    __syncthreads();
-    // global bases per unit (before the scan turns na into offsets)
     for (int i = tid; i < TX * TY; i += NT) {
         const int cx = i / TY, cy = i % TY;
+        s.abase[i] = s.ebase[i] = 0;
         if (cx >= nx || cy >= ny) continue;
         const int64_t e = ((x0 + cx) * p.Y + (y0 + cy)) * p.S + tk;
-        int64_t ab = p.blk_v[e / kMcScanBlock] + (int64_t) p.cnt_v[e];
-        int64_t eb = p.blk_f[e / kMcScanBlock] + (int64_t) p.cnt_f[e];
-        for (int j = 0; j < NZC; ++j) {
-            const int u = i * NZC + j;
-            const int64_t n_a = s.na[u], n_e = s.ebase[u];
-            s.abase[u] = ab;
-            s.ebase[u] = eb;
-            ab += n_a;
-            eb += n_e;
-        }
+        s.abase[i] = p.blk_v[e / kMcScanBlock] + (int64_t) p.cnt_v[e];
+        s.ebase[i] = p.blk_f[e / kMcScanBlock] + (int64_t) p.cnt_f[e];
+    }
+    __syncthreads();
+    for (int u = tid; u < L::NUNITS; u += NT) {
+        const int j = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
+        const int64_t zlo = z0 + 64 * j;
+        uint64_t act = 0;
+        if (cx < nx && cy < ny && zlo < p.Z - 1)
+            act = cell_active(its_unit_cells(s, cx, cy, j)) & low_bits(p.Z - 1 - zlo);
+        s.amask[u] = act;
+        s.na[u] = (uint32_t) __popcll(act);
     }
     __syncthreads();
     block_exscan_lds<L::NUNITS, NT>(s.na, s.tsum);
     const int nactive = (int) s.na[L::NUNITS];
-    for (int j0 = 0; j0 < nactive; j0 += NT) {
+    if (tid == 0) s.carry[0] = 0;
+    for (int j0 = 0, bt = 0; j0 < nactive; j0 += NT, bt ^= 1) {
+        s.unit[tid] = -1;
+        __syncthreads();
         for (int u = tid; u < L::NUNITS; u += NT) {
-            const uint32_t a0 = s.na[u], a1 = s.na[u + 1];
-            if (a1 == a0 || a1 <= (uint32_t) j0 || a0 >= (uint32_t) (j0 + NT)) continue;
-            const int j = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
-            const CellMasks m = its_unit_cells(s, cx, cy, j);
-            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - (z0 + 64 * j));
-            const uint32_t end = min(a1, (uint32_t) (j0 + NT));
-            uint32_t idx = a0, eo = 0;
-            while (act && idx < end) {
-                const int k = __ffsll((unsigned long long) act) - 1;
-                act &= act - 1;
-                if (idx >= (uint32_t) j0) s.list[idx - j0] = ((uint32_t) u << 16) | ((uint32_t) k << 10) | eo;
-                eo += s.ncross[cell_case(m, k)];
-                ++idx;
-            }
+            const int a0 = (int) s.na[u], a1 = (int) s.na[u + 1];
+            if (a1 > a0 && a0 < j0 + NT && a1 > j0) s.unit[max(a0, j0) - j0] = u;
         }
         __syncthreads();
-        if (j0 + tid < nactive) {
-            const uint32_t ent = s.list[tid];
-            const int u = (int) (ent >> 16), k = (int) ((ent >> 10) & 63), eo = (int) (ent & 1023);
+        block_maxscan_lds<NT>(s.unit, s.tsum);
+        const int jj = j0 + tid;
+        const bool live = jj < nactive;
+        int u = 0, k = 0, cs = 0;
+        if (live) {
+            u = s.unit[tid];
+            k = nth_set_bit(s.amask[u], jj - (int) s.na[u]);
+            const int jc = u % NZC, r = u / NZC;
+            cs = cell_case(its_unit_cells(s, r / TY, r % TY, jc), k);
+        }
+        s.esc[tid] = live ? (uint32_t) s.ncross[cs] : 0u;
+        __syncthreads();
+        block_exscan_lds<NT, NT>(s.esc, s.tsum);
+        if (tid == NT - 1 && j0 + NT < nactive) {
+            const int rl = s.unit[NT - 1] / NZC;
+            const int rs = (int) s.na[rl * NZC], re = (int) s.na[(rl + 1) * NZC];
+            s.carry[bt ^ 1] = re > j0 + NT ? (rs < j0 ? s.carry[bt] : 0) + (int) (s.esc[NT] - s.esc[max(rs - j0, 0)])
+                                           : 0;
+        }
+        if (live) {
             const int jc = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
-            const CellMasks m = its_unit_cells(s, cx, cy, jc);
-            const uint64_t act = cell_active(m);
-            const int cs = cell_case(m, k);
-            const int64_t a = s.abase[u] + __popcll(act & (k ? (~0ull >> (64 - k)) : 0ull));
-            int64_t e = s.ebase[u] + eo;
-            const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + 64 * jc + k;
+            const int rs = (int) s.na[r * NZC];
+            const int64_t a = s.abase[r] + (jj - rs);
+            int64_t e = s.ebase[r] + (int64_t) (s.esc[tid] - s.esc[max(rs - j0, 0)]) + (rs < j0 ? s.carry[bt] : 0);
+            const int zl = 64 * jc + k;
+            const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + zl;
             const int64_t id = (gx * (p.Y - 1) + gy) * (p.Z - 1) + gz;
             float cv[8];
             {
@@ -1197,37 +1247,32 @@ __global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
                 cv[6] = r11[0];
                 cv[7] = r11[1];
             }
-            if (a >= p.cap_v) goto next;   // capacities from the count pass (defensive)
-            o.cell_indices[a] = id;
-            o.cell_dense[a] = id;
-            o.cell_offsets[a] = e;
-            {
-            const float lo[3] = {grid_coord(gx, p.X, p.amin[0], p.amax[0]), grid_coord(gy, p.Y, p.amin[1], p.amax[1]),
-                                 grid_coord(gz, p.Z, p.amin[2], p.amax[2])};
-            const float hi[3] = {grid_coord(gx + 1, p.X, p.amin[0], p.amax[0]),
-                                 grid_coord(gy + 1, p.Y, p.amin[1], p.amax[1]),
-                                 grid_coord(gz + 1, p.Z, p.amin[2], p.amax[2])};
-            const int st = crossing_status(cs);
-            for_edges([&](auto E) {
-                if (!((st >> E) & 1) || e >= p.cap_f) return;
-                constexpr int ca = edge_a(E), cb = edge_b(E);
-                const float t = edge_t(cv[ca], cv[cb], p.level);
-                const float px = lerp1(t, (ca >> 2) ? hi[0] : lo[0], (cb >> 2) ? hi[0] : lo[0]);
-                const float py = lerp1(t, ((ca >> 1) & 1) ? hi[1] : lo[1], ((cb >> 1) & 1) ? hi[1] : lo[1]);
-                const float pz = lerp1(t, (ca & 1) ? hi[2] : lo[2], (cb & 1) ? hi[2] : lo[2]);
-                *reinterpret_cast<float3 *>(o.points + 3 * e) = make_float3(px, py, pz);
-                o.edge_local[e] = (uint8_t) E;
-                o.is_out[e] = cv[ca] <= cv[cb] ? 1 : 0;
-                o.entry_cell[e] = (int32_t) a;
-                if (o.normals) {
-                    float nxv, nyv, nzv;
-                    cell_normal(lo, hi, cv, px, py, pz, nxv, nyv, nzv);
-                    *reinterpret_cast<float3 *>(o.normals + 3 * e) = make_float3(nxv, nyv, nzv);
-                }
-                ++e;
-            });
+            if (a < p.cap_v) {   // capacities from the count pass (defensive)
+                o.cell_indices[a] = id;
+                o.cell_dense[a] = id;
+                o.cell_offsets[a] = e;
+                const float lo[3] = {s.cx[cx], s.cy[cy], s.cz[zl]};
+                const float hi[3] = {s.cx[cx + 1], s.cy[cy + 1], s.cz[zl + 1]};
+                const int st = crossing_status(cs);
+                for_edges([&](auto E) {
+                    if (!((st >> E) & 1) || e >= p.cap_f) return;
+                    constexpr int ca = edge_a(E), cb = edge_b(E);
+                    const float t = edge_t(cv[ca], cv[cb], p.level);
+                    const float px = lerp1(t, (ca >> 2) ? hi[0] : lo[0], (cb >> 2) ? hi[0] : lo[0]);
+                    const float py = lerp1(t, ((ca >> 1) & 1) ? hi[1] : lo[1], ((cb >> 1) & 1) ? hi[1] : lo[1]);
+                    const float pz = lerp1(t, (ca & 1) ? hi[2] : lo[2], (cb & 1) ? hi[2] : lo[2]);
+                    *reinterpret_cast<float3 *>(o.points + 3 * e) = make_float3(px, py, pz);
+                    o.edge_local[e] = (uint8_t) E;
+                    o.is_out[e] = cv[ca] <= cv[cb] ? 1 : 0;
+                    o.entry_cell[e] = (int32_t) a;
+                    if (o.normals) {
+                        float nxv, nyv, nzv;
+                        cell_normal(lo, hi, cv, px, py, pz, nxv, nyv, nzv);
+                        *reinterpret_cast<float3 *>(o.normals + 3 * e) = make_float3(nxv, nyv, nzv);
+                    }
+                    ++e;
+                });
             }
-        next:;
         }
         __syncthreads();
     }

@@ -48,6 +48,9 @@ def _initial_caps(shape):
     return nv, 2 * nv + 1024
 
 
+_ws_bytes_cache: dict = {}
+
+
 def mc_dense_raw(values: torch.Tensor, shape, aabb_min, aabb_max, level: float, method: int,
                  cfg: int | None = None, want_keys: bool = False):
     """Run dense marching cubes on a (X, Y, Z) float32 CUDA tensor.
@@ -57,33 +60,36 @@ def mc_dense_raw(values: torch.Tensor, shape, aabb_min, aabb_max, level: float, 
     with want_keys, info["keys"] holds each vertex's grid edge).
     """
     dev = values.device
-    shape = [int(s) for s in shape]
+    shape = tuple(int(s) for s in shape)
     if cfg is None:
         cfg = autotune.mc_config(shape)
     L = _lib.lib()
     special_cap = DEFAULT_SPECIAL_CAP
     cap_v, cap_f = _initial_caps(shape)
     counts = _lib.I64x6()
+    shape_c, amin_c, amax_c = _lib.i64x3(shape), _lib.f32x3(aabb_min), _lib.f32x3(aabb_max)
+    stream = _lib.stream_ptr(dev)
     for attempt in range(4):
-        ws_bytes = L.isoext_mc_dense_workspace_bytes(_lib.i64x3(shape), cfg, special_cap)
+        key = (shape, cfg, special_cap)
+        ws_bytes = _ws_bytes_cache.get(key)
+        if ws_bytes is None:
+            ws_bytes = L.isoext_mc_dense_workspace_bytes(shape_c, cfg, special_cap)
+            _ws_bytes_cache[key] = ws_bytes
         if ws_bytes < 0:
             raise ValueError(f"invalid tile config {cfg}")
         ws = workspace.get("mc_dense", ws_bytes, dev)
         out_v = torch.empty((cap_v, 3), dtype=torch.float32, device=dev)
         out_f = torch.empty((cap_f, 3), dtype=torch.int32, device=dev)
         keys = torch.empty(cap_v, dtype=torch.int64, device=dev) if want_keys else None
-        with torch.cuda.device(dev):
+        with _lib.on_device(dev):
             if want_keys:
-                rc = L.isoext_mc_dense_keyed(values.data_ptr(), _lib.i64x3(shape), _lib.f32x3(aabb_min),
-                                             _lib.f32x3(aabb_max), float(level), int(method), int(cfg),
-                                             ws.data_ptr(), ws.numel(), special_cap, out_v.data_ptr(), cap_v,
-                                             out_f.data_ptr(), cap_f, keys.data_ptr(), counts,
-                                             _lib.stream_ptr(dev))
+                rc = L.isoext_mc_dense_keyed(values.data_ptr(), shape_c, amin_c, amax_c, float(level), int(method),
+                                             int(cfg), ws.data_ptr(), ws.numel(), special_cap, out_v.data_ptr(),
+                                             cap_v, out_f.data_ptr(), cap_f, keys.data_ptr(), counts, stream)
             else:
-                rc = L.isoext_mc_dense(values.data_ptr(), _lib.i64x3(shape), _lib.f32x3(aabb_min),
-                                       _lib.f32x3(aabb_max), float(level), int(method), int(cfg),
-                                       ws.data_ptr(), ws.numel(), special_cap, out_v.data_ptr(), cap_v,
-                                       out_f.data_ptr(), cap_f, counts, _lib.stream_ptr(dev))
+                rc = L.isoext_mc_dense(values.data_ptr(), shape_c, amin_c, amax_c, float(level), int(method),
+                                       int(cfg), ws.data_ptr(), ws.numel(), special_cap, out_v.data_ptr(), cap_v,
+                                       out_f.data_ptr(), cap_f, counts, stream)
         if rc == _lib.E_CAPACITY:
             cap_v, cap_f = int(counts[0]) + 1, int(counts[1]) + 1
             continue
