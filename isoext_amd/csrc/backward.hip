@@ -35,9 +35,10 @@ __global__ void k_bwd_dense(const float *grad_v, const int64_t *keys, int64_t nv
     int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
     if (i >= nv) return;
     int64_t key = keys[i];
-    int axis = (int) (key % 3);
-    int64_t p = key / 3;
-    int64_t z = p % g.Z, y = (p / g.Z) % g.Y, x = p / (g.Y * g.Z);
+    const int64_t p = div_fast(key, 3);
+    const int axis = (int) (key - 3 * p);
+    const int64_t r = div_fast(p, g.Z), z = p - r * g.Z;
+    const int64_t x = div_fast(r, g.Y), y = r - x * g.Y;
     int64_t stride = axis == 0 ? g.Y * g.Z : (axis == 1 ? g.Z : 1);
     int64_t q = p + stride;
     float pa[3] = {grid_coord(x, g.X, g.amin[0], g.amax[0]), grid_coord(y, g.Y, g.amin[1], g.amax[1]),
@@ -58,11 +59,12 @@ __global__ void k_bwd_sparse(const float *grad_v, const int64_t *vsrc, int64_t n
     int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
     if (i >= nv) return;
     int64_t src = vsrc[i];
-    int64_t slot = src / 12;
-    int e = (int) (src % 12);
-    int64_t id = ids[slot];
-    int64_t cz = g.Z - 1, cy = g.Y - 1;
-    int64_t z = id % cz, r = id / cz, y = r % cy, x = r / cy;
+    const int64_t slot = div_fast(src, 12);
+    const int e = (int) (src - 12 * slot);
+    const int64_t id = ids[slot];
+    const int64_t cz = g.Z - 1, cy = g.Y - 1;
+    const int64_t r = div_fast(id, cz), z = id - r * cz;
+    const int64_t x = div_fast(r, cy), y = r - x * cy;
     int ca = edge_a(e), cb = edge_b(e);
     float pa[3] = {grid_coord(x + (ca >> 2), g.X, g.amin[0], g.amax[0]),
                    grid_coord(y + ((ca >> 1) & 1), g.Y, g.amin[1], g.amax[1]),

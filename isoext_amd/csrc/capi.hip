@@ -557,9 +557,7 @@ int isoext_its_emit(const isoext_grid *g, float level, const void *scratch, int6
             HIP_CHECK(iso::its_emit(to_desc(g), level, (const int64_t *) scratch, o, st), "intersection emit");
         }
     }
-    HIP_CHECK(hipMemcpyAsync(cell_offsets + ncells, &npts, sizeof(int64_t), hipMemcpyHostToDevice, st),
-              "write offsets");
-    HIP_CHECK(hipStreamSynchronize(st), "intersection emit");
+    HIP_CHECK(iso::fill_i64(cell_offsets + ncells, 1, npts, st), "write offsets");
     return ISOEXT_OK;
 }
 
@@ -613,16 +611,11 @@ int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_
     w.out_v = out_v;
     w.out_f = out_f;
     w.out_vcell = out_vcell;
-    unsigned long long zero = 0;
-    HIP_CHECK(hipMemcpyAsync(w.n_fp64, &zero, sizeof(zero), hipMemcpyHostToDevice, st), "reset counter");
     // re-solve in FP64 where the FP32 QEF is ill-conditioned, and always in the
     // regime where the reference truncates singular values (cond >= 1/svd_tol)
     float cond_fp64 = std::min(1e6f, 0.5f / std::max(svd_tol, 1e-30f));
-    int64_t nv = 0, nf = 0;
-    HIP_CHECK(iso::dc_run(to_desc(g), to_in(its), reg, svd_tol, cond_fp64, w, nv, nf, st), "dual contouring");
-    unsigned long long nfp = 0;
-    HIP_CHECK(hipMemcpyAsync(&nfp, w.n_fp64, sizeof(nfp), hipMemcpyDeviceToHost, st), "read counter");
-    HIP_CHECK(hipStreamSynchronize(st), "dual contouring");
+    int64_t nv = 0, nf = 0, nfp = 0;
+    HIP_CHECK(iso::dc_run(to_desc(g), to_in(its), reg, svd_tol, cond_fp64, w, nv, nf, nfp, st), "dual contouring");
     counts[0] = nv;
     counts[1] = nf;
     counts[2] = (int64_t) nfp;

@@ -43,6 +43,17 @@ ISO_HD float grid_coord(int64_t i, int64_t res, float amin, float amax) {
     return amin + prod;
 }
 
+// a / b for 0 <= a < 2^52 and 0 < b: 32-bit division when both fit, else a
+// correctly rounded FP64 quotient (within 1 of the integer one for such a)
+// corrected by one step.  Replaces the much longer 64-bit integer division.
+ISO_HD int64_t div_fast(int64_t a, int64_t b) {
+    if (((uint64_t) a | (uint64_t) b) >> 32 == 0) return (int64_t) ((uint32_t) a / (uint32_t) b);
+    int64_t q = (int64_t) ((double) a / (double) b);
+    if (q * b > a) --q;
+    else if ((q + 1) * b <= a) ++q;
+    return q;
+}
+
 ISO_HD float lerp1(float t, float a, float b) {
     float s = 1.0f - t;
     float u = s * a;

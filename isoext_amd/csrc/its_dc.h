@@ -63,8 +63,12 @@ int64_t its_scratch_pairs(const GridDesc &g);
 hipError_t its_count(const GridDesc &g, float level, int64_t *bsum, int64_t *tot, hipStream_t st);
 hipError_t its_emit(const GridDesc &g, float level, const int64_t *bsum_pre, const ItsOut &o, hipStream_t st);
 hipError_t its_normals(const GridDesc &g, const ItsIn &in, float *normals, hipStream_t st);
+// p[0, n) = v on the stream (no host synchronisation).
+hipError_t fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t st);
+// Runs dual contouring and synchronises once; n_fp64 returns the number of
+// FP64 re-solves.
 hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float cond_fp64, DcScratch &w,
-                  int64_t &nv, int64_t &nf, hipStream_t st);
+                  int64_t &nv, int64_t &nf, int64_t &n_fp64, hipStream_t st);
 // Backward of dual contouring: dL/d(output vertex) -> dL/d(grid values)
 // (through the QEF solve and the edge interpolation of the intersection
 // points) and, when grad_normals != nullptr, dL/d(normals).

@@ -48,10 +48,10 @@ __device__ __forceinline__ void cell_xyz(const CellSrc &s, int64_t id, int64_t &
         x = r / cy32;
         return;
     }
-    z = id % cz;
-    int64_t r = id / cz;
-    y = r % cy;
-    x = r / cy;
+    const int64_t r = div_fast(id, cz);
+    z = id - r * cz;
+    x = div_fast(r, cy);
+    y = r - x * cy;
 }
 
 __device__ __forceinline__ void cell_values(const CellSrc &s, int64_t i, int64_t x, int64_t y, int64_t z,
@@ -773,11 +773,19 @@ hipError_t its_normals(const GridDesc &g, const ItsIn &in, float *normals, hipSt
     return hipGetLastError();
 }
 
+hipError_t fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i64, dim3((unsigned) std::min<int64_t>(nblocks(n), 4096)), dim3(kBlock), 0, st, p, n,
+                       v);
+    return hipGetLastError();
+}
+
 hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float cond_fp64, DcScratch &w,
-                  int64_t &nv, int64_t &nf, hipStream_t st) {
+                  int64_t &nv, int64_t &nf, int64_t &n_fp64, hipStream_t st) {
     CellSrc s = make_src(g);
-    nv = nf = 0;
+    nv = nf = n_fp64 = 0;
     if (in.ncells == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, (int64_t *) w.n_fp64, (int64_t) 1, (int64_t) 0);
     hipLaunchKernelGGL(k_fill_u8, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, in.ncells, (uint8_t) 0);
     hipLaunchKernelGGL(k_dc_qef, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in, s, reg, svd_tol, cond_fp64,
                        w.dual_v, w.cond, w.n_fp64);
@@ -809,13 +817,23 @@ hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, 
     hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * in.npts)), dim3(kBlock), 0, st, w.tri_cells, w.tot,
                        w.rep_slot, w.slot_min, w.final_id, w.out_f);
     ISO_TRY(hipGetLastError());
-    int64_t tot[4];
-    ISO_TRY(hipMemcpyAsync(tot, w.tot, sizeof(tot), hipMemcpyDeviceToHost, st));
+    // totals and the re-solve counter through a small pinned buffer of the calling thread
+    struct PinnedTot {
+        int64_t *p = nullptr;
+        ~PinnedTot() {
+            if (p) (void) hipHostFree(p);
+        }
+    };
+    thread_local PinnedTot pt;
+    if (!pt.p) ISO_TRY(hipHostMalloc((void **) &pt.p, 8 * sizeof(int64_t), 0));
+    ISO_TRY(hipMemcpyAsync(pt.p, w.tot, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    ISO_TRY(hipMemcpyAsync(pt.p + 4, w.n_fp64, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipStreamSynchronize(st));
-    const int64_t nq = tot[0];
+    const int64_t nq = pt.p[0];
     w.nquads = nq;
-    nv = tot[2];
+    nv = pt.p[2];
     nf = 2 * nq;
+    n_fp64 = pt.p[4];
     return hipSuccess;
 }
 

@@ -224,11 +224,6 @@ __global__ void __launch_bounds__(256) k_classify_v4(const float *__restrict__ v
     }
 }
 
-// a / b for non-negative operands, in 32 bits when both fit.
-__device__ __forceinline__ int64_t div_nonneg(int64_t a, int64_t b) {
-    if (((uint64_t) a | (uint64_t) b) >> 32 == 0) return (int64_t) ((uint32_t) a / (uint32_t) b);
-    return a / b;
-}
 
 // Lane-per-point classify of kClassifyWords consecutive words [w0, w0 + 16)
 // of one grid (w0 wave-uniform): lane l loads value z = 64c + l of word
@@ -238,7 +233,7 @@ __device__ __forceinline__ void classify_task(const float *__restrict__ values, 
                                               int64_t nwords, float level, int64_t w0, uint64_t *__restrict__ bits) {
     constexpr int U = kClassifyWords;
     const int lane = threadIdx.x & 63;
-    int64_t row = div_nonneg(w0, W);
+    int64_t row = div_fast(w0, W);
     int64_t c = w0 - row * W;
     float v[U];
     bool ok[U];
@@ -309,15 +304,17 @@ __device__ __forceinline__ void block_exscan2(uint32_t &a, uint32_t &b, uint32_t
 
 // First chunk (point row r, chunk c) of entry i, as the global chunk index r * W + c.
 __device__ __forceinline__ int64_t entry_chunk(const McParams &p, int64_t i) {
-    const int64_t r = i / p.S, s = i - r * p.S;
+    const int64_t r = div_fast(i, p.S), s = i - r * p.S;
     return r * p.W + s * p.nzc;
 }
 
 // One workgroup per kMcScanBlock entries.  Lanes walk the block's chunks in
 // memory order (coalesced mask loads), add their counts into the chunk's
-// entry in LDS, then the entries are scanned.
+// entry in LDS, then the entries are scanned.  NT = 1024 for grids with few
+// blocks (more lanes in flight per block), 256 otherwise.
+template <int NT>
 __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
-    constexpr int NT = 256, PER = kMcScanBlock / NT;
+    constexpr int PER = kMcScanBlock / NT;
     __shared__ uint32_t ent_v[kMcScanBlock], ent_f[kMcScanBlock];
     __shared__ uint32_t sa[NT / 64], sb[NT / 64];
     __shared__ uint8_t ntri[256];
@@ -333,8 +330,8 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
     const int64_t g1 = i1 < n ? entry_chunk(p, i1) : p.X * p.Y * p.W;
     const int64_t YW = p.Y * p.W;
     for (int64_t g = g0 + tid; g < g1; g += NT) {
-        const int64_t r = g / p.W, c = g - r * p.W;
-        const int64_t x = r / p.Y, y = r - x * p.Y;
+        const int64_t r = div_fast(g, p.W), c = g - r * p.W;
+        const int64_t x = div_fast(r, p.Y), y = r - x * p.Y;
         const bool hasx = x + 1 < p.X, hasy = y + 1 < p.Y, hn = c + 1 < p.W;
         const int64_t zlo = 64 * c;
         const uint64_t *q = p.bits + g;
@@ -355,7 +352,7 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
                 cf += ntri[cell_case(m, j)];
             }
         }
-        const int e = (int) (r * p.S + c / p.nzc - i0);
+        const int e = (int) (r * p.S + div_fast(c, p.nzc) - i0);
         if (cv) atomicAdd(&ent_v[e], cv);
         if (cf) atomicAdd(&ent_f[e], cf);
     }
@@ -386,7 +383,11 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_count(McParams p) { count_block(p, blockIdx.x); }
+template <int NT>
+__global__ void __launch_bounds__(NT) k_count(McParams p) { count_block<NT>(p, blockIdx.x); }
+
+// Blocks below which the count pass uses 1024-lane workgroups.
+constexpr int64_t kCountWideBelow = 2048;
 
 // Exclusive scan of the block totals (one workgroup, carried across rounds);
 // writes the mesh size and flags meshes beyond int32 indices.
@@ -1056,16 +1057,17 @@ __global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *__res
 
 // One workgroup per count block (grid = the batch's block count); the
 // item comes from a search over the item starts staged in LDS.
-__global__ void __launch_bounds__(256) k_count_batch(const McBatchItem *__restrict__ items,
-                                                     const McBatchHdr *__restrict__ hdr) {
+template <int NT>
+__global__ void __launch_bounds__(NT) k_count_batch(const McBatchItem *__restrict__ items,
+                                                    const McBatchHdr *__restrict__ hdr) {
     __shared__ int64_t starts[kMaxBatch];
     const int n = (int) hdr->n;
     const int64_t gb = blockIdx.x;
     if (gb >= hdr->nblk) return;
-    for (int i = threadIdx.x; i < n; i += 256) starts[i] = items[i].blk0;
+    for (int i = threadIdx.x; i < n; i += NT) starts[i] = items[i].blk0;
     __syncthreads();
     const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, gb));
-    count_block(items[i].p, gb - items[i].blk0);
+    count_block<NT>(items[i].p, gb - items[i].blk0);
 }
 
 __global__ void __launch_bounds__(1024) k_scan_top_batch(const McBatchItem *__restrict__ items,
@@ -1387,7 +1389,10 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(1)) != hipSuccess) return e;
 
-    hipLaunchKernelGGL(k_count, dim3((unsigned) p.nblk), dim3(256), 0, stream, p);
+    if (p.nblk < kCountWideBelow)
+        hipLaunchKernelGGL(k_count<1024>, dim3((unsigned) p.nblk), dim3(1024), 0, stream, p);
+    else
+        hipLaunchKernelGGL(k_count<256>, dim3((unsigned) p.nblk), dim3(256), 0, stream, p);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(2)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, p);
@@ -1416,7 +1421,10 @@ hipError_t its_dense_count_launch(McParams &p, hipStream_t stream) {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_its_reset, dim3(1), dim3(64), 0, stream, p.err, p.totals);
     launch_classify(p, stream);
-    hipLaunchKernelGGL(k_count, dim3((unsigned) p.nblk), dim3(256), 0, stream, p);
+    if (p.nblk < kCountWideBelow)
+        hipLaunchKernelGGL(k_count<1024>, dim3((unsigned) p.nblk), dim3(1024), 0, stream, p);
+    else
+        hipLaunchKernelGGL(k_count<256>, dim3((unsigned) p.nblk), dim3(256), 0, stream, p);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, p);
     return hipGetLastError();
 }
@@ -1447,7 +1455,10 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     hipLaunchKernelGGL(k_classify_batch, dim3(cb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const unsigned nb = (unsigned) std::max<int64_t>(1, h.nblk);
-    hipLaunchKernelGGL(k_count_batch, dim3(nb), dim3(256), 0, stream, items, hdr);
+    if (h.nblk < kCountWideBelow)
+        hipLaunchKernelGGL(k_count_batch<1024>, dim3(nb), dim3(1024), 0, stream, items, hdr);
+    else
+        hipLaunchKernelGGL(k_count_batch<256>, dim3(nb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const unsigned sb = (unsigned) std::max<int64_t>(1, std::min<int64_t>(h.n, 256));
     hipLaunchKernelGGL(k_scan_top_batch, dim3(sb), dim3(1024), 0, stream, items, hdr);

@@ -31,11 +31,11 @@ constexpr int kB = 256;
 static __constant__ uint64_t cSTris[2][256];
 
 __device__ __forceinline__ void cell_xyz(int64_t id, int64_t Y, int64_t Z, int64_t &x, int64_t &y, int64_t &z) {
-    int64_t cz = Z - 1, cy = Y - 1;
-    z = id % cz;
-    int64_t r = id / cz;
-    y = r % cy;
-    x = r / cy;
+    const int64_t cz = Z - 1, cy = Y - 1;
+    const int64_t r = div_fast(id, cz);
+    z = id - r * cz;
+    x = div_fast(r, cy);
+    y = r - x * cy;
 }
 
 __global__ void k_corner_points(const int64_t *ids, int64_t n, SparseGeom g, float *out) {
