@@ -258,6 +258,24 @@ __global__ void k_its_normals(CellSrc s, ItsIn in, float *normals) {
 
 // ---- QEF solve ----------------------------------------------------------------
 
+// Rotation parameters of a Jacobi step: t = tan, c = cos, s = sin.  The FP32
+// solve uses the hardware reciprocal / square-root approximations (about 1
+// ulp): its result is refined against an FP64 residual, which removes the
+// solver's error; the FP64 solve stays IEEE.
+__device__ __forceinline__ void jacobi_rot(float app, float aqq, float apq, float &c, float &s) {
+    const float theta = (aqq - app) * __builtin_amdgcn_rcpf(2.0f * apq);
+    const float t = (theta >= 0.0f ? 1.0f : -1.0f) *
+                    __builtin_amdgcn_rcpf(fabsf(theta) + __builtin_amdgcn_sqrtf(theta * theta + 1.0f));
+    c = __builtin_amdgcn_rsqf(t * t + 1.0f);
+    s = t * c;
+}
+__device__ __forceinline__ void jacobi_rot(double app, double aqq, double apq, double &c, double &s) {
+    const double theta = (aqq - app) / (2.0 * apq);
+    const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+    c = 1.0 / sqrt(t * t + 1.0);
+    s = t * c;
+}
+
 // Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (in registers).
 template <typename T>
 __device__ __forceinline__ void jacobi_sym3(T a[3][3], T w[3], T v[3][3], int sweeps) {
@@ -273,9 +291,8 @@ __device__ __forceinline__ void jacobi_sym3(T a[3][3], T w[3], T v[3][3], int sw
             const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
             T apq = a[p][q];
             if (apq == T(0)) continue;
-            T theta = (a[q][q] - a[p][p]) / (T(2) * apq);
-            T t = (theta >= T(0) ? T(1) : T(-1)) / (fabs(theta) + sqrt(theta * theta + T(1)));
-            T c = T(1) / sqrt(t * t + T(1)), s = t * c;
+            T c, s;
+            jacobi_rot(a[p][p], a[q][q], apq, c, s);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 T akp = a[k][p], akq = a[k][q];
@@ -457,15 +474,25 @@ __device__ __forceinline__ int64_t active_of(const ActHash &t, int64_t cid) {
 // Quad of entry e, if e is the canonical copy of an interior crossing edge:
 // the copy held by the cell whose min corner owns the edge (local edges 0,
 // 3, 8).  These copies are in (p0, p1) order, the reference's quad order.
-__device__ bool entry_quad(const ItsIn &in, const CellSrc &s, const ActHash &ah, int64_t e, int64_t q[4]) {
+// The 4 cells around a crossing edge all hold it, so on a dense grid they
+// are all active when they exist: counting there needs only the bounds
+// (lookups = false; a sparse grid may lack some of them, so it always looks
+// them up).  The entry's own cell is q[0] (its active index is
+// entry_cell[e]); the other three come from the active-cell hash.
+__device__ bool entry_quad(const ItsIn &in, const CellSrc &s, const ActHash &ah, int64_t e, int64_t q[4],
+                           bool lookups) {
     int k = in.edge_local[e];
     if (k != 0 && k != 3 && k != 8) return false;
     const int axis = edge_axis(k);
+    const int64_t own = in.entry_cell[e];
     int64_t x, y, z;
-    cell_xyz(s, in.cell_dense[in.entry_cell[e]], x, y, z);
-    for (int j = 0; j < 4; ++j) {
+    cell_xyz(s, in.cell_dense[own], x, y, z);
+    // offsets subtracted are 0 or 1: in bounds iff the lower cells exist
+    if ((axis != 0 && x < 1) || (axis != 1 && y < 1) || (axis != 2 && z < 1)) return false;
+    if (!lookups && !s.ids) return true;
+    q[0] = own;
+    for (int j = 1; j < 4; ++j) {
         int64_t cx = x - kNb[axis][j][0], cy = y - kNb[axis][j][1], cz = z - kNb[axis][j][2];
-        if (cx < 0 || cy < 0 || cz < 0 || cx >= s.X - 1 || cy >= s.Y - 1 || cz >= s.Z - 1) return false;
         q[j] = active_of(ah, (cx * (s.Y - 1) + cy) * (s.Z - 1) + cz);
         if (q[j] < 0) return false;
     }
@@ -483,7 +510,7 @@ __global__ void k_quad_count(ItsIn in, CellSrc s, ActHash ah, int64_t *bsum) {
     int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t nq = 0;
     int64_t q[4];
-    if (e < in.npts && entry_quad(in, s, ah, e, q)) nq = 1;
+    if (e < in.npts && entry_quad(in, s, ah, e, q, false)) nq = 1;
     int64_t eq, dummy, tq, td;
     block_scan2(nq, 0, eq, dummy, tq, td);
     if (threadIdx.x == 0) {
@@ -505,7 +532,7 @@ __global__ void k_quad_emit(ItsIn in, CellSrc s, ActHash ah, const int64_t *bsum
     int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t nq = 0;
     int64_t q[4] = {0, 0, 0, 0};
-    if (e < in.npts && entry_quad(in, s, ah, e, q)) nq = 1;
+    if (e < in.npts && entry_quad(in, s, ah, e, q, true)) nq = 1;
     int64_t eq, dummy, tq, td;
     block_scan2(nq, 0, eq, dummy, tq, td);
     if (!nq) return;
