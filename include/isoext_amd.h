@@ -253,6 +253,22 @@ int isoext_mc_batch_count(int n, const isoext_mc_item *items, int cfg, void *wor
 int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *workspace, int64_t workspace_bytes,
                          int64_t special_cap, int64_t *counts, void *stream);
 
+/* ---- OBJ output ------------------------------------------------------------- */
+
+/* write_obj (src/isoext/utils.py:42-63): writes "v x y z" per vertex and
+ * "f a b c" per face (indices + 1) to `path`, byte-identical to the
+ * reference's text: each coordinate is printed as Python's repr of the float
+ * value (shortest round-trip decimal; fixed notation for decimal exponents in
+ * (-4, 16], else d.ddde+XX).  v: nv x 3 doubles (float32 vertices widened
+ * exactly), f: nf x 3 zero-based indices, both in host memory.  An empty mesh
+ * (nv or nf 0) truncates the file and writes nothing.  Lines are formatted on
+ * `threads` host threads (<= 0: 8). */
+int isoext_write_obj(const char *path, const double *v, int64_t nv, const int64_t *f, int64_t nf, int threads);
+
+/* Python's repr of x into out (NUL-terminated); returns its length, or minus
+ * the length when cap is too small.  The formatter isoext_write_obj uses. */
+int64_t isoext_format_float(double x, char *out, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

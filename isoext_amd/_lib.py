@@ -63,6 +63,10 @@ def _declare(L):
     L.isoext_uniform_points.argtypes = [I64x3, F32x3, F32x3, vp, vp]
     L.isoext_fill_f32.restype = c_int
     L.isoext_fill_f32.argtypes = [vp, c_i64, c_f, vp]
+    L.isoext_write_obj.restype = c_int
+    L.isoext_write_obj.argtypes = [ctypes.c_char_p, vp, c_i64, vp, c_i64, c_int]
+    L.isoext_format_float.restype = c_i64
+    L.isoext_format_float.argtypes = [ctypes.c_double, ctypes.c_char_p, c_i64]
     for name, restype, argtypes in EXTRA_DECLS:
         fn = getattr(L, name, None)
         if fn is not None:

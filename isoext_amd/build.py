@@ -22,7 +22,8 @@ CSRC = HERE / "csrc"
 LIB = HERE / "libisoext_amd.so"
 ARCH = os.environ.get("ISOEXT_ARCH", "gfx950")
 
-SOURCES = ["capi.hip", "mc_dense.hip", "fixup.hip", "its_dc.hip", "sparse.hip", "backward.hip", "sdf.hip"]
+SOURCES = ["capi.hip", "mc_dense.hip", "fixup.hip", "its_dc.hip", "sparse.hip", "backward.hip", "sdf.hip",
+           "obj_writer.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

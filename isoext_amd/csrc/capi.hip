@@ -1017,3 +1017,8 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
 }
 
 }   // extern "C"
+
+// Error reporting for the library's other translation units.
+namespace iso {
+int set_error(int code, const char *what) { return fail(code, "%s", what); }
+}   // namespace iso

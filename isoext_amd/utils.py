@@ -3,7 +3,8 @@
 ``write_obj`` is the reference's OBJ format: one ``v x y z`` line per vertex
 with each float32 widened to a Python float and printed with repr, then
 ``f a b c`` lines with 1-based indices; a None or empty mesh writes an empty
-file (src/isoext/utils.py:42-63).  ``make_grid`` and ``gaussian_smooth`` are
+file (src/isoext/utils.py:42-63).  The text is produced by the library's
+multi-threaded C++ writer (``isoext_write_obj``).  ``make_grid`` and ``gaussian_smooth`` are
 the reference's input-side conveniences restated in torch.
 """
 
@@ -27,15 +28,25 @@ def gaussian_smooth(field: torch.Tensor, sigma: float = 1.0, kernel_size: int | 
     return F.conv3d(f5, k3).reshape(field.shape)
 
 
-def write_obj(obj_path: str, v: torch.Tensor, f: torch.Tensor) -> None:
-    """Write a mesh as Wavefront OBJ text (vertices, then 1-based faces)."""
-    with open(obj_path, "w") as out:
-        if v is None or f is None or v.numel() == 0 or f.numel() == 0:
-            return
-        verts = v.tolist()
-        faces = (f + 1).tolist()
-        out.writelines([f"v {a} {b} {c}\n" for a, b, c in verts])
-        out.writelines([f"f {a} {b} {c}\n" for a, b, c in faces])
+def write_obj(obj_path: str, v: torch.Tensor, f: torch.Tensor, threads: int = 8) -> None:
+    """Write a mesh as Wavefront OBJ text (vertices, then 1-based faces) with
+    the reference's exact bytes (src/isoext/utils.py:42-63): coordinates as
+    Python float reprs.  Formatting runs in the library's C++ writer on
+    `threads` host threads; an empty mesh leaves an empty file."""
+    import ctypes
+
+    from . import _lib
+    L = _lib.lib()
+    # opening in Python first raises exactly what the reference's open() raises
+    with open(obj_path, "w"):
+        pass
+    if v is None or f is None or v.numel() == 0 or f.numel() == 0:
+        return
+    path = str(obj_path).encode()
+    vh = v.detach().to("cpu", torch.float64).contiguous()
+    fh = f.detach().to("cpu", torch.int64).contiguous()
+    _lib.check(L.isoext_write_obj(path, ctypes.c_void_p(vh.data_ptr()), vh.shape[0],
+                                  ctypes.c_void_p(fh.data_ptr()), fh.shape[0], int(threads)), "write_obj")
 
 
 def make_grid(aabb: list[float], res: int | list[int], device: str = "cuda") -> torch.Tensor:
