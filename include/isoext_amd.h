@@ -198,6 +198,31 @@ int isoext_sparse_corner_points(const int64_t *cell_ids, int64_t n, const int64_
 int isoext_sparse_crossing_flags(const float *values, int64_t n, float level, uint8_t *flags,
                                  void *stream);
 
+/* ---- SparseGrid population (src/grid/sparse.cu:71-121) ----------------------- */
+
+/* Scratch bytes of isoext_sparse_union / isoext_sparse_difference. */
+int64_t isoext_sparse_setop_scratch_bytes(int64_t na, int64_t nb);
+
+/* SparseGrid::add_cells (src/grid/sparse.cu:71-96): sorted union of the
+ * grid's sorted unique ids a[na] with b_sorted[nb] (sorted, duplicates
+ * allowed).  out (capacity na + nb) receives the result, slot[k] the index in
+ * a of out[k] or -1 for a new id (slot may be NULL), *n_out (device memory)
+ * the result length.  Stream-ordered, no host synchronisation. */
+int isoext_sparse_union(const int64_t *a, int64_t na, const int64_t *b_sorted, int64_t nb, void *scratch,
+                        int64_t scratch_bytes, int64_t *out, int64_t *slot, int64_t *n_out, void *stream);
+
+/* SparseGrid::remove_cells (src/grid/sparse.cu:98-121): a minus b_sorted,
+ * same conventions (out capacity na). */
+int isoext_sparse_difference(const int64_t *a, int64_t na, const int64_t *b_sorted, int64_t nb, void *scratch,
+                             int64_t scratch_bytes, int64_t *out, int64_t *slot, int64_t *n_out, void *stream);
+
+/* Values after a set operation: row k of new_values (n x 8) = row slot[k] of
+ * old_values, or default_value where slot[k] < 0.  The reference resets
+ * every value instead (src/grid/sparse.cu:94-95,119-120); this keeps the
+ * values of surviving cells when the caller asks for it. */
+int isoext_sparse_gather_values(const float *old_values, const int64_t *slot, int64_t n, float default_value,
+                                float *new_values, void *stream);
+
 /* Workspace bytes for isoext_mc_sparse on n cells with `ninst` crossing-edge
  * instances (from isoext_mc_sparse_count). */
 int64_t isoext_mc_sparse_workspace_bytes(int64_t n, int64_t ninst);

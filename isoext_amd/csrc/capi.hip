@@ -864,6 +864,41 @@ int isoext_sparse_crossing_flags(const float *values, int64_t n, float level, ui
     return ISOEXT_OK;
 }
 
+int64_t isoext_sparse_setop_scratch_bytes(int64_t na, int64_t nb) {
+    if (na < 0 || nb < 0) return fail(ISOEXT_E_ARG, "negative sizes");
+    return iso::sparse_setop_scratch_bytes(na, nb);
+}
+
+static int sparse_setop_impl(const int64_t *a, int64_t na, const int64_t *b, int64_t nb, bool difference,
+                             void *scratch, int64_t scratch_bytes, int64_t *out, int64_t *slot, int64_t *n_out,
+                             void *stream) {
+    if (na < 0 || nb < 0 || (na > 0 && !a) || (nb > 0 && !b) || !out || !n_out)
+        return fail(ISOEXT_E_ARG, "invalid set-operation arguments");
+    if (scratch_bytes < iso::sparse_setop_scratch_bytes(na, nb))
+        return fail(ISOEXT_E_WORKSPACE, "set-operation scratch too small");
+    HIP_CHECK(iso::sparse_setop(a, na, b, nb, difference, scratch, out, slot, n_out, (hipStream_t) stream),
+              difference ? "sparse difference" : "sparse union");
+    return ISOEXT_OK;
+}
+
+int isoext_sparse_union(const int64_t *a, int64_t na, const int64_t *b_sorted, int64_t nb, void *scratch,
+                        int64_t scratch_bytes, int64_t *out, int64_t *slot, int64_t *n_out, void *stream) {
+    return sparse_setop_impl(a, na, b_sorted, nb, false, scratch, scratch_bytes, out, slot, n_out, stream);
+}
+
+int isoext_sparse_difference(const int64_t *a, int64_t na, const int64_t *b_sorted, int64_t nb, void *scratch,
+                             int64_t scratch_bytes, int64_t *out, int64_t *slot, int64_t *n_out, void *stream) {
+    return sparse_setop_impl(a, na, b_sorted, nb, true, scratch, scratch_bytes, out, slot, n_out, stream);
+}
+
+int isoext_sparse_gather_values(const float *old_values, const int64_t *slot, int64_t n, float default_value,
+                                float *new_values, void *stream) {
+    if (n < 0 || (n > 0 && (!slot || !new_values))) return fail(ISOEXT_E_ARG, "invalid gather arguments");
+    HIP_CHECK(iso::sparse_gather_rows(old_values, slot, n, default_value, new_values, (hipStream_t) stream),
+              "sparse gather values");
+    return ISOEXT_OK;
+}
+
 int64_t isoext_mc_sparse_workspace_bytes(int64_t n, int64_t ninst) {
     return (int64_t) iso::sparse_mc_work_bytes(n, ninst);
 }

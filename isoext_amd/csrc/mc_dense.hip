@@ -345,11 +345,17 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
                            bxyn = hn ? q[YW + p.W + 1] : 0ull;
             const CellMasks m = cell_masks(b, bn, by, byn, bx, bxn, bxy, bxyn);
             uint64_t act = cell_active(m) & low_bits(p.Z - 1 - zlo);
-            if (p.its) cv = (uint32_t) __popcll(act);
-            while (act) {
-                const int j = __ffsll((unsigned long long) act) - 1;
-                act &= act - 1;
-                cf += ntri[cell_case(m, j)];
+            if (p.its) {
+                // crossing edges of the active cells, bit-parallel over the 64
+                // cells: edge e crosses where its two corner masks differ
+                cv = (uint32_t) __popcll(act);
+                for_edges([&](auto E) { cf += (uint32_t) __popcll((m.m[edge_a(E)] ^ m.m[edge_b(E)]) & act); });
+            } else {
+                while (act) {
+                    const int j = __ffsll((unsigned long long) act) - 1;
+                    act &= act - 1;
+                    cf += ntri[cell_case(m, j)];
+                }
             }
         }
         const int e = (int) (r * p.S + div_fast(c, p.nzc) - i0);

@@ -346,6 +346,80 @@ __global__ void k_bit_add(int64_t *pre, int64_t nw, const int64_t *bsum) {
     if (i < nw) pre[i] += bsum[blockIdx.x];
 }
 
+// ---- population: sorted-key set operations (src/grid/sparse.cu:71-121) ----------
+
+// Bitmap of the elements a set operation keeps, one bit per element, written
+// a 64-bit wave ballot at a time (element i -> bit i).
+__device__ __forceinline__ bool found_sorted(const int64_t *a, int64_t n, int64_t key) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n && a[lo] == key;
+}
+
+// union: b (sorted) elements that start a run and are absent from a
+// difference: a elements absent from b (sorted)
+__global__ void k_setop_flags(const int64_t *a, int64_t na, const int64_t *b, int64_t nb, int difference,
+                              unsigned long long *bits) {
+    const int64_t n = difference ? na : nb;
+    for (int64_t i0 = (int64_t) blockIdx.x * kB; i0 < n; i0 += (int64_t) gridDim.x * kB) {
+        const int64_t i = i0 + threadIdx.x;
+        bool keep = false;
+        if (i < n) {
+            if (difference) keep = !found_sorted(b, nb, a[i]);
+            else keep = (i == 0 || b[i] != b[i - 1]) && !found_sorted(a, na, b[i]);
+        }
+        const unsigned long long m = __ballot(keep);
+        if ((threadIdx.x & 63) == 0 && i < n) bits[i >> 6] = m;
+    }
+}
+
+// Kept elements of `src` (bitmap + per-32-bit-word exclusive prefix) to
+// consecutive positions of `out`; `slot` (optional) gets the source index.
+__global__ void k_setop_compact(const int64_t *src, int64_t n, const uint32_t *bits, const int64_t *pre,
+                                int64_t *out, int64_t *slot) {
+    for (int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x; i < n; i += (int64_t) gridDim.x * kB) {
+        const uint32_t w = bits[i >> 5];
+        if (!((w >> (i & 31)) & 1)) continue;
+        const int64_t k = pre[i >> 5] + __popc(w & ((1u << (i & 31)) - 1));
+        out[k] = src[i];
+        if (slot) slot[k] = i;
+    }
+}
+
+// Merge of two sorted, disjoint arrays: output k takes a[i] or b[k - i], i the
+// number of a elements among the first k outputs (merge-path search on the
+// diagonal); slot[k] = i when it comes from a, -1 from b.
+__global__ void k_merge_disjoint(const int64_t *a, int64_t na, const int64_t *b, const int64_t *nb_p, int64_t *out,
+                                 int64_t *slot, int64_t *n_out) {
+    const int64_t nb = *nb_p, n = na + nb;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *n_out = n;
+    for (int64_t k = (int64_t) blockIdx.x * kB + threadIdx.x; k < n; k += (int64_t) gridDim.x * kB) {
+        int64_t lo = max((int64_t) 0, k - nb), hi = min(k, na);
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (a[mid] < b[k - 1 - mid]) lo = mid + 1;
+            else hi = mid;
+        }
+        const int64_t i = lo, j = k - lo;
+        const bool from_a = j >= nb || (i < na && a[i] < b[j]);
+        out[k] = from_a ? a[i] : b[j];
+        if (slot) slot[k] = from_a ? i : -1;
+    }
+}
+
+// Per-cell values after a set operation: the old row of the cell's source
+// slot, or the default value for a new cell.
+__global__ void k_gather_rows(const float *old_values, const int64_t *slot, int64_t n, float dflt, float *out) {
+    for (int64_t t = (int64_t) blockIdx.x * kB + threadIdx.x; t < 8 * n; t += (int64_t) gridDim.x * kB) {
+        const int64_t s = slot[t >> 3];
+        out[t] = s >= 0 ? old_values[8 * s + (t & 7)] : dflt;
+    }
+}
+
 unsigned nb(int64_t n) { return (unsigned) std::max<int64_t>(1, (n + kB - 1) / kB); }
 unsigned gs(int64_t n) { return (unsigned) std::max<int64_t>(1, std::min<int64_t>(8192, (n + kB - 1) / kB)); }
 
@@ -356,6 +430,50 @@ unsigned gs(int64_t n) { return (unsigned) std::max<int64_t>(1, std::min<int64_t
         hipError_t e_ = (x);             \
         if (e_ != hipSuccess) return e_; \
     } while (0)
+
+// Scratch of a set operation on na (or nb) keys: the bitmap, its word prefix,
+// the block sums, the compacted b and two counters.
+int64_t sparse_setop_scratch_bytes(int64_t na, int64_t nb) {
+    const int64_t n = std::max(na, nb), nw64 = n / 64 + 2, nw32 = 2 * nw64;
+    return 8 * (nw64 + nw32 + (nw32 + 1023) / 1024 + 8 + nb + 8) + 256;
+}
+
+hipError_t sparse_setop(const int64_t *a, int64_t na, const int64_t *b, int64_t nb, bool difference, void *scratch,
+                        int64_t *out, int64_t *slot, int64_t *n_out, hipStream_t st) {
+    const int64_t n = difference ? na : nb, nw64 = std::max(na, nb) / 64 + 2, nw32 = 2 * nw64;
+    char *p = (char *) scratch;
+    unsigned long long *bits = (unsigned long long *) p;
+    int64_t *pre = (int64_t *) (p + 8 * nw64);
+    int64_t *bsum = pre + nw32;
+    const int64_t nblk = (nw32 + 1023) / 1024;
+    int64_t *tot = bsum + nblk + 8;
+    int64_t *bc = tot + 8;   // compacted b (union)
+    hipLaunchKernelGGL(k_fill_u64, dim3(gs(nw64)), dim3(kB), 0, st, bits, nw64, 0ull);
+    if (n > 0)
+        hipLaunchKernelGGL(k_setop_flags, dim3(gs(n)), dim3(kB), 0, st, a, na, b, nb, difference ? 1 : 0, bits);
+    hipLaunchKernelGGL(k_bit_block, dim3((unsigned) nblk), dim3(1024), 0, st, (const uint32_t *) bits, nw32, pre, bsum);
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, bsum, nblk, tot);
+    hipLaunchKernelGGL(k_bit_add, dim3((unsigned) nblk), dim3(1024), 0, st, pre, nw32, bsum);
+    if (difference) {
+        if (na > 0)
+            hipLaunchKernelGGL(k_setop_compact, dim3(gs(na)), dim3(kB), 0, st, a, na, (const uint32_t *) bits, pre,
+                               out, slot);
+        ISO_TRY(hipMemcpyAsync(n_out, tot, sizeof(int64_t), hipMemcpyDeviceToDevice, st));
+    } else {
+        if (nb > 0)
+            hipLaunchKernelGGL(k_setop_compact, dim3(gs(nb)), dim3(kB), 0, st, b, nb, (const uint32_t *) bits, pre, bc,
+                               (int64_t *) nullptr);
+        hipLaunchKernelGGL(k_merge_disjoint, dim3(gs(na + nb)), dim3(kB), 0, st, a, na, bc, tot, out, slot, n_out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t sparse_gather_rows(const float *old_values, const int64_t *slot, int64_t n, float dflt, float *out,
+                              hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_rows, dim3(gs(8 * n)), dim3(kB), 0, st, old_values, slot, n, dflt, out);
+    return hipGetLastError();
+}
 
 hipError_t sparse_corner_points(const int64_t *ids, int64_t n, const SparseGeom &g, float *out, hipStream_t st) {
     if (n == 0) return hipSuccess;

@@ -12,6 +12,10 @@ Same semantics as the reference, including its quirks:
   cell count (src/grid/sparse.cu:128-142);
 * add_cells / remove_cells reset every value to default_value
   (src/grid/sparse.cu:94-95,119-120).
+Both are available without the quirk: get_potential_cell_indices(...,
+cells_only=True) enumerates cells, add_cells / remove_cells(...,
+keep_values=True) keep the values of surviving cells.  The id sets are built
+on the GPU by the library's sorted-key union / difference kernels.
 Empty results are returned as None, as the reference's empty ndarray.
 """
 
@@ -29,6 +33,10 @@ vp, i64, f32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_float
 _lib.declare("isoext_sparse_corner_points", ctypes.c_int, [vp, i64, _lib.I64x3, _lib.F32x3, _lib.F32x3, vp, vp])
 _lib.declare("isoext_sparse_crossing_flags", ctypes.c_int, [vp, i64, f32, vp, vp])
 _lib.declare("isoext_mc_sparse_workspace_bytes", i64, [i64, i64])
+_lib.declare("isoext_sparse_setop_scratch_bytes", i64, [i64, i64])
+_lib.declare("isoext_sparse_union", ctypes.c_int, [vp, i64, vp, i64, vp, i64, vp, vp, vp, vp])
+_lib.declare("isoext_sparse_difference", ctypes.c_int, [vp, i64, vp, i64, vp, i64, vp, vp, vp, vp])
+_lib.declare("isoext_sparse_gather_values", ctypes.c_int, [vp, vp, i64, f32, vp, vp])
 
 INT32_MAX = 2**31 - 1
 
@@ -128,25 +136,60 @@ class SparseGrid(Grid):
         self.values = torch.full((self.cell_ids.numel(), 8), self.default_value, dtype=torch.float32,
                                  device=self.device)
 
-    def add_cells(self, new_cell_indices: torch.Tensor) -> None:
-        """Union with the given cells (sorted, unique); resets all values."""
-        ids = _ids_in(new_cell_indices, "new_cell_indices")
-        self.cell_ids = torch.unique(torch.cat([self.cell_ids, ids]), sorted=True)
-        self._reset_values()
+    def _setop(self, ids: torch.Tensor, difference: bool, keep_values: bool):
+        """Sorted-key union (add) or difference (remove) on the GPU; values
+        reset to default_value as in the reference, or carried over for the
+        surviving cells with keep_values."""
+        L = _lib.lib()
+        a = self.cell_ids.contiguous()
+        b = torch.sort(ids).values.contiguous()
+        na, nb = a.numel(), b.numel()
+        cap = na if difference else na + nb
+        out = torch.empty(max(cap, 1), dtype=torch.int64, device=self.device)
+        slot = torch.empty(max(cap, 1), dtype=torch.int64, device=self.device) if keep_values else None
+        n_out = torch.zeros(1, dtype=torch.int64, device=self.device)
+        scratch = workspace.get("sparse_setop", L.isoext_sparse_setop_scratch_bytes(na, nb), self.device)
+        fn = L.isoext_sparse_difference if difference else L.isoext_sparse_union
+        with _lib.on_device(self.device):
+            stream = _lib.stream_ptr(self.device)
+            _lib.check(fn(a.data_ptr() if na else None, na, b.data_ptr() if nb else None, nb, scratch.data_ptr(),
+                          scratch.numel(), out.data_ptr(), slot.data_ptr() if slot is not None else None,
+                          n_out.data_ptr(), stream), "remove_cells" if difference else "add_cells")
+            n = int(n_out.item())
+            if keep_values:
+                vals = torch.empty((n, 8), dtype=torch.float32, device=self.device)
+                _lib.check(L.isoext_sparse_gather_values(self.values.data_ptr() if na else None,
+                                                         slot.data_ptr(), n, float(self.default_value),
+                                                         vals.data_ptr(), stream), "cell values")
+        self.cell_ids = out[:n].clone() if cap > 2 * n + 65536 else out[:n]
+        if keep_values:
+            self.values = vals
+        else:
+            self._reset_values()
 
-    def remove_cells(self, new_cell_indices: torch.Tensor) -> None:
-        """Set difference with the given cells; resets all values."""
+    def add_cells(self, new_cell_indices: torch.Tensor, keep_values: bool = False) -> None:
+        """Union with the given cells (any order, duplicates allowed).  Resets
+        all values to default_value like the reference
+        (src/grid/sparse.cu:71-96); keep_values=True keeps the values of the
+        cells already present (new cells get default_value)."""
         ids = _ids_in(new_cell_indices, "new_cell_indices")
-        keep = ~torch.isin(self.cell_ids, ids)
-        self.cell_ids = self.cell_ids[keep].contiguous()
-        self._reset_values()
+        self._setop(ids, False, keep_values)
 
-    def get_potential_cell_indices(self, chunk_size: int):
-        """Chunks of consecutive ids covering [0, X*Y*Z) (the reference's range)."""
+    def remove_cells(self, new_cell_indices: torch.Tensor, keep_values: bool = False) -> None:
+        """Set difference with the given cells (src/grid/sparse.cu:98-121);
+        values reset as in the reference unless keep_values=True."""
+        ids = _ids_in(new_cell_indices, "new_cell_indices")
+        self._setop(ids, True, keep_values)
+
+    def get_potential_cell_indices(self, chunk_size: int, cells_only: bool = False):
+        """Chunks of consecutive ids covering [0, X*Y*Z) -- the reference's
+        range, which counts points (src/grid/sparse.cu:128-142); with
+        cells_only=True the cell range [0, (X-1)(Y-1)(Z-1)) instead."""
         chunk_size = int(chunk_size)
         if chunk_size <= 0:
             raise ValueError("chunk_size must be positive")
-        total = self.shape[0] * self.shape[1] * self.shape[2]
+        X, Y, Z = self.shape
+        total = (X - 1) * (Y - 1) * (Z - 1) if cells_only else X * Y * Z
         dt = torch.int32 if total - 1 <= INT32_MAX else torch.int64
         return [torch.arange(s, min(s + chunk_size, total), device=self.device, dtype=dt)
                 for s in range(0, total, chunk_size)]
