@@ -19,7 +19,7 @@ from dataclasses import dataclass
 
 import torch
 
-from . import _lib, autotune, workspace
+from . import _lib, workspace
 
 
 class McItem(ctypes.Structure):
@@ -66,7 +66,8 @@ BATCH_SPECIAL_CAP = 4096
 
 def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_batch"):
     """Marching cubes over a ragged list of dense grids.  Returns a list of
-    (v, f) pairs (zero-row tensors for empty meshes)."""
+    (v, f) pairs (zero-row tensors for empty meshes).  cfg: one emit tile
+    configuration for every grid, or None for each grid's own."""
     if not jobs:
         return []
     L = _decl()
@@ -84,7 +85,7 @@ def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_bat
         it.level = float(j.level)
         it.method = int(j.method)
     if cfg is None:
-        cfg = autotune.mc_config(max((tuple(j.values.shape) for j in jobs), key=lambda s: s[0] * s[1] * s[2]))
+        cfg = -1   # each grid uses its own tile configuration
     special = BATCH_SPECIAL_CAP
     counts = (ctypes.c_int64 * (2 * n))()
     stream = _lib.stream_ptr(dev)
@@ -110,7 +111,8 @@ def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_bat
         if nv < 0:   # special lists overflowed the batch capacity: run this grid alone
             from .mc import mc_dense_raw
             j = jobs[i]
-            v, f, _ = mc_dense_raw(j.values, tuple(j.values.shape), j.aabb_min, j.aabb_max, j.level, j.method, cfg)
+            v, f, _ = mc_dense_raw(j.values, tuple(j.values.shape), j.aabb_min, j.aabb_max, j.level, j.method,
+                                   cfg if cfg >= 0 else None)
             res.append((v, f))
         else:
             res.append((v[:nv], f[:nf]))

@@ -288,7 +288,8 @@ constexpr int kBatchSeg = iso::kMaxSpSegBatch;   // special-list segments per ba
 
 struct ItemLayout {
     bool cells;
-    int64_t W, S, NI, NJ, NK, nblk, nwords, ntiles, seg_cap[4];
+    int cfg, nzc;   // the item's emit tile configuration
+    int64_t W, S, NI, NJ, NK, nblk, nwords, ntiles, tile0, seg_cap[4];
     size_t o_hdr, o_bits, o_cnt_v, o_cnt_f, o_bsum_v, o_bsum_f, o_blk_v, o_blk_f;
     size_t o_ctr, o_scand, o_sscell, o_skept, o_shole, o_cand, o_scell, o_kept, o_hole;
 };
@@ -300,14 +301,20 @@ inline int64_t word_span(int64_t nwords) {
 }
 
 struct BatchLayout {
-    size_t o_hdr, o_items, o_summary, total;
+    size_t o_hdr, o_items, o_order, o_summary, total;
     std::vector<ItemLayout> it;
     iso::McBatchHdr h;
+    // emit tiles are laid out by configuration: items of configuration c own
+    // tiles [tile_begin[c], tile_end[c]); order lists the items by tile start
+    std::vector<int64_t> tile_begin, tile_end;
+    std::vector<int32_t> order;
 };
 
+// cfg >= 0: every item uses that tile configuration; cfg < 0: each item its
+// own (iso::mc_auto_config of its shape).
 bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_cap, BatchLayout &B) {
     int tx, ty, nzc;
-    if (!iso::mc_config_geom(cfg, tx, ty, nzc)) return false;
+    if (cfg >= 0 && !iso::mc_config_geom(cfg, tx, ty, nzc)) return false;
     size_t o = 0;
     auto take = [&](size_t bytes) {
         size_t r = o;
@@ -316,12 +323,16 @@ bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_c
     };
     B.o_hdr = take(sizeof(iso::McBatchHdr));
     B.o_items = take(sizeof(iso::McBatchItem) * (size_t) n);
+    B.o_order = take(sizeof(int32_t) * (size_t) n);
     B.o_summary = take(sizeof(iso::McBatchSummary) * (size_t) n);
     B.it.assign(n, ItemLayout{});
     B.h = iso::McBatchHdr{n, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
         ItemLayout &L = B.it[i];
         const int64_t X = items[i].shape[0], Y = items[i].shape[1], Z = items[i].shape[2];
+        L.cfg = cfg >= 0 ? cfg : iso::mc_auto_config(items[i].shape);
+        iso::mc_config_geom(L.cfg, tx, ty, nzc);
+        L.nzc = nzc;
         L.cells = X >= 2 && Y >= 2 && Z >= 2;
         L.W = L.cells ? (Z + 63) / 64 : 0;
         L.S = L.cells ? (L.W + nzc - 1) / nzc : 0;
@@ -353,6 +364,21 @@ bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_c
         L.o_scell = take(sizeof(iso::SpecialEdge) * kBatchSeg * L.seg_cap[1]);
         L.o_kept = take(sizeof(int64_t) * kBatchSeg * L.seg_cap[2]);
         L.o_hole = take(sizeof(int64_t) * kBatchSeg * L.seg_cap[3]);
+    }
+    const int ncfg = iso::mc_config_count();
+    B.tile_begin.assign(ncfg, 0);
+    B.tile_end.assign(ncfg, 0);
+    B.order.clear();
+    int64_t t = 0;
+    for (int c = 0; c < ncfg; ++c) {
+        B.tile_begin[c] = t;
+        for (int i = 0; i < n; ++i) {
+            if (B.it[i].cfg != c) continue;
+            B.it[i].tile0 = t;
+            t += B.it[i].ntiles;
+            B.order.push_back(i);
+        }
+        B.tile_end[c] = t;
     }
     B.total = o;
     return true;
@@ -463,27 +489,28 @@ void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t 
 
 // Upload the descriptors of a batch phase (pinned staging) and return the
 // staging area for the summaries.
-int batch_upload(int n, const isoext_mc_item *items, int cfg, char *ws, int64_t special_cap, const BatchLayout &B,
-                 hipStream_t st, iso::McBatchSummary **summary_host) {
-    int tx, ty, nzc;
-    iso::mc_config_geom(cfg, tx, ty, nzc);
-    const size_t desc = sizeof(iso::McBatchHdr) + sizeof(iso::McBatchItem) * (size_t) n;
+int batch_upload(int n, const isoext_mc_item *items, char *ws, const BatchLayout &B, hipStream_t st,
+                 iso::McBatchSummary **summary_host) {
+    const size_t desc = sizeof(iso::McBatchHdr) + sizeof(iso::McBatchItem) * (size_t) n + sizeof(int32_t) * (size_t) n;
     const size_t sum = sizeof(iso::McBatchSummary) * (size_t) n;
     char *host = (char *) g_pinned.get(desc + sum + 64);
     if (!host) return fail(ISOEXT_E_DEVICE, "pinned staging allocation failed");
     iso::McBatchHdr *h = (iso::McBatchHdr *) host;
     *h = B.h;
     iso::McBatchItem *it = (iso::McBatchItem *) (host + sizeof(iso::McBatchHdr));
-    int64_t w0 = 0, b0 = 0, t0 = 0;
+    int64_t w0 = 0, b0 = 0;
     for (int i = 0; i < n; ++i) {
-        fill_item(items[i], B.it[i], ws, w0, b0, t0, it[i]);
-        it[i].p.nzc = nzc;
+        fill_item(items[i], B.it[i], ws, w0, b0, B.it[i].tile0, it[i]);
+        it[i].p.nzc = B.it[i].nzc;
         w0 += word_span(B.it[i].nwords);
         b0 += B.it[i].nblk;
-        t0 += B.it[i].ntiles;
     }
+    int32_t *order = (int32_t *) (it + n);
+    for (int k = 0; k < n; ++k) order[k] = B.order[k];
     HIP_CHECK(hipMemcpyAsync(ws + B.o_hdr, h, sizeof(iso::McBatchHdr), hipMemcpyHostToDevice, st), "upload batch");
     HIP_CHECK(hipMemcpyAsync(ws + B.o_items, it, sizeof(iso::McBatchItem) * (size_t) n, hipMemcpyHostToDevice, st),
+              "upload batch");
+    HIP_CHECK(hipMemcpyAsync(ws + B.o_order, order, sizeof(int32_t) * (size_t) n, hipMemcpyHostToDevice, st),
               "upload batch");
     *summary_host = (iso::McBatchSummary *) (host + ((desc + 63) & ~size_t(63)));
     return ISOEXT_OK;
@@ -493,7 +520,7 @@ int batch_check(int n, const isoext_mc_item *items, int cfg) {
     int tx, ty, nzc;
     if (n < 1) return fail(ISOEXT_E_ARG, "empty batch");
     if (n > iso::kMaxBatch) return fail(ISOEXT_E_ARG, "batch of %d grids exceeds %d", n, iso::kMaxBatch);
-    if (!iso::mc_config_geom(cfg, tx, ty, nzc)) return fail(ISOEXT_E_ARG, "unknown tile config %d", cfg);
+    if (cfg >= 0 && !iso::mc_config_geom(cfg, tx, ty, nzc)) return fail(ISOEXT_E_ARG, "unknown tile config %d", cfg);
     for (int i = 0; i < n; ++i) {
         if (items[i].method != 0 && items[i].method != 1)
             return fail(ISOEXT_E_ARG, "Unknown method: %d", items[i].method);
@@ -985,7 +1012,7 @@ int isoext_mc_batch_count(int n, const isoext_mc_item *items, int cfg, void *wor
     hipStream_t st = (hipStream_t) stream;
     char *ws = (char *) workspace;
     iso::McBatchSummary *sum = nullptr;
-    if ((rc = batch_upload(n, items, cfg, ws, special_cap, B, st, &sum)) != ISOEXT_OK) return rc;
+    if ((rc = batch_upload(n, items, ws, B, st, &sum)) != ISOEXT_OK) return rc;
     HIP_CHECK(iso::mc_batch_count_launch((const iso::McBatchItem *) (ws + B.o_items),
                                          (const iso::McBatchHdr *) (ws + B.o_hdr), B.h,
                                          (iso::McBatchSummary *) (ws + B.o_summary), st),
@@ -1015,10 +1042,11 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
     hipStream_t st = (hipStream_t) stream;
     char *ws = (char *) workspace;
     iso::McBatchSummary *sum = nullptr;
-    if ((rc = batch_upload(n, items, cfg, ws, special_cap, B, st, &sum)) != ISOEXT_OK) return rc;
-    HIP_CHECK(iso::mc_batch_emit_launch(cfg, (const iso::McBatchItem *) (ws + B.o_items),
-                                        (const iso::McBatchHdr *) (ws + B.o_hdr), B.h,
-                                        (iso::McBatchSummary *) (ws + B.o_summary), st),
+    if ((rc = batch_upload(n, items, ws, B, st, &sum)) != ISOEXT_OK) return rc;
+    HIP_CHECK(iso::mc_batch_emit_launch(B.tile_begin.data(), B.tile_end.data(),
+                                        (const iso::McBatchItem *) (ws + B.o_items),
+                                        (const int32_t *) (ws + B.o_order), (const iso::McBatchHdr *) (ws + B.o_hdr),
+                                        B.h, (iso::McBatchSummary *) (ws + B.o_summary), st),
               "launch batch emit");
     HIP_CHECK(hipMemcpyAsync(sum, ws + B.o_summary, sizeof(iso::McBatchSummary) * (size_t) n, hipMemcpyDeviceToHost,
                              st),

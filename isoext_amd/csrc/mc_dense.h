@@ -147,7 +147,11 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
 // Both end with a kernel that writes one McBatchSummary per item.
 hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr, const McBatchHdr &h,
                                  McBatchSummary *summary, hipStream_t stream);
-hipError_t mc_batch_emit_launch(int cfg, const McBatchItem *items, const McBatchHdr *hdr, const McBatchHdr &h,
+// Items may use different tile configurations: configuration c owns the
+// emit tiles [tile_begin[c], tile_end[c]) (one launch each); order lists the
+// items by first tile.
+hipError_t mc_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_end, const McBatchItem *items,
+                                const int32_t *order, const McBatchHdr *hdr, const McBatchHdr &h,
                                 McBatchSummary *summary, hipStream_t stream);
 
 }   // namespace iso

@@ -1085,17 +1085,19 @@ __global__ void __launch_bounds__(1024) k_scan_top_batch(const McBatchItem *__re
     }
 }
 
-// One workgroup per emit tile (grid = the batch's tile count).
+// One workgroup per emit tile of the items using this configuration (tiles
+// [t_begin, t_begin + grid)); items are found in tile order.
 template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_emit_batch(const McBatchItem *__restrict__ items,
-                                                   const McBatchHdr *__restrict__ hdr) {
+                                                   const int32_t *__restrict__ order,
+                                                   const McBatchHdr *__restrict__ hdr, int64_t t_begin) {
     __shared__ int64_t starts[kMaxBatch];
     const int n = (int) hdr->n;
-    const int64_t gt = blockIdx.x;
+    const int64_t gt = t_begin + blockIdx.x;
     if (gt >= hdr->ntiles) return;
-    for (int i = threadIdx.x; i < n; i += NT) starts[i] = items[i].tile0;
+    for (int k = threadIdx.x; k < n; k += NT) starts[k] = items[order[k]].tile0;
     __syncthreads();
-    const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, gt));
+    const int i = order[__builtin_amdgcn_readfirstlane(find_start(starts, n, gt))];
     const int64_t t = gt - items[i].tile0;
     emit_tile<TX, TY, NZC, NT>(items[i].p, t, (int) (t % items[i].p.seg.nseg));
 }
@@ -1474,21 +1476,26 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     return hipGetLastError();
 }
 
-hipError_t mc_batch_emit_launch(int cfg, const McBatchItem *items, const McBatchHdr *hdr, const McBatchHdr &h,
+hipError_t mc_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_end, const McBatchItem *items,
+                                const int32_t *order, const McBatchHdr *hdr, const McBatchHdr &h,
                                 McBatchSummary *summary, hipStream_t stream) {
     hipError_t e = hipSuccess;
-    const unsigned tb = (unsigned) std::max<int64_t>(1, h.ntiles);
-    switch (cfg) {
-#define X(id, tx, ty, nzc, nt)                                                                           \
-    case id:                                                                                             \
-        hipLaunchKernelGGL((k_emit_batch<tx, ty, nzc, nt>), dim3(tb), dim3(nt), 0, stream, items, hdr);  \
+    for (int c = 0; c < mc_config_count(); ++c) {
+        const int64_t nt = tile_end[c] - tile_begin[c];
+        if (nt <= 0) continue;
+        switch (c) {
+#define X(id, tx, ty, nzc, nth)                                                                              \
+    case id:                                                                                                 \
+        hipLaunchKernelGGL((k_emit_batch<tx, ty, nzc, nth>), dim3((unsigned) nt), dim3(nth), 0, stream, items, \
+                           order, hdr, tile_begin[c]);                                                       \
         break;
-        ISO_MC_CONFIGS(X)
+            ISO_MC_CONFIGS(X)
 #undef X
-    default:
-        return hipErrorInvalidValue;
+        default:
+            return hipErrorInvalidValue;
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sp_gather_batch, dim3(kMaxSpSegBatch, (unsigned) h.n), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_batch_summary, dim3((unsigned) ((h.n + 255) / 256)), dim3(256), 0, stream, items, hdr,

@@ -68,3 +68,27 @@ def test_batch_degenerate_and_snapped_items(gpu):
         got = canon(*out[k])
         ref = alone(jobs[k].values, jobs[k].level, 0)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+
+
+def test_batch_mixed_tile_configs(gpu):
+    """Grids of very different sizes each keep their own tile configuration
+    (separate emit launches per configuration) and still equal their alone
+    results."""
+    import torch
+
+    from isoext_amd import _lib, workloads
+    from isoext_amd.batch import GridJob, mc_batch
+    shapes = [(20, 30, 40), (300, 290, 310), (40, 40, 40), (520, 520, 128), (7, 90, 65), (600, 600, 300)]
+    L = _lib.lib()
+    cfgs = {int(L.isoext_mc_auto_config(_lib.i64x3(s))) for s in shapes}
+    assert len(cfgs) >= 3
+    jobs = []
+    for k, s in enumerate(shapes):
+        t = torch.empty(s, dtype=torch.float32, device="cuda")
+        workloads.multi_sphere(s, 6, 100 + k, out=t)
+        jobs.append(GridJob(t, 0.0, k % 2))
+    out = mc_batch(jobs)
+    for j, (v, f) in zip(jobs, out):
+        got = canon(v, f)
+        ref = alone(j.values, j.level, j.method)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
