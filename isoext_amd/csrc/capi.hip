@@ -301,7 +301,7 @@ inline int64_t word_span(int64_t nwords) {
 }
 
 struct BatchLayout {
-    size_t o_hdr, o_items, o_order, o_summary, total;
+    size_t o_hdr, o_items, o_order, o_summary, o_fixcnt, total;
     std::vector<ItemLayout> it;
     iso::McBatchHdr h;
     // emit tiles are laid out by configuration: items of configuration c own
@@ -325,6 +325,7 @@ bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_c
     B.o_items = take(sizeof(iso::McBatchItem) * (size_t) n);
     B.o_order = take(sizeof(int32_t) * (size_t) n);
     B.o_summary = take(sizeof(iso::McBatchSummary) * (size_t) n);
+    B.o_fixcnt = take(sizeof(int64_t) * 4 * (size_t) n);
     B.it.assign(n, ItemLayout{});
     B.h = iso::McBatchHdr{n, 0, 0, 0};
     for (int i = 0; i < n; ++i) {
@@ -1052,28 +1053,41 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
                              st),
               "read batch counts");
     HIP_CHECK(hipStreamSynchronize(st), "batch emit");
+    // weld fix-ups of every item that needs one, back to back on the stream,
+    // then one read of their final counts
+    int64_t *fix_dev = (int64_t *) (ws + B.o_fixcnt);
+    std::vector<int> fixed;
     for (int i = 0; i < n; ++i) {
-        int64_t nv = sum[i].nv, nf = sum[i].nf;
+        const int64_t nv = sum[i].nv, nf = sum[i].nf;
         counts[2 * i] = nv;
         counts[2 * i + 1] = nf;
         if (nv > items[i].cap_v || nf > items[i].cap_f)
             return fail(ISOEXT_E_CAPACITY, "item %d: output capacity too small", i);
-        if (sum[i].need > special_cap) {   // special lists overflowed: the caller re-runs this item alone
+        if (sum[i].sp.need > special_cap) {   // special lists overflowed: the caller re-runs this item alone
             counts[2 * i] = counts[2 * i + 1] = -1;
             continue;
         }
-        if (sum[i].n_cand > 0 || sum[i].n_hole > 0) {
-            iso::SpecialLists sp_dev;
-            HIP_CHECK(hipMemcpyAsync(&sp_dev, ws + B.it[i].o_hdr + kOffSp, sizeof(sp_dev), hipMemcpyDeviceToHost, st),
-                      "read special lists");
-            HIP_CHECK(hipStreamSynchronize(st), "read special lists");
+        if (sum[i].sp.n_cand > 0 || sum[i].sp.n_hole > 0) {
             const ItemLayout &IL = B.it[i];
             iso::FixupGrid g{items[i].shape[0], items[i].shape[1], items[i].shape[2],
                              (const int64_t *) (ws + IL.o_blk_v), (const int64_t *) (ws + IL.o_blk_f),
                              (const uint32_t *) (ws + IL.o_cnt_v), (const uint32_t *) (ws + IL.o_cnt_f), IL.S};
-            HIP_CHECK(iso::weld_fixup(sp_dev, items[i].out_v, items[i].out_f, nv, nf, g, st, nullptr), "weld fix-up");
-            counts[2 * i] = nv;
-            counts[2 * i + 1] = nf;
+            HIP_CHECK(iso::weld_fixup_async(sum[i].sp, items[i].out_v, items[i].out_f, nv, nf, g, st, nullptr,
+                                            fix_dev + 4 * i),
+                      "weld fix-up");
+            fixed.push_back(i);
+        }
+    }
+    if (!fixed.empty()) {
+        int64_t *fc = (int64_t *) g_pinned_rb.get(sizeof(int64_t) * 4 * (size_t) n);
+        if (!fc) return fail(ISOEXT_E_DEVICE, "pinned read-back allocation failed");
+        HIP_CHECK(hipMemcpyAsync(fc, fix_dev, sizeof(int64_t) * 4 * (size_t) n, hipMemcpyDeviceToHost, st),
+                  "read fix-up counts");
+        HIP_CHECK(hipStreamSynchronize(st), "weld fix-up");
+        for (int i : fixed) {
+            if (fc[4 * i + 2]) return fail(ISOEXT_E_DEVICE, "item %d: inconsistent special lists", i);
+            counts[2 * i] = fc[4 * i];
+            counts[2 * i + 1] = fc[4 * i + 1];
         }
     }
     return ISOEXT_OK;

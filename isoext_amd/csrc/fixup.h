@@ -20,5 +20,11 @@ struct FixupGrid {
 
 hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
                       const FixupGrid &g, hipStream_t st, int64_t *keys = nullptr);
+// Stream-ordered variant: no host synchronisation (except to grow the
+// scratch); final_dev[0], [1] receive the final vertex / face counts and
+// final_dev[2] is non-zero when the lists were inconsistent.  nv / nf are the
+// provisional counts.
+hipError_t weld_fixup_async(const SpecialLists &sp, float *v, int32_t *f, int64_t nv, int64_t nf,
+                            const FixupGrid &g, hipStream_t st, int64_t *keys, int64_t *final_dev);
 
 }   // namespace iso

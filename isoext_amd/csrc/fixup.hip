@@ -403,14 +403,44 @@ hipError_t bitmap_prefix(const uint32_t *bits, int64_t nw, int64_t *pre, int64_t
         if (e_ != hipSuccess) return e_;         \
     } while (0)
 
+hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
+                           const FixupGrid &g, hipStream_t st, int64_t *vkeys, int64_t *final_dev);
+
+__global__ void k_fix_final(const int64_t *misc, int64_t *final_dev) {
+    if (threadIdx.x == 0) {
+        final_dev[0] = misc[2];
+        final_dev[1] = misc[3];
+        final_dev[2] = misc[4];
+    }
+}
+
 // Resolves the special lists in place.  nv / nf hold the provisional counts on
 // entry and the final counts on return.  Synchronises the stream once, at
 // the end.
 hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
                       const FixupGrid &g, hipStream_t st, int64_t *vkeys) {
+    return weld_fixup_impl(sp, v, f, nv, nf, g, st, vkeys, nullptr);
+}
+
+hipError_t weld_fixup_async(const SpecialLists &sp, float *v, int32_t *f, int64_t nv, int64_t nf,
+                            const FixupGrid &g, hipStream_t st, int64_t *vkeys, int64_t *final_dev) {
+    return weld_fixup_impl(sp, v, f, nv, nf, g, st, vkeys, final_dev);
+}
+
+// final_dev == nullptr: read the final counts back (one sync) into nv / nf;
+// else leave them in final_dev.
+hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
+                           const FixupGrid &g, hipStream_t st, int64_t *vkeys, int64_t *final_dev) {
     const int64_t ncand = (int64_t) sp.n_cand, nscell = (int64_t) sp.n_scell;
     const int64_t nkept = (int64_t) sp.n_kept, nhole = (int64_t) sp.n_hole;
-    if (ncand == 0 && nhole == 0) return hipSuccess;
+    if (ncand == 0 && nhole == 0) {
+        if (final_dev) {
+            const int64_t h[3] = {nv, nf, 0};
+            ISO_TRY(hipMemcpyAsync(final_dev, h, sizeof(h), hipMemcpyHostToDevice, st));
+            ISO_TRY(hipStreamSynchronize(st));   // h lives on this stack frame
+        }
+        return hipSuccess;
+    }
     const int64_t nslots = pow2_at_least(2 * (ncand + nscell) + 16), mask = nslots - 1;
     const int64_t npslots = pow2_at_least(2 * ncand + 16), pmask = npslots - 1;
     const int64_t vw = nv / 32 + 2, fw = nf / 32 + 2;
@@ -481,6 +511,10 @@ hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv,
     hipLaunchKernelGGL(k_reloc, dim3(grid_for(nfrel)), dim3(256), 0, st, fkeep, fpre, fw, nf, misc + 3, nfrel,
                        rf_src, rf_dst, (float *) nullptr, (int64_t *) nullptr, f, misc + 4);
     ISO_TRY(hipGetLastError());
+    if (final_dev) {
+        hipLaunchKernelGGL(k_fix_final, dim3(1), dim3(64), 0, st, misc, final_dev);
+        return hipGetLastError();
+    }
     // final counts through a small pinned buffer of the calling thread
     struct PinnedCounts {
         int64_t *p = nullptr;

@@ -123,8 +123,8 @@ struct McBatchHdr {
 
 // Per-item result of a batch phase, gathered for one read-back.
 struct McBatchSummary {
-    int64_t err, nv, nf, need;
-    unsigned long long n_cand, n_hole;
+    int64_t err, nv, nf;
+    SpecialLists sp;   // the item's contiguous special lists (counts, need)
 };
 
 struct ItsOut;

@@ -61,6 +61,28 @@ __device__ __forceinline__ uint64_t low_bits(int64_t n) {
     return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1));
 }
 
+// Position of the r-th set bit (r < popcount(m)).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
+    int pos = 0;
+    uint32_t x = (uint32_t) m;
+    int c = __popc(x);
+    if (r >= c) {
+        r -= c;
+        x = (uint32_t) (m >> 32);
+        pos = 32;
+    }
+#pragma unroll
+    for (int half = 16; half >= 1; half >>= 1) {
+        c = __popc(x & ((1u << half) - 1));
+        if (r >= c) {
+            r -= c;
+            x >>= half;
+            pos += half;
+        }
+    }
+    return pos;
+}
+
 // Vertices owned by the points of one 64-point chunk of a point row: b the
 // chunk's mask, bn0 bit 0 of the next chunk, bx / by the masks of the rows at
 // x+1 / y+1 (0 when absent), zlo the chunk's first z.
@@ -329,38 +351,86 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
     const int64_t g0 = entry_chunk(p, i0);
     const int64_t g1 = i1 < n ? entry_chunk(p, i1) : p.X * p.Y * p.W;
     const int64_t YW = p.Y * p.W;
-    for (int64_t g = g0 + tid; g < g1; g += NT) {
-        const int64_t r = div_fast(g, p.W), c = g - r * p.W;
+    const int lane = tid & 63, wave = tid >> 6;
+    __shared__ int32_t mark[NT / 64][64];
+    for (int64_t gb = g0; gb < g1; gb += NT) {   // every lane runs every round (wave-wide steps below)
+        const int64_t g = gb + tid;
+        const bool valid = g < g1;
+        const int64_t gg = valid ? g : g0;
+        const int64_t r = div_fast(gg, p.W), c = gg - r * p.W;
         const int64_t x = div_fast(r, p.Y), y = r - x * p.Y;
         const bool hasx = x + 1 < p.X, hasy = y + 1 < p.Y, hn = c + 1 < p.W;
         const int64_t zlo = 64 * c;
-        const uint64_t *q = p.bits + g;
+        const uint64_t *q = p.bits + gg;
         const uint64_t b = q[0], bn = hn ? q[1] : 0ull;
         const uint64_t bx = hasx ? q[YW] : 0ull, by = hasy ? q[p.W] : 0ull;
         uint32_t cv = p.its ? 0u : (uint32_t) chunk_vertices(b, bn & 1, bx, by, hasx, hasy, zlo, p.Z);
         uint32_t cf = 0;
+        uint64_t act = 0, bxy = 0;
+        uint32_t nxt = 0;   // bit 0 of the next word of rows (0,0), (0,1), (1,0), (1,1)
         if (hasx && hasy) {
-            const uint64_t bxy = q[YW + p.W];
+            bxy = q[YW + p.W];
             const uint64_t byn = hn ? q[p.W + 1] : 0ull, bxn = hn ? q[YW + 1] : 0ull,
                            bxyn = hn ? q[YW + p.W + 1] : 0ull;
             const CellMasks m = cell_masks(b, bn, by, byn, bx, bxn, bxy, bxyn);
-            uint64_t act = cell_active(m) & low_bits(p.Z - 1 - zlo);
+            act = cell_active(m) & low_bits(p.Z - 1 - zlo);
+            nxt = (uint32_t) (bn & 1) | (uint32_t) (byn & 1) << 1 | (uint32_t) (bxn & 1) << 2 |
+                  (uint32_t) (bxyn & 1) << 3;
             if (p.its) {
                 // crossing edges of the active cells, bit-parallel over the 64
                 // cells: edge e crosses where its two corner masks differ
                 cv = (uint32_t) __popcll(act);
                 for_edges([&](auto E) { cf += (uint32_t) __popcll((m.m[edge_a(E)] ^ m.m[edge_b(E)]) & act); });
-            } else {
-                while (act) {
-                    const int j = __ffsll((unsigned long long) act) - 1;
-                    act &= act - 1;
-                    cf += ntri[cell_case(m, j)];
-                }
             }
         }
+        if (!valid) act = 0;
         const int e = (int) (r * p.S + div_fast(c, p.nzc) - i0);
-        if (cv) atomicAdd(&ent_v[e], cv);
-        if (cf) atomicAdd(&ent_f[e], cf);
+        if (valid && cv) atomicAdd(&ent_v[e], cv);
+        if (valid && cf) atomicAdd(&ent_f[e], cf);
+        if (p.its) continue;
+        // triangles: the wave's active cells are dealt out one per lane per
+        // round (owner lane by a running maximum over marked first slots),
+        // so a word with many active cells does not serialise its wave
+        const int cnt = __popcll(act);
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        const int total = __shfl(incl, 63), excl = incl - cnt;
+        for (int base = 0; base < total; base += 64) {
+            mark[wave][lane] = -1;
+            __builtin_amdgcn_wave_barrier();
+            if (cnt > 0 && excl < base + 64 && incl > base) mark[wave][max(excl, base) - base] = lane;
+            __builtin_amdgcn_wave_barrier();
+            int own = mark[wave][lane];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(own, o);
+                if (lane >= o) own = max(own, t);
+            }
+            own = max(own, 0);
+            const int k = base + lane;
+            const uint64_t oact = __shfl(act, own);
+            const uint64_t w00 = __shfl(b, own), w01 = __shfl(by, own), w10 = __shfl(bx, own), w11 = __shfl(bxy, own);
+            const uint32_t on = (uint32_t) __shfl((int) nxt, own);
+            const int oexcl = __shfl(excl, own), oe = __shfl(e, own);
+            if (k < total) {
+                const int j = nth_set_bit(oact, k - oexcl);
+                uint32_t up;   // bit j + 1 of the four rows
+                if (j < 63)
+                    up = (uint32_t) ((w00 >> (j + 1)) & 1) | (uint32_t) ((w01 >> (j + 1)) & 1) << 1 |
+                         (uint32_t) ((w10 >> (j + 1)) & 1) << 2 | (uint32_t) ((w11 >> (j + 1)) & 1) << 3;
+                else
+                    up = on;
+                const int cs = (int) (((w00 >> j) & 1) | ((up & 1) << 1) | (((w01 >> j) & 1) << 2) |
+                                      (((up >> 1) & 1) << 3) | (((w10 >> j) & 1) << 4) | (((up >> 2) & 1) << 5) |
+                                      (((w11 >> j) & 1) << 6) | (((up >> 3) & 1) << 7));
+                atomicAdd(&ent_f[oe], (uint32_t) ntri[cs]);
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
     }
     __syncthreads();
     uint32_t cv[PER], cf[PER], tv = 0, tf = 0;
@@ -554,27 +624,6 @@ __device__ __forceinline__ void block_maxscan_lds(int32_t *a, uint32_t *tsum) {
     __syncthreads();
 }
 
-// Position of the r-th set bit (r < popcount(m)).
-__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
-    int pos = 0;
-    uint32_t x = (uint32_t) m;
-    int c = __popc(x);
-    if (r >= c) {
-        r -= c;
-        x = (uint32_t) (m >> 32);
-        pos = 32;
-    }
-#pragma unroll
-    for (int half = 16; half >= 1; half >>= 1) {
-        c = __popc(x & ((1u << half) - 1));
-        if (r >= c) {
-            r -= c;
-            x >>= half;
-            pos += half;
-        }
-    }
-    return pos;
-}
 
 // Weld-candidate test with a division-free prefilter: an edge's vertex can
 // lie within 16 ulps of an endpoint coordinate only when its parameter t is
@@ -1116,9 +1165,7 @@ __global__ void k_batch_summary(const McBatchItem *items, const McBatchHdr *hdr,
         s.err = *p.err;
         s.nv = p.totals[0];
         s.nf = p.totals[1];
-        s.need = p.sp->need;
-        s.n_cand = p.sp->n_cand;
-        s.n_hole = p.sp->n_hole;
+        s.sp = *p.sp;
         out[i] = s;
     }
 }
