@@ -818,7 +818,17 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
         return fail(ISOEXT_E_WORKSPACE, "special-vertex lists need capacity %lld", (long long) need);
     if (sp_dev.n_cand > 0 || sp_dev.n_hole > 0) {
         iso::FixupGrid g{shape[0], shape[1], shape[2], p.blk_v, p.blk_f, p.cnt_v, p.cnt_f, p.S};
-        HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key), "weld fix-up");
+        if (iso::fixup_small_ok(sp_dev)) {   // one workgroup, one launch
+            int64_t fin[3];
+            int64_t *keys1[1] = {out_key};
+            HIP_CHECK(iso::weld_fixup_small_batch(1, &sp_dev, &out_v, &out_f, keys1, &nv, &nf, &g, st, fin),
+                      "weld fix-up");
+            if (fin[2]) return fail(ISOEXT_E_DEVICE, "weld fix-up: inconsistent special lists");
+            nv = fin[0];
+            nf = fin[1];
+        } else {
+            HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key), "weld fix-up");
+        }
         counts[0] = nv;
         counts[1] = nf;
     }
@@ -1053,10 +1063,15 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
                              st),
               "read batch counts");
     HIP_CHECK(hipStreamSynchronize(st), "batch emit");
-    // weld fix-ups of every item that needs one, back to back on the stream,
-    // then one read of their final counts
+    // weld fix-ups: items with small lists in one launch (a workgroup each),
+    // the others back to back on the stream, then one read of their counts
     int64_t *fix_dev = (int64_t *) (ws + B.o_fixcnt);
-    std::vector<int> fixed;
+    std::vector<int> fixed, small;
+    std::vector<iso::SpecialLists> s_sp;
+    std::vector<float *> s_v;
+    std::vector<int32_t *> s_f;
+    std::vector<int64_t> s_nv, s_nf;
+    std::vector<iso::FixupGrid> s_g;
     for (int i = 0; i < n; ++i) {
         const int64_t nv = sum[i].nv, nf = sum[i].nf;
         counts[2 * i] = nv;
@@ -1072,10 +1087,31 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
             iso::FixupGrid g{items[i].shape[0], items[i].shape[1], items[i].shape[2],
                              (const int64_t *) (ws + IL.o_blk_v), (const int64_t *) (ws + IL.o_blk_f),
                              (const uint32_t *) (ws + IL.o_cnt_v), (const uint32_t *) (ws + IL.o_cnt_f), IL.S};
+            if (iso::fixup_small_ok(sum[i].sp)) {
+                small.push_back(i);
+                s_sp.push_back(sum[i].sp);
+                s_v.push_back(items[i].out_v);
+                s_f.push_back(items[i].out_f);
+                s_nv.push_back(nv);
+                s_nf.push_back(nf);
+                s_g.push_back(g);
+                continue;
+            }
             HIP_CHECK(iso::weld_fixup_async(sum[i].sp, items[i].out_v, items[i].out_f, nv, nf, g, st, nullptr,
                                             fix_dev + 4 * i),
                       "weld fix-up");
             fixed.push_back(i);
+        }
+    }
+    if (!small.empty()) {   // synchronises the stream, so the fix-ups above are done too
+        std::vector<int64_t> fin(3 * small.size());
+        HIP_CHECK(iso::weld_fixup_small_batch((int) small.size(), s_sp.data(), s_v.data(), s_f.data(), nullptr,
+                                              s_nv.data(), s_nf.data(), s_g.data(), st, fin.data()),
+                  "weld fix-up");
+        for (size_t k = 0; k < small.size(); ++k) {
+            if (fin[3 * k + 2]) return fail(ISOEXT_E_DEVICE, "item %d: inconsistent special lists", small[k]);
+            counts[2 * small[k]] = fin[3 * k];
+            counts[2 * small[k] + 1] = fin[3 * k + 1];
         }
     }
     if (!fixed.empty()) {

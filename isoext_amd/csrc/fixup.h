@@ -20,6 +20,36 @@ struct FixupGrid {
 
 hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
                       const FixupGrid &g, hipStream_t st, int64_t *keys = nullptr);
+// One weld fix-up for one workgroup: every phase of weld_fixup in a single
+// kernel (hash tables in global scratch, removed / dropped lists sorted in
+// LDS), for lists small enough (fixup_small_ok).  Items are independent: a
+// batch launches one workgroup per item.
+struct FixSmallItem {
+    SpecialLists sp;
+    float *v;
+    int32_t *f;
+    int64_t *vkeys;
+    int64_t nv, nf;
+    FixupGrid g;
+    int64_t *keys;   // nslots
+    void *info;      // nslots vertex records
+    int32_t *pslot;  // npslots
+    void *groups;    // npslots position groups
+    int64_t nslots, npslots;
+    int64_t *final_dev;   // final vertex / face counts, inconsistency flag
+};
+constexpr int64_t kFixSmallMax = 2048;   // removed vertices and dropped faces the small path holds
+bool fixup_small_ok(const SpecialLists &sp);
+// Scratch bytes of one small fix-up item and the item's table sizes.
+size_t fixup_small_scratch(const SpecialLists &sp, int64_t &nslots, int64_t &npslots);
+hipError_t fixup_small_launch(const FixSmallItem *items_dev, int n, hipStream_t st);
+// Small fix-ups of n items in one launch and one synchronisation; final[3i],
+// final[3i+1] = final vertex / face counts of item i, final[3i+2] != 0 when
+// its lists were inconsistent.
+hipError_t weld_fixup_small_batch(int n, const SpecialLists *sp, float *const *v, int32_t *const *f,
+                                  int64_t *const *vkeys, const int64_t *nv, const int64_t *nf, const FixupGrid *g,
+                                  hipStream_t st, int64_t *final_host);
+
 // Stream-ordered variant: no host synchronisation (except to grow the
 // scratch); final_dev[0], [1] receive the final vertex / face counts and
 // final_dev[2] is non-zero when the lists were inconsistent.  nv / nf are the

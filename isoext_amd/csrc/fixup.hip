@@ -16,6 +16,8 @@
 //   3. remove non-representatives and unused vertices, drop degenerate faces,
 //      remap faces, compact (stable, so order is preserved).
 
+#include <vector>
+
 #include "common.h"
 #include "fixup.h"
 #include "mc_dense.h"
@@ -402,6 +404,292 @@ hipError_t bitmap_prefix(const uint32_t *bits, int64_t nw, int64_t *pre, int64_t
         hipError_t e_ = (x);                     \
         if (e_ != hipSuccess) return e_;         \
     } while (0)
+
+// ---- single-workgroup fix-up --------------------------------------------------
+
+namespace {
+
+constexpr int kFixNT = 1024;
+
+// Ascending bitonic sort of a[0, n) in LDS, padded with INT32_MAX to m (a
+// power of two >= n, <= 2 * kFixSmallMax).
+__device__ void lds_sort(int32_t *a, int n, int m) {
+    for (int i = n + (int) threadIdx.x; i < m; i += kFixNT) a[i] = INT32_MAX;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += kFixNT) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int32_t x = a[i], y = a[l];
+                    const bool up = (i & k) == 0;
+                    if (up ? (x > y) : (x < y)) {
+                        a[i] = y;
+                        a[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ int lower_bound_lds(const int32_t *a, int n, int64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int64_t) a[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ int pow2_at_least_dev(int n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+}   // namespace
+
+__global__ void __launch_bounds__(kFixNT) k_fix_small(const FixSmallItem *items) {
+    const FixSmallItem &it = items[blockIdx.x];
+    __shared__ int32_t rv[2 * kFixSmallMax];    // removed vertex ids, sorted
+    __shared__ int32_t hf[2 * kFixSmallMax];    // dropped face slots, sorted
+    __shared__ int32_t psrc[kFixSmallMax], pdst[kFixSmallMax];   // vertex relocation pairs by source
+    __shared__ int nrv, nhf, bad;
+    __shared__ int64_t f0;
+    const int tid = threadIdx.x;
+    const SpecialLists &sp = it.sp;
+    const int64_t ncand = (int64_t) sp.n_cand, nscell = (int64_t) sp.n_scell;
+    const int64_t nkept = (int64_t) sp.n_kept, nhole = (int64_t) sp.n_hole;
+    const int64_t nv = it.nv, nf = it.nf;
+    int64_t *keys = it.keys;
+    VInfo *info = (VInfo *) it.info;
+    PGroup *groups = (PGroup *) it.groups;
+    const int64_t mask = it.nslots - 1, pmask = it.npslots - 1;
+    if (tid == 0) {
+        nrv = 0;
+        nhf = 0;
+        bad = 0;
+    }
+    for (int64_t i = tid; i < it.nslots; i += kFixNT) {
+        keys[i] = kEmpty;
+        info[i] = VInfo{0, 0, 0, -1, 0};
+    }
+    for (int64_t i = tid; i < it.npslots; i += kFixNT) {
+        it.pslot[i] = -1;
+        groups[i] = PGroup{INT64_MAX, 0, 0};
+    }
+    __syncthreads();
+    for (int64_t i = tid; i < nscell + ncand; i += kFixNT) {
+        if (i < nscell) insert_scell(sp.scell[i], keys, info, mask);
+        else insert_cand(sp.cand, i - nscell, keys, info, mask, it.pslot, groups, pmask);
+    }
+    __syncthreads();
+    // kept faces: mark their vertices, point corners at representatives
+    for (int64_t i = tid; i < nkept; i += kFixNT) {
+        const int64_t fs = sp.kept[i];
+        if (fs < 0 || fs >= nf) continue;
+        for (int j = 0; j < 3; ++j) {
+            const int64_t vv = it.f[3 * fs + j];
+            const int64_t sl = h1_find_or_insert(keys, mask, vv, false);
+            if (sl < 0) continue;
+            info[sl].referenced = 1;
+            if (info[sl].is_cand && info[sl].group >= 0) {
+                const int64_t rep = groups[info[sl].group].rep;
+                if (rep != vv) it.f[3 * fs + j] = (int32_t) rep;
+            }
+        }
+    }
+    __syncthreads();
+    for (int64_t sl = tid; sl < it.nslots; sl += kFixNT) {
+        if (keys[sl] == kEmpty) continue;
+        VInfo &vi = info[sl];
+        const bool used = vi.referenced || vi.count_s < edge_cell_count(vi.edge, it.g);
+        if (vi.is_cand && vi.group >= 0) {
+            if (used) atomicOr(&groups[vi.group].used, 1);
+        } else {
+            vi.is_cand = 0;
+            vi.referenced = used ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    // removed vertices and dropped faces into LDS lists
+    for (int64_t sl = tid; sl < it.nslots; sl += kFixNT) {
+        const int64_t vid = keys[sl];
+        if (vid < 0 || vid >= nv) continue;
+        const VInfo &vi = info[sl];
+        const bool removed = (vi.is_cand && vi.group >= 0)
+                                 ? (vid != groups[vi.group].rep || !groups[vi.group].used)
+                                 : !vi.referenced;
+        if (removed) {
+            const int k = atomicAdd(&nrv, 1);
+            if (k < kFixSmallMax) rv[k] = (int32_t) vid;
+            else bad = 1;
+        }
+    }
+    for (int64_t i = tid; i < nhole; i += kFixNT) {
+        const int64_t fs = sp.hole[i];
+        if (fs < 0 || fs >= nf) continue;
+        const int k = atomicAdd(&nhf, 1);
+        if (k < kFixSmallMax) hf[k] = (int32_t) fs;
+        else bad = 1;
+    }
+    __syncthreads();
+    if (bad) {
+        if (tid == 0) {
+            it.final_dev[0] = nv;
+            it.final_dev[1] = nf;
+            it.final_dev[2] = 1;
+        }
+        return;
+    }
+    const int kv = nrv, kf = nhf;
+    lds_sort(rv, kv, pow2_at_least_dev(max(kv, 1)));
+    lds_sort(hf, kf, pow2_at_least_dev(max(kf, 1)));
+    const int64_t nv2 = nv - kv, nf2 = nf - kf;
+    // vertices: the i-th removed slot below nv2 takes the i-th surviving
+    // vertex at or above nv2 (sorted order on both sides)
+    const int hv = lower_bound_lds(rv, kv, nv2);   // removed slots below nv2
+    for (int64_t t = nv2 + tid; t < nv; t += kFixNT) {
+        const int lo = lower_bound_lds(rv, kv, t);
+        if (lo < kv && rv[lo] == t) continue;   // removed itself
+        const int rank = (int) (t - nv2) - (lo - hv);
+        const int64_t d = rv[rank];
+        it.v[3 * d + 0] = it.v[3 * t + 0];
+        it.v[3 * d + 1] = it.v[3 * t + 1];
+        it.v[3 * d + 2] = it.v[3 * t + 2];
+        if (it.vkeys) it.vkeys[d] = it.vkeys[t];
+        psrc[rank] = (int32_t) t;
+        pdst[rank] = (int32_t) d;
+    }
+    if (tid == 0) f0 = first_face_for_tail(it.g, nv2);
+    __syncthreads();
+    // face ids of relocated tail vertices (faces from the first that can
+    // reference the tail); the pairs are sorted by source
+    if (hv > 0) {
+        for (int64_t i = f0 + tid; i < nf; i += kFixNT) {
+            for (int j = 0; j < 3; ++j) {
+                const int64_t v0 = it.f[3 * i + j];
+                if (v0 < nv2) continue;
+                const int lo = lower_bound_lds(psrc, hv, v0);
+                it.f[3 * i + j] = (lo < hv && psrc[lo] == v0) ? pdst[lo] : -1;
+            }
+        }
+    }
+    __syncthreads();
+    // faces: the i-th dropped slot below nf2 takes the i-th kept face at or above nf2
+    const int hfl = lower_bound_lds(hf, kf, nf2);
+    for (int64_t t = nf2 + tid; t < nf; t += kFixNT) {
+        const int lo = lower_bound_lds(hf, kf, t);
+        if (lo < kf && hf[lo] == t) continue;
+        const int rank = (int) (t - nf2) - (lo - hfl);
+        const int64_t d = hf[rank];
+        it.f[3 * d + 0] = it.f[3 * t + 0];
+        it.f[3 * d + 1] = it.f[3 * t + 1];
+        it.f[3 * d + 2] = it.f[3 * t + 2];
+    }
+    if (tid == 0) {
+        it.final_dev[0] = nv2;
+        it.final_dev[1] = nf2;
+        it.final_dev[2] = 0;
+    }
+}
+
+bool fixup_small_ok(const SpecialLists &sp) {
+    return (int64_t) (sp.n_cand + sp.n_scell) <= kFixSmallMax && (int64_t) sp.n_hole <= kFixSmallMax;
+}
+
+size_t fixup_small_scratch(const SpecialLists &sp, int64_t &nslots, int64_t &npslots) {
+    nslots = pow2_at_least(2 * (int64_t) (sp.n_cand + sp.n_scell) + 16);
+    npslots = pow2_at_least(2 * (int64_t) sp.n_cand + 16);
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    return up(sizeof(int64_t) * nslots) + up(sizeof(VInfo) * nslots) + up(sizeof(int32_t) * npslots) +
+           up(sizeof(PGroup) * npslots);
+}
+
+hipError_t fixup_small_launch(const FixSmallItem *items_dev, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fix_small, dim3((unsigned) n), dim3(kFixNT), 0, st, items_dev);
+    return hipGetLastError();
+}
+
+hipError_t weld_fixup_small_batch(int n, const SpecialLists *sp, float *const *v, int32_t *const *f,
+                                  int64_t *const *vkeys, const int64_t *nv, const int64_t *nf, const FixupGrid *g,
+                                  hipStream_t st, int64_t *final_host) {
+    if (n <= 0) return hipSuccess;
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    // device: item records, per-item tables, final counts; host: pinned staging
+    std::vector<int64_t> ns(n), nps(n);
+    std::vector<size_t> off(n);
+    size_t bytes = up(sizeof(FixSmallItem) * (size_t) n) + up(sizeof(int64_t) * 3 * (size_t) n);
+    for (int i = 0; i < n; ++i) {
+        off[i] = bytes;
+        bytes += fixup_small_scratch(sp[i], ns[i], nps[i]);
+    }
+    struct Cache {
+        char *dev = nullptr;
+        size_t dev_bytes = 0;
+        char *host = nullptr;
+        size_t host_bytes = 0;
+        ~Cache() {
+            if (host) (void) hipHostFree(host);
+        }
+    };
+    thread_local Cache c;
+    if (c.dev_bytes < bytes) {
+        if (c.dev) {
+            ISO_TRY(hipStreamSynchronize(st));
+            ISO_TRY(hipFree(c.dev));
+            c.dev = nullptr;
+            c.dev_bytes = 0;
+        }
+        ISO_TRY(hipMalloc((void **) &c.dev, bytes + bytes / 2));
+        c.dev_bytes = bytes + bytes / 2;
+    }
+    const size_t hb = up(sizeof(FixSmallItem) * (size_t) n) + sizeof(int64_t) * 3 * (size_t) n;
+    if (c.host_bytes < hb) {
+        if (c.host) {
+            ISO_TRY(hipStreamSynchronize(st));
+            ISO_TRY(hipHostFree(c.host));
+            c.host = nullptr;
+            c.host_bytes = 0;
+        }
+        ISO_TRY(hipHostMalloc((void **) &c.host, hb + hb / 2, 0));
+        c.host_bytes = hb + hb / 2;
+    }
+    FixSmallItem *hi = (FixSmallItem *) c.host;
+    int64_t *fin_dev = (int64_t *) (c.dev + up(sizeof(FixSmallItem) * (size_t) n));
+    for (int i = 0; i < n; ++i) {
+        char *base = c.dev + off[i];
+        FixSmallItem &it = hi[i];
+        it.sp = sp[i];
+        it.v = v[i];
+        it.f = f[i];
+        it.vkeys = vkeys ? vkeys[i] : nullptr;
+        it.nv = nv[i];
+        it.nf = nf[i];
+        it.g = g[i];
+        it.nslots = ns[i];
+        it.npslots = nps[i];
+        it.keys = (int64_t *) base;
+        base += up(sizeof(int64_t) * ns[i]);
+        it.info = base;
+        base += up(sizeof(VInfo) * ns[i]);
+        it.pslot = (int32_t *) base;
+        base += up(sizeof(int32_t) * nps[i]);
+        it.groups = base;
+        it.final_dev = fin_dev + 3 * i;
+    }
+    ISO_TRY(hipMemcpyAsync(c.dev, hi, sizeof(FixSmallItem) * (size_t) n, hipMemcpyHostToDevice, st));
+    ISO_TRY(fixup_small_launch((const FixSmallItem *) c.dev, n, st));
+    int64_t *fin_host = (int64_t *) (c.host + up(sizeof(FixSmallItem) * (size_t) n));
+    ISO_TRY(hipMemcpyAsync(fin_host, fin_dev, sizeof(int64_t) * 3 * (size_t) n, hipMemcpyDeviceToHost, st));
+    ISO_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < 3 * n; ++i) final_host[i] = fin_host[i];
+    return hipSuccess;
+}
 
 hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
                            const FixupGrid &g, hipStream_t st, int64_t *vkeys, int64_t *final_dev);
