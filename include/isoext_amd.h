@@ -280,6 +280,37 @@ int isoext_mc_batch_count(int n, const isoext_mc_item *items, int cfg, void *wor
 int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *workspace, int64_t workspace_bytes,
                          int64_t special_cap, int64_t *counts, void *stream);
 
+/* ---- batched dual contouring of dense grids (no counterpart: the reference
+ * runs get_intersection + dual_contouring once per grid, src/its.cu:93-159,
+ * src/dc.cu:161-218) ------------------------------------------------------------
+ * Per item: get_intersection(grid, level, compute_normals=True) followed by
+ * dual_contouring(grid, level, its) with the given reg / svd_tol.  Items use
+ * values, shape, aabb, level (method must be 0 or 1 and is not used; out_* are
+ * not read).  At most 256 items.  Results equal isoext_its_count/emit +
+ * isoext_dc per item. */
+
+/* Workspace (zero-filled before first use) shared by both phases. */
+int64_t isoext_dc_batch_workspace_bytes(int n, const isoext_mc_item *items);
+
+/* Phase 1: counts[2i] = active cells, counts[2i + 1] = intersection points of
+ * item i.  Synchronises `stream`. */
+int isoext_dc_batch_count(int n, const isoext_mc_item *items, void *workspace, int64_t workspace_bytes,
+                          int64_t *counts, void *stream);
+
+/* Scratch for phase 2 given phase 1's counts. */
+int64_t isoext_dc_batch_scratch_bytes(int n, const int64_t *counts);
+
+/* Phase 2 (same workspace as phase 1, untouched in between): intersections,
+ * QEF solve, quads and the vertex weld of every grid, one launch per stage.
+ * out_v (cap_v >= total active cells rows) and out_f (cap_f >= 2 x total
+ * intersections rows) receive the meshes back to back: out_counts[4i..4i+3] =
+ * vertices, faces, first vertex row, first face row of item i (face corners
+ * are item-local vertex ids); out_counts[4n] = FP64 re-solves.  Synchronises
+ * `stream`. */
+int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *counts, float reg, float svd_tol,
+                        void *workspace, int64_t workspace_bytes, void *scratch, int64_t scratch_bytes, float *out_v,
+                        int64_t cap_v, int32_t *out_f, int64_t cap_f, int64_t *out_counts, void *stream);
+
 /* ---- OBJ output ------------------------------------------------------------- */
 
 /* write_obj (src/isoext/utils.py:42-63): writes "v x y z" per vertex and

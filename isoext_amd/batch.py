@@ -1,4 +1,4 @@
-"""Ragged batches of dense marching-cubes extractions.
+"""Ragged batches of dense marching-cubes and dual-contouring extractions.
 
 ``mc_batch(jobs)`` extracts every grid of a list of (values, level, method,
 aabb) jobs with one launch per stage for the whole batch (classify, count,
@@ -9,6 +9,13 @@ The kernels walk batch-wide index spaces (words, count blocks, tiles) and
 find their grid by binary search over a descriptor array uploaded from
 pinned memory.  Results are identical to one ``marching_cubes`` call per
 grid (tests/test_batch_gpu.py).
+
+``dc_batch(jobs)`` does the same for dual contouring: phase 1 counts every
+grid's active cells and intersections (the mask pipeline in intersection
+mode), phase 2 writes the concatenated intersection sets with normals and
+runs the QEF solve, quad emission and vertex weld of all grids at once; the
+meshes come back as views of two shared output buffers.  Results are
+identical to ``dual_contouring(grid, level)`` per grid.
 """
 
 from __future__ import annotations
@@ -48,6 +55,16 @@ def _decl():
         L.isoext_mc_batch_emit.restype = c_int
         L.isoext_mc_batch_emit.argtypes = [c_int, ctypes.POINTER(McItem), c_int, vp, i64, i64,
                                            ctypes.POINTER(i64), vp]
+        L.isoext_dc_batch_workspace_bytes.restype = i64
+        L.isoext_dc_batch_workspace_bytes.argtypes = [c_int, ctypes.POINTER(McItem)]
+        L.isoext_dc_batch_count.restype = c_int
+        L.isoext_dc_batch_count.argtypes = [c_int, ctypes.POINTER(McItem), vp, i64, ctypes.POINTER(i64), vp]
+        L.isoext_dc_batch_scratch_bytes.restype = i64
+        L.isoext_dc_batch_scratch_bytes.argtypes = [c_int, ctypes.POINTER(i64)]
+        L.isoext_dc_batch_run.restype = c_int
+        L.isoext_dc_batch_run.argtypes = [c_int, ctypes.POINTER(McItem), ctypes.POINTER(i64), ctypes.c_float,
+                                          ctypes.c_float, vp, i64, vp, i64, vp, i64, vp, i64, ctypes.POINTER(i64),
+                                          vp]
         _declared = True
         return L
 
@@ -64,19 +81,12 @@ class GridJob:
 BATCH_SPECIAL_CAP = 4096
 
 
-def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_batch"):
-    """Marching cubes over a ragged list of dense grids.  Returns a list of
-    (v, f) pairs (zero-row tensors for empty meshes).  cfg: one emit tile
-    configuration for every grid, or None for each grid's own."""
-    if not jobs:
-        return []
-    L = _decl()
+def _items(jobs, who):
     dev = jobs[0].values.device
-    n = len(jobs)
-    items = (McItem * n)()
+    items = (McItem * len(jobs))()
     for it, j in zip(items, jobs):
         if j.values.device != dev or j.values.dtype != torch.float32 or not j.values.is_contiguous():
-            raise TypeError("mc_batch: values must be contiguous float32 tensors on one device")
+            raise TypeError(f"{who}: values must be contiguous float32 tensors on one device")
         it.values = j.values.data_ptr()
         for a in range(3):
             it.shape[a] = int(j.values.shape[a])
@@ -84,6 +94,66 @@ def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_bat
             it.aabb_max[a] = float(j.aabb_max[a])
         it.level = float(j.level)
         it.method = int(j.method)
+    return dev, items
+
+
+MAX_DC_BATCH = 256
+
+
+def dc_batch(jobs: list[GridJob], reg: float = 1e-2, svd_tol: float = 1e-6, ws_kind: str = "dc_batch"):
+    """Dual contouring (intersections with trilinear normals, then the QEF
+    solve) over a ragged list of dense grids, at most 256 per call.  Returns
+    a list of (v, f) pairs, (None, None) for an empty mesh, as
+    ``dual_contouring(grid, level, reg=reg, svd_tol=svd_tol)`` per grid."""
+    if not jobs:
+        return []
+    if len(jobs) > MAX_DC_BATCH:
+        return dc_batch(jobs[:MAX_DC_BATCH], reg, svd_tol, ws_kind) + dc_batch(jobs[MAX_DC_BATCH:], reg, svd_tol,
+                                                                               ws_kind)
+    L = _decl()
+    dev, items = _items(jobs, "dc_batch")
+    for it in items:
+        it.method = 0   # not used by dual contouring
+    n = len(jobs)
+    counts = (ctypes.c_int64 * (2 * n))()
+    out_counts = (ctypes.c_int64 * (4 * n + 1))()
+    stream = _lib.stream_ptr(dev)
+    with torch.cuda.device(dev):
+        ws_bytes = L.isoext_dc_batch_workspace_bytes(n, items)
+        if ws_bytes < 0:
+            raise ValueError(_lib.last_error())
+        ws = workspace.get(ws_kind, ws_bytes, dev)
+        _lib.check(L.isoext_dc_batch_count(n, items, ws.data_ptr(), ws.numel(), counts, stream), "dc_batch count")
+        C = sum(int(counts[2 * i]) for i in range(n))
+        P = sum(int(counts[2 * i + 1]) for i in range(n))
+        if C == 0:
+            return [(None, None)] * n
+        sc = workspace.get(ws_kind + "_scratch", L.isoext_dc_batch_scratch_bytes(n, counts), dev)
+        out_v = torch.empty((C, 3), dtype=torch.float32, device=dev)
+        out_f = torch.empty((max(2 * P, 1), 3), dtype=torch.int32, device=dev)
+        rc = L.isoext_dc_batch_run(n, items, counts, float(reg), float(svd_tol), ws.data_ptr(), ws.numel(),
+                                   sc.data_ptr(), sc.numel(), out_v.data_ptr(), out_v.shape[0], out_f.data_ptr(),
+                                   out_f.shape[0], out_counts, stream)
+    _lib.check(rc, "dc_batch")
+    res = []
+    for i in range(n):
+        nv, nf, v0, f0 = (int(out_counts[4 * i + k]) for k in range(4))
+        if nf == 0:
+            res.append((None, None))
+        else:
+            res.append((out_v[v0:v0 + nv], out_f[f0:f0 + nf]))
+    return res
+
+
+def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_batch"):
+    """Marching cubes over a ragged list of dense grids.  Returns a list of
+    (v, f) pairs (zero-row tensors for empty meshes).  cfg: one emit tile
+    configuration for every grid, or None for each grid's own."""
+    if not jobs:
+        return []
+    L = _decl()
+    n = len(jobs)
+    dev, items = _items(jobs, "mc_batch")
     if cfg is None:
         cfg = -1   # each grid uses its own tile configuration
     special = BATCH_SPECIAL_CAP

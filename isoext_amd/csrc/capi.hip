@@ -490,8 +490,10 @@ void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t 
 
 // Upload the descriptors of a batch phase (pinned staging) and return the
 // staging area for the summaries.
+// its_caps != nullptr: the intersection pass (p.its = 1) with per-item
+// capacities (cells, entries) its_caps[2i], its_caps[2i + 1].
 int batch_upload(int n, const isoext_mc_item *items, char *ws, const BatchLayout &B, hipStream_t st,
-                 iso::McBatchSummary **summary_host) {
+                 iso::McBatchSummary **summary_host, const int64_t *its_caps = nullptr) {
     const size_t desc = sizeof(iso::McBatchHdr) + sizeof(iso::McBatchItem) * (size_t) n + sizeof(int32_t) * (size_t) n;
     const size_t sum = sizeof(iso::McBatchSummary) * (size_t) n;
     char *host = (char *) g_pinned.get(desc + sum + 64);
@@ -503,6 +505,14 @@ int batch_upload(int n, const isoext_mc_item *items, char *ws, const BatchLayout
     for (int i = 0; i < n; ++i) {
         fill_item(items[i], B.it[i], ws, w0, b0, B.it[i].tile0, it[i]);
         it[i].p.nzc = B.it[i].nzc;
+        if (its_caps) {
+            iso::McParams &p = it[i].p;
+            p.its = 1;
+            p.out_v = nullptr;
+            p.out_f = nullptr;
+            p.cap_v = its_caps[2 * i];
+            p.cap_f = its_caps[2 * i + 1];
+        }
         w0 += word_span(B.it[i].nwords);
         b0 += B.it[i].nblk;
     }
@@ -531,6 +541,44 @@ int batch_check(int n, const isoext_mc_item *items, int cfg) {
     }
     return ISOEXT_OK;
 }
+
+// Scratch of a batched dual contouring run: the concatenated intersection
+// sets (C active cells, P entries), the per-grid descriptors and counts, and
+// the DC work arrays for C cells and P entries.
+struct DcBatchLayout {
+    DcLayout dc;
+    size_t o_dc, o_ci, o_cd, o_co, o_pts, o_nrm, o_el, o_io, o_ec, o_outs, o_items, o_citem, o_vb, o_qb, total;
+};
+
+DcBatchLayout dc_batch_layout(int n, int64_t C, int64_t P) {
+    DcBatchLayout L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t r = o;
+        o = align256(o + bytes);
+        return r;
+    };
+    const size_t c1 = (size_t) std::max<int64_t>(C, 1), p1 = (size_t) std::max<int64_t>(P, 1);
+    L.o_ci = take(sizeof(int64_t) * c1);
+    L.o_cd = take(sizeof(int64_t) * c1);
+    L.o_co = take(sizeof(int64_t) * (c1 + 1));
+    L.o_pts = take(sizeof(float) * 3 * p1);
+    L.o_nrm = take(sizeof(float) * 3 * p1);
+    L.o_el = take(p1);
+    L.o_io = take(p1);
+    L.o_ec = take(sizeof(int32_t) * p1);
+    L.o_outs = take(sizeof(iso::ItsOut) * (size_t) n);
+    L.o_items = take(iso::dc_batch_item_bytes() * (size_t) n);
+    L.o_citem = take(sizeof(int32_t) * c1);
+    L.o_vb = take(sizeof(int64_t) * (size_t) (n + 1));
+    L.o_qb = take(sizeof(int64_t) * (size_t) (n + 1));
+    L.dc = dc_layout(C, P);
+    L.o_dc = take(L.dc.total);
+    L.total = o;
+    return L;
+}
+
+thread_local Pinned g_pinned_its;   // batched DC: intersection output descriptors
 
 }   // namespace
 
@@ -1129,6 +1177,175 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
     return ISOEXT_OK;
 }
 
+
+// ---- batched dual contouring of dense grids ----------------------------------------
+
+int64_t isoext_dc_batch_workspace_bytes(int n, const isoext_mc_item *items) {
+    if (n > iso::kMaxDcBatchGrids || batch_check(n, items, -1) != ISOEXT_OK) return -1;
+    BatchLayout B;
+    if (!batch_layout(n, items, -1, 0, B)) return -1;
+    return (int64_t) B.total;
+}
+
+int isoext_dc_batch_count(int n, const isoext_mc_item *items, void *workspace, int64_t workspace_bytes,
+                          int64_t *counts, void *stream) {
+    if (n > iso::kMaxDcBatchGrids) return fail(ISOEXT_E_ARG, "batch of %d grids exceeds %d", n, iso::kMaxDcBatchGrids);
+    int rc = batch_check(n, items, -1);
+    if (rc != ISOEXT_OK) return rc;
+    for (int i = 0; i < 2 * n; ++i) counts[i] = 0;
+    BatchLayout B;
+    batch_layout(n, items, -1, 0, B);
+    if ((int64_t) B.total > workspace_bytes)
+        return fail(ISOEXT_E_WORKSPACE, "workspace too small: %lld < %lld bytes", (long long) workspace_bytes,
+                    (long long) B.total);
+    hipStream_t st = (hipStream_t) stream;
+    char *ws = (char *) workspace;
+    iso::McBatchSummary *sum = nullptr;
+    if ((rc = batch_upload(n, items, ws, B, st, &sum, counts)) != ISOEXT_OK) return rc;
+    HIP_CHECK(iso::mc_batch_count_launch((const iso::McBatchItem *) (ws + B.o_items),
+                                         (const iso::McBatchHdr *) (ws + B.o_hdr), B.h,
+                                         (iso::McBatchSummary *) (ws + B.o_summary), st),
+              "launch batch intersection count");
+    HIP_CHECK(hipMemcpyAsync(sum, ws + B.o_summary, sizeof(iso::McBatchSummary) * (size_t) n, hipMemcpyDeviceToHost,
+                             st),
+              "read batch counts");
+    HIP_CHECK(hipStreamSynchronize(st), "batch intersection count");
+    for (int i = 0; i < n; ++i) {
+        if (sum[i].err & iso::ISO_ERR_TOO_LARGE)
+            return fail(ISOEXT_E_TOO_LARGE, "item %d: more than 2^31-1 active cells or intersections", i);
+        counts[2 * i] = sum[i].nv;
+        counts[2 * i + 1] = sum[i].nf;
+    }
+    return ISOEXT_OK;
+}
+
+int64_t isoext_dc_batch_scratch_bytes(int n, const int64_t *counts) {
+    if (n < 1 || n > iso::kMaxDcBatchGrids) return -1;
+    int64_t C = 0, P = 0;
+    for (int i = 0; i < n; ++i) {
+        C += counts[2 * i];
+        P += counts[2 * i + 1];
+    }
+    return (int64_t) dc_batch_layout(n, C, P).total;
+}
+
+int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *counts, float reg, float svd_tol,
+                        void *workspace, int64_t workspace_bytes, void *scratch, int64_t scratch_bytes, float *out_v,
+                        int64_t cap_v, int32_t *out_f, int64_t cap_f, int64_t *out_counts, void *stream) {
+    if (n > iso::kMaxDcBatchGrids) return fail(ISOEXT_E_ARG, "batch of %d grids exceeds %d", n, iso::kMaxDcBatchGrids);
+    int rc = batch_check(n, items, -1);
+    if (rc != ISOEXT_OK) return rc;
+    for (int i = 0; i < 4 * n + 1; ++i) out_counts[i] = 0;
+    BatchLayout B;
+    batch_layout(n, items, -1, 0, B);
+    if ((int64_t) B.total > workspace_bytes) return fail(ISOEXT_E_WORKSPACE, "workspace too small");
+    int64_t C = 0, P = 0;
+    std::vector<int64_t> cell0(n), ent0(n);
+    for (int i = 0; i < n; ++i) {
+        if (counts[2 * i] < 0 || counts[2 * i + 1] < 0) return fail(ISOEXT_E_ARG, "negative count");
+        cell0[i] = C;
+        ent0[i] = P;
+        C += counts[2 * i];
+        P += counts[2 * i + 1];
+    }
+    // cell and corner indices of the batch are int32 on the device
+    if (C > 0x3fffffffll || 6 * P > 0x7fffffffll)
+        return fail(ISOEXT_E_TOO_LARGE, "batch of %lld cells / %lld intersections is too large", (long long) C,
+                    (long long) P);
+    const DcBatchLayout L = dc_batch_layout(n, C, P);
+    if ((int64_t) L.total > scratch_bytes) return fail(ISOEXT_E_WORKSPACE, "dual contouring scratch too small");
+    if (cap_v < C || cap_f < 2 * P) return fail(ISOEXT_E_CAPACITY, "output capacity too small");
+    if (C == 0) return ISOEXT_OK;
+    hipStream_t st = (hipStream_t) stream;
+    char *ws = (char *) workspace, *sc = (char *) scratch;
+    // 1. intersections with normals, concatenated in grid order
+    iso::McBatchSummary *sum = nullptr;
+    if ((rc = batch_upload(n, items, ws, B, st, &sum, counts)) != ISOEXT_OK) return rc;
+    iso::ItsOut *ho = (iso::ItsOut *) g_pinned_its.get(sizeof(iso::ItsOut) * (size_t) n);
+    if (!ho) return fail(ISOEXT_E_DEVICE, "pinned staging allocation failed");
+    for (int i = 0; i < n; ++i) {
+        iso::ItsOut o;
+        o.cell_indices = (int64_t *) (sc + L.o_ci) + cell0[i];
+        o.cell_dense = (int64_t *) (sc + L.o_cd) + cell0[i];
+        o.cell_offsets = (int64_t *) (sc + L.o_co) + cell0[i];
+        o.points = (float *) (sc + L.o_pts) + 3 * ent0[i];
+        o.normals = (float *) (sc + L.o_nrm) + 3 * ent0[i];
+        o.edge_local = (uint8_t *) (sc + L.o_el) + ent0[i];
+        o.is_out = (uint8_t *) (sc + L.o_io) + ent0[i];
+        o.entry_cell = (int32_t *) (sc + L.o_ec) + ent0[i];
+        o.a_base = cell0[i];
+        o.e_base = ent0[i];
+        ho[i] = o;
+    }
+    HIP_CHECK(hipMemcpyAsync(sc + L.o_outs, ho, sizeof(iso::ItsOut) * (size_t) n, hipMemcpyHostToDevice, st),
+              "upload intersection descriptors");
+    HIP_CHECK(iso::its_batch_emit_launch(B.tile_begin.data(), B.tile_end.data(),
+                                         (const iso::McBatchItem *) (ws + B.o_items),
+                                         (const int32_t *) (ws + B.o_order), (const iso::ItsOut *) (sc + L.o_outs),
+                                         (const iso::McBatchHdr *) (ws + B.o_hdr), st),
+              "batch intersections");
+    HIP_CHECK(iso::fill_i64((int64_t *) (sc + L.o_co) + C, 1, P, st), "batch intersections");
+    // 2. dual contouring of all grids at once
+    iso::ItsIn in;
+    in.ncells = C;
+    in.npts = P;
+    in.cell_indices = (const int64_t *) (sc + L.o_ci);
+    in.cell_dense = (const int64_t *) (sc + L.o_cd);
+    in.cell_offsets = (const int64_t *) (sc + L.o_co);
+    in.points = (const float *) (sc + L.o_pts);
+    in.normals = (const float *) (sc + L.o_nrm);
+    in.edge_local = (const uint8_t *) (sc + L.o_el);
+    in.is_out = (const uint8_t *) (sc + L.o_io);
+    in.entry_cell = (const int32_t *) (sc + L.o_ec);
+    char *dw = sc + L.o_dc;
+    const DcLayout &D = L.dc;
+    iso::DcScratch w;
+    w.dual_v = (float *) (dw + D.o_dual);
+    w.cond = (float *) (dw + D.o_cond);
+    w.n_fp64 = (unsigned long long *) (dw + D.o_cnt);
+    w.used = (uint8_t *) (dw + D.o_used);
+    w.bsum = (int64_t *) (dw + D.o_bsum);
+    w.tot = (int64_t *) (dw + D.o_tot);
+    w.tri_cells = (int32_t *) (dw + D.o_tri);
+    w.cap_tri = 6 * std::max<int64_t>(P, 1);
+    w.hslots = (int32_t *) (dw + D.o_hs);
+    w.slot_min = (int32_t *) (dw + D.o_hm);
+    w.hmask = D.hslots - 1;
+    w.akey = (int64_t *) (dw + D.o_ak);
+    w.aval = (int32_t *) (dw + D.o_av);
+    w.amask = D.hslots - 1;
+    w.rep_slot = (int32_t *) (dw + D.o_rep);
+    w.final_id = (int32_t *) (dw + D.o_fin);
+    w.out_v = out_v;
+    w.out_f = out_f;
+    w.out_vcell = nullptr;
+    const iso::DcBatchDev dd{sc + L.o_items, (int32_t *) (sc + L.o_citem), (int64_t *) (sc + L.o_vb),
+                             (int64_t *) (sc + L.o_qb)};
+    std::vector<iso::GridDesc> g(n);
+    for (int i = 0; i < n; ++i) {
+        isoext_grid gi{items[i].values, nullptr, 0, {items[i].shape[0], items[i].shape[1], items[i].shape[2]},
+                       {items[i].aabb_min[0], items[i].aabb_min[1], items[i].aabb_min[2]},
+                       {items[i].aabb_max[0], items[i].aabb_max[1], items[i].aabb_max[2]}};
+        g[i] = to_desc(&gi);
+    }
+    const float cond_fp64 = std::min(1e6f, 0.5f / std::max(svd_tol, 1e-30f));
+    std::vector<int64_t> nv(n), nq(n);
+    int64_t nfp = 0;
+    HIP_CHECK(iso::dc_run_batch(n, g.data(), cell0.data(), in, reg, svd_tol, cond_fp64, w, dd, nv.data(), nq.data(),
+                                nfp, st),
+              "batched dual contouring");
+    int64_t v0 = 0, f0 = 0;
+    for (int i = 0; i < n; ++i) {
+        out_counts[4 * i] = nv[i];
+        out_counts[4 * i + 1] = 2 * nq[i];
+        out_counts[4 * i + 2] = v0;
+        out_counts[4 * i + 3] = f0;
+        v0 += nv[i];
+        f0 += 2 * nq[i];
+    }
+    out_counts[4 * n] = nfp;
+    return ISOEXT_OK;
+}
 }   // extern "C"
 
 // Error reporting for the library's other translation units.

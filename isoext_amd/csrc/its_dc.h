@@ -24,6 +24,10 @@ struct ItsOut {
     uint8_t *edge_local;
     uint8_t *is_out;
     int32_t *entry_cell;
+    // batches: the arrays above start at the grid's first cell / entry of
+    // batch-wide arrays, whose indices (cell_offsets, entry_cell values) are
+    // offset by these bases
+    int64_t a_base = 0, e_base = 0;
 };
 
 struct ItsIn {
@@ -69,6 +73,22 @@ hipError_t fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t st);
 // FP64 re-solves.
 hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float cond_fp64, DcScratch &w,
                   int64_t &nv, int64_t &nf, int64_t &n_fp64, hipStream_t st);
+// Batched dual contouring of dense grids g[0, n) (n <= 256) whose
+// intersection sets are concatenated in `in` (grid i's active cells start at
+// cell0[i]; cell offsets and entry cells index the concatenated arrays).
+// Output vertices come out grouped by grid in grid order (grid i's first
+// vertex is the sum of nv[0..i)), face corners are grid-local vertex ids, and
+// quads are grouped likewise.  One host synchronisation; nv / nq per grid.
+struct DcBatchDev {
+    void *items;              // dc_batch_item_bytes() * n
+    int32_t *cell_item;       // per active cell
+    int64_t *vbase, *qbase;   // n + 1 each
+};
+constexpr int kMaxDcBatchGrids = 256;
+size_t dc_batch_item_bytes();
+hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const ItsIn &in, float reg, float svd_tol,
+                        float cond_fp64, DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq,
+                        int64_t &n_fp64, hipStream_t st);
 // Backward of dual contouring: dL/d(output vertex) -> dL/d(grid values)
 // (through the QEF solve and the edge interpolation of the intersection
 // points) and, when grad_normals != nullptr, dL/d(normals).

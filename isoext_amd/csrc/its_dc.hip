@@ -256,6 +256,28 @@ __global__ void k_its_normals(CellSrc s, ItsIn in, float *normals) {
     normals[3 * e + 2] = nz;
 }
 
+// ---- batches of dense grids ----------------------------------------------------
+// A batch concatenates the grids' intersection sets (cells, entries) into one
+// index space; each cell knows its grid (cell_item, written by the QEF pass
+// from the grids' first cells), dense cell ids are made unique across grids
+// by a per-grid key base in the active-cell hash, and output vertices are
+// renumbered per grid by the grids' vertex bases.  A grid's vertices and
+// quads are contiguous in the batch-wide scans (cells and entries are in grid
+// order), so its bases are the scans' prefixes at its first cell / entry,
+// recorded by the thread that handles that cell / entry.
+struct DcItem {
+    CellSrc s;          // the grid
+    int64_t cell0;      // first active cell of the grid in the batch
+    int64_t key_base;   // added to the grid's dense cell ids in the active-cell hash
+};
+struct DcBatch {
+    const DcItem *items;   // nullptr: a single grid (the kernels' own CellSrc)
+    int n;
+    int32_t *cell_item;    // per active cell: its grid
+    int64_t *vbase;        // n + 1: first output vertex of each grid, total
+    int64_t *qbase;        // n + 1: first quad of each grid, total
+};
+
 // ---- QEF solve ----------------------------------------------------------------
 
 // Rotation parameters of a Jacobi step: t = tan, c = cos, s = sin.  The FP32
@@ -338,10 +360,27 @@ __device__ __forceinline__ T pinv3(T A[3][3], const T b[3], T tol, T x[3], int s
     return wmin > T(0) ? wmax / wmin : T(INFINITY);
 }
 
-__global__ void k_dc_qef(const ItsIn in, const CellSrc s, float reg, float svd_tol, float cond_fp64,
-                         float *dual_v, float *cond_out, unsigned long long *n_fp64) {
+__global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_tol, float cond_fp64,
+                         float *dual_v, float *cond_out, unsigned long long *n_fp64, DcBatch bt) {
+    __shared__ int64_t starts[kMaxDcBatchGrids];
+    if (bt.items) {
+        for (int i = threadIdx.x; i < bt.n; i += kBlock) starts[i] = bt.items[i].cell0;
+        __syncthreads();
+    }
     int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     if (a >= in.ncells) return;
+    int item = 0;
+    if (bt.items) {
+        int lo_i = 0, hi_i = bt.n;
+        while (hi_i - lo_i > 1) {
+            const int mid = (lo_i + hi_i) >> 1;
+            if (starts[mid] <= a) lo_i = mid;
+            else hi_i = mid;
+        }
+        item = lo_i;
+        bt.cell_item[a] = item;
+    }
+    const CellSrc &s = bt.items ? bt.items[item].s : s1;
     const int64_t lo = in.cell_offsets[a], hi = in.cell_offsets[a + 1];
     const int64_t id = in.cell_dense[a];
     int64_t x, y, z;
@@ -444,10 +483,10 @@ struct ActHash {
 };
 
 __global__ void k_act_insert(const int64_t *cell_dense, int64_t ncells, int64_t *akey, int32_t *aval,
-                             int64_t amask) {
+                             int64_t amask, DcBatch bt) {
     const int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     if (a >= ncells) return;
-    const int64_t key = cell_dense[a];
+    const int64_t key = cell_dense[a] + (bt.items ? bt.items[bt.cell_item[a]].key_base : 0);
     int64_t h = (int64_t) (mix64((uint64_t) key) & (uint64_t) amask);
     while (true) {
         const unsigned long long prev =
@@ -480,7 +519,7 @@ __device__ __forceinline__ int64_t active_of(const ActHash &t, int64_t cid) {
 // them up).  The entry's own cell is q[0] (its active index is
 // entry_cell[e]); the other three come from the active-cell hash.
 __device__ bool entry_quad(const ItsIn &in, const CellSrc &s, const ActHash &ah, int64_t e, int64_t q[4],
-                           bool lookups) {
+                           bool lookups, int64_t key_base = 0) {
     int k = in.edge_local[e];
     if (k != 0 && k != 3 && k != 8) return false;
     const int axis = edge_axis(k);
@@ -493,7 +532,7 @@ __device__ bool entry_quad(const ItsIn &in, const CellSrc &s, const ActHash &ah,
     q[0] = own;
     for (int j = 1; j < 4; ++j) {
         int64_t cx = x - kNb[axis][j][0], cy = y - kNb[axis][j][1], cz = z - kNb[axis][j][2];
-        q[j] = active_of(ah, (cx * (s.Y - 1) + cy) * (s.Z - 1) + cz);
+        q[j] = active_of(ah, (cx * (s.Y - 1) + cy) * (s.Z - 1) + cz + key_base);
         if (q[j] < 0) return false;
     }
     if (!in.is_out[e]) {
@@ -506,11 +545,14 @@ __device__ bool entry_quad(const ItsIn &in, const CellSrc &s, const ActHash &ah,
     return true;
 }
 
-__global__ void k_quad_count(ItsIn in, CellSrc s, ActHash ah, int64_t *bsum) {
+__global__ void k_quad_count(ItsIn in, CellSrc s1, ActHash ah, int64_t *bsum, DcBatch bt) {
     int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t nq = 0;
     int64_t q[4];
-    if (e < in.npts && entry_quad(in, s, ah, e, q, false)) nq = 1;
+    if (e < in.npts) {
+        const DcItem *it = bt.items ? &bt.items[bt.cell_item[in.entry_cell[e]]] : nullptr;
+        if (entry_quad(in, it ? it->s : s1, ah, e, q, false, it ? it->key_base : 0)) nq = 1;
+    }
     int64_t eq, dummy, tq, td;
     block_scan2(nq, 0, eq, dummy, tq, td);
     if (threadIdx.x == 0) {
@@ -527,16 +569,23 @@ __device__ __forceinline__ float dist3(const float *a, const float *b) {
 }
 
 // get_triangles_op (src/dc.cu:101-157): split along the shorter diagonal.
-__global__ void k_quad_emit(ItsIn in, CellSrc s, ActHash ah, const int64_t *bsum_pre, const float *dual_v,
-                            int32_t *tri_cells, uint8_t *used) {
+__global__ void k_quad_emit(ItsIn in, CellSrc s1, ActHash ah, const int64_t *bsum_pre, const float *dual_v,
+                            int32_t *tri_cells, uint8_t *used, DcBatch bt) {
     int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t nq = 0;
     int64_t q[4] = {0, 0, 0, 0};
-    if (e < in.npts && entry_quad(in, s, ah, e, q, true)) nq = 1;
+    int item = 0;
+    if (e < in.npts) {
+        if (bt.items) item = bt.cell_item[in.entry_cell[e]];
+        const DcItem *it = bt.items ? &bt.items[item] : nullptr;
+        if (entry_quad(in, it ? it->s : s1, ah, e, q, true, it ? it->key_base : 0)) nq = 1;
+    }
     int64_t eq, dummy, tq, td;
     block_scan2(nq, 0, eq, dummy, tq, td);
-    if (!nq) return;
     const int64_t slot = bsum_pre[2 * blockIdx.x] + eq;
+    // batch: the grid's first entry records the grid's first quad
+    if (bt.items && e < in.npts && e == in.cell_offsets[bt.items[item].cell0]) bt.qbase[item] = slot;
+    if (!nq) return;
     const float *v0 = dual_v + 3 * q[0], *v1 = dual_v + 3 * q[1], *v2 = dual_v + 3 * q[2], *v3 = dual_v + 3 * q[3];
     int32_t t[6];
     if (dist3(v0, v2) > dist3(v1, v3)) {
@@ -553,11 +602,13 @@ __global__ void k_quad_emit(ItsIn in, CellSrc s, ActHash ah, const int64_t *bsum
 // ---- weld of dual vertices by position ------------------------------------------
 
 __global__ void k_pos_insert(const float *dv, const uint8_t *used, int64_t n, int32_t *slots, int64_t mask,
-                             int32_t *rep) {
+                             int32_t *rep, const int32_t *cell_item) {
     int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     if (a >= n || !used[a]) return;
     float x = dv[3 * a], y = dv[3 * a + 1], z = dv[3 * a + 2];
-    uint64_t h = f3_hash(x, y, z) & (uint64_t) mask;
+    // batches: vertices of different grids never weld (the grid joins the hash and the test)
+    const int32_t item = cell_item ? cell_item[a] : 0;
+    uint64_t h = (f3_hash(x, y, z) ^ (uint64_t) mix64((uint64_t) item)) & (uint64_t) mask;
     for (int64_t probe = 0; probe <= mask; ++probe) {
         int64_t sl = (int64_t) ((h + probe) & (uint64_t) mask);
         int32_t cur = slots[sl];
@@ -569,7 +620,8 @@ __global__ void k_pos_insert(const float *dv, const uint8_t *used, int64_t n, in
             }
             cur = prev;
         }
-        if (dv[3 * cur] == x && dv[3 * cur + 1] == y && dv[3 * cur + 2] == z) {
+        if (dv[3 * cur] == x && dv[3 * cur + 1] == y && dv[3 * cur + 2] == z &&
+            (!cell_item || cell_item[cur] == item)) {
             rep[a] = (int32_t) sl;
             return;
         }
@@ -596,13 +648,18 @@ __global__ void k_keep_flags(const uint8_t *used, int64_t n, const int32_t *rep_
 
 __global__ void k_keep_emit(const float *dv, const uint8_t *used, int64_t n, const int32_t *rep_slot,
                             const int32_t *slot_min, const int64_t *bsum_pre, int32_t *final_id, float *out_v,
-                            int32_t *out_vcell) {
+                            int32_t *out_vcell, DcBatch bt) {
     int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t k = (a < n && used[a] && slot_min[rep_slot[a]] == (int32_t) a) ? 1 : 0;
     int64_t ek, d, tk, td;
     block_scan2(k, 0, ek, d, tk, td);
+    const int64_t id = bsum_pre[2 * blockIdx.x] + ek;
+    // batch: the grid's first cell records the grid's first vertex
+    if (bt.items && a < n) {
+        const int item = bt.cell_item[a];
+        if (a == bt.items[item].cell0) bt.vbase[item] = id;
+    }
     if (!k) return;
-    int64_t id = bsum_pre[2 * blockIdx.x] + ek;
     final_id[a] = (int32_t) id;
     out_v[3 * id + 0] = dv[3 * a + 0];
     out_v[3 * id + 1] = dv[3 * a + 1];
@@ -612,11 +669,27 @@ __global__ void k_keep_emit(const float *dv, const uint8_t *used, int64_t n, con
 
 // Face corners: 6 per quad; the quad count is read from the device.
 __global__ void k_remap_faces(const int32_t *tri_cells, const int64_t *nquads, const int32_t *rep_slot,
-                              const int32_t *slot_min, const int32_t *final_id, int32_t *out_f) {
+                              const int32_t *slot_min, const int32_t *final_id, int32_t *out_f, DcBatch bt) {
     int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     if (i >= 6 * *nquads) return;
     int32_t a = tri_cells[i];
-    out_f[i] = final_id[slot_min[rep_slot[a]]];
+    out_f[i] = final_id[slot_min[rep_slot[a]]] - (bt.items ? (int32_t) bt.vbase[bt.cell_item[a]] : 0);
+}
+
+// Bases of grids without cells (no thread recorded them: they equal the
+// next grid's) and the totals; tot = {quads, -, vertices, -}.
+__global__ void k_item_bases(DcBatch bt, int64_t ncells, const int64_t *tot) {
+    if (threadIdx.x == 0) {
+        bt.vbase[bt.n] = tot[2];
+        bt.qbase[bt.n] = tot[0];
+        for (int i = bt.n - 1; i >= 0; --i) {
+            const int64_t next = i + 1 < bt.n ? bt.items[i + 1].cell0 : ncells;
+            if (bt.items[i].cell0 == next) {
+                bt.vbase[i] = bt.vbase[i + 1];
+                bt.qbase[i] = bt.qbase[i + 1];
+            }
+        }
+    }
 }
 
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
@@ -807,32 +880,31 @@ hipError_t fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float cond_fp64, DcScratch &w,
-                  int64_t &nv, int64_t &nf, int64_t &n_fp64, hipStream_t st) {
-    CellSrc s = make_src(g);
-    nv = nf = n_fp64 = 0;
-    if (in.ncells == 0) return hipSuccess;
+// Launches the DC kernels (no synchronisation).  bt.items == nullptr: one
+// grid (s); otherwise the batch's grids.
+static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float svd_tol, float cond_fp64,
+                            DcScratch &w, const DcBatch &bt, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, (int64_t *) w.n_fp64, (int64_t) 1, (int64_t) 0);
     hipLaunchKernelGGL(k_fill_u8, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, in.ncells, (uint8_t) 0);
     hipLaunchKernelGGL(k_dc_qef, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in, s, reg, svd_tol, cond_fp64,
-                       w.dual_v, w.cond, w.n_fp64);
+                       w.dual_v, w.cond, w.n_fp64, bt);
     // active-cell hash for the quads' neighbour lookups
     const unsigned nbh = (unsigned) std::min<int64_t>(nblocks(w.amask + 1), 4096);
     hipLaunchKernelGGL(k_fill_i64, dim3(nbh), dim3(kBlock), 0, st, w.akey, w.amask + 1, (int64_t) -1);
     hipLaunchKernelGGL(k_act_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in.cell_dense, in.ncells,
-                       w.akey, w.aval, w.amask);
+                       w.akey, w.aval, w.amask, bt);
     const ActHash ah{w.akey, w.aval, w.amask};
     unsigned nbq = nblocks(in.npts);
-    hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum);
+    hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, bt);
     hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbq, w.tot);
     hipLaunchKernelGGL(k_quad_emit, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, w.dual_v, w.tri_cells,
-                       w.used);
+                       w.used, bt);
     // weld dual vertices by position
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.hslots, w.hmask + 1, -1);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.slot_min, w.hmask + 1,
                        INT32_MAX);
     hipLaunchKernelGGL(k_pos_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.dual_v, w.used, in.ncells,
-                       w.hslots, w.hmask, w.rep_slot);
+                       w.hslots, w.hmask, w.rep_slot, (const int32_t *) bt.cell_item);
     hipLaunchKernelGGL(k_slot_min, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, in.ncells, w.rep_slot,
                        w.slot_min);
     unsigned nbv = nblocks(in.ncells);
@@ -840,9 +912,19 @@ hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, 
                        w.bsum);
     hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbv, w.tot + 2);
     hipLaunchKernelGGL(k_keep_emit, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, in.ncells, w.rep_slot,
-                       w.slot_min, w.bsum, w.final_id, w.out_v, w.out_vcell);
+                       w.slot_min, w.bsum, w.final_id, w.out_v, w.out_vcell, bt);
+    if (bt.items) hipLaunchKernelGGL(k_item_bases, dim3(1), dim3(64), 0, st, bt, in.ncells, (const int64_t *) w.tot);
     hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * in.npts)), dim3(kBlock), 0, st, w.tri_cells, w.tot,
-                       w.rep_slot, w.slot_min, w.final_id, w.out_f);
+                       w.rep_slot, w.slot_min, w.final_id, w.out_f, bt);
+    return hipGetLastError();
+}
+
+hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float cond_fp64, DcScratch &w,
+                  int64_t &nv, int64_t &nf, int64_t &n_fp64, hipStream_t st) {
+    CellSrc s = make_src(g);
+    nv = nf = n_fp64 = 0;
+    if (in.ncells == 0) return hipSuccess;
+    ISO_TRY(dc_launch(s, in, reg, svd_tol, cond_fp64, w, DcBatch{}, st));
     ISO_TRY(hipGetLastError());
     // totals and the re-solve counter through a small pinned buffer of the calling thread
     struct PinnedTot {
@@ -861,6 +943,58 @@ hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, 
     nv = pt.p[2];
     nf = 2 * nq;
     n_fp64 = pt.p[4];
+    return hipSuccess;
+}
+
+size_t dc_batch_item_bytes() { return sizeof(DcItem); }
+
+hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const ItsIn &in, float reg, float svd_tol,
+                        float cond_fp64, DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq,
+                        int64_t &n_fp64, hipStream_t st) {
+    n_fp64 = 0;
+    for (int i = 0; i < n; ++i) nv[i] = nq[i] = 0;
+    if (n < 1 || n > kMaxDcBatchGrids) return hipErrorInvalidValue;
+    if (in.ncells == 0) return hipSuccess;
+    // descriptors up and counts back through one pinned buffer of the calling thread
+    struct PinnedStage {
+        char *p = nullptr;
+        size_t bytes = 0;
+        ~PinnedStage() {
+            if (p) (void) hipHostFree(p);
+        }
+    };
+    thread_local PinnedStage ps;
+    const size_t up = sizeof(DcItem) * (size_t) n, down = sizeof(int64_t) * (2 * (size_t) n + 3);
+    const size_t want = ((up + 255) & ~size_t(255)) + down;
+    if (ps.bytes < want) {
+        if (ps.p) ISO_TRY(hipHostFree(ps.p));
+        ps.p = nullptr;
+        ps.bytes = 0;
+        ISO_TRY(hipHostMalloc((void **) &ps.p, want, 0));
+        ps.bytes = want;
+    }
+    DcItem *hi = (DcItem *) ps.p;
+    int64_t key_base = 0;
+    for (int i = 0; i < n; ++i) {
+        hi[i].s = make_src(g[i]);
+        hi[i].cell0 = cell0[i];
+        hi[i].key_base = key_base;
+        key_base += hi[i].s.n;
+    }
+    int64_t *rb = (int64_t *) (ps.p + ((up + 255) & ~size_t(255)));
+    ISO_TRY(hipMemcpyAsync(d.items, hi, up, hipMemcpyHostToDevice, st));
+    const DcBatch bt{(const DcItem *) d.items, n, d.cell_item, d.vbase, d.qbase};
+    ISO_TRY(dc_launch(hi[0].s, in, reg, svd_tol, cond_fp64, w, bt, st));
+    ISO_TRY(hipMemcpyAsync(rb, d.vbase, sizeof(int64_t) * (size_t) (n + 1), hipMemcpyDeviceToHost, st));
+    ISO_TRY(hipMemcpyAsync(rb + n + 1, d.qbase, sizeof(int64_t) * (size_t) (n + 1), hipMemcpyDeviceToHost, st));
+    ISO_TRY(hipMemcpyAsync(rb + 2 * n + 2, w.n_fp64, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    ISO_TRY(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) {
+        nv[i] = rb[i + 1] - rb[i];
+        nq[i] = rb[n + 2 + i] - rb[n + 1 + i];
+    }
+    w.nquads = rb[2 * n + 1];
+    n_fp64 = rb[2 * n + 2];
     return hipSuccess;
 }
 

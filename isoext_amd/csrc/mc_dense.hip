@@ -207,6 +207,8 @@ __device__ __forceinline__ int64_t edge_key(const McParams &p, int64_t gx, int64
 }
 
 // ---- classify ------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 // k_classify_v4 (Z % 64 == 0, float4-aligned rows): word w covers values
 // [64w, 64w + 64); 16 lanes load it with dwordx4 loads and merge their
 // nibbles with three in-group shuffles.
@@ -224,7 +226,8 @@ __global__ void __launch_bounds__(256) k_classify_v4(const float *__restrict__ v
             const int64_t w = base + 4 * u + grp;
             v[u][0] = v[u][1] = v[u][2] = v[u][3] = 0.0f;
             if (w < nwords) {
-                const float4 f = *reinterpret_cast<const float4 *>(values + 64 * w + 4 * q);
+                // streamed once: nontemporal loads keep the values out of L2's working set
+                const f32x4 f = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(values + 64 * w + 4 * q));
                 v[u][0] = f.x;
                 v[u][1] = f.y;
                 v[u][2] = f.z;
@@ -263,7 +266,8 @@ __device__ __forceinline__ void classify_task(const float *__restrict__ values, 
     for (int u = 0; u < U; ++u) {
         const int64_t z = 64 * c + lane;
         ok[u] = w0 + u < nwords && z < Z;
-        v[u] = values[ok[u] ? row * Z + z : 0];   // lanes past the row read a valid dummy
+        // lanes past the row read a valid dummy
+        v[u] = __builtin_nontemporal_load(values + (ok[u] ? row * Z + z : 0));
         if (++c == W) {
             c = 0;
             ++row;
@@ -1207,11 +1211,10 @@ __device__ __forceinline__ CellMasks its_unit_cells(const ItsLds<TX, TY, NZC, NT
 // + a block scan of crossing counts (with a carry for the row that straddles
 // two batches).
 template <int TX, int TY, int NZC, int NT>
-__global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
+__device__ __forceinline__ void its_tile(const McParams &p, const ItsOut &o, int64_t tile) {
     using L = ItsLds<TX, TY, NZC, NT>;
     __shared__ L s;
     const int tid = threadIdx.x;
-    const int64_t tile = blockIdx.x;
     const int64_t tk = tile % p.NK, tj = (tile / p.NK) % p.NJ, ti = tile / (p.NK * p.NJ);
     const int64_t x0 = ti * TX, y0 = tj * TY, c0 = tk * NZC, z0 = 64 * c0;
     const int nx = (int) min<int64_t>(TX, p.X - 1 - x0);
@@ -1307,7 +1310,7 @@ __global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
             if (a < p.cap_v) {   // capacities from the count pass (defensive)
                 o.cell_indices[a] = id;
                 o.cell_dense[a] = id;
-                o.cell_offsets[a] = e;
+                o.cell_offsets[a] = e + o.e_base;
                 const float lo[3] = {s.cx[cx], s.cy[cy], s.cz[zl]};
                 const float hi[3] = {s.cx[cx + 1], s.cy[cy + 1], s.cz[zl + 1]};
                 const int st = crossing_status(cs);
@@ -1321,7 +1324,7 @@ __global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
                     *reinterpret_cast<float3 *>(o.points + 3 * e) = make_float3(px, py, pz);
                     o.edge_local[e] = (uint8_t) E;
                     o.is_out[e] = cv[ca] <= cv[cb] ? 1 : 0;
-                    o.entry_cell[e] = (int32_t) a;
+                    o.entry_cell[e] = (int32_t) (a + o.a_base);
                     if (o.normals) {
                         float nxv, nyv, nzv;
                         cell_normal(lo, hi, cv, px, py, pz, nxv, nyv, nzv);
@@ -1333,6 +1336,28 @@ __global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
         }
         __syncthreads();
     }
+}
+
+template <int TX, int TY, int NZC, int NT>
+__global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
+    its_tile<TX, TY, NZC, NT>(p, o, blockIdx.x);
+}
+
+// Batch: one workgroup per intersection tile of the grids using this
+// configuration, grids found in tile order (as the marching-cubes emit).
+template <int TX, int TY, int NZC, int NT>
+__global__ void __launch_bounds__(NT) k_its_tiles_batch(const McBatchItem *__restrict__ items,
+                                                        const int32_t *__restrict__ order,
+                                                        const ItsOut *__restrict__ outs,
+                                                        const McBatchHdr *__restrict__ hdr, int64_t t_begin) {
+    __shared__ int64_t starts[kMaxBatch];
+    const int n = (int) hdr->n;
+    const int64_t gt = t_begin + blockIdx.x;
+    if (gt >= hdr->ntiles) return;
+    for (int k = threadIdx.x; k < n; k += NT) starts[k] = items[order[k]].tile0;
+    __syncthreads();
+    const int i = order[__builtin_amdgcn_readfirstlane(find_start(starts, n, gt))];
+    its_tile<TX, TY, NZC, NT>(items[i].p, outs[i], gt - items[i].tile0);
 }
 
 __global__ void k_its_reset(int *err, int64_t *totals) {
@@ -1521,6 +1546,28 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     hipLaunchKernelGGL(k_batch_summary, dim3((unsigned) ((h.n + 255) / 256)), dim3(256), 0, stream, items, hdr,
                        summary);
     return hipGetLastError();
+}
+
+hipError_t its_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_end, const McBatchItem *items,
+                                 const int32_t *order, const ItsOut *outs, const McBatchHdr *hdr, hipStream_t stream) {
+    for (int c = 0; c < mc_config_count(); ++c) {
+        const int64_t nt = tile_end[c] - tile_begin[c];
+        if (nt <= 0) continue;
+        switch (c) {
+#define X(id, tx, ty, nzc, nth)                                                                                  \
+    case id:                                                                                                     \
+        hipLaunchKernelGGL((k_its_tiles_batch<tx, ty, nzc, nth>), dim3((unsigned) nt), dim3(nth), 0, stream, items, \
+                           order, outs, hdr, tile_begin[c]);                                                     \
+        break;
+            ISO_MC_CONFIGS(X)
+#undef X
+        default:
+            return hipErrorInvalidValue;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t mc_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_end, const McBatchItem *items,

@@ -6,7 +6,8 @@ footprint (mask workspace, count arrays, outputs) fits in the HBM budget
 that in-flight requests leave free; admitted marching-cubes requests are
 grouped by whichever worker becomes free next into one ragged batch
 (``isoext_amd.batch.mc_batch``: one launch per stage for the whole batch),
-dual-contouring requests run one by one.  Workers own separate HIP streams,
+and so are dual-contouring requests (``isoext_amd.batch.dc_batch``).
+Workers own separate HIP streams,
 so batches overlap on the device; a request that arrives while others run is
 picked up by the next free worker (continuous batching, no fixed batch
 boundaries).  Each result is identical to the plain per-call path on the
@@ -124,19 +125,18 @@ class Server:
 
     # ---- workers -----------------------------------------------------------------
     def _take_locked(self):
-        """Next unit of work: the oldest admitted request decides its kind;
-        a DC request runs alone, MC requests go as one batch of every
-        admitted MC request (in admission order, up to the size limits)."""
+        """Next unit of work: the oldest admitted request decides its kind,
+        and the batch takes every admitted request of that kind (in
+        admission order, up to the size limits)."""
         if not self._mc and not self._dc:
             return None
-        if self._dc and (not self._mc or self._dc[0]._seq < self._mc[0]._seq):
-            return [self._dc.popleft()]
+        q = self._dc if self._dc and (not self._mc or self._dc[0]._seq < self._mc[0]._seq) else self._mc
         batch, pts = [], 0
-        while self._mc and len(batch) < self.max_batch:
-            r = self._mc[0]
+        while q and len(batch) < self.max_batch:
+            r = q[0]
             if batch and pts + r.points > self.max_batch_points:
                 break
-            batch.append(self._mc.popleft())
+            batch.append(q.popleft())
             pts += r.points
         return batch
 
@@ -155,7 +155,7 @@ class Server:
             try:
                 with torch.cuda.stream(stream):
                     if work[0].kind == "dc":
-                        self._run_dc(work[0])
+                        self._run_dc(work)
                     else:
                         self._run_mc(work)
                 stream.synchronize()
@@ -178,12 +178,14 @@ class Server:
             self.stats["batches"] += 1
             self.stats["batched_requests"] += len(reqs)
 
-    def _run_dc(self, r):
-        from .dc import dual_contouring
-        from .grid import UniformGrid
-        r.result = dual_contouring(UniformGrid.wrap(r.values), level=r.level)
+    def _run_dc(self, reqs):
+        from .batch import GridJob, dc_batch
+        out = dc_batch([GridJob(r.values, r.level) for r in reqs], ws_kind="serving_dc")
+        for r, res in zip(reqs, out):
+            r.result = res
         with self._lock:
-            self.stats["dc_requests"] += 1
+            self.stats["dc_requests"] += len(reqs)
+            self.stats["dc_batches"] = self.stats.get("dc_batches", 0) + 1
 
 
 # ---- plain per-call path (the baseline) --------------------------------------------
