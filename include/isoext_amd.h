@@ -298,7 +298,7 @@ int isoext_dc_batch_count(int n, const isoext_mc_item *items, void *workspace, i
                           int64_t *counts, void *stream);
 
 /* Scratch for phase 2 given phase 1's counts. */
-int64_t isoext_dc_batch_scratch_bytes(int n, const int64_t *counts);
+int64_t isoext_dc_batch_scratch_bytes(int n, const isoext_mc_item *items, const int64_t *counts);
 
 /* Phase 2 (same workspace as phase 1, untouched in between): intersections,
  * QEF solve, quads and the vertex weld of every grid, one launch per stage.

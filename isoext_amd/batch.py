@@ -60,7 +60,7 @@ def _decl():
         L.isoext_dc_batch_count.restype = c_int
         L.isoext_dc_batch_count.argtypes = [c_int, ctypes.POINTER(McItem), vp, i64, ctypes.POINTER(i64), vp]
         L.isoext_dc_batch_scratch_bytes.restype = i64
-        L.isoext_dc_batch_scratch_bytes.argtypes = [c_int, ctypes.POINTER(i64)]
+        L.isoext_dc_batch_scratch_bytes.argtypes = [c_int, ctypes.POINTER(McItem), ctypes.POINTER(i64)]
         L.isoext_dc_batch_run.restype = c_int
         L.isoext_dc_batch_run.argtypes = [c_int, ctypes.POINTER(McItem), ctypes.POINTER(i64), ctypes.c_float,
                                           ctypes.c_float, vp, i64, vp, i64, vp, i64, vp, i64, ctypes.POINTER(i64),
@@ -128,7 +128,7 @@ def dc_batch(jobs: list[GridJob], reg: float = 1e-2, svd_tol: float = 1e-6, ws_k
         P = sum(int(counts[2 * i + 1]) for i in range(n))
         if C == 0:
             return [(None, None)] * n
-        sc = workspace.get(ws_kind + "_scratch", L.isoext_dc_batch_scratch_bytes(n, counts), dev)
+        sc = workspace.get(ws_kind + "_scratch", L.isoext_dc_batch_scratch_bytes(n, items, counts), dev)
         out_v = torch.empty((C, 3), dtype=torch.float32, device=dev)
         out_f = torch.empty((max(2 * P, 1), 3), dtype=torch.int32, device=dev)
         rc = L.isoext_dc_batch_run(n, items, counts, float(reg), float(svd_tol), ws.data_ptr(), ws.numel(),
@@ -166,18 +166,25 @@ def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_bat
         ws = workspace.get(ws_kind, ws_bytes, dev)
         _lib.check(L.isoext_mc_batch_count(n, items, cfg, ws.data_ptr(), ws.numel(), special, counts, stream),
                    "mc_batch count")
+        # one vertex and one face buffer for the batch; each grid gets a slice
+        nvs = [int(counts[2 * i]) for i in range(n)]
+        nfs = [int(counts[2 * i + 1]) for i in range(n)]
+        all_v = torch.empty((max(sum(nvs), 1), 3), dtype=torch.float32, device=dev)
+        all_f = torch.empty((max(sum(nfs), 1), 3), dtype=torch.int32, device=dev)
+        v0 = f0 = 0
         outs = []
         for i, it in enumerate(items):
-            nv, nf = int(counts[2 * i]), int(counts[2 * i + 1])
-            v = torch.empty((max(nv, 1), 3), dtype=torch.float32, device=dev)
-            f = torch.empty((max(nf, 1), 3), dtype=torch.int32, device=dev)
-            it.out_v, it.cap_v, it.out_f, it.cap_f = v.data_ptr(), nv, f.data_ptr(), nf
-            outs.append((v, f))
+            it.out_v, it.cap_v = all_v.data_ptr() + 12 * v0, nvs[i]
+            it.out_f, it.cap_f = all_f.data_ptr() + 12 * f0, nfs[i]
+            outs.append((v0, f0))
+            v0 += nvs[i]
+            f0 += nfs[i]
         rc = L.isoext_mc_batch_emit(n, items, cfg, ws.data_ptr(), ws.numel(), special, counts, stream)
     _lib.check(rc, "mc_batch emit")
     res = []
-    for i, (v, f) in enumerate(outs):
+    for i, (v0, f0) in enumerate(outs):
         nv, nf = int(counts[2 * i]), int(counts[2 * i + 1])
+        v, f = all_v[v0:v0 + nvs[i]], all_f[f0:f0 + nfs[i]]
         if nv < 0:   # special lists overflowed the batch capacity: run this grid alone
             from .mc import mc_dense_raw
             j = jobs[i]

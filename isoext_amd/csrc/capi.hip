@@ -248,7 +248,7 @@ bool grid_has_cells(const isoext_grid *g) {
 }
 
 struct DcLayout {
-    size_t o_dual, o_cond, o_cnt, o_used, o_bsum, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, o_ak, o_av, total;
+    size_t o_dual, o_cond, o_cnt, o_used, o_bnd, o_bsum, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, o_ak, o_av, total;
     int64_t hslots;
 };
 
@@ -268,6 +268,7 @@ DcLayout dc_layout(int64_t ncells, int64_t npts) {
     L.o_cond = take(sizeof(float) * std::max<int64_t>(ncells, 1));
     L.o_cnt = take(64);
     L.o_used = take(std::max<int64_t>(ncells, 1));
+    L.o_bnd = take(std::max<int64_t>(ncells, 1));
     L.o_bsum = take(sizeof(int64_t) * 2 * nb);
     L.o_tot = take(64);
     L.o_tri = take(sizeof(int32_t) * 6 * std::max<int64_t>(npts, 1));
@@ -547,11 +548,25 @@ int batch_check(int n, const isoext_mc_item *items, int cfg) {
 // the DC work arrays for C cells and P entries.
 struct DcBatchLayout {
     DcLayout dc;
-    size_t o_dc, o_ci, o_cd, o_co, o_pts, o_nrm, o_el, o_io, o_ec, o_outs, o_items, o_citem, o_vb, o_qb, total;
+    size_t o_dc, o_ci, o_cd, o_co, o_pts, o_nrm, o_el, o_io, o_ec, o_outs, o_items, o_citem, o_vb, o_qb, o_tm, o_tp,
+        total;
+    std::vector<int64_t> tab0;   // per grid: first lookup-table entry
 };
 
-DcBatchLayout dc_batch_layout(int n, int64_t C, int64_t P) {
+// Lookup-table entries of a dense grid: (X-1)(Y-1) cell rows x ceil(Z/64) words.
+int64_t dc_table_entries(const isoext_mc_item &it) {
+    const int64_t X = it.shape[0], Y = it.shape[1], Z = it.shape[2];
+    return (X >= 2 && Y >= 2 && Z >= 2) ? (X - 1) * (Y - 1) * ((Z + 63) / 64) : 0;
+}
+
+DcBatchLayout dc_batch_layout(int n, const isoext_mc_item *items, int64_t C, int64_t P) {
     DcBatchLayout L;
+    L.tab0.assign(n, 0);
+    int64_t T = 0;
+    for (int i = 0; i < n; ++i) {
+        L.tab0[i] = T;
+        T += dc_table_entries(items[i]);
+    }
     size_t o = 0;
     auto take = [&](size_t bytes) {
         size_t r = o;
@@ -572,6 +587,8 @@ DcBatchLayout dc_batch_layout(int n, int64_t C, int64_t P) {
     L.o_citem = take(sizeof(int32_t) * c1);
     L.o_vb = take(sizeof(int64_t) * (size_t) (n + 1));
     L.o_qb = take(sizeof(int64_t) * (size_t) (n + 1));
+    L.o_tm = take(sizeof(uint64_t) * (size_t) std::max<int64_t>(T, 1));
+    L.o_tp = take(sizeof(int32_t) * (size_t) std::max<int64_t>(T, 1));
     L.dc = dc_layout(C, P);
     L.o_dc = take(L.dc.total);
     L.total = o;
@@ -672,6 +689,7 @@ int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_
     w.cond = (float *) (ws + L.o_cond);
     w.n_fp64 = (unsigned long long *) (ws + L.o_cnt);
     w.used = (uint8_t *) (ws + L.o_used);
+    w.bnd = (uint8_t *) (ws + L.o_bnd);
     w.bsum = (int64_t *) (ws + L.o_bsum);
     w.tot = (int64_t *) (ws + L.o_tot);
     w.tri_cells = (int32_t *) (ws + L.o_tri);
@@ -1219,14 +1237,14 @@ int isoext_dc_batch_count(int n, const isoext_mc_item *items, void *workspace, i
     return ISOEXT_OK;
 }
 
-int64_t isoext_dc_batch_scratch_bytes(int n, const int64_t *counts) {
+int64_t isoext_dc_batch_scratch_bytes(int n, const isoext_mc_item *items, const int64_t *counts) {
     if (n < 1 || n > iso::kMaxDcBatchGrids) return -1;
     int64_t C = 0, P = 0;
     for (int i = 0; i < n; ++i) {
         C += counts[2 * i];
         P += counts[2 * i + 1];
     }
-    return (int64_t) dc_batch_layout(n, C, P).total;
+    return (int64_t) dc_batch_layout(n, items, C, P).total;
 }
 
 int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *counts, float reg, float svd_tol,
@@ -1252,7 +1270,7 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     if (C > 0x3fffffffll || 6 * P > 0x7fffffffll)
         return fail(ISOEXT_E_TOO_LARGE, "batch of %lld cells / %lld intersections is too large", (long long) C,
                     (long long) P);
-    const DcBatchLayout L = dc_batch_layout(n, C, P);
+    const DcBatchLayout L = dc_batch_layout(n, items, C, P);
     if ((int64_t) L.total > scratch_bytes) return fail(ISOEXT_E_WORKSPACE, "dual contouring scratch too small");
     if (cap_v < C || cap_f < 2 * P) return fail(ISOEXT_E_CAPACITY, "output capacity too small");
     if (C == 0) return ISOEXT_OK;
@@ -1275,6 +1293,8 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
         o.entry_cell = (int32_t *) (sc + L.o_ec) + ent0[i];
         o.a_base = cell0[i];
         o.e_base = ent0[i];
+        o.amask_tab = (uint64_t *) (sc + L.o_tm) + L.tab0[i];
+        o.apre_tab = (int32_t *) (sc + L.o_tp) + L.tab0[i];
         ho[i] = o;
     }
     HIP_CHECK(hipMemcpyAsync(sc + L.o_outs, ho, sizeof(iso::ItsOut) * (size_t) n, hipMemcpyHostToDevice, st),
@@ -1304,6 +1324,7 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     w.cond = (float *) (dw + D.o_cond);
     w.n_fp64 = (unsigned long long *) (dw + D.o_cnt);
     w.used = (uint8_t *) (dw + D.o_used);
+    w.bnd = (uint8_t *) (dw + D.o_bnd);
     w.bsum = (int64_t *) (dw + D.o_bsum);
     w.tot = (int64_t *) (dw + D.o_tot);
     w.tri_cells = (int32_t *) (dw + D.o_tri);
@@ -1322,7 +1343,11 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     const iso::DcBatchDev dd{sc + L.o_items, (int32_t *) (sc + L.o_citem), (int64_t *) (sc + L.o_vb),
                              (int64_t *) (sc + L.o_qb)};
     std::vector<iso::GridDesc> g(n);
+    std::vector<const uint64_t *> tm(n);
+    std::vector<const int32_t *> tp(n);
     for (int i = 0; i < n; ++i) {
+        tm[i] = (const uint64_t *) (sc + L.o_tm) + L.tab0[i];
+        tp[i] = (const int32_t *) (sc + L.o_tp) + L.tab0[i];
         isoext_grid gi{items[i].values, nullptr, 0, {items[i].shape[0], items[i].shape[1], items[i].shape[2]},
                        {items[i].aabb_min[0], items[i].aabb_min[1], items[i].aabb_min[2]},
                        {items[i].aabb_max[0], items[i].aabb_max[1], items[i].aabb_max[2]}};
@@ -1331,8 +1356,8 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     const float cond_fp64 = std::min(1e6f, 0.5f / std::max(svd_tol, 1e-30f));
     std::vector<int64_t> nv(n), nq(n);
     int64_t nfp = 0;
-    HIP_CHECK(iso::dc_run_batch(n, g.data(), cell0.data(), in, reg, svd_tol, cond_fp64, w, dd, nv.data(), nq.data(),
-                                nfp, st),
+    HIP_CHECK(iso::dc_run_batch(n, g.data(), cell0.data(), tm.data(), tp.data(), in, reg, svd_tol, cond_fp64, w, dd,
+                                nv.data(), nq.data(), nfp, st),
               "batched dual contouring");
     int64_t v0 = 0, f0 = 0;
     for (int i = 0; i < n; ++i) {

@@ -28,6 +28,11 @@ struct ItsOut {
     // batch-wide arrays, whose indices (cell_offsets, entry_cell values) are
     // offset by these bases
     int64_t a_base = 0, e_base = 0;
+    // optional, dense grids: per (cell row, 64-cell word) -- index
+    // (x * (Y-1) + y) * W + z / 64, W = ceil(Z / 64) -- the word's active-cell
+    // mask and the (batch-wide) active index of its first active cell
+    uint64_t *amask_tab = nullptr;
+    int32_t *apre_tab = nullptr;
 };
 
 struct ItsIn {
@@ -47,6 +52,7 @@ struct DcScratch {
     float *cond;
     unsigned long long *n_fp64;
     uint8_t *used;
+    uint8_t *bnd;         // vertex on its cell's box (may coincide with a neighbour's)
     int64_t *bsum;
     int64_t *tot;
     int32_t *tri_cells;
@@ -86,9 +92,12 @@ struct DcBatchDev {
 };
 constexpr int kMaxDcBatchGrids = 256;
 size_t dc_batch_item_bytes();
-hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const ItsIn &in, float reg, float svd_tol,
-                        float cond_fp64, DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq,
-                        int64_t &n_fp64, hipStream_t st);
+// amask / apre: per grid, the lookup tables written by the intersection pass
+// (ItsOut::amask_tab / apre_tab; the active-cell hash is not built).
+hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const uint64_t *const *amask,
+                        const int32_t *const *apre, const ItsIn &in, float reg, float svd_tol, float cond_fp64,
+                        DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq, int64_t &n_fp64,
+                        hipStream_t st);
 // Backward of dual contouring: dL/d(output vertex) -> dL/d(grid values)
 // (through the QEF solve and the edge interpolation of the intersection
 // points) and, when grad_normals != nullptr, dL/d(normals).

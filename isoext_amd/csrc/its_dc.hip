@@ -266,13 +266,17 @@ __global__ void k_its_normals(CellSrc s, ItsIn in, float *normals) {
 // order), so its bases are the scans' prefixes at its first cell / entry,
 // recorded by the thread that handles that cell / entry.
 struct DcItem {
-    CellSrc s;          // the grid
-    int64_t cell0;      // first active cell of the grid in the batch
-    int64_t key_base;   // added to the grid's dense cell ids in the active-cell hash
+    CellSrc s;                // the grid
+    int64_t cell0;            // first active cell of the grid in the batch
+    int64_t key_base;         // added to the grid's dense cell ids in the active-cell hash
+    const uint64_t *amask;    // lookup tables of the grid's active cells (instead of the hash)
+    const int32_t *apre;
+    int64_t W;                // words per cell row in the tables
 };
 struct DcBatch {
     const DcItem *items;   // nullptr: a single grid (the kernels' own CellSrc)
     int n;
+    bool tables;           // every grid has lookup tables (no active-cell hash)
     int32_t *cell_item;    // per active cell: its grid
     int64_t *vbase;        // n + 1: first output vertex of each grid, total
     int64_t *qbase;        // n + 1: first quad of each grid, total
@@ -360,8 +364,11 @@ __device__ __forceinline__ T pinv3(T A[3][3], const T b[3], T tol, T x[3], int s
     return wmin > T(0) ? wmax / wmin : T(INFINITY);
 }
 
+// Also clears used[a] and sets bnd[a] when the clipped vertex lies on its
+// cell's box (the only vertices another cell's vertex can coincide with).
 __global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_tol, float cond_fp64,
-                         float *dual_v, float *cond_out, unsigned long long *n_fp64, DcBatch bt) {
+                         float *dual_v, float *cond_out, unsigned long long *n_fp64, uint8_t *used, uint8_t *bnd,
+                         DcBatch bt) {
     __shared__ int64_t starts[kMaxDcBatchGrids];
     if (bt.items) {
         for (int i = threadIdx.x; i < bt.n; i += kBlock) starts[i] = bt.items[i].cell0;
@@ -452,7 +459,14 @@ __global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_
         atomicAdd(n_fp64, 1ull);
     }
     // clip to the cell (fix_dual_v_op, src/dc.cu:82-99)
-    for (int k = 0; k < 3; ++k) dual_v[3 * a + k] = fminf(fmaxf(out[k], blo[k]), bhi[k]);
+    bool on_box = false;
+    for (int k = 0; k < 3; ++k) {
+        const float v = fminf(fmaxf(out[k], blo[k]), bhi[k]);
+        dual_v[3 * a + k] = v;
+        on_box |= v == blo[k] || v == bhi[k];
+    }
+    used[a] = 0;
+    bnd[a] = on_box ? 1 : 0;
     if (cond_out) cond_out[a] = cond;
 }
 
@@ -518,8 +532,14 @@ __device__ __forceinline__ int64_t active_of(const ActHash &t, int64_t cid) {
 // (lookups = false; a sparse grid may lack some of them, so it always looks
 // them up).  The entry's own cell is q[0] (its active index is
 // entry_cell[e]); the other three come from the active-cell hash.
+// Active index of active cell (x, y, z) of a dense grid from its lookup tables.
+__device__ __forceinline__ int64_t active_of_tab(const DcItem &it, int64_t x, int64_t y, int64_t z) {
+    const int64_t t = (x * (it.s.Y - 1) + y) * it.W + (z >> 6);
+    return (int64_t) it.apre[t] + __popcll(it.amask[t] & ((1ull << (z & 63)) - 1ull));
+}
+
 __device__ bool entry_quad(const ItsIn &in, const CellSrc &s, const ActHash &ah, int64_t e, int64_t q[4],
-                           bool lookups, int64_t key_base = 0) {
+                           bool lookups, int64_t key_base = 0, const DcItem *tab = nullptr) {
     int k = in.edge_local[e];
     if (k != 0 && k != 3 && k != 8) return false;
     const int axis = edge_axis(k);
@@ -532,7 +552,7 @@ __device__ bool entry_quad(const ItsIn &in, const CellSrc &s, const ActHash &ah,
     q[0] = own;
     for (int j = 1; j < 4; ++j) {
         int64_t cx = x - kNb[axis][j][0], cy = y - kNb[axis][j][1], cz = z - kNb[axis][j][2];
-        q[j] = active_of(ah, (cx * (s.Y - 1) + cy) * (s.Z - 1) + cz + key_base);
+        q[j] = tab ? active_of_tab(*tab, cx, cy, cz) : active_of(ah, (cx * (s.Y - 1) + cy) * (s.Z - 1) + cz + key_base);
         if (q[j] < 0) return false;
     }
     if (!in.is_out[e]) {
@@ -578,7 +598,9 @@ __global__ void k_quad_emit(ItsIn in, CellSrc s1, ActHash ah, const int64_t *bsu
     if (e < in.npts) {
         if (bt.items) item = bt.cell_item[in.entry_cell[e]];
         const DcItem *it = bt.items ? &bt.items[item] : nullptr;
-        if (entry_quad(in, it ? it->s : s1, ah, e, q, true, it ? it->key_base : 0)) nq = 1;
+        if (entry_quad(in, it ? it->s : s1, ah, e, q, true, it ? it->key_base : 0,
+                       it && it->amask ? it : nullptr))
+            nq = 1;
     }
     int64_t eq, dummy, tq, td;
     block_scan2(nq, 0, eq, dummy, tq, td);
@@ -600,11 +622,16 @@ __global__ void k_quad_emit(ItsIn in, CellSrc s1, ActHash ah, const int64_t *bsu
 }
 
 // ---- weld of dual vertices by position ------------------------------------------
+// vertex_welding (src/dc.cu:205-208) merges equal positions.  Vertices are
+// clipped to their cells' boxes, and cell boxes meet only on their faces, so
+// two cells' vertices can be equal only when both lie on their boxes (bnd):
+// only those go through the position hash; every other used vertex is its
+// own representative.
 
-__global__ void k_pos_insert(const float *dv, const uint8_t *used, int64_t n, int32_t *slots, int64_t mask,
-                             int32_t *rep, const int32_t *cell_item) {
+__global__ void k_pos_insert(const float *dv, const uint8_t *used, const uint8_t *bnd, int64_t n, int32_t *slots,
+                             int64_t mask, int32_t *rep, const int32_t *cell_item) {
     int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
-    if (a >= n || !used[a]) return;
+    if (a >= n || !used[a] || !bnd[a]) return;
     float x = dv[3 * a], y = dv[3 * a + 1], z = dv[3 * a + 2];
     // batches: vertices of different grids never weld (the grid joins the hash and the test)
     const int32_t item = cell_item ? cell_item[a] : 0;
@@ -628,16 +655,23 @@ __global__ void k_pos_insert(const float *dv, const uint8_t *used, int64_t n, in
     }
 }
 
-__global__ void k_slot_min(const uint8_t *used, int64_t n, const int32_t *rep_slot, int32_t *slot_min) {
+__global__ void k_slot_min(const uint8_t *used, const uint8_t *bnd, int64_t n, const int32_t *rep_slot,
+                           int32_t *slot_min) {
     int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
-    if (a >= n || !used[a]) return;
+    if (a >= n || !used[a] || !bnd[a]) return;
     atomicMin(&slot_min[rep_slot[a]], (int32_t) a);
 }
 
-__global__ void k_keep_flags(const uint8_t *used, int64_t n, const int32_t *rep_slot, const int32_t *slot_min,
-                             int64_t *bsum) {
+// Used vertex a is kept when it represents its position.
+__device__ __forceinline__ bool kept_vertex(const uint8_t *used, const uint8_t *bnd, const int32_t *rep_slot,
+                                            const int32_t *slot_min, int64_t a) {
+    return used[a] && (!bnd[a] || slot_min[rep_slot[a]] == (int32_t) a);
+}
+
+__global__ void k_keep_flags(const uint8_t *used, const uint8_t *bnd, int64_t n, const int32_t *rep_slot,
+                             const int32_t *slot_min, int64_t *bsum) {
     int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
-    int64_t k = (a < n && used[a] && slot_min[rep_slot[a]] == (int32_t) a) ? 1 : 0;
+    int64_t k = (a < n && kept_vertex(used, bnd, rep_slot, slot_min, a)) ? 1 : 0;
     int64_t ek, d, tk, td;
     block_scan2(k, 0, ek, d, tk, td);
     if (threadIdx.x == 0) {
@@ -646,21 +680,17 @@ __global__ void k_keep_flags(const uint8_t *used, int64_t n, const int32_t *rep_
     }
 }
 
-__global__ void k_keep_emit(const float *dv, const uint8_t *used, int64_t n, const int32_t *rep_slot,
-                            const int32_t *slot_min, const int64_t *bsum_pre, int32_t *final_id, float *out_v,
-                            int32_t *out_vcell, DcBatch bt) {
+__global__ void k_keep_emit(const float *dv, const uint8_t *used, const uint8_t *bnd, int64_t n,
+                            const int32_t *rep_slot, const int32_t *slot_min, const int64_t *bsum_pre,
+                            int32_t *final_id, float *out_v, int32_t *out_vcell, DcBatch bt) {
     int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
-    int64_t k = (a < n && used[a] && slot_min[rep_slot[a]] == (int32_t) a) ? 1 : 0;
+    int64_t k = (a < n && kept_vertex(used, bnd, rep_slot, slot_min, a)) ? 1 : 0;
     int64_t ek, d, tk, td;
     block_scan2(k, 0, ek, d, tk, td);
-    const int64_t id = bsum_pre[2 * blockIdx.x] + ek;
-    // batch: the grid's first cell records the grid's first vertex
-    if (bt.items && a < n) {
-        const int item = bt.cell_item[a];
-        if (a == bt.items[item].cell0) bt.vbase[item] = id;
-    }
     if (!k) return;
-    final_id[a] = (int32_t) id;
+    const int64_t id = bsum_pre[2 * blockIdx.x] + ek;
+    // batch: face corners use grid-local vertex ids
+    final_id[a] = (int32_t) (bt.items ? id - bt.vbase[bt.cell_item[a]] : id);
     out_v[3 * id + 0] = dv[3 * a + 0];
     out_v[3 * id + 1] = dv[3 * a + 1];
     out_v[3 * id + 2] = dv[3 * a + 2];
@@ -668,26 +698,48 @@ __global__ void k_keep_emit(const float *dv, const uint8_t *used, int64_t n, con
 }
 
 // Face corners: 6 per quad; the quad count is read from the device.
-__global__ void k_remap_faces(const int32_t *tri_cells, const int64_t *nquads, const int32_t *rep_slot,
-                              const int32_t *slot_min, const int32_t *final_id, int32_t *out_f, DcBatch bt) {
+__global__ void k_remap_faces(const int32_t *tri_cells, const int64_t *nquads, const uint8_t *bnd,
+                              const int32_t *rep_slot, const int32_t *slot_min, const int32_t *final_id,
+                              int32_t *out_f) {
     int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     if (i >= 6 * *nquads) return;
-    int32_t a = tri_cells[i];
-    out_f[i] = final_id[slot_min[rep_slot[a]]] - (bt.items ? (int32_t) bt.vbase[bt.cell_item[a]] : 0);
+    const int32_t a = tri_cells[i];
+    out_f[i] = final_id[bnd[a] ? slot_min[rep_slot[a]] : a];
 }
 
-// Bases of grids without cells (no thread recorded them: they equal the
-// next grid's) and the totals; tot = {quads, -, vertices, -}.
-__global__ void k_item_bases(DcBatch bt, int64_t ncells, const int64_t *tot) {
+// Batch: first output vertex of each grid = kept vertices before its first
+// cell (block prefix of the keep scan + the kept cells of that block before
+// it); one workgroup per grid, vbase[n] = the total.
+__global__ void k_item_vbase(DcBatch bt, const uint8_t *used, const uint8_t *bnd, const int32_t *rep_slot,
+                             const int32_t *slot_min, const int64_t *bsum_pre, int64_t ncells, const int64_t *tot) {
+    __shared__ int part[kBlock / 64];
+    const int i = blockIdx.x;
+    if (i == 0 && threadIdx.x == 0) bt.vbase[bt.n] = tot[2];
+    const int64_t c0 = bt.items[i].cell0;
+    if (c0 >= ncells) {
+        if (threadIdx.x == 0) bt.vbase[i] = tot[2];
+        return;
+    }
+    const int64_t blk = c0 / kBlock, a = blk * kBlock + threadIdx.x;
+    int k = (a < c0 && kept_vertex(used, bnd, rep_slot, slot_min, a)) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) k += __shfl_down(k, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = k;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        bt.vbase[bt.n] = tot[2];
+        int t = 0;
+        for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+        bt.vbase[i] = bsum_pre[2 * blk] + t;
+    }
+}
+
+// First quads of grids without cells (no thread recorded them: they equal
+// the next grid's) and the total; tot = {quads, ...}.
+__global__ void k_item_qbase(DcBatch bt, int64_t ncells, const int64_t *tot) {
+    if (threadIdx.x == 0) {
         bt.qbase[bt.n] = tot[0];
         for (int i = bt.n - 1; i >= 0; --i) {
             const int64_t next = i + 1 < bt.n ? bt.items[i + 1].cell0 : ncells;
-            if (bt.items[i].cell0 == next) {
-                bt.vbase[i] = bt.vbase[i + 1];
-                bt.qbase[i] = bt.qbase[i + 1];
-            }
+            if (bt.items[i].cell0 == next) bt.qbase[i] = bt.qbase[i + 1];
         }
     }
 }
@@ -885,14 +937,14 @@ hipError_t fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t st) {
 static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float svd_tol, float cond_fp64,
                             DcScratch &w, const DcBatch &bt, hipStream_t st) {
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, (int64_t *) w.n_fp64, (int64_t) 1, (int64_t) 0);
-    hipLaunchKernelGGL(k_fill_u8, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, in.ncells, (uint8_t) 0);
     hipLaunchKernelGGL(k_dc_qef, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in, s, reg, svd_tol, cond_fp64,
-                       w.dual_v, w.cond, w.n_fp64, bt);
-    // active-cell hash for the quads' neighbour lookups
-    const unsigned nbh = (unsigned) std::min<int64_t>(nblocks(w.amask + 1), 4096);
-    hipLaunchKernelGGL(k_fill_i64, dim3(nbh), dim3(kBlock), 0, st, w.akey, w.amask + 1, (int64_t) -1);
-    hipLaunchKernelGGL(k_act_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in.cell_dense, in.ncells,
-                       w.akey, w.aval, w.amask, bt);
+                       w.dual_v, w.cond, w.n_fp64, w.used, w.bnd, bt);
+    if (!bt.tables) {   // active-cell hash for the quads' neighbour lookups
+        const unsigned nbh = (unsigned) std::min<int64_t>(nblocks(w.amask + 1), 4096);
+        hipLaunchKernelGGL(k_fill_i64, dim3(nbh), dim3(kBlock), 0, st, w.akey, w.amask + 1, (int64_t) -1);
+        hipLaunchKernelGGL(k_act_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in.cell_dense, in.ncells,
+                           w.akey, w.aval, w.amask, bt);
+    }
     const ActHash ah{w.akey, w.aval, w.amask};
     unsigned nbq = nblocks(in.npts);
     hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, bt);
@@ -903,19 +955,23 @@ static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float 
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.hslots, w.hmask + 1, -1);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.slot_min, w.hmask + 1,
                        INT32_MAX);
-    hipLaunchKernelGGL(k_pos_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.dual_v, w.used, in.ncells,
-                       w.hslots, w.hmask, w.rep_slot, (const int32_t *) bt.cell_item);
-    hipLaunchKernelGGL(k_slot_min, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, in.ncells, w.rep_slot,
-                       w.slot_min);
+    hipLaunchKernelGGL(k_pos_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.dual_v, w.used, w.bnd,
+                       in.ncells, w.hslots, w.hmask, w.rep_slot, (const int32_t *) bt.cell_item);
+    hipLaunchKernelGGL(k_slot_min, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, w.bnd, in.ncells,
+                       w.rep_slot, w.slot_min);
     unsigned nbv = nblocks(in.ncells);
-    hipLaunchKernelGGL(k_keep_flags, dim3(nbv), dim3(kBlock), 0, st, w.used, in.ncells, w.rep_slot, w.slot_min,
-                       w.bsum);
+    hipLaunchKernelGGL(k_keep_flags, dim3(nbv), dim3(kBlock), 0, st, w.used, w.bnd, in.ncells, w.rep_slot,
+                       w.slot_min, w.bsum);
     hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbv, w.tot + 2);
-    hipLaunchKernelGGL(k_keep_emit, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, in.ncells, w.rep_slot,
-                       w.slot_min, w.bsum, w.final_id, w.out_v, w.out_vcell, bt);
-    if (bt.items) hipLaunchKernelGGL(k_item_bases, dim3(1), dim3(64), 0, st, bt, in.ncells, (const int64_t *) w.tot);
-    hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * in.npts)), dim3(kBlock), 0, st, w.tri_cells, w.tot,
-                       w.rep_slot, w.slot_min, w.final_id, w.out_f, bt);
+    if (bt.items) {
+        hipLaunchKernelGGL(k_item_vbase, dim3(bt.n), dim3(kBlock), 0, st, bt, w.used, w.bnd, w.rep_slot, w.slot_min,
+                           w.bsum, in.ncells, (const int64_t *) w.tot);
+        hipLaunchKernelGGL(k_item_qbase, dim3(1), dim3(64), 0, st, bt, in.ncells, (const int64_t *) w.tot);
+    }
+    hipLaunchKernelGGL(k_keep_emit, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, w.bnd, in.ncells,
+                       w.rep_slot, w.slot_min, w.bsum, w.final_id, w.out_v, w.out_vcell, bt);
+    hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * in.npts)), dim3(kBlock), 0, st, w.tri_cells, w.tot, w.bnd,
+                       w.rep_slot, w.slot_min, w.final_id, w.out_f);
     return hipGetLastError();
 }
 
@@ -948,9 +1004,10 @@ hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, 
 
 size_t dc_batch_item_bytes() { return sizeof(DcItem); }
 
-hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const ItsIn &in, float reg, float svd_tol,
-                        float cond_fp64, DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq,
-                        int64_t &n_fp64, hipStream_t st) {
+hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const uint64_t *const *amask,
+                        const int32_t *const *apre, const ItsIn &in, float reg, float svd_tol, float cond_fp64,
+                        DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq, int64_t &n_fp64,
+                        hipStream_t st) {
     n_fp64 = 0;
     for (int i = 0; i < n; ++i) nv[i] = nq[i] = 0;
     if (n < 1 || n > kMaxDcBatchGrids) return hipErrorInvalidValue;
@@ -975,15 +1032,20 @@ hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const It
     }
     DcItem *hi = (DcItem *) ps.p;
     int64_t key_base = 0;
+    bool tables = amask && apre;
     for (int i = 0; i < n; ++i) {
         hi[i].s = make_src(g[i]);
         hi[i].cell0 = cell0[i];
         hi[i].key_base = key_base;
         key_base += hi[i].s.n;
+        hi[i].amask = amask ? amask[i] : nullptr;
+        hi[i].apre = apre ? apre[i] : nullptr;
+        hi[i].W = (g[i].Z + 63) / 64;
+        if (g[i].sparse_ids) tables = false;
     }
     int64_t *rb = (int64_t *) (ps.p + ((up + 255) & ~size_t(255)));
     ISO_TRY(hipMemcpyAsync(d.items, hi, up, hipMemcpyHostToDevice, st));
-    const DcBatch bt{(const DcItem *) d.items, n, d.cell_item, d.vbase, d.qbase};
+    const DcBatch bt{(const DcItem *) d.items, n, tables, d.cell_item, d.vbase, d.qbase};
     ISO_TRY(dc_launch(hi[0].s, in, reg, svd_tol, cond_fp64, w, bt, st));
     ISO_TRY(hipMemcpyAsync(rb, d.vbase, sizeof(int64_t) * (size_t) (n + 1), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipMemcpyAsync(rb + n + 1, d.qbase, sizeof(int64_t) * (size_t) (n + 1), hipMemcpyDeviceToHost, st));

@@ -1255,6 +1255,18 @@ __device__ __forceinline__ void its_tile(const McParams &p, const ItsOut &o, int
     }
     __syncthreads();
     block_exscan_lds<L::NUNITS, NT>(s.na, s.tsum);
+    if (o.amask_tab) {
+        // per (cell row, word): active cells and the active index of the
+        // first of them, for the neighbour lookups of the quads
+        for (int u = tid; u < L::NUNITS; u += NT) {
+            const int j = u % NZC, r = u / NZC, cy = r % TY, cx = r / TY;
+            const int64_t c = c0 + j;
+            if (cx >= nx || cy >= ny || 64 * c >= p.Z - 1) continue;
+            const int64_t t = ((x0 + cx) * (p.Y - 1) + (y0 + cy)) * p.W + c;
+            o.amask_tab[t] = s.amask[u];
+            o.apre_tab[t] = (int32_t) (s.abase[r] + (int64_t) (s.na[u] - s.na[r * NZC]) + o.a_base);
+        }
+    }
     const int nactive = (int) s.na[L::NUNITS];
     if (tid == 0) s.carry[0] = 0;
     for (int j0 = 0, bt = 0; j0 < nactive; j0 += NT, bt ^= 1) {

@@ -88,8 +88,12 @@ class Server:
         self._t0 = time.perf_counter()
 
     def submit(self, req: Request):
+        self.submit_many([req])
+
+    def submit_many(self, reqs):
+        """Enqueue requests that arrived together (one admission pass)."""
         with self._lock:
-            self._pending.append(req)
+            self._pending.extend(reqs)
             self._admit_locked()
             self._lock.notify_all()
 
@@ -240,11 +244,18 @@ def drive(server: Server, reqs):
     (wall seconds, latencies in seconds)."""
     server.reset_clock()
     t0 = time.perf_counter()
-    for r in reqs:
-        dt = r.arrival - (time.perf_counter() - t0)
+    i = 0
+    while i < len(reqs):
+        dt = reqs[i].arrival - (time.perf_counter() - t0)
         if dt > 0:
             time.sleep(dt)
-        server.submit(r)
+        # every request that has arrived by now goes in one submission
+        now = time.perf_counter() - t0
+        j = i + 1
+        while j < len(reqs) and reqs[j].arrival <= now:
+            j += 1
+        server.submit_many(reqs[i:j])
+        i = j
     for r in reqs:
         r.done.wait()
     wall = time.perf_counter() - t0
