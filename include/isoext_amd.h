@@ -118,6 +118,12 @@ typedef struct isoext_its {
     const uint8_t *edge_local;   /* edge index 0..11 inside its cell */
     const uint8_t *is_out;       /* value at the lower corner <= value at the upper */
     const int32_t *entry_cell;   /* active-cell index of each entry */
+    /* Optional (UniformGrid sets written by isoext_its_emit, else NULL):
+     * per (cell row x * (Y-1) + y, 64-cell word z / 64) the word's active-cell
+     * mask and the active index of its first active cell.  isoext_dc then
+     * finds a quad's neighbour cells from them instead of a hash. */
+    const uint64_t *amask_tab;
+    const int32_t *apre_tab;
 } isoext_its;
 
 /* Scratch bytes shared by isoext_its_count / isoext_its_emit for `g`. */
@@ -128,13 +134,19 @@ int64_t isoext_its_scratch_bytes(const isoext_grid *g);
 int isoext_its_count(const isoext_grid *g, float level, void *scratch, int64_t scratch_bytes,
                      int64_t counts[2], void *stream);
 
+/* Entries of the lookup tables of a UniformGrid intersection set:
+ * (X-1) * (Y-1) * ceil(Z / 64). */
+int64_t isoext_its_table_entries(const int64_t shape[3]);
+
 /* get_intersection pass 2 (get_its_op src/its.cu:37-90, plus
  * compute_normals_op src/its.cu:186-268 when normals != NULL), using the
- * scratch of the preceding isoext_its_count on the same grid and level. */
+ * scratch of the preceding isoext_its_count on the same grid and level.
+ * amask_tab / apre_tab (UniformGrid only, isoext_its_table_entries each, or
+ * NULL) receive the set's lookup tables (see isoext_its). */
 int isoext_its_emit(const isoext_grid *g, float level, const void *scratch, int64_t ncells,
                     int64_t npts, int64_t *cell_indices, int64_t *cell_dense, int64_t *cell_offsets,
                     float *points, float *normals, uint8_t *edge_local, uint8_t *is_out,
-                    int32_t *entry_cell, void *stream);
+                    int32_t *entry_cell, uint64_t *amask_tab, int32_t *apre_tab, void *stream);
 
 /* compute_intersection_normals (src/its.cu:271-284) for an existing set. */
 int isoext_its_normals(const isoext_grid *g, const isoext_its *its, float *normals, void *stream);

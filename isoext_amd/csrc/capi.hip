@@ -238,6 +238,8 @@ iso::ItsIn to_in(const isoext_its *t) {
     in.edge_local = t->edge_local;
     in.is_out = t->is_out;
     in.entry_cell = t->entry_cell;
+    in.amask_tab = t->amask_tab;
+    in.apre_tab = t->apre_tab;
     return in;
 }
 
@@ -637,12 +639,23 @@ int isoext_its_count(const isoext_grid *g, float level, void *scratch, int64_t s
     return ISOEXT_OK;
 }
 
+int64_t isoext_its_table_entries(const int64_t shape[3]) {
+    if (shape[0] < 2 || shape[1] < 2 || shape[2] < 2) return 0;
+    return (shape[0] - 1) * (shape[1] - 1) * ((shape[2] + 63) / 64);
+}
+
 int isoext_its_emit(const isoext_grid *g, float level, const void *scratch, int64_t ncells, int64_t npts,
                     int64_t *cell_indices, int64_t *cell_dense, int64_t *cell_offsets, float *points, float *normals,
-                    uint8_t *edge_local, uint8_t *is_out, int32_t *entry_cell, void *stream) {
+                    uint8_t *edge_local, uint8_t *is_out, int32_t *entry_cell, uint64_t *amask_tab,
+                    int32_t *apre_tab, void *stream) {
     hipStream_t st = (hipStream_t) stream;
     if (grid_has_cells(g) && ncells > 0) {
         iso::ItsOut o{cell_indices, cell_dense, cell_offsets, points, normals, edge_local, is_out, entry_cell};
+        if (!g->sparse_cell_ids && amask_tab && apre_tab) {
+            if (ncells > 0x7fffffffll) return fail(ISOEXT_E_TOO_LARGE, "more than 2^31-1 active cells");
+            o.amask_tab = amask_tab;
+            o.apre_tab = apre_tab;
+        }
         if (!g->sparse_cell_ids) {
             iso::McParams p = its_params(g, level, (char *) scratch, ncells, npts);
             HIP_CHECK(iso::its_dense_emit_launch(iso::mc_auto_config(g->shape), p, o, st), "intersection emit");

@@ -45,6 +45,9 @@ struct ItsIn {
     const uint8_t *edge_local;
     const uint8_t *is_out;
     const int32_t *entry_cell;
+    // optional (dense grids): the lookup tables of ItsOut::amask_tab / apre_tab
+    const uint64_t *amask_tab = nullptr;
+    const int32_t *apre_tab = nullptr;
 };
 
 struct DcScratch {

@@ -596,11 +596,14 @@ __global__ void k_quad_emit(ItsIn in, CellSrc s1, ActHash ah, const int64_t *bsu
     int64_t q[4] = {0, 0, 0, 0};
     int item = 0;
     if (e < in.npts) {
-        if (bt.items) item = bt.cell_item[in.entry_cell[e]];
-        const DcItem *it = bt.items ? &bt.items[item] : nullptr;
-        if (entry_quad(in, it ? it->s : s1, ah, e, q, true, it ? it->key_base : 0,
-                       it && it->amask ? it : nullptr))
-            nq = 1;
+        if (bt.items) {
+            item = bt.cell_item[in.entry_cell[e]];
+            const DcItem *it = &bt.items[item];
+            if (entry_quad(in, it->s, ah, e, q, true, it->key_base, it->amask ? it : nullptr)) nq = 1;
+        } else {
+            const DcItem one{s1, 0, 0, in.amask_tab, in.apre_tab, (s1.Z + 63) / 64};
+            if (entry_quad(in, s1, ah, e, q, true, 0, in.amask_tab && in.apre_tab && !s1.ids ? &one : nullptr)) nq = 1;
+        }
     }
     int64_t eq, dummy, tq, td;
     block_scan2(nq, 0, eq, dummy, tq, td);
@@ -939,7 +942,8 @@ static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float 
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, (int64_t *) w.n_fp64, (int64_t) 1, (int64_t) 0);
     hipLaunchKernelGGL(k_dc_qef, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in, s, reg, svd_tol, cond_fp64,
                        w.dual_v, w.cond, w.n_fp64, w.used, w.bnd, bt);
-    if (!bt.tables) {   // active-cell hash for the quads' neighbour lookups
+    const bool tables = bt.items ? bt.tables : (in.amask_tab && in.apre_tab && !s.ids);
+    if (!tables) {   // active-cell hash for the quads' neighbour lookups
         const unsigned nbh = (unsigned) std::min<int64_t>(nblocks(w.amask + 1), 4096);
         hipLaunchKernelGGL(k_fill_i64, dim3(nbh), dim3(kBlock), 0, st, w.akey, w.amask + 1, (int64_t) -1);
         hipLaunchKernelGGL(k_act_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in.cell_dense, in.ncells,

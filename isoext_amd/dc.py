@@ -40,13 +40,15 @@ class IsoIts(ctypes.Structure):
     _fields_ = [("ncells", ctypes.c_int64), ("npts", ctypes.c_int64), ("cell_indices", ctypes.c_void_p),
                 ("cell_dense", ctypes.c_void_p), ("cell_offsets", ctypes.c_void_p), ("points", ctypes.c_void_p),
                 ("normals", ctypes.c_void_p), ("edge_local", ctypes.c_void_p), ("is_out", ctypes.c_void_p),
-                ("entry_cell", ctypes.c_void_p)]
+                ("entry_cell", ctypes.c_void_p), ("amask_tab", ctypes.c_void_p), ("apre_tab", ctypes.c_void_p)]
 
 
 vp, i64, f32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_float
 _lib.declare("isoext_its_scratch_bytes", i64, [P(IsoGrid)])
 _lib.declare("isoext_its_count", ctypes.c_int, [P(IsoGrid), f32, vp, i64, i64 * 2, vp])
-_lib.declare("isoext_its_emit", ctypes.c_int, [P(IsoGrid), f32, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp])
+_lib.declare("isoext_its_table_entries", i64, [i64 * 3])
+_lib.declare("isoext_its_emit", ctypes.c_int, [P(IsoGrid), f32, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                               vp])
 _lib.declare("isoext_its_normals", ctypes.c_int, [P(IsoGrid), P(IsoIts), vp, vp])
 _lib.declare("isoext_dc_workspace_bytes", i64, [i64, i64])
 _lib.declare("isoext_dc", ctypes.c_int, [P(IsoGrid), P(IsoIts), f32, f32, vp, i64, vp, i64, vp, i64, vp, vp,
@@ -78,7 +80,7 @@ class Intersection:
     """Edge/surface intersections of a grid (points, optional normals)."""
 
     def __init__(self, grid, level, cell_indices, cell_dense, cell_offsets, points, normals, edge_local, is_out,
-                 entry_cell):
+                 entry_cell, tables=None):
         self._grid = grid
         self._level = float(level)
         self.cell_indices = cell_indices
@@ -89,6 +91,9 @@ class Intersection:
         self.edge_local = edge_local
         self.is_out = is_out
         self.entry_cell = entry_cell
+        # dense grids: per (cell row, 64-cell word) active masks and first
+        # active indices, the neighbour lookups of dual contouring's quads
+        self._tables = tables
         self._has_normals = normals is not None
 
     def get_points(self) -> torch.Tensor:
@@ -126,6 +131,9 @@ class Intersection:
         s.edge_local = self.edge_local.data_ptr()
         s.is_out = self.is_out.data_ptr()
         s.entry_cell = self.entry_cell.data_ptr()
+        if self._tables is not None:
+            s.amask_tab = self._tables[0].data_ptr()
+            s.apre_tab = self._tables[1].data_ptr()
         return s
 
 
@@ -157,14 +165,20 @@ def get_intersection(grid, level: float = 0.0, compute_normals: bool = False) ->
         edge_local = torch.empty(npt, dtype=torch.uint8, device=dev)
         is_out = torch.empty(npt, dtype=torch.uint8, device=dev)
         entry_cell = torch.empty(npt, dtype=torch.int32, device=dev)
+        tables = None
+        if not g.sparse_cell_ids and nc > 0:
+            nt = int(L.isoext_its_table_entries(g.shape))
+            tables = (torch.empty(nt, dtype=torch.int64, device=dev), torch.empty(nt, dtype=torch.int32, device=dev))
         _lib.check(L.isoext_its_emit(ctypes.byref(g), float(level), scratch.data_ptr(), nc, npt,
                                      cell_indices.data_ptr(), cell_dense.data_ptr(), cell_offsets.data_ptr(),
                                      points.data_ptr() if npt else None,
                                      normals.data_ptr() if (normals is not None and npt) else None,
                                      edge_local.data_ptr() if npt else None, is_out.data_ptr() if npt else None,
-                                     entry_cell.data_ptr() if npt else None, stream), "get_intersection")
+                                     entry_cell.data_ptr() if npt else None,
+                                     tables[0].data_ptr() if tables else None,
+                                     tables[1].data_ptr() if tables else None, stream), "get_intersection")
     return Intersection(grid, level, cell_indices, cell_dense, cell_offsets, points, normals, edge_local, is_out,
-                        entry_cell)
+                        entry_cell, tables)
 
 
 def compute_normals(grid, its: Intersection) -> torch.Tensor:

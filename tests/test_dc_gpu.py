@@ -190,3 +190,33 @@ def test_dc_api(gpu):
     g2 = isoext.UniformGrid([8, 8, 8])
     g2.set_values(torch.ones((8, 8, 8), device="cuda"))
     assert isoext.dual_contouring(g2) == (None, None)
+
+
+@pytest.mark.parametrize("shape", [(40, 52, 70), (33, 17, 129), (2, 2, 2), (9, 64, 64)])
+def test_dc_lookup_tables_match_hash(gpu, shape):
+    """Quads found through the intersection's per-row lookup tables equal
+    those found through the active-cell hash (tables dropped), and the
+    tables agree with the active cells' own indices."""
+    import isoext_amd as iso
+    from isoext_amd.dc import dc_raw
+    vals = multi_sphere_field(shape, 6, seed=sum(shape))
+    g = make_grid(vals)
+    its = iso.get_intersection(g, 0.0, compute_normals=True)
+    if len(its) == 0:
+        return
+    assert its._tables is not None
+    amask, apre = (t.cpu().numpy() for t in its._tables)
+    X, Y, Z = shape
+    W = (Z + 63) // 64
+    cd = its.cell_dense.cpu().numpy()
+    z, r = cd % (Z - 1), cd // (Z - 1)
+    t = r * W + z // 64
+    below = np.array([bin(int(m) & ((1 << int(b)) - 1)).count("1")
+                      for m, b in zip(amask[t].view(np.uint64), z % 64)])
+    assert np.array_equal(apre[t].astype(np.int64) + below, np.arange(len(cd)))
+    v1, f1, i1 = dc_raw(g, its, its.normals, 1e-2, 1e-6)
+    its._tables = None
+    v0, f0, i0 = dc_raw(g, its, its.normals, 1e-2, 1e-6)
+    assert np.array_equal(v1.cpu().numpy(), v0.cpu().numpy())
+    assert np.array_equal(f1.cpu().numpy(), f0.cpu().numpy())
+    assert i1["quads"] == i0["quads"]
