@@ -250,7 +250,8 @@ bool grid_has_cells(const isoext_grid *g) {
 }
 
 struct DcLayout {
-    size_t o_dual, o_cond, o_cnt, o_used, o_bnd, o_bsum, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, o_ak, o_av, total;
+    size_t o_dual, o_cond, o_cnt, o_used, o_bnd, o_bsum, o_part, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, o_ak, o_av,
+        total;
     int64_t hslots;
 };
 
@@ -272,6 +273,7 @@ DcLayout dc_layout(int64_t ncells, int64_t npts) {
     L.o_used = take(std::max<int64_t>(ncells, 1));
     L.o_bnd = take(std::max<int64_t>(ncells, 1));
     L.o_bsum = take(sizeof(int64_t) * 2 * nb);
+    L.o_part = take(sizeof(int64_t) * 2 * iso::dc_scan_part_pairs(nb));
     L.o_tot = take(64);
     L.o_tri = take(sizeof(int32_t) * 6 * std::max<int64_t>(npts, 1));
     L.o_hs = take(sizeof(int32_t) * hs);
@@ -704,6 +706,7 @@ int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_
     w.used = (uint8_t *) (ws + L.o_used);
     w.bnd = (uint8_t *) (ws + L.o_bnd);
     w.bsum = (int64_t *) (ws + L.o_bsum);
+    w.part = (int64_t *) (ws + L.o_part);
     w.tot = (int64_t *) (ws + L.o_tot);
     w.tri_cells = (int32_t *) (ws + L.o_tri);
     w.cap_tri = 6 * std::max<int64_t>(its->npts, 1);
@@ -1339,6 +1342,7 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     w.used = (uint8_t *) (dw + D.o_used);
     w.bnd = (uint8_t *) (dw + D.o_bnd);
     w.bsum = (int64_t *) (dw + D.o_bsum);
+    w.part = (int64_t *) (dw + D.o_part);
     w.tot = (int64_t *) (dw + D.o_tot);
     w.tri_cells = (int32_t *) (dw + D.o_tri);
     w.cap_tri = 6 * std::max<int64_t>(P, 1);

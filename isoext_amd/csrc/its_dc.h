@@ -57,6 +57,7 @@ struct DcScratch {
     uint8_t *used;
     uint8_t *bnd;         // vertex on its cell's box (may coincide with a neighbour's)
     int64_t *bsum;
+    int64_t *part;        // chunk sums of the block-sum scans (dc_scan_part_pairs(nb) pairs)
     int64_t *tot;
     int32_t *tri_cells;
     int64_t cap_tri;
@@ -94,6 +95,8 @@ struct DcBatchDev {
     int64_t *vbase, *qbase;   // n + 1 each
 };
 constexpr int kMaxDcBatchGrids = 256;
+// Pairs of scan scratch for nb block-sum pairs.
+inline int64_t dc_scan_part_pairs(int64_t nb) { return nb / 1024 + 2; }
 size_t dc_batch_item_bytes();
 // amask / apre: per grid, the lookup tables written by the intersection pass
 // (ItsOut::amask_tab / apre_tab; the active-cell hash is not built).

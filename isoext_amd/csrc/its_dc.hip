@@ -364,11 +364,14 @@ __device__ __forceinline__ T pinv3(T A[3][3], const T b[3], T tol, T x[3], int s
     return wmin > T(0) ? wmax / wmin : T(INFINITY);
 }
 
-// Also clears used[a] and sets bnd[a] when the clipped vertex lies on its
-// cell's box (the only vertices another cell's vertex can coincide with).
+// Also sets bnd[a] when the clipped vertex lies on its cell's box (the only
+// vertices another cell's vertex can coincide with), and used[a]: with
+// dense_used (a dense grid whose intersection set is the grid's own), whether
+// the cell belongs to a quad -- one of its crossing edges has all four cells
+// around it inside the grid; otherwise 0, set later by the quad pass.
 __global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_tol, float cond_fp64,
                          float *dual_v, float *cond_out, unsigned long long *n_fp64, uint8_t *used, uint8_t *bnd,
-                         DcBatch bt) {
+                         bool dense_used, DcBatch bt) {
     __shared__ int64_t starts[kMaxDcBatchGrids];
     if (bt.items) {
         for (int i = threadIdx.x; i < bt.n; i += kBlock) starts[i] = bt.items[i].cell0;
@@ -465,7 +468,20 @@ __global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_
         dual_v[3 * a + k] = v;
         on_box |= v == blo[k] || v == bhi[k];
     }
-    used[a] = 0;
+    uint8_t u = 0;
+    if (dense_used) {
+        const int64_t cc[3] = {x, y, z}, N[3] = {s.X, s.Y, s.Z};
+        for (int64_t j = lo; j < hi && !u; ++j) {
+            const int k = in.edge_local[j], ax = edge_axis(k), ca = edge_a(k);
+            const int off[3] = {ca >> 2, (ca >> 1) & 1, ca & 1};
+            bool ok = true;
+            // across the edge, cells cc[d] - 1 and cc[d] (offset 0) or cc[d] and cc[d] + 1 (offset 1)
+            for (int d = 0; d < 3; ++d)
+                if (d != ax) ok = ok && (off[d] ? cc[d] + 3 <= N[d] : cc[d] >= 1);
+            u = ok ? 1 : 0;
+        }
+    }
+    used[a] = u;
     bnd[a] = on_box ? 1 : 0;
     if (cond_out) cond_out[a] = cond;
 }
@@ -589,8 +605,22 @@ __device__ __forceinline__ float dist3(const float *a, const float *b) {
 }
 
 // get_triangles_op (src/dc.cu:101-157): split along the shorter diagonal.
+// Vertex id of cell a's dual vertex after the weld.
+__device__ __forceinline__ int32_t welded_id(const uint8_t *bnd, const int32_t *rep_slot, const int32_t *slot_min,
+                                             const int32_t *final_id, int32_t a) {
+    return final_id[bnd[a] ? slot_min[rep_slot[a]] : a];
+}
+
+// Quad corners go to tri_cells (cells; used[] marked here), or, when the
+// weld ran first (wd.final_id != nullptr), straight to out_f as vertex ids.
+struct WeldIds {
+    const uint8_t *bnd;
+    const int32_t *rep_slot, *slot_min, *final_id;
+    int32_t *out_f;
+};
+
 __global__ void k_quad_emit(ItsIn in, CellSrc s1, ActHash ah, const int64_t *bsum_pre, const float *dual_v,
-                            int32_t *tri_cells, uint8_t *used, DcBatch bt) {
+                            int32_t *tri_cells, uint8_t *used, WeldIds wd, DcBatch bt) {
     int64_t e = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t nq = 0;
     int64_t q[4] = {0, 0, 0, 0};
@@ -619,6 +649,10 @@ __global__ void k_quad_emit(ItsIn in, CellSrc s1, ActHash ah, const int64_t *bsu
     } else {
         int32_t u[6] = {(int32_t) q[2], (int32_t) q[0], (int32_t) q[1], (int32_t) q[0], (int32_t) q[2], (int32_t) q[3]};
         for (int k = 0; k < 6; ++k) t[k] = u[k];
+    }
+    if (wd.final_id) {
+        for (int k = 0; k < 6; ++k) wd.out_f[6 * slot + k] = welded_id(wd.bnd, wd.rep_slot, wd.slot_min, wd.final_id, t[k]);
+        return;
     }
     for (int k = 0; k < 6; ++k) tri_cells[6 * slot + k] = t[k];
     for (int k = 0; k < 4; ++k) used[q[k]] = 1;
@@ -706,8 +740,7 @@ __global__ void k_remap_faces(const int32_t *tri_cells, const int64_t *nquads, c
                               int32_t *out_f) {
     int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     if (i >= 6 * *nquads) return;
-    const int32_t a = tri_cells[i];
-    out_f[i] = final_id[bnd[a] ? slot_min[rep_slot[a]] : a];
+    out_f[i] = welded_id(bnd, rep_slot, slot_min, final_id, tri_cells[i]);
 }
 
 // Batch: first output vertex of each grid = kept vertices before its first
@@ -755,6 +788,95 @@ __global__ void k_fill_i64(int64_t *p, int64_t n, int64_t v) {
 }
 __global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
     for (int64_t i = (int64_t) blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t) gridDim.x * kBlock) p[i] = v;
+}
+
+// exclusive scan of block sums (pairs), single block, in place; totals out
+// Parallel exclusive scan of the block-sum pairs for large inputs: chunk
+// sums, one workgroup over the chunk sums, then each chunk rescanned with its
+// prefix (kScanChunk pairs per workgroup).
+constexpr int kScanChunk = 1024;
+
+template <int NT>
+__device__ __forceinline__ void block_scan2_nt(int64_t a, int64_t b, int64_t &ea, int64_t &eb, int64_t &ta,
+                                               int64_t &tb) {
+    __shared__ int64_t sa[NT / 64], sb[NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t ia = a, ib = b;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t ya = __shfl_up(ia, o), yb = __shfl_up(ib, o);
+        if (lane >= o) {
+            ia += ya;
+            ib += yb;
+        }
+    }
+    if (lane == 63) {
+        sa[wave] = ia;
+        sb[wave] = ib;
+    }
+    __syncthreads();
+    int64_t wa = 0, wb = 0;
+    ta = tb = 0;
+    for (int w = 0; w < NT / 64; ++w) {
+        if (w < wave) {
+            wa += sa[w];
+            wb += sb[w];
+        }
+        ta += sa[w];
+        tb += sb[w];
+    }
+    ea = wa + ia - a;
+    eb = wb + ib - b;
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kScanChunk) k_scan_chunk_sums(const int64_t *bsum, int64_t nb, int64_t *part) {
+    const int64_t i = (int64_t) blockIdx.x * kScanChunk + threadIdx.x;
+    const int64_t a = i < nb ? bsum[2 * i] : 0, b = i < nb ? bsum[2 * i + 1] : 0;
+    int64_t ea, eb, ta, tb;
+    block_scan2_nt<kScanChunk>(a, b, ea, eb, ta, tb);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ta;
+        part[2 * blockIdx.x + 1] = tb;
+    }
+}
+
+// Exclusive scan of np chunk sums in one workgroup (loop with a carry); totals to tot.
+__global__ void __launch_bounds__(kScanChunk) k_scan_chunk_tops(int64_t *part, int64_t np, int64_t *tot) {
+    __shared__ int64_t carry[2];
+    if (threadIdx.x == 0) carry[0] = carry[1] = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < np; base += kScanChunk) {
+        const int64_t i = base + threadIdx.x;
+        const int64_t a = i < np ? part[2 * i] : 0, b = i < np ? part[2 * i + 1] : 0;
+        int64_t ea, eb, ta, tb;
+        block_scan2_nt<kScanChunk>(a, b, ea, eb, ta, tb);
+        const int64_t ca = carry[0], cb = carry[1];
+        if (i < np) {
+            part[2 * i] = ca + ea;
+            part[2 * i + 1] = cb + eb;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            carry[0] = ca + ta;
+            carry[1] = cb + tb;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tot[0] = carry[0];
+        tot[1] = carry[1];
+    }
+}
+
+__global__ void __launch_bounds__(kScanChunk) k_scan_chunk_down(int64_t *bsum, int64_t nb, const int64_t *part) {
+    const int64_t i = (int64_t) blockIdx.x * kScanChunk + threadIdx.x;
+    const int64_t a = i < nb ? bsum[2 * i] : 0, b = i < nb ? bsum[2 * i + 1] : 0;
+    int64_t ea, eb, ta, tb;
+    block_scan2_nt<kScanChunk>(a, b, ea, eb, ta, tb);
+    if (i < nb) {
+        bsum[2 * i] = part[2 * blockIdx.x] + ea;
+        bsum[2 * i + 1] = part[2 * blockIdx.x + 1] + eb;
+    }
 }
 
 // exclusive scan of block sums (pairs), single block, in place; totals out
@@ -937,24 +1059,47 @@ hipError_t fill_i64(int64_t *p, int64_t n, int64_t v, hipStream_t st) {
 
 // Launches the DC kernels (no synchronisation).  bt.items == nullptr: one
 // grid (s); otherwise the batch's grids.
+// Exclusive scan of nb block-sum pairs in place, totals to tot; part holds
+// dc_scan_part_pairs(nb) pairs.
+static void scan_pairs(int64_t *bsum, int64_t nb, int64_t *tot, int64_t *part, hipStream_t st) {
+    if (nb <= 4 * kScanChunk || !part) {
+        hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, bsum, nb, tot);
+        return;
+    }
+    const int64_t np = (nb + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_scan_chunk_sums, dim3((unsigned) np), dim3(kScanChunk), 0, st, (const int64_t *) bsum, nb,
+                       part);
+    hipLaunchKernelGGL(k_scan_chunk_tops, dim3(1), dim3(kScanChunk), 0, st, part, np, tot);
+    hipLaunchKernelGGL(k_scan_chunk_down, dim3((unsigned) np), dim3(kScanChunk), 0, st, bsum, nb,
+                       (const int64_t *) part);
+}
+
 static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float svd_tol, float cond_fp64,
                             DcScratch &w, const DcBatch &bt, hipStream_t st) {
+    // Dense grids with lookup tables (the intersection set is the grid's
+    // own): no active-cell hash, used[] comes from the QEF pass, and the weld
+    // runs before the quads, which then write vertex ids directly.
+    // Otherwise: hash, quads (marking used[]), weld, face remap.
+    const bool tables = bt.items ? bt.tables : (in.amask_tab && in.apre_tab && !s.ids);
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, (int64_t *) w.n_fp64, (int64_t) 1, (int64_t) 0);
     hipLaunchKernelGGL(k_dc_qef, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in, s, reg, svd_tol, cond_fp64,
-                       w.dual_v, w.cond, w.n_fp64, w.used, w.bnd, bt);
-    const bool tables = bt.items ? bt.tables : (in.amask_tab && in.apre_tab && !s.ids);
+                       w.dual_v, w.cond, w.n_fp64, w.used, w.bnd, tables, bt);
+    const ActHash ah{w.akey, w.aval, w.amask};
+    const unsigned nbq = nblocks(in.npts), nbv = nblocks(in.ncells);
+    auto quads = [&](const WeldIds &wd) {
+        hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, bt);
+        scan_pairs(w.bsum, (int64_t) nbq, w.tot, w.part, st);
+        hipLaunchKernelGGL(k_quad_emit, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, w.dual_v, w.tri_cells,
+                           w.used, wd, bt);
+        if (bt.items) hipLaunchKernelGGL(k_item_qbase, dim3(1), dim3(64), 0, st, bt, in.ncells, (const int64_t *) w.tot);
+    };
     if (!tables) {   // active-cell hash for the quads' neighbour lookups
         const unsigned nbh = (unsigned) std::min<int64_t>(nblocks(w.amask + 1), 4096);
         hipLaunchKernelGGL(k_fill_i64, dim3(nbh), dim3(kBlock), 0, st, w.akey, w.amask + 1, (int64_t) -1);
         hipLaunchKernelGGL(k_act_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in.cell_dense, in.ncells,
                            w.akey, w.aval, w.amask, bt);
+        quads(WeldIds{});
     }
-    const ActHash ah{w.akey, w.aval, w.amask};
-    unsigned nbq = nblocks(in.npts);
-    hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, bt);
-    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbq, w.tot);
-    hipLaunchKernelGGL(k_quad_emit, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, w.dual_v, w.tri_cells,
-                       w.used, bt);
     // weld dual vertices by position
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.hslots, w.hmask + 1, -1);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.slot_min, w.hmask + 1,
@@ -963,19 +1108,19 @@ static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float 
                        in.ncells, w.hslots, w.hmask, w.rep_slot, (const int32_t *) bt.cell_item);
     hipLaunchKernelGGL(k_slot_min, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, w.bnd, in.ncells,
                        w.rep_slot, w.slot_min);
-    unsigned nbv = nblocks(in.ncells);
     hipLaunchKernelGGL(k_keep_flags, dim3(nbv), dim3(kBlock), 0, st, w.used, w.bnd, in.ncells, w.rep_slot,
                        w.slot_min, w.bsum);
-    hipLaunchKernelGGL(k_scan_pairs, dim3(1), dim3(1024), 0, st, w.bsum, (int64_t) nbv, w.tot + 2);
-    if (bt.items) {
+    scan_pairs(w.bsum, (int64_t) nbv, w.tot + 2, w.part, st);
+    if (bt.items)
         hipLaunchKernelGGL(k_item_vbase, dim3(bt.n), dim3(kBlock), 0, st, bt, w.used, w.bnd, w.rep_slot, w.slot_min,
                            w.bsum, in.ncells, (const int64_t *) w.tot);
-        hipLaunchKernelGGL(k_item_qbase, dim3(1), dim3(64), 0, st, bt, in.ncells, (const int64_t *) w.tot);
-    }
     hipLaunchKernelGGL(k_keep_emit, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, w.bnd, in.ncells,
                        w.rep_slot, w.slot_min, w.bsum, w.final_id, w.out_v, w.out_vcell, bt);
-    hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * in.npts)), dim3(kBlock), 0, st, w.tri_cells, w.tot, w.bnd,
-                       w.rep_slot, w.slot_min, w.final_id, w.out_f);
+    if (tables)
+        quads(WeldIds{w.bnd, w.rep_slot, w.slot_min, w.final_id, w.out_f});
+    else
+        hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * in.npts)), dim3(kBlock), 0, st, w.tri_cells, w.tot,
+                           w.bnd, w.rep_slot, w.slot_min, w.final_id, w.out_f);
     return hipGetLastError();
 }
 

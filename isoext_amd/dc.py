@@ -118,7 +118,7 @@ class Intersection:
     def __len__(self):
         return self.points.shape[0]
 
-    def _struct(self, normals=None) -> IsoIts:
+    def _struct(self, normals=None, grid=None) -> IsoIts:
         s = IsoIts()
         s.ncells = self.cell_indices.numel()
         s.npts = self.points.shape[0]
@@ -131,7 +131,9 @@ class Intersection:
         s.edge_local = self.edge_local.data_ptr()
         s.is_out = self.is_out.data_ptr()
         s.entry_cell = self.entry_cell.data_ptr()
-        if self._tables is not None:
+        # the tables index cell rows of the grid shape they were built for
+        same_shape = grid is None or [int(x) for x in grid.shape] == [int(x) for x in self._grid.shape]
+        if self._tables is not None and same_shape and getattr(grid, "cell_ids", None) is None:
             s.amask_tab = self._tables[0].data_ptr()
             s.apre_tab = self._tables[1].data_ptr()
         return s
@@ -202,7 +204,7 @@ def dc_raw(grid, its: Intersection, normals: torch.Tensor, reg: float, svd_tol: 
     dev = _device(grid)
     L = _lib.lib()
     g = grid_struct(grid)
-    s = its._struct(normals=normals)
+    s = its._struct(normals=normals, grid=grid)
     nc, npt = int(s.ncells), int(s.npts)
     ws = workspace.get("dc", L.isoext_dc_workspace_bytes(nc, npt), dev)
     out_v = torch.empty((max(nc, 1), 3), dtype=torch.float32, device=dev)
