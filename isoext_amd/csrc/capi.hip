@@ -294,16 +294,11 @@ constexpr int kBatchSeg = iso::kMaxSpSegBatch;   // special-list segments per ba
 struct ItemLayout {
     bool cells;
     int cfg, nzc;   // the item's emit tile configuration
-    int64_t W, S, NI, NJ, NK, nblk, nwords, ntiles, tile0, seg_cap[4];
+    int64_t W, S, NI, NJ, NK, nblk, nctask, ntiles, tile0, seg_cap[4];
     size_t o_hdr, o_bits, o_cnt_v, o_cnt_f, o_bsum_v, o_bsum_f, o_blk_v, o_blk_f;
     size_t o_ctr, o_scand, o_sscell, o_skept, o_shole, o_cand, o_scell, o_kept, o_hole;
 };
 
-// Words an item occupies in the batch word space (padded so that every item
-// starts at a multiple of the classify task size).
-inline int64_t word_span(int64_t nwords) {
-    return (nwords + iso::kClassifyWords - 1) / iso::kClassifyWords * iso::kClassifyWords;
-}
 
 struct BatchLayout {
     size_t o_hdr, o_items, o_order, o_summary, o_fixcnt, total;
@@ -346,9 +341,9 @@ bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_c
         L.NJ = L.cells ? (Y - 1 + ty - 1) / ty : 0;
         L.NK = L.cells ? ((Z - 1 + 63) / 64 + nzc - 1) / nzc : 0;
         L.nblk = L.cells ? (X * Y * L.S + iso::kMcScanBlock - 1) / iso::kMcScanBlock : 0;
-        L.nwords = L.cells ? X * Y * L.W : 0;
+        L.nctask = L.cells ? (X * Y * Z + iso::kFlatPts - 1) / iso::kFlatPts : 0;
         L.ntiles = L.NI * L.NJ * L.NK;
-        B.h.nwords += word_span(L.nwords);
+        B.h.nctask += L.nctask;
         B.h.nblk += L.nblk;
         B.h.ntiles += L.ntiles;
         const int64_t R = L.cells ? X * Y : 0;
@@ -430,7 +425,7 @@ hipEvent_t *timing_events() {
     return ev;
 }
 
-void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t word0, int64_t blk0,
+void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t ctask0, int64_t blk0,
                int64_t tile0, iso::McBatchItem &out) {
     iso::McParams &p = out.p;
     p = iso::McParams{};
@@ -487,7 +482,7 @@ void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t 
     sp.scell = (iso::SpecialEdge *) (ws + L.o_scell);
     sp.kept = (int64_t *) (ws + L.o_kept);
     sp.hole = (int64_t *) (ws + L.o_hole);
-    out.word0 = word0;
+    out.ctask0 = ctask0;
     out.blk0 = blk0;
     out.tile0 = tile0;
     out.mode = p.vec4 ? (p.Z % 64 == 0 ? 0 : 1) : 2;
@@ -518,7 +513,7 @@ int batch_upload(int n, const isoext_mc_item *items, char *ws, const BatchLayout
             p.cap_v = its_caps[2 * i];
             p.cap_f = its_caps[2 * i + 1];
         }
-        w0 += word_span(B.it[i].nwords);
+        w0 += B.it[i].nctask;
         b0 += B.it[i].nblk;
     }
     int32_t *order = (int32_t *) (it + n);

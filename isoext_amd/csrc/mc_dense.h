@@ -63,10 +63,9 @@ inline int64_t sp_seg_cap(int64_t special_cap, int k, int nseg) {
 // Entries of (point row, z-segment) counted and scanned per workgroup.
 constexpr int kMcScanBlock = 1024;
 
-// 64-point words classified per wave task by the lane-per-point classify;
-// batch items start their words at multiples of it, so that no task spans
-// two grids.
-constexpr int kClassifyWords = 16;
+// Flat classify: 64-point chunks per wave task (plus one read past the end).
+constexpr int kFlatChunks = 32;
+constexpr int64_t kFlatPts = 64 * kFlatChunks;
 
 struct McParams {
     const float *values;
@@ -111,14 +110,14 @@ struct McTimes {
 struct McBatchItem {
     McParams p;
     SpecialLists sp_init;   // contiguous special lists (counters zeroed by reset)
-    int64_t word0;          // first classify word
+    int64_t ctask0;         // first flat classify task
     int64_t blk0;           // first count block
     int64_t tile0;          // first emit tile
     int mode;               // classify loads: 0 Z%64 == 0 float4, 1 float4, 2 scalar
 };
 
 struct McBatchHdr {
-    int64_t n, nwords, nblk, ntiles;
+    int64_t n, nctask, nblk, ntiles;   // items, flat classify tasks, count blocks, emit tiles
 };
 
 // Per-item result of a batch phase, gathered for one read-back.

@@ -250,44 +250,59 @@ __global__ void __launch_bounds__(256) k_classify_v4(const float *__restrict__ v
 }
 
 
-// Lane-per-point classify of kClassifyWords consecutive words [w0, w0 + 16)
-// of one grid (w0 wave-uniform): lane l loads value z = 64c + l of word
-// (row, c), one 256-byte coalesced load per word whatever the row alignment,
-// and the wave's compare mask is the word.  Lane u stores word u.
-__device__ __forceinline__ void classify_task(const float *__restrict__ values, int64_t Z, int64_t W,
-                                              int64_t nwords, float level, int64_t w0, uint64_t *__restrict__ bits) {
-    constexpr int U = kClassifyWords;
+// Flat classify (any row length): task t covers the grid's flat points
+// [t * kFlatPts, (t + 1) * kFlatPts).  A wave loads kFlatChunks + 1
+// consecutive 64-point chunks (coalesced 256-byte loads, no idle lanes
+// whatever Z is), their compare ballots go to LDS, and each lane assembles
+// one (row, word) mask of a word that starts inside the task: 64 bits of the
+// flat bit stream at the word's start (a funnel shift of two ballots), cut
+// at the row's end.  The extra chunk covers words that run past the task.
+__device__ __forceinline__ int64_t words_before(int64_t f, int64_t Z, int64_t W) {
+    const int64_t r = div_fast(f, Z);
+    return r * W + min<int64_t>(W, (f - r * Z + 63) >> 6);
+}
+
+__device__ __forceinline__ void classify_flat_task(const float *__restrict__ values, int64_t N, int64_t Z,
+                                                   int64_t W, float level, int64_t t, uint64_t *__restrict__ bits,
+                                                   uint64_t *ballots) {
+    constexpr int K = kFlatChunks + 1;
     const int lane = threadIdx.x & 63;
-    int64_t row = div_fast(w0, W);
-    int64_t c = w0 - row * W;
-    float v[U];
-    bool ok[U];
+    const int64_t f0 = t * kFlatPts;
+    float v[K];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const int64_t z = 64 * c + lane;
-        ok[u] = w0 + u < nwords && z < Z;
-        // lanes past the row read a valid dummy
-        v[u] = __builtin_nontemporal_load(values + (ok[u] ? row * Z + z : 0));
-        if (++c == W) {
-            c = 0;
-            ++row;
-        }
+    for (int k = 0; k < K; ++k) {
+        const int64_t f = f0 + 64 * k + lane;
+        v[k] = __builtin_nontemporal_load(values + (f < N ? f : 0));   // past the end: a valid dummy
     }
     uint64_t mine = 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const uint64_t m = __ballot(ok[u] && inside(v[u], level));
-        if (lane == u) mine = m;
+    for (int k = 0; k < K; ++k) {
+        const uint64_t m = __ballot(f0 + 64 * k + lane < N && inside(v[k], level));
+        if (lane == k) mine = m;
     }
-    if (lane < U && w0 + lane < nwords) bits[w0 + lane] = mine;
+    if (lane < K) ballots[lane] = mine;
+    __builtin_amdgcn_wave_barrier();
+    const int64_t g0 = words_before(f0, Z, W), g1 = words_before(min(f0 + kFlatPts, N), Z, W);
+    for (int64_t g = g0 + lane; g < g1; g += 64) {
+        const int64_t row = div_fast(g, W), c = g - row * W;
+        const int o = (int) (row * Z + 64 * c - f0), k = o >> 6, sh = o & 63;
+        uint64_t w = ballots[k] >> sh;
+        if (sh) w |= ballots[k + 1] << (64 - sh);
+        const int64_t len = Z - 64 * c;
+        if (len < 64) w &= (1ull << len) - 1ull;
+        bits[g] = w;
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
-__global__ void __launch_bounds__(256) k_classify(const float *__restrict__ values, int64_t Z, int64_t W,
-                                                  int64_t nwords, float level, uint64_t *__restrict__ bits) {
+__global__ void __launch_bounds__(256) k_classify(const float *__restrict__ values, int64_t N, int64_t Z, int64_t W,
+                                                  float level, uint64_t *__restrict__ bits) {
+    __shared__ uint64_t ballots[4][kFlatChunks + 1];
     const int64_t wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x * 4 + (threadIdx.x >> 6)));
     const int64_t nwave = (int64_t) gridDim.x * 4;
-    const int64_t ntask = (nwords + kClassifyWords - 1) / kClassifyWords;
-    for (int64_t t = wave; t < ntask; t += nwave) classify_task(values, Z, W, nwords, level, t * kClassifyWords, bits);
+    const int64_t ntask = (N + kFlatPts - 1) / kFlatPts;
+    for (int64_t t = wave; t < ntask; t += nwave)
+        classify_flat_task(values, N, Z, W, level, t, bits, ballots[threadIdx.x >> 6]);
 }
 
 // ---- count + block scan ----------------------------------------------------------
@@ -1093,24 +1108,22 @@ __global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, c
     }
 }
 
-// Tasks of kClassifyWords words over the batch word space; item starts are
-// multiples of kClassifyWords, so each task lies in one grid.
+// Flat classify tasks over the batch (each grid owns a range of whole tasks).
 __global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *__restrict__ items,
                                                         const McBatchHdr *__restrict__ hdr) {
     __shared__ int64_t starts[kMaxBatch];
+    __shared__ uint64_t ballots[4][kFlatChunks + 1];
     const int n = (int) hdr->n;
-    const int64_t ntask = hdr->nwords / kClassifyWords;
-    for (int i = threadIdx.x; i < n; i += 256) starts[i] = items[i].word0;
+    const int64_t ntask = hdr->nctask;
+    for (int i = threadIdx.x; i < n; i += 256) starts[i] = items[i].ctask0;
     __syncthreads();
     const int64_t wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x * 4 + (threadIdx.x >> 6)));
     const int64_t nwave = (int64_t) gridDim.x * 4;
     for (int64_t t = wave; t < ntask; t += nwave) {
-        const int64_t gw = t * kClassifyWords;
-        const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, gw));
+        const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, t));
         const McParams &p = items[i].p;
-        const int64_t w0 = gw - items[i].word0;
-        const int64_t nw = p.X * p.Y * p.W;
-        if (w0 < nw) classify_task(p.values, p.Z, p.W, nw, p.level, w0, p.bits);
+        classify_flat_task(p.values, p.X * p.Y * p.Z, p.Z, p.W, p.level, t - items[i].ctask0, p.bits,
+                           ballots[threadIdx.x >> 6]);
     }
 }
 
@@ -1454,17 +1467,16 @@ hipError_t mc_upload_luts() {
 }
 
 // Inside masks of a whole grid: dwordx4 loads when rows are whole aligned
-// words, the lane-per-point kernel otherwise.
+// words, the flat kernel otherwise.
 static void launch_classify(const McParams &p, hipStream_t stream) {
     const int64_t nwords = p.X * p.Y * p.W;
     if (p.vec4 && p.Z % 64 == 0) {
         const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nwords + 63) / 64, 8192));
         hipLaunchKernelGGL(k_classify_v4, dim3(blocks), dim3(256), 0, stream, p.values, nwords, p.level, p.bits);
     } else {
-        const int64_t ntask = (nwords + kClassifyWords - 1) / kClassifyWords;
+        const int64_t N = p.X * p.Y * p.Z, ntask = (N + kFlatPts - 1) / kFlatPts;
         const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((ntask + 3) / 4, 8192));
-        hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, stream, p.values, p.Z, p.W, nwords, p.level,
-                           p.bits);
+        hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, stream, p.values, N, p.Z, p.W, p.level, p.bits);
     }
 }
 
@@ -1543,7 +1555,7 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     const unsigned rb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.n * 4 * kSpSeg + 255) / 256, 1024));
     hipLaunchKernelGGL(k_reset_batch, dim3(rb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const unsigned cb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.nwords / kClassifyWords + 3) / 4, 8192));
+    const unsigned cb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.nctask + 3) / 4, 8192));
     hipLaunchKernelGGL(k_classify_batch, dim3(cb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const unsigned nb = (unsigned) std::max<int64_t>(1, h.nblk);
