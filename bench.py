@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--method", default="nagae")
     ap.add_argument("--cfg", type=int, default=None, help="tile configuration (default: autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fused", action="store_true", help="skip the fused sphere-field extraction")
     ap.add_argument("--cpu-planes", type=int, default=129, help="x-planes in the CPU sample")
     ap.add_argument("--serving", type=int, default=1000,
                     help="requests in the serving comparison (0 = skip)")
@@ -395,6 +396,47 @@ def serving_bench(n_requests: int, seed: int = 0):
     }
 
 
+def fused_bench(args, shape, m, v_grid, f_grid, grid_ms_step):
+    """The same extraction with the sphere-union field evaluated inside the
+    kernels (isoext_mc_spheres): time per extraction, whether the mesh is
+    identical to the grid path's, and the time the generator takes to write
+    the grid that the grid path reads."""
+    import torch
+
+    from isoext_amd.fused import sphere_tensor
+    from isoext_amd.mc import mc_dense_raw
+    from isoext_amd.workloads import sphere_params, spheres_field
+    c, r = sphere_params(args.spheres, args.seed)
+    sp = sphere_tensor(c, r)
+    amin, amax = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+    for _ in range(2):
+        v, f, _ = mc_dense_raw(None, shape, amin, amax, 0.0, m, args.cfg, spheres=sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps = max(3, args.steps // 2)
+    for _ in range(steps):
+        v, f, _ = mc_dense_raw(None, shape, amin, amax, 0.0, m, args.cfg, spheres=sp)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    same = bool(v.shape == v_grid.shape and f.shape == f_grid.shape and torch.equal(v, v_grid)
+                and torch.equal(f, f_grid))
+    del v, f
+    buf = torch.empty(shape, dtype=torch.float32, device="cuda")
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    spheres_field(shape, c, r, out=buf)
+    b.record()
+    torch.cuda.synchronize()
+    gen_ms = a.elapsed_time(b)
+    del buf
+    npts = shape[0] * shape[1] * shape[2]
+    return {"workload": f"marching cubes of the {args.spheres}-sphere union evaluated in the kernels "
+                        f"(isoext_mc_spheres), no grid stored",
+            "ms_per_extraction": ms, "gvoxels_s": npts / (ms * 1e-3) / 1e9,
+            "identical_to_grid_path": same, "grid_path_ms_per_step": grid_ms_step,
+            "grid_generation_ms": gen_ms}
+
+
 def cpu_baseline(values_gpu, planes: int, level: float, method: str):
     """The CPU oracle (C++ restatement of the reference, OpenMP) on the first
     `planes` x-planes of the same field; returns Gvoxels/s and details."""
@@ -537,6 +579,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if not args.no_fused:
+        out["config"]["fused_spheres"] = fused_bench(args, shape, m, v, f, ms_step)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(grid.values, min(args.cpu_planes, N), 0.0, args.method)

@@ -89,6 +89,24 @@ int isoext_mc_dense(const float *values, const int64_t shape[3], const float aab
                     float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f,
                     int64_t counts[6], void *stream);
 
+/* Marching cubes of a field that is never stored (no reference counterpart;
+ * the step before the path: the reference's users build the grid with
+ * UniformGrid.set_values(sdf(points))).  The field is the union of nsph
+ * spheres (1..256), v(p) = min_k sqrt((dx*dx + dy*dy) + dz*dz) - r_k in fp32
+ * at the grid points of (shape, aabb); `spheres` is a device array of nsph
+ * float4 (cx, cy, cz, r), 16-byte aligned.  The classify pass evaluates only
+ * the spheres that can reach below `level` in each block of points and
+ * writes inside bits; the emit pass evaluates the corners of active cells
+ * over the spheres that can be the minimum in its tile.  The result is
+ * identical to isoext_mc_dense on the grid such a generator writes
+ * (isoext_sdf_spheres), without writing or reading that grid.  Same
+ * workspace (isoext_mc_dense_workspace_bytes), outputs and counts as
+ * isoext_mc_dense.  Synchronises `stream`. */
+int isoext_mc_spheres(const float *spheres, int nsph, const int64_t shape[3], const float aabb_min[3],
+                      const float aabb_max[3], float level, int method, int cfg, void *workspace,
+                      int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v, int32_t *out_f,
+                      int64_t cap_f, int64_t counts[6], void *stream);
+
 /* ---- grid descriptor used by intersections, dual contouring, sparse MC ---- */
 
 /* A UniformGrid (sparse_cell_ids == NULL: values is (X, Y, Z)) or a

@@ -56,6 +56,9 @@ def _declare(L):
     L.isoext_mc_dense.restype = c_int
     L.isoext_mc_dense.argtypes = [vp, I64x3, F32x3, F32x3, c_f, c_int, c_int, vp, c_i64, c_i64,
                                   vp, c_i64, vp, c_i64, I64x6, vp]
+    L.isoext_mc_spheres.restype = c_int
+    L.isoext_mc_spheres.argtypes = [vp, c_int, I64x3, F32x3, F32x3, c_f, c_int, c_int, vp, c_i64, c_i64,
+                                    vp, c_i64, vp, c_i64, I64x6, vp]
     L.isoext_mc_dense_keyed.restype = c_int
     L.isoext_mc_dense_keyed.argtypes = [vp, I64x3, F32x3, F32x3, c_f, c_int, c_int, vp, c_i64, c_i64,
                                         vp, c_i64, vp, c_i64, vp, I64x6, vp]

@@ -758,7 +758,17 @@ int64_t isoext_mc_dense_workspace_bytes(const int64_t shape[3], int cfg, int64_t
 static int mc_dense_impl(const float *values, const int64_t shape[3], const float aabb_min[3],
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
-                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream);
+                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream,
+                         const float *spheres = nullptr, int nsph = 0);
+
+int isoext_mc_spheres(const float *spheres, int nsph, const int64_t shape[3], const float aabb_min[3],
+                      const float aabb_max[3], float level, int method, int cfg, void *workspace,
+                      int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v, int32_t *out_f,
+                      int64_t cap_f, int64_t counts[6], void *stream) {
+    if (!spheres) return fail(ISOEXT_E_ARG, "spheres is null");
+    return mc_dense_impl(nullptr, shape, aabb_min, aabb_max, level, method, cfg, workspace, workspace_bytes,
+                         special_cap, out_v, cap_v, out_f, cap_f, nullptr, counts, stream, spheres, nsph);
+}
 
 int isoext_mc_dense(const float *values, const int64_t shape[3], const float aabb_min[3],
                     const float aabb_max[3], float level, int method, int cfg, void *workspace,
@@ -779,7 +789,8 @@ int isoext_mc_dense_keyed(const float *values, const int64_t shape[3], const flo
 static int mc_dense_impl(const float *values, const int64_t shape[3], const float aabb_min[3],
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
-                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream) {
+                         int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream,
+                         const float *spheres, int nsph) {
     for (int i = 0; i < 6; ++i) counts[i] = 0;
     if (method != 0 && method != 1) return fail(ISOEXT_E_ARG, "Unknown method: %d", method);
     for (int a = 0; a < 3; ++a)
@@ -790,7 +801,10 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     if ((int64_t) L.total > workspace_bytes)
         return fail(ISOEXT_E_WORKSPACE, "workspace too small: %lld < %lld bytes",
                     (long long) workspace_bytes, (long long) L.total);
-    if (!values) return fail(ISOEXT_E_ARG, "values is null");
+    if (!values && !spheres) return fail(ISOEXT_E_ARG, "values is null");
+    if (spheres && (nsph < 1 || nsph > iso::kMaxFusedSpheres))
+        return fail(ISOEXT_E_ARG, "sphere count %d outside [1, %d]", nsph, iso::kMaxFusedSpheres);
+    if (spheres && ((uintptr_t) spheres & 15)) return fail(ISOEXT_E_ARG, "spheres must be 16-byte aligned");
     hipStream_t st = (hipStream_t) stream;
     char *ws = (char *) workspace;
 
@@ -848,6 +862,12 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     p.seg.scell = (iso::SpecialEdge *) (ws + L.o_sscell);
     p.seg.kept = (int64_t *) (ws + L.o_skept);
     p.seg.hole = (int64_t *) (ws + L.o_shole);
+    if (spheres) {
+        p.values = nullptr;
+        p.vec4 = 0;
+        p.spheres = (const float4 *) spheres;
+        p.nsph = nsph;
+    }
     hipEvent_t *ev = g_timing ? timing_events() : nullptr;
     if (g_timing && !ev) return fail(ISOEXT_E_DEVICE, "timing events unavailable");
     HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, ev), "launch marching cubes");

@@ -97,7 +97,13 @@ struct McParams {
     int64_t *totals;    // [0] vertices, [1] faces (provisional)
     SpecialLists *sp;   // contiguous lists (written by the gather pass)
     SpecialSegs seg;    // segmented lists (written by emit)
+    // fused field (values unused when nsph > 0): v(p) = min_k |p - c_k| - r_k
+    // over nsph <= kMaxFusedSpheres spheres (cx, cy, cz, r), evaluated where
+    // needed with the sphere-union generator's formula
+    const float4 *spheres;
+    int nsph;
 };
+constexpr int kMaxFusedSpheres = 256;
 
 // Stage timings (ms) of the last timed launch: classify, count, scan, emit,
 // and the whole extraction (first to last kernel).

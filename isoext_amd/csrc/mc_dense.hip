@@ -539,6 +539,154 @@ __device__ __forceinline__ void scan_top(const McParams &p) {
 
 __global__ void __launch_bounds__(1024) k_scan_top(McParams p) { scan_top(p); }
 
+// ---- fused sphere-union field ------------------------------------------------------
+// v(p) = min_k (sqrt((dx*dx + dy*dy) + dz*dz) - r_k): the sphere-union
+// generator's formula and op order (sdf.hip), evaluated where the pipeline
+// needs it instead of read from a grid.  Classify needs only sign bits
+// (v - level < 0 iff some sphere has d_k - level < 0, since inside() is
+// monotone in v), emit the 8 corners of its active cells.  Spheres that
+// cannot matter inside a box of grid points are culled with bounds computed
+// in FP64 from the box's grid coordinates and a margin (1e-5 of the
+// distances involved) far above the FP32 evaluation error, so every value
+// evaluated is bit-identical to the generated grid's (fminf is exact and
+// order-independent).
+
+__device__ __forceinline__ float sphere_d(float px, float py, float pz, float4 sp) {
+    const float dx = px - sp.x, dy = py - sp.y, dz = pz - sp.z;
+    float d2 = dx * dx + dy * dy;
+    d2 = d2 + dz * dz;
+    return sqrtf(d2) - sp.w;
+}
+
+// World box of grid points [i0, i1] per axis (grid_coord is monotone).
+struct Box3 {
+    double lo[3], hi[3];
+};
+__device__ __forceinline__ Box3 point_box(const McParams &p, int64_t x0, int64_t x1, int64_t y0, int64_t y1,
+                                          int64_t z0, int64_t z1) {
+    const int64_t a0[3] = {x0, y0, z0}, a1[3] = {x1, y1, z1}, n[3] = {p.X, p.Y, p.Z};
+    Box3 b;
+    for (int a = 0; a < 3; ++a) {
+        const double u = grid_coord(a0[a], n[a], p.amin[a], p.amax[a]);
+        const double v = grid_coord(a1[a], n[a], p.amin[a], p.amax[a]);
+        b.lo[a] = fmin(u, v);
+        b.hi[a] = fmax(u, v);
+    }
+    return b;
+}
+
+// Lower / upper bound of |p - c| - r over the box, and the culling margin.
+__device__ __forceinline__ void sphere_bounds(const Box3 &b, float4 sp, double &lb, double &ub, double &m) {
+    const double c[3] = {sp.x, sp.y, sp.z};
+    double dn = 0.0, dx = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double g = fmax(fmax(b.lo[a] - c[a], c[a] - b.hi[a]), 0.0);
+        const double f = fmax(fabs(c[a] - b.lo[a]), fabs(c[a] - b.hi[a]));
+        dn += g * g;
+        dx += f * f;
+    }
+    dx = sqrt(dx);
+    lb = sqrt(dn) - (double) sp.w;
+    ub = dx - (double) sp.w;
+    m = 1e-5 * (dx + fabs((double) sp.w)) + 1e-30;
+}
+
+// Inside masks of the fused field: a wave task is a 4 x 4 block of point
+// rows times one 64-point word; the spheres that can reach below the level
+// inside the block's box are found once per task (one per lane, ballots),
+// then each row's word is the ballot of "inside some of them".
+__global__ void __launch_bounds__(256) k_classify_spheres(McParams p) {
+    __shared__ float4 sph[kMaxFusedSpheres];
+    for (int i = threadIdx.x; i < p.nsph; i += 256) sph[i] = p.spheres[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int64_t nwave = (int64_t) gridDim.x * 4;
+    const int64_t nbx = (p.X + 3) / 4, nby = (p.Y + 3) / 4, ntask = nbx * nby * p.W;
+    constexpr int NCH = kMaxFusedSpheres / 64;
+    for (int64_t t = wave; t < ntask; t += nwave) {
+        const int64_t c = t % p.W, r = t / p.W, yb = r % nby, xb = r / nby;
+        const int64_t xa = 4 * xb, ya = 4 * yb, za = 64 * c;
+        const Box3 box = point_box(p, xa, min(xa + 3, p.X - 1), ya, min(ya + 3, p.Y - 1), za, min(za + 63, p.Z - 1));
+        uint64_t rel[NCH];
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const int k = 64 * ch + lane;
+            bool keep = false;
+            if (k < p.nsph) {
+                double lb, ub, m;
+                sphere_bounds(box, sph[k], lb, ub, m);
+                keep = lb - m < (double) p.level;
+            }
+            rel[ch] = __ballot(keep);
+        }
+        const int64_t z = za + lane;
+        const bool zok = z < p.Z;
+        const float pz = grid_coord(zok ? z : p.Z - 1, p.Z, p.amin[2], p.amax[2]);
+        uint64_t mine = 0;
+        for (int i = 0; i < 16; ++i) {
+            const int64_t x = xa + (i >> 2), y = ya + (i & 3);
+            if (x >= p.X || y >= p.Y) continue;
+            const float px = grid_coord(x, p.X, p.amin[0], p.amax[0]);
+            const float py = grid_coord(y, p.Y, p.amin[1], p.amax[1]);
+            bool in = false;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                uint64_t m = rel[ch];
+                while (m && !in) {
+                    const int k = 64 * ch + __ffsll((unsigned long long) m) - 1;
+                    m &= m - 1;
+                    in = inside(sphere_d(px, py, pz, sph[k]), p.level);
+                }
+            }
+            const uint64_t word = __ballot(zok && in);
+            if (lane == i) mine = word;
+        }
+        if (lane < 16) {
+            const int64_t x = xa + (lane >> 2), y = ya + (lane & 3);
+            if (x < p.X && y < p.Y) p.bits[(x * p.Y + y) * p.W + c] = mine;
+        }
+    }
+}
+
+// LDS of the fused emit: the spheres that can be the minimum somewhere in
+// the tile's box.
+template <bool F>
+struct FusedLds {};
+template <>
+struct FusedLds<true> {
+    float4 sph[kMaxFusedSpheres];
+    int n;
+    double red[16];
+};
+
+// Keeps sphere k unless lb_k - m_k > min_j (ub_j + m_j) (then it is larger
+// than another sphere's value everywhere in the box).  Block-wide; all
+// threads call it.
+template <int NT>
+__device__ void cull_spheres(const McParams &p, const Box3 &box, FusedLds<true> &f) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) f.n = 0;
+    double best = INFINITY;
+    for (int k = tid; k < p.nsph; k += NT) {
+        double lb, ub, m;
+        sphere_bounds(box, p.spheres[k], lb, ub, m);
+        best = fmin(best, ub + m);
+    }
+    for (int o = 32; o > 0; o >>= 1) best = fmin(best, __shfl_xor(best, o));
+    if (lane == 0) f.red[wave] = best;
+    __syncthreads();
+    double tmin = INFINITY;
+    for (int w = 0; w < NT / 64; ++w) tmin = fmin(tmin, f.red[w]);
+    for (int k = tid; k < p.nsph; k += NT) {
+        double lb, ub, m;
+        const float4 sp = p.spheres[k];
+        sphere_bounds(box, sp, lb, ub, m);
+        if (lb - m <= tmin) f.sph[atomicAdd(&f.n, 1)] = sp;
+    }
+    __syncthreads();
+}
+
 // ---- emit ------------------------------------------------------------------------
 template <int TX, int TY, int NZC, int NT>
 struct EmitLds {
@@ -657,10 +805,11 @@ __device__ __forceinline__ bool edge_maybe_candidate(const CellCtx &cc, float ep
     return !(fabsf(cc.level - cc.v[ca]) > d && fabsf(cc.v[cb] - cc.level) > d);
 }
 
-template <int TX, int TY, int NZC, int NT>
+template <int TX, int TY, int NZC, int NT, bool FUSED = false>
 __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int spseg) {
     using L = EmitLds<TX, TY, NZC, NT>;
     __shared__ L s;
+    __shared__ FusedLds<FUSED> fl;
     const int tid = threadIdx.x;
     if (*(volatile int *) p.err & ISO_ERR_TOO_LARGE) return;
 
@@ -714,6 +863,10 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             s.cz[c - TX - TY - 2] = grid_coord(z0 + (c - TX - TY - 2), p.Z, p.amin[2], p.amax[2]);
         }
     }
+    if constexpr (FUSED)
+        cull_spheres<NT>(p, point_box(p, x0, min(x0 + TX, p.X - 1), y0, min(y0 + TY, p.Y - 1), z0,
+                                      min(z0 + 64 * NZC, p.Z - 1)),
+                         fl);
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
         const int i = tid + k * NT;
@@ -815,7 +968,17 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             const int zl = 64 * jc + k;
             const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + zl;
             CellCtx cc;
-            {
+            if constexpr (FUSED) {
+                const float xs[2] = {s.cx[cx], s.cx[cx + 1]}, ys[2] = {s.cy[cy], s.cy[cy + 1]};
+                const float zs[2] = {s.cz[zl], s.cz[zl + 1]};
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    float best = INFINITY;
+                    for (int j = 0; j < fl.n; ++j)
+                        best = fminf(best, sphere_d(xs[q >> 2], ys[(q >> 1) & 1], zs[q & 1], fl.sph[j]));
+                    cc.v[q] = best;
+                }
+            } else {
                 const float *r00 = p.values + (gx * p.Y + gy) * p.Z + gz;
                 const float *r01 = r00 + p.Z;
                 const float *r10 = r00 + p.Y * p.Z;
@@ -997,6 +1160,11 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
 template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_emit(McParams p) {
     emit_tile<TX, TY, NZC, NT>(p, blockIdx.x, (int) (blockIdx.x % p.seg.nseg));
+}
+
+template <int TX, int TY, int NZC, int NT>
+__global__ void __launch_bounds__(NT) k_emit_spheres(McParams p) {
+    emit_tile<TX, TY, NZC, NT, true>(p, blockIdx.x, (int) (blockIdx.x % p.seg.nseg));
 }
 
 __global__ void mc_dense_reset(McParams p, SpecialLists init) {
@@ -1470,7 +1638,11 @@ hipError_t mc_upload_luts() {
 // words, the flat kernel otherwise.
 static void launch_classify(const McParams &p, hipStream_t stream) {
     const int64_t nwords = p.X * p.Y * p.W;
-    if (p.vec4 && p.Z % 64 == 0) {
+    if (p.nsph > 0) {
+        const int64_t ntask = ((p.X + 3) / 4) * ((p.Y + 3) / 4) * p.W;
+        const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((ntask + 3) / 4, 8192));
+        hipLaunchKernelGGL(k_classify_spheres, dim3(blocks), dim3(256), 0, stream, p);
+    } else if (p.vec4 && p.Z % 64 == 0) {
         const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nwords + 63) / 64, 8192));
         hipLaunchKernelGGL(k_classify_v4, dim3(blocks), dim3(256), 0, stream, p.values, nwords, p.level, p.bits);
     } else {
@@ -1507,7 +1679,10 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     switch (cfg) {
 #define X(id, tx, ty, nzc, nt)                                                                     \
     case id:                                                                                       \
-        hipLaunchKernelGGL((k_emit<tx, ty, nzc, nt>), dim3(tiles), dim3(nt), 0, stream, p);        \
+        if (p.nsph > 0)                                                                            \
+            hipLaunchKernelGGL((k_emit_spheres<tx, ty, nzc, nt>), dim3(tiles), dim3(nt), 0, stream, p); \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_emit<tx, ty, nzc, nt>), dim3(tiles), dim3(nt), 0, stream, p);    \
         break;
         ISO_MC_CONFIGS(X)
 #undef X
