@@ -591,10 +591,34 @@ __device__ __forceinline__ void sphere_bounds(const Box3 &b, float4 sp, double &
     m = 1e-5 * (dx + fabs((double) sp.w)) + 1e-30;
 }
 
+// Classify-side bounds in FP32 without square roots: whether sphere sp can
+// reach below `level` somewhere in the box (keep) and whether it is below
+// it on the whole box (all_in).  Distances are compared squared; the margin
+// is 1e-5 of an L1 bound of the largest distance (>= the Euclidean one),
+// far above the rounding of the squared sums (a few ulps).
+__device__ __forceinline__ void sphere_sign_bounds(const float lo[3], const float hi[3], float4 sp, float level,
+                                                   bool &keep, bool &all_in) {
+    const float c[3] = {sp.x, sp.y, sp.z};
+    float dn = 0.0f, dx = 0.0f, l1 = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        const float g = fmaxf(fmaxf(lo[a] - c[a], c[a] - hi[a]), 0.0f);
+        const float f = fmaxf(fabsf(c[a] - lo[a]), fabsf(c[a] - hi[a]));
+        dn += g * g;
+        dx += f * f;
+        l1 += f;
+    }
+    const float m = 1e-5f * (l1 + fabsf(sp.w)) + 1e-30f;
+    const float tk = level + sp.w + m;   // keep iff min distance < tk
+    keep = tk > 0.0f && dn < tk * tk;
+    const float ta = level + sp.w - m;   // all inside iff max distance < ta
+    all_in = ta > 0.0f && dx < ta * ta;
+}
+
 // Inside masks of the fused field: a wave task is a 4 x 4 block of point
-// rows times one 64-point word; the spheres that can reach below the level
-// inside the block's box are found once per task (one per lane, ballots),
-// then each row's word is the ballot of "inside some of them".
+// rows times one 64-point word.  Once per task (one sphere per lane,
+// ballots): if some sphere is below the level on the whole box, every point
+// is inside; if none reaches below it anywhere, none is; otherwise each
+// row's word is the ballot of "inside one of the spheres that reach below".
 __global__ void __launch_bounds__(256) k_classify_spheres(McParams p) {
     __shared__ float4 sph[kMaxFusedSpheres];
     for (int i = threadIdx.x; i < p.nsph; i += 256) sph[i] = p.spheres[i];
@@ -607,28 +631,48 @@ __global__ void __launch_bounds__(256) k_classify_spheres(McParams p) {
     for (int64_t t = wave; t < ntask; t += nwave) {
         const int64_t c = t % p.W, r = t / p.W, yb = r % nby, xb = r / nby;
         const int64_t xa = 4 * xb, ya = 4 * yb, za = 64 * c;
-        const Box3 box = point_box(p, xa, min(xa + 3, p.X - 1), ya, min(ya + 3, p.Y - 1), za, min(za + 63, p.Z - 1));
-        uint64_t rel[NCH];
-#pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
-            const int k = 64 * ch + lane;
-            bool keep = false;
-            if (k < p.nsph) {
-                double lb, ub, m;
-                sphere_bounds(box, sph[k], lb, ub, m);
-                keep = lb - m < (double) p.level;
-            }
-            rel[ch] = __ballot(keep);
-        }
+        const int nxr = (int) min<int64_t>(4, p.X - xa), nyr = (int) min<int64_t>(4, p.Y - ya);
+        const int nzp = (int) min<int64_t>(64, p.Z - za);
+        // grid coordinates: lanes 0-3 the block's x rows, 4-7 its y rows, every lane its z
+        const float cxy = lane < 4 ? grid_coord(xa + min(lane, nxr - 1), p.X, p.amin[0], p.amax[0])
+                                   : grid_coord(ya + min(max(lane - 4, 0), nyr - 1), p.Y, p.amin[1], p.amax[1]);
         const int64_t z = za + lane;
         const bool zok = z < p.Z;
         const float pz = grid_coord(zok ? z : p.Z - 1, p.Z, p.amin[2], p.amax[2]);
+        float blo[3], bhi[3];
+        {
+            const float x0c = __shfl(cxy, 0), x1c = __shfl(cxy, nxr - 1);
+            const float y0c = __shfl(cxy, 4), y1c = __shfl(cxy, 4 + nyr - 1);
+            const float z0c = __shfl(pz, 0), z1c = __shfl(pz, nzp - 1);
+            blo[0] = fminf(x0c, x1c);
+            bhi[0] = fmaxf(x0c, x1c);
+            blo[1] = fminf(y0c, y1c);
+            bhi[1] = fmaxf(y0c, y1c);
+            blo[2] = fminf(z0c, z1c);
+            bhi[2] = fmaxf(z0c, z1c);
+        }
+        uint64_t rel[NCH], any_rel = 0, full = 0;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const int k = 64 * ch + lane;
+            bool keep = false, all_in = false;
+            if (k < p.nsph) sphere_sign_bounds(blo, bhi, sph[k], p.level, keep, all_in);
+            rel[ch] = __ballot(keep);
+            any_rel |= rel[ch];
+            full |= __ballot(all_in);
+        }
+        if (full || !any_rel) {   // the whole block is inside / outside
+            if (lane < 16) {
+                const int64_t x = xa + (lane >> 2), y = ya + (lane & 3);
+                if (x < p.X && y < p.Y) p.bits[(x * p.Y + y) * p.W + c] = full ? low_bits(p.Z - za) : 0ull;
+            }
+            continue;
+        }
         uint64_t mine = 0;
         for (int i = 0; i < 16; ++i) {
             const int64_t x = xa + (i >> 2), y = ya + (i & 3);
+            const float px = __shfl(cxy, i >> 2), py = __shfl(cxy, 4 + (i & 3));
             if (x >= p.X || y >= p.Y) continue;
-            const float px = grid_coord(x, p.X, p.amin[0], p.amax[0]);
-            const float py = grid_coord(y, p.Y, p.amin[1], p.amax[1]);
             bool in = false;
 #pragma unroll
             for (int ch = 0; ch < NCH; ++ch) {
@@ -649,24 +693,25 @@ __global__ void __launch_bounds__(256) k_classify_spheres(McParams p) {
     }
 }
 
-// LDS of the fused emit: the spheres that can be the minimum somewhere in
-// the tile's box.
-template <bool F>
+// LDS of the fused emit: per 64-point z chunk of the tile, the spheres that
+// can be the minimum somewhere in the chunk's box (about 2 KB in all).
+template <bool F, int NZC>
 struct FusedLds {};
-template <>
-struct FusedLds<true> {
-    float4 sph[kMaxFusedSpheres];
-    int n;
+template <int NZC>
+struct FusedLds<true, NZC> {
+    static constexpr int CAP = 128 / NZC > 8 ? 128 / NZC : 8;   // spheres per chunk
+    float4 sph[NZC][CAP];
+    int n[NZC];   // culled spheres; > CAP: evaluate all of p.spheres instead
     double red[16];
 };
 
 // Keeps sphere k unless lb_k - m_k > min_j (ub_j + m_j) (then it is larger
 // than another sphere's value everywhere in the box).  Block-wide; all
 // threads call it.
-template <int NT>
-__device__ void cull_spheres(const McParams &p, const Box3 &box, FusedLds<true> &f) {
+template <int NT, int CAP>
+__device__ void cull_spheres(const McParams &p, const Box3 &box, float4 *out, int &n_out, double *red) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) f.n = 0;
+    if (tid == 0) n_out = 0;
     double best = INFINITY;
     for (int k = tid; k < p.nsph; k += NT) {
         double lb, ub, m;
@@ -674,15 +719,18 @@ __device__ void cull_spheres(const McParams &p, const Box3 &box, FusedLds<true> 
         best = fmin(best, ub + m);
     }
     for (int o = 32; o > 0; o >>= 1) best = fmin(best, __shfl_xor(best, o));
-    if (lane == 0) f.red[wave] = best;
+    if (lane == 0) red[wave] = best;
     __syncthreads();
     double tmin = INFINITY;
-    for (int w = 0; w < NT / 64; ++w) tmin = fmin(tmin, f.red[w]);
+    for (int w = 0; w < NT / 64; ++w) tmin = fmin(tmin, red[w]);
     for (int k = tid; k < p.nsph; k += NT) {
         double lb, ub, m;
         const float4 sp = p.spheres[k];
         sphere_bounds(box, sp, lb, ub, m);
-        if (lb - m <= tmin) f.sph[atomicAdd(&f.n, 1)] = sp;
+        if (lb - m <= tmin) {
+            const int slot = atomicAdd(&n_out, 1);
+            if (slot < CAP) out[slot] = sp;
+        }
     }
     __syncthreads();
 }
@@ -809,7 +857,7 @@ template <int TX, int TY, int NZC, int NT, bool FUSED = false>
 __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int spseg) {
     using L = EmitLds<TX, TY, NZC, NT>;
     __shared__ L s;
-    __shared__ FusedLds<FUSED> fl;
+    __shared__ FusedLds<FUSED, NZC> fl;
     const int tid = threadIdx.x;
     if (*(volatile int *) p.err & ISO_ERR_TOO_LARGE) return;
 
@@ -863,10 +911,15 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             s.cz[c - TX - TY - 2] = grid_coord(z0 + (c - TX - TY - 2), p.Z, p.amin[2], p.amax[2]);
         }
     }
-    if constexpr (FUSED)
-        cull_spheres<NT>(p, point_box(p, x0, min(x0 + TX, p.X - 1), y0, min(y0 + TY, p.Y - 1), z0,
-                                      min(z0 + 64 * NZC, p.Z - 1)),
-                         fl);
+    if constexpr (FUSED) {
+        using F = FusedLds<true, NZC>;
+        for (int j = 0; j < NZC; ++j) {
+            const int64_t za = min(z0 + 64 * j, p.Z - 1);
+            cull_spheres<NT, F::CAP>(p, point_box(p, x0, min(x0 + TX, p.X - 1), y0, min(y0 + TY, p.Y - 1), za,
+                                                  min(za + 64, p.Z - 1)),
+                                     fl.sph[j], fl.n[j], fl.red);
+        }
+    }
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
         const int i = tid + k * NT;
@@ -971,13 +1024,28 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             if constexpr (FUSED) {
                 const float xs[2] = {s.cx[cx], s.cx[cx + 1]}, ys[2] = {s.cy[cy], s.cy[cy + 1]};
                 const float zs[2] = {s.cz[zl], s.cz[zl + 1]};
+                using F = FusedLds<true, NZC>;
+                const int ns = fl.n[jc];
+                float best[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    float best = INFINITY;
-                    for (int j = 0; j < fl.n; ++j)
-                        best = fminf(best, sphere_d(xs[q >> 2], ys[(q >> 1) & 1], zs[q & 1], fl.sph[j]));
-                    cc.v[q] = best;
+                for (int q = 0; q < 8; ++q) best[q] = INFINITY;
+                if (ns <= F::CAP) {   // the chunk's culled spheres (LDS)
+                    for (int j = 0; j < ns; ++j) {
+                        const float4 sp = fl.sph[jc][j];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q)
+                            best[q] = fminf(best[q], sphere_d(xs[q >> 2], ys[(q >> 1) & 1], zs[q & 1], sp));
+                    }
+                } else {   // too many kept: all spheres (the same minimum)
+                    for (int j = 0; j < p.nsph; ++j) {
+                        const float4 sp = p.spheres[j];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q)
+                            best[q] = fminf(best[q], sphere_d(xs[q >> 2], ys[(q >> 1) & 1], zs[q & 1], sp));
+                    }
                 }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) cc.v[q] = best[q];
             } else {
                 const float *r00 = p.values + (gx * p.Y + gy) * p.Z + gz;
                 const float *r01 = r00 + p.Z;
