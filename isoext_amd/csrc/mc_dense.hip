@@ -984,10 +984,25 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
         const int jj = j0 + tid;
         const bool live = jj < nactive;
         int u = 0, cs = 0, k = 0;
+        float pre[8];   // corner values, loaded before the face scan so that the scan hides their latency
         if (live) {
             u = s.unit[tid];
             k = nth_set_bit(s.amask[u], jj - (int) s.na[u]);
             const int jc = u % NZC, r = u / NZC, cyy = r % TY, cxx = r / TY;
+            if constexpr (!FUSED) {
+                const float *r00 = p.values + ((x0 + cxx) * p.Y + (y0 + cyy)) * p.Z + (z0 + 64 * jc + k);
+                const float *r01 = r00 + p.Z;
+                const float *r10 = r00 + p.Y * p.Z;
+                const float *r11 = r10 + p.Z;
+                pre[0] = r00[0];
+                pre[1] = r00[1];
+                pre[2] = r01[0];
+                pre[3] = r01[1];
+                pre[4] = r10[0];
+                pre[5] = r10[1];
+                pre[6] = r11[0];
+                pre[7] = r11[1];
+            }
             // corner k of the cell sits at (k>>2, (k>>1)&1, k&1)
             const uint64_t w00 = PW(s, cxx, cyy, jc), w01 = PW(s, cxx, cyy + 1, jc);
             const uint64_t w10 = PW(s, cxx + 1, cyy, jc), w11 = PW(s, cxx + 1, cyy + 1, jc);
@@ -1047,18 +1062,8 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
 #pragma unroll
                 for (int q = 0; q < 8; ++q) cc.v[q] = best[q];
             } else {
-                const float *r00 = p.values + (gx * p.Y + gy) * p.Z + gz;
-                const float *r01 = r00 + p.Z;
-                const float *r10 = r00 + p.Y * p.Z;
-                const float *r11 = r10 + p.Z;
-                cc.v[0] = r00[0];
-                cc.v[1] = r00[1];
-                cc.v[2] = r01[0];
-                cc.v[3] = r01[1];
-                cc.v[4] = r10[0];
-                cc.v[5] = r10[1];
-                cc.v[6] = r11[0];
-                cc.v[7] = r11[1];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) cc.v[q] = pre[q];
             }
             cc.c[0][0] = s.cx[cx];
             cc.c[0][1] = s.cx[cx + 1];
@@ -1530,11 +1535,24 @@ __device__ __forceinline__ void its_tile(const McParams &p, const ItsOut &o, int
         const int jj = j0 + tid;
         const bool live = jj < nactive;
         int u = 0, k = 0, cs = 0;
+        float cv[8];   // corner values, loaded before the entry scan so that the scan hides their latency
         if (live) {
             u = s.unit[tid];
             k = nth_set_bit(s.amask[u], jj - (int) s.na[u]);
             const int jc = u % NZC, r = u / NZC;
             cs = cell_case(its_unit_cells(s, r / TY, r % TY, jc), k);
+            const float *r00 = p.values + ((x0 + r / TY) * p.Y + (y0 + r % TY)) * p.Z + (z0 + 64 * jc + k);
+            const float *r01 = r00 + p.Z;
+            const float *r10 = r00 + p.Y * p.Z;
+            const float *r11 = r10 + p.Z;
+            cv[0] = r00[0];
+            cv[1] = r00[1];
+            cv[2] = r01[0];
+            cv[3] = r01[1];
+            cv[4] = r10[0];
+            cv[5] = r10[1];
+            cv[6] = r11[0];
+            cv[7] = r11[1];
         }
         s.esc[tid] = live ? (uint32_t) s.ncross[cs] : 0u;
         __syncthreads();
@@ -1553,21 +1571,6 @@ __device__ __forceinline__ void its_tile(const McParams &p, const ItsOut &o, int
             const int zl = 64 * jc + k;
             const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + zl;
             const int64_t id = (gx * (p.Y - 1) + gy) * (p.Z - 1) + gz;
-            float cv[8];
-            {
-                const float *r00 = p.values + (gx * p.Y + gy) * p.Z + gz;
-                const float *r01 = r00 + p.Z;
-                const float *r10 = r00 + p.Y * p.Z;
-                const float *r11 = r10 + p.Z;
-                cv[0] = r00[0];
-                cv[1] = r00[1];
-                cv[2] = r01[0];
-                cv[3] = r01[1];
-                cv[4] = r10[0];
-                cv[5] = r10[1];
-                cv[6] = r11[0];
-                cv[7] = r11[1];
-            }
             if (a < p.cap_v) {   // capacities from the count pass (defensive)
                 o.cell_indices[a] = id;
                 o.cell_dense[a] = id;
