@@ -362,11 +362,13 @@ def serving_bench(n_requests: int, seed: int = 0):
     def stream(rate, s):
         return serving.make_stream(pool, n_requests, rate=rate, seed=s, dc_frac=0.2)
 
-    # warm-up both paths (library load, workspaces, autotune tables)
-    warm = stream(0.0, 99)[:16]
+    # warm-up both paths (library load, workspaces grown to their steady-state
+    # sizes, autotune tables): a full saturated stream of another seed through
+    # the server, a fifth of one through the plain path
+    warm = stream(0.0, seed + 99)
     srv = serving.Server(hbm_budget=200e9, max_batch=64, workers=4)
     serving.drive(srv, warm)
-    for r in warm:
+    for r in warm[: max(16, n_requests // 5)]:
         serving.run_plain(r)
     torch.cuda.synchronize()
     # saturation throughput: every request available at t = 0
