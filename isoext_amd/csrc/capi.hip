@@ -250,8 +250,8 @@ bool grid_has_cells(const isoext_grid *g) {
 }
 
 struct DcLayout {
-    size_t o_dual, o_cond, o_cnt, o_used, o_bnd, o_bsum, o_part, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, o_ak, o_av,
-        total;
+    size_t o_dual, o_cond, o_cnt, o_used, o_bnd, o_qn, o_bsum, o_part, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, o_ak,
+        o_av, total;
     int64_t hslots;
 };
 
@@ -272,6 +272,7 @@ DcLayout dc_layout(int64_t ncells, int64_t npts) {
     L.o_cnt = take(64);
     L.o_used = take(std::max<int64_t>(ncells, 1));
     L.o_bnd = take(std::max<int64_t>(ncells, 1));
+    L.o_qn = take(std::max<int64_t>(ncells, 1));
     L.o_bsum = take(sizeof(int64_t) * 2 * nb);
     L.o_part = take(sizeof(int64_t) * 2 * iso::dc_scan_part_pairs(nb));
     L.o_tot = take(64);
@@ -700,6 +701,7 @@ int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_
     w.n_fp64 = (unsigned long long *) (ws + L.o_cnt);
     w.used = (uint8_t *) (ws + L.o_used);
     w.bnd = (uint8_t *) (ws + L.o_bnd);
+    w.qn = (uint8_t *) (ws + L.o_qn);
     w.bsum = (int64_t *) (ws + L.o_bsum);
     w.part = (int64_t *) (ws + L.o_part);
     w.tot = (int64_t *) (ws + L.o_tot);
@@ -1356,6 +1358,7 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     w.n_fp64 = (unsigned long long *) (dw + D.o_cnt);
     w.used = (uint8_t *) (dw + D.o_used);
     w.bnd = (uint8_t *) (dw + D.o_bnd);
+    w.qn = (uint8_t *) (dw + D.o_qn);
     w.bsum = (int64_t *) (dw + D.o_bsum);
     w.part = (int64_t *) (dw + D.o_part);
     w.tot = (int64_t *) (dw + D.o_tot);

@@ -56,6 +56,7 @@ struct DcScratch {
     unsigned long long *n_fp64;
     uint8_t *used;
     uint8_t *bnd;         // vertex on its cell's box (may coincide with a neighbour's)
+    uint8_t *qn;          // dense sets: quads emitted per cell
     int64_t *bsum;
     int64_t *part;        // chunk sums of the block-sum scans (dc_scan_part_pairs(nb) pairs)
     int64_t *tot;

@@ -369,9 +369,11 @@ __device__ __forceinline__ T pinv3(T A[3][3], const T b[3], T tol, T x[3], int s
 // dense_used (a dense grid whose intersection set is the grid's own), whether
 // the cell belongs to a quad -- one of its crossing edges has all four cells
 // around it inside the grid; otherwise 0, set later by the quad pass.
+// With dense_used it also writes qn[a]: the quads the cell emits (its
+// canonical crossing edges 0, 3, 8 whose lower neighbours exist).
 __global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_tol, float cond_fp64,
                          float *dual_v, float *cond_out, unsigned long long *n_fp64, uint8_t *used, uint8_t *bnd,
-                         bool dense_used, DcBatch bt) {
+                         uint8_t *qn, bool dense_used, DcBatch bt) {
     __shared__ int64_t starts[kMaxDcBatchGrids];
     if (bt.items) {
         for (int i = threadIdx.x; i < bt.n; i += kBlock) starts[i] = bt.items[i].cell0;
@@ -468,18 +470,21 @@ __global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_
         dual_v[3 * a + k] = v;
         on_box |= v == blo[k] || v == bhi[k];
     }
-    uint8_t u = 0;
+    uint8_t u = 0, nq = 0;
     if (dense_used) {
         const int64_t cc[3] = {x, y, z}, N[3] = {s.X, s.Y, s.Z};
-        for (int64_t j = lo; j < hi && !u; ++j) {
+        for (int64_t j = lo; j < hi; ++j) {
             const int k = in.edge_local[j], ax = edge_axis(k), ca = edge_a(k);
             const int off[3] = {ca >> 2, (ca >> 1) & 1, ca & 1};
             bool ok = true;
             // across the edge, cells cc[d] - 1 and cc[d] (offset 0) or cc[d] and cc[d] + 1 (offset 1)
             for (int d = 0; d < 3; ++d)
                 if (d != ax) ok = ok && (off[d] ? cc[d] + 3 <= N[d] : cc[d] >= 1);
-            u = ok ? 1 : 0;
+            u |= ok ? 1 : 0;
+            // canonical edges sit at the cell's lowest corner (offsets 0): this cell emits their quad
+            nq += (ok && ca == 0) ? 1 : 0;
         }
+        qn[a] = nq;
     }
     used[a] = u;
     bnd[a] = on_box ? 1 : 0;
@@ -705,15 +710,17 @@ __device__ __forceinline__ bool kept_vertex(const uint8_t *used, const uint8_t *
     return used[a] && (!bnd[a] || slot_min[rep_slot[a]] == (int32_t) a);
 }
 
+// Block sums of (kept vertices, quads per cell from qn, or 0 without it).
 __global__ void k_keep_flags(const uint8_t *used, const uint8_t *bnd, int64_t n, const int32_t *rep_slot,
-                             const int32_t *slot_min, int64_t *bsum) {
+                             const int32_t *slot_min, const uint8_t *qn, int64_t *bsum) {
     int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
     int64_t k = (a < n && kept_vertex(used, bnd, rep_slot, slot_min, a)) ? 1 : 0;
-    int64_t ek, d, tk, td;
-    block_scan2(k, 0, ek, d, tk, td);
+    int64_t q = (qn && a < n) ? qn[a] : 0;
+    int64_t ek, eq, tk, tq;
+    block_scan2(k, q, ek, eq, tk, tq);
     if (threadIdx.x == 0) {
         bsum[2 * blockIdx.x] = tk;
-        bsum[2 * blockIdx.x + 1] = 0;
+        bsum[2 * blockIdx.x + 1] = tq;
     }
 }
 
@@ -746,26 +753,93 @@ __global__ void k_remap_faces(const int32_t *tri_cells, const int64_t *nquads, c
 // Batch: first output vertex of each grid = kept vertices before its first
 // cell (block prefix of the keep scan + the kept cells of that block before
 // it); one workgroup per grid, vbase[n] = the total.
+// With qn (quads counted per cell), the first quads too: bsum_pre's second
+// component and tot[3] hold the quad scan.
 __global__ void k_item_vbase(DcBatch bt, const uint8_t *used, const uint8_t *bnd, const int32_t *rep_slot,
-                             const int32_t *slot_min, const int64_t *bsum_pre, int64_t ncells, const int64_t *tot) {
-    __shared__ int part[kBlock / 64];
+                             const int32_t *slot_min, const uint8_t *qn, const int64_t *bsum_pre, int64_t ncells,
+                             const int64_t *tot) {
+    __shared__ int part[kBlock / 64], partq[kBlock / 64];
     const int i = blockIdx.x;
-    if (i == 0 && threadIdx.x == 0) bt.vbase[bt.n] = tot[2];
+    if (i == 0 && threadIdx.x == 0) {
+        bt.vbase[bt.n] = tot[2];
+        if (qn) bt.qbase[bt.n] = tot[3];
+    }
     const int64_t c0 = bt.items[i].cell0;
     if (c0 >= ncells) {
-        if (threadIdx.x == 0) bt.vbase[i] = tot[2];
+        if (threadIdx.x == 0) {
+            bt.vbase[i] = tot[2];
+            if (qn) bt.qbase[i] = tot[3];
+        }
         return;
     }
     const int64_t blk = c0 / kBlock, a = blk * kBlock + threadIdx.x;
     int k = (a < c0 && kept_vertex(used, bnd, rep_slot, slot_min, a)) ? 1 : 0;
-    for (int o = 32; o > 0; o >>= 1) k += __shfl_down(k, o);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = k;
+    int q = (qn && a < c0) ? qn[a] : 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        k += __shfl_down(k, o);
+        q += __shfl_down(q, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[threadIdx.x >> 6] = k;
+        partq[threadIdx.x >> 6] = q;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < kBlock / 64; ++w) t += part[w];
+        int t = 0, tq = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            t += part[w];
+            tq += partq[w];
+        }
         bt.vbase[i] = bsum_pre[2 * blk] + t;
+        if (qn) bt.qbase[i] = bsum_pre[2 * blk + 1] + tq;
     }
+}
+
+// Quads of dense sets, one thread per active cell: the cell's canonical
+// entries in entry order (the per-entry kernel's order), at the cell's
+// offset in the quad scan (second component of bsum_pre); corners are
+// written as welded vertex ids.
+__global__ void k_quad_emit_cells(ItsIn in, CellSrc s1, const uint8_t *qn, const int64_t *bsum_pre,
+                                  const float *dual_v, WeldIds wd, DcBatch bt) {
+    const int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nq = a < in.ncells ? qn[a] : 0;
+    int64_t ed, eq, td, tq;
+    block_scan2(0, nq, ed, eq, td, tq);
+    if (!nq) return;
+    int64_t slot = bsum_pre[2 * blockIdx.x + 1] + eq;
+    const DcItem *it = nullptr;
+    DcItem one;
+    if (bt.items) {
+        it = &bt.items[bt.cell_item[a]];
+    } else {
+        one = DcItem{s1, 0, 0, in.amask_tab, in.apre_tab, (s1.Z + 63) / 64};
+        it = &one;
+    }
+    const ActHash ah{nullptr, nullptr, 0};
+    const int64_t lo = in.cell_offsets[a], hi = in.cell_offsets[a + 1];
+    for (int64_t e = lo; e < hi; ++e) {
+        int64_t q[4];
+        if (!entry_quad(in, it->s, ah, e, q, true, it->key_base, it)) continue;
+        const float *v0 = dual_v + 3 * q[0], *v1 = dual_v + 3 * q[1], *v2 = dual_v + 3 * q[2],
+                    *v3 = dual_v + 3 * q[3];
+        int32_t t[6];
+        if (dist3(v0, v2) > dist3(v1, v3)) {
+            const int32_t u[6] = {(int32_t) q[1], (int32_t) q[3], (int32_t) q[0],
+                                  (int32_t) q[3], (int32_t) q[1], (int32_t) q[2]};
+            for (int k = 0; k < 6; ++k) t[k] = u[k];
+        } else {
+            const int32_t u[6] = {(int32_t) q[2], (int32_t) q[0], (int32_t) q[1],
+                                  (int32_t) q[0], (int32_t) q[2], (int32_t) q[3]};
+            for (int k = 0; k < 6; ++k) t[k] = u[k];
+        }
+        for (int k = 0; k < 6; ++k)
+            wd.out_f[6 * slot + k] = welded_id(wd.bnd, wd.rep_slot, wd.slot_min, wd.final_id, t[k]);
+        ++slot;
+    }
+}
+
+__global__ void k_copy_i64(int64_t *dst, const int64_t *src) {
+    if (threadIdx.x == 0) *dst = *src;
 }
 
 // First quads of grids without cells (no thread recorded them: they equal
@@ -1077,28 +1151,27 @@ static void scan_pairs(int64_t *bsum, int64_t nb, int64_t *tot, int64_t *part, h
 static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float svd_tol, float cond_fp64,
                             DcScratch &w, const DcBatch &bt, hipStream_t st) {
     // Dense grids with lookup tables (the intersection set is the grid's
-    // own): no active-cell hash, used[] comes from the QEF pass, and the weld
-    // runs before the quads, which then write vertex ids directly.
-    // Otherwise: hash, quads (marking used[]), weld, face remap.
+    // own): no active-cell hash; the QEF pass also gives each cell's quad
+    // membership and quad count, so the weld runs first and the quads are
+    // written per cell as vertex ids, with vertex and quad offsets from one
+    // scan.  Otherwise: hash, per-entry quads (marking used[]), weld, face
+    // remap.
     const bool tables = bt.items ? bt.tables : (in.amask_tab && in.apre_tab && !s.ids);
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, (int64_t *) w.n_fp64, (int64_t) 1, (int64_t) 0);
     hipLaunchKernelGGL(k_dc_qef, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in, s, reg, svd_tol, cond_fp64,
-                       w.dual_v, w.cond, w.n_fp64, w.used, w.bnd, tables, bt);
+                       w.dual_v, w.cond, w.n_fp64, w.used, w.bnd, w.qn, tables, bt);
     const ActHash ah{w.akey, w.aval, w.amask};
     const unsigned nbq = nblocks(in.npts), nbv = nblocks(in.ncells);
-    auto quads = [&](const WeldIds &wd) {
-        hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, bt);
-        scan_pairs(w.bsum, (int64_t) nbq, w.tot, w.part, st);
-        hipLaunchKernelGGL(k_quad_emit, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, w.dual_v, w.tri_cells,
-                           w.used, wd, bt);
-        if (bt.items) hipLaunchKernelGGL(k_item_qbase, dim3(1), dim3(64), 0, st, bt, in.ncells, (const int64_t *) w.tot);
-    };
-    if (!tables) {   // active-cell hash for the quads' neighbour lookups
+    if (!tables) {   // active-cell hash for the quads' neighbour lookups, then the quads
         const unsigned nbh = (unsigned) std::min<int64_t>(nblocks(w.amask + 1), 4096);
         hipLaunchKernelGGL(k_fill_i64, dim3(nbh), dim3(kBlock), 0, st, w.akey, w.amask + 1, (int64_t) -1);
         hipLaunchKernelGGL(k_act_insert, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, in.cell_dense, in.ncells,
                            w.akey, w.aval, w.amask, bt);
-        quads(WeldIds{});
+        hipLaunchKernelGGL(k_quad_count, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, bt);
+        scan_pairs(w.bsum, (int64_t) nbq, w.tot, w.part, st);
+        hipLaunchKernelGGL(k_quad_emit, dim3(nbq), dim3(kBlock), 0, st, in, s, ah, w.bsum, w.dual_v, w.tri_cells,
+                           w.used, WeldIds{}, bt);
+        if (bt.items) hipLaunchKernelGGL(k_item_qbase, dim3(1), dim3(64), 0, st, bt, in.ncells, (const int64_t *) w.tot);
     }
     // weld dual vertices by position
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.hslots, w.hmask + 1, -1);
@@ -1108,16 +1181,20 @@ static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float 
                        in.ncells, w.hslots, w.hmask, w.rep_slot, (const int32_t *) bt.cell_item);
     hipLaunchKernelGGL(k_slot_min, dim3(nblocks(in.ncells)), dim3(kBlock), 0, st, w.used, w.bnd, in.ncells,
                        w.rep_slot, w.slot_min);
+    const uint8_t *qn = tables ? w.qn : nullptr;
     hipLaunchKernelGGL(k_keep_flags, dim3(nbv), dim3(kBlock), 0, st, w.used, w.bnd, in.ncells, w.rep_slot,
-                       w.slot_min, w.bsum);
+                       w.slot_min, qn, w.bsum);
+    // tot[2] = kept vertices, tot[3] = quads (tables); tot[0] = quads after the copy
     scan_pairs(w.bsum, (int64_t) nbv, w.tot + 2, w.part, st);
+    if (tables) hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, st, w.tot, (const int64_t *) (w.tot + 3));
     if (bt.items)
         hipLaunchKernelGGL(k_item_vbase, dim3(bt.n), dim3(kBlock), 0, st, bt, w.used, w.bnd, w.rep_slot, w.slot_min,
-                           w.bsum, in.ncells, (const int64_t *) w.tot);
+                           qn, w.bsum, in.ncells, (const int64_t *) w.tot);
     hipLaunchKernelGGL(k_keep_emit, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, w.bnd, in.ncells,
                        w.rep_slot, w.slot_min, w.bsum, w.final_id, w.out_v, w.out_vcell, bt);
     if (tables)
-        quads(WeldIds{w.bnd, w.rep_slot, w.slot_min, w.final_id, w.out_f});
+        hipLaunchKernelGGL(k_quad_emit_cells, dim3(nbv), dim3(kBlock), 0, st, in, s, w.qn, w.bsum, w.dual_v,
+                           WeldIds{w.bnd, w.rep_slot, w.slot_min, w.final_id, w.out_f}, bt);
     else
         hipLaunchKernelGGL(k_remap_faces, dim3(nblocks(6 * in.npts)), dim3(kBlock), 0, st, w.tri_cells, w.tot,
                            w.bnd, w.rep_slot, w.slot_min, w.final_id, w.out_f);
