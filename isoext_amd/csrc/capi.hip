@@ -250,8 +250,8 @@ bool grid_has_cells(const isoext_grid *g) {
 }
 
 struct DcLayout {
-    size_t o_dual, o_cond, o_cnt, o_used, o_bnd, o_qn, o_bsum, o_part, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin, o_ak,
-        o_av, total;
+    size_t o_dual, o_cond, o_cnt, o_used, o_bnd, o_qn, o_qm, o_bsum, o_part, o_tot, o_tri, o_hs, o_hm, o_rep, o_fin,
+        o_ak, o_av, total;
     int64_t hslots;
 };
 
@@ -273,6 +273,7 @@ DcLayout dc_layout(int64_t ncells, int64_t npts) {
     L.o_used = take(std::max<int64_t>(ncells, 1));
     L.o_bnd = take(std::max<int64_t>(ncells, 1));
     L.o_qn = take(std::max<int64_t>(ncells, 1));
+    L.o_qm = take(std::max<int64_t>(ncells, 1));
     L.o_bsum = take(sizeof(int64_t) * 2 * nb);
     L.o_part = take(sizeof(int64_t) * 2 * iso::dc_scan_part_pairs(nb));
     L.o_tot = take(64);
@@ -543,13 +544,13 @@ int batch_check(int n, const isoext_mc_item *items, int cfg) {
     return ISOEXT_OK;
 }
 
-// Scratch of a batched dual contouring run: the concatenated intersection
-// sets (C active cells, P entries), the per-grid descriptors and counts, and
-// the DC work arrays for C cells and P entries.
+// Scratch of a batched dual contouring run: the grids' active cells (C in
+// all; no intersection entries are stored: each cell's edges are recomputed
+// from its corner values), their lookup tables, the per-grid descriptors and
+// counts, and the DC work arrays for C cells.
 struct DcBatchLayout {
     DcLayout dc;
-    size_t o_dc, o_ci, o_cd, o_co, o_pts, o_nrm, o_el, o_io, o_ec, o_outs, o_items, o_citem, o_vb, o_qb, o_tm, o_tp,
-        total;
+    size_t o_dc, o_cd, o_outs, o_items, o_citem, o_vb, o_qb, o_tm, o_tp, total;
     std::vector<int64_t> tab0;   // per grid: first lookup-table entry
 };
 
@@ -573,15 +574,9 @@ DcBatchLayout dc_batch_layout(int n, const isoext_mc_item *items, int64_t C, int
         o = align256(o + bytes);
         return r;
     };
-    const size_t c1 = (size_t) std::max<int64_t>(C, 1), p1 = (size_t) std::max<int64_t>(P, 1);
-    L.o_ci = take(sizeof(int64_t) * c1);
+    (void) P;
+    const size_t c1 = (size_t) std::max<int64_t>(C, 1);
     L.o_cd = take(sizeof(int64_t) * c1);
-    L.o_co = take(sizeof(int64_t) * (c1 + 1));
-    L.o_pts = take(sizeof(float) * 3 * p1);
-    L.o_nrm = take(sizeof(float) * 3 * p1);
-    L.o_el = take(p1);
-    L.o_io = take(p1);
-    L.o_ec = take(sizeof(int32_t) * p1);
     L.o_outs = take(sizeof(iso::ItsOut) * (size_t) n);
     L.o_items = take(iso::dc_batch_item_bytes() * (size_t) n);
     L.o_citem = take(sizeof(int32_t) * c1);
@@ -589,7 +584,7 @@ DcBatchLayout dc_batch_layout(int n, const isoext_mc_item *items, int64_t C, int
     L.o_qb = take(sizeof(int64_t) * (size_t) (n + 1));
     L.o_tm = take(sizeof(uint64_t) * (size_t) std::max<int64_t>(T, 1));
     L.o_tp = take(sizeof(int32_t) * (size_t) std::max<int64_t>(T, 1));
-    L.dc = dc_layout(C, P);
+    L.dc = dc_layout(C, 0);
     L.o_dc = take(L.dc.total);
     L.total = o;
     return L;
@@ -702,6 +697,7 @@ int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_
     w.used = (uint8_t *) (ws + L.o_used);
     w.bnd = (uint8_t *) (ws + L.o_bnd);
     w.qn = (uint8_t *) (ws + L.o_qn);
+    w.qmask = (uint8_t *) (ws + L.o_qm);
     w.bsum = (int64_t *) (ws + L.o_bsum);
     w.part = (int64_t *) (ws + L.o_part);
     w.tot = (int64_t *) (ws + L.o_tot);
@@ -1315,15 +1311,8 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     iso::ItsOut *ho = (iso::ItsOut *) g_pinned_its.get(sizeof(iso::ItsOut) * (size_t) n);
     if (!ho) return fail(ISOEXT_E_DEVICE, "pinned staging allocation failed");
     for (int i = 0; i < n; ++i) {
-        iso::ItsOut o;
-        o.cell_indices = (int64_t *) (sc + L.o_ci) + cell0[i];
+        iso::ItsOut o{};   // active cells and lookup tables only
         o.cell_dense = (int64_t *) (sc + L.o_cd) + cell0[i];
-        o.cell_offsets = (int64_t *) (sc + L.o_co) + cell0[i];
-        o.points = (float *) (sc + L.o_pts) + 3 * ent0[i];
-        o.normals = (float *) (sc + L.o_nrm) + 3 * ent0[i];
-        o.edge_local = (uint8_t *) (sc + L.o_el) + ent0[i];
-        o.is_out = (uint8_t *) (sc + L.o_io) + ent0[i];
-        o.entry_cell = (int32_t *) (sc + L.o_ec) + ent0[i];
         o.a_base = cell0[i];
         o.e_base = ent0[i];
         o.amask_tab = (uint64_t *) (sc + L.o_tm) + L.tab0[i];
@@ -1337,19 +1326,11 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
                                          (const int32_t *) (ws + B.o_order), (const iso::ItsOut *) (sc + L.o_outs),
                                          (const iso::McBatchHdr *) (ws + B.o_hdr), st),
               "batch intersections");
-    HIP_CHECK(iso::fill_i64((int64_t *) (sc + L.o_co) + C, 1, P, st), "batch intersections");
-    // 2. dual contouring of all grids at once
-    iso::ItsIn in;
+    // 2. dual contouring of all grids at once, edges recomputed from the corner values
+    iso::ItsIn in{};
     in.ncells = C;
     in.npts = P;
-    in.cell_indices = (const int64_t *) (sc + L.o_ci);
     in.cell_dense = (const int64_t *) (sc + L.o_cd);
-    in.cell_offsets = (const int64_t *) (sc + L.o_co);
-    in.points = (const float *) (sc + L.o_pts);
-    in.normals = (const float *) (sc + L.o_nrm);
-    in.edge_local = (const uint8_t *) (sc + L.o_el);
-    in.is_out = (const uint8_t *) (sc + L.o_io);
-    in.entry_cell = (const int32_t *) (sc + L.o_ec);
     char *dw = sc + L.o_dc;
     const DcLayout &D = L.dc;
     iso::DcScratch w;
@@ -1359,11 +1340,12 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     w.used = (uint8_t *) (dw + D.o_used);
     w.bnd = (uint8_t *) (dw + D.o_bnd);
     w.qn = (uint8_t *) (dw + D.o_qn);
+    w.qmask = (uint8_t *) (dw + D.o_qm);
     w.bsum = (int64_t *) (dw + D.o_bsum);
     w.part = (int64_t *) (dw + D.o_part);
     w.tot = (int64_t *) (dw + D.o_tot);
     w.tri_cells = (int32_t *) (dw + D.o_tri);
-    w.cap_tri = 6 * std::max<int64_t>(P, 1);
+    w.cap_tri = 6;
     w.hslots = (int32_t *) (dw + D.o_hs);
     w.slot_min = (int32_t *) (dw + D.o_hm);
     w.hmask = D.hslots - 1;
@@ -1390,9 +1372,11 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     }
     const float cond_fp64 = std::min(1e6f, 0.5f / std::max(svd_tol, 1e-30f));
     std::vector<int64_t> nv(n), nq(n);
+    std::vector<float> levels(n);
+    for (int i = 0; i < n; ++i) levels[i] = items[i].level;
     int64_t nfp = 0;
-    HIP_CHECK(iso::dc_run_batch(n, g.data(), cell0.data(), tm.data(), tp.data(), in, reg, svd_tol, cond_fp64, w, dd,
-                                nv.data(), nq.data(), nfp, st),
+    HIP_CHECK(iso::dc_run_batch(n, g.data(), cell0.data(), tm.data(), tp.data(), levels.data(), in, reg, svd_tol,
+                                cond_fp64, w, dd, nv.data(), nq.data(), nfp, st),
               "batched dual contouring");
     int64_t v0 = 0, f0 = 0;
     for (int i = 0; i < n; ++i) {

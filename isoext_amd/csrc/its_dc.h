@@ -57,6 +57,7 @@ struct DcScratch {
     uint8_t *used;
     uint8_t *bnd;         // vertex on its cell's box (may coincide with a neighbour's)
     uint8_t *qn;          // dense sets: quads emitted per cell
+    uint8_t *qmask;       // dense grids without a set: canonical quad edges and orientations per cell
     int64_t *bsum;
     int64_t *part;        // chunk sums of the block-sum scans (dc_scan_part_pairs(nb) pairs)
     int64_t *tot;
@@ -101,10 +102,13 @@ inline int64_t dc_scan_part_pairs(int64_t nb) { return nb / 1024 + 2; }
 size_t dc_batch_item_bytes();
 // amask / apre: per grid, the lookup tables written by the intersection pass
 // (ItsOut::amask_tab / apre_tab; the active-cell hash is not built).
+// levels != nullptr: no intersection set (in holds only ncells and
+// cell_dense); each cell's edges, points and normals come from the grid's
+// values at levels[i], with identical results.
 hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const uint64_t *const *amask,
-                        const int32_t *const *apre, const ItsIn &in, float reg, float svd_tol, float cond_fp64,
-                        DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq, int64_t &n_fp64,
-                        hipStream_t st);
+                        const int32_t *const *apre, const float *levels, const ItsIn &in, float reg, float svd_tol,
+                        float cond_fp64, DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq,
+                        int64_t &n_fp64, hipStream_t st);
 // Backward of dual contouring: dL/d(output vertex) -> dL/d(grid values)
 // (through the QEF solve and the edge interpolation of the intersection
 // points) and, when grad_normals != nullptr, dL/d(normals).

@@ -272,6 +272,7 @@ struct DcItem {
     const uint64_t *amask;    // lookup tables of the grid's active cells (instead of the hash)
     const int32_t *apre;
     int64_t W;                // words per cell row in the tables
+    float level;              // without an intersection set: the grid's iso-level
 };
 struct DcBatch {
     const DcItem *items;   // nullptr: a single grid (the kernels' own CellSrc)
@@ -364,59 +365,68 @@ __device__ __forceinline__ T pinv3(T A[3][3], const T b[3], T tol, T x[3], int s
     return wmin > T(0) ? wmax / wmin : T(INFINITY);
 }
 
-// Also sets bnd[a] when the clipped vertex lies on its cell's box (the only
-// vertices another cell's vertex can coincide with), and used[a]: with
-// dense_used (a dense grid whose intersection set is the grid's own), whether
-// the cell belongs to a quad -- one of its crossing edges has all four cells
-// around it inside the grid; otherwise 0, set later by the quad pass.
-// With dense_used it also writes qn[a]: the quads the cell emits (its
-// canonical crossing edges 0, 3, 8 whose lower neighbours exist).
-__global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_tol, float cond_fp64,
-                         float *dual_v, float *cond_out, unsigned long long *n_fp64, uint8_t *used, uint8_t *bnd,
-                         uint8_t *qn, bool dense_used, DcBatch bt) {
-    __shared__ int64_t starts[kMaxDcBatchGrids];
-    if (bt.items) {
-        for (int i = threadIdx.x; i < bt.n; i += kBlock) starts[i] = bt.items[i].cell0;
-        __syncthreads();
+// Entries of one cell read from an intersection set.
+struct ItsEdges {
+    const float *pts, *nrm;
+    int64_t lo;
+    int n;
+    __device__ __forceinline__ void point(int i, float p[3]) const {
+        for (int k = 0; k < 3; ++k) p[k] = pts[3 * (lo + i) + k];
     }
-    int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
-    if (a >= in.ncells) return;
-    int item = 0;
-    if (bt.items) {
-        int lo_i = 0, hi_i = bt.n;
-        while (hi_i - lo_i > 1) {
-            const int mid = (lo_i + hi_i) >> 1;
-            if (starts[mid] <= a) lo_i = mid;
-            else hi_i = mid;
+    __device__ __forceinline__ void point_normal(int i, float p[3], float nv[3]) const {
+        for (int k = 0; k < 3; ++k) {
+            p[k] = pts[3 * (lo + i) + k];
+            nv[k] = nrm[3 * (lo + i) + k];
         }
-        item = lo_i;
-        bt.cell_item[a] = item;
     }
-    const CellSrc &s = bt.items ? bt.items[item].s : s1;
-    const int64_t lo = in.cell_offsets[a], hi = in.cell_offsets[a + 1];
-    const int64_t id = in.cell_dense[a];
-    int64_t x, y, z;
-    cell_xyz(s, id, x, y, z);
-    float blo[3] = {grid_coord(x, s.X, s.amin[0], s.amax[0]), grid_coord(y, s.Y, s.amin[1], s.amax[1]),
-                    grid_coord(z, s.Z, s.amin[2], s.amax[2])};
-    float bhi[3] = {grid_coord(x + 1, s.X, s.amin[0], s.amax[0]),
-                    grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]),
-                    grid_coord(z + 1, s.Z, s.amin[2], s.amax[2])};
-    // mass point and cell scale
+};
+
+// Entries of one cell recomputed from its 8 corner values with the
+// intersection pass's formulas (get_its_op, src/its.cu:37-90; trilinear
+// normals, src/its.cu:186-268) in the same edge order: the same floats as an
+// intersection set of the grid with computed normals holds.
+struct GridEdges {
+    float cv[8], lo[3], hi[3], level;
+    int8_t e[12];
+    int n;
+    __device__ __forceinline__ void point(int i, float p[3]) const {
+        const int ca = edge_a(e[i]), cb = edge_b(e[i]);
+        const float t = edge_t(cv[ca], cv[cb], level);
+        p[0] = lerp1(t, (ca >> 2) ? hi[0] : lo[0], (cb >> 2) ? hi[0] : lo[0]);
+        p[1] = lerp1(t, ((ca >> 1) & 1) ? hi[1] : lo[1], ((cb >> 1) & 1) ? hi[1] : lo[1]);
+        p[2] = lerp1(t, (ca & 1) ? hi[2] : lo[2], (cb & 1) ? hi[2] : lo[2]);
+    }
+    __device__ __forceinline__ void point_normal(int i, float p[3], float nv[3]) const {
+        point(i, p);
+        cell_normal(lo, hi, cv, p[0], p[1], p[2], nv[0], nv[1], nv[2]);
+    }
+};
+
+// The QEF solve of one cell over its n >= 1 entries: FP32, centred at the
+// mass point and scaled by 1/h, one FP64-residual refinement step, FP64
+// absolute re-solve when ill-conditioned; clipped to the cell (fix_dual_v_op,
+// src/dc.cu:82-99).  Returns the condition number; on_box when the clipped
+// vertex lies on the cell's box.
+template <class Ent>
+__device__ float qef_cell(const Ent &en, const float blo[3], const float bhi[3], float reg, float svd_tol,
+                          float cond_fp64, unsigned long long *n_fp64, float out[3], bool &on_box) {
     float c[3] = {0.f, 0.f, 0.f};
-    for (int64_t j = lo; j < hi; ++j)
-        for (int k = 0; k < 3; ++k) c[k] += in.points[3 * j + k];
-    const float inv_n = 1.0f / (float) (hi - lo);
+    for (int j = 0; j < en.n; ++j) {
+        float p[3];
+        en.point(j, p);
+        for (int k = 0; k < 3; ++k) c[k] += p[k];
+    }
+    const float inv_n = 1.0f / (float) en.n;
     for (int k = 0; k < 3; ++k) c[k] *= inv_n;
     const float h = fmaxf(bhi[0] - blo[0], fmaxf(bhi[1] - blo[1], bhi[2] - blo[2]));
     const float ih = 1.0f / h;
     // A = sum n n^T + reg I ; r = sum n (n . (p - c)) / h   (centred, scaled)
     float A[3][3] = {{reg, 0.f, 0.f}, {0.f, reg, 0.f}, {0.f, 0.f, reg}};
     float r[3] = {0.f, 0.f, 0.f};
-    for (int64_t j = lo; j < hi; ++j) {
-        float n[3] = {in.normals[3 * j], in.normals[3 * j + 1], in.normals[3 * j + 2]};
-        float d = (n[0] * ((in.points[3 * j] - c[0]) * ih) + n[1] * ((in.points[3 * j + 1] - c[1]) * ih)) +
-                  n[2] * ((in.points[3 * j + 2] - c[2]) * ih);
+    for (int j = 0; j < en.n; ++j) {
+        float pp[3], n[3];
+        en.point_normal(j, pp, n);
+        float d = (n[0] * ((pp[0] - c[0]) * ih) + n[1] * ((pp[1] - c[1]) * ih)) + n[2] * ((pp[2] - c[2]) * ih);
         for (int p = 0; p < 3; ++p) {
             for (int q = 0; q < 3; ++q) A[p][q] += n[p] * n[q];
             r[p] += n[p] * d;
@@ -441,15 +451,16 @@ __global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_
         pinv3<float>(A2, res, svd_tol, dx, 6);
         for (int p = 0; p < 3; ++p) xs[p] += dx[p];
     }
-    float out[3];
     for (int k = 0; k < 3; ++k) out[k] = c[k] + xs[k] * h;
     if (!(cond <= cond_fp64)) {
         // FP64 re-solve in absolute coordinates exactly as the reference
         // poses it (minimum-norm when singular values are truncated).
         double Ad[3][3] = {{reg, 0, 0}, {0, reg, 0}, {0, 0, reg}}, bd[3] = {0, 0, 0}, pa[3] = {0, 0, 0};
-        for (int64_t j = lo; j < hi; ++j) {
-            double n[3] = {in.normals[3 * j], in.normals[3 * j + 1], in.normals[3 * j + 2]};
-            double p[3] = {in.points[3 * j], in.points[3 * j + 1], in.points[3 * j + 2]};
+        for (int j = 0; j < en.n; ++j) {
+            float pf[3], nf[3];
+            en.point_normal(j, pf, nf);
+            double n[3] = {nf[0], nf[1], nf[2]};
+            double p[3] = {pf[0], pf[1], pf[2]};
             double d = n[0] * p[0] + n[1] * p[1] + n[2] * p[2];
             for (int u = 0; u < 3; ++u) {
                 for (int v = 0; v < 3; ++v) Ad[u][v] += n[u] * n[v];
@@ -457,38 +468,161 @@ __global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_
                 pa[u] += p[u];
             }
         }
-        for (int u = 0; u < 3; ++u) bd[u] += (double) reg * (pa[u] / (double) (hi - lo));
+        for (int u = 0; u < 3; ++u) bd[u] += (double) reg * (pa[u] / (double) en.n);
         double xd[3];
         pinv3<double>(Ad, bd, (double) svd_tol, xd, 12);
         for (int k = 0; k < 3; ++k) out[k] = (float) xd[k];
         atomicAdd(n_fp64, 1ull);
     }
-    // clip to the cell (fix_dual_v_op, src/dc.cu:82-99)
-    bool on_box = false;
+    on_box = false;
     for (int k = 0; k < 3; ++k) {
         const float v = fminf(fmaxf(out[k], blo[k]), bhi[k]);
-        dual_v[3 * a + k] = v;
+        out[k] = v;
         on_box |= v == blo[k] || v == bhi[k];
     }
+    return cond;
+}
+
+// Grid of active cell a in a batch (binary search over the grids' first
+// cells staged in LDS); writes cell_item[a].
+__device__ __forceinline__ int batch_item_of(const DcBatch &bt, const int64_t *starts, int64_t a) {
+    int lo_i = 0, hi_i = bt.n;
+    while (hi_i - lo_i > 1) {
+        const int mid = (lo_i + hi_i) >> 1;
+        if (starts[mid] <= a) lo_i = mid;
+        else hi_i = mid;
+    }
+    bt.cell_item[a] = lo_i;
+    return lo_i;
+}
+
+// Whether the quad of edge k of cell (x, y, z) has all four cells inside the
+// grid: across the edge, cells c - 1 and c (corner offset 0) or c and c + 1
+// (offset 1) in each of the other two axes.
+__device__ __forceinline__ bool edge_interior(int k, int64_t x, int64_t y, int64_t z, const CellSrc &s) {
+    const int ax = edge_axis(k), ca = edge_a(k);
+    const int off[3] = {ca >> 2, (ca >> 1) & 1, ca & 1};
+    const int64_t cc[3] = {x, y, z}, N[3] = {s.X, s.Y, s.Z};
+    bool ok = true;
+    for (int d = 0; d < 3; ++d)
+        if (d != ax) ok = ok && (off[d] ? cc[d] + 3 <= N[d] : cc[d] >= 1);
+    return ok;
+}
+
+// QEF pass over an intersection set.  Also sets bnd[a] when the clipped
+// vertex lies on its cell's box (the only vertices another cell's vertex can
+// coincide with), and used[a]: with dense_used (a dense grid whose
+// intersection set is the grid's own), whether the cell belongs to a quad --
+// one of its crossing edges has all four cells around it inside the grid --
+// and qn[a], the quads the cell emits (its canonical crossing edges 0, 3, 8
+// whose lower neighbours exist); otherwise used[a] = 0, set later by the
+// quad pass.
+__global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_tol, float cond_fp64,
+                         float *dual_v, float *cond_out, unsigned long long *n_fp64, uint8_t *used, uint8_t *bnd,
+                         uint8_t *qn, bool dense_used, DcBatch bt) {
+    __shared__ int64_t starts[kMaxDcBatchGrids];
+    if (bt.items) {
+        for (int i = threadIdx.x; i < bt.n; i += kBlock) starts[i] = bt.items[i].cell0;
+        __syncthreads();
+    }
+    int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    if (a >= in.ncells) return;
+    const int item = bt.items ? batch_item_of(bt, starts, a) : 0;
+    const CellSrc &s = bt.items ? bt.items[item].s : s1;
+    const int64_t lo = in.cell_offsets[a], hi = in.cell_offsets[a + 1];
+    int64_t x, y, z;
+    cell_xyz(s, in.cell_dense[a], x, y, z);
+    const float blo[3] = {grid_coord(x, s.X, s.amin[0], s.amax[0]), grid_coord(y, s.Y, s.amin[1], s.amax[1]),
+                          grid_coord(z, s.Z, s.amin[2], s.amax[2])};
+    const float bhi[3] = {grid_coord(x + 1, s.X, s.amin[0], s.amax[0]), grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]),
+                          grid_coord(z + 1, s.Z, s.amin[2], s.amax[2])};
+    const ItsEdges en{in.points, in.normals, lo, (int) (hi - lo)};
+    float out[3];
+    bool on_box;
+    const float cond = qef_cell(en, blo, bhi, reg, svd_tol, cond_fp64, n_fp64, out, on_box);
+    for (int k = 0; k < 3; ++k) dual_v[3 * a + k] = out[k];
     uint8_t u = 0, nq = 0;
     if (dense_used) {
-        const int64_t cc[3] = {x, y, z}, N[3] = {s.X, s.Y, s.Z};
         for (int64_t j = lo; j < hi; ++j) {
-            const int k = in.edge_local[j], ax = edge_axis(k), ca = edge_a(k);
-            const int off[3] = {ca >> 2, (ca >> 1) & 1, ca & 1};
-            bool ok = true;
-            // across the edge, cells cc[d] - 1 and cc[d] (offset 0) or cc[d] and cc[d] + 1 (offset 1)
-            for (int d = 0; d < 3; ++d)
-                if (d != ax) ok = ok && (off[d] ? cc[d] + 3 <= N[d] : cc[d] >= 1);
+            const int k = in.edge_local[j];
+            const bool ok = edge_interior(k, x, y, z, s);
             u |= ok ? 1 : 0;
             // canonical edges sit at the cell's lowest corner (offsets 0): this cell emits their quad
-            nq += (ok && ca == 0) ? 1 : 0;
+            nq += (ok && edge_a(k) == 0) ? 1 : 0;
         }
         qn[a] = nq;
     }
     used[a] = u;
     bnd[a] = on_box ? 1 : 0;
     if (cond_out) cond_out[a] = cond;
+}
+
+// QEF pass of a dense grid without an intersection set: each active cell's
+// crossing edges, points and normals are recomputed from its corner values.
+// Writes dual_v, used, bnd, qn as k_dc_qef with dense_used, and qmask[a]:
+// bit b (b = 0, 1, 2 for canonical edges 0, 3, 8) when the edge emits a quad,
+// bit b + 3 its orientation (value at the lower corner <= value at the upper).
+__global__ void k_dc_qef_grid(const int64_t *cell_dense, int64_t ncells, const CellSrc s1, float level, float reg,
+                              float svd_tol, float cond_fp64, float *dual_v, unsigned long long *n_fp64,
+                              uint8_t *used, uint8_t *bnd, uint8_t *qn, uint8_t *qmask, DcBatch bt) {
+    __shared__ int64_t starts[kMaxDcBatchGrids];
+    if (bt.items) {
+        for (int i = threadIdx.x; i < bt.n; i += kBlock) starts[i] = bt.items[i].cell0;
+        __syncthreads();
+    }
+    const int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    if (a >= ncells) return;
+    const int item = bt.items ? batch_item_of(bt, starts, a) : 0;
+    const CellSrc &s = bt.items ? bt.items[item].s : s1;
+    if (bt.items) level = bt.items[item].level;
+    int64_t x, y, z;
+    cell_xyz(s, cell_dense[a], x, y, z);
+    GridEdges en;
+    {
+        const float *r00 = s.values + (x * s.Y + y) * s.Z + z;
+        const float *r01 = r00 + s.Z;
+        const float *r10 = r00 + s.Y * s.Z;
+        const float *r11 = r10 + s.Z;
+        en.cv[0] = r00[0];
+        en.cv[1] = r00[1];
+        en.cv[2] = r01[0];
+        en.cv[3] = r01[1];
+        en.cv[4] = r10[0];
+        en.cv[5] = r10[1];
+        en.cv[6] = r11[0];
+        en.cv[7] = r11[1];
+    }
+    en.lo[0] = grid_coord(x, s.X, s.amin[0], s.amax[0]);
+    en.lo[1] = grid_coord(y, s.Y, s.amin[1], s.amax[1]);
+    en.lo[2] = grid_coord(z, s.Z, s.amin[2], s.amax[2]);
+    en.hi[0] = grid_coord(x + 1, s.X, s.amin[0], s.amax[0]);
+    en.hi[1] = grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]);
+    en.hi[2] = grid_coord(z + 1, s.Z, s.amin[2], s.amax[2]);
+    en.level = level;
+    const int st = crossing12(case_bits(en.cv, level));
+    en.n = 0;
+    uint8_t u = 0, nq = 0, qm = 0;
+    for (int k = 0; k < 12; ++k) {
+        if (!((st >> k) & 1)) continue;
+        en.e[en.n++] = (int8_t) k;
+        const bool ok = edge_interior(k, x, y, z, s);
+        u |= ok ? 1 : 0;
+        if (ok && edge_a(k) == 0) {
+            const int b = k == 0 ? 0 : (k == 3 ? 1 : 2);
+            ++nq;
+            qm |= (uint8_t) (1u << b);
+            if (en.cv[edge_a(k)] <= en.cv[edge_b(k)]) qm |= (uint8_t) (8u << b);
+        }
+    }
+    float out[3];
+    bool on_box = false;
+    if (en.n > 0) qef_cell(en, en.lo, en.hi, reg, svd_tol, cond_fp64, n_fp64, out, on_box);
+    else out[0] = out[1] = out[2] = 0.0f;   // not reached: active cells have a crossing edge
+    for (int k = 0; k < 3; ++k) dual_v[3 * a + k] = out[k];
+    used[a] = u;
+    bnd[a] = on_box ? 1 : 0;
+    qn[a] = nq;
+    qmask[a] = qm;
 }
 
 // ---- quads -------------------------------------------------------------------
@@ -636,7 +770,7 @@ __global__ void k_quad_emit(ItsIn in, CellSrc s1, ActHash ah, const int64_t *bsu
             const DcItem *it = &bt.items[item];
             if (entry_quad(in, it->s, ah, e, q, true, it->key_base, it->amask ? it : nullptr)) nq = 1;
         } else {
-            const DcItem one{s1, 0, 0, in.amask_tab, in.apre_tab, (s1.Z + 63) / 64};
+            const DcItem one{s1, 0, 0, in.amask_tab, in.apre_tab, (s1.Z + 63) / 64, 0.0f};
             if (entry_quad(in, s1, ah, e, q, true, 0, in.amask_tab && in.apre_tab && !s1.ids ? &one : nullptr)) nq = 1;
         }
     }
@@ -812,7 +946,7 @@ __global__ void k_quad_emit_cells(ItsIn in, CellSrc s1, const uint8_t *qn, const
     if (bt.items) {
         it = &bt.items[bt.cell_item[a]];
     } else {
-        one = DcItem{s1, 0, 0, in.amask_tab, in.apre_tab, (s1.Z + 63) / 64};
+        one = DcItem{s1, 0, 0, in.amask_tab, in.apre_tab, (s1.Z + 63) / 64, 0.0f};
         it = &one;
     }
     const ActHash ah{nullptr, nullptr, 0};
@@ -820,6 +954,63 @@ __global__ void k_quad_emit_cells(ItsIn in, CellSrc s1, const uint8_t *qn, const
     for (int64_t e = lo; e < hi; ++e) {
         int64_t q[4];
         if (!entry_quad(in, it->s, ah, e, q, true, it->key_base, it)) continue;
+        const float *v0 = dual_v + 3 * q[0], *v1 = dual_v + 3 * q[1], *v2 = dual_v + 3 * q[2],
+                    *v3 = dual_v + 3 * q[3];
+        int32_t t[6];
+        if (dist3(v0, v2) > dist3(v1, v3)) {
+            const int32_t u[6] = {(int32_t) q[1], (int32_t) q[3], (int32_t) q[0],
+                                  (int32_t) q[3], (int32_t) q[1], (int32_t) q[2]};
+            for (int k = 0; k < 6; ++k) t[k] = u[k];
+        } else {
+            const int32_t u[6] = {(int32_t) q[2], (int32_t) q[0], (int32_t) q[1],
+                                  (int32_t) q[0], (int32_t) q[2], (int32_t) q[3]};
+            for (int k = 0; k < 6; ++k) t[k] = u[k];
+        }
+        for (int k = 0; k < 6; ++k)
+            wd.out_f[6 * slot + k] = welded_id(wd.bnd, wd.rep_slot, wd.slot_min, wd.final_id, t[k]);
+        ++slot;
+    }
+}
+
+// Quads of a dense grid without an intersection set, one thread per active
+// cell: its canonical edges from qmask in edge order (0, 3, 8: the entry
+// order of the intersection set), neighbours from the lookup tables, the
+// reference's orientation and shorter-diagonal split, corners as welded
+// vertex ids at the cell's offset in the quad scan.
+__global__ void k_quad_emit_grid(const int64_t *cell_dense, int64_t ncells, CellSrc s1, const uint64_t *amask_tab,
+                                 const int32_t *apre_tab, const uint8_t *qn, const uint8_t *qmask,
+                                 const int64_t *bsum_pre, const float *dual_v, WeldIds wd, DcBatch bt) {
+    const int64_t a = (int64_t) blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nq = a < ncells ? qn[a] : 0;
+    int64_t ed, eq, td, tq;
+    block_scan2(0, nq, ed, eq, td, tq);
+    if (!nq) return;
+    int64_t slot = bsum_pre[2 * blockIdx.x + 1] + eq;
+    DcItem one;
+    const DcItem *it;
+    if (bt.items) {
+        it = &bt.items[bt.cell_item[a]];
+    } else {
+        one = DcItem{s1, 0, 0, amask_tab, apre_tab, (s1.Z + 63) / 64, 0.0f};
+        it = &one;
+    }
+    int64_t x, y, z;
+    cell_xyz(it->s, cell_dense[a], x, y, z);
+    const int qm = qmask[a];
+    for (int b = 0; b < 3; ++b) {
+        if (!((qm >> b) & 1)) continue;
+        const int axis = edge_axis(b == 0 ? 0 : (b == 1 ? 3 : 8));
+        int64_t q[4];
+        q[0] = a;
+        for (int j = 1; j < 4; ++j)
+            q[j] = active_of_tab(*it, x - kNb[axis][j][0], y - kNb[axis][j][1], z - kNb[axis][j][2]);
+        if (!((qm >> (b + 3)) & 1)) {
+            const int64_t t0 = q[0], t1 = q[1];
+            q[0] = q[3];
+            q[1] = q[2];
+            q[2] = t1;
+            q[3] = t0;
+        }
         const float *v0 = dual_v + 3 * q[0], *v1 = dual_v + 3 * q[1], *v2 = dual_v + 3 * q[2],
                     *v3 = dual_v + 3 * q[3];
         int32_t t[6];
@@ -1201,6 +1392,38 @@ static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float 
     return hipGetLastError();
 }
 
+// Dense grids without an intersection set (active cells and lookup tables
+// only): QEF from the corner values, weld, one scan for vertex and quad
+// offsets, quads per cell.
+static hipError_t dc_launch_grid(const CellSrc &s, const int64_t *cell_dense, int64_t ncells, const uint64_t *amask,
+                                 const int32_t *apre, float level, float reg, float svd_tol, float cond_fp64,
+                                 DcScratch &w, const DcBatch &bt, hipStream_t st) {
+    const unsigned nbv = nblocks(ncells);
+    hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, (int64_t *) w.n_fp64, (int64_t) 1, (int64_t) 0);
+    hipLaunchKernelGGL(k_dc_qef_grid, dim3(nbv), dim3(kBlock), 0, st, cell_dense, ncells, s, level, reg, svd_tol,
+                       cond_fp64, w.dual_v, w.n_fp64, w.used, w.bnd, w.qn, w.qmask, bt);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.hslots, w.hmask + 1, -1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.slot_min, w.hmask + 1,
+                       INT32_MAX);
+    hipLaunchKernelGGL(k_pos_insert, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, w.bnd, ncells, w.hslots,
+                       w.hmask, w.rep_slot, (const int32_t *) bt.cell_item);
+    hipLaunchKernelGGL(k_slot_min, dim3(nbv), dim3(kBlock), 0, st, w.used, w.bnd, ncells, w.rep_slot, w.slot_min);
+    hipLaunchKernelGGL(k_keep_flags, dim3(nbv), dim3(kBlock), 0, st, w.used, w.bnd, ncells, w.rep_slot, w.slot_min,
+                       (const uint8_t *) w.qn, w.bsum);
+    // tot[2] = kept vertices, tot[3] = quads; tot[0] = quads after the copy
+    scan_pairs(w.bsum, (int64_t) nbv, w.tot + 2, w.part, st);
+    hipLaunchKernelGGL(k_copy_i64, dim3(1), dim3(64), 0, st, w.tot, (const int64_t *) (w.tot + 3));
+    if (bt.items)
+        hipLaunchKernelGGL(k_item_vbase, dim3(bt.n), dim3(kBlock), 0, st, bt, w.used, w.bnd, w.rep_slot, w.slot_min,
+                           (const uint8_t *) w.qn, w.bsum, ncells, (const int64_t *) w.tot);
+    hipLaunchKernelGGL(k_keep_emit, dim3(nbv), dim3(kBlock), 0, st, w.dual_v, w.used, w.bnd, ncells, w.rep_slot,
+                       w.slot_min, w.bsum, w.final_id, w.out_v, w.out_vcell, bt);
+    hipLaunchKernelGGL(k_quad_emit_grid, dim3(nbv), dim3(kBlock), 0, st, cell_dense, ncells, s, amask, apre,
+                       (const uint8_t *) w.qn, (const uint8_t *) w.qmask, w.bsum, w.dual_v,
+                       WeldIds{w.bnd, w.rep_slot, w.slot_min, w.final_id, w.out_f}, bt);
+    return hipGetLastError();
+}
+
 hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, float cond_fp64, DcScratch &w,
                   int64_t &nv, int64_t &nf, int64_t &n_fp64, hipStream_t st) {
     CellSrc s = make_src(g);
@@ -1231,9 +1454,9 @@ hipError_t dc_run(const GridDesc &g, const ItsIn &in, float reg, float svd_tol, 
 size_t dc_batch_item_bytes() { return sizeof(DcItem); }
 
 hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const uint64_t *const *amask,
-                        const int32_t *const *apre, const ItsIn &in, float reg, float svd_tol, float cond_fp64,
-                        DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq, int64_t &n_fp64,
-                        hipStream_t st) {
+                        const int32_t *const *apre, const float *levels, const ItsIn &in, float reg, float svd_tol,
+                        float cond_fp64, DcScratch &w, const DcBatchDev &d, int64_t *nv, int64_t *nq,
+                        int64_t &n_fp64, hipStream_t st) {
     n_fp64 = 0;
     for (int i = 0; i < n; ++i) nv[i] = nq[i] = 0;
     if (n < 1 || n > kMaxDcBatchGrids) return hipErrorInvalidValue;
@@ -1267,12 +1490,18 @@ hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const ui
         hi[i].amask = amask ? amask[i] : nullptr;
         hi[i].apre = apre ? apre[i] : nullptr;
         hi[i].W = (g[i].Z + 63) / 64;
+        hi[i].level = levels ? levels[i] : 0.0f;
         if (g[i].sparse_ids) tables = false;
     }
+    if (levels && !tables) return hipErrorInvalidValue;   // the grid path needs the tables
     int64_t *rb = (int64_t *) (ps.p + ((up + 255) & ~size_t(255)));
     ISO_TRY(hipMemcpyAsync(d.items, hi, up, hipMemcpyHostToDevice, st));
     const DcBatch bt{(const DcItem *) d.items, n, tables, d.cell_item, d.vbase, d.qbase};
-    ISO_TRY(dc_launch(hi[0].s, in, reg, svd_tol, cond_fp64, w, bt, st));
+    if (levels)
+        ISO_TRY(dc_launch_grid(hi[0].s, in.cell_dense, in.ncells, nullptr, nullptr, 0.0f, reg, svd_tol, cond_fp64, w,
+                               bt, st));
+    else
+        ISO_TRY(dc_launch(hi[0].s, in, reg, svd_tol, cond_fp64, w, bt, st));
     ISO_TRY(hipMemcpyAsync(rb, d.vbase, sizeof(int64_t) * (size_t) (n + 1), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipMemcpyAsync(rb + n + 1, d.qbase, sizeof(int64_t) * (size_t) (n + 1), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipMemcpyAsync(rb + 2 * n + 2, w.n_fp64, sizeof(int64_t), hipMemcpyDeviceToHost, st));

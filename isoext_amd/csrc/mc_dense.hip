@@ -1541,6 +1541,9 @@ __device__ __forceinline__ void its_tile(const McParams &p, const ItsOut &o, int
             k = nth_set_bit(s.amask[u], jj - (int) s.na[u]);
             const int jc = u % NZC, r = u / NZC;
             cs = cell_case(its_unit_cells(s, r / TY, r % TY, jc), k);
+        }
+        if (live && o.points) {
+            const int jc = u % NZC, r = u / NZC;
             const float *r00 = p.values + ((x0 + r / TY) * p.Y + (y0 + r % TY)) * p.Z + (z0 + 64 * jc + k);
             const float *r01 = r00 + p.Z;
             const float *r10 = r00 + p.Y * p.Z;
@@ -1572,9 +1575,11 @@ __device__ __forceinline__ void its_tile(const McParams &p, const ItsOut &o, int
             const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + zl;
             const int64_t id = (gx * (p.Y - 1) + gy) * (p.Z - 1) + gz;
             if (a < p.cap_v) {   // capacities from the count pass (defensive)
-                o.cell_indices[a] = id;
+                if (o.cell_indices) o.cell_indices[a] = id;
                 o.cell_dense[a] = id;
-                o.cell_offsets[a] = e + o.e_base;
+                if (o.cell_offsets) o.cell_offsets[a] = e + o.e_base;
+            }
+            if (a < p.cap_v && o.points) {
                 const float lo[3] = {s.cx[cx], s.cy[cy], s.cz[zl]};
                 const float hi[3] = {s.cx[cx + 1], s.cy[cy + 1], s.cz[zl + 1]};
                 const int st = crossing_status(cs);

@@ -239,7 +239,17 @@ def dual_contouring(grid, level: float = 0.0, intersection: Intersection | None 
     Without `intersection`, intersections and normals are computed at
     `level`; an intersection without normals gets trilinear normals.
     Returns (vertices (N, 3) float32, faces (M, 3) int32) or (None, None).
+
+    On a UniformGrid without `intersection` no intersection set is stored:
+    each active cell's crossing edges, points and trilinear normals are
+    recomputed from its corner values (the same floats, in the same order, as
+    get_intersection(grid, level, True) holds), which gives the same mesh.
     """
+    from .grid import UniformGrid
+    if intersection is None and isinstance(grid, UniformGrid):
+        from .batch import GridJob, dc_batch
+        job = GridJob(grid.values, float(level), 0, tuple(grid.aabb_min), tuple(grid.aabb_max))
+        return dc_batch([job], reg=reg, svd_tol=svd_tol, ws_kind="dc_grid")[0]
     its = intersection if intersection is not None else get_intersection(grid, level, True)
     if its.points.shape[0] == 0:
         return None, None

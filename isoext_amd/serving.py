@@ -61,7 +61,7 @@ def footprint(req: Request) -> int:
 
 class Server:
     def __init__(self, device=0, hbm_budget: float = 200e9, max_batch: int = 32,
-                 max_batch_points: int = 1 << 30, workers: int = 4, cfg: int | None = None):
+                 max_batch_points: int = 1 << 30, workers: int = 6, cfg: int | None = None):
         self.device = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
         self.budget = int(hbm_budget)
         self.max_batch = max_batch

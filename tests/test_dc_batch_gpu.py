@@ -1,6 +1,8 @@
-"""Batched dual contouring: every grid's mesh in a batch equals
-dual_contouring(grid, level) on that grid alone, element for element (same
-vertex and face order), including grids without cells or crossings."""
+"""Batched dual contouring (no intersection set stored: each cell's edges are
+recomputed from its corner values): every grid's mesh in a batch equals dual
+contouring of that grid alone through a stored intersection set
+(get_intersection with normals, then the DC kernels), element for element
+(same vertex and face order), including grids without cells or crossings."""
 
 import numpy as np
 import pytest
@@ -11,9 +13,14 @@ pytestmark = pytest.mark.gpu
 
 
 def alone(job):
-    from isoext_amd.dc import dual_contouring
+    from isoext_amd.dc import dc_raw, get_intersection
     from isoext_amd.grid import UniformGrid
-    return dual_contouring(UniformGrid.wrap(job.values, job.aabb_min, job.aabb_max), level=job.level)
+    g = UniformGrid.wrap(job.values, job.aabb_min, job.aabb_max)
+    its = get_intersection(g, job.level, True)
+    if len(its) == 0:
+        return None, None
+    v, f, _ = dc_raw(g, its, its.normals, 1e-2, 1e-6)
+    return (None, None) if f.shape[0] == 0 else (v, f)
 
 
 def assert_same(got, ref, tag):
