@@ -140,8 +140,14 @@ def dc_batch(jobs: list[GridJob], reg: float = 1e-2, svd_tol: float = 1e-6, ws_k
         nv, nf, v0, f0 = (int(out_counts[4 * i + k]) for k in range(4))
         if nf == 0:
             res.append((None, None))
-        else:
-            res.append((out_v[v0:v0 + nv], out_f[f0:f0 + nf]))
+            continue
+        v, f = out_v[v0:v0 + nv], out_f[f0:f0 + nf]
+        if n == 1:   # one grid: do not pin buffers sized for the worst case behind a smaller mesh
+            if out_v.shape[0] > 2 * nv + (1 << 16):
+                v = v.clone()
+            if out_f.shape[0] > 2 * nf + (1 << 16):
+                f = f.clone()
+        res.append((v, f))
     return res
 
 

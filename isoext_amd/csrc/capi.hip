@@ -918,9 +918,14 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
             int64_t *keys1[1] = {out_key};
             HIP_CHECK(iso::weld_fixup_small_batch(1, &sp_dev, &out_v, &out_f, keys1, &nv, &nf, &g, st, fin),
                       "weld fix-up");
-            if (fin[2]) return fail(ISOEXT_E_DEVICE, "weld fix-up: inconsistent special lists");
-            nv = fin[0];
-            nf = fin[1];
+            if (fin[2] == iso::kFixSmallOverflow) {
+                HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key), "weld fix-up");
+            } else if (fin[2]) {
+                return fail(ISOEXT_E_DEVICE, "weld fix-up: inconsistent special lists");
+            } else {
+                nv = fin[0];
+                nf = fin[1];
+            }
         } else {
             HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key), "weld fix-up");
         }
@@ -1204,9 +1209,17 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
                                               s_nv.data(), s_nf.data(), s_g.data(), st, fin.data()),
                   "weld fix-up");
         for (size_t k = 0; k < small.size(); ++k) {
-            if (fin[3 * k + 2]) return fail(ISOEXT_E_DEVICE, "item %d: inconsistent special lists", small[k]);
-            counts[2 * small[k]] = fin[3 * k];
-            counts[2 * small[k] + 1] = fin[3 * k + 1];
+            const int i = small[k];
+            if (fin[3 * k + 2] == iso::kFixSmallOverflow) {   // lists too long for one workgroup
+                HIP_CHECK(iso::weld_fixup_async(s_sp[k], items[i].out_v, items[i].out_f, s_nv[k], s_nf[k], s_g[k], st,
+                                                nullptr, fix_dev + 4 * i),
+                          "weld fix-up");
+                fixed.push_back(i);
+                continue;
+            }
+            if (fin[3 * k + 2]) return fail(ISOEXT_E_DEVICE, "item %d: inconsistent special lists", i);
+            counts[2 * i] = fin[3 * k];
+            counts[2 * i + 1] = fin[3 * k + 1];
         }
     }
     if (!fixed.empty()) {

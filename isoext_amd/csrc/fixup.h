@@ -38,7 +38,11 @@ struct FixSmallItem {
     int64_t nslots, npslots;
     int64_t *final_dev;   // final vertex / face counts, inconsistency flag
 };
-constexpr int64_t kFixSmallMax = 2048;   // removed vertices and dropped faces the small path holds
+constexpr int64_t kFixSmallMax = 2048;     // removed vertices and dropped faces the small path holds
+constexpr int64_t kFixSmallSlots = 2048;   // candidate + special-edge entries the small path takes (one workgroup is slower beyond)
+// final[2] of an item whose removed vertices did not fit the small path: its
+// outputs are untouched and the multi-kernel fix-up must run instead
+constexpr int64_t kFixSmallOverflow = 2;
 bool fixup_small_ok(const SpecialLists &sp);
 // Scratch bytes of one small fix-up item and the item's table sizes.
 size_t fixup_small_scratch(const SpecialLists &sp, int64_t &nslots, int64_t &npslots);

@@ -487,19 +487,14 @@ __global__ void __launch_bounds__(kFixNT) k_fix_small(const FixSmallItem *items)
         else insert_cand(sp.cand, i - nscell, keys, info, mask, it.pslot, groups, pmask);
     }
     __syncthreads();
-    // kept faces: mark their vertices, point corners at representatives
+    // kept faces: mark their vertices (their corners are pointed at the
+    // representatives below, once the lists are known to fit)
     for (int64_t i = tid; i < nkept; i += kFixNT) {
         const int64_t fs = sp.kept[i];
         if (fs < 0 || fs >= nf) continue;
         for (int j = 0; j < 3; ++j) {
-            const int64_t vv = it.f[3 * fs + j];
-            const int64_t sl = h1_find_or_insert(keys, mask, vv, false);
-            if (sl < 0) continue;
-            info[sl].referenced = 1;
-            if (info[sl].is_cand && info[sl].group >= 0) {
-                const int64_t rep = groups[info[sl].group].rep;
-                if (rep != vv) it.f[3 * fs + j] = (int32_t) rep;
-            }
+            const int64_t sl = h1_find_or_insert(keys, mask, it.f[3 * fs + j], false);
+            if (sl >= 0) info[sl].referenced = 1;
         }
     }
     __syncthreads();
@@ -537,14 +532,28 @@ __global__ void __launch_bounds__(kFixNT) k_fix_small(const FixSmallItem *items)
         else bad = 1;
     }
     __syncthreads();
-    if (bad) {
+    if (bad) {   // the lists do not fit in LDS: outputs untouched, the multi-kernel fix-up takes over
         if (tid == 0) {
             it.final_dev[0] = nv;
             it.final_dev[1] = nf;
-            it.final_dev[2] = 1;
+            it.final_dev[2] = kFixSmallOverflow;
         }
         return;
     }
+    // kept faces: corners to the representatives
+    for (int64_t i = tid; i < nkept; i += kFixNT) {
+        const int64_t fs = sp.kept[i];
+        if (fs < 0 || fs >= nf) continue;
+        for (int j = 0; j < 3; ++j) {
+            const int64_t vv = it.f[3 * fs + j];
+            const int64_t sl = h1_find_or_insert(keys, mask, vv, false);
+            if (sl >= 0 && info[sl].is_cand && info[sl].group >= 0) {
+                const int64_t rep = groups[info[sl].group].rep;
+                if (rep != vv) it.f[3 * fs + j] = (int32_t) rep;
+            }
+        }
+    }
+    __syncthreads();
     const int kv = nrv, kf = nhf;
     lds_sort(rv, kv, pow2_at_least_dev(max(kv, 1)));
     lds_sort(hf, kf, pow2_at_least_dev(max(kf, 1)));
@@ -598,7 +607,7 @@ __global__ void __launch_bounds__(kFixNT) k_fix_small(const FixSmallItem *items)
 }
 
 bool fixup_small_ok(const SpecialLists &sp) {
-    return (int64_t) (sp.n_cand + sp.n_scell) <= kFixSmallMax && (int64_t) sp.n_hole <= kFixSmallMax;
+    return (int64_t) (sp.n_cand + sp.n_scell) <= kFixSmallSlots && (int64_t) sp.n_hole <= kFixSmallMax;
 }
 
 size_t fixup_small_scratch(const SpecialLists &sp, int64_t &nslots, int64_t &npslots) {

@@ -29,7 +29,7 @@ def assert_same(got, ref, tag):
         return
     assert got[0] is not None, tag
     assert got[0].shape == ref[0].shape and got[1].shape == ref[1].shape, tag
-    assert np.array_equal(got[0].cpu().numpy(), ref[0].cpu().numpy()), tag
+    assert np.array_equal(got[0].cpu().numpy(), ref[0].cpu().numpy(), equal_nan=True), tag
     assert np.array_equal(got[1].cpu().numpy(), ref[1].cpu().numpy()), tag
 
 
@@ -94,3 +94,29 @@ def test_dc_batch_repeated_calls_and_one_grid(gpu):
         assert_same(first[k], ref, ("first", k))
         assert_same(third[k], ref, ("third", k))
     assert_same(second[0], alone(jobs[2]), "second")
+
+
+def test_dc_grid_path_special_values(gpu):
+    """dual_contouring(grid) without an intersection (no stored set) equals
+    the stored-set path on grids with NaN / inf values, values exactly at the
+    level, a reversed box and a one-cell-thick slab."""
+    import torch
+
+    import isoext_amd as iso
+    from isoext_amd.batch import GridJob
+    rng = np.random.default_rng(41)
+    base = multi_sphere_field((30, 31, 29), 4, seed=3)
+    odd = base.copy()
+    odd[rng.random(odd.shape) < 0.01] = np.nan
+    odd[rng.random(odd.shape) < 0.005] = np.inf
+    snapped = np.round(base * 8.0).astype(np.float32) / np.float32(8.0)   # many values exactly at 0
+    cases = [
+        GridJob(torch.from_numpy(odd).cuda()),
+        GridJob(torch.from_numpy(snapped).cuda()),
+        GridJob(torch.from_numpy(base).cuda(), 0.0, 0, (1.0, 1.0, 1.0), (-1.0, -1.0, -1.0)),
+        GridJob(torch.from_numpy(multi_sphere_field((2, 40, 40), 3, seed=8)).cuda()),
+    ]
+    for k, j in enumerate(cases):
+        g = iso.UniformGrid.wrap(j.values, j.aabb_min, j.aabb_max)
+        got = iso.dual_contouring(g, level=j.level)
+        assert_same(got, alone(j), k)
