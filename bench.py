@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fused", action="store_true", help="skip the fused sphere-field extraction")
     ap.add_argument("--cpu-planes", type=int, default=129, help="x-planes in the CPU sample")
-    ap.add_argument("--serving", type=int, default=1000,
+    ap.add_argument("--serving", type=int, default=10000,
                     help="requests in the serving comparison (0 = skip)")
     ap.add_argument("--workload", default="mc1024", choices=["mc1024", "dc256", "sparse2048"],
                     help="mc1024: plain MC at 1024^3 (default); dc256: dual contouring of the 256^3 CSG "
@@ -365,11 +365,22 @@ def serving_bench(n_requests: int, seed: int = 0):
     # warm-up both paths (library load, workspaces grown to their steady-state
     # sizes, autotune tables): a full saturated stream of another seed through
     # the server, a fifth of one through the plain path
+    def consume(r):   # the client takes each mesh as it completes (10k meshes would hold ~70 GB)
+        r.result = None
+
+    def drop(reqs):
+        for r in reqs:
+            r.result = None
+
     warm = stream(0.0, seed + 99)
-    srv = serving.Server(hbm_budget=200e9, max_batch=64, workers=4)
+    # defaults: 32 requests / 2^30 points per batch, 6 workers
+    srv = serving.Server(hbm_budget=200e9, on_done=consume)
     serving.drive(srv, warm)
+    drop(warm)
     for r in warm[: max(16, n_requests // 5)]:
         serving.run_plain(r)
+    drop(warm)
+    del warm
     torch.cuda.synchronize()
     # saturation throughput: every request available at t = 0; the median of
     # three passes over the same stream for each path
@@ -378,18 +389,22 @@ def serving_bench(n_requests: int, seed: int = 0):
         sat = stream(0.0, seed)
         w, lat_sat = serving.drive(srv, sat)
         walls_b.append(w)
+        drop(sat)
     for _ in range(3):
         plain = stream(0.0, seed)
-        w, _ = serving.drive_plain(plain)
+        w, _ = serving.drive_plain(plain, on_done=consume)
         walls_p.append(w)
+        drop(plain)
     wall_b, wall_p = sorted(walls_b)[1], sorted(walls_p)[1]
     thr_b, thr_p = n_requests / wall_b, n_requests / wall_p
     # latency under a Poisson load at 80% of the plain path's capacity
     rate = 0.8 * thr_p
     lb = stream(rate, seed + 1)
     _, lat_b = serving.drive(srv, lb)
+    drop(lb)
     lp = stream(rate, seed + 1)
-    _, lat_p = serving.drive_plain(lp)
+    _, lat_p = serving.drive_plain(lp, on_done=consume)
+    drop(lp)
     stats = dict(srv.stats)
     srv.close()
     q = lambda a, x: float(np.percentile(a, x) * 1e3)

@@ -347,9 +347,7 @@ __device__ __forceinline__ void jacobi_sym3(T a[3][3], T w[3], T v[3][3], int sw
 // Truncated pseudo-inverse x = sum_j (v_j . b / w_j) v_j over |w_j| > tol*max|w|
 // (the reference's rule, src/batched_la.cu:160-175).  Returns cond = max/min |w|.
 template <typename T>
-__device__ __forceinline__ T pinv3(T A[3][3], const T b[3], T tol, T x[3], int sweeps) {
-    T w[3], v[3][3];
-    jacobi_sym3(A, w, v, sweeps);
+__device__ __forceinline__ T pinv3_apply(const T w[3], const T v[3][3], const T b[3], T tol, T x[3]) {
     T wmax = fmax(fabs(w[0]), fmax(fabs(w[1]), fabs(w[2])));
     T wmin = fmin(fabs(w[0]), fmin(fabs(w[1]), fabs(w[2])));
     T thr = tol * wmax;
@@ -363,6 +361,13 @@ __device__ __forceinline__ T pinv3(T A[3][3], const T b[3], T tol, T x[3], int s
         for (int i = 0; i < 3; ++i) x[i] += c * v[i][j];
     }
     return wmin > T(0) ? wmax / wmin : T(INFINITY);
+}
+
+template <typename T>
+__device__ __forceinline__ T pinv3(T A[3][3], const T b[3], T tol, T x[3], int sweeps) {
+    T w[3], v[3][3];
+    jacobi_sym3(A, w, v, sweeps);
+    return pinv3_apply(w, v, b, tol, x);
 }
 
 // Entries of one cell read from an intersection set.
@@ -435,20 +440,19 @@ __device__ float qef_cell(const Ent &en, const float blo[3], const float bhi[3],
     float Ac[3][3];
     for (int p = 0; p < 3; ++p)
         for (int q = 0; q < 3; ++q) Ac[p][q] = A[p][q];
-    float xs[3];
-    float cond = pinv3<float>(Ac, r, svd_tol, xs, 6);
+    float xs[3], ew[3], ev[3][3];
+    jacobi_sym3<float>(Ac, ew, ev, 6);
+    float cond = pinv3_apply<float>(ew, ev, r, svd_tol, xs);
     // one step of iterative refinement: residual r - A x in FP64 (removes the
-    // eigen-solver's error), correction solved in FP32
+    // eigen-solver's error), correction solved in FP32 with the same
+    // decomposition
     {
         float res[3];
         for (int p = 0; p < 3; ++p)
             res[p] = (float) ((double) r[p] - ((double) A[p][0] * xs[0] + (double) A[p][1] * xs[1] +
                                                (double) A[p][2] * xs[2]));
-        float A2[3][3];
-        for (int p = 0; p < 3; ++p)
-            for (int q = 0; q < 3; ++q) A2[p][q] = A[p][q];
         float dx[3];
-        pinv3<float>(A2, res, svd_tol, dx, 6);
+        pinv3_apply<float>(ew, ev, res, svd_tol, dx);
         for (int p = 0; p < 3; ++p) xs[p] += dx[p];
     }
     for (int k = 0; k < 3; ++k) out[k] = c[k] + xs[k] * h;

@@ -61,7 +61,11 @@ def footprint(req: Request) -> int:
 
 class Server:
     def __init__(self, device=0, hbm_budget: float = 200e9, max_batch: int = 32,
-                 max_batch_points: int = 1 << 30, workers: int = 6, cfg: int | None = None):
+                 max_batch_points: int = 1 << 30, workers: int = 6, cfg: int | None = None, on_done=None):
+        """on_done(request), if given, runs on the worker thread when a
+        request completes (a client that consumes each mesh as it arrives,
+        e.g. copies it out and drops it)."""
+        self.on_done = on_done
         self.device = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
         self.budget = int(hbm_budget)
         self.max_batch = max_batch
@@ -169,6 +173,8 @@ class Server:
             t = self.now()
             for r in work:
                 r.t_done = t
+                if self.on_done is not None:
+                    self.on_done(r)
                 r.done.set()
             self._release(work)
 
@@ -263,9 +269,10 @@ def drive(server: Server, reqs):
     return wall, lat
 
 
-def drive_plain(reqs):
+def drive_plain(reqs, on_done=None):
     """The per-call baseline on the same stream: one request at a time, in
-    arrival order, each started no earlier than its arrival."""
+    arrival order, each started no earlier than its arrival; on_done as for
+    Server."""
     t0 = time.perf_counter()
     lat = []
     for r in reqs:
@@ -275,4 +282,6 @@ def drive_plain(reqs):
         r.result = run_plain(r)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t0 - r.arrival)
+        if on_done is not None:
+            on_done(r)
     return time.perf_counter() - t0, np.array(lat)
