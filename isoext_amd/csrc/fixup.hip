@@ -295,6 +295,37 @@ __device__ int64_t bit_select(const uint32_t *bits, const int64_t *pre, int64_t 
     return -1;
 }
 
+// bit_select by one whole wave (every lane returns the same value): a 64-ary
+// search over the word prefix -- each round the lanes test 64 words spread
+// over the remaining range and a ballot keeps the span between the last word
+// whose prefix is <= i and the next -- so a bitmap of 2^17 words takes three
+// rounds of independent loads instead of 17 dependent ones.
+__device__ int64_t bit_select_wave(const uint32_t *bits, const int64_t *pre, int64_t nw, int64_t i, bool ones) {
+    const int lane = threadIdx.x & 63;
+    int64_t lo = 0, hi = nw;   // before(lo) <= i; hi == nw or before(hi) > i
+    while (hi - lo > 1) {
+        const int64_t span = hi - lo - 1;                  // candidates lo + 1 .. hi - 1
+        const int64_t w = lo + 1 + (span * lane) / 64;     // non-decreasing in the lane
+        const bool ok = w < hi && (ones ? pre[w] : 32 * w - pre[w]) <= i;
+        const uint64_t m = __ballot(ok);                    // lanes 0 .. c-1 (the predicate is monotone)
+        const int c = __popcll(m);
+        const int64_t lo2 = c > 0 ? lo + 1 + (span * (c - 1)) / 64 : lo;
+        const int64_t hi2 = c < 64 ? lo + 1 + (span * c) / 64 : hi;
+        lo = lo2;
+        hi = hi2 > lo2 ? hi2 : lo2 + 1;
+    }
+    const int64_t before = ones ? pre[lo] : 32 * lo - pre[lo];
+    const uint32_t wd = ones ? bits[lo] : ~bits[lo];
+    int64_t r = i - before;
+    for (int b = 0; b < 32; ++b) {
+        if ((wd >> b) & 1) {
+            if (r == 0) return 32 * lo + b;
+            --r;
+        }
+    }
+    return -1;
+}
+
 // Tail relocation: the i-th removed slot below n_final takes the i-th kept
 // element at or above n_final (disjoint ranges, so the moves do not race);
 // n_final is read from the device.  The pairs are recorded sorted by source
@@ -311,10 +342,15 @@ __global__ void k_reloc(const uint32_t *keep, const int64_t *pre, int64_t nw, in
     const uint32_t wd = keep[n_final >> 5];
     const uint32_t below = (n_final & 31) ? (wd & ((1u << (n_final & 31)) - 1)) : 0u;
     const int64_t holes_low = n_final - (pre[n_final >> 5] + __popc(below));
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t) gridDim.x * blockDim.x) {
+    // one wave per element (the searches are wave-cooperative)
+    const int64_t wave = ((int64_t) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwave = ((int64_t) gridDim.x * blockDim.x) >> 6;
+    const bool lead = (threadIdx.x & 63) == 0;
+    for (int64_t i = wave; i < k; i += nwave) {
         if (i < holes_low) {
-            const int64_t d = bit_select(keep, pre, nw, i, false);
-            const int64_t sidx = bit_select(keep, pre, nw, (n_final - holes_low) + i, true);
+            const int64_t d = bit_select_wave(keep, pre, nw, i, false);
+            const int64_t sidx = bit_select_wave(keep, pre, nw, (n_final - holes_low) + i, true);
+            if (!lead) continue;
             dst[i] = d;
             src[i] = sidx;
             if (v) {
@@ -328,7 +364,7 @@ __global__ void k_reloc(const uint32_t *keep, const int64_t *pre, int64_t nw, in
                 f[3 * d + 1] = f[3 * sidx + 1];
                 f[3 * d + 2] = f[3 * sidx + 2];
             }
-        } else {
+        } else if (lead) {
             dst[i] = -1;
             src[i] = INT64_MAX;
         }
@@ -380,6 +416,8 @@ __global__ void k_remap_tail(int32_t *f, int64_t nf, const int64_t *nv_final_p, 
 }
 
 unsigned grid_for(int64_t n) { return (unsigned) std::max<int64_t>(1, std::min<int64_t>(4096, (n + 255) / 256)); }
+// k_reloc: one wave (of four per workgroup) per element.
+unsigned reloc_grid(int64_t n) { return (unsigned) std::max<int64_t>(1, std::min<int64_t>(4096, (n + 3) / 4)); }
 
 int64_t pow2_at_least(int64_t n) {
     int64_t p = 1;
@@ -801,11 +839,11 @@ hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t
     // vertices: tail survivors fill the removed slots; faces: representatives
     // were applied above, relocated tail vertices now; then tail faces fill
     // the dropped slots
-    hipLaunchKernelGGL(k_reloc, dim3(grid_for(nrel)), dim3(256), 0, st, vkeep, vpre, vw, nv, misc + 2, nrel, rv_src,
+    hipLaunchKernelGGL(k_reloc, dim3(reloc_grid(nrel)), dim3(256), 0, st, vkeep, vpre, vw, nv, misc + 2, nrel, rv_src,
                        rv_dst, v, vkeys, (int32_t *) nullptr, misc + 4);
     hipLaunchKernelGGL(k_remap_tail, dim3(grid_for(nf)), dim3(256), 0, st, f, nf, misc + 2, nv, nrel, rv_src, rv_dst,
                        g);
-    hipLaunchKernelGGL(k_reloc, dim3(grid_for(nfrel)), dim3(256), 0, st, fkeep, fpre, fw, nf, misc + 3, nfrel,
+    hipLaunchKernelGGL(k_reloc, dim3(reloc_grid(nfrel)), dim3(256), 0, st, fkeep, fpre, fw, nf, misc + 3, nfrel,
                        rf_src, rf_dst, (float *) nullptr, (int64_t *) nullptr, f, misc + 4);
     ISO_TRY(hipGetLastError());
     if (final_dev) {

@@ -140,6 +140,11 @@ hipError_t its_dense_count_launch(McParams &p, hipStream_t stream);
 hipError_t its_dense_emit_launch(int cfg, McParams &p, const ItsOut &o, hipStream_t stream);
 
 constexpr int kMaxBatch = 256;        // items per batch
+// Tile configuration of batched intersection passes (16 x 16 cell rows x 2
+// words): in a batch every grid's tiles share one launch, so the
+// single-grid rule (enough tiles to fill the GPU) does not apply, and this
+// configuration measured fastest over a mix of 32^3 to 512^3 grids.
+constexpr int kBatchTileConfig = 2;
 constexpr int kMaxSpSegBatch = 16;    // special-list segments per batch item
 
 int mc_config_count();

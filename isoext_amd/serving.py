@@ -179,9 +179,9 @@ class Server:
             self._release(work)
 
     def _run_mc(self, reqs):
-        from .batch import GridJob, mc_batch
+        from .batch import BATCH_CFG, GridJob, mc_batch
         jobs = [GridJob(r.values, r.level, r.method) for r in reqs]
-        out = mc_batch(jobs, cfg=self.cfg, ws_kind="serving")
+        out = mc_batch(jobs, cfg=self.cfg if self.cfg is not None else BATCH_CFG, ws_kind="serving")
         for r, (v, f) in zip(reqs, out):
             r.result = (v, f) if f.shape[0] > 0 else (None, None)
         with self._lock:

@@ -39,7 +39,7 @@ def make_jobs(seed, n):
     return jobs, fields
 
 
-@pytest.mark.parametrize("cfg", [0, 3])
+@pytest.mark.parametrize("cfg", [0, 2, 3])
 def test_batch_equals_alone_and_oracle(gpu, cfg):
     from isoext_amd.batch import mc_batch
     jobs, fields = make_jobs(7 + cfg, 24)
@@ -87,7 +87,7 @@ def test_batch_mixed_tile_configs(gpu):
         t = torch.empty(s, dtype=torch.float32, device="cuda")
         workloads.multi_sphere(s, 6, 100 + k, out=t)
         jobs.append(GridJob(t, 0.0, k % 2))
-    out = mc_batch(jobs)
+    out = mc_batch(jobs, cfg=None)   # each grid its own configuration
     for j, (v, f) in zip(jobs, out):
         got = canon(v, f)
         ref = alone(j.values, j.level, j.method)
