@@ -27,6 +27,35 @@ ISO_HD constexpr int edge_a(int e) { return (int) ((kEdgeAPacked >> (4 * e)) & 1
 ISO_HD constexpr int edge_b(int e) { return (int) ((kEdgeBPacked >> (4 * e)) & 15); }
 ISO_HD constexpr int edge_axis(int e) { return (int) ((kEdgeAxisPacked >> (2 * e)) & 3); }
 
+// Inclusive scan of `v` over the 64 lanes of a wave (lane i gets
+// op(v_0, ..., v_i)) with DPP lane moves instead of LDS permutes: four
+// row_shr steps scan each 16-lane row, then row_bcast:15 carries a row's
+// last lane into the next row (used by rows 1 and 3) and row_bcast:31
+// carries lane 31 into rows 2 and 3.  All 64 lanes must be active.
+template <class Op>
+__device__ __forceinline__ int wave_incl_scan(int v, Op op) {
+    const int lane = (int) __lane_id(), r = lane & 15;
+    int t = __builtin_amdgcn_mov_dpp(v, 0x111, 0xf, 0xf, false);
+    if (r >= 1) v = op(v, t);
+    t = __builtin_amdgcn_mov_dpp(v, 0x112, 0xf, 0xf, false);
+    if (r >= 2) v = op(v, t);
+    t = __builtin_amdgcn_mov_dpp(v, 0x114, 0xf, 0xf, false);
+    if (r >= 4) v = op(v, t);
+    t = __builtin_amdgcn_mov_dpp(v, 0x118, 0xf, 0xf, false);
+    if (r >= 8) v = op(v, t);
+    t = __builtin_amdgcn_mov_dpp(v, 0x142, 0xf, 0xf, false);
+    if (lane & 16) v = op(v, t);
+    t = __builtin_amdgcn_mov_dpp(v, 0x143, 0xf, 0xf, false);
+    if (lane >= 32) v = op(v, t);
+    return v;
+}
+__device__ __forceinline__ int wave_incl_sum(int v) {
+    return wave_incl_scan(v, [](int a, int b) { return a + b; });
+}
+__device__ __forceinline__ int wave_incl_max(int v) {
+    return wave_incl_scan(v, [](int a, int b) { return a > b ? a : b; });
+}
+
 ISO_HD int32_t f2i_bits(float f) {
     union {
         float f;

@@ -314,15 +314,7 @@ __device__ __forceinline__ void block_exscan2(uint32_t &a, uint32_t &b, uint32_t
                                               uint32_t *sa, uint32_t *sb) {
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t ia = a, ib = b;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t ya = __shfl_up(ia, o), yb = __shfl_up(ib, o);
-        if (lane >= o) {
-            ia += ya;
-            ib += yb;
-        }
-    }
+    const uint32_t ia = (uint32_t) wave_incl_sum((int) a), ib = (uint32_t) wave_incl_sum((int) b);
     if (lane == 63) {
         sa[wave] = ia;
         sb[wave] = ib;
@@ -411,25 +403,14 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
         // round (owner lane by a running maximum over marked first slots),
         // so a word with many active cells does not serialise its wave
         const int cnt = __popcll(act);
-        int incl = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o);
-            if (lane >= o) incl += t;
-        }
+        const int incl = wave_incl_sum(cnt);
         const int total = __shfl(incl, 63), excl = incl - cnt;
         for (int base = 0; base < total; base += 64) {
             mark[wave][lane] = -1;
             __builtin_amdgcn_wave_barrier();
             if (cnt > 0 && excl < base + 64 && incl > base) mark[wave][max(excl, base) - base] = lane;
             __builtin_amdgcn_wave_barrier();
-            int own = mark[wave][lane];
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int t = __shfl_up(own, o);
-                if (lane >= o) own = max(own, t);
-            }
-            own = max(own, 0);
+            const int own = max(wave_incl_max(mark[wave][lane]), 0);
             const int k = base + lane;
             const uint64_t oact = __shfl(act, own);
             const uint64_t w00 = __shfl(b, own), w01 = __shfl(by, own), w10 = __shfl(bx, own), w11 = __shfl(bxy, own);
@@ -795,12 +776,7 @@ __device__ __forceinline__ void block_exscan_lds(uint32_t *a, uint32_t *tsum) {
     const int lo = tid * PER, hi = min(lo + PER, N);
     uint32_t sum = 0;
     for (int i = lo; i < hi; ++i) sum += a[i];
-    uint32_t inc = sum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(inc, o);
-        if (lane >= o) inc += y;
-    }
+    const uint32_t inc = (uint32_t) wave_incl_sum((int) sum);
     if (lane == 63) tsum[wave] = inc;
     __syncthreads();
     uint32_t run = 0, tot = 0;
@@ -824,12 +800,7 @@ template <int NT>
 __device__ __forceinline__ void block_maxscan_lds(int32_t *a, uint32_t *tsum) {
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int32_t m = a[tid];
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int32_t y = __shfl_up(m, o);
-        if (lane >= o) m = max(m, y);
-    }
+    int32_t m = wave_incl_max(a[tid]);
     if (lane == 63) tsum[wave] = (uint32_t) m;
     __syncthreads();
 #pragma unroll
