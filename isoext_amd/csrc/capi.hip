@@ -507,6 +507,7 @@ int batch_upload(int n, const isoext_mc_item *items, char *ws, const BatchLayout
     for (int i = 0; i < n; ++i) {
         fill_item(items[i], B.it[i], ws, w0, b0, B.it[i].tile0, it[i]);
         it[i].p.nzc = B.it[i].nzc;
+        iso::mc_params_divisors(it[i].p);
         if (its_caps) {
             iso::McParams &p = it[i].p;
             p.its = 1;

@@ -83,6 +83,27 @@ ISO_HD int64_t div_fast(int64_t a, int64_t b) {
     return q;
 }
 
+// Unsigned division of any 32-bit dividend by an invariant divisor d in
+// [1, 2^31) with a multiply-high, an add and two shifts (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", PLDI
+// 1994, section 4): l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// t = mulhi(m, a), a / d = (t + ((a - t) >> min(l, 1))) >> max(l - 1, 0).
+struct UDiv32 {
+    uint32_t m = 0;
+    int s1 = 0, s2 = 0;
+    UDiv32() = default;
+    ISO_HD explicit UDiv32(uint32_t d) {
+        const int l = d > 1 ? 32 - __builtin_clz(d - 1) : 0;
+        m = (uint32_t) (div_fast((int64_t) (((1ull << l) - d) << 32), (int64_t) d) + 1);
+        s1 = l > 0 ? 1 : 0;
+        s2 = l > 0 ? l - 1 : 0;
+    }
+    ISO_HD uint32_t div(uint32_t a) const {
+        const uint32_t t = (uint32_t) (((uint64_t) m * a) >> 32);
+        return (t + ((a - t) >> s1)) >> s2;
+    }
+};
+
 ISO_HD float lerp1(float t, float a, float b) {
     float s = 1.0f - t;
     float u = s * a;

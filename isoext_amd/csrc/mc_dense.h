@@ -102,8 +102,16 @@ struct McParams {
     // needed with the sphere-union generator's formula
     const float4 *spheres;
     int nsph;
+    // chunk index -> (row, chunk) -> (x, y) and chunk -> z-segment by
+    // multiply-high, valid when narrow (X * Y * W < 2^32); set by
+    // mc_params_divisors once W, Y and nzc are final
+    int narrow;
+    UDiv32 div_w, div_y, div_zc;
 };
 constexpr int kMaxFusedSpheres = 256;
+
+// Fill McParams' invariant-divisor fields from X, Y, W and nzc.
+void mc_params_divisors(McParams &p);
 
 // Stage timings (ms) of the last timed launch: classify, count, scan, emit,
 // and the whole extraction (first to last kernel).

@@ -364,29 +364,46 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
     const int64_t YW = p.Y * p.W;
     const int lane = tid & 63, wave = tid >> 6;
     __shared__ int32_t mark[NT / 64][64];
+    const bool narrow = p.narrow;
     for (int64_t gb = g0; gb < g1; gb += NT) {   // every lane runs every round (wave-wide steps below)
         const int64_t g = gb + tid;
         const bool valid = g < g1;
         const int64_t gg = valid ? g : g0;
-        const int64_t r = div_fast(gg, p.W), c = gg - r * p.W;
-        const int64_t x = div_fast(r, p.Y), y = r - x * p.Y;
+        int64_t r, c, x, y, seg;
+        if (narrow) {
+            const uint32_t g32 = (uint32_t) gg, r32 = p.div_w.div(g32), c32 = g32 - r32 * (uint32_t) p.W,
+                           x32 = p.div_y.div(r32);
+            r = r32;
+            c = c32;
+            x = x32;
+            y = r32 - x32 * (uint32_t) p.Y;
+            seg = p.div_zc.div(c32);
+        } else {
+            r = div_fast(gg, p.W);
+            c = gg - r * p.W;
+            x = div_fast(r, p.Y);
+            y = r - x * p.Y;
+            seg = div_fast(c, p.nzc);
+        }
         const bool hasx = x + 1 < p.X, hasy = y + 1 < p.Y, hn = c + 1 < p.W;
         const int64_t zlo = 64 * c;
+        // all eight words are loaded before any is used: one memory round
+        // trip per round instead of two
+        const bool hxy = hasx && hasy, hxyn = hxy && hn;
         const uint64_t *q = p.bits + gg;
         const uint64_t b = q[0], bn = hn ? q[1] : 0ull;
         const uint64_t bx = hasx ? q[YW] : 0ull, by = hasy ? q[p.W] : 0ull;
+        const uint64_t bxy = hxy ? q[YW + p.W] : 0ull;
+        const uint64_t byn = hxyn ? q[p.W + 1] : 0ull, bxn = hxyn ? q[YW + 1] : 0ull,
+                       bxyn = hxyn ? q[YW + p.W + 1] : 0ull;
         uint32_t cv = p.its ? 0u : (uint32_t) chunk_vertices(b, bn & 1, bx, by, hasx, hasy, zlo, p.Z);
         uint32_t cf = 0;
-        uint64_t act = 0, bxy = 0;
-        uint32_t nxt = 0;   // bit 0 of the next word of rows (0,0), (0,1), (1,0), (1,1)
-        if (hasx && hasy) {
-            bxy = q[YW + p.W];
-            const uint64_t byn = hn ? q[p.W + 1] : 0ull, bxn = hn ? q[YW + 1] : 0ull,
-                           bxyn = hn ? q[YW + p.W + 1] : 0ull;
-            const CellMasks m = cell_masks(b, bn, by, byn, bx, bxn, bxy, bxyn);
-            act = cell_active(m) & low_bits(p.Z - 1 - zlo);
-            nxt = (uint32_t) (bn & 1) | (uint32_t) (byn & 1) << 1 | (uint32_t) (bxn & 1) << 2 |
-                  (uint32_t) (bxyn & 1) << 3;
+        const CellMasks m = cell_masks(b, bn, by, byn, bx, bxn, bxy, bxyn);
+        uint64_t act = hxy ? cell_active(m) & low_bits(p.Z - 1 - zlo) : 0ull;
+        // bit 0 of the next word of rows (0,0), (0,1), (1,0), (1,1)
+        const uint32_t nxt = (uint32_t) (bn & 1) | (uint32_t) (byn & 1) << 1 | (uint32_t) (bxn & 1) << 2 |
+                             (uint32_t) (bxyn & 1) << 3;
+        if (hxy) {
             if (p.its) {
                 // crossing edges of the active cells, bit-parallel over the 64
                 // cells: edge e crosses where its two corner masks differ
@@ -395,7 +412,7 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
             }
         }
         if (!valid) act = 0;
-        const int e = (int) (r * p.S + div_fast(c, p.nzc) - i0);
+        const int e = (int) (r * p.S + seg - i0);
         if (valid && cv) atomicAdd(&ent_v[e], cv);
         if (valid && cf) atomicAdd(&ent_f[e], cf);
         if (p.its) continue;
@@ -1700,8 +1717,18 @@ static void launch_classify(const McParams &p, hipStream_t stream) {
     }
 }
 
+void mc_params_divisors(McParams &p) {
+    p.narrow = p.X > 0 && p.Y > 0 && p.W > 0 && p.nzc > 0 && p.X * p.Y * p.W <= (int64_t) UINT32_MAX;
+    if (p.narrow) {
+        p.div_w = UDiv32((uint32_t) p.W);
+        p.div_y = UDiv32((uint32_t) p.Y);
+        p.div_zc = UDiv32((uint32_t) p.nzc);
+    }
+}
+
 hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
                            hipEvent_t *ev) {
+    mc_params_divisors(p);
     hipError_t e = mc_upload_luts();
     if (e != hipSuccess) return e;
     auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], stream) : hipSuccess; };
@@ -1744,6 +1771,7 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
 }
 
 hipError_t its_dense_count_launch(McParams &p, hipStream_t stream) {
+    mc_params_divisors(p);
     hipError_t e = mc_upload_luts();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_its_reset, dim3(1), dim3(64), 0, stream, p.err, p.totals);
