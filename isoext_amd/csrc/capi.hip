@@ -55,7 +55,7 @@ struct McLayout {
     int64_t W, S, NI, NJ, NK, nblk;
     size_t o_bits, o_cnt_v, o_cnt_f, o_bsum_v, o_bsum_f, o_blk_v, o_blk_f;
     size_t o_ctr, o_scand, o_sscell, o_skept, o_shole;
-    size_t o_cand, o_scell, o_kept, o_hole, total;
+    size_t o_cand, o_scell, o_kept, o_hole, o_tflag, o_tlist, o_tcnt, total;
     int64_t cap, seg_cap[4];
 };
 
@@ -99,6 +99,9 @@ bool mc_layout(const int64_t shape[3], int cfg, int64_t cap, McLayout &L) {
     L.o_scell = take(sizeof(iso::SpecialEdge) * iso::kSpSeg * L.seg_cap[1]);
     L.o_kept = take(sizeof(int64_t) * iso::kSpSeg * L.seg_cap[2]);
     L.o_hole = take(sizeof(int64_t) * iso::kSpSeg * L.seg_cap[3]);
+    L.o_tflag = take(sizeof(int32_t) * L.NI * L.NJ * L.NK);
+    L.o_tlist = take(sizeof(int32_t) * L.NI * L.NJ * L.NK);
+    L.o_tcnt = take(sizeof(int32_t));
     L.total = o;
     return true;
 }
@@ -861,6 +864,11 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     p.seg.scell = (iso::SpecialEdge *) (ws + L.o_sscell);
     p.seg.kept = (int64_t *) (ws + L.o_skept);
     p.seg.hole = (int64_t *) (ws + L.o_shole);
+    p.tile_flag = (int *) (ws + L.o_tflag);
+    p.tile_list = (int *) (ws + L.o_tlist);
+    p.tile_count = (int *) (ws + L.o_tcnt);
+    p.tx = L.tx;
+    p.ty = L.ty;
     if (spheres) {
         p.values = nullptr;
         p.vec4 = 0;

@@ -107,6 +107,15 @@ struct McParams {
     // mc_params_divisors once W, Y and nzc are final
     int narrow;
     UDiv32 div_w, div_y, div_zc;
+    // optional active-tile list (single-grid marching cubes): the count pass
+    // appends each emit tile (TX x TY cell rows, segment tk) that has a face
+    // once (tile_flag, zeroed by the reset kernel, marks it), and emit block b
+    // < *tile_count takes tile tile_list[b]; the other blocks exit
+    int *tile_flag;
+    int *tile_list;
+    int *tile_count;
+    int tx, ty;
+    UDiv32 div_s, div_tx, div_ty;
 };
 constexpr int kMaxFusedSpheres = 256;
 

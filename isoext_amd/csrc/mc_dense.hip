@@ -358,6 +358,11 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
     const int64_t i0 = blk * kMcScanBlock;
     const int64_t i1 = min(i0 + (int64_t) kMcScanBlock, n);
     for (int k = tid; k < kMcScanBlock; k += NT) ent_v[k] = ent_f[k] = 0;
+    // last tile seen per slot (tile id mod 256) by this workgroup: repeats of
+    // a tile skip the global flag
+    __shared__ int tseen[256];
+    if (p.tile_list)
+        for (int k = tid; k < 256; k += NT) tseen[k] = -1;
     __syncthreads();
     const int64_t g0 = entry_chunk(p, i0);
     const int64_t g1 = i1 < n ? entry_chunk(p, i1) : p.X * p.Y * p.W;
@@ -466,6 +471,13 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
         if (i < n) {
             p.cnt_v[i] = tv;
             p.cnt_f[i] = tf;
+            if (p.tile_list && cf[e]) {   // list only with narrow indices (set on the host)
+                const uint32_t r = p.div_s.div((uint32_t) i), sg = (uint32_t) i - r * (uint32_t) p.S,
+                               x = p.div_y.div(r), y = r - x * (uint32_t) p.Y;
+                const int t = (int) ((p.div_tx.div(x) * (uint32_t) p.NJ + p.div_ty.div(y)) * (uint32_t) p.NK + sg);
+                if (atomicExch(&tseen[t & 255], t) != t && atomicCAS(&p.tile_flag[t], 0, 1) == 0)
+                    p.tile_list[atomicAdd(p.tile_count, 1)] = t;
+            }
         }
         tv += cv[e];
         tf += cf[e];
@@ -1218,20 +1230,36 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
     }
 }
 
+// Tile of emit block b: the b-th listed active tile when the count pass
+// built the list (blocks past the list's length have nothing to do), else b.
+__device__ __forceinline__ int64_t emit_block_tile(const McParams &p) {
+    if (!p.tile_list) return blockIdx.x;
+    return (int) blockIdx.x < *p.tile_count ? (int64_t) p.tile_list[blockIdx.x] : -1;
+}
+
 template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_emit(McParams p) {
-    emit_tile<TX, TY, NZC, NT>(p, blockIdx.x, (int) (blockIdx.x % p.seg.nseg));
+    const int64_t tile = emit_block_tile(p);
+    if (tile >= 0) emit_tile<TX, TY, NZC, NT>(p, tile, (int) (blockIdx.x % p.seg.nseg));
 }
 
 template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_emit_spheres(McParams p) {
-    emit_tile<TX, TY, NZC, NT, true>(p, blockIdx.x, (int) (blockIdx.x % p.seg.nseg));
+    const int64_t tile = emit_block_tile(p);
+    if (tile >= 0) emit_tile<TX, TY, NZC, NT, true>(p, tile, (int) (blockIdx.x % p.seg.nseg));
 }
 
+// Reset of the per-call counters (and the active-tile flags when listed);
+// grid-stride over max(4 * nseg, tiles) threads.
 __global__ void mc_dense_reset(McParams p, SpecialLists init) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < 4 * p.seg.nseg) p.seg.ctr[t * kSpStride] = 0;
+    if (p.tile_flag) {
+        const int64_t ntiles = p.NI * p.NJ * p.NK;
+        for (int64_t i = t; i < ntiles; i += (int64_t) gridDim.x * blockDim.x) p.tile_flag[i] = 0;
+    }
     if (t == 0) {
+        if (p.tile_count) *p.tile_count = 0;
         *p.err = 0;
         p.totals[0] = p.totals[1] = 0;
         init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
@@ -1723,6 +1751,14 @@ void mc_params_divisors(McParams &p) {
         p.div_w = UDiv32((uint32_t) p.W);
         p.div_y = UDiv32((uint32_t) p.Y);
         p.div_zc = UDiv32((uint32_t) p.nzc);
+        p.div_s = UDiv32((uint32_t) p.S);
+    }
+    // the active-tile list needs narrow indices and tile ids below 2^31
+    if (p.tile_list && p.tx > 0 && p.ty > 0 && p.narrow && p.NI * p.NJ * p.NK <= (int64_t) INT32_MAX) {
+        p.div_tx = UDiv32((uint32_t) p.tx);
+        p.div_ty = UDiv32((uint32_t) p.ty);
+    } else {
+        p.tile_flag = p.tile_list = p.tile_count = nullptr;
     }
 }
 
@@ -1733,7 +1769,10 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     if (e != hipSuccess) return e;
     auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], stream) : hipSuccess; };
     if ((e = mark(0)) != hipSuccess) return e;
-    hipLaunchKernelGGL(mc_dense_reset, dim3((4 * kSpSeg + 255) / 256), dim3(256), 0, stream, p, sp_init);
+    const int64_t ntiles = p.NI * p.NJ * p.NK;
+    const unsigned rblocks = (unsigned) std::max<int64_t>(
+        (4 * kSpSeg + 255) / 256, p.tile_flag ? std::min<int64_t>((ntiles + 255) / 256, 1024) : 0);
+    hipLaunchKernelGGL(mc_dense_reset, dim3(rblocks), dim3(256), 0, stream, p, sp_init);
     if ((e = hipGetLastError()) != hipSuccess) return e;
 
     launch_classify(p, stream);
