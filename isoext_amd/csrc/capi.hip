@@ -306,7 +306,7 @@ struct ItemLayout {
 
 
 struct BatchLayout {
-    size_t o_hdr, o_items, o_order, o_summary, o_fixcnt, total;
+    size_t o_hdr, o_items, o_order, o_summary, o_fixcnt, o_tflag, o_tlist, o_tcnt, total;
     std::vector<ItemLayout> it;
     iso::McBatchHdr h;
     // emit tiles are laid out by configuration: items of configuration c own
@@ -386,7 +386,20 @@ bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_c
         }
         B.tile_end[c] = t;
     }
+    B.o_tflag = take(sizeof(int32_t) * (size_t) std::max<int64_t>(B.h.ntiles, 1));
+    B.o_tlist = take(sizeof(int32_t) * (size_t) std::max<int64_t>(B.h.ntiles, 1));
+    B.o_tcnt = take(sizeof(int32_t) * iso::kMaxTileConfigs);
     B.total = o;
+    return true;
+}
+
+// Marching-cubes batches emit only the tiles the count pass listed as
+// active, when every grid's chunk indices fit 32 bits and the batch's tile
+// ids fit an int (the same decision in the count and the emit call).
+bool batch_tile_list(int n, const isoext_mc_item *items, const BatchLayout &B) {
+    if (B.h.ntiles > (int64_t) INT32_MAX) return false;
+    for (int i = 0; i < n; ++i)
+        if (B.it[i].cells && items[i].shape[0] * items[i].shape[1] * B.it[i].W > (int64_t) UINT32_MAX) return false;
     return true;
 }
 
@@ -505,12 +518,30 @@ int batch_upload(int n, const isoext_mc_item *items, char *ws, const BatchLayout
     if (!host) return fail(ISOEXT_E_DEVICE, "pinned staging allocation failed");
     iso::McBatchHdr *h = (iso::McBatchHdr *) host;
     *h = B.h;
+    const bool listed = !its_caps && batch_tile_list(n, items, B);
+    if (listed) {
+        h->tile_flag = (int *) (ws + B.o_tflag);
+        h->tile_count = (int *) (ws + B.o_tcnt);
+    }
     iso::McBatchItem *it = (iso::McBatchItem *) (host + sizeof(iso::McBatchHdr));
     int64_t w0 = 0, b0 = 0;
     for (int i = 0; i < n; ++i) {
         fill_item(items[i], B.it[i], ws, w0, b0, B.it[i].tile0, it[i]);
         it[i].p.nzc = B.it[i].nzc;
+        if (listed) {
+            iso::McParams &p = it[i].p;
+            int tx, ty, nzc;
+            iso::mc_config_geom(B.it[i].cfg, tx, ty, nzc);
+            p.tile_flag = h->tile_flag;
+            p.tile_list = (int *) (ws + B.o_tlist) + B.tile_begin[B.it[i].cfg];
+            p.tile_count = h->tile_count + B.it[i].cfg;
+            p.tile_base = (int) B.it[i].tile0;
+            p.tx = tx;
+            p.ty = ty;
+        }
         iso::mc_params_divisors(it[i].p);
+        if (listed && B.it[i].cells && !it[i].p.tile_list)
+            return fail(ISOEXT_E_DEVICE, "batch item %d: active-tile list unavailable", i);
         if (its_caps) {
             iso::McParams &p = it[i].p;
             p.its = 1;
@@ -1163,10 +1194,13 @@ int isoext_mc_batch_emit(int n, const isoext_mc_item *items, int cfg, void *work
     char *ws = (char *) workspace;
     iso::McBatchSummary *sum = nullptr;
     if ((rc = batch_upload(n, items, ws, B, st, &sum)) != ISOEXT_OK) return rc;
+    const bool listed = batch_tile_list(n, items, B);
     HIP_CHECK(iso::mc_batch_emit_launch(B.tile_begin.data(), B.tile_end.data(),
                                         (const iso::McBatchItem *) (ws + B.o_items),
                                         (const int32_t *) (ws + B.o_order), (const iso::McBatchHdr *) (ws + B.o_hdr),
-                                        B.h, (iso::McBatchSummary *) (ws + B.o_summary), st),
+                                        B.h, (iso::McBatchSummary *) (ws + B.o_summary), st,
+                                        listed ? (const int *) (ws + B.o_tlist) : nullptr,
+                                        listed ? (const int *) (ws + B.o_tcnt) : nullptr),
               "launch batch emit");
     HIP_CHECK(hipMemcpyAsync(sum, ws + B.o_summary, sizeof(iso::McBatchSummary) * (size_t) n, hipMemcpyDeviceToHost,
                              st),

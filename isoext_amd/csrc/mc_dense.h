@@ -115,6 +115,7 @@ struct McParams {
     int *tile_list;
     int *tile_count;
     int tx, ty;
+    int tile_base;   // id of the grid's first tile in the list's id space (a batch's tile ids)
     UDiv32 div_s, div_tx, div_ty;
 };
 constexpr int kMaxFusedSpheres = 256;
@@ -141,7 +142,12 @@ struct McBatchItem {
 
 struct McBatchHdr {
     int64_t n, nctask, nblk, ntiles;   // items, flat classify tasks, count blocks, emit tiles
+    // active-tile list of the batch (null: every tile runs): one flag per
+    // batch tile and one list length per tile configuration
+    int *tile_flag = nullptr;
+    int *tile_count = nullptr;
 };
+constexpr int kMaxTileConfigs = 8;
 
 // Per-item result of a batch phase, gathered for one read-back.
 struct McBatchSummary {
@@ -182,8 +188,12 @@ hipError_t its_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_
 // Items may use different tile configurations: configuration c owns the
 // emit tiles [tile_begin[c], tile_end[c]) (one launch each); order lists the
 // items by first tile.
+// tile_list / tile_count: the batch's active-tile list (configuration c's
+// entries start at tile_begin[c], its length is tile_count[c]); null: every
+// tile runs.
 hipError_t mc_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_end, const McBatchItem *items,
                                 const int32_t *order, const McBatchHdr *hdr, const McBatchHdr &h,
-                                McBatchSummary *summary, hipStream_t stream);
+                                McBatchSummary *summary, hipStream_t stream, const int *tile_list = nullptr,
+                                const int *tile_count = nullptr);
 
 }   // namespace iso

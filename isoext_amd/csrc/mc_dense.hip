@@ -474,7 +474,8 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
             if (p.tile_list && cf[e]) {   // list only with narrow indices (set on the host)
                 const uint32_t r = p.div_s.div((uint32_t) i), sg = (uint32_t) i - r * (uint32_t) p.S,
                                x = p.div_y.div(r), y = r - x * (uint32_t) p.Y;
-                const int t = (int) ((p.div_tx.div(x) * (uint32_t) p.NJ + p.div_ty.div(y)) * (uint32_t) p.NK + sg);
+                const int t = p.tile_base +
+                              (int) ((p.div_tx.div(x) * (uint32_t) p.NJ + p.div_ty.div(y)) * (uint32_t) p.NK + sg);
                 if (atomicExch(&tseen[t & 255], t) != t && atomicCAS(&p.tile_flag[t], 0, 1) == 0)
                     p.tile_list[atomicAdd(p.tile_count, 1)] = t;
             }
@@ -1349,6 +1350,11 @@ __device__ __forceinline__ int find_start(const int64_t *st, int n, int64_t x) {
 
 __global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, const McBatchHdr *hdr) {
     const int n = (int) hdr->n;
+    if (hdr->tile_flag) {
+        for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < hdr->ntiles; t += (int64_t) gridDim.x * 256)
+            hdr->tile_flag[t] = 0;
+        if (blockIdx.x == 0 && threadIdx.x < kMaxTileConfigs) hdr->tile_count[threadIdx.x] = 0;
+    }
     for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < (int64_t) n * 4 * kSpSeg;
          t += (int64_t) gridDim.x * 256) {
         const int i = (int) (t / (4 * kSpSeg)), c = (int) (t % (4 * kSpSeg));
@@ -1413,10 +1419,15 @@ __global__ void __launch_bounds__(1024) k_scan_top_batch(const McBatchItem *__re
 template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_emit_batch(const McBatchItem *__restrict__ items,
                                                    const int32_t *__restrict__ order,
-                                                   const McBatchHdr *__restrict__ hdr, int64_t t_begin) {
+                                                   const McBatchHdr *__restrict__ hdr, int64_t t_begin,
+                                                   const int *__restrict__ tlist, const int *__restrict__ tcount) {
     __shared__ int64_t starts[kMaxBatch];
     const int n = (int) hdr->n;
-    const int64_t gt = t_begin + blockIdx.x;
+    int64_t gt = t_begin + blockIdx.x;
+    if (tlist) {   // this configuration's active tiles only
+        if ((int) blockIdx.x >= *tcount) return;
+        gt = tlist[blockIdx.x];
+    }
     if (gt >= hdr->ntiles) return;
     for (int k = threadIdx.x; k < n; k += NT) starts[k] = items[order[k]].tile0;
     __syncthreads();
@@ -1695,11 +1706,13 @@ int64_t mc_config_tiles(int cfg, const int64_t shape[3]) {
 }
 
 // First tile in the order that still gives every CU several workgroups
-// (16x16x128 cells from about 512^3 up, 16x16x64 near 384^3, 8x8x128 at
-// 256^3, 8x8x64 below).  The 16x16x256 tile (config 0) is left out: it
-// emitted 3-25% slower than 16x16x128 at 384^3..1024^3 on MI355X.
+// (16x16x64 cells from about 384^3 up, 8x8x128 near 256^3); when none
+// does, the one with the most tiles.  With empty tiles skipped through the
+// active-tile list, 16x16x64 extracted fastest at 384^3..1024^3 on MI355X
+// (0.5-8% ahead of 16x16x128, which never has more tiles and so is not in
+// the order); 16x16x256 never wins.
 int mc_auto_config(const int64_t shape[3]) {
-    static const int order[] = {2, 3, 1, 5};
+    static const int order[] = {3, 1, 5};
     constexpr int64_t kWant = 2048;
     int best = 0;
     int64_t best_tiles = -1;
@@ -1754,7 +1767,8 @@ void mc_params_divisors(McParams &p) {
         p.div_s = UDiv32((uint32_t) p.S);
     }
     // the active-tile list needs narrow indices and tile ids below 2^31
-    if (p.tile_list && p.tx > 0 && p.ty > 0 && p.narrow && p.NI * p.NJ * p.NK <= (int64_t) INT32_MAX) {
+    if (p.tile_list && p.tx > 0 && p.ty > 0 && p.narrow && p.tile_base >= 0 &&
+        p.tile_base + p.NI * p.NJ * p.NK <= (int64_t) INT32_MAX) {
         p.div_tx = UDiv32((uint32_t) p.tx);
         p.div_ty = UDiv32((uint32_t) p.ty);
     } else {
@@ -1886,7 +1900,8 @@ hipError_t its_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_
 
 hipError_t mc_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_end, const McBatchItem *items,
                                 const int32_t *order, const McBatchHdr *hdr, const McBatchHdr &h,
-                                McBatchSummary *summary, hipStream_t stream) {
+                                McBatchSummary *summary, hipStream_t stream, const int *tile_list,
+                                const int *tile_count) {
     hipError_t e = hipSuccess;
     for (int c = 0; c < mc_config_count(); ++c) {
         const int64_t nt = tile_end[c] - tile_begin[c];
@@ -1895,7 +1910,8 @@ hipError_t mc_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_e
 #define X(id, tx, ty, nzc, nth)                                                                              \
     case id:                                                                                                 \
         hipLaunchKernelGGL((k_emit_batch<tx, ty, nzc, nth>), dim3((unsigned) nt), dim3(nth), 0, stream, items, \
-                           order, hdr, tile_begin[c]);                                                       \
+                           order, hdr, tile_begin[c], tile_list ? tile_list + tile_begin[c] : nullptr,        \
+                           tile_list ? tile_count + c : nullptr);                                            \
         break;
             ISO_MC_CONFIGS(X)
 #undef X

@@ -78,7 +78,8 @@ def test_batch_mixed_tile_configs(gpu):
 
     from isoext_amd import _lib, workloads
     from isoext_amd.batch import GridJob, mc_batch
-    shapes = [(20, 30, 40), (300, 290, 310), (40, 40, 40), (520, 520, 128), (7, 90, 65), (600, 600, 300)]
+    shapes = [(20, 30, 40), (300, 290, 310), (40, 40, 40), (520, 520, 128), (7, 90, 65), (600, 600, 300),
+              (200, 250, 256)]
     L = _lib.lib()
     cfgs = {int(L.isoext_mc_auto_config(_lib.i64x3(s))) for s in shapes}
     assert len(cfgs) >= 3
