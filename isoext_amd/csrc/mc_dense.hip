@@ -495,57 +495,60 @@ __global__ void __launch_bounds__(NT) k_count(McParams p) { count_block<NT>(p, b
 // Blocks below which the count pass uses 1024-lane workgroups.
 constexpr int64_t kCountWideBelow = 2048;
 
-// Exclusive scan of the block totals (one workgroup, carried across rounds);
-// writes the mesh size and flags meshes beyond int32 indices.
+// Exclusive scan of the block totals by one workgroup: thread t sums a
+// contiguous run of ceil(nblk / 1024) totals, one block-wide scan of the run
+// sums, then each thread writes its run's prefixes.  Writes the mesh size and
+// flags meshes beyond int32 indices.
 __device__ __forceinline__ void scan_top(const McParams &p) {
     constexpr int NT = 1024, NW = NT / 64;
     __shared__ int64_t sv[NW], sf[NW];
-    __shared__ int64_t carry_v, carry_f;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) carry_v = carry_f = 0;
-    __syncthreads();
-    for (int64_t b0 = 0; b0 < p.nblk; b0 += NT) {
-        const int64_t i = b0 + tid;
-        const int64_t v = i < p.nblk ? (int64_t) p.bsum_v[i] : 0, f = i < p.nblk ? (int64_t) p.bsum_f[i] : 0;
-        int64_t iv = v, iff = f;
+    const int64_t per = (p.nblk + NT - 1) / NT;
+    const int64_t lo = min<int64_t>((int64_t) tid * per, p.nblk), hi = min<int64_t>(lo + per, p.nblk);
+    int64_t v = 0, f = 0;
+#pragma unroll 4
+    for (int64_t i = lo; i < hi; ++i) {
+        v += (int64_t) p.bsum_v[i];
+        f += (int64_t) p.bsum_f[i];
+    }
+    int64_t iv = v, iff = f;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            int64_t yv = __shfl_up(iv, o), yf = __shfl_up(iff, o);
-            if (lane >= o) {
-                iv += yv;
-                iff += yf;
-            }
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t yv = __shfl_up(iv, o), yf = __shfl_up(iff, o);
+        if (lane >= o) {
+            iv += yv;
+            iff += yf;
         }
-        if (lane == 63) {
-            sv[wave] = iv;
-            sf[wave] = iff;
+    }
+    if (lane == 63) {
+        sv[wave] = iv;
+        sf[wave] = iff;
+    }
+    __syncthreads();
+    int64_t rv = iv - v, rf = iff - f, tv = 0, tf = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if (w < wave) {
+            rv += sv[w];
+            rf += sf[w];
         }
-        __syncthreads();
-        int64_t pv = carry_v, pf = carry_f, tv = 0, tf = 0;
-        for (int w = 0; w < NW; ++w) {
-            if (w < wave) {
-                pv += sv[w];
-                pf += sf[w];
-            }
-            tv += sv[w];
-            tf += sf[w];
-        }
-        if (i < p.nblk) {
-            p.blk_v[i] = pv + iv - v;
-            p.blk_f[i] = pf + iff - f;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            carry_v += tv;
-            carry_f += tf;
-        }
-        __syncthreads();
+        tv += sv[w];
+        tf += sf[w];
+    }
+#pragma unroll 4
+    for (int64_t i = lo; i < hi; ++i) {
+        const int64_t a = (int64_t) p.bsum_v[i], b = (int64_t) p.bsum_f[i];
+        p.blk_v[i] = rv;
+        p.blk_f[i] = rf;
+        rv += a;
+        rf += b;
     }
     if (tid == 0) {
-        p.totals[0] = carry_v;
-        p.totals[1] = carry_f;
-        if (carry_v > 0x7fffffffll || carry_f > 0x7fffffffll) set_err(p, ISO_ERR_TOO_LARGE);
+        p.totals[0] = tv;
+        p.totals[1] = tf;
+        if (tv > 0x7fffffffll || tf > 0x7fffffffll) set_err(p, ISO_ERR_TOO_LARGE);
     }
+    __syncthreads();   // sv / sf are reused by the next call in the same workgroup
 }
 
 __global__ void __launch_bounds__(1024) k_scan_top(McParams p) { scan_top(p); }
