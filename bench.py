@@ -373,7 +373,7 @@ def serving_bench(n_requests: int, seed: int = 0):
             r.result = None
 
     warm = stream(0.0, seed + 99)
-    # defaults: 32 requests / 2^30 points per batch, 6 workers
+    # defaults: 32 requests / 2^30 points per batch, 8 workers
     srv = serving.Server(hbm_budget=200e9, on_done=consume)
     serving.drive(srv, warm)
     drop(warm)

@@ -61,7 +61,7 @@ def footprint(req: Request) -> int:
 
 class Server:
     def __init__(self, device=0, hbm_budget: float = 200e9, max_batch: int = 32,
-                 max_batch_points: int = 1 << 30, workers: int = 6, cfg: int | None = None, on_done=None):
+                 max_batch_points: int = 1 << 30, workers: int = 8, cfg: int | None = None, on_done=None):
         """on_done(request), if given, runs on the worker thread when a
         request completes (a client that consumes each mesh as it arrives,
         e.g. copies it out and drops it)."""
