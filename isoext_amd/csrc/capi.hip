@@ -211,6 +211,14 @@ iso::McParams its_params(const isoext_grid *g, float level, char *ws, int64_t ca
     p.cap_f = cap_pts;
     p.err = (int *) (ws + kOffErr);
     p.totals = (int64_t *) (ws + kOffTotals);
+    // tiles without an active cell are skipped: their lookup-table entries
+    // are never read (only cells next to a crossing edge are looked up, and
+    // those are active)
+    p.tile_flag = (int *) (ws + L.o_tflag);
+    p.tile_list = (int *) (ws + L.o_tlist);
+    p.tile_count = (int *) (ws + L.o_tcnt);
+    p.tx = L.tx;
+    p.ty = L.ty;
     return p;
 }
 

@@ -471,7 +471,9 @@ __device__ __forceinline__ void count_block(const McParams &p, int64_t blk) {
         if (i < n) {
             p.cnt_v[i] = tv;
             p.cnt_f[i] = tf;
-            if (p.tile_list && cf[e]) {   // list only with narrow indices (set on the host)
+            // tiles with a face (intersection mode: with an active cell); the
+            // list exists only with narrow indices (decided on the host)
+            if (p.tile_list && (p.its ? cv[e] : cf[e])) {
                 const uint32_t r = p.div_s.div((uint32_t) i), sg = (uint32_t) i - r * (uint32_t) p.S,
                                x = p.div_y.div(r), y = r - x * (uint32_t) p.Y;
                 const int t = p.tile_base +
@@ -1639,7 +1641,8 @@ __device__ __forceinline__ void its_tile(const McParams &p, const ItsOut &o, int
 
 template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_its_tiles(McParams p, ItsOut o) {
-    its_tile<TX, TY, NZC, NT>(p, o, blockIdx.x);
+    const int64_t tile = emit_block_tile(p);
+    if (tile >= 0) its_tile<TX, TY, NZC, NT>(p, o, tile);
 }
 
 // Batch: one workgroup per intersection tile of the grids using this
@@ -1659,10 +1662,16 @@ __global__ void __launch_bounds__(NT) k_its_tiles_batch(const McBatchItem *__res
     its_tile<TX, TY, NZC, NT>(items[i].p, outs[i], gt - items[i].tile0);
 }
 
-__global__ void k_its_reset(int *err, int64_t *totals) {
-    if (threadIdx.x == 0) {
-        *err = 0;
-        totals[0] = totals[1] = 0;
+__global__ void k_its_reset(McParams p) {
+    const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        *p.err = 0;
+        p.totals[0] = p.totals[1] = 0;
+        if (p.tile_count) *p.tile_count = 0;
+    }
+    if (p.tile_flag) {
+        const int64_t ntiles = p.NI * p.NJ * p.NK;
+        for (int64_t i = t; i < ntiles; i += (int64_t) gridDim.x * blockDim.x) p.tile_flag[i] = 0;
     }
 }
 
@@ -1830,7 +1839,9 @@ hipError_t its_dense_count_launch(McParams &p, hipStream_t stream) {
     mc_params_divisors(p);
     hipError_t e = mc_upload_luts();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_its_reset, dim3(1), dim3(64), 0, stream, p.err, p.totals);
+    const unsigned rblocks =
+        p.tile_flag ? (unsigned) std::max<int64_t>(1, std::min<int64_t>((p.NI * p.NJ * p.NK + 255) / 256, 1024)) : 1;
+    hipLaunchKernelGGL(k_its_reset, dim3(rblocks), dim3(256), 0, stream, p);
     launch_classify(p, stream);
     if (p.nblk < kCountWideBelow)
         hipLaunchKernelGGL(k_count<1024>, dim3((unsigned) p.nblk), dim3(1024), 0, stream, p);
@@ -1841,6 +1852,7 @@ hipError_t its_dense_count_launch(McParams &p, hipStream_t stream) {
 }
 
 hipError_t its_dense_emit_launch(int cfg, McParams &p, const ItsOut &o, hipStream_t stream) {
+    mc_params_divisors(p);
     const unsigned tiles = (unsigned) (p.NI * p.NJ * p.NK);
     switch (cfg) {
 #define X(id, tx, ty, nzc, nt)                                                                        \

@@ -135,3 +135,16 @@ def test_api_drop_in(gpu):
         g.set_values(torch.zeros((32, 32, 32), dtype=torch.float64, device="cuda"))
     with pytest.raises(RuntimeError):
         g.set_values(torch.zeros((32, 32, 31), device="cuda"))
+
+
+@pytest.mark.parametrize("cfg", range(6))
+def test_sparse_surface_repeated_calls(gpu, cfg):
+    """A small sphere leaves most emit tiles without a face (only the tiles
+    the count pass lists run); moving it between calls on the same shape and
+    workspace must not leave tiles of the previous call listed or flagged."""
+    shape = (100, 90, 129)
+    for center in [(-0.7, -0.6, -0.8), (0.6, 0.5, 0.7), (0.0, -0.7, 0.75)]:
+        axes = [np.linspace(-1, 1, n, dtype=np.float32) for n in shape]
+        x, y, z = np.meshgrid(*axes, indexing="ij")
+        vals = (np.sqrt((x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2) - 0.2).astype(np.float32)
+        assert_same_mesh(vals, 0.0, "nagae", cfg)
