@@ -526,7 +526,7 @@ int batch_upload(int n, const isoext_mc_item *items, char *ws, const BatchLayout
     if (!host) return fail(ISOEXT_E_DEVICE, "pinned staging allocation failed");
     iso::McBatchHdr *h = (iso::McBatchHdr *) host;
     *h = B.h;
-    const bool listed = !its_caps && batch_tile_list(n, items, B);
+    const bool listed = batch_tile_list(n, items, B);
     if (listed) {
         h->tile_flag = (int *) (ws + B.o_tflag);
         h->tile_count = (int *) (ws + B.o_tcnt);
@@ -1385,10 +1385,13 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
     }
     HIP_CHECK(hipMemcpyAsync(sc + L.o_outs, ho, sizeof(iso::ItsOut) * (size_t) n, hipMemcpyHostToDevice, st),
               "upload intersection descriptors");
+    const bool listed = batch_tile_list(n, items, B);
     HIP_CHECK(iso::its_batch_emit_launch(B.tile_begin.data(), B.tile_end.data(),
                                          (const iso::McBatchItem *) (ws + B.o_items),
                                          (const int32_t *) (ws + B.o_order), (const iso::ItsOut *) (sc + L.o_outs),
-                                         (const iso::McBatchHdr *) (ws + B.o_hdr), st),
+                                         (const iso::McBatchHdr *) (ws + B.o_hdr), st,
+                                         listed ? (const int *) (ws + B.o_tlist) : nullptr,
+                                         listed ? (const int *) (ws + B.o_tcnt) : nullptr),
               "batch intersections");
     // 2. dual contouring of all grids at once, edges recomputed from the corner values
     iso::ItsIn in{};

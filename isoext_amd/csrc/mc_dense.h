@@ -183,7 +183,8 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
 // Intersections of a batch of dense grids (items counted with p.its = 1):
 // one launch per tile configuration, outs[i] the grid's output arrays.
 hipError_t its_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_end, const McBatchItem *items,
-                                 const int32_t *order, const ItsOut *outs, const McBatchHdr *hdr, hipStream_t stream);
+                                 const int32_t *order, const ItsOut *outs, const McBatchHdr *hdr, hipStream_t stream,
+                                 const int *tile_list = nullptr, const int *tile_count = nullptr);
 
 // Items may use different tile configurations: configuration c owns the
 // emit tiles [tile_begin[c], tile_end[c]) (one launch each); order lists the

@@ -1651,10 +1651,15 @@ template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_its_tiles_batch(const McBatchItem *__restrict__ items,
                                                         const int32_t *__restrict__ order,
                                                         const ItsOut *__restrict__ outs,
-                                                        const McBatchHdr *__restrict__ hdr, int64_t t_begin) {
+                                                        const McBatchHdr *__restrict__ hdr, int64_t t_begin,
+                                                        const int *__restrict__ tlist, const int *__restrict__ tcount) {
     __shared__ int64_t starts[kMaxBatch];
     const int n = (int) hdr->n;
-    const int64_t gt = t_begin + blockIdx.x;
+    int64_t gt = t_begin + blockIdx.x;
+    if (tlist) {   // this configuration's tiles with an active cell only
+        if ((int) blockIdx.x >= *tcount) return;
+        gt = tlist[blockIdx.x];
+    }
     if (gt >= hdr->ntiles) return;
     for (int k = threadIdx.x; k < n; k += NT) starts[k] = items[order[k]].tile0;
     __syncthreads();
@@ -1892,7 +1897,8 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
 }
 
 hipError_t its_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_end, const McBatchItem *items,
-                                 const int32_t *order, const ItsOut *outs, const McBatchHdr *hdr, hipStream_t stream) {
+                                 const int32_t *order, const ItsOut *outs, const McBatchHdr *hdr, hipStream_t stream,
+                                 const int *tile_list, const int *tile_count) {
     for (int c = 0; c < mc_config_count(); ++c) {
         const int64_t nt = tile_end[c] - tile_begin[c];
         if (nt <= 0) continue;
@@ -1900,7 +1906,8 @@ hipError_t its_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_
 #define X(id, tx, ty, nzc, nth)                                                                                  \
     case id:                                                                                                     \
         hipLaunchKernelGGL((k_its_tiles_batch<tx, ty, nzc, nth>), dim3((unsigned) nt), dim3(nth), 0, stream, items, \
-                           order, outs, hdr, tile_begin[c]);                                                     \
+                           order, outs, hdr, tile_begin[c], tile_list ? tile_list + tile_begin[c] : nullptr,    \
+                           tile_list ? tile_count + c : nullptr);                                            \
         break;
             ISO_MC_CONFIGS(X)
 #undef X
