@@ -22,11 +22,18 @@ Parity pins (what checks each function against something independent):
   closed surfaces, vertices on their edges (tests/test_oracle.py).
 * ``mc_sparse``: equality with ``mc_dense`` on the same field, and the
   notebook's 1024^3 sparse counts.
-* ``intersection``: count = sum of popcounts of crossing masks, points on
-  edges, normals against the analytic sphere gradient; DC notebook count.
-* ``dual_contouring``: notebook cuboid counts (6,146 V / 12,288 F), FP64
-  QEF solve checked against numpy.linalg.lstsq/pinv on random systems, and
-  exact recovery of planes/corners.
+* ``intersection``: the notebook cuboid's 24,576 points; points on their
+  edges.  ``cell_normal`` (trilinear central differences): within 1e-5 of
+  a/|a| on linear fields a.p + c; the closed-form derivative (y, x, 0) of
+  the bilinear field x*y at the clamped cell coordinates (0.01 / 0.99) of
+  edge points and of interior points; the (0, 0, 1) fallback at a saddle's
+  centre; anisotropic cells (tests/test_oracle_pins.py).
+* ``pinv_solve`` / ``dual_contouring``: notebook cuboid counts (6,146 V /
+  12,288 F); ``numpy.linalg.pinv(A, rcond=tol)`` on random symmetric PSD
+  systems whose eigenvalues straddle the truncation threshold; with reg = 0,
+  a plane z = d gives the origin-centred minimum-norm vertex (0, 0, d)
+  before the clip, two planes x = a, y = b give (a, b, 0), three planes
+  give their corner (tests/test_oracle_pins.py, tests/test_oracle.py).
 """
 
 from __future__ import annotations
@@ -91,6 +98,11 @@ def lib():
         L.oracle_dual_contouring.argtypes = [fp, fp, P(c_i64), P(ctypes.c_uint8), c_i64,
                                              P(c_i64), P(c_i64), c_i64, ctypes.c_int, P(c_i64),
                                              c_i64, c_i64, c_i64, c_i64, fp, fp, c_f, c_f]
+        L.oracle_cell_normal.restype = None
+        L.oracle_cell_normal.argtypes = [fp, fp, fp, fp]
+        dp = P(ctypes.c_double)
+        L.oracle_pinv_solve.restype = ctypes.c_double
+        L.oracle_pinv_solve.argtypes = [dp, dp, ctypes.c_double, dp]
         for name, t in (("oracle_free_mesh", _Mesh), ("oracle_free_its", _Its),
                         ("oracle_free_dc", _DC)):
             getattr(L, name).argtypes = [P(t)]
@@ -224,6 +236,29 @@ def dual_contouring(its, shape, reg=1e-2, svd_tol=1e-6, aabb_min=(-1, -1, -1),
     res["V"], res["F"] = _take_mesh(ctypes.cast(addr, P(_Mesh)))
     lib().oracle_free_dc(r)
     return res
+
+
+def cell_normal(corner_pos, corner_val, point):
+    """compute_normals_op (src/its.cu:186-268) at one point of one cell:
+    corner_pos (8, 3), corner_val (8,) in the reference's corner order."""
+    cp = np.ascontiguousarray(corner_pos, np.float32).reshape(8, 3)
+    cv = np.ascontiguousarray(corner_val, np.float32).reshape(8)
+    p = np.ascontiguousarray(point, np.float32).reshape(3)
+    n = np.zeros(3, np.float32)
+    lib().oracle_cell_normal(_fp(cp), _fp(cv), _fp(p), _fp(n))
+    return n
+
+
+def pinv_solve(A, b, tol):
+    """The truncated pseudo-inverse solve of src/batched_la.cu:151-178 on one
+    symmetric 3x3 system (FP64).  Returns (x, condition number)."""
+    a = np.ascontiguousarray(A, np.float64).reshape(9)
+    bb = np.ascontiguousarray(b, np.float64).reshape(3)
+    x = np.zeros(3, np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    cond = lib().oracle_pinv_solve(a.ctypes.data_as(dp), bb.ctypes.data_as(dp), float(tol),
+                                   x.ctypes.data_as(dp))
+    return x, cond
 
 
 # ---- canonical mesh comparison ----------------------------------------------

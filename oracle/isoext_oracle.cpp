@@ -490,6 +490,29 @@ OracleIts *oracle_intersection(const float *values, const int64_t *cell_ids, int
     return r;
 }
 
+// Single-point entry to compute_normals_op (src/its.cu:186-268) for the
+// closed-form checks of the normal: cp = 8 corner positions (x, y, z each),
+// cv = 8 corner values, p = the point; writes the unit normal to n.
+void oracle_cell_normal(const float *cp, const float *cv, const float *p, float *n) {
+    F3 c[8];
+    for (int k = 0; k < 8; ++k) c[k] = F3{cp[3 * k], cp[3 * k + 1], cp[3 * k + 2]};
+    F3 r;
+    cell_normal(c, cv, F3{p[0], p[1], p[2]}, r);
+    n[0] = r.x;
+    n[1] = r.y;
+    n[2] = r.z;
+}
+
+// Single-system entry to the truncated pseudo-inverse solve
+// (src/batched_la.cu:151-178) used by oracle_dual_contouring: A is a
+// symmetric 3x3 (row-major), singular values s_j <= tol * s_max are dropped.
+// Returns the condition number s_max / s_min.
+double oracle_pinv_solve(const double *A, const double *b, double tol, double *x) {
+    double cond = 0.0;
+    pinv_solve(A, b, tol, x, &cond);
+    return cond;
+}
+
 // ---- dual contouring (src/dc.cu:161-218) ------------------------------------
 struct OracleDC {
     OracleMesh *mesh;       // welded mesh built from the FP32-rounded dual vertices
