@@ -171,43 +171,6 @@ def bench_dc256(args):
     }
 
 
-def narrow_band_ids(n, band_h, sdf, device="cuda", block=16):
-    """Sorted int64 ids of the cells of an n^3-point grid whose centre lies
-    within band_h cells of the surface: coarse blocks of block^3 cells are
-    kept when their centre is within the band plus the block's half-diagonal
-    (times a Lipschitz margin of 2), then their cells are tested."""
-    import torch
-    nc = n - 1
-    h = 2.0 / nc
-    nb = (nc + block - 1) // block
-    bi = torch.arange(nb, device=device, dtype=torch.float32)
-    bc = -1.0 + (bi * block + block / 2.0) * h
-    keep = []
-    margin = band_h * h + 2.0 * (3 ** 0.5) * block * h / 2
-    for x in range(nb):
-        P = torch.stack(torch.meshgrid(bc[x:x + 1], bc, bc, indexing="ij"), -1).reshape(-1, 3)
-        d = sdf(P).abs()
-        sel = torch.nonzero(d < margin).flatten()
-        keep.append(sel + x * nb * nb)
-    blocks = torch.cat(keep)
-    ids = []
-    off = torch.arange(block, device=device)
-    lx, ly, lz = torch.meshgrid(off, off, off, indexing="ij")
-    lx, ly, lz = lx.reshape(-1), ly.reshape(-1), lz.reshape(-1)
-    for chunk in blocks.split(4096):
-        bx, r = chunk // (nb * nb), chunk % (nb * nb)
-        by, bz = r // nb, r % nb
-        cx = (bx[:, None] * block + lx[None]).reshape(-1)
-        cy = (by[:, None] * block + ly[None]).reshape(-1)
-        cz = (bz[:, None] * block + lz[None]).reshape(-1)
-        ok = (cx < nc) & (cy < nc) & (cz < nc)
-        cx, cy, cz = cx[ok], cy[ok], cz[ok]
-        ctr = torch.stack([-1.0 + (cx + 0.5) * h, -1.0 + (cy + 0.5) * h, -1.0 + (cz + 0.5) * h], -1).float()
-        sel = sdf(ctr).abs() < band_h * h
-        ids.append(((cx * nc + cy) * nc + cz)[sel])
-    return torch.sort(torch.cat(ids)).values
-
-
 def bench_sparse2048(args):
     """Differentiable MC on a SparseGrid at 2048^3 effective resolution:
     narrow band |sdf| < 2h around a sphere r 0.7 plus 3-octave seeded
@@ -226,7 +189,7 @@ def bench_sparse2048(args):
     n = 2048
     sdf = lambda p: workloads.noisy_sphere_eval(p, r=0.7, amp=0.02, seed=0)
     t0 = time.perf_counter()
-    ids = narrow_band_ids(n, 2.0, sdf)
+    ids = workloads.narrow_band_ids(n, 2.0, sdf)
     print(f"sparse2048: narrow band {ids.numel()} cells", file=sys.stderr, flush=True)
     g = iso.SparseGrid([n, n, n])
     g.add_cells(ids)

@@ -49,6 +49,21 @@ int hip_fail(hipError_t e, const char *what) {
         if (e_ != hipSuccess) return hip_fail(e_, what); \
     } while (0)
 
+// The aabb must be finite: with a NaN or infinite box no grid coordinate
+// equals another, which the position-keyed weld passes rely on.
+int check_aabb(const float *amin, const float *amax) {
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(amin[a]) || !std::isfinite(amax[a]))
+            return fail(ISOEXT_E_ARG, "aabb_min and aabb_max must be finite");
+    return ISOEXT_OK;
+}
+
+#define AABB_CHECK(amin, amax)                          \
+    do {                                                \
+        const int rc_ = check_aabb((amin), (amax));     \
+        if (rc_ != ISOEXT_OK) return rc_;               \
+    } while (0)
+
 // Workspace layout for dense marching cubes.
 struct McLayout {
     int tx, ty, nzc;
@@ -583,6 +598,7 @@ int batch_check(int n, const isoext_mc_item *items, int cfg) {
         for (int a = 0; a < 3; ++a)
             if (items[i].shape[a] < 1) return fail(ISOEXT_E_ARG, "grid shape must be positive");
         if (!items[i].values) return fail(ISOEXT_E_ARG, "values is null");
+        AABB_CHECK(items[i].aabb_min, items[i].aabb_max);
     }
     return ISOEXT_OK;
 }
@@ -653,6 +669,7 @@ int64_t isoext_its_scratch_bytes(const isoext_grid *g) {
 int isoext_its_count(const isoext_grid *g, float level, void *scratch, int64_t scratch_bytes, int64_t counts[2],
                      void *stream) {
     counts[0] = counts[1] = 0;
+    AABB_CHECK(g->aabb_min, g->aabb_max);
     if (!grid_has_cells(g)) return ISOEXT_OK;
     if (scratch_bytes < isoext_its_scratch_bytes(g)) return fail(ISOEXT_E_WORKSPACE, "intersection scratch too small");
     hipStream_t st = (hipStream_t) stream;
@@ -726,6 +743,7 @@ int isoext_dc(const isoext_grid *g, const isoext_its *its, float reg, float svd_
               int64_t workspace_bytes, float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f, float *dual_v,
               int32_t *out_vcell, int64_t counts[4], void *stream) {
     counts[0] = counts[1] = counts[2] = counts[3] = 0;
+    AABB_CHECK(g->aabb_min, g->aabb_max);
     if (!grid_has_cells(g) || its->ncells == 0) return ISOEXT_OK;
     if (!its->normals) return fail(ISOEXT_E_ARG, "intersection has no normals");
     DcLayout L = dc_layout(its->ncells, its->npts);
@@ -834,6 +852,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
                          const float *spheres, int nsph) {
     for (int i = 0; i < 6; ++i) counts[i] = 0;
     if (method != 0 && method != 1) return fail(ISOEXT_E_ARG, "Unknown method: %d", method);
+    AABB_CHECK(aabb_min, aabb_max);
     for (int a = 0; a < 3; ++a)
         if (shape[a] < 1) return fail(ISOEXT_E_ARG, "grid shape must be positive");
     if (shape[0] < 2 || shape[1] < 2 || shape[2] < 2) return ISOEXT_OK;   // no cells
@@ -1102,6 +1121,7 @@ static iso::SparseMcIn smc_in(const isoext_grid *g, float level, int method) {
 
 int isoext_mc_sparse_count(const isoext_grid *g, float level, int64_t *scratch, int64_t *ninst, void *stream) {
     *ninst = 0;
+    AABB_CHECK(g->aabb_min, g->aabb_max);
     if (!g->sparse_cell_ids || g->n_sparse == 0) return ISOEXT_OK;
     HIP_CHECK(iso::sparse_mc_count(smc_in(g, level, 0), scratch, *ninst, (hipStream_t) stream), "sparse count");
     return ISOEXT_OK;
@@ -1112,6 +1132,7 @@ int isoext_mc_sparse(const isoext_grid *g, float level, int method, int64_t nins
                      int64_t *vsrc, int64_t counts[2], void *stream) {
     counts[0] = counts[1] = 0;
     if (method != 0 && method != 1) return fail(ISOEXT_E_ARG, "Unknown method: %d", method);
+    AABB_CHECK(g->aabb_min, g->aabb_max);
     if (!g->sparse_cell_ids || g->n_sparse == 0 || ninst == 0) return ISOEXT_OK;
     if (workspace_bytes < isoext_mc_sparse_workspace_bytes(g->n_sparse, ninst))
         return fail(ISOEXT_E_WORKSPACE, "sparse marching cubes workspace too small");

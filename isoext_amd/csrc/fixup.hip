@@ -680,8 +680,9 @@ hipError_t weld_fixup_small_batch(int n, const SpecialLists *sp, float *const *v
         size_t dev_bytes = 0;
         char *host = nullptr;
         size_t host_bytes = 0;
-        ~Cache() {
+        ~Cache() {   // at thread exit; the HIP runtime outlives thread-local objects
             if (host) (void) hipHostFree(host);
+            if (dev) (void) hipFree(dev);
         }
     };
     thread_local Cache c;
@@ -797,20 +798,26 @@ hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t
     size_t o_misc = take(sizeof(int64_t) * 8);
     size_t o_rv = take(sizeof(int64_t) * 2 * nrel);
     size_t o_rf = take(sizeof(int64_t) * 2 * nfrel);
-    static thread_local char *cache = nullptr;
-    static thread_local size_t cache_bytes = 0;
-    if (cache_bytes < bytes) {
-        if (cache) {
+    struct DevCache {   // per thread, freed when the thread ends
+        char *p = nullptr;
+        size_t bytes = 0;
+        ~DevCache() {
+            if (p) (void) hipFree(p);
+        }
+    };
+    static thread_local DevCache cache;
+    if (cache.bytes < bytes) {
+        if (cache.p) {
             ISO_TRY(hipStreamSynchronize(st));
-            ISO_TRY(hipFree(cache));
-            cache = nullptr;
-            cache_bytes = 0;
+            ISO_TRY(hipFree(cache.p));
+            cache.p = nullptr;
+            cache.bytes = 0;
         }
         size_t want = bytes + bytes / 2;
-        ISO_TRY(hipMalloc((void **) &cache, want));
-        cache_bytes = want;
+        ISO_TRY(hipMalloc((void **) &cache.p, want));
+        cache.bytes = want;
     }
-    char *base = cache;
+    char *base = cache.p;
     int64_t *keys = (int64_t *) (base + o_keys);
     VInfo *info = (VInfo *) (base + o_info);
     int32_t *pslot = (int32_t *) (base + o_pslot);

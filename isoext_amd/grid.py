@@ -40,7 +40,10 @@ def _vec3(v, name):
     if len(vals) != 3:
         raise TypeError(f"{name} must be a sequence of 3 floats")
     # round through float32 exactly as the reference's std::array<float, 3>
-    return [float(np.float32(x)) for x in vals]
+    out = [float(np.float32(x)) for x in vals]
+    if not all(np.isfinite(out)):
+        raise ValueError(f"{name} must be finite")
+    return out
 
 
 def check_cuda_f32(t, ndim=None, last=None, name="tensor"):

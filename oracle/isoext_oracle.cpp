@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <parallel/algorithm>
 #include <vector>
 
 #include "../isoext_amd/csrc/mc_luts.h"
@@ -76,10 +77,16 @@ struct Mesh {
 };
 
 // vertex_welding (src/utils.cu:32-59): sort positions lexicographically,
-// unique by equality, face index = lower_bound rank.
-void weld(const std::vector<F3> &tri_v, Mesh &out) {
+// unique by equality, face index = lower_bound rank.  With more than one
+// thread the sort is libstdc++'s parallel multiway mergesort and the
+// lower_bound pass is split over threads; the result is the same.
+void weld(const std::vector<F3> &tri_v, Mesh &out, int threads) {
     std::vector<F3> s(tri_v);
-    std::sort(s.begin(), s.end(), less3);
+    const int nth = std::max(1, threads);
+    if (nth > 1 && s.size() > (size_t) 1 << 16)
+        __gnu_parallel::sort(s.begin(), s.end(), less3, __gnu_parallel::multiway_mergesort_tag(nth));
+    else
+        std::sort(s.begin(), s.end(), less3);
     s.erase(std::unique(s.begin(), s.end(), eq3), s.end());
     out.v.resize(s.size() * 3);
     for (size_t i = 0; i < s.size(); ++i) {
@@ -88,7 +95,9 @@ void weld(const std::vector<F3> &tri_v, Mesh &out) {
         out.v[3 * i + 2] = s[i].z;
     }
     out.f.resize(tri_v.size());
-    for (size_t i = 0; i < tri_v.size(); ++i)
+    const int64_t n = (int64_t) tri_v.size();
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int64_t i = 0; i < n; ++i)
         out.f[i] = (int32_t) (std::lower_bound(s.begin(), s.end(), tri_v[i], less3) - s.begin());
 }
 
@@ -263,9 +272,9 @@ void oracle_free_mesh(OracleMesh *m) {
     std::free(m);
 }
 
-static OracleMesh *to_result(const std::vector<F3> &tri_v, uint64_t ndeg) {
+static OracleMesh *to_result(const std::vector<F3> &tri_v, uint64_t ndeg, int threads = 1) {
     Mesh m;
-    weld(tri_v, m);
+    weld(tri_v, m, threads);
     OracleMesh *r = (OracleMesh *) std::calloc(1, sizeof(OracleMesh));
     r->nv = (int64_t) m.v.size() / 3;
     r->nf = (int64_t) m.f.size() / 3;
@@ -309,7 +318,7 @@ OracleMesh *oracle_mc_dense(const float *values, int64_t nx, int64_t ny, int64_t
         tri_v.insert(tri_v.end(), part[t].begin(), part[t].end());
         ndeg += pdeg[t];
     }
-    return to_result(tri_v, ndeg);
+    return to_result(tri_v, ndeg, nth);
 }
 
 // Cases and active cells only (classification stage, for count checks and

@@ -27,3 +27,35 @@ def test_version_and_error_without_gpu():
     assert lib.isoext_version() == 1
     lib.isoext_last_error.restype = ctypes.c_char_p
     assert lib.isoext_last_error() == b""
+
+
+def test_non_finite_aabb_rejected_before_any_device_work():
+    """A NaN or infinite box is an argument error, returned before any HIP
+    call (so this runs without a GPU)."""
+    import math
+    lib = ctypes.CDLL(str(ROOT / "isoext_amd/libisoext_amd.so"))
+    lib.isoext_last_error.restype = ctypes.c_char_p
+
+    class Grid(ctypes.Structure):
+        _fields_ = [("values", ctypes.c_void_p), ("sparse_cell_ids", ctypes.c_void_p), ("n_sparse", ctypes.c_int64),
+                    ("shape", ctypes.c_int64 * 3), ("aabb_min", ctypes.c_float * 3), ("aabb_max", ctypes.c_float * 3)]
+    for bad in (math.nan, math.inf, -math.inf):
+        g = Grid(None, None, 16, (ctypes.c_int64 * 3)(8, 8, 8), (ctypes.c_float * 3)(-1, bad, -1),
+                 (ctypes.c_float * 3)(1, 1, 1))
+        n = ctypes.c_int64(7)
+        assert lib.isoext_mc_sparse_count(ctypes.byref(g), ctypes.c_float(0.0), None, ctypes.byref(n), None) == -1
+        assert n.value == 0 and b"finite" in lib.isoext_last_error()
+        counts = (ctypes.c_int64 * 6)()
+        rc = lib.isoext_mc_dense(None, g.shape, g.aabb_min, g.aabb_max, ctypes.c_float(0.0), 0, 0, None,
+                                 ctypes.c_int64(0), ctypes.c_int64(0), None, ctypes.c_int64(0), None, ctypes.c_int64(0),
+                                 counts, None)
+        assert rc == -1 and b"finite" in lib.isoext_last_error()
+
+
+def test_python_grid_rejects_non_finite_aabb():
+    import pytest
+    torch = pytest.importorskip("torch")
+    from isoext_amd.grid import _vec3
+    with pytest.raises(ValueError):
+        _vec3([0.0, float("nan"), 0.0], "aabb_min")
+    assert torch is not None

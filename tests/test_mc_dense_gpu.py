@@ -148,3 +148,63 @@ def test_sparse_surface_repeated_calls(gpu, cfg):
         x, y, z = np.meshgrid(*axes, indexing="ij")
         vals = (np.sqrt((x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2) - 0.2).astype(np.float32)
         assert_same_mesh(vals, 0.0, "nagae", cfg)
+
+
+@pytest.mark.parametrize("method", ["nagae", "lorensen"])
+def test_non_finite_values(gpu, method):
+    """NaN, +inf and -inf corners.  The rule the oracle states: NaN is
+    outside (v - level < 0 is false), a parameter t that comes out NaN
+    (inf/inf, NaN operands) is 0, so the vertex sits on the edge's first
+    endpoint and welds by position like any exact-level corner."""
+    rng = np.random.default_rng(17)
+    vals = multi_sphere_field((48, 40, 70), 10, seed=17)
+    flat = vals.reshape(-1)
+    idx = rng.integers(0, flat.size, 1500)
+    flat[idx] = rng.choice(np.array([np.nan, np.inf, -np.inf], np.float32), idx.size)
+    assert_same_mesh(vals, 0.0, method)
+
+
+def test_notebook_gyroid_64(gpu):
+    from conftest import grid_points
+    p = grid_points((64, 64, 64))
+    s = np.float32(6)
+    x, y, z = p[..., 0] * s, p[..., 1] * s, p[..., 2] * s
+    g = (np.sin(x) * np.cos(y) + np.sin(y) * np.cos(z) + np.sin(z) * np.cos(x)).astype(np.float32)
+    vg, fg, _ = run_gpu(g)
+    assert len(vg) == 38760                            # doc/grids.ipynb:172
+    assert_same_mesh(g)
+
+
+def test_notebook_occupancy_smoothed_64(gpu):
+    """occupancy_grids notebook: gaussian_smooth(sigma=5) of the r = 0.7
+    occupancy, level 0.5 -> 8,232 V / 16,460 F (doc/occupancy_grids.ipynb:129).
+    The smoothing runs in torch on the host, so the field does not depend on
+    the device's convolution."""
+    import torch
+
+    from isoext_amd.utils import gaussian_smooth
+    occ = torch.from_numpy((sphere_field((64, 64, 64), 0.7) < 0).astype(np.float32))
+    sm = gaussian_smooth(occ, sigma=5.0).numpy()
+    vg, fg, _ = run_gpu(sm, 0.5)
+    assert (len(vg), len(fg)) == (8232, 16460)
+    assert_same_mesh(sm, 0.5)
+
+
+def test_notebook_quickstart_256(gpu):
+    import torch
+
+    import isoext_amd as iso
+    from isoext_amd.sdf import IntersectionOp, NegationOp, RotationOp, SphereSDF, TorusSDF, UnionOp
+    g = iso.UniformGrid([256, 256, 256])
+    torus = TorusSDF(R=0.75, r=0.15)
+    shape = IntersectionOp([SphereSDF(0.75), NegationOp(UnionOp([
+        torus, RotationOp(torus, axis=[1, 0, 0], angle=90), RotationOp(torus, axis=[0, 1, 0], angle=90)]))])
+    pts = g.get_points().cpu()
+    g.set_values(shape(pts).cuda())
+    v, f = iso.marching_cubes(g, 0.0)
+    assert (v.shape[0], f.shape[0]) == (196176, 392348)   # doc/quickstart.ipynb:21
+    vals = g.values.cpu().numpy()
+    vo, fo = oracle.mc_dense(vals, 0.0)
+    a, b = oracle.canonical(vo, fo), oracle.canonical(v.cpu().numpy(), f.cpu().numpy())
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert torch.is_tensor(v)

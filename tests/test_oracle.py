@@ -169,3 +169,12 @@ def test_dc_solve_matches_lstsq_and_recovers_corner():
     r = oracle.dual_contouring(its, (2, 2, 2), reg=0.0, svd_tol=1e-6)
     assert np.allclose(r["dual_v64"][0], np.array(pts, np.float32).astype(np.float64)[[0, 2, 4], [0, 1, 2]],
                        atol=1e-7)
+
+
+def test_notebook_occupancy_smoothed_64():
+    torch = pytest.importorskip("torch")
+    from isoext_amd.utils import gaussian_smooth
+    occ = torch.from_numpy((sphere_field((64, 64, 64), 0.7) < 0).astype(np.float32))
+    sm = gaussian_smooth(occ, sigma=5.0).numpy()
+    v, f = oracle.mc_dense(sm, 0.5)
+    assert (len(v), len(f)) == (8232, 16460)         # doc/occupancy_grids.ipynb:129
