@@ -25,6 +25,8 @@ import time
 
 METRIC = "MC Gvoxels/s & % HBM peak at 1024³ fp32; serving p50/p99 vs per-call plain"
 HBM_PEAK_GBS = 8000.0
+READ_ONCE_CFG = 3   # tile configuration served by the read-once pipeline (k_stream + k_faces)
+POOL_GRIDS = 320    # distinct serving grids: more than the 8 x 32 requests in flight
 
 
 def parse():
@@ -40,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fused", action="store_true", help="skip the fused sphere-field extraction")
     ap.add_argument("--cpu-planes", type=int, default=129, help="x-planes in the CPU sample")
+    ap.add_argument("--no-dc", action="store_true", help="skip the 256^3 dual-contouring summary")
+    ap.add_argument("--no-sparse", action="store_true", help="skip the 2048^3 sparse summary")
     ap.add_argument("--serving", type=int, default=10000,
                     help="requests in the serving comparison (0 = skip)")
     ap.add_argument("--workload", default="mc1024", choices=["mc1024", "dc256", "sparse2048"],
@@ -234,7 +238,11 @@ def bench_sparse2048(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "SparseGrid 2048^3, |sdf| < 2h band, sphere r 0.7 + 3-octave noise amp 0.02",
                    "active_cells": int(N), "vertices": nv, "faces": nf, "forward_ms": ms_f,
-                   "build_s": t_build, "grad_check": check},
+                   "build_s": t_build, "grad_check": check,
+                   "roofline_forward": {"bound": "hbm", "achieved": alg_f / (ms_f * 1e-3) / 1e9,
+                                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                        "frac": alg_f / (ms_f * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                        "alg_bytes": alg_f}},
         "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": None,
@@ -313,17 +321,20 @@ def serving_bench(n_requests: int, seed: int = 0):
     seeded request stream: dense grids with cubic sides log-uniform in
     [32, 512], SDF = union of 1-64 random spheres, 80% marching cubes / 20%
     dual contouring, HBM admission budget 200 GB.  Inputs come from a pool
-    of 48 distinct grids generated on the GPU before timing."""
+    of 320 distinct grids generated on the GPU before timing, taken in a
+    fixed permutation, so that the 8 workers x 32 requests in flight never
+    hold one grid twice (no batch sees a grid that another request of the
+    batch already pulled into the caches)."""
     import numpy as np
     import torch
 
     from isoext_amd import serving
-    pool = serving.make_pool(48, seed=seed)
+    pool = serving.make_pool(POOL_GRIDS, seed=seed)
     torch.cuda.synchronize()
     pts = float(np.mean([g.numel() for g in pool]))
 
     def stream(rate, s):
-        return serving.make_stream(pool, n_requests, rate=rate, seed=s, dc_frac=0.2)
+        return serving.make_stream(pool, n_requests, rate=rate, seed=s, dc_frac=0.2, cycle=True)
 
     # warm-up both paths (library load, workspaces grown to their steady-state
     # sizes, autotune tables): a full saturated stream of another seed through
@@ -408,6 +419,16 @@ def fused_bench(args, shape, m, v_grid, f_grid, grid_ms_step):
     ms = (time.perf_counter() - t0) * 1e3 / steps
     same = bool(v.shape == v_grid.shape and f.shape == f_grid.shape and torch.equal(v, v_grid)
                 and torch.equal(f, f_grid))
+    same_set = same
+    if not same and v.shape == v_grid.shape and f.shape == f_grid.shape:
+        # the grid path at this size runs the read-once pipeline, whose
+        # vertex order differs: compare the canonical meshes
+        import numpy as np
+
+        import oracle
+        a = oracle.canonical(v.cpu().numpy(), f.cpu().numpy())
+        b = oracle.canonical(v_grid.cpu().numpy(), f_grid.cpu().numpy())
+        same_set = bool(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]))
     del v, f
     buf = torch.empty(shape, dtype=torch.float32, device="cuda")
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -421,18 +442,47 @@ def fused_bench(args, shape, m, v_grid, f_grid, grid_ms_step):
     return {"workload": f"marching cubes of the {args.spheres}-sphere union evaluated in the kernels "
                         f"(isoext_mc_spheres), no grid stored",
             "ms_per_extraction": ms, "gvoxels_s": npts / (ms * 1e-3) / 1e9,
-            "identical_to_grid_path": same, "grid_path_ms_per_step": grid_ms_step,
+            "identical_to_grid_path": same, "same_mesh_as_grid_path": same_set,
+            "grid_path_ms_per_step": grid_ms_step,
             "grid_generation_ms": gen_ms}
 
 
+def host_info():
+    """nproc, the affinity mask's size, the CPU model (lscpu) and the OpenMP
+    thread setting of this host."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+            elif line.startswith("Socket(s):"):
+                info["sockets"] = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        info["model"] = None
+    return info
+
+
+def cpu_threads():
+    """Threads the CPU baseline uses: the host's share for one GPU (the
+    OpenMP setting, 16 on the GPU hosts), at most the affinity mask."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    share = int(omp) if omp and omp.isdigit() else 16
+    return max(1, min(share, aff))
+
+
 def cpu_baseline(values_gpu, planes: int, level: float, method: str):
-    """The CPU oracle (C++ restatement of the reference, OpenMP) on the first
-    `planes` x-planes of the same field; returns Gvoxels/s and details."""
+    """The CPU oracle (C++ restatement of the reference, OpenMP, parallel
+    weld sort) on the first `planes` x-planes of the same field; returns
+    Gvoxels/s and details."""
     import numpy as np
 
     import oracle
     sample = values_gpu[:planes].contiguous().cpu().numpy()
-    threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1)
+    threads = cpu_threads()
     oracle.mc_dense(sample[:9], level, method, threads=threads)   # warm-up (load, page-in)
 
     def timed(arr, nthreads, reps):
@@ -450,6 +500,7 @@ def cpu_baseline(values_gpu, planes: int, level: float, method: str):
             "sample": f"oracle marching cubes ({method}) on the first {planes} x-planes "
                       f"({sample.shape[0]}x{sample.shape[1]}x{sample.shape[2]} points) of the same "
                       f"field, {threads} threads, median of 5 runs, {t:.2f} s each",
+            "host": host_info(),
             "single_core": {"value": int(np.prod(one.shape)) / t1 / 1e9, "unit": "Gvoxels/s", "cores": 1,
                             "sample": f"first {one.shape[0]} x-planes, median of 3 runs, {t1:.2f} s each"}}
 
@@ -516,17 +567,28 @@ def main():
     npts = N * N * N
     kms = sorted(kernel_ms)[len(kernel_ms) // 2]
     kmean = sum(kernel_ms) / len(kernel_ms)
-    names = ["classify", "count", "scan", "emit", "total"]
+    read_once = cfg == READ_ONCE_CFG
+    names = (["reset", "stream", "unused", "faces", "total"] if read_once
+             else ["classify", "count", "scan", "emit", "total"])
     stage_mean = {k: sum(s[i] for s in stages) / len(stages) for i, k in enumerate(names)}
+    stage_mean.pop("unused", None)
     alg_bytes = 4 * npts + 12 * nv + 12 * nf
     pipe_gbs = alg_bytes / (stage_mean["total"] * 1e-3) / 1e9
-    # dominant kernel: classify reads every value once (4 bytes per point)
-    achieved = 4 * npts / (stage_mean["classify"] * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "mc_dense_1024_traffic.json")
+    if read_once:
+        # dominant kernel: k_stream reads every value once and writes the vertices
+        dom = {"kernel": "k_stream (4 B per grid point read + 12 B per vertex written)",
+               "bytes": 4 * npts + 12 * nv, "ms": stage_mean["stream"]}
+        tname = "mc_read_once_1024_traffic.json"
+    else:
+        dom = {"kernel": "k_classify (4 B per grid point read)", "bytes": 4 * npts, "ms": stage_mean["classify"]}
+        tname = "mc_dense_1024_traffic.json"
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
+    traffic, traffic_source = None, None
+    tfile = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", tname)
     if os.path.exists(tfile) and N == 1024:
         try:
             traffic = json.load(open(tfile)).get("traffic_bytes")
+            traffic_source = f"profiles/{tname}: rocprofv3 FETCH_SIZE + WRITE_SIZE runs of this kernel at 1024^3"
         except (OSError, ValueError):
             traffic = None
     out = {
@@ -557,13 +619,14 @@ def main():
             "cell_rate_gcells_s": (N - 1) ** 3 / (ms_step * 1e-3) / 1e9,
         },
         "roofline": {
-            "kernel": "k_classify (4 B per grid point read)",
+            "kernel": dom["kernel"],
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_source,
         },
         "cpu_baseline": None,
     }
@@ -574,9 +637,28 @@ def main():
             out["cpu_baseline"] = cpu_baseline(grid.values, min(args.cpu_planes, N), 0.0, args.method)
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": repr(e)}
-    if args.serving > 0:
-        del grid, v, f
+    del grid, v, f
+    torch.cuda.empty_cache()
+    sub = argparse.Namespace(steps=10, warmup=2)
+    if not args.no_dc:
+        r = bench_dc256(sub)
+        out["config"]["dc256"] = {"workload": r["config"]["workload"], "ms_per_step": r["ms_per_step"],
+                                  "gvoxels_s": r["value"], "dual_error_h": r["config"]["dual_error_h"],
+                                  "fp64_resolves": r["config"]["fp64_resolves"],
+                                  "active_cells": r["config"]["active_cells"],
+                                  "intersections": r["config"]["intersections"], "roofline": r["roofline"],
+                                  "cpu_baseline": r["cpu_baseline"]}
         torch.cuda.empty_cache()
+    if not args.no_sparse:
+        r = bench_sparse2048(sub)
+        out["config"]["sparse2048"] = {"workload": r["config"]["workload"], "ms_per_step": r["ms_per_step"],
+                                       "forward_ms": r["config"]["forward_ms"], "gcells_s": r["value"],
+                                       "active_cells": r["config"]["active_cells"],
+                                       "vertices": r["config"]["vertices"], "faces": r["config"]["faces"],
+                                       "grad_check": r["config"]["grad_check"], "roofline": r["roofline"],
+                                       "roofline_forward": r["config"].get("roofline_forward")}
+        torch.cuda.empty_cache()
+    if args.serving > 0:
         out["config"]["serving"] = serving_bench(args.serving, args.seed)
     print(json.dumps(out), flush=True)
 

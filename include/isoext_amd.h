@@ -42,7 +42,9 @@ int isoext_version(void);
  * cubes brackets its kernels with HIP events on the caller's stream;
  * isoext_last_kernel_ms() returns the extraction's device time (first to
  * last kernel, weld fix-up excluded) and isoext_mc_last_stage_ms() the
- * stages: classify, count, scan, emit, total.  Per calling thread. */
+ * stages: classify, count, scan, emit, total for the four-stage pipeline;
+ * reset, stream, 0, faces, total for the read-once pipeline (tile
+ * configuration 3).  Per calling thread. */
 void isoext_set_timing(int on);
 float isoext_last_kernel_ms(void);
 void isoext_mc_last_stage_ms(float out[5]);

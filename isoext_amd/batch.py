@@ -100,16 +100,19 @@ def _items(jobs, who):
 MAX_DC_BATCH = 256
 
 
-def dc_batch(jobs: list[GridJob], reg: float = 1e-2, svd_tol: float = 1e-6, ws_kind: str = "dc_batch"):
+def dc_batch(jobs: list[GridJob], reg: float = 1e-2, svd_tol: float = 1e-6, ws_kind: str = "dc_batch",
+             on_sizes=None):
     """Dual contouring (intersections with trilinear normals, then the QEF
     solve) over a ragged list of dense grids, at most 256 per call.  Returns
     a list of (v, f) pairs, (None, None) for an empty mesh, as
-    ``dual_contouring(grid, level, reg=reg, svd_tol=svd_tol)`` per grid."""
+    ``dual_contouring(grid, level, reg=reg, svd_tol=svd_tol)`` per grid.
+    on_sizes(out_bytes), if given, receives after phase 1 the output bytes
+    each grid's mesh buffers take (12 per active cell, 24 per intersection)."""
     if not jobs:
         return []
     if len(jobs) > MAX_DC_BATCH:
-        return dc_batch(jobs[:MAX_DC_BATCH], reg, svd_tol, ws_kind) + dc_batch(jobs[MAX_DC_BATCH:], reg, svd_tol,
-                                                                               ws_kind)
+        return (dc_batch(jobs[:MAX_DC_BATCH], reg, svd_tol, ws_kind, on_sizes) +
+                dc_batch(jobs[MAX_DC_BATCH:], reg, svd_tol, ws_kind, on_sizes))
     L = _decl()
     dev, items = _items(jobs, "dc_batch")
     for it in items:
@@ -126,6 +129,8 @@ def dc_batch(jobs: list[GridJob], reg: float = 1e-2, svd_tol: float = 1e-6, ws_k
         _lib.check(L.isoext_dc_batch_count(n, items, ws.data_ptr(), ws.numel(), counts, stream), "dc_batch count")
         C = sum(int(counts[2 * i]) for i in range(n))
         P = sum(int(counts[2 * i + 1]) for i in range(n))
+        if on_sizes is not None:
+            on_sizes([12 * int(counts[2 * i]) + 24 * int(counts[2 * i + 1]) for i in range(n)])
         if C == 0:
             return [(None, None)] * n
         sc = workspace.get(ws_kind + "_scratch", L.isoext_dc_batch_scratch_bytes(n, items, counts), dev)
@@ -158,11 +163,13 @@ def dc_batch(jobs: list[GridJob], reg: float = 1e-2, svd_tol: float = 1e-6, ws_k
 BATCH_CFG = 2
 
 
-def mc_batch(jobs: list[GridJob], cfg: int | None = BATCH_CFG, ws_kind: str = "mc_batch"):
+def mc_batch(jobs: list[GridJob], cfg: int | None = BATCH_CFG, ws_kind: str = "mc_batch", on_sizes=None):
     """Marching cubes over a ragged list of dense grids.  Returns a list of
     (v, f) pairs (zero-row tensors for empty meshes).  cfg: one emit tile
     configuration for every grid (default BATCH_CFG), or None for each
-    grid's own (isoext_mc_auto_config)."""
+    grid's own (isoext_mc_auto_config).  on_sizes(out_bytes), if given,
+    receives after phase 1 each grid's output bytes (12 per provisional
+    vertex and face)."""
     if not jobs:
         return []
     L = _decl()
@@ -183,6 +190,8 @@ def mc_batch(jobs: list[GridJob], cfg: int | None = BATCH_CFG, ws_kind: str = "m
         # one vertex and one face buffer for the batch; each grid gets a slice
         nvs = [int(counts[2 * i]) for i in range(n)]
         nfs = [int(counts[2 * i + 1]) for i in range(n)]
+        if on_sizes is not None:
+            on_sizes([12 * a + 12 * b for a, b in zip(nvs, nfs)])
         all_v = torch.empty((max(sum(nvs), 1), 3), dtype=torch.float32, device=dev)
         all_f = torch.empty((max(sum(nfs), 1), 3), dtype=torch.int32, device=dev)
         v0 = f0 = 0

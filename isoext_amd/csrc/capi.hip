@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -72,6 +73,11 @@ struct McLayout {
     size_t o_ctr, o_scand, o_sscell, o_skept, o_shole;
     size_t o_cand, o_scell, o_kept, o_hole, o_tflag, o_tlist, o_tcnt, total;
     int64_t cap, seg_cap[4];
+    // read-once pipeline (tile configuration iso::kStrTileConfig): regions,
+    // look-back words, region-major offsets, wave bases
+    bool stream;
+    int64_t NRX, NRY, NR;
+    size_t o_ticket, o_lb, o_vofs, o_fofs, o_wbv, o_wbf;
 };
 
 constexpr size_t kHeader = 4096;   // err, totals, SpecialLists
@@ -97,7 +103,14 @@ bool mc_layout(const int64_t shape[3], int cfg, int64_t cap, McLayout &L) {
         o = align256(o + n);
         return r;
     };
-    L.o_bits = take(sizeof(uint64_t) * R * L.W);
+    // the read-once pipeline serves tile configuration kStrTileConfig; its
+    // entries are region-major (whole regions, so at least R * W)
+    L.stream = cfg == iso::kStrTileConfig;
+    L.NRX = (X + iso::kStrRX - 1) / iso::kStrRX;
+    L.NRY = (Y + iso::kStrRY - 1) / iso::kStrRY;
+    L.NR = L.NRX * L.NRY * L.W;
+    const int64_t nent = L.stream ? L.NR * iso::kStrWaves * iso::kStrRX * iso::kStrRows : 0;
+    L.o_bits = take(sizeof(uint64_t) * std::max<int64_t>(R * L.W, nent));
     L.o_cnt_v = take(sizeof(uint32_t) * R * L.S);
     L.o_cnt_f = take(sizeof(uint32_t) * R * L.S);
     L.o_bsum_v = take(sizeof(uint32_t) * L.nblk);
@@ -117,6 +130,15 @@ bool mc_layout(const int64_t shape[3], int cfg, int64_t cap, McLayout &L) {
     L.o_tflag = take(sizeof(int32_t) * L.NI * L.NJ * L.NK);
     L.o_tlist = take(sizeof(int32_t) * L.NI * L.NJ * L.NK);
     L.o_tcnt = take(sizeof(int32_t));
+    L.o_ticket = L.o_lb = L.o_vofs = L.o_fofs = L.o_wbv = L.o_wbf = 0;
+    if (L.stream) {
+        L.o_ticket = take(sizeof(int32_t));
+        L.o_lb = take(sizeof(uint64_t) * L.NR);
+        L.o_vofs = take(sizeof(uint32_t) * nent);
+        L.o_fofs = take(sizeof(uint32_t) * nent);
+        L.o_wbv = take(sizeof(int64_t) * L.NR * iso::kStrWaves);
+        L.o_wbf = take(sizeof(int64_t) * L.NR * iso::kStrWaves);
+    }
     L.total = o;
     return true;
 }
@@ -818,7 +840,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
                          int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream,
-                         const float *spheres = nullptr, int nsph = 0);
+                         const float *spheres = nullptr, int nsph = 0, const iso::FixupScratch *na = nullptr);
 
 int isoext_mc_spheres(const float *spheres, int nsph, const int64_t shape[3], const float aabb_min[3],
                       const float aabb_max[3], float level, int method, int cfg, void *workspace,
@@ -845,11 +867,13 @@ int isoext_mc_dense_keyed(const float *values, const int64_t shape[3], const flo
                          special_cap, out_v, cap_v, out_f, cap_f, out_key, counts, stream);
 }
 
+// na != nullptr: the allocation-free variant -- the weld fix-up runs in
+// na's device scratch, read-backs go through stack memory, no timing events.
 static int mc_dense_impl(const float *values, const int64_t shape[3], const float aabb_min[3],
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
                          int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream,
-                         const float *spheres, int nsph) {
+                         const float *spheres, int nsph, const iso::FixupScratch *na) {
     for (int i = 0; i < 6; ++i) counts[i] = 0;
     if (method != 0 && method != 1) return fail(ISOEXT_E_ARG, "Unknown method: %d", method);
     AABB_CHECK(aabb_min, aabb_max);
@@ -933,9 +957,23 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
         p.spheres = (const float4 *) spheres;
         p.nsph = nsph;
     }
-    hipEvent_t *ev = g_timing ? timing_events() : nullptr;
-    if (g_timing && !ev) return fail(ISOEXT_E_DEVICE, "timing events unavailable");
-    HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, ev), "launch marching cubes");
+    hipEvent_t *ev = (g_timing && !na) ? timing_events() : nullptr;
+    if (g_timing && !na && !ev) return fail(ISOEXT_E_DEVICE, "timing events unavailable");
+    const bool read_once = L.stream && !spheres && L.NR <= (int64_t) INT32_MAX;
+    if (read_once) {
+        p.NRX = L.NRX;
+        p.NRY = L.NRY;
+        p.NR = L.NR;
+        p.ticket = (int *) (ws + L.o_ticket);
+        p.lb = (uint64_t *) (ws + L.o_lb);
+        p.vofs = (uint32_t *) (ws + L.o_vofs);
+        p.fofs = (uint32_t *) (ws + L.o_fofs);
+        p.wb_v = (int64_t *) (ws + L.o_wbv);
+        p.wb_f = (int64_t *) (ws + L.o_wbf);
+        HIP_CHECK(iso::mc_stream_launch(p, sp_host, st, ev), "launch marching cubes");
+    } else {
+        HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, ev), "launch marching cubes");
+    }
 
     // read back counts and the special lists' descriptor (one pinned copy of
     // the workspace header: err, totals, SpecialLists)
@@ -946,7 +984,8 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     };
     static_assert(kOffTotals == kOffErr + 8 && kOffSp >= kOffErr + (int) sizeof(Header), "header layout");
     const size_t rb_bytes = kOffSp - kOffErr + sizeof(iso::SpecialLists);
-    char *rb = (char *) g_pinned_rb.get(rb_bytes);
+    alignas(64) char rb_stack[kOffSp + sizeof(iso::SpecialLists)];
+    char *rb = na ? rb_stack : (char *) g_pinned_rb.get(rb_bytes);
     if (!rb) return fail(ISOEXT_E_DEVICE, "pinned read-back allocation failed");
     HIP_CHECK(hipMemcpyAsync(rb, ws + kOffErr, rb_bytes, hipMemcpyDeviceToHost, st), "read counts");
     HIP_CHECK(hipStreamSynchronize(st), "marching cubes");
@@ -980,13 +1019,25 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
         return fail(ISOEXT_E_WORKSPACE, "special-vertex lists need capacity %lld", (long long) need);
     if (sp_dev.n_cand > 0 || sp_dev.n_hole > 0) {
         iso::FixupGrid g{shape[0], shape[1], shape[2], p.blk_v, p.blk_f, p.cnt_v, p.cnt_f, p.S};
+        if (read_once) g = iso::FixupGrid{shape[0], shape[1], shape[2]};   // region-major numbering: no skip tables
+        // the fixed scratch of the allocation-free variant must hold the
+        // fix-up the lists call for
+        const size_t multi_bytes = iso::weld_fixup_bytes(sp_dev, nv, nf);
+        if (na && multi_bytes > na->dev_bytes)
+            return fail(ISOEXT_E_WORKSPACE, "weld fix-up needs %zu scratch bytes, the fixed limit is %zu",
+                        multi_bytes, na->dev_bytes);
         if (iso::fixup_small_ok(sp_dev)) {   // one workgroup, one launch
             int64_t fin[3];
             int64_t *keys1[1] = {out_key};
-            HIP_CHECK(iso::weld_fixup_small_batch(1, &sp_dev, &out_v, &out_f, keys1, &nv, &nf, &g, st, fin),
+            size_t hb = 0;
+            const size_t sb = iso::weld_fixup_small_batch_bytes(1, &sp_dev, &hb);
+            if (na && (sb > na->dev_bytes || hb > na->host_bytes))
+                return fail(ISOEXT_E_WORKSPACE, "weld fix-up needs %zu scratch bytes, the fixed limit is %zu", sb,
+                            na->dev_bytes);
+            HIP_CHECK(iso::weld_fixup_small_batch(1, &sp_dev, &out_v, &out_f, keys1, &nv, &nf, &g, st, fin, na),
                       "weld fix-up");
             if (fin[2] == iso::kFixSmallOverflow) {
-                HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key), "weld fix-up");
+                HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key, na), "weld fix-up");
             } else if (fin[2]) {
                 return fail(ISOEXT_E_DEVICE, "weld fix-up: inconsistent special lists");
             } else {
@@ -994,7 +1045,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
                 nf = fin[1];
             }
         } else {
-            HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key), "weld fix-up");
+            HIP_CHECK(iso::weld_fixup(sp_dev, out_v, out_f, nv, nf, g, st, out_key, na), "weld fix-up");
         }
         counts[0] = nv;
         counts[1] = nf;

@@ -18,8 +18,21 @@ struct FixupGrid {
     int64_t S = 0;
 };
 
+// Scratch a caller supplies so that a fix-up allocates nothing: device bytes
+// (at least the *_bytes functions' figures) and host staging (small: item
+// records and counts).  Without it the fix-up keeps per-thread caches.
+struct FixupScratch {
+    char *dev;
+    size_t dev_bytes;
+    char *host;
+    size_t host_bytes;
+};
+
 hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
-                      const FixupGrid &g, hipStream_t st, int64_t *keys = nullptr);
+                      const FixupGrid &g, hipStream_t st, int64_t *keys = nullptr,
+                      const FixupScratch *ext = nullptr);
+// Device scratch bytes weld_fixup needs for these lists and provisional counts.
+size_t weld_fixup_bytes(const SpecialLists &sp, int64_t nv, int64_t nf);
 // One weld fix-up for one workgroup: every phase of weld_fixup in a single
 // kernel (hash tables in global scratch, removed / dropped lists sorted in
 // LDS), for lists small enough (fixup_small_ok).  Items are independent: a
@@ -52,7 +65,9 @@ hipError_t fixup_small_launch(const FixSmallItem *items_dev, int n, hipStream_t 
 // its lists were inconsistent.
 hipError_t weld_fixup_small_batch(int n, const SpecialLists *sp, float *const *v, int32_t *const *f,
                                   int64_t *const *vkeys, const int64_t *nv, const int64_t *nf, const FixupGrid *g,
-                                  hipStream_t st, int64_t *final_host);
+                                  hipStream_t st, int64_t *final_host, const FixupScratch *ext = nullptr);
+// Device bytes (and host staging bytes) of weld_fixup_small_batch.
+size_t weld_fixup_small_batch_bytes(int n, const SpecialLists *sp, size_t *host_bytes);
 
 // Stream-ordered variant: no host synchronisation (except to grow the
 // scratch); final_dev[0], [1] receive the final vertex / face counts and

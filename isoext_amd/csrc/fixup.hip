@@ -662,14 +662,26 @@ hipError_t fixup_small_launch(const FixSmallItem *items_dev, int n, hipStream_t 
     return hipGetLastError();
 }
 
+size_t weld_fixup_small_batch_bytes(int n, const SpecialLists *sp, size_t *host_bytes) {
+    auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t bytes = up(sizeof(FixSmallItem) * (size_t) n) + up(sizeof(int64_t) * 3 * (size_t) n);
+    for (int i = 0; i < n; ++i) {
+        int64_t a, b;
+        bytes += fixup_small_scratch(sp[i], a, b);
+    }
+    if (host_bytes) *host_bytes = up(sizeof(FixSmallItem) * (size_t) n) + sizeof(int64_t) * 3 * (size_t) n;
+    return bytes;
+}
+
 hipError_t weld_fixup_small_batch(int n, const SpecialLists *sp, float *const *v, int32_t *const *f,
                                   int64_t *const *vkeys, const int64_t *nv, const int64_t *nf, const FixupGrid *g,
-                                  hipStream_t st, int64_t *final_host) {
+                                  hipStream_t st, int64_t *final_host, const FixupScratch *ext) {
     if (n <= 0) return hipSuccess;
+    if (n > kMaxBatch) return hipErrorInvalidValue;
     auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
     // device: item records, per-item tables, final counts; host: pinned staging
-    std::vector<int64_t> ns(n), nps(n);
-    std::vector<size_t> off(n);
+    int64_t ns[kMaxBatch], nps[kMaxBatch];
+    size_t off[kMaxBatch];
     size_t bytes = up(sizeof(FixSmallItem) * (size_t) n) + up(sizeof(int64_t) * 3 * (size_t) n);
     for (int i = 0; i < n; ++i) {
         off[i] = bytes;
@@ -685,32 +697,41 @@ hipError_t weld_fixup_small_batch(int n, const SpecialLists *sp, float *const *v
             if (dev) (void) hipFree(dev);
         }
     };
-    thread_local Cache c;
-    if (c.dev_bytes < bytes) {
-        if (c.dev) {
-            ISO_TRY(hipStreamSynchronize(st));
-            ISO_TRY(hipFree(c.dev));
-            c.dev = nullptr;
-            c.dev_bytes = 0;
-        }
-        ISO_TRY(hipMalloc((void **) &c.dev, bytes + bytes / 2));
-        c.dev_bytes = bytes + bytes / 2;
-    }
     const size_t hb = up(sizeof(FixSmallItem) * (size_t) n) + sizeof(int64_t) * 3 * (size_t) n;
-    if (c.host_bytes < hb) {
-        if (c.host) {
-            ISO_TRY(hipStreamSynchronize(st));
-            ISO_TRY(hipHostFree(c.host));
-            c.host = nullptr;
-            c.host_bytes = 0;
+    char *dev, *host;
+    if (ext) {   // the caller's scratch: nothing is allocated here
+        if (ext->dev_bytes < bytes || ext->host_bytes < hb) return hipErrorOutOfMemory;
+        dev = ext->dev;
+        host = ext->host;
+    } else {
+        thread_local Cache c;
+        if (c.dev_bytes < bytes) {
+            if (c.dev) {
+                ISO_TRY(hipStreamSynchronize(st));
+                ISO_TRY(hipFree(c.dev));
+                c.dev = nullptr;
+                c.dev_bytes = 0;
+            }
+            ISO_TRY(hipMalloc((void **) &c.dev, bytes + bytes / 2));
+            c.dev_bytes = bytes + bytes / 2;
         }
-        ISO_TRY(hipHostMalloc((void **) &c.host, hb + hb / 2, 0));
-        c.host_bytes = hb + hb / 2;
+        if (c.host_bytes < hb) {
+            if (c.host) {
+                ISO_TRY(hipStreamSynchronize(st));
+                ISO_TRY(hipHostFree(c.host));
+                c.host = nullptr;
+                c.host_bytes = 0;
+            }
+            ISO_TRY(hipHostMalloc((void **) &c.host, hb + hb / 2, 0));
+            c.host_bytes = hb + hb / 2;
+        }
+        dev = c.dev;
+        host = c.host;
     }
-    FixSmallItem *hi = (FixSmallItem *) c.host;
-    int64_t *fin_dev = (int64_t *) (c.dev + up(sizeof(FixSmallItem) * (size_t) n));
+    FixSmallItem *hi = (FixSmallItem *) host;
+    int64_t *fin_dev = (int64_t *) (dev + up(sizeof(FixSmallItem) * (size_t) n));
     for (int i = 0; i < n; ++i) {
-        char *base = c.dev + off[i];
+        char *base = dev + off[i];
         FixSmallItem &it = hi[i];
         it.sp = sp[i];
         it.v = v[i];
@@ -730,9 +751,9 @@ hipError_t weld_fixup_small_batch(int n, const SpecialLists *sp, float *const *v
         it.groups = base;
         it.final_dev = fin_dev + 3 * i;
     }
-    ISO_TRY(hipMemcpyAsync(c.dev, hi, sizeof(FixSmallItem) * (size_t) n, hipMemcpyHostToDevice, st));
-    ISO_TRY(fixup_small_launch((const FixSmallItem *) c.dev, n, st));
-    int64_t *fin_host = (int64_t *) (c.host + up(sizeof(FixSmallItem) * (size_t) n));
+    ISO_TRY(hipMemcpyAsync(dev, hi, sizeof(FixSmallItem) * (size_t) n, hipMemcpyHostToDevice, st));
+    ISO_TRY(fixup_small_launch((const FixSmallItem *) dev, n, st));
+    int64_t *fin_host = (int64_t *) (host + up(sizeof(FixSmallItem) * (size_t) n));
     ISO_TRY(hipMemcpyAsync(fin_host, fin_dev, sizeof(int64_t) * 3 * (size_t) n, hipMemcpyDeviceToHost, st));
     ISO_TRY(hipStreamSynchronize(st));
     for (int i = 0; i < 3 * n; ++i) final_host[i] = fin_host[i];
@@ -740,7 +761,8 @@ hipError_t weld_fixup_small_batch(int n, const SpecialLists *sp, float *const *v
 }
 
 hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
-                           const FixupGrid &g, hipStream_t st, int64_t *vkeys, int64_t *final_dev);
+                           const FixupGrid &g, hipStream_t st, int64_t *vkeys, int64_t *final_dev,
+                           const FixupScratch *ext);
 
 __global__ void k_fix_final(const int64_t *misc, int64_t *final_dev) {
     if (threadIdx.x == 0) {
@@ -754,19 +776,54 @@ __global__ void k_fix_final(const int64_t *misc, int64_t *final_dev) {
 // entry and the final counts on return.  Synchronises the stream once, at
 // the end.
 hipError_t weld_fixup(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
-                      const FixupGrid &g, hipStream_t st, int64_t *vkeys) {
-    return weld_fixup_impl(sp, v, f, nv, nf, g, st, vkeys, nullptr);
+                      const FixupGrid &g, hipStream_t st, int64_t *vkeys, const FixupScratch *ext) {
+    return weld_fixup_impl(sp, v, f, nv, nf, g, st, vkeys, nullptr, ext);
 }
 
 hipError_t weld_fixup_async(const SpecialLists &sp, float *v, int32_t *f, int64_t nv, int64_t nf,
                             const FixupGrid &g, hipStream_t st, int64_t *vkeys, int64_t *final_dev) {
-    return weld_fixup_impl(sp, v, f, nv, nf, g, st, vkeys, final_dev);
+    return weld_fixup_impl(sp, v, f, nv, nf, g, st, vkeys, final_dev, nullptr);
 }
 
 // final_dev == nullptr: read the final counts back (one sync) into nv / nf;
 // else leave them in final_dev.
+// Scratch bytes of the multi-kernel fix-up for these lists and provisional counts.
+static size_t fixup_layout(const SpecialLists &sp, int64_t nv, int64_t nf, size_t o[12]) {
+    const int64_t ncand = (int64_t) sp.n_cand, nscell = (int64_t) sp.n_scell, nhole = (int64_t) sp.n_hole;
+    const int64_t nslots = pow2_at_least(2 * (ncand + nscell) + 16);
+    const int64_t npslots = pow2_at_least(2 * ncand + 16);
+    const int64_t vw = nv / 32 + 2, fw = nf / 32 + 2;
+    size_t bytes = 0;
+    auto take = [&](size_t n) {
+        size_t r = bytes;
+        bytes += (n + 255) & ~size_t(255);
+        return r;
+    };
+    const int64_t nrel = ncand + nscell + 16, nfrel = nhole + 16;
+    o[0] = take(sizeof(int64_t) * nslots);
+    o[1] = take(sizeof(VInfo) * nslots);
+    o[2] = take(sizeof(int32_t) * npslots);
+    o[3] = take(sizeof(PGroup) * npslots);
+    o[4] = take(sizeof(uint32_t) * vw);
+    o[5] = take(sizeof(uint32_t) * fw);
+    o[6] = take(sizeof(int64_t) * vw);
+    o[7] = take(sizeof(int64_t) * fw);
+    o[8] = take(sizeof(int64_t) * ((vw + 1023) / 1024 + 1));
+    o[9] = take(sizeof(int64_t) * ((fw + 1023) / 1024 + 1));
+    o[10] = take(sizeof(int64_t) * 8);
+    o[11] = take(sizeof(int64_t) * 2 * nrel);
+    take(sizeof(int64_t) * 2 * nfrel);
+    return bytes;
+}
+
+size_t weld_fixup_bytes(const SpecialLists &sp, int64_t nv, int64_t nf) {
+    size_t o[12];
+    return fixup_layout(sp, nv, nf, o);
+}
+
 hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t &nv, int64_t &nf,
-                           const FixupGrid &g, hipStream_t st, int64_t *vkeys, int64_t *final_dev) {
+                           const FixupGrid &g, hipStream_t st, int64_t *vkeys, int64_t *final_dev,
+                           const FixupScratch *ext) {
     const int64_t ncand = (int64_t) sp.n_cand, nscell = (int64_t) sp.n_scell;
     const int64_t nkept = (int64_t) sp.n_kept, nhole = (int64_t) sp.n_hole;
     if (ncand == 0 && nhole == 0) {
@@ -780,44 +837,41 @@ hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t
     const int64_t nslots = pow2_at_least(2 * (ncand + nscell) + 16), mask = nslots - 1;
     const int64_t npslots = pow2_at_least(2 * ncand + 16), pmask = npslots - 1;
     const int64_t vw = nv / 32 + 2, fw = nf / 32 + 2;
-    // scratch layout (a cached per-thread buffer, grown on demand)
-    size_t bytes = 0;
-    auto take = [&](size_t n) {
-        size_t o = bytes;
-        bytes += (n + 255) & ~size_t(255);
-        return o;
-    };
     const int64_t nrel = ncand + nscell + 16;   // removed vertices are hashed vertices
     const int64_t nfrel = nhole + 16;           // dropped faces are listed holes
-    size_t o_keys = take(sizeof(int64_t) * nslots), o_info = take(sizeof(VInfo) * nslots);
-    size_t o_pslot = take(sizeof(int32_t) * npslots), o_groups = take(sizeof(PGroup) * npslots);
-    size_t o_vkeep = take(sizeof(uint32_t) * vw), o_fkeep = take(sizeof(uint32_t) * fw);
-    size_t o_vpre = take(sizeof(int64_t) * vw), o_fpre = take(sizeof(int64_t) * fw);
-    size_t o_vscr = take(sizeof(int64_t) * ((vw + 1023) / 1024 + 1));
-    size_t o_fscr = take(sizeof(int64_t) * ((fw + 1023) / 1024 + 1));
-    size_t o_misc = take(sizeof(int64_t) * 8);
-    size_t o_rv = take(sizeof(int64_t) * 2 * nrel);
-    size_t o_rf = take(sizeof(int64_t) * 2 * nfrel);
-    struct DevCache {   // per thread, freed when the thread ends
-        char *p = nullptr;
-        size_t bytes = 0;
-        ~DevCache() {
-            if (p) (void) hipFree(p);
+    size_t lo[12];
+    const size_t bytes = fixup_layout(sp, nv, nf, lo);
+    const size_t o_keys = lo[0], o_info = lo[1], o_pslot = lo[2], o_groups = lo[3], o_vkeep = lo[4],
+                 o_fkeep = lo[5], o_vpre = lo[6], o_fpre = lo[7], o_vscr = lo[8], o_fscr = lo[9], o_misc = lo[10],
+                 o_rv = lo[11];
+    const size_t o_rf = o_rv + ((sizeof(int64_t) * 2 * nrel + 255) & ~size_t(255));
+    char *base;
+    if (ext) {   // the caller's scratch: nothing is allocated here
+        if (ext->dev_bytes < bytes || (!final_dev && ext->host_bytes < 8 * sizeof(int64_t)))
+            return hipErrorOutOfMemory;
+        base = ext->dev;
+    } else {   // a cached per-thread buffer, grown on demand
+        struct DevCache {   // freed when the thread ends
+            char *p = nullptr;
+            size_t bytes = 0;
+            ~DevCache() {
+                if (p) (void) hipFree(p);
+            }
+        };
+        static thread_local DevCache cache;
+        if (cache.bytes < bytes) {
+            if (cache.p) {
+                ISO_TRY(hipStreamSynchronize(st));
+                ISO_TRY(hipFree(cache.p));
+                cache.p = nullptr;
+                cache.bytes = 0;
+            }
+            size_t want = bytes + bytes / 2;
+            ISO_TRY(hipMalloc((void **) &cache.p, want));
+            cache.bytes = want;
         }
-    };
-    static thread_local DevCache cache;
-    if (cache.bytes < bytes) {
-        if (cache.p) {
-            ISO_TRY(hipStreamSynchronize(st));
-            ISO_TRY(hipFree(cache.p));
-            cache.p = nullptr;
-            cache.bytes = 0;
-        }
-        size_t want = bytes + bytes / 2;
-        ISO_TRY(hipMalloc((void **) &cache.p, want));
-        cache.bytes = want;
+        base = cache.p;
     }
-    char *base = cache.p;
     int64_t *keys = (int64_t *) (base + o_keys);
     VInfo *info = (VInfo *) (base + o_info);
     int32_t *pslot = (int32_t *) (base + o_pslot);
@@ -864,12 +918,18 @@ hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t
             if (p) (void) hipHostFree(p);
         }
     };
-    thread_local PinnedCounts pc;
-    if (!pc.p) ISO_TRY(hipHostMalloc((void **) &pc.p, 8 * sizeof(int64_t), 0));
-    ISO_TRY(hipMemcpyAsync(pc.p, misc, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    int64_t *hp;
+    if (ext) {
+        hp = (int64_t *) ext->host;
+    } else {
+        thread_local PinnedCounts pc;
+        if (!pc.p) ISO_TRY(hipHostMalloc((void **) &pc.p, 8 * sizeof(int64_t), 0));
+        hp = pc.p;
+    }
+    ISO_TRY(hipMemcpyAsync(hp, misc, 5 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipStreamSynchronize(st));
     int64_t h[5];
-    for (int i = 0; i < 5; ++i) h[i] = pc.p[i];
+    for (int i = 0; i < 5; ++i) h[i] = hp[i];
     if (h[4]) return hipErrorInvalidValue;   // more relocations than the lists allow: inconsistent lists
     nv = h[2];
     nf = h[3];

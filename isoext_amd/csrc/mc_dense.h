@@ -117,7 +117,27 @@ struct McParams {
     int tx, ty;
     int tile_base;   // id of the grid's first tile in the list's id space (a batch's tile ids)
     UDiv32 div_s, div_tx, div_ty;
+    // read-once pipeline (k_stream, then k_faces): regions of kStrRX x-slabs x
+    // kStrRY point rows x one 64-point word, ordered (x block, y block, word)
+    // with the word fastest; a region's four waves own 8 rows each
+    int64_t NRX, NRY, NR;
+    int *ticket;         // regions are taken in dispatch order (one atomic per workgroup)
+    uint64_t *lb;        // decoupled look-back word per region
+    uint32_t *vofs;      // (X*Y, W): vertices of the wave before entry (point row, word)
+    uint32_t *fofs;      // (X*Y, W): faces of the wave before entry (cell row, word)
+    int64_t *wb_v;       // (NR, 4): first vertex id of each wave
+    int64_t *wb_f;       // (NR, 4): first face id of each wave
 };
+
+// Read-once pipeline geometry.
+constexpr int kStrRX = 16;        // x-slabs per region
+constexpr int kStrRows = 8;       // point rows per wave
+constexpr int kStrWaves = 4;
+constexpr int kStrRY = kStrRows * kStrWaves;
+constexpr int kStrCap = 512;      // vertices a wave stages in LDS before its base is known
+// The read-once pipeline serves single grids whose emit tiles are 16 x 16
+// cell rows x one word (tile configuration 3).
+constexpr int kStrTileConfig = 3;
 constexpr int kMaxFusedSpheres = 256;
 
 // Fill McParams' invariant-divisor fields from X, Y, W and nzc.
@@ -176,6 +196,10 @@ int mc_auto_config(const int64_t shape[3]);
 bool mc_config_geom(int cfg, int &tx, int &ty, int &nzc);
 hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
                            hipEvent_t *ev = nullptr);
+// The read-once pipeline: reset, k_stream (masks, counts, vertices, look-back
+// bases), k_faces over the listed tiles, special gather.  Same outputs as
+// mc_dense_launch up to the order of vertices and faces.
+hipError_t mc_stream_launch(McParams &p, const SpecialLists &sp_init, hipStream_t stream, hipEvent_t *ev = nullptr);
 // Batch phases: classify + count + scan of every item, then emit + gather.
 // Both end with a kernel that writes one McBatchSummary per item.
 hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr, const McBatchHdr &h,

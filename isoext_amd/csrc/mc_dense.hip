@@ -1949,4 +1949,766 @@ hipError_t mc_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_e
     return hipGetLastError();
 }
 
+// ==== read-once pipeline ============================================================
+// k_stream reads every value once.  Workgroup = region of kStrRX x-slabs x
+// kStrRY point rows x one 64-point word, taken by ticket in dispatch order;
+// wave k owns point rows [8k, 8k + 8) of the region and slides along x with
+// two slabs of its rows (plus the row above, and the first value of the next
+// word) in registers.  Per slab and row: the inside mask (ballot), the
+// vertices the row's points own (x-, y-, z-edges whose endpoint masks differ)
+// and the faces of the cell row (LUT triangle counts of the active cells).
+// Vertex positions are computed while the values are in registers and staged
+// in LDS; when the region's counts are known, a decoupled look-back over
+// the region tickets gives its first vertex and face id, and the staged
+// vertices are written in final order.  A wave whose region holds more than
+// kStrCap vertices re-reads its rows after the look-back and writes them
+// directly.  Masks and per-row offsets are written region-major
+// (str_entry), each wave's 16 x 8 entries as whole lines.
+// Vertex order: region, wave, slab, row, z, axis (x < y < z); face order:
+// region, wave, slab, row, z, LUT slot.  k_faces then emits the faces of the
+// listed tiles from the masks and offsets alone.
+
+// Region-major index of point row (x, y), word w.
+__device__ __forceinline__ int64_t str_entry(const McParams &p, int64_t x, int64_t y, int64_t w) {
+    const int64_t rx = x / kStrRX, s = x - rx * kStrRX;
+    const int64_t ry = y / kStrRY, yy = y - ry * kStrRY;
+    const int64_t r = (rx * p.NRY + ry) * p.W + w;
+    return ((r * kStrWaves + (yy >> 3)) * kStrRX + s) * kStrRows + (yy & 7);
+}
+// (region, wave) of point row (x, y), word w: index into wb_v / wb_f.
+__device__ __forceinline__ int64_t str_wave(const McParams &p, int64_t x, int64_t y, int64_t w) {
+    const int64_t r = ((x / kStrRX) * p.NRY + y / kStrRY) * p.W + w;
+    return r * kStrWaves + ((y % kStrRY) >> 3);
+}
+
+// Look-back word of a region: status in bits 62-63 (1: the region's own
+// counts, 2: inclusive prefix), vertices in bits 31-61, faces in bits 0-30.
+constexpr uint64_t kLbAgg = 1ull << 62, kLbIncl = 2ull << 62, kLbField = (1ull << 31) - 1;
+constexpr int kLbSpinLimit = 1 << 22;
+
+// Bit 31 of a vofs entry: the point row owns a weld-candidate vertex in the word.
+constexpr uint32_t kStrCandFlag = 0x80000000u;
+
+struct StrStage {
+    float x, y, z;
+    uint32_t info;   // axis (bits 0-1), lane (2-7), row (8-10), slab (11-15), candidate (16)
+};
+
+struct StrWaveLds {
+    float v[2][kStrRows + 1][64];   // values of two slabs (by slab parity), rows 0..8
+    float h[2][kStrRows + 1];       // z-halo values (first point of the next word)
+    uint64_t m[2][kStrRows + 1];    // inside masks
+    uint32_t hb[2];                 // inside bits of the z-halo points, bit j = row j
+};
+
+struct StreamLds {
+    StrStage st[kStrWaves][kStrCap];
+    StrWaveLds wv[kStrWaves];
+    uint8_t ntri[256];
+    int64_t region;
+    uint32_t cv[kStrWaves], cf[kStrWaves];
+    int64_t base_v, base_f;
+};
+
+// Values of the wave's rows (8 own + the row above) of one slab at this
+// lane's z, and the first value of the next word (z-halo), in flight.
+struct StrSlab {
+    float v[kStrRows + 1];
+    float h[kStrRows + 1];
+};
+
+__device__ __forceinline__ void str_load(const McParams &p, int64_t x, int64_t y0, int nrow, int64_t z0, int lane,
+                                         bool zin, bool hz, StrSlab &o) {
+    const int64_t base = (x * p.Y + y0) * p.Z + z0;
+#pragma unroll
+    for (int j = 0; j <= kStrRows; ++j) {
+        const float *row = p.values + base + (int64_t) j * p.Z;
+        // rows 1..7 are read by this wave only: nontemporal; row 0 and the
+        // row above are also another wave's rows, kept in L2 for it
+        if (j < nrow) {
+            if (j == 0 || j == kStrRows) {
+                o.v[j] = zin ? row[lane] : 0.0f;
+            } else {
+                o.v[j] = zin ? __builtin_nontemporal_load(row + lane) : 0.0f;
+            }
+            o.h[j] = hz ? row[64] : 0.0f;
+        } else {
+            o.v[j] = 0.0f;
+            o.h[j] = 0.0f;
+        }
+    }
+}
+
+// Masks of a loaded slab, then values, halos and masks into the wave's LDS
+// slot (nothing of the slab stays in registers).
+__device__ __forceinline__ void str_commit(const StrSlab &sl, int nrow, bool zin, bool hz, float level, int lane,
+                                           StrWaveLds &w, int slot) {
+    uint32_t hbits = 0;
+#pragma unroll
+    for (int j = 0; j <= kStrRows; ++j) {
+        const uint64_t m = j < nrow ? __ballot(zin && inside(sl.v[j], level)) : 0ull;
+        if (j < nrow && hz && inside(sl.h[j], level)) hbits |= 1u << j;
+        w.v[slot][j][lane] = sl.v[j];
+        if (lane == 0) {
+            w.m[slot][j] = m;
+            w.h[slot][j] = sl.h[j];
+        }
+    }
+    if (lane == 0) w.hb[slot] = hbits;
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void str_clear(int lane, StrWaveLds &w, int slot) {
+    if (lane <= kStrRows) w.m[slot][lane] = 0ull;
+    if (lane == 0) w.hb[slot] = 0u;
+    __builtin_amdgcn_wave_barrier();
+}
+
+// A wave-uniform 64-bit word from LDS, in scalar registers.
+__device__ __forceinline__ uint64_t lds_uniform(const uint64_t &a) {
+    const uint64_t v = a;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t) v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t) (v >> 32));
+    return ((uint64_t) hi << 32) | lo;
+}
+
+// One slab of the wave's rows: vertices (MODE 0: counted and staged; MODE 1:
+// written at base + their wave-relative id, the slow path of an overfull
+// wave) and, in MODE 0, face counts.  Slab x is LDS slot `cs`, slab x + 1
+// slot `ns`.
+template <int MODE, bool KEYED>
+__device__ __forceinline__ void str_slab(const McParams &p, StreamLds &s, int wave, int lane, int step, int64_t x,
+                                         int64_t y0, int ny, int64_t z0, bool hasx, int cs, int ns, uint64_t pv,
+                                         uint64_t ev, float cz0, float cz1, uint32_t &vcount, uint32_t &fcount,
+                                         uint32_t &my_vofs, uint32_t &my_fofs, bool &any_faces, int64_t base,
+                                         int spseg) {
+    StrWaveLds &w = s.wv[wave];
+    const uint64_t below = (1ull << lane) - 1ull;
+    const float cx0 = grid_coord(x, p.X, p.amin[0], p.amax[0]);
+    const float cx1 = grid_coord(x + 1, p.X, p.amin[0], p.amax[0]);
+    const uint32_t hbc = __builtin_amdgcn_readfirstlane(w.hb[cs]);
+    const uint32_t hbn = __builtin_amdgcn_readfirstlane(w.hb[ns]);
+#pragma unroll 1
+    for (int j = 0; j < ny; ++j) {
+        const int64_t y = y0 + j;
+        const bool hasy = y + 1 < p.Y;
+        const uint64_t m = lds_uniform(w.m[cs][j]);
+        const uint64_t my = lds_uniform(w.m[cs][j + 1]);
+        const uint64_t mx = lds_uniform(w.m[ns][j]);
+        const uint64_t zs = (m >> 1) | ((uint64_t) ((hbc >> j) & 1u) << 63);
+        const uint64_t zm = (m ^ zs) & ev;
+        const uint64_t ym = hasy ? (m ^ my) & pv : 0ull;
+        const uint64_t xm = hasx ? (m ^ mx) & pv : 0ull;
+        const uint32_t nvr = (uint32_t) (__popcll(xm) + __popcll(ym) + __popcll(zm));
+        if (MODE == 0 && lane == j) my_vofs = vcount;
+        bool row_cand = false;
+        if (nvr) {
+            const float v0 = w.v[cs][j][lane];
+            float vz = w.v[cs][j][lane < 63 ? lane + 1 : 63];
+            if (lane == 63) vz = w.h[cs][j];
+            uint32_t idx = vcount + (uint32_t) (__popcll(xm & below) + __popcll(ym & below) + __popcll(zm & below));
+            const float cy0 = grid_coord(y, p.Y, p.amin[1], p.amax[1]);
+            const float cy1 = grid_coord(y + 1, p.Y, p.amin[1], p.amax[1]);
+            const int64_t owner = (x * p.Y + y) * p.Z + z0 + lane;
+            const uint32_t mine = (uint32_t) ((xm >> lane) & 1ull) | (uint32_t) ((ym >> lane) & 1ull) << 1 |
+                                  (uint32_t) ((zm >> lane) & 1ull) << 2;
+#pragma unroll
+            for (int axis = 0; axis < 3; ++axis) {
+                if (!((mine >> axis) & 1u)) continue;
+                const float v1 = axis == 0 ? w.v[ns][j][lane] : (axis == 1 ? w.v[cs][j + 1][lane] : vz);
+                const float tt = edge_t(v0, v1, p.level);
+                const float px = lerp1(tt, cx0, axis == 0 ? cx1 : cx0);
+                const float py = lerp1(tt, cy0, axis == 1 ? cy1 : cy0);
+                const float pz = lerp1(tt, cz0, axis == 2 ? cz1 : cz0);
+                const float ra = axis == 0 ? px : (axis == 1 ? py : pz);
+                const float a = axis == 0 ? cx0 : (axis == 1 ? cy0 : cz0);
+                const float b = axis == 0 ? cx1 : (axis == 1 ? cy1 : cz1);
+                const bool cand = p.all_special || near_endpoint(ra, a, b);
+                row_cand |= cand;
+                if constexpr (MODE == 0) {
+                    if (idx < (uint32_t) kStrCap) {
+                        StrStage e;
+                        e.x = px;
+                        e.y = py;
+                        e.z = pz;
+                        e.info = (uint32_t) axis | ((uint32_t) lane << 2) | ((uint32_t) j << 8) |
+                                 ((uint32_t) step << 11) | ((uint32_t) cand << 16);
+                        s.st[wave][idx] = e;
+                    }
+                } else {
+                    const int64_t vid = base + idx;
+                    if (vid < p.cap_v) {
+                        *reinterpret_cast<float3 *>(p.out_v + 3 * vid) = make_float3(px, py, pz);
+                        if (KEYED) p.out_key[vid] = owner * 3 + axis;
+                        if (cand) {
+                            const int64_t i = seg_push(p, 0, spseg);
+                            if (i >= 0) {
+                                SpecialVertex sv;
+                                sv.vid = vid;
+                                sv.edge = owner * 3 + axis;
+                                sv.x = px;
+                                sv.y = py;
+                                sv.z = pz;
+                                sv.pad = 0.0f;
+                                p.seg.cand[i] = sv;
+                            }
+                        }
+                    }
+                }
+                ++idx;
+            }
+        }
+        vcount += nvr;
+        if constexpr (MODE == 0) {
+            // bit 31 of a row's offset: the row owns a weld candidate
+            if (__ballot(row_cand) && lane == j) my_vofs |= kStrCandFlag;
+            // faces of cell row (x, y): active cells have corners on both sides
+            if (lane == j) my_fofs = fcount;
+            if (!hasx || !hasy) continue;
+            const uint64_t mxy = lds_uniform(w.m[ns][j + 1]);
+            const uint32_t hj = (hbc >> j) & 3u, hn = (hbn >> j) & 3u;   // halo bits of rows j, j + 1
+            const uint64_t o = m | my | mx | mxy, n = m & my & mx & mxy;
+            const uint64_t o1 = (o >> 1) | ((uint64_t) ((hj | hn) != 0) << 63);
+            const uint64_t n1 = (n >> 1) | ((uint64_t) ((hj & hn) == 3u) << 63);
+            const uint64_t act = (o | o1) & ~(n & n1) & ev;
+            if (!act) continue;
+            int nt = 0;
+            if ((act >> lane) & 1ull) {
+                // corner k at (k >> 2, (k >> 1) & 1, k & 1): this lane's z and z + 1
+                auto up = [&](uint64_t mm, uint32_t h) {
+                    return lane < 63 ? (int) ((mm >> (lane + 1)) & 1ull) : (int) (h & 1u);
+                };
+                const int c = (int) ((m >> lane) & 1ull) | up(m, hj) << 1 | (int) ((my >> lane) & 1ull) << 2 |
+                              up(my, hj >> 1) << 3 | (int) ((mx >> lane) & 1ull) << 4 | up(mx, hn) << 5 |
+                              (int) ((mxy >> lane) & 1ull) << 6 | up(mxy, hn >> 1) << 7;
+                nt = (int) s.ntri[c];
+            }
+            uint32_t nf = 0;
+#pragma unroll
+            for (int q = 1; q <= 5; ++q) nf += (uint32_t) __popcll(__ballot(nt >= q));
+            fcount += nf;
+            any_faces = true;
+        }
+    }
+}
+
+// The wave's rows, slab by slab: loads run one slab ahead of the slab whose
+// masks are being committed, which runs one ahead of the slab processed.
+template <int MODE, bool KEYED>
+__device__ __forceinline__ void str_rows(const McParams &p, StreamLds &s, int wave, int lane, int64_t x0, int nx,
+                                         int64_t y0, int ny, int nrow, int64_t z0, bool zin, bool hz, uint64_t pv,
+                                         uint64_t ev, float cz0, float cz1, uint32_t &vcount, uint32_t &fcount,
+                                         bool &any_faces, uint64_t (&emask)[kStrRX / 8], uint32_t (&evofs)[kStrRX / 8],
+                                         uint32_t (&efofs)[kStrRX / 8], int64_t base, int spseg) {
+    StrWaveLds &w = s.wv[wave];
+    StrSlab R1, R2;
+    {
+        StrSlab R0;
+        str_load(p, x0, y0, nrow, z0, lane, zin, hz, R0);
+        if (x0 + 1 < p.X) str_load(p, x0 + 1, y0, nrow, z0, lane, zin, hz, R1);
+        str_commit(R0, nrow, zin, hz, p.level, lane, w, 0);
+    }
+    for (int st = 0; st < nx; ++st) {
+        const int64_t x = x0 + st;
+        const bool hasx = x + 1 < p.X;
+        const int cs = st & 1, ns = cs ^ 1;
+        if (st + 1 < nx && x + 2 < p.X) str_load(p, x + 2, y0, nrow, z0, lane, zin, hz, R2);
+        if (hasx) str_commit(R1, nrow, zin, hz, p.level, lane, w, ns);
+        else str_clear(lane, w, ns);
+        uint32_t my_vofs = 0, my_fofs = 0;
+        str_slab<MODE, KEYED>(p, s, wave, lane, st, x, y0, ny, z0, hasx, cs, ns, pv, ev, cz0, cz1, vcount, fcount,
+                              my_vofs, my_fofs, any_faces, base, spseg);
+        if constexpr (MODE == 0) {
+            // this slab's 8 entries go to lanes (st % 8) * 8 + j, slot st / 8
+            const uint64_t my_mask = lane < kStrRows ? w.m[cs][lane] : 0ull;
+            const int src = lane & 7;
+            const uint64_t mm = __shfl(my_mask, src);
+            const uint32_t vv = (uint32_t) __shfl((int) my_vofs, src), ff = (uint32_t) __shfl((int) my_fofs, src);
+            if ((lane >> 3) == (st & 7)) {
+#pragma unroll
+                for (int q = 0; q < kStrRX / 8; ++q)
+                    if ((st >> 3) == q) {
+                        emask[q] = mm;
+                        evofs[q] = vv;
+                        efofs[q] = ff;
+                    }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        R1 = R2;
+    }
+}
+
+template <bool KEYED>
+__global__ void __launch_bounds__(256) k_stream(McParams p) {
+    __shared__ StreamLds s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s.region = atomicAdd(p.ticket, 1);
+    s.ntri[tid] = (uint8_t) (cTris[p.method][tid] & 15);
+    __syncthreads();
+    const int64_t r = s.region;
+    const int64_t w = r % p.W, rt = r / p.W;
+    const int64_t ry = rt % p.NRY, rx = rt / p.NRY;
+    const int64_t x0 = rx * kStrRX, y0 = ry * kStrRY + wave * kStrRows, z0 = 64 * w;
+    const int nx = (int) min<int64_t>(kStrRX, p.X - x0);
+    const int ny = (int) max<int64_t>(0, min<int64_t>(kStrRows, p.Y - y0));
+    const int nrow = (int) max<int64_t>(0, min<int64_t>(kStrRows + 1, p.Y - y0));   // rows incl. the one above
+    const bool zin = z0 + lane < p.Z, hz = z0 + 64 < p.Z;
+    const uint64_t pv = low_bits(p.Z - z0), ev = low_bits(p.Z - 1 - z0);
+    const float cz0 = grid_coord(z0 + lane, p.Z, p.amin[2], p.amax[2]);
+    const float cz1 = grid_coord(z0 + lane + 1, p.Z, p.amin[2], p.amax[2]);
+    const int spseg = (int) (r % p.seg.nseg);
+
+    uint32_t vcount = 0, fcount = 0;
+    bool any_faces = false;
+    // lane (s % 8) * 8 + j holds entry (slab s, row j) in slot s / 8
+    uint64_t emask[kStrRX / 8] = {};
+    uint32_t evofs[kStrRX / 8] = {}, efofs[kStrRX / 8] = {};
+    if (ny > 0)
+        str_rows<0, KEYED>(p, s, wave, lane, x0, nx, y0, ny, nrow, z0, zin, hz, pv, ev, cz0, cz1, vcount, fcount,
+                           any_faces, emask, evofs, efofs, 0, spseg);
+    // region-major entries of this wave: whole lines
+    {
+        const int64_t e0 = (r * kStrWaves + wave) * (int64_t) (kStrRX * kStrRows);
+#pragma unroll
+        for (int q = 0; q < kStrRX / 8; ++q) {
+            const int64_t e = e0 + q * 64 + lane;
+            p.bits[e] = emask[q];
+            p.vofs[e] = evofs[q];
+            p.fofs[e] = efofs[q];
+        }
+    }
+    if (lane == 0) {
+        s.cv[wave] = vcount;
+        s.cf[wave] = fcount;
+    }
+    // the wave's emit tile (16 x 16 cell rows, one word) has a face
+    if (any_faces && lane == 0 && p.tile_list) {
+        const int64_t tj = (ry * kStrRY + wave * kStrRows) / 16;
+        const int t = (int) ((rx * p.NJ + tj) * p.NK + w);
+        if (atomicCAS(&p.tile_flag[t], 0, 1) == 0) p.tile_list[atomicAdd(p.tile_count, 1)] = t;
+    }
+    __syncthreads();
+    // decoupled look-back: publish the region's counts, add the predecessors'
+    if (wave == 0) {
+        uint64_t tv = 0, tf = 0;
+#pragma unroll
+        for (int k = 0; k < kStrWaves; ++k) {
+            tv += s.cv[k];
+            tf += s.cf[k];
+        }
+        if (lane == 0)
+            __hip_atomic_store(&p.lb[r], kLbAgg | (tv << 31) | tf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t ev_ = 0, ef_ = 0;   // exclusive prefix
+        int64_t j = r - 1;
+        int spins = 0;
+        while (j >= 0) {
+            const int64_t idx = j - lane;
+            const uint64_t wd = idx >= 0 ? __hip_atomic_load(&p.lb[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                         : kLbIncl;
+            const uint64_t st = wd >> 62;
+            const uint64_t incl = __ballot(st == 2), none = __ballot(st == 0);
+            const int first = incl ? __ffsll((unsigned long long) incl) - 1 : 64;
+            if (none & low_bits(first + 1)) {
+                if (++spins > kLbSpinLimit) {
+                    if (lane == 0) set_err(p, ISO_ERR_SPIN);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            uint64_t cv = lane <= first ? (wd >> 31) & kLbField : 0, cf = lane <= first ? wd & kLbField : 0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                cv += __shfl_xor(cv, o);
+                cf += __shfl_xor(cf, o);
+            }
+            ev_ += cv;
+            ef_ += cf;
+            if (first < 64) break;
+            j -= 64;
+        }
+        const uint64_t iv = ev_ + tv, iff = ef_ + tf;
+        if (lane == 0) {
+            if (iv > kLbField || iff > kLbField) set_err(p, ISO_ERR_TOO_LARGE);
+            __hip_atomic_store(&p.lb[r], kLbIncl | ((iv & kLbField) << 31) | (iff & kLbField), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            s.base_v = (int64_t) ev_;
+            s.base_f = (int64_t) ef_;
+            if (r == p.NR - 1) {
+                p.totals[0] = (int64_t) iv;
+                p.totals[1] = (int64_t) iff;
+            }
+        }
+    }
+    __syncthreads();
+    int64_t bv = s.base_v, bf = s.base_f;
+    for (int k = 0; k < wave; ++k) {
+        bv += s.cv[k];
+        bf += s.cf[k];
+    }
+    if (lane == 0) {
+        p.wb_v[r * kStrWaves + wave] = bv;
+        p.wb_f[r * kStrWaves + wave] = bf;
+    }
+    if (vcount <= (uint32_t) kStrCap) {
+        // staged vertices, in order
+        for (uint32_t i = (uint32_t) lane; i < vcount; i += 64) {
+            const StrStage e = s.st[wave][i];
+            const int64_t vid = bv + i;
+            if (vid >= p.cap_v) continue;   // the host sees totals beyond the capacity and retries
+            *reinterpret_cast<float3 *>(p.out_v + 3 * vid) = make_float3(e.x, e.y, e.z);
+            const int axis = (int) (e.info & 3), ln = (int) ((e.info >> 2) & 63), j = (int) ((e.info >> 8) & 7),
+                      stp = (int) ((e.info >> 11) & 31);
+            const int64_t owner = ((x0 + stp) * p.Y + (y0 + j)) * p.Z + z0 + ln;
+            if (KEYED) p.out_key[vid] = owner * 3 + axis;
+            if ((e.info >> 16) & 1) {
+                const int64_t q = seg_push(p, 0, spseg);
+                if (q >= 0) {
+                    SpecialVertex sv;
+                    sv.vid = vid;
+                    sv.edge = owner * 3 + axis;
+                    sv.x = e.x;
+                    sv.y = e.y;
+                    sv.z = e.z;
+                    sv.pad = 0.0f;
+                    p.seg.cand[q] = sv;
+                }
+            }
+        }
+    } else {
+        // more vertices than the stage holds: read the rows again and write
+        // every vertex at its final id
+        uint32_t vc = 0, fc = 0;
+        bool dummy = false;
+        str_rows<1, KEYED>(p, s, wave, lane, x0, nx, y0, ny, nrow, z0, zin, hz, pv, ev, cz0, cz1, vc, fc, dummy,
+                           emask, evofs, efofs, bv, spseg);
+    }
+}
+
+// k_faces: one workgroup per listed tile of TX x TY cell rows x one word.
+// The tile stages its mask patch (rows + 2, words + 1, from the region-major
+// masks) and its vertex / face bases (wave base + row offset), compacts the
+// active cells and writes their faces; vertex ids are the owner row's base
+// plus the rank of the edge in (z, axis) order.  A cell is special when one
+// of its vertices is a weld candidate (cbits): its faces are tested for
+// bitwise-equal corners (positions read back from the vertex output) and
+// recorded for the fix-up, with its crossing edges.
+template <int TX, int TY, int NT>
+__device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int spseg) {
+    constexpr int NZC = 1;
+    using L = EmitLds<TX, TY, NZC, NT>;
+    __shared__ L s;
+    const int tid = threadIdx.x;
+    if (*(volatile int *) p.err & ISO_ERR_TOO_LARGE) return;
+    const int64_t tk = tile % p.NK, tj = (tile / p.NK) % p.NJ, ti = tile / (p.NK * p.NJ);
+    const int64_t x0 = ti * TX, y0 = tj * TY, c0 = tk;
+    const int nx = (int) min<int64_t>(TX, p.X - 1 - x0);
+    const int ny = (int) min<int64_t>(TY, p.Y - 1 - y0);
+    const int hx = (int) min<int64_t>(TX + 2, p.X - 1 - x0);
+    const int hy = (int) min<int64_t>(TY + 2, p.Y - 1 - y0);
+    constexpr int NP = L::PX * L::PY * L::PC, IT = (NP + NT - 1) / NT;
+    constexpr int CR = (TX * TY + NT - 1) / NT, PR = ((TX + 1) * (TY + 1) * L::QC + NT - 1) / NT;
+    uint64_t wv[IT];
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = tid + k * NT;
+        const int j = i % L::PC, rr = i / L::PC, py = rr % L::PY, px = rr / L::PY;
+        const int64_t gx = x0 + px, gy = y0 + py, c = c0 + j;
+        wv[k] = (i < NP && gx < p.X && gy < p.Y && c < p.W) ? p.bits[str_entry(p, gx, gy, c)] : 0ull;
+    }
+    int64_t fbase_pf[CR], vbase_pf[PR];
+#pragma unroll
+    for (int k = 0; k < CR; ++k) {
+        const int i = tid + k * NT, cx = i / TY, cy = i % TY;
+        fbase_pf[k] = 0;
+        if (i < TX * TY && cx < nx && cy < ny) {
+            const int64_t gx = x0 + cx, gy = y0 + cy;
+            fbase_pf[k] = p.wb_f[str_wave(p, gx, gy, c0)] + (int64_t) p.fofs[str_entry(p, gx, gy, c0)];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PR; ++k) {
+        const int i = tid + k * NT;
+        const int j = i % L::QC, rr = i / L::QC, py = rr % (TY + 1), px = rr / (TY + 1);
+        const int64_t gx = x0 + px, gy = y0 + py, c = c0 + j;
+        vbase_pf[k] = 0;
+        if (i < (TX + 1) * (TY + 1) * L::QC && gx < p.X && gy < p.Y && c < p.W) {
+            const uint32_t o = p.vofs[str_entry(p, gx, gy, c)];
+            // bit 62 of the prefetched base carries the row's candidate flag
+            vbase_pf[k] = (p.wb_v[str_wave(p, gx, gy, c)] + (int64_t) (o & ~kStrCandFlag)) |
+                          ((int64_t) (o >> 31) << 62);
+        }
+    }
+    for (int c = tid; c < 256; c += NT) s.tris[c] = cTris[p.method][c];
+    for (int c = tid; c < TX + TY + 64 + 3; c += NT) {
+        if (c <= TX) {
+            s.cx[c] = grid_coord(x0 + c, p.X, p.amin[0], p.amax[0]);
+        } else if (c <= TX + TY + 1) {
+            s.cy[c - TX - 1] = grid_coord(y0 + (c - TX - 1), p.Y, p.amin[1], p.amax[1]);
+        } else {
+            s.cz[c - TX - TY - 2] = grid_coord(64 * c0 + (c - TX - TY - 2), p.Z, p.amin[2], p.amax[2]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+        const int i = tid + k * NT;
+        if (i < NP) s.w[i] = wv[k];
+    }
+    __syncthreads();
+    for (int u = tid; u < L::NUNITS; u += NT) {
+        const int cy = u % TY, cx = u / TY;
+        const int64_t zlo = 64 * c0;
+        uint64_t act = 0;
+        if (cx < nx && cy < ny && zlo < p.Z - 1) act = cell_active(unit_cells(s, cx, cy, 0)) & low_bits(p.Z - 1 - zlo);
+        s.amask[u] = act;
+        s.na[u] = (uint32_t) __popcll(act);
+    }
+    __syncthreads();
+    block_exscan_lds<L::NUNITS, NT>(s.na, s.tsum);
+    const int nactive = (int) s.na[L::NUNITS];
+    if (nactive == 0) return;
+#pragma unroll
+    for (int k = 0; k < CR; ++k) {
+        const int i = tid + k * NT;
+        if (i < TX * TY) s.fb[i] = (int32_t) fbase_pf[k];
+    }
+    // the candidate flags of the patch's (point row, word) entries
+    __shared__ uint8_t vflag[(TX + 1) * (TY + 1) * L::QC];
+#pragma unroll
+    for (int k = 0; k < PR; ++k) {
+        const int i = tid + k * NT;
+        if (i < (TX + 1) * (TY + 1) * L::QC) {
+            s.vb[i] = (int32_t) (vbase_pf[k] & ((1ll << 62) - 1));
+            vflag[i] = (uint8_t) ((uint64_t) vbase_pf[k] >> 62);
+        }
+    }
+    if (tid == 0) s.carry[0] = 0;
+    for (int j0 = 0, bt = 0; j0 < nactive; j0 += NT, bt ^= 1) {
+        s.unit[tid] = -1;
+        __syncthreads();
+        for (int u = tid; u < L::NUNITS; u += NT) {
+            const int a0 = (int) s.na[u], a1 = (int) s.na[u + 1];
+            if (a1 > a0 && a0 < j0 + NT && a1 > j0) s.unit[max(a0, j0) - j0] = u;
+        }
+        __syncthreads();
+        block_maxscan_lds<NT>(s.unit, s.tsum);
+        const int jj = j0 + tid;
+        const bool live = jj < nactive;
+        int u = 0, cs = 0, k = 0;
+        if (live) {
+            u = s.unit[tid];
+            k = nth_set_bit(s.amask[u], jj - (int) s.na[u]);
+            const int cyy = u % TY, cxx = u / TY;
+            const uint64_t w00 = PW(s, cxx, cyy, 0), w01 = PW(s, cxx, cyy + 1, 0);
+            const uint64_t w10 = PW(s, cxx + 1, cyy, 0), w11 = PW(s, cxx + 1, cyy + 1, 0);
+            uint32_t up;
+            if (k < 63) {
+                up = (uint32_t) ((w00 >> (k + 1)) & 1) | (uint32_t) ((w01 >> (k + 1)) & 1) << 1 |
+                     (uint32_t) ((w10 >> (k + 1)) & 1) << 2 | (uint32_t) ((w11 >> (k + 1)) & 1) << 3;
+            } else {
+                up = (uint32_t) (PW(s, cxx, cyy, 1) & 1) | (uint32_t) (PW(s, cxx, cyy + 1, 1) & 1) << 1 |
+                     (uint32_t) (PW(s, cxx + 1, cyy, 1) & 1) << 2 | (uint32_t) (PW(s, cxx + 1, cyy + 1, 1) & 1) << 3;
+            }
+            cs = (int) (((w00 >> k) & 1) | ((up & 1) << 1) | (((w01 >> k) & 1) << 2) | (((up >> 1) & 1) << 3) |
+                        (((w10 >> k) & 1) << 4) | (((up >> 2) & 1) << 5) | (((w11 >> k) & 1) << 6) |
+                        (((up >> 3) & 1) << 7));
+        }
+        s.fsc[tid] = live ? (uint32_t) (s.tris[cs] & 15) : 0u;
+        __syncthreads();
+        block_exscan_lds<NT, NT>(s.fsc, s.tsum);
+        if (tid == NT - 1 && j0 + NT < nactive) {
+            const int rl = s.unit[NT - 1];
+            const int rs = (int) s.na[rl], re = (int) s.na[rl + 1];
+            s.carry[bt ^ 1] = re > j0 + NT ? (rs < j0 ? s.carry[bt] : 0) + (int) (s.fsc[NT] - s.fsc[max(rs - j0, 0)])
+                                           : 0;
+        }
+        if (live) {
+            const int rrow = u, cy = u % TY, cx = u / TY;
+            const int rs = (int) s.na[rrow];
+            const int fo = (int) (s.fsc[tid] - s.fsc[max(rs - j0, 0)]) + (rs < j0 ? s.carry[bt] : 0);
+            const int zl = k;
+            const int64_t gx = x0 + cx, gy = y0 + cy, gz = 64 * c0 + zl;
+            const uint32_t status = (uint32_t) crossing_status(cs);
+            int32_t id[12];
+            {
+                const uint64_t b00 = PW(s, cx, cy, 0), b01 = PW(s, cx, cy + 1, 0);
+                const uint64_t b10 = PW(s, cx + 1, cy, 0), b11 = PW(s, cx + 1, cy + 1, 0);
+                const uint64_t b20 = PW(s, cx + 2, cy, 0), b21 = PW(s, cx + 2, cy + 1, 0);
+                const uint64_t b02 = PW(s, cx, cy + 2, 0), b12 = PW(s, cx + 1, cy + 2, 0);
+                const bool hx0 = cx < hx, hx1 = cx + 1 < hx, hy0 = cy < hy, hy1 = cy + 1 < hy;
+                const uint64_t xm00 = hx0 ? b00 ^ b10 : 0ull, xm01 = hx0 ? b01 ^ b11 : 0ull;
+                const uint64_t xm10 = hx1 ? b10 ^ b20 : 0ull, xm11 = hx1 ? b11 ^ b21 : 0ull;
+                const uint64_t ym00 = hy0 ? b00 ^ b01 : 0ull, ym10 = hy0 ? b10 ^ b11 : 0ull;
+                const uint64_t ym01 = hy1 ? b01 ^ b02 : 0ull, ym11 = hy1 ? b11 ^ b12 : 0ull;
+                const uint64_t zm00 = b00 ^ (b00 >> 1), zm01 = b01 ^ (b01 >> 1), zm10 = b10 ^ (b10 >> 1);
+                const uint64_t zm11 = b11 ^ (b11 >> 1);
+                const uint64_t low = (1ull << k) - 1;
+                auto rank = [&](uint64_t xm, uint64_t ym, uint64_t zm) {
+                    return __popcll(xm & low) + __popcll(ym & low) + __popcll(zm & low);
+                };
+                auto bit = [&](uint64_t m, int q) { return (int) ((m >> q) & 1); };
+                constexpr int QC = L::QC, QY = L::QY;
+                const int v00 = s.vb[(cx * QY + cy) * QC], v01 = s.vb[(cx * QY + cy + 1) * QC];
+                const int v10 = s.vb[((cx + 1) * QY + cy) * QC];
+                const int v11 = s.vb[((cx + 1) * QY + cy + 1) * QC];
+                const int r00 = v00 + rank(xm00, ym00, zm00), r01 = v01 + rank(xm01, ym01, zm01);
+                const int r10 = v10 + rank(xm10, ym10, zm10), r11 = v11 + rank(xm11, ym11, zm11);
+                const int x00 = bit(xm00, k), y00 = bit(ym00, k), z00 = bit(zm00, k);
+                const int x01 = bit(xm01, k), y01 = bit(ym01, k), z01 = bit(zm01, k);
+                const int x10 = bit(xm10, k), y10 = bit(ym10, k), z10 = bit(zm10, k);
+                id[8] = r00;
+                id[3] = r00 + x00;
+                id[0] = r00 + x00 + y00;
+                id[11] = r01;
+                id[2] = r01 + x01 + y01;
+                id[7] = r10 + x10;
+                id[4] = r10 + x10 + y10;
+                id[6] = r11 + bit(xm11, k) + bit(ym11, k);
+                if (k < 63) {
+                    const int s00 = r00 + x00 + y00 + z00, s01 = r01 + x01 + y01 + z01;
+                    const int s10 = r10 + x10 + y10 + z10;
+                    id[9] = s00;
+                    id[1] = s00 + bit(xm00, k + 1);
+                    id[10] = s01;
+                    id[5] = s10 + bit(xm10, k + 1);
+                } else {
+                    id[9] = vertex_id(s, hx, hy, cx, cy, zl + 1, 0);
+                    id[1] = vertex_id(s, hx, hy, cx, cy, zl + 1, 1);
+                    id[10] = vertex_id(s, hx, hy, cx, cy + 1, zl + 1, 0);
+                    id[5] = vertex_id(s, hx, hy, cx + 1, cy, zl + 1, 1);
+                }
+#pragma unroll
+                for (int e = 0; e < 12; ++e) s.ids[e][tid] = id[e];
+            }
+            // a weld candidate on one of the cell's edges makes it special:
+            // edges whose owner row is flagged are tested exactly, on the
+            // vertex's along-axis coordinate (the criterion k_stream used)
+            bool special = p.all_special && status != 0;
+            if (!special) {
+                constexpr int QC = L::QC, QY = L::QY;
+                uint32_t m = status;
+                while (m) {
+                    const int e = __ffs(m) - 1;
+                    m &= m - 1;
+                    const int ca = edge_a(e), cb = edge_b(e), axis = edge_axis(e);
+                    const int ox = cx + (ca >> 2), oy = cy + ((ca >> 1) & 1), oz = zl + (ca & 1);
+                    if (!vflag[(ox * QY + oy) * QC + (oz >> 6)]) continue;
+                    const int64_t v = id[e];
+                    if (v < 0 || v >= p.cap_v) continue;
+                    const float *pv = p.out_v + 3 * v;
+                    const float ra = pv[axis];
+                    float a, b;
+                    if (axis == 0) {
+                        a = s.cx[cx + (ca >> 2)];
+                        b = s.cx[cx + (cb >> 2)];
+                    } else if (axis == 1) {
+                        a = s.cy[cy + ((ca >> 1) & 1)];
+                        b = s.cy[cy + ((cb >> 1) & 1)];
+                    } else {
+                        a = s.cz[zl + (ca & 1)];
+                        b = s.cz[zl + (cb & 1)];
+                    }
+                    if (near_endpoint(ra, a, b)) special = true;
+                }
+            }
+            const uint64_t wl = s.tris[cs];
+            const int ntri = (int) (wl & 15);
+            int64_t slot = (int64_t) s.fb[rrow] + fo;
+            for (int q = 0; q < ntri; ++q, ++slot) {
+                const int e0 = (int) ((wl >> (4 + 12 * q)) & 15);
+                const int e1 = (int) ((wl >> (8 + 12 * q)) & 15);
+                const int e2 = (int) ((wl >> (12 + 12 * q)) & 15);
+                const int32_t i0 = s.ids[e0][tid], i1 = s.ids[e1][tid], i2 = s.ids[e2][tid];
+                bool keep = true;
+                if (special) {
+                    auto at = [&](int32_t v) {
+                        return v >= 0 && v < p.cap_v ? *reinterpret_cast<const float3 *>(p.out_v + 3 * (int64_t) v)
+                                                     : make_float3(0.0f, 0.0f, 0.0f);
+                    };
+                    const float3 a = at(i0), b = at(i1), c = at(i2);
+                    const bool d01 = a.x == b.x && a.y == b.y && a.z == b.z;
+                    const bool d02 = a.x == c.x && a.y == c.y && a.z == c.z;
+                    const bool d12 = b.x == c.x && b.y == c.y && b.z == c.z;
+                    keep = !(d01 || d02 || d12);
+                    const int64_t i = seg_push(p, keep ? 2 : 3, spseg);
+                    if (i >= 0) (keep ? p.seg.kept : p.seg.hole)[i] = slot;
+                }
+                if (slot >= 0 && slot < p.cap_f) {
+                    int32_t *dst = p.out_f + 3 * slot;
+                    *reinterpret_cast<int3 *>(dst) = keep ? make_int3(i0, i1, i2) : make_int3(-1, -1, -1);
+                }
+            }
+            if (special) {
+                uint32_t m = status;
+                while (m) {
+                    const int e = __ffs(m) - 1;
+                    m &= m - 1;
+                    const int64_t i = seg_push(p, 1, spseg);
+                    if (i >= 0) {
+                        SpecialEdge se;
+                        se.vid = s.ids[e][tid];
+                        se.edge = edge_key(p, gx, gy, gz, e);
+                        p.seg.scell[i] = se;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int TX, int TY, int NT>
+__global__ void __launch_bounds__(NT) k_faces(McParams p) {
+    const int64_t tile = emit_block_tile(p);
+    if (tile >= 0) faces_tile<TX, TY, NT>(p, tile, (int) (blockIdx.x % p.seg.nseg));
+}
+
+// Reset of the read-once pipeline's per-call state.
+__global__ void k_stream_reset(McParams p, SpecialLists init) {
+    const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x, nt = (int64_t) gridDim.x * blockDim.x;
+    if (t < 4 * p.seg.nseg) p.seg.ctr[t * kSpStride] = 0;
+    for (int64_t i = t; i < p.NR; i += nt) p.lb[i] = 0;
+    if (p.tile_flag) {
+        const int64_t ntiles = p.NI * p.NJ * p.NK;
+        for (int64_t i = t; i < ntiles; i += nt) p.tile_flag[i] = 0;
+    }
+    if (t == 0) {
+        *p.ticket = 0;
+        if (p.tile_count) *p.tile_count = 0;
+        *p.err = 0;
+        p.totals[0] = p.totals[1] = 0;
+        init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
+        init.need = 0;
+        *p.sp = init;
+    }
+}
+
+hipError_t mc_stream_launch(McParams &p, const SpecialLists &sp_init, hipStream_t stream, hipEvent_t *ev) {
+    mc_params_divisors(p);
+    hipError_t e = mc_upload_luts();
+    if (e != hipSuccess) return e;
+    auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], stream) : hipSuccess; };
+    if ((e = mark(0)) != hipSuccess) return e;
+    const int64_t ntiles = p.NI * p.NJ * p.NK;
+    const int64_t rwork = std::max<int64_t>(p.NR, std::max<int64_t>(ntiles, 1024));
+    const unsigned rblocks = (unsigned) std::min<int64_t>((rwork + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_stream_reset, dim3(rblocks), dim3(256), 0, stream, p, sp_init);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // stage times: reset, k_stream, (nothing), k_faces, total
+    if ((e = mark(1)) != hipSuccess) return e;
+    if (p.out_key)
+        hipLaunchKernelGGL(k_stream<true>, dim3((unsigned) p.NR), dim3(256), 0, stream, p);
+    else
+        hipLaunchKernelGGL(k_stream<false>, dim3((unsigned) p.NR), dim3(256), 0, stream, p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = mark(2)) != hipSuccess) return e;
+    if ((e = mark(3)) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_faces<16, 16, 256>), dim3((unsigned) ntiles), dim3(256), 0, stream, p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = mark(4)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sp_gather, dim3(p.seg.nseg), dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
 }   // namespace iso

@@ -45,18 +45,43 @@ class Request:
         return int(self.values.numel())
 
 
-def footprint(req: Request) -> int:
-    """Estimated device bytes a request holds while in flight: the inside
-    masks and count arrays (about N/8 + N/16 bytes), and outputs sized for a
-    surface crossing every x-plane (12 B per vertex + 24 B per face, at most
-    3 vertices per cross-section cell)."""
+def work_bytes(req: Request) -> int:
+    """Device bytes of a request's intermediate state: the inside masks and
+    count arrays (about N/8 + N/16 bytes) plus a fixed 1 MiB."""
+    n = int(req.values.numel())
+    return n // 8 + n // 16 + (1 << 20)
+
+
+def out_estimate(req: Request) -> int:
+    """Output bytes assumed at admission, before the request's counts are
+    known: a surface crossing every x-plane (12 B per vertex + 24 B per face,
+    at most 3 vertices per cross-section cell; twice that for DC)."""
     X, Y, Z = (int(s) for s in req.values.shape)
-    n = X * Y * Z
-    surface = 3 * max(X * Y, Y * Z, X * Z)
-    out = surface * (12 + 24)
-    if req.kind == "dc":
-        out *= 2
-    return n // 8 + n // 16 + out + (1 << 20)
+    out = 3 * max(X * Y, Y * Z, X * Z) * (12 + 24)
+    return 2 * out if req.kind == "dc" else out
+
+
+def footprint(req: Request) -> int:
+    """Device bytes a request is admitted with (work + output estimate);
+    after its batch's first phase the reservation becomes work + the exact
+    output bytes."""
+    return work_bytes(req) + out_estimate(req)
+
+
+def validate(req: Request) -> str | None:
+    """Why a request cannot run (None when it can): the checks of the
+    per-call API, done at submission so that a bad request never joins a
+    batch."""
+    v = req.values
+    if not isinstance(v, torch.Tensor) or v.dtype != torch.float32 or v.dim() != 3 or not v.is_cuda:
+        return "values must be a 3-D float32 CUDA tensor"
+    if not v.is_contiguous():
+        return "values must be C-contiguous"
+    if req.kind not in ("mc", "dc"):
+        return f"unknown request kind {req.kind!r}"
+    if req.kind == "mc" and req.method not in (0, 1):
+        return f"Unknown method: {req.method}"
+    return None
 
 
 class Server:
@@ -95,9 +120,21 @@ class Server:
         self.submit_many([req])
 
     def submit_many(self, reqs):
-        """Enqueue requests that arrived together (one admission pass)."""
+        """Enqueue requests that arrived together (one admission pass).  A
+        request that fails validation completes at once with its error."""
+        ok = []
+        for r in reqs:
+            err = validate(r)
+            if err is None:
+                ok.append(r)
+                continue
+            r.error = err
+            r.t_admit = r.t_done = self.now()
+            if self.on_done is not None:
+                self.on_done(r)
+            r.done.set()
         with self._lock:
-            self._pending.extend(reqs)
+            self._pending.extend(ok)
             self._admit_locked()
             self._lock.notify_all()
 
@@ -128,6 +165,19 @@ class Server:
         with self._lock:
             for r in reqs:
                 self._in_use -= r._bytes
+            self._admit_locked()
+            self._lock.notify_all()
+
+    def _sized(self, reqs, out_bytes):
+        """A batch's first phase has counted its meshes: each reservation
+        becomes work + exact output bytes (frees budget when the estimate was
+        high, takes more when it was low)."""
+        with self._lock:
+            for r, ob in zip(reqs, out_bytes):
+                exact = work_bytes(r) + int(ob)
+                self._in_use += exact - r._bytes
+                r._bytes = exact
+            self.stats["max_in_use"] = max(self.stats["max_in_use"], self._in_use)
             self._admit_locked()
             self._lock.notify_all()
 
@@ -167,9 +217,11 @@ class Server:
                     else:
                         self._run_mc(work)
                 stream.synchronize()
-            except Exception as e:  # reported per request, the server keeps running
-                for r in work:
-                    r.error = repr(e)
+            except Exception as e:  # the batch failed: run its requests one by one
+                if len(work) == 1:
+                    work[0].error = repr(e)
+                else:
+                    self._run_alone(work, stream)
             t = self.now()
             for r in work:
                 r.t_done = t
@@ -178,10 +230,30 @@ class Server:
                 r.done.set()
             self._release(work)
 
+    def _run_alone(self, reqs, stream):
+        """After a batch failure: each request by itself on the worker's
+        stream, so one failing request leaves its batch-mates' results
+        untouched (each equals its per-call result)."""
+        for r in reqs:
+            try:
+                with torch.cuda.stream(stream):
+                    if r.kind == "dc":
+                        self._run_dc([r])
+                    else:
+                        self._run_mc([r])
+                stream.synchronize()
+                r.error = None
+            except Exception as e:
+                r.result = None
+                r.error = repr(e)
+        with self._lock:
+            self.stats["isolated_batches"] = self.stats.get("isolated_batches", 0) + 1
+
     def _run_mc(self, reqs):
         from .batch import BATCH_CFG, GridJob, mc_batch
         jobs = [GridJob(r.values, r.level, r.method) for r in reqs]
-        out = mc_batch(jobs, cfg=self.cfg if self.cfg is not None else BATCH_CFG, ws_kind="serving")
+        out = mc_batch(jobs, cfg=self.cfg if self.cfg is not None else BATCH_CFG, ws_kind="serving",
+                       on_sizes=lambda b: self._sized(reqs, b))
         for r, (v, f) in zip(reqs, out):
             r.result = (v, f) if f.shape[0] > 0 else (None, None)
         with self._lock:
@@ -190,7 +262,8 @@ class Server:
 
     def _run_dc(self, reqs):
         from .batch import GridJob, dc_batch
-        out = dc_batch([GridJob(r.values, r.level) for r in reqs], ws_kind="serving_dc")
+        out = dc_batch([GridJob(r.values, r.level) for r in reqs], ws_kind="serving_dc",
+                       on_sizes=lambda b: self._sized(reqs, b))
         for r, res in zip(reqs, out):
             r.result = res
         with self._lock:
@@ -232,15 +305,19 @@ def make_pool(n_grids: int, seed: int = 0, side_lo: int = 32, side_hi: int = 512
     return pool
 
 
-def make_stream(pool, n: int, rate: float, seed: int = 0, dc_frac: float = 0.2):
+def make_stream(pool, n: int, rate: float, seed: int = 0, dc_frac: float = 0.2, cycle: bool = False):
     """n requests, Poisson arrivals at `rate` per second (rate <= 0: all at
-    t = 0), grids drawn from the pool, dc_frac of them dual contouring."""
+    t = 0), dc_frac of them dual contouring.  Grids are drawn from the pool
+    at random, or with cycle=True in a fixed seeded permutation repeated, so
+    that any len(pool) consecutive requests use distinct grids."""
     rng = np.random.default_rng(seed + 1)
     t = np.cumsum(rng.exponential(1.0 / rate, size=n)) if rate > 0 else np.zeros(n)
+    perm = np.random.default_rng(seed + 2).permutation(len(pool))
     reqs = []
     for i in range(n):
         kind = "dc" if rng.random() < dc_frac else "mc"
-        reqs.append(Request(rid=i, values=pool[int(rng.integers(len(pool)))], kind=kind,
+        g = int(perm[i % len(pool)]) if cycle else int(rng.integers(len(pool)))
+        reqs.append(Request(rid=i, values=pool[g], kind=kind,
                             method=int(rng.integers(0, 2)) if kind == "mc" else 0, arrival=float(t[i])))
     return reqs
 

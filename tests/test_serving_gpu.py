@@ -55,3 +55,41 @@ def test_admission_under_pressure(gpu):
     srv, reqs, lat = run(48e6, workers=2, n=24, seed=9)
     biggest = max(serving.footprint(r) for r in reqs)
     assert srv.stats["max_in_use"] <= max(48e6, biggest)
+
+
+def test_failing_request_leaves_batch_mates_alone(gpu, monkeypatch):
+    """A batch that fails (forced from the host: mc_batch raises whenever one
+    marked grid is in it) is re-run request by request; only the marked
+    request reports an error, every other result is bit-identical to the
+    per-call path.  A request that fails validation (float64 values)
+    completes at once with its error and never joins a batch."""
+    import torch
+
+    from isoext_amd import batch, serving
+    pool = serving.make_pool(8, seed=7, side_lo=24, side_hi=80)
+    torch.cuda.synchronize()
+    marked = pool[3]
+    real = batch.mc_batch
+
+    def failing(jobs, *a, **k):
+        if any(j.values is marked for j in jobs):
+            raise RuntimeError("forced batch failure")
+        return real(jobs, *a, **k)
+    monkeypatch.setattr(batch, "mc_batch", failing)
+    reqs = [serving.Request(rid=i, values=g, kind="mc", method=i % 2) for i, g in enumerate(pool)]
+    reqs.append(serving.Request(rid=99, values=pool[0].double(), kind="mc"))
+    srv = serving.Server(hbm_budget=200e9, max_batch=16, workers=1)
+    try:
+        srv.submit_many(reqs)
+        for r in reqs:
+            assert r.done.wait(120)
+    finally:
+        srv.close()
+    assert reqs[-1].error and "float32" in reqs[-1].error
+    for r in reqs[:-1]:
+        if r.values is marked:
+            assert r.error and "forced" in r.error
+            continue
+        assert r.error is None, r.error
+        assert same(canon(r.result), canon(serving.run_plain(r))), r.rid
+    assert srv.stats.get("isolated_batches", 0) >= 1
