@@ -21,7 +21,8 @@ _lock = threading.Lock()
 _table = None
 
 # (TX, TY, NZC) of each compiled configuration, in library order
-CONFIGS = [(16, 16, 4), (8, 8, 2), (16, 16, 2), (16, 16, 1), (16, 16, 8), (8, 8, 1)]
+# (configuration 6 is the read-once pipeline: k_stream regions + 16 x 16 face tiles)
+CONFIGS = [(16, 16, 4), (8, 8, 2), (16, 16, 2), (16, 16, 1), (16, 16, 8), (8, 8, 1), (16, 16, 1)]
 
 
 def size_class(shape) -> str:

@@ -77,7 +77,7 @@ struct McLayout {
     // look-back words, region-major offsets, wave bases
     bool stream;
     int64_t NRX, NRY, NR;
-    size_t o_ticket, o_lb, o_vofs, o_fofs, o_wbv, o_wbf;
+    size_t o_vofs, o_fofs, o_wcv, o_wcf, o_wbv, o_wbf, o_csum, o_cofs, o_actr, o_aoff, o_wover;
 };
 
 constexpr size_t kHeader = 4096;   // err, totals, SpecialLists
@@ -130,14 +130,22 @@ bool mc_layout(const int64_t shape[3], int cfg, int64_t cap, McLayout &L) {
     L.o_tflag = take(sizeof(int32_t) * L.NI * L.NJ * L.NK);
     L.o_tlist = take(sizeof(int32_t) * L.NI * L.NJ * L.NK);
     L.o_tcnt = take(sizeof(int32_t));
-    L.o_ticket = L.o_lb = L.o_vofs = L.o_fofs = L.o_wbv = L.o_wbf = 0;
+    L.o_vofs = L.o_fofs = L.o_wcv = L.o_wcf = L.o_wbv = L.o_wbf = L.o_csum = L.o_cofs = 0;
+    L.o_actr = L.o_aoff = L.o_wover = 0;
     if (L.stream) {
-        L.o_ticket = take(sizeof(int32_t));
-        L.o_lb = take(sizeof(uint64_t) * L.NR);
+        const int64_t nw = L.NR * iso::kStrWaves;
+        const int64_t nchunk = (nw + iso::kStrScanChunk - 1) / iso::kStrScanChunk;
         L.o_vofs = take(sizeof(uint32_t) * nent);
         L.o_fofs = take(sizeof(uint32_t) * nent);
-        L.o_wbv = take(sizeof(int64_t) * L.NR * iso::kStrWaves);
-        L.o_wbf = take(sizeof(int64_t) * L.NR * iso::kStrWaves);
+        L.o_wcv = take(sizeof(uint32_t) * nw);
+        L.o_wcf = take(sizeof(uint32_t) * nw);
+        L.o_wbv = take(sizeof(int64_t) * nw);
+        L.o_wbf = take(sizeof(int64_t) * nw);
+        L.o_csum = take(sizeof(int64_t) * 2 * nchunk);
+        L.o_cofs = take(sizeof(int64_t) * 2 * nchunk);
+        L.o_actr = take(sizeof(unsigned long long) * iso::kStrArenaSeg * iso::kSpStride);
+        L.o_aoff = take(sizeof(int32_t) * nw * iso::kStrRX);
+        L.o_wover = take(sizeof(uint8_t) * nw);
     }
     L.total = o;
     return true;
@@ -964,12 +972,31 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
         p.NRX = L.NRX;
         p.NRY = L.NRY;
         p.NR = L.NR;
-        p.ticket = (int *) (ws + L.o_ticket);
-        p.lb = (uint64_t *) (ws + L.o_lb);
         p.vofs = (uint32_t *) (ws + L.o_vofs);
         p.fofs = (uint32_t *) (ws + L.o_fofs);
+        p.wcnt_v = (uint32_t *) (ws + L.o_wcv);
+        p.wcnt_f = (uint32_t *) (ws + L.o_wcf);
         p.wb_v = (int64_t *) (ws + L.o_wbv);
         p.wb_f = (int64_t *) (ws + L.o_wbf);
+        const int64_t nchunk = (L.NR * iso::kStrWaves + iso::kStrScanChunk - 1) / iso::kStrScanChunk;
+        p.csum_v = (int64_t *) (ws + L.o_csum);
+        p.csum_f = p.csum_v + nchunk;
+        p.cofs_v = (int64_t *) (ws + L.o_cofs);
+        p.cofs_f = p.cofs_v + nchunk;
+        p.arena_ctr = (unsigned long long *) (ws + L.o_actr);
+        p.arena_off = (int32_t *) (ws + L.o_aoff);
+        p.wover = (uint8_t *) (ws + L.o_wover);
+        // the vertex arena: the face buffer, written by the faces kernel only
+        // after the flush has emptied it (16-byte entries, 16-byte aligned)
+        p.arena = nullptr;
+        p.arena_seg_cap = 0;
+        if (out_f && cap_f > 0) {
+            const uintptr_t a = ((uintptr_t) out_f + 15) & ~(uintptr_t) 15;
+            const int64_t bytes = (int64_t) ((uintptr_t) out_f + 12 * (uintptr_t) cap_f - a);
+            const int64_t entries = std::min<int64_t>(bytes / 16, (int64_t) INT32_MAX);
+            p.arena = (void *) a;
+            p.arena_seg_cap = entries / iso::kStrArenaSeg;
+        }
         HIP_CHECK(iso::mc_stream_launch(p, sp_host, st, ev), "launch marching cubes");
     } else {
         HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, ev), "launch marching cubes");

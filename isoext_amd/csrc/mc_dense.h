@@ -117,16 +117,26 @@ struct McParams {
     int tx, ty;
     int tile_base;   // id of the grid's first tile in the list's id space (a batch's tile ids)
     UDiv32 div_s, div_tx, div_ty;
-    // read-once pipeline (k_stream, then k_faces): regions of kStrRX x-slabs x
-    // kStrRY point rows x one 64-point word, ordered (x block, y block, word)
-    // with the word fastest; a region's four waves own 8 rows each
+    // read-once pipeline (k_stream, k_stream_scan*, k_stream_flush, k_faces):
+    // regions of kStrRX x-slabs x kStrRY point rows x one 64-point word,
+    // ordered (x block, y block, word) with the word fastest; a region's
+    // four waves own 8 rows each
     int64_t NRX, NRY, NR;
-    int *ticket;         // regions are taken in dispatch order (one atomic per workgroup)
-    uint64_t *lb;        // decoupled look-back word per region
-    uint32_t *vofs;      // (X*Y, W): vertices of the wave before entry (point row, word)
-    uint32_t *fofs;      // (X*Y, W): faces of the wave before entry (cell row, word)
-    int64_t *wb_v;       // (NR, 4): first vertex id of each wave
-    int64_t *wb_f;       // (NR, 4): first face id of each wave
+    uint32_t *vofs;      // region-major entries: vertices of the wave before (point row, word)
+    uint32_t *fofs;      // region-major entries: faces of the wave before (cell row, word)
+    uint32_t *wcnt_v;    // (NR, 4): vertices of each wave
+    uint32_t *wcnt_f;    // (NR, 4): faces of each wave
+    int64_t *wb_v;       // (NR, 4): first vertex id of each wave (chunk-relative until the flush)
+    int64_t *wb_f;       // (NR, 4): first face id of each wave (chunk-relative until the flush)
+    int64_t *csum_v, *csum_f;   // per chunk of kStrScanChunk waves: vertices / faces
+    int64_t *cofs_v, *cofs_f;   // per chunk: first vertex / face id
+    // vertex arena: k_stream writes each slab's vertices where a segment
+    // counter puts them, k_stream_flush copies them to their final ids
+    void *arena;         // kStrArenaSeg segments of arena_seg_cap entries (the face buffer, not yet used)
+    int64_t arena_seg_cap;
+    unsigned long long *arena_ctr;   // one counter per segment, on its own 256-byte line
+    int32_t *arena_off;  // (NR, 4, kStrRX): arena entry of each slab's first vertex
+    uint8_t *wover;      // (NR, 4): the wave's vertices did not fit its arena segment
 };
 
 // Read-once pipeline geometry.
@@ -134,10 +144,11 @@ constexpr int kStrRX = 16;        // x-slabs per region
 constexpr int kStrRows = 8;       // point rows per wave
 constexpr int kStrWaves = 4;
 constexpr int kStrRY = kStrRows * kStrWaves;
-constexpr int kStrCap = 512;      // vertices a wave stages in LDS before its base is known
+constexpr int kStrArenaSeg = 256;  // arena segments (spread the reservation atomics)
+constexpr int kStrScanChunk = 256; // waves per first-level scan block
 // The read-once pipeline serves single grids whose emit tiles are 16 x 16
-// cell rows x one word (tile configuration 3).
-constexpr int kStrTileConfig = 3;
+// cell rows x one word (tile configuration 6).
+constexpr int kStrTileConfig = 6;
 constexpr int kMaxFusedSpheres = 256;
 
 // Fill McParams' invariant-divisor fields from X, Y, W and nzc.
@@ -196,8 +207,9 @@ int mc_auto_config(const int64_t shape[3]);
 bool mc_config_geom(int cfg, int &tx, int &ty, int &nzc);
 hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
                            hipEvent_t *ev = nullptr);
-// The read-once pipeline: reset, k_stream (masks, counts, vertices, look-back
-// bases), k_faces over the listed tiles, special gather.  Same outputs as
+// The read-once pipeline: reset, k_stream (masks, counts, vertices into the
+// arena), k_stream_scan (wave bases), k_stream_flush (vertices to their
+// ids), k_faces over the listed tiles, special gather.  Same outputs as
 // mc_dense_launch up to the order of vertices and faces.
 hipError_t mc_stream_launch(McParams &p, const SpecialLists &sp_init, hipStream_t stream, hipEvent_t *ev = nullptr);
 // Batch phases: classify + count + scan of every item, then emit + gather.

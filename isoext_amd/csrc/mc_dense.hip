@@ -1690,7 +1690,8 @@ __global__ void k_its_reset(McParams p) {
     X(2, 16, 16, 2, 256)    \
     X(3, 16, 16, 1, 256)    \
     X(4, 16, 16, 8, 256)    \
-    X(5, 8, 8, 1, 256)
+    X(5, 8, 8, 1, 256)      \
+    X(6, 16, 16, 1, 256)
 
 int mc_config_count() {
     int n = 0;
@@ -1951,22 +1952,25 @@ hipError_t mc_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_e
 
 // ==== read-once pipeline ============================================================
 // k_stream reads every value once.  Workgroup = region of kStrRX x-slabs x
-// kStrRY point rows x one 64-point word, taken by ticket in dispatch order;
-// wave k owns point rows [8k, 8k + 8) of the region and slides along x with
-// two slabs of its rows (plus the row above, and the first value of the next
-// word) in registers.  Per slab and row: the inside mask (ballot), the
-// vertices the row's points own (x-, y-, z-edges whose endpoint masks differ)
-// and the faces of the cell row (LUT triangle counts of the active cells).
-// Vertex positions are computed while the values are in registers and staged
-// in LDS; when the region's counts are known, a decoupled look-back over
-// the region tickets gives its first vertex and face id, and the staged
-// vertices are written in final order.  A wave whose region holds more than
-// kStrCap vertices re-reads its rows after the look-back and writes them
-// directly.  Masks and per-row offsets are written region-major
-// (str_entry), each wave's 16 x 8 entries as whole lines.
-// Vertex order: region, wave, slab, row, z, axis (x < y < z); face order:
-// region, wave, slab, row, z, LUT slot.  k_faces then emits the faces of the
-// listed tiles from the masks and offsets alone.
+// kStrRY point rows x one 64-point word; the regions are spread so that
+// each XCD sweeps one contiguous part of the grid (the neighbouring words
+// and rows whose halo values a region reads are then in flight on the same
+// L2).  Wave k owns point rows [8k, 8k + 8) of the region and slides along
+// x with the rows of the next slabs in flight.  Per slab it ballots the
+// inside masks of its 9 rows (the 8 it owns and the one above) into a
+// per-wave LDS table and checks whether the slab is all inside or all
+// outside; only a slab that is not uniform with its successor does more.
+// There one lane per row derives the row's crossing-edge and active-cell
+// masks, and the slab's vertices and active cells are each handled
+// compacted, one per lane: the vertex positions go to a segment of the
+// arena (one atomic per slab), the cells' LUT triangle counts give each
+// row's first face.  Vertex order in a wave: slab, row, axis (x < y < z),
+// z; face order: slab, row, z, LUT slot.  At the end the wave writes its
+// masks and per-row offsets region-major (str_entry) as whole lines.
+// k_stream_scan1/2 turn the waves' counts into first ids, k_stream_flush
+// moves the vertices from the arena to their ids (and flags the rows that
+// own weld candidates), and k_faces emits the faces of the listed tiles
+// from the masks and offsets alone.
 
 // Region-major index of point row (x, y), word w.
 __device__ __forceinline__ int64_t str_entry(const McParams &p, int64_t x, int64_t y, int64_t w) {
@@ -1980,409 +1984,579 @@ __device__ __forceinline__ int64_t str_wave(const McParams &p, int64_t x, int64_
     const int64_t r = ((x / kStrRX) * p.NRY + y / kStrRY) * p.W + w;
     return r * kStrWaves + ((y % kStrRY) >> 3);
 }
-
-// Look-back word of a region: status in bits 62-63 (1: the region's own
-// counts, 2: inclusive prefix), vertices in bits 31-61, faces in bits 0-30.
-constexpr uint64_t kLbAgg = 1ull << 62, kLbIncl = 2ull << 62, kLbField = (1ull << 31) - 1;
-constexpr int kLbSpinLimit = 1 << 22;
+// Region of k_stream workgroup b: XCD b % 8 takes regions [k * per, (k + 1) * per).
+__device__ __forceinline__ int64_t str_region(int64_t b, int64_t NR) {
+    const int64_t per = (NR + 7) / 8;
+    return (b & 7) * per + (b >> 3);
+}
 
 // Bit 31 of a vofs entry: the point row owns a weld-candidate vertex in the word.
 constexpr uint32_t kStrCandFlag = 0x80000000u;
 
+// An arena entry: a vertex position and where it came from.
 struct StrStage {
     float x, y, z;
-    uint32_t info;   // axis (bits 0-1), lane (2-7), row (8-10), slab (11-15), candidate (16)
+    uint32_t info;   // axis (bits 0-1), z (2-7), row (8-10), slab (11-15), candidate (16)
 };
 
-struct StrWaveLds {
-    float v[2][kStrRows + 1][64];   // values of two slabs (by slab parity), rows 0..8
-    float h[2][kStrRows + 1];       // z-halo values (first point of the next word)
-    uint64_t m[2][kStrRows + 1];    // inside masks
-    uint32_t hb[2];                 // inside bits of the z-halo points, bit j = row j
-};
-
-struct StreamLds {
-    StrStage st[kStrWaves][kStrCap];
-    StrWaveLds wv[kStrWaves];
-    uint8_t ntri[256];
-    int64_t region;
-    uint32_t cv[kStrWaves], cf[kStrWaves];
-    int64_t base_v, base_f;
-};
-
-// Values of the wave's rows (8 own + the row above) of one slab at this
-// lane's z, and the first value of the next word (z-halo), in flight.
+// One slab of the wave's rows: rows 0..8 at this lane's z, and in lanes
+// 0..8 the z-halo value of row `lane` (the next word's first point).
 struct StrSlab {
     float v[kStrRows + 1];
-    float h[kStrRows + 1];
+    float h;
+};
+
+// Per-wave LDS of k_stream.
+struct StrLds {
+    float v[2][kStrRows + 1][64];            // values of slabs st (slot st & 1) and st + 1
+    float h[2][kStrRows + 1];                // their z-halo values
+    uint64_t mt[kStrRX + 1][kStrRows + 1];   // inside masks: slabs 0..16 x rows 0..8
+    uint32_t ht[kStrRX + 1];                 // z-halo inside bits of each slab (bit j: row j)
+    uint32_t vt[kStrRX * kStrRows];          // first vertex of each (slab, row), wave-relative
+    uint32_t ft[kStrRX * kStrRows];          // first face of each (slab, row), wave-relative
+    uint64_t em[kStrRows][4];                // surface slab: x / y / z crossing edges, active cells per row
+};
+// Grid coordinates of a region's points.
+struct StrCoords {
+    float cx[kStrRX + 1];
+    float cy[kStrRY + 1];
+    float cz[65];
 };
 
 __device__ __forceinline__ void str_load(const McParams &p, int64_t x, int64_t y0, int nrow, int64_t z0, int lane,
                                          bool zin, bool hz, StrSlab &o) {
-    const int64_t base = (x * p.Y + y0) * p.Z + z0;
+    const float *base = p.values + (x * p.Y + y0) * p.Z + z0;
 #pragma unroll
     for (int j = 0; j <= kStrRows; ++j) {
-        const float *row = p.values + base + (int64_t) j * p.Z;
-        // rows 1..7 are read by this wave only: nontemporal; row 0 and the
-        // row above are also another wave's rows, kept in L2 for it
+        const float *row = base + (int64_t) j * p.Z;
         if (j < nrow) {
-            if (j == 0 || j == kStrRows) {
-                o.v[j] = zin ? row[lane] : 0.0f;
-            } else {
-                o.v[j] = zin ? __builtin_nontemporal_load(row + lane) : 0.0f;
-            }
-            o.h[j] = hz ? row[64] : 0.0f;
+            // rows 1..7 are read by this wave only: nontemporal; row 0 and the
+            // row above are also another wave's rows, kept in L2 for it
+            if (j == 0 || j == kStrRows) o.v[j] = zin ? row[lane] : 0.0f;
+            else o.v[j] = zin ? __builtin_nontemporal_load(row + lane) : 0.0f;
         } else {
             o.v[j] = 0.0f;
-            o.h[j] = 0.0f;
         }
     }
+    o.h = (hz && lane < nrow) ? base[(int64_t) lane * p.Z + 64] : 0.0f;
 }
 
-// Masks of a loaded slab, then values, halos and masks into the wave's LDS
-// slot (nothing of the slab stays in registers).
-__device__ __forceinline__ void str_commit(const StrSlab &sl, int nrow, bool zin, bool hz, float level, int lane,
-                                           StrWaveLds &w, int slot) {
-    uint32_t hbits = 0;
+// Region geometry of the calling wave.
+struct StrWave {
+    int64_t r, w, rx, ry, x0, y0, z0;
+    int nx, ny, nrow;
+    bool zin, hz;
+    uint64_t pv, ev;
+    float cz0, cz1;
+};
+__device__ __forceinline__ StrWave str_wave_geom(const McParams &p, int64_t r, int wave, int lane) {
+    StrWave g;
+    g.r = r;
+    g.w = r % p.W;
+    const int64_t rt = r / p.W;
+    g.ry = rt % p.NRY;
+    g.rx = rt / p.NRY;
+    g.x0 = g.rx * kStrRX;
+    g.y0 = g.ry * kStrRY + wave * kStrRows;
+    g.z0 = 64 * g.w;
+    g.nx = (int) min<int64_t>(kStrRX, p.X - g.x0);
+    g.ny = (int) max<int64_t>(0, min<int64_t>(kStrRows, p.Y - g.y0));
+    g.nrow = (int) max<int64_t>(0, min<int64_t>(kStrRows + 1, p.Y - g.y0));   // rows incl. the one above
+    g.zin = g.z0 + lane < p.Z;
+    g.hz = g.z0 + 64 < p.Z;
+    g.pv = low_bits(p.Z - g.z0);
+    g.ev = low_bits(p.Z - 1 - g.z0);
+    g.cz0 = grid_coord(g.z0 + lane, p.Z, p.amin[2], p.amax[2]);
+    g.cz1 = grid_coord(g.z0 + lane + 1, p.Z, p.amin[2], p.amax[2]);
+    return g;
+}
+
+// Vertex on a crossing edge from the value v0 at its owner point to v1;
+// cand: the along-axis coordinate is within 16 ulps of an endpoint's.
+__device__ __forceinline__ float3 str_vertex(float v0, float v1, float level, int axis, float cx0, float cx1,
+                                             float cy0, float cy1, float cz0, float cz1, bool all_special,
+                                             bool &cand) {
+    const float tt = edge_t(v0, v1, level);
+    const float px = lerp1(tt, cx0, axis == 0 ? cx1 : cx0);
+    const float py = lerp1(tt, cy0, axis == 1 ? cy1 : cy0);
+    const float pz = lerp1(tt, cz0, axis == 2 ? cz1 : cz0);
+    const float ra = axis == 0 ? px : (axis == 1 ? py : pz);
+    const float a = axis == 0 ? cx0 : (axis == 1 ? cy0 : cz0);
+    const float b = axis == 0 ? cx1 : (axis == 1 ? cy1 : cz1);
+    cand = all_special || near_endpoint(ra, a, b);
+    return make_float3(px, py, pz);
+}
+
+// Inclusive sum over lanes 0..15 (one DPP row) of each lane's v.
+__device__ __forceinline__ uint32_t str_row16_incl(uint32_t v) {
+    v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    return v;
+}
+
+// Commit slab s: inside masks to mt[s] (lane j: row j), halo bits to
+// ht[s], values to ring slot s & 1; uin / uout: every point of the slab
+// (its 9 rows and their halo values) is inside / outside.
+__device__ __forceinline__ void str_commit(const StrSlab &sl, const StrWave &g, float level, int lane, StrLds &w,
+                                           int s, bool &uin, bool &uout) {
+    const int slot = s & 1;
+    uint64_t m[kStrRows + 1];
+    uint64_t all = ~0ull, any = 0ull;
 #pragma unroll
     for (int j = 0; j <= kStrRows; ++j) {
-        const uint64_t m = j < nrow ? __ballot(zin && inside(sl.v[j], level)) : 0ull;
-        if (j < nrow && hz && inside(sl.h[j], level)) hbits |= 1u << j;
+        m[j] = __ballot(g.zin && j < g.nrow && inside(sl.v[j], level));
         w.v[slot][j][lane] = sl.v[j];
-        if (lane == 0) {
-            w.m[slot][j] = m;
-            w.h[slot][j] = sl.h[j];
+        if (j < g.nrow) {
+            all &= m[j];
+            any |= m[j];
         }
     }
-    if (lane == 0) w.hb[slot] = hbits;
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ void str_clear(int lane, StrWaveLds &w, int slot) {
-    if (lane <= kStrRows) w.m[slot][lane] = 0ull;
-    if (lane == 0) w.hb[slot] = 0u;
-    __builtin_amdgcn_wave_barrier();
-}
-
-// A wave-uniform 64-bit word from LDS, in scalar registers.
-__device__ __forceinline__ uint64_t lds_uniform(const uint64_t &a) {
-    const uint64_t v = a;
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t) v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t) (v >> 32));
-    return ((uint64_t) hi << 32) | lo;
-}
-
-// One slab of the wave's rows: vertices (MODE 0: counted and staged; MODE 1:
-// written at base + their wave-relative id, the slow path of an overfull
-// wave) and, in MODE 0, face counts.  Slab x is LDS slot `cs`, slab x + 1
-// slot `ns`.
-template <int MODE, bool KEYED>
-__device__ __forceinline__ void str_slab(const McParams &p, StreamLds &s, int wave, int lane, int step, int64_t x,
-                                         int64_t y0, int ny, int64_t z0, bool hasx, int cs, int ns, uint64_t pv,
-                                         uint64_t ev, float cz0, float cz1, uint32_t &vcount, uint32_t &fcount,
-                                         uint32_t &my_vofs, uint32_t &my_fofs, bool &any_faces, int64_t base,
-                                         int spseg) {
-    StrWaveLds &w = s.wv[wave];
-    const uint64_t below = (1ull << lane) - 1ull;
-    const float cx0 = grid_coord(x, p.X, p.amin[0], p.amax[0]);
-    const float cx1 = grid_coord(x + 1, p.X, p.amin[0], p.amax[0]);
-    const uint32_t hbc = __builtin_amdgcn_readfirstlane(w.hb[cs]);
-    const uint32_t hbn = __builtin_amdgcn_readfirstlane(w.hb[ns]);
-#pragma unroll 1
-    for (int j = 0; j < ny; ++j) {
-        const int64_t y = y0 + j;
-        const bool hasy = y + 1 < p.Y;
-        const uint64_t m = lds_uniform(w.m[cs][j]);
-        const uint64_t my = lds_uniform(w.m[cs][j + 1]);
-        const uint64_t mx = lds_uniform(w.m[ns][j]);
-        const uint64_t zs = (m >> 1) | ((uint64_t) ((hbc >> j) & 1u) << 63);
-        const uint64_t zm = (m ^ zs) & ev;
-        const uint64_t ym = hasy ? (m ^ my) & pv : 0ull;
-        const uint64_t xm = hasx ? (m ^ mx) & pv : 0ull;
-        const uint32_t nvr = (uint32_t) (__popcll(xm) + __popcll(ym) + __popcll(zm));
-        if (MODE == 0 && lane == j) my_vofs = vcount;
-        bool row_cand = false;
-        if (nvr) {
-            const float v0 = w.v[cs][j][lane];
-            float vz = w.v[cs][j][lane < 63 ? lane + 1 : 63];
-            if (lane == 63) vz = w.h[cs][j];
-            uint32_t idx = vcount + (uint32_t) (__popcll(xm & below) + __popcll(ym & below) + __popcll(zm & below));
-            const float cy0 = grid_coord(y, p.Y, p.amin[1], p.amax[1]);
-            const float cy1 = grid_coord(y + 1, p.Y, p.amin[1], p.amax[1]);
-            const int64_t owner = (x * p.Y + y) * p.Z + z0 + lane;
-            const uint32_t mine = (uint32_t) ((xm >> lane) & 1ull) | (uint32_t) ((ym >> lane) & 1ull) << 1 |
-                                  (uint32_t) ((zm >> lane) & 1ull) << 2;
+    const uint32_t hb = (uint32_t) __ballot(g.hz && lane < g.nrow && inside(sl.h, level));
+    if (lane <= kStrRows) w.h[slot][lane] = sl.h;
+    const uint32_t hall = g.hz ? (uint32_t) low_bits(g.nrow) : 0u;
+    uin = (all & g.pv) == g.pv && hb == hall;
+    uout = any == 0ull && hb == 0u;
+    uint64_t mine;
+    if (uin || uout) {
+        mine = (uin && lane < g.nrow) ? g.pv : 0ull;
+    } else {
+        mine = 0ull;
 #pragma unroll
-            for (int axis = 0; axis < 3; ++axis) {
-                if (!((mine >> axis) & 1u)) continue;
-                const float v1 = axis == 0 ? w.v[ns][j][lane] : (axis == 1 ? w.v[cs][j + 1][lane] : vz);
-                const float tt = edge_t(v0, v1, p.level);
-                const float px = lerp1(tt, cx0, axis == 0 ? cx1 : cx0);
-                const float py = lerp1(tt, cy0, axis == 1 ? cy1 : cy0);
-                const float pz = lerp1(tt, cz0, axis == 2 ? cz1 : cz0);
-                const float ra = axis == 0 ? px : (axis == 1 ? py : pz);
-                const float a = axis == 0 ? cx0 : (axis == 1 ? cy0 : cz0);
-                const float b = axis == 0 ? cx1 : (axis == 1 ? cy1 : cz1);
-                const bool cand = p.all_special || near_endpoint(ra, a, b);
-                row_cand |= cand;
-                if constexpr (MODE == 0) {
-                    if (idx < (uint32_t) kStrCap) {
-                        StrStage e;
-                        e.x = px;
-                        e.y = py;
-                        e.z = pz;
-                        e.info = (uint32_t) axis | ((uint32_t) lane << 2) | ((uint32_t) j << 8) |
-                                 ((uint32_t) step << 11) | ((uint32_t) cand << 16);
-                        s.st[wave][idx] = e;
-                    }
-                } else {
-                    const int64_t vid = base + idx;
-                    if (vid < p.cap_v) {
-                        *reinterpret_cast<float3 *>(p.out_v + 3 * vid) = make_float3(px, py, pz);
-                        if (KEYED) p.out_key[vid] = owner * 3 + axis;
-                        if (cand) {
-                            const int64_t i = seg_push(p, 0, spseg);
-                            if (i >= 0) {
-                                SpecialVertex sv;
-                                sv.vid = vid;
-                                sv.edge = owner * 3 + axis;
-                                sv.x = px;
-                                sv.y = py;
-                                sv.z = pz;
-                                sv.pad = 0.0f;
-                                p.seg.cand[i] = sv;
-                            }
+        for (int j = 0; j <= kStrRows; ++j) mine = lane == j ? m[j] : mine;
+    }
+    if (lane <= kStrRows) w.mt[s][lane] = mine;
+    if (lane == 0) w.ht[s] = hb;
+}
+
+// Largest q in [0, 8) with P[q] <= i (P[0] = 0, P nondecreasing), and P[q].
+__device__ __forceinline__ int str_row_of(uint32_t i, const uint32_t (&P)[kStrRows + 1], uint32_t &base) {
+    int r = 0;
+    base = 0;
+#pragma unroll
+    for (int q = 1; q < kStrRows; ++q) {
+        const bool ge = i >= P[q];
+        r += ge ? 1 : 0;
+        base = ge ? P[q] : base;
+    }
+    return r;
+}
+
+// The work of a slab that is not uniform with its successor: vertices into
+// the arena, per-row first vertex / face (lanes 0..7: vrow, frow).
+template <bool KEYED>
+__device__ __forceinline__ void str_surface(const McParams &p, const StrWave &g, StrLds &w, const StrCoords &sc,
+                                            const uint8_t *ntri, int lane, int st, bool hasx, int seg,
+                                            uint32_t &vrow, uint32_t &frow, uint32_t &vcount, uint32_t &fcount,
+                                            bool &over, int32_t &my_off, bool &any_faces) {
+    const int cs = st & 1, ns = cs ^ 1;
+    const int j = lane & 7;
+    const uint32_t hc_all = w.ht[st], hn_all = hasx ? w.ht[st + 1] : 0u;
+    // one lane per row: crossing edges and active cells
+    uint64_t xm = 0, ym = 0, zm = 0, act = 0;
+    if (lane < g.ny) {
+        const uint64_t m = w.mt[st][j], my = w.mt[st][j + 1];
+        const uint64_t mx = hasx ? w.mt[st + 1][j] : 0ull, mxy = hasx ? w.mt[st + 1][j + 1] : 0ull;
+        const bool yok = j + 1 < g.nrow;
+        const uint32_t hc = (hc_all >> j) & 1u, hcy = (hc_all >> (j + 1)) & 1u;
+        const uint32_t hn = (hn_all >> j) & 1u, hny = (hn_all >> (j + 1)) & 1u;
+        zm = (m ^ ((m >> 1) | ((uint64_t) hc << 63))) & g.ev;
+        if (yok) ym = (m ^ my) & g.pv;
+        if (hasx) xm = (m ^ mx) & g.pv;
+        if (hasx && yok) {
+            const uint64_t o = m | my | mx | mxy, n = m & my & mx & mxy;
+            const uint64_t o1 = (o >> 1) | ((uint64_t) ((hc | hcy | hn | hny) != 0u) << 63);
+            const uint64_t n1 = (n >> 1) | ((uint64_t) ((hc & hcy & hn & hny) != 0u) << 63);
+            act = (o | o1) & ~(n & n1) & g.ev;
+        }
+    }
+    if (lane < kStrRows) {
+        w.em[j][0] = xm;
+        w.em[j][1] = ym;
+        w.em[j][2] = zm;
+        w.em[j][3] = act;
+    }
+    const uint32_t nv = (uint32_t) (__popcll(xm) + __popcll(ym) + __popcll(zm)), na = (uint32_t) __popcll(act);
+    const uint32_t vin = str_row16_incl(nv), ain = str_row16_incl(na);
+    uint32_t VP[kStrRows + 1], AP[kStrRows + 1];
+    VP[0] = AP[0] = 0;
+#pragma unroll
+    for (int q = 0; q < kStrRows; ++q) {
+        VP[q + 1] = (uint32_t) __builtin_amdgcn_readlane((int) vin, q);
+        AP[q + 1] = (uint32_t) __builtin_amdgcn_readlane((int) ain, q);
+    }
+    const uint32_t nslab = VP[kStrRows], nact = AP[kStrRows];
+    vrow = vcount + vin - nv;
+    __builtin_amdgcn_wave_barrier();
+    // vertices, one per lane, into the arena
+    if (nslab && !over) {
+        unsigned long long off = 0;
+        if (lane == 0) off = atomicAdd(&p.arena_ctr[seg * kSpStride], (unsigned long long) nslab);
+        off = __shfl(off, 0);
+        if ((int64_t) off + nslab > p.arena_seg_cap) {
+            over = true;
+        } else {
+            const int64_t a0 = (int64_t) seg * p.arena_seg_cap + (int64_t) off;
+            if (lane == st) my_off = (int32_t) a0;
+            StrStage *arena = reinterpret_cast<StrStage *>(p.arena) + a0;
+            const float cx0 = sc.cx[st], cx1 = sc.cx[st + 1];
+            const int yl0 = (int) (g.y0 - g.ry * kStrRY);
+            for (uint32_t b = 0; b < nslab; b += 64) {
+                const uint32_t i = b + (uint32_t) lane;
+                if (i < nslab) {
+                    uint32_t base;
+                    const int rr = str_row_of(i, VP, base);
+                    const uint64_t exm = w.em[rr][0], eym = w.em[rr][1], ezm = w.em[rr][2];
+                    const uint32_t nxm = (uint32_t) __popcll(exm), nym = (uint32_t) __popcll(eym);
+                    uint32_t k = i - base;
+                    int axis = 0;
+                    uint64_t mm = exm;
+                    if (k >= nxm) {
+                        k -= nxm;
+                        axis = 1;
+                        mm = eym;
+                        if (k >= nym) {
+                            k -= nym;
+                            axis = 2;
+                            mm = ezm;
                         }
                     }
+                    const int z = nth_set_bit(mm, (int) k);
+                    const float v0 = w.v[cs][rr][z];
+                    const float v1 = axis == 0 ? w.v[ns][rr][z]
+                                               : (axis == 1 ? w.v[cs][rr + 1][z] : (z < 63 ? w.v[cs][rr][z + 1] : w.h[cs][rr]));
+                    bool cand;
+                    const float3 pos = str_vertex(v0, v1, p.level, axis, cx0, cx1, sc.cy[yl0 + rr], sc.cy[yl0 + rr + 1],
+                                                  sc.cz[z], sc.cz[z + 1], p.all_special, cand);
+                    StrStage e;
+                    e.x = pos.x;
+                    e.y = pos.y;
+                    e.z = pos.z;
+                    e.info = (uint32_t) axis | ((uint32_t) z << 2) | ((uint32_t) rr << 8) | ((uint32_t) st << 11) |
+                             ((uint32_t) cand << 16);
+                    arena[i] = e;
                 }
-                ++idx;
             }
         }
-        vcount += nvr;
-        if constexpr (MODE == 0) {
-            // bit 31 of a row's offset: the row owns a weld candidate
-            if (__ballot(row_cand) && lane == j) my_vofs |= kStrCandFlag;
-            // faces of cell row (x, y): active cells have corners on both sides
-            if (lane == j) my_fofs = fcount;
-            if (!hasx || !hasy) continue;
-            const uint64_t mxy = lds_uniform(w.m[ns][j + 1]);
-            const uint32_t hj = (hbc >> j) & 3u, hn = (hbn >> j) & 3u;   // halo bits of rows j, j + 1
-            const uint64_t o = m | my | mx | mxy, n = m & my & mx & mxy;
-            const uint64_t o1 = (o >> 1) | ((uint64_t) ((hj | hn) != 0) << 63);
-            const uint64_t n1 = (n >> 1) | ((uint64_t) ((hj & hn) == 3u) << 63);
-            const uint64_t act = (o | o1) & ~(n & n1) & ev;
-            if (!act) continue;
+    }
+    // active cells, one per lane: LUT triangle counts -> each row's first face
+    if (nact) {
+        any_faces = true;
+        const uint32_t aex = ain - na;   // lanes 0..7: the row's first active cell
+        uint32_t F0 = 0, fr = 0;
+        for (uint32_t b = 0; b < nact; b += 64) {
+            const uint32_t i = b + (uint32_t) lane;
             int nt = 0;
-            if ((act >> lane) & 1ull) {
-                // corner k at (k >> 2, (k >> 1) & 1, k & 1): this lane's z and z + 1
+            if (i < nact) {
+                uint32_t base;
+                const int rr = str_row_of(i, AP, base);
+                const int z = nth_set_bit(w.em[rr][3], (int) (i - base));
+                const uint64_t a = w.mt[st][rr], ay = w.mt[st][rr + 1];
+                const uint64_t ax = w.mt[st + 1][rr], axy = w.mt[st + 1][rr + 1];
+                auto bit = [&](uint64_t mm) { return (int) ((mm >> z) & 1ull); };
                 auto up = [&](uint64_t mm, uint32_t h) {
-                    return lane < 63 ? (int) ((mm >> (lane + 1)) & 1ull) : (int) (h & 1u);
+                    return z < 63 ? (int) ((mm >> (z + 1)) & 1ull) : (int) (h & 1u);
                 };
-                const int c = (int) ((m >> lane) & 1ull) | up(m, hj) << 1 | (int) ((my >> lane) & 1ull) << 2 |
-                              up(my, hj >> 1) << 3 | (int) ((mx >> lane) & 1ull) << 4 | up(mx, hn) << 5 |
-                              (int) ((mxy >> lane) & 1ull) << 6 | up(mxy, hn >> 1) << 7;
-                nt = (int) s.ntri[c];
+                const int c = bit(a) | up(a, hc_all >> rr) << 1 | bit(ay) << 2 | up(ay, hc_all >> (rr + 1)) << 3 |
+                              bit(ax) << 4 | up(ax, hn_all >> rr) << 5 | bit(axy) << 6 |
+                              up(axy, hn_all >> (rr + 1)) << 7;
+                nt = (int) ntri[c];
             }
-            uint32_t nf = 0;
-#pragma unroll
-            for (int q = 1; q <= 5; ++q) nf += (uint32_t) __popcll(__ballot(nt >= q));
-            fcount += nf;
-            any_faces = true;
+            const int incl = wave_incl_sum(nt), excl = incl - nt;
+            const uint32_t tot = (uint32_t) __shfl(incl, 63);
+            const bool here = aex >= b && aex < b + 64;
+            const int ex = __shfl(excl, here ? (int) (aex - b) : lane);
+            if (lane < kStrRows && here) fr = F0 + (uint32_t) ex;
+            F0 += tot;
         }
+        if (aex >= nact) fr = F0;
+        frow = fcount + fr;
+        fcount += F0;
     }
-}
-
-// The wave's rows, slab by slab: loads run one slab ahead of the slab whose
-// masks are being committed, which runs one ahead of the slab processed.
-template <int MODE, bool KEYED>
-__device__ __forceinline__ void str_rows(const McParams &p, StreamLds &s, int wave, int lane, int64_t x0, int nx,
-                                         int64_t y0, int ny, int nrow, int64_t z0, bool zin, bool hz, uint64_t pv,
-                                         uint64_t ev, float cz0, float cz1, uint32_t &vcount, uint32_t &fcount,
-                                         bool &any_faces, uint64_t (&emask)[kStrRX / 8], uint32_t (&evofs)[kStrRX / 8],
-                                         uint32_t (&efofs)[kStrRX / 8], int64_t base, int spseg) {
-    StrWaveLds &w = s.wv[wave];
-    StrSlab R1, R2;
-    {
-        StrSlab R0;
-        str_load(p, x0, y0, nrow, z0, lane, zin, hz, R0);
-        if (x0 + 1 < p.X) str_load(p, x0 + 1, y0, nrow, z0, lane, zin, hz, R1);
-        str_commit(R0, nrow, zin, hz, p.level, lane, w, 0);
-    }
-    for (int st = 0; st < nx; ++st) {
-        const int64_t x = x0 + st;
-        const bool hasx = x + 1 < p.X;
-        const int cs = st & 1, ns = cs ^ 1;
-        if (st + 1 < nx && x + 2 < p.X) str_load(p, x + 2, y0, nrow, z0, lane, zin, hz, R2);
-        if (hasx) str_commit(R1, nrow, zin, hz, p.level, lane, w, ns);
-        else str_clear(lane, w, ns);
-        uint32_t my_vofs = 0, my_fofs = 0;
-        str_slab<MODE, KEYED>(p, s, wave, lane, st, x, y0, ny, z0, hasx, cs, ns, pv, ev, cz0, cz1, vcount, fcount,
-                              my_vofs, my_fofs, any_faces, base, spseg);
-        if constexpr (MODE == 0) {
-            // this slab's 8 entries go to lanes (st % 8) * 8 + j, slot st / 8
-            const uint64_t my_mask = lane < kStrRows ? w.m[cs][lane] : 0ull;
-            const int src = lane & 7;
-            const uint64_t mm = __shfl(my_mask, src);
-            const uint32_t vv = (uint32_t) __shfl((int) my_vofs, src), ff = (uint32_t) __shfl((int) my_fofs, src);
-            if ((lane >> 3) == (st & 7)) {
-#pragma unroll
-                for (int q = 0; q < kStrRX / 8; ++q)
-                    if ((st >> 3) == q) {
-                        emask[q] = mm;
-                        evofs[q] = vv;
-                        efofs[q] = ff;
-                    }
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        R1 = R2;
-    }
+    vcount += nslab;
 }
 
 template <bool KEYED>
 __global__ void __launch_bounds__(256) k_stream(McParams p) {
-    __shared__ StreamLds s;
+    __shared__ StrLds sw[kStrWaves];
+    __shared__ StrCoords sc;
+    __shared__ uint8_t ntri[256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) s.region = atomicAdd(p.ticket, 1);
-    s.ntri[tid] = (uint8_t) (cTris[p.method][tid] & 15);
-    __syncthreads();
-    const int64_t r = s.region;
-    const int64_t w = r % p.W, rt = r / p.W;
-    const int64_t ry = rt % p.NRY, rx = rt / p.NRY;
-    const int64_t x0 = rx * kStrRX, y0 = ry * kStrRY + wave * kStrRows, z0 = 64 * w;
-    const int nx = (int) min<int64_t>(kStrRX, p.X - x0);
-    const int ny = (int) max<int64_t>(0, min<int64_t>(kStrRows, p.Y - y0));
-    const int nrow = (int) max<int64_t>(0, min<int64_t>(kStrRows + 1, p.Y - y0));   // rows incl. the one above
-    const bool zin = z0 + lane < p.Z, hz = z0 + 64 < p.Z;
-    const uint64_t pv = low_bits(p.Z - z0), ev = low_bits(p.Z - 1 - z0);
-    const float cz0 = grid_coord(z0 + lane, p.Z, p.amin[2], p.amax[2]);
-    const float cz1 = grid_coord(z0 + lane + 1, p.Z, p.amin[2], p.amax[2]);
-    const int spseg = (int) (r % p.seg.nseg);
-
-    uint32_t vcount = 0, fcount = 0;
-    bool any_faces = false;
-    // lane (s % 8) * 8 + j holds entry (slab s, row j) in slot s / 8
-    uint64_t emask[kStrRX / 8] = {};
-    uint32_t evofs[kStrRX / 8] = {}, efofs[kStrRX / 8] = {};
-    if (ny > 0)
-        str_rows<0, KEYED>(p, s, wave, lane, x0, nx, y0, ny, nrow, z0, zin, hz, pv, ev, cz0, cz1, vcount, fcount,
-                           any_faces, emask, evofs, efofs, 0, spseg);
-    // region-major entries of this wave: whole lines
+    const int64_t r = str_region(blockIdx.x, p.NR);
+    if (r >= p.NR) return;
+    const StrWave g = str_wave_geom(p, r, wave, lane);
+    ntri[tid] = (uint8_t) (cTris[p.method][tid] & 15);
     {
-        const int64_t e0 = (r * kStrWaves + wave) * (int64_t) (kStrRX * kStrRows);
-#pragma unroll
-        for (int q = 0; q < kStrRX / 8; ++q) {
-            const int64_t e = e0 + q * 64 + lane;
-            p.bits[e] = emask[q];
-            p.vofs[e] = evofs[q];
-            p.fofs[e] = efofs[q];
+        const int64_t xr = g.rx * kStrRX, yr = g.ry * kStrRY;
+        if (tid <= kStrRX) {
+            sc.cx[tid] = grid_coord(xr + tid, p.X, p.amin[0], p.amax[0]);
+        } else if (tid <= kStrRX + 1 + kStrRY) {
+            const int k = tid - kStrRX - 1;
+            sc.cy[k] = grid_coord(yr + k, p.Y, p.amin[1], p.amax[1]);
+        } else if (tid <= kStrRX + kStrRY + 2 + 64) {
+            const int k = tid - kStrRX - kStrRY - 2;
+            sc.cz[k] = grid_coord(g.z0 + k, p.Z, p.amin[2], p.amax[2]);
         }
     }
+    __syncthreads();
+    const int64_t wid = r * kStrWaves + wave;
+    StrLds &w = sw[wave];
+    const int seg = (int) (wid % kStrArenaSeg);
+    uint32_t vcount = 0, fcount = 0;
+    bool any_faces = false, over = false;
+    int32_t my_off = 0;   // lane s: arena entry of slab s's first vertex
+    if (g.ny > 0) {
+        const float level = p.level;
+        StrSlab S1, S2, S3;
+        bool uin0, uout0, uin1 = false, uout1 = false;
+        {
+            StrSlab S0;
+            str_load(p, g.x0, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, S0);
+            if (g.x0 + 1 < p.X) str_load(p, g.x0 + 1, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, S1);
+            if (1 < g.nx && g.x0 + 2 < p.X) str_load(p, g.x0 + 2, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, S2);
+            str_commit(S0, g, level, lane, w, 0, uin0, uout0);
+        }
+        for (int st = 0; st < g.nx; ++st) {
+            const int64_t x = g.x0 + st;
+            const bool hasx = x + 1 < p.X;
+            // slab st + 3 in flight (at most the next region's first slab)
+            if (st + 2 < g.nx && x + 3 < p.X) str_load(p, x + 3, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, S3);
+            if (hasx) str_commit(S1, g, level, lane, w, st + 1, uin1, uout1);
+            __builtin_amdgcn_wave_barrier();
+            const bool trivial = hasx ? ((uin0 && uin1) || (uout0 && uout1)) : (uin0 || uout0);
+            uint32_t vrow = vcount, frow = fcount;
+            if (!trivial)
+                str_surface<KEYED>(p, g, w, sc, ntri, lane, st, hasx, seg, vrow, frow, vcount, fcount, over, my_off,
+                                   any_faces);
+            if (lane < kStrRows) {
+                w.vt[st * kStrRows + lane] = vrow;
+                w.ft[st * kStrRows + lane] = frow;
+            }
+            S1 = S2;
+            S2 = S3;
+            uin0 = uin1;
+            uout0 = uout1;
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    // region-major entries of this wave: whole lines
+    const int64_t e0 = wid * (int64_t) (kStrRX * kStrRows);
+#pragma unroll
+    for (int q = 0; q < kStrRX * kStrRows / 64; ++q) {
+        const int e = q * 64 + lane, s = e / kStrRows, j = e % kStrRows;
+        const bool ok = g.ny > 0 && s < g.nx;
+        p.bits[e0 + e] = ok ? w.mt[s][j] : 0ull;
+        p.vofs[e0 + e] = ok ? w.vt[e] : 0u;
+        p.fofs[e0 + e] = ok ? w.ft[e] : 0u;
+    }
+    if (lane < kStrRX) p.arena_off[wid * kStrRX + lane] = my_off;
     if (lane == 0) {
-        s.cv[wave] = vcount;
-        s.cf[wave] = fcount;
+        p.wcnt_v[wid] = vcount;
+        p.wcnt_f[wid] = fcount;
+        p.wover[wid] = over ? 1 : 0;
     }
     // the wave's emit tile (16 x 16 cell rows, one word) has a face
     if (any_faces && lane == 0 && p.tile_list) {
-        const int64_t tj = (ry * kStrRY + wave * kStrRows) / 16;
-        const int t = (int) ((rx * p.NJ + tj) * p.NK + w);
+        const int64_t tj = g.y0 / 16;
+        const int t = (int) ((g.rx * p.NJ + tj) * p.NK + g.w);
         if (atomicCAS(&p.tile_flag[t], 0, 1) == 0) p.tile_list[atomicAdd(p.tile_count, 1)] = t;
     }
-    __syncthreads();
-    // decoupled look-back: publish the region's counts, add the predecessors'
-    if (wave == 0) {
-        uint64_t tv = 0, tf = 0;
-#pragma unroll
-        for (int k = 0; k < kStrWaves; ++k) {
-            tv += s.cv[k];
-            tf += s.cf[k];
-        }
-        if (lane == 0)
-            __hip_atomic_store(&p.lb[r], kLbAgg | (tv << 31) | tf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint64_t ev_ = 0, ef_ = 0;   // exclusive prefix
-        int64_t j = r - 1;
-        int spins = 0;
-        while (j >= 0) {
-            const int64_t idx = j - lane;
-            const uint64_t wd = idx >= 0 ? __hip_atomic_load(&p.lb[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                         : kLbIncl;
-            const uint64_t st = wd >> 62;
-            const uint64_t incl = __ballot(st == 2), none = __ballot(st == 0);
-            const int first = incl ? __ffsll((unsigned long long) incl) - 1 : 64;
-            if (none & low_bits(first + 1)) {
-                if (++spins > kLbSpinLimit) {
-                    if (lane == 0) set_err(p, ISO_ERR_SPIN);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            uint64_t cv = lane <= first ? (wd >> 31) & kLbField : 0, cf = lane <= first ? wd & kLbField : 0;
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                cv += __shfl_xor(cv, o);
-                cf += __shfl_xor(cf, o);
-            }
-            ev_ += cv;
-            ef_ += cf;
-            if (first < 64) break;
-            j -= 64;
-        }
-        const uint64_t iv = ev_ + tv, iff = ef_ + tf;
-        if (lane == 0) {
-            if (iv > kLbField || iff > kLbField) set_err(p, ISO_ERR_TOO_LARGE);
-            __hip_atomic_store(&p.lb[r], kLbIncl | ((iv & kLbField) << 31) | (iff & kLbField), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            s.base_v = (int64_t) ev_;
-            s.base_f = (int64_t) ef_;
-            if (r == p.NR - 1) {
-                p.totals[0] = (int64_t) iv;
-                p.totals[1] = (int64_t) iff;
-            }
-        }
+}
+
+// First level of the wave-count scan: each block of kStrScanChunk waves
+// writes the waves' first ids relative to the chunk and the chunk's sums.
+__global__ void __launch_bounds__(kStrScanChunk) k_stream_scan1(McParams p) {
+    constexpr int NW = kStrScanChunk / 64;
+    __shared__ int sv[NW], sf[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t nw = p.NR * kStrWaves;
+    const int64_t i = (int64_t) blockIdx.x * kStrScanChunk + tid;
+    const int v = i < nw ? (int) p.wcnt_v[i] : 0, f = i < nw ? (int) p.wcnt_f[i] : 0;
+    const int iv = wave_incl_sum(v), iff = wave_incl_sum(f);
+    if (lane == 63) {
+        sv[wave] = iv;
+        sf[wave] = iff;
     }
     __syncthreads();
-    int64_t bv = s.base_v, bf = s.base_f;
-    for (int k = 0; k < wave; ++k) {
-        bv += s.cv[k];
-        bf += s.cf[k];
+    int64_t bv = 0, bf = 0, tv = 0, tf = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+        if (q < wave) {
+            bv += sv[q];
+            bf += sf[q];
+        }
+        tv += sv[q];
+        tf += sf[q];
     }
+    if (i < nw) {
+        p.wb_v[i] = bv + iv - v;
+        p.wb_f[i] = bf + iff - f;
+    }
+    if (tid == 0) {
+        p.csum_v[blockIdx.x] = tv;
+        p.csum_f[blockIdx.x] = tf;
+    }
+}
+
+// Second level (one workgroup; each thread a contiguous run of chunks):
+// the chunks' first ids, the mesh size; meshes beyond int32 indices are
+// flagged.
+__global__ void __launch_bounds__(1024) k_stream_scan2(McParams p) {
+    constexpr int NT = 1024, NW = NT / 64;
+    __shared__ int64_t sv[NW], sf[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t n = (p.NR * kStrWaves + kStrScanChunk - 1) / kStrScanChunk;
+    const int64_t per = (n + NT - 1) / NT;
+    const int64_t lo = min<int64_t>((int64_t) tid * per, n), hi = min<int64_t>(lo + per, n);
+    int64_t v = 0, f = 0;
+    for (int64_t i = lo; i < hi; ++i) {
+        v += p.csum_v[i];
+        f += p.csum_f[i];
+    }
+    int64_t iv = v, iff = f;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t yv = __shfl_up(iv, o), yf = __shfl_up(iff, o);
+        if (lane >= o) {
+            iv += yv;
+            iff += yf;
+        }
+    }
+    if (lane == 63) {
+        sv[wave] = iv;
+        sf[wave] = iff;
+    }
+    __syncthreads();
+    int64_t rv = iv - v, rf = iff - f, tv = 0, tf = 0;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+        if (q < wave) {
+            rv += sv[q];
+            rf += sf[q];
+        }
+        tv += sv[q];
+        tf += sf[q];
+    }
+    for (int64_t i = lo; i < hi; ++i) {
+        p.cofs_v[i] = rv;
+        p.cofs_f[i] = rf;
+        rv += p.csum_v[i];
+        rf += p.csum_f[i];
+    }
+    if (tid == 0) {
+        p.totals[0] = tv;
+        p.totals[1] = tf;
+        if (tv > 0x7fffffffll || tf > 0x7fffffffll) set_err(p, ISO_ERR_TOO_LARGE);
+    }
+}
+
+// Each wave finalises its first vertex / face id, then moves its vertices
+// from the arena to their ids, one per lane (keys, weld-candidate records
+// and the owner rows' candidate flags on the way); a wave whose vertices
+// did not fit the arena reads its rows again and writes them directly.
+template <bool KEYED>
+__global__ void __launch_bounds__(256) k_stream_flush(McParams p) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t r = blockIdx.x, wid = r * kStrWaves + wave;
+    const int64_t ch = wid / kStrScanChunk;
+    const int64_t base = p.wb_v[wid] + p.cofs_v[ch];
     if (lane == 0) {
-        p.wb_v[r * kStrWaves + wave] = bv;
-        p.wb_f[r * kStrWaves + wave] = bf;
+        p.wb_v[wid] = base;
+        p.wb_f[wid] += p.cofs_f[ch];
     }
-    if (vcount <= (uint32_t) kStrCap) {
-        // staged vertices, in order
-        for (uint32_t i = (uint32_t) lane; i < vcount; i += 64) {
-            const StrStage e = s.st[wave][i];
-            const int64_t vid = bv + i;
-            if (vid >= p.cap_v) continue;   // the host sees totals beyond the capacity and retries
-            *reinterpret_cast<float3 *>(p.out_v + 3 * vid) = make_float3(e.x, e.y, e.z);
-            const int axis = (int) (e.info & 3), ln = (int) ((e.info >> 2) & 63), j = (int) ((e.info >> 8) & 7),
-                      stp = (int) ((e.info >> 11) & 31);
-            const int64_t owner = ((x0 + stp) * p.Y + (y0 + j)) * p.Z + z0 + ln;
-            if (KEYED) p.out_key[vid] = owner * 3 + axis;
-            if ((e.info >> 16) & 1) {
-                const int64_t q = seg_push(p, 0, spseg);
-                if (q >= 0) {
-                    SpecialVertex sv;
-                    sv.vid = vid;
-                    sv.edge = owner * 3 + axis;
-                    sv.x = e.x;
-                    sv.y = e.y;
-                    sv.z = e.z;
-                    sv.pad = 0.0f;
-                    p.seg.cand[q] = sv;
-                }
+    const uint32_t cnt = p.wcnt_v[wid];
+    if (cnt == 0 || (*(volatile int *) p.err & ISO_ERR_TOO_LARGE)) return;
+    const StrWave g = str_wave_geom(p, r, wave, lane);
+    const int spseg = (int) (r % p.seg.nseg);
+    const int64_t e0 = wid * (int64_t) (kStrRX * kStrRows);
+    auto emit = [&](int64_t vid, float3 pos, int st, int j, int z, int axis, bool cand) {
+        if (vid >= p.cap_v) return;   // the host sees totals beyond the capacity and retries
+        *reinterpret_cast<float3 *>(p.out_v + 3 * vid) = pos;
+        const int64_t owner = ((g.x0 + st) * p.Y + (g.y0 + j)) * p.Z + g.z0 + z;
+        if (KEYED) p.out_key[vid] = owner * 3 + axis;
+        if (cand) {
+            atomicOr(&p.vofs[e0 + st * kStrRows + j], kStrCandFlag);
+            const int64_t q = seg_push(p, 0, spseg);
+            if (q >= 0) {
+                SpecialVertex sv;
+                sv.vid = vid;
+                sv.edge = owner * 3 + axis;
+                sv.x = pos.x;
+                sv.y = pos.y;
+                sv.z = pos.z;
+                sv.pad = 0.0f;
+                p.seg.cand[q] = sv;
             }
         }
-    } else {
-        // more vertices than the stage holds: read the rows again and write
-        // every vertex at its final id
-        uint32_t vc = 0, fc = 0;
-        bool dummy = false;
-        str_rows<1, KEYED>(p, s, wave, lane, x0, nx, y0, ny, nrow, z0, zin, hz, pv, ev, cz0, cz1, vc, fc, dummy,
-                           emask, evofs, efofs, bv, spseg);
+    };
+    if (!p.wover[wid]) {
+        // slab st's vertices: [S[st], S[st + 1]) of the wave, at arena_off[st]
+        const uint32_t s0l = lane < g.nx ? (p.vofs[e0 + lane * kStrRows] & ~kStrCandFlag) : 0xffffffffu;
+        const int a0l = lane < kStrRX ? p.arena_off[wid * kStrRX + lane] : 0;
+        uint32_t S[kStrRX];
+#pragma unroll
+        for (int q = 0; q < kStrRX; ++q) S[q] = (uint32_t) __builtin_amdgcn_readlane((int) s0l, q);
+        const StrStage *arena = reinterpret_cast<const StrStage *>(p.arena);
+        for (uint32_t b = 0; b < cnt; b += 64) {
+            const uint32_t i = b + (uint32_t) lane;
+            int st = 0;
+#pragma unroll
+            for (int q = 1; q < kStrRX; ++q) st += i >= S[q] ? 1 : 0;
+            const uint32_t s0 = (uint32_t) __shfl((int) s0l, st);
+            const int a0 = __shfl(a0l, st);
+            if (i < cnt) {
+                const StrStage e = arena[(int64_t) a0 + (i - s0)];
+                emit(base + i, make_float3(e.x, e.y, e.z), st, (int) ((e.info >> 8) & 7), (int) ((e.info >> 2) & 63),
+                     (int) (e.info & 3), (e.info >> 16) & 1u);
+            }
+        }
+        return;
+    }
+    // slow path: the wave's rows again, vertices written at their ids
+    for (int st = 0; st < g.nx; ++st) {
+        const int64_t x = g.x0 + st;
+        const bool hasx = x + 1 < p.X;
+        StrSlab A, B;
+        str_load(p, x, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, A);
+        if (hasx) str_load(p, x + 1, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, B);
+        uint64_t ma[kStrRows + 1], mb[kStrRows + 1];
+#pragma unroll
+        for (int j = 0; j <= kStrRows; ++j) {
+            ma[j] = __ballot(g.zin && j < g.nrow && inside(A.v[j], p.level));
+            mb[j] = hasx ? __ballot(g.zin && j < g.nrow && inside(B.v[j], p.level)) : 0ull;
+        }
+        const uint32_t ha = (uint32_t) __ballot(g.hz && lane < g.nrow && inside(A.h, p.level));
+        const float cx0 = grid_coord(x, p.X, p.amin[0], p.amax[0]);
+        const float cx1 = grid_coord(x + 1, p.X, p.amin[0], p.amax[0]);
+        const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int j = 0; j < kStrRows; ++j) {
+            if (j >= g.ny) break;
+            const int64_t y = g.y0 + j;
+            const uint64_t m = ma[j];
+            const uint64_t zm = (m ^ ((m >> 1) | ((uint64_t) ((ha >> j) & 1u) << 63))) & g.ev;
+            const uint64_t ym = (j + 1 < g.nrow) ? (m ^ ma[j + 1]) & g.pv : 0ull;
+            const uint64_t xm = hasx ? (m ^ mb[j]) & g.pv : 0ull;
+            if (!(xm | ym | zm)) continue;
+            const int64_t rb = base + (int64_t) (p.vofs[e0 + st * kStrRows + j] & ~kStrCandFlag);
+            const float cy0 = grid_coord(y, p.Y, p.amin[1], p.amax[1]);
+            const float cy1 = grid_coord(y + 1, p.Y, p.amin[1], p.amax[1]);
+            const float zn = __shfl(A.v[j], (lane + 1) & 63), hv = __shfl(A.h, j);
+            uint32_t off = 0;
+#pragma unroll
+            for (int axis = 0; axis < 3; ++axis) {
+                const uint64_t mm = axis == 0 ? xm : (axis == 1 ? ym : zm);
+                if ((mm >> lane) & 1ull) {
+                    const float v1 = axis == 0 ? B.v[j] : (axis == 1 ? A.v[j + 1] : (lane < 63 ? zn : hv));
+                    bool cand;
+                    const float3 pos = str_vertex(A.v[j], v1, p.level, axis, cx0, cx1, cy0, cy1, g.cz0, g.cz1,
+                                                  p.all_special, cand);
+                    emit(rb + off + (uint32_t) __popcll(mm & below), pos, st, j, lane, axis, cand);
+                }
+                off += (uint32_t) __popcll(mm);
+            }
+        }
     }
 }
 
@@ -2390,10 +2564,22 @@ __global__ void __launch_bounds__(256) k_stream(McParams p) {
 // The tile stages its mask patch (rows + 2, words + 1, from the region-major
 // masks) and its vertex / face bases (wave base + row offset), compacts the
 // active cells and writes their faces; vertex ids are the owner row's base
-// plus the rank of the edge in (z, axis) order.  A cell is special when one
-// of its vertices is a weld candidate (cbits): its faces are tested for
-// bitwise-equal corners (positions read back from the vertex output) and
-// recorded for the fix-up, with its crossing edges.
+// plus the rank of the edge in (axis, z) order.  A cell is special when one
+// of its vertices is a weld candidate (the owner row's vofs flag): its
+// faces are tested for bitwise-equal corners (positions read back from the
+// vertex output) and recorded for the fix-up, with its crossing edges.
+
+// Id of the vertex on the x- (axis 0) or y-edge (axis 1) of tile-local
+// point row (opx, opy) at z = 0 of the patch's second word.
+template <int TX, int TY, int NZC, int NT>
+__device__ __forceinline__ int str_vertex_id_next(const EmitLds<TX, TY, NZC, NT> &s, int hx, int hy, int opx,
+                                                  int opy, int axis) {
+    using L = EmitLds<TX, TY, NZC, NT>;
+    const uint64_t b = PW(s, opx, opy, 1);
+    const uint64_t xm = opx < hx ? (b ^ PW(s, opx + 1, opy, 1)) : 0ull;
+    return s.vb[(opx * L::QY + opy) * L::QC + 1] + (axis == 1 ? __popcll(xm) : 0);
+}
+
 template <int TX, int TY, int NT>
 __device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int spseg) {
     constexpr int NZC = 1;
@@ -2543,40 +2729,36 @@ __device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int 
                 const uint64_t ym01 = hy1 ? b01 ^ b02 : 0ull, ym11 = hy1 ? b11 ^ b12 : 0ull;
                 const uint64_t zm00 = b00 ^ (b00 >> 1), zm01 = b01 ^ (b01 >> 1), zm10 = b10 ^ (b10 >> 1);
                 const uint64_t zm11 = b11 ^ (b11 >> 1);
+                // ids in the row order of k_stream: x-edges by z, then
+                // y-edges, then z-edges
                 const uint64_t low = (1ull << k) - 1;
-                auto rank = [&](uint64_t xm, uint64_t ym, uint64_t zm) {
-                    return __popcll(xm & low) + __popcll(ym & low) + __popcll(zm & low);
-                };
-                auto bit = [&](uint64_t m, int q) { return (int) ((m >> q) & 1); };
+                auto pc = [](uint64_t m) { return __popcll(m); };
                 constexpr int QC = L::QC, QY = L::QY;
                 const int v00 = s.vb[(cx * QY + cy) * QC], v01 = s.vb[(cx * QY + cy + 1) * QC];
                 const int v10 = s.vb[((cx + 1) * QY + cy) * QC];
                 const int v11 = s.vb[((cx + 1) * QY + cy + 1) * QC];
-                const int r00 = v00 + rank(xm00, ym00, zm00), r01 = v01 + rank(xm01, ym01, zm01);
-                const int r10 = v10 + rank(xm10, ym10, zm10), r11 = v11 + rank(xm11, ym11, zm11);
-                const int x00 = bit(xm00, k), y00 = bit(ym00, k), z00 = bit(zm00, k);
-                const int x01 = bit(xm01, k), y01 = bit(ym01, k), z01 = bit(zm01, k);
-                const int x10 = bit(xm10, k), y10 = bit(ym10, k), z10 = bit(zm10, k);
-                id[8] = r00;
-                id[3] = r00 + x00;
-                id[0] = r00 + x00 + y00;
-                id[11] = r01;
-                id[2] = r01 + x01 + y01;
-                id[7] = r10 + x10;
-                id[4] = r10 + x10 + y10;
-                id[6] = r11 + bit(xm11, k) + bit(ym11, k);
+                const int X00 = pc(xm00), X01 = pc(xm01), X10 = pc(xm10), X11 = pc(xm11);
+                const int XY00 = X00 + pc(ym00), XY01 = X01 + pc(ym01), XY10 = X10 + pc(ym10);
+                const int XY11 = X11 + pc(ym11);
+                id[8] = v00 + pc(xm00 & low);
+                id[3] = v00 + X00 + pc(ym00 & low);
+                id[0] = v00 + XY00 + pc(zm00 & low);
+                id[11] = v01 + pc(xm01 & low);
+                id[2] = v01 + XY01 + pc(zm01 & low);
+                id[7] = v10 + X10 + pc(ym10 & low);
+                id[4] = v10 + XY10 + pc(zm10 & low);
+                id[6] = v11 + XY11 + pc(zm11 & low);
                 if (k < 63) {
-                    const int s00 = r00 + x00 + y00 + z00, s01 = r01 + x01 + y01 + z01;
-                    const int s10 = r10 + x10 + y10 + z10;
-                    id[9] = s00;
-                    id[1] = s00 + bit(xm00, k + 1);
-                    id[10] = s01;
-                    id[5] = s10 + bit(xm10, k + 1);
+                    const uint64_t low1 = (low << 1) | 1ull;
+                    id[9] = v00 + pc(xm00 & low1);
+                    id[1] = v00 + X00 + pc(ym00 & low1);
+                    id[10] = v01 + pc(xm01 & low1);
+                    id[5] = v10 + X10 + pc(ym10 & low1);
                 } else {
-                    id[9] = vertex_id(s, hx, hy, cx, cy, zl + 1, 0);
-                    id[1] = vertex_id(s, hx, hy, cx, cy, zl + 1, 1);
-                    id[10] = vertex_id(s, hx, hy, cx, cy + 1, zl + 1, 0);
-                    id[5] = vertex_id(s, hx, hy, cx + 1, cy, zl + 1, 1);
+                    id[9] = str_vertex_id_next(s, hx, hy, cx, cy, 0);
+                    id[1] = str_vertex_id_next(s, hx, hy, cx, cy, 1);
+                    id[10] = str_vertex_id_next(s, hx, hy, cx, cy + 1, 0);
+                    id[5] = str_vertex_id_next(s, hx, hy, cx + 1, cy, 1);
                 }
 #pragma unroll
                 for (int e = 0; e < 12; ++e) s.ids[e][tid] = id[e];
@@ -2668,13 +2850,12 @@ __global__ void __launch_bounds__(NT) k_faces(McParams p) {
 __global__ void k_stream_reset(McParams p, SpecialLists init) {
     const int64_t t = (int64_t) blockIdx.x * blockDim.x + threadIdx.x, nt = (int64_t) gridDim.x * blockDim.x;
     if (t < 4 * p.seg.nseg) p.seg.ctr[t * kSpStride] = 0;
-    for (int64_t i = t; i < p.NR; i += nt) p.lb[i] = 0;
+    if (t < kStrArenaSeg) p.arena_ctr[t * kSpStride] = 0;
     if (p.tile_flag) {
         const int64_t ntiles = p.NI * p.NJ * p.NK;
         for (int64_t i = t; i < ntiles; i += nt) p.tile_flag[i] = 0;
     }
     if (t == 0) {
-        *p.ticket = 0;
         if (p.tile_count) *p.tile_count = 0;
         *p.err = 0;
         p.totals[0] = p.totals[1] = 0;
@@ -2691,18 +2872,27 @@ hipError_t mc_stream_launch(McParams &p, const SpecialLists &sp_init, hipStream_
     auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], stream) : hipSuccess; };
     if ((e = mark(0)) != hipSuccess) return e;
     const int64_t ntiles = p.NI * p.NJ * p.NK;
-    const int64_t rwork = std::max<int64_t>(p.NR, std::max<int64_t>(ntiles, 1024));
+    const int64_t rwork = std::max<int64_t>(ntiles, 1024);
     const unsigned rblocks = (unsigned) std::min<int64_t>((rwork + 255) / 256, 2048);
     hipLaunchKernelGGL(k_stream_reset, dim3(rblocks), dim3(256), 0, stream, p, sp_init);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    // stage times: reset, k_stream, (nothing), k_faces, total
+    // stage times: reset, k_stream, scan + flush, k_faces, total
     if ((e = mark(1)) != hipSuccess) return e;
+    const unsigned sblocks = (unsigned) (8 * ((p.NR + 7) / 8));
     if (p.out_key)
-        hipLaunchKernelGGL(k_stream<true>, dim3((unsigned) p.NR), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_stream<true>, dim3(sblocks), dim3(256), 0, stream, p);
     else
-        hipLaunchKernelGGL(k_stream<false>, dim3((unsigned) p.NR), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_stream<false>, dim3(sblocks), dim3(256), 0, stream, p);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(2)) != hipSuccess) return e;
+    const int64_t nchunk = (p.NR * kStrWaves + kStrScanChunk - 1) / kStrScanChunk;
+    hipLaunchKernelGGL(k_stream_scan1, dim3((unsigned) nchunk), dim3(kStrScanChunk), 0, stream, p);
+    hipLaunchKernelGGL(k_stream_scan2, dim3(1), dim3(1024), 0, stream, p);
+    if (p.out_key)
+        hipLaunchKernelGGL(k_stream_flush<true>, dim3((unsigned) p.NR), dim3(256), 0, stream, p);
+    else
+        hipLaunchKernelGGL(k_stream_flush<false>, dim3((unsigned) p.NR), dim3(256), 0, stream, p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(3)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_faces<16, 16, 256>), dim3((unsigned) ntiles), dim3(256), 0, stream, p);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -118,6 +118,10 @@ def compare_dc(vals, its_np, reg=1e-2, svd_tol=1e-6, user_normals=None):
     tol = 2 * TOL_H * h
     m13 = np.all(np.abs(q13 - ref_corners) <= tol, axis=(1, 2))
     m02 = np.all(np.abs(q02 - ref_corners) <= tol, axis=(1, 2))
+    bad = np.nonzero(~(m13 | m02))[0]
+    for k in bad[:4]:
+        print(f"quad {k}: oracle cells {ref['quads'][k]} corners {ref_corners[k].tolist()}; "
+              f"faces {fg[2 * k].tolist()} {fg[2 * k + 1].tolist()} -> {q13[k].tolist()}", flush=True)
     assert np.all(m13 | m02), f"{int((~(m13 | m02)).sum())} quads differ in cells or orientation"
     d02 = np.linalg.norm(ref_corners[:, 0] - ref_corners[:, 2], axis=1)
     d13 = np.linalg.norm(ref_corners[:, 1] - ref_corners[:, 3], axis=1)

@@ -1,4 +1,4 @@
-"""The read-once marching-cubes pipeline (tile configuration 3: k_stream +
+"""The read-once marching-cubes pipeline (tile configuration 6: k_stream +
 k_faces) against the oracle, bit-exact after canonical ordering.
 
 Shapes cover regions cut by the grid's end on every axis (X not a multiple
@@ -15,7 +15,7 @@ from conftest import multi_sphere_field, sphere_field
 
 pytestmark = pytest.mark.gpu
 
-CFG = 3
+CFG = 6
 
 
 def _run(vals, level=0.0, method="nagae", keyed=False):
@@ -67,11 +67,7 @@ def test_exact_level_corners(gpu):
 
 
 def test_multi_sphere_384(gpu):
-    """A size where the automatic rule itself picks configuration 3."""
-    from isoext_amd import autotune
-    shape = (384, 384, 384)
-    assert autotune.mc_config(shape) == CFG
-    assert _same(multi_sphere_field(shape, 40, seed=2)) > 100000
+    assert _same(multi_sphere_field((384, 384, 384), 40, seed=2)) > 100000
 
 
 def test_keys_name_each_vertex_edge(gpu):
