@@ -46,10 +46,13 @@ namespace iso {
 static __constant__ uint64_t cTris[2][256];
 
 
-// Append to segment `seg` of special list k (0 cand, 1 scell, 2 kept, 3
-// hole); returns the entry index, or -1 when the segment is full (the counter
+// Append to special list k (0 cand, 1 scell, 2 kept, 3 hole) in segment
+// (seg + thread) mod nseg: a workgroup's entries spread over all segments,
+// so a grid with few workgroups needs no more capacity than its entries.
+// Returns the entry index, or -1 when the segment is full (the counter
 // still counts so the host can size a retry).
 __device__ __forceinline__ int64_t seg_push(const McParams &p, int k, int seg) {
+    seg = (int) ((unsigned) (seg + (int) threadIdx.x) % (unsigned) p.seg.nseg);
     const unsigned long long i = atomicAdd(&p.seg.ctr[(k * p.seg.nseg + seg) * kSpStride], 1ull);
     return (int64_t) i < p.seg.cap[k] ? (int64_t) seg * p.seg.cap[k] + (int64_t) i : -1;
 }

@@ -125,12 +125,24 @@ def compare_dc(vals, its_np, reg=1e-2, svd_tol=1e-6, user_normals=None):
     assert np.all(m13 | m02), f"{int((~(m13 | m02)).sum())} quads differ in cells or orientation"
     d02 = np.linalg.norm(ref_corners[:, 0] - ref_corners[:, 2], axis=1)
     d13 = np.linalg.norm(ref_corners[:, 1] - ref_corners[:, 3], axis=1)
-    # recover our split from positions of the shared diagonal
-    vdiag0 = vg[fg[0::2][:, 0]]
-    ours_split02 = np.all(np.abs(vdiag0 - ref_corners[:, 2]) <= tol, axis=1) & ~np.all(
-        np.abs(vdiag0 - ref_corners[:, 1]) <= tol, axis=1)
-    flips = ours_split02 != ref["split02"]
+    # our split from the positions of the first face's first two vertices:
+    # (v2, v0) for the 0-2 diagonal, (v1, v3) for the 1-3 one.  Corners that
+    # coincide within the tolerance can match both readings; such a quad
+    # gives the same surface either way and counts as no flip.
+    t1 = fg[0::2]
+    p0, p1 = vg[t1[:, 0]], vg[t1[:, 1]]
+
+    def at(pos, k):
+        return np.all(np.abs(pos - ref_corners[:, k]) <= tol, axis=1)
+    as02 = at(p0, 2) & at(p1, 0)
+    as13 = at(p0, 1) & at(p1, 3)
+    assert np.all(as02 | as13)
+    flips = np.where(as02 & ~as13, ~ref["split02"], np.where(as13 & ~as02, ref["split02"], False))
     near_tie = np.abs(d02 - d13) <= 8 * TOL_H * h
+    for k in np.nonzero(flips & ~near_tie)[0][:4]:
+        print(f"flip {k}: oracle cells {ref['quads'][k]} split02 {bool(ref['split02'][k])} d02 {d02[k]:.9g} "
+              f"d13 {d13[k]:.9g} corners {ref_corners[k].tolist()}; ours {q13[k].tolist()} "
+              f"faces {fg[2 * k].tolist()} {fg[2 * k + 1].tolist()}", flush=True)
     assert np.all(near_tie[flips]), f"{int((flips & ~near_tie).sum())} diagonal flips away from ties"
     return err, int(flips.sum()), info
 
