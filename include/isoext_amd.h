@@ -109,6 +109,28 @@ int isoext_mc_spheres(const float *spheres, int nsph, const int64_t shape[3], co
                       int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v, int32_t *out_f,
                       int64_t cap_f, int64_t counts[6], void *stream);
 
+/* Allocation-free marching cubes: no reference counterpart.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same result, outputs and
+ * counts as isoext_mc_dense with the default tile configuration
+ * (isoext_mc_auto_config), but every intermediate -- inside masks, counts,
+ * special lists, the weld fix-up's tables -- lives in one fixed device array
+ * per GPU (96 MiB, held until the process ends); no hipMalloc, hipFree or
+ * other allocation happens in the call.  Grids whose workspace exceeds
+ * isoext_mc_dense_noalloc_limit() bytes (a cube of about 620^3 points) return
+ * ISOEXT_E_TOO_LARGE; a special-list need (counts[2]: weld candidates, spread
+ * over 256 segments) above 32,768 returns ISOEXT_E_WORKSPACE.
+ * The caller supplies the outputs (on ISOEXT_E_CAPACITY counts[0..1] hold the
+ * sizes to allocate).  Every call on a GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams or threads) race.  Synchronises
+ * `stream`. */
+int isoext_mc_dense_noalloc(const float *values, const int64_t shape[3], const float aabb_min[3],
+                            const float aabb_max[3], float level, int method, float *out_v, int64_t cap_v,
+                            int32_t *out_f, int64_t cap_f, int64_t counts[6], void *stream);
+
+/* Workspace bytes the allocation-free entry point can give one grid. */
+int64_t isoext_mc_dense_noalloc_limit(void);
+
 /* ---- grid descriptor used by intersections, dual contouring, sparse MC ---- */
 
 /* A UniformGrid (sparse_cell_ids == NULL: values is (X, Y, Z)) or a

@@ -1111,6 +1111,80 @@ __global__ void k_fill(float *p, int64_t n, float v) {
         p[i] = v;
 }
 
+// ---- allocation-free marching cubes ----------------------------------------------
+// isoext_mc_dense_noalloc runs the dense pipeline of isoext_mc_dense in one
+// fixed device array per GPU instead of a caller workspace and per-thread
+// caches: the mask workspace, the special lists and the weld fix-up's tables
+// all live in `scratch`.  Host code never passes `scratch` itself to a
+// kernel (in host code the name is a stand-in, not a device address): the
+// first call on a device looks up that device's copy with
+// hipGetSymbolAddress, clears it with a kernel, and keeps the address.
+constexpr size_t kNoAllocScratchBytes = size_t(96) << 20;   // per GPU, held until the process ends
+constexpr size_t kNoAllocFixupBytes = size_t(16) << 20;     // the weld fix-up's part of it
+constexpr int64_t kNoAllocSpecialCap = 32768;               // special-list capacity (counts[2] reports a call's need)
+constexpr int kNoAllocMaxDevices = 64;
+
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_noalloc_mu;
+static char *g_noalloc_ptr[kNoAllocMaxDevices];
+
+__global__ void k_zero_u64(uint64_t *p, int64_t n) {
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// The current device's scratch address, set up on the first call there.
+static int noalloc_scratch(hipStream_t st, char **out) {
+    int dev = -1;
+    HIP_CHECK(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(ISOEXT_E_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(ISOEXT_E_DEVICE, "scratch has no device address");
+        hipLaunchKernelGGL(k_zero_u64, dim3(1024), dim3(256), 0, st, (uint64_t *) ptr,
+                           (int64_t) (kNoAllocScratchBytes / sizeof(uint64_t)));
+        HIP_CHECK(hipGetLastError(), "clear scratch");
+        HIP_CHECK(hipStreamSynchronize(st), "clear scratch");
+        g_noalloc_ptr[dev] = (char *) ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return ISOEXT_OK;
+}
+
+int64_t isoext_mc_dense_noalloc_limit(void) { return (int64_t) (kNoAllocScratchBytes - kNoAllocFixupBytes); }
+
+int isoext_mc_dense_noalloc(const float *values, const int64_t shape[3], const float aabb_min[3],
+                            const float aabb_max[3], float level, int method, float *out_v, int64_t cap_v,
+                            int32_t *out_f, int64_t cap_f, int64_t counts[6], void *stream) {
+    for (int i = 0; i < 6; ++i) counts[i] = 0;
+    for (int a = 0; a < 3; ++a)
+        if (shape[a] < 1) return fail(ISOEXT_E_ARG, "grid shape must be positive");
+    const int cfg = iso::mc_auto_config(shape);
+    McLayout L;
+    if (shape[0] >= 2 && shape[1] >= 2 && shape[2] >= 2) {
+        if (!mc_layout(shape, cfg, kNoAllocSpecialCap, L)) return fail(ISOEXT_E_ARG, "no tile configuration");
+        const size_t limit = kNoAllocScratchBytes - kNoAllocFixupBytes;
+        if (L.total > limit)
+            return fail(ISOEXT_E_TOO_LARGE,
+                        "grid %lld x %lld x %lld needs %zu bytes of scratch; the allocation-free entry point "
+                        "holds %zu (use isoext_mc_dense)",
+                        (long long) shape[0], (long long) shape[1], (long long) shape[2], L.total, limit);
+    }
+    hipStream_t st = (hipStream_t) stream;
+    char *base = nullptr;
+    const int rc = noalloc_scratch(st, &base);
+    if (rc != ISOEXT_OK) return rc;
+    alignas(256) char host_stage[4096];
+    const iso::FixupScratch fx{base + (kNoAllocScratchBytes - kNoAllocFixupBytes), kNoAllocFixupBytes, host_stage,
+                               sizeof(host_stage)};
+    return mc_dense_impl(values, shape, aabb_min, aabb_max, level, method, cfg, base,
+                         (int64_t) (kNoAllocScratchBytes - kNoAllocFixupBytes), kNoAllocSpecialCap, out_v, cap_v,
+                         out_f, cap_f, nullptr, counts, stream, nullptr, 0, &fx);
+}
+
 int isoext_fill_f32(float *ptr, int64_t n, float value, void *stream) {
     if (n <= 0) return ISOEXT_OK;
     unsigned grid = (unsigned) std::min<int64_t>(65536, (n + 255) / 256);
