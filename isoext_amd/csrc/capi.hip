@@ -77,7 +77,7 @@ struct McLayout {
     // look-back words, region-major offsets, wave bases
     bool stream;
     int64_t NRX, NRY, NR;
-    size_t o_vofs, o_fofs, o_wcv, o_wcf, o_wbv, o_wbf, o_csum, o_cofs, o_actr, o_aoff, o_wover;
+    size_t o_vofs, o_fofs, o_wcv, o_wcf, o_wbv, o_wbf, o_csum, o_cofs, o_actr, o_aoff, o_wover, o_alist, o_acnt;
 };
 
 constexpr size_t kHeader = 4096;   // err, totals, SpecialLists
@@ -131,7 +131,7 @@ bool mc_layout(const int64_t shape[3], int cfg, int64_t cap, McLayout &L) {
     L.o_tlist = take(sizeof(int32_t) * L.NI * L.NJ * L.NK);
     L.o_tcnt = take(sizeof(int32_t));
     L.o_vofs = L.o_fofs = L.o_wcv = L.o_wcf = L.o_wbv = L.o_wbf = L.o_csum = L.o_cofs = 0;
-    L.o_actr = L.o_aoff = L.o_wover = 0;
+    L.o_actr = L.o_aoff = L.o_wover = L.o_alist = L.o_acnt = 0;
     if (L.stream) {
         const int64_t nw = L.NR * iso::kStrWaves;
         const int64_t nchunk = (nw + iso::kStrScanChunk - 1) / iso::kStrScanChunk;
@@ -146,6 +146,8 @@ bool mc_layout(const int64_t shape[3], int cfg, int64_t cap, McLayout &L) {
         L.o_actr = take(sizeof(unsigned long long) * iso::kStrArenaSeg * iso::kSpStride);
         L.o_aoff = take(sizeof(int32_t) * nw * iso::kStrRX);
         L.o_wover = take(sizeof(uint8_t) * nw);
+        L.o_alist = take(sizeof(int64_t) * nw);
+        L.o_acnt = take(sizeof(int));
     }
     L.total = o;
     return true;
@@ -967,7 +969,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     }
     hipEvent_t *ev = (g_timing && !na) ? timing_events() : nullptr;
     if (g_timing && !na && !ev) return fail(ISOEXT_E_DEVICE, "timing events unavailable");
-    const bool read_once = L.stream && !spheres && L.NR <= (int64_t) INT32_MAX;
+    const bool read_once = L.stream && !spheres && L.NR <= (int64_t) INT32_MAX && shape[2] < (int64_t(1) << 26);
     if (read_once) {
         p.NRX = L.NRX;
         p.NRY = L.NRY;
@@ -986,6 +988,8 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
         p.arena_ctr = (unsigned long long *) (ws + L.o_actr);
         p.arena_off = (int32_t *) (ws + L.o_aoff);
         p.wover = (uint8_t *) (ws + L.o_wover);
+        p.alist = (int64_t *) (ws + L.o_alist);
+        p.acount = (int *) (ws + L.o_acnt);
         // the vertex arena: the face buffer, written by the faces kernel only
         // after the flush has emptied it (16-byte entries, 16-byte aligned)
         p.arena = nullptr;

@@ -128,6 +128,8 @@ struct McParams {
     uint32_t *wcnt_f;    // (NR, 4): faces of each wave
     int64_t *wb_v;       // (NR, 4): first vertex id of each wave (chunk-relative until the flush)
     int64_t *wb_f;       // (NR, 4): first face id of each wave (chunk-relative until the flush)
+    int64_t *alist;      // waves with vertices (k_stream_scan1), *acount of them
+    int *acount;
     int64_t *csum_v, *csum_f;   // per chunk of kStrScanChunk waves: vertices / faces
     int64_t *cofs_v, *cofs_f;   // per chunk: first vertex / face id
     // vertex arena: k_stream writes each slab's vertices where a segment

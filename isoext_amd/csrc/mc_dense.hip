@@ -2026,24 +2026,6 @@ struct StrCoords {
     float cz[65];
 };
 
-__device__ __forceinline__ void str_load(const McParams &p, int64_t x, int64_t y0, int nrow, int64_t z0, int lane,
-                                         bool zin, bool hz, StrSlab &o) {
-    const float *base = p.values + (x * p.Y + y0) * p.Z + z0;
-#pragma unroll
-    for (int j = 0; j <= kStrRows; ++j) {
-        const float *row = base + (int64_t) j * p.Z;
-        if (j < nrow) {
-            // rows 1..7 are read by this wave only: nontemporal; row 0 and the
-            // row above are also another wave's rows, kept in L2 for it
-            if (j == 0 || j == kStrRows) o.v[j] = zin ? row[lane] : 0.0f;
-            else o.v[j] = zin ? __builtin_nontemporal_load(row + lane) : 0.0f;
-        } else {
-            o.v[j] = 0.0f;
-        }
-    }
-    o.h = (hz && lane < nrow) ? base[(int64_t) lane * p.Z + 64] : 0.0f;
-}
-
 // Region geometry of the calling wave.
 struct StrWave {
     int64_t r, w, rx, ry, x0, y0, z0;
@@ -2051,6 +2033,11 @@ struct StrWave {
     bool zin, hz;
     uint64_t pv, ev;
     float cz0, cz1;
+    const float *wbase;  // point (0, y0, z0)
+    int64_t slab;        // points between x-slabs (Y Z)
+    uint32_t rowbytes;   // bytes between point rows (4 Z)
+    uint32_t loff;       // this lane's byte offset in a row (clamped to the grid)
+    uint32_t hoff;       // this lane's halo value: row min(lane, nrow - 1), point 64
 };
 __device__ __forceinline__ StrWave str_wave_geom(const McParams &p, int64_t r, int wave, int lane) {
     StrWave g;
@@ -2071,7 +2058,42 @@ __device__ __forceinline__ StrWave str_wave_geom(const McParams &p, int64_t r, i
     g.ev = low_bits(p.Z - 1 - g.z0);
     g.cz0 = grid_coord(g.z0 + lane, p.Z, p.amin[2], p.amax[2]);
     g.cz1 = grid_coord(g.z0 + lane + 1, p.Z, p.amin[2], p.amax[2]);
+    g.wbase = p.values ? p.values + g.y0 * p.Z + g.z0 : nullptr;
+    g.slab = p.Y * p.Z;
+    g.rowbytes = (uint32_t) (4 * p.Z);
+    g.loff = 4u * (uint32_t) min<int64_t>(lane, p.Z - 1 - g.z0);
+    g.hoff = (uint32_t) max(0, min(lane, g.nrow - 1)) * g.rowbytes + (g.hz ? 256u : g.loff);
     return g;
+}
+
+// Rows 0..8 of slab x at this lane's z, and the z-halo values.  Branch
+// free: rows past the grid re-read its last row, lanes past its end the
+// row's last point, and without a next word the halo lanes their own row
+// (all masked out of every ballot); each address is a uniform row pointer
+// plus the lane's 32-bit offset.
+// FULL: the wave has all 9 rows, a whole word and a next word (no clamps;
+// raw buffer loads from the slab's row 0, per-lane 32-bit offsets).
+template <bool FULL>
+__device__ __forceinline__ void str_load(const McParams &p, int64_t x, const StrWave &g, StrSlab &o) {
+    const float *basep = g.wbase + x * g.slab;
+    if constexpr (FULL) {
+        const __amdgpu_buffer_rsrc_t r =
+            __builtin_amdgcn_make_buffer_rsrc((void *) basep, (short) 0, 0x7fffffff, 0x00020000);
+#pragma unroll
+        for (int j = 0; j <= kStrRows; ++j)
+            o.v[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                   r, (int) (g.loff + (uint32_t) j * g.rowbytes), 0, 0));
+        o.h = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int) g.hoff, 0, 0));
+    } else {
+        const char *base = reinterpret_cast<const char *>(basep);
+        const char *row = base;
+#pragma unroll
+        for (int j = 0; j <= kStrRows; ++j) {
+            o.v[j] = *reinterpret_cast<const float *>(row + g.loff);
+            if (j + 1 < g.nrow) row += g.rowbytes;
+        }
+        o.h = *reinterpret_cast<const float *>(base + g.hoff);
+    }
 }
 
 // Vertex on a crossing edge from the value v0 at its owner point to v1;
@@ -2102,6 +2124,7 @@ __device__ __forceinline__ uint32_t str_row16_incl(uint32_t v) {
 // Commit slab s: inside masks to mt[s] (lane j: row j), halo bits to
 // ht[s], values to ring slot s & 1; uin / uout: every point of the slab
 // (its 9 rows and their halo values) is inside / outside.
+template <bool FULL>
 __device__ __forceinline__ void str_commit(const StrSlab &sl, const StrWave &g, float level, int lane, StrLds &w,
                                            int s, bool &uin, bool &uout) {
     const int slot = s & 1;
@@ -2109,21 +2132,27 @@ __device__ __forceinline__ void str_commit(const StrSlab &sl, const StrWave &g, 
     uint64_t all = ~0ull, any = 0ull;
 #pragma unroll
     for (int j = 0; j <= kStrRows; ++j) {
-        m[j] = __ballot(g.zin && j < g.nrow && inside(sl.v[j], level));
+        if (FULL) {
+            m[j] = __ballot(inside(sl.v[j], level));
+        } else {
+            const uint64_t b = __ballot(inside(sl.v[j], level)) & g.pv;
+            m[j] = j < g.nrow ? b : 0ull;
+        }
         w.v[slot][j][lane] = sl.v[j];
-        if (j < g.nrow) {
+        if (FULL || j < g.nrow) {
             all &= m[j];
             any |= m[j];
         }
     }
-    const uint32_t hb = (uint32_t) __ballot(g.hz && lane < g.nrow && inside(sl.h, level));
+    const uint32_t rows = FULL ? 0x1ffu : (uint32_t) low_bits(g.nrow);
+    const uint32_t hb = (FULL || g.hz) ? (uint32_t) __ballot(inside(sl.h, level)) & rows : 0u;
     if (lane <= kStrRows) w.h[slot][lane] = sl.h;
-    const uint32_t hall = g.hz ? (uint32_t) low_bits(g.nrow) : 0u;
-    uin = (all & g.pv) == g.pv && hb == hall;
+    const uint32_t hall = (FULL || g.hz) ? rows : 0u;
+    uin = (FULL ? all == ~0ull : (all & g.pv) == g.pv) && hb == hall;
     uout = any == 0ull && hb == 0u;
     uint64_t mine;
     if (uin || uout) {
-        mine = (uin && lane < g.nrow) ? g.pv : 0ull;
+        mine = (uin && (FULL || lane < g.nrow)) ? g.pv : 0ull;
     } else {
         mine = 0ull;
 #pragma unroll
@@ -2194,13 +2223,21 @@ __device__ __forceinline__ void str_surface(const McParams &p, const StrWave &g,
     __builtin_amdgcn_wave_barrier();
     // vertices, one per lane, into the arena
     if (nslab && !over) {
-        unsigned long long off = 0;
-        if (lane == 0) off = atomicAdd(&p.arena_ctr[seg * kSpStride], (unsigned long long) nslab);
-        off = __shfl(off, 0);
-        if ((int64_t) off + nslab > p.arena_seg_cap) {
+        // arena segment: a hash of (wave, slab), so neighbouring surface
+        // waves spread over all segments; a full one sends the reservation on
+        // to up to three others before the wave takes the flush's slow path
+        const uint32_t hsh = ((uint32_t) seg * (uint32_t) kStrRX + (uint32_t) st) * 0x9E3779B1u;
+        int64_t a0 = -1;
+        for (int t = 0; t < 4 && a0 < 0; ++t) {
+            const int sg = (int) ((hsh >> 24) + (uint32_t) t * 67u) & (kStrArenaSeg - 1);
+            unsigned long long off = 0;
+            if (lane == 0) off = atomicAdd(&p.arena_ctr[sg * kSpStride], (unsigned long long) nslab);
+            off = __shfl(off, 0);
+            if ((int64_t) off + nslab <= p.arena_seg_cap) a0 = (int64_t) sg * p.arena_seg_cap + (int64_t) off;
+        }
+        if (a0 < 0) {
             over = true;
         } else {
-            const int64_t a0 = (int64_t) seg * p.arena_seg_cap + (int64_t) off;
             if (lane == st) my_off = (int32_t) a0;
             StrStage *arena = reinterpret_cast<StrStage *>(p.arena) + a0;
             const float cx0 = sc.cx[st], cx1 = sc.cx[st + 1];
@@ -2280,12 +2317,55 @@ __device__ __forceinline__ void str_surface(const McParams &p, const StrWave &g,
     vcount += nslab;
 }
 
+// The wave's slab sweep (FULL: all 9 rows, a whole word and a next word).
+template <bool FULL, bool KEYED>
+__device__ __forceinline__ void str_sweep(const McParams &p, const StrWave &g, StrLds &w, const StrCoords &sc,
+                                          const uint8_t *ntri, int lane, int seg, uint32_t &vcount, uint32_t &fcount,
+                                          bool &over, int32_t &my_off, bool &any_faces) {
+    const float level = p.level;
+    // slab q is in buffer q % 3: step st loads slab st + 3 into the
+    // buffer of slab st (committed a step earlier), commits slab st + 1
+    // every step loads and commits: slabs past the region's halo slab or
+    // the grid re-read the last one in reach (an L2 hit; unused)
+    const int64_t xlast = min<int64_t>(g.x0 + g.nx, p.X - 1);
+    StrSlab B0, B1, B2;
+    bool uin0, uout0, uin1 = false, uout1 = false;
+    str_load<FULL>(p, g.x0, g, B0);
+    str_load<FULL>(p, min<int64_t>(g.x0 + 1, xlast), g, B1);
+    str_load<FULL>(p, min<int64_t>(g.x0 + 2, xlast), g, B2);
+    str_commit<FULL>(B0, g, level, lane, w, 0, uin0, uout0);
+    auto step = [&](int st, StrSlab &ld, const StrSlab &cm) {
+        const int64_t x = g.x0 + st;
+        const bool hasx = x + 1 < p.X;
+        str_load<FULL>(p, min<int64_t>(x + 3, xlast), g, ld);
+        str_commit<FULL>(cm, g, level, lane, w, st + 1, uin1, uout1);
+        __builtin_amdgcn_wave_barrier();
+        const bool trivial = hasx ? ((uin0 && uin1) || (uout0 && uout1)) : (uin0 || uout0);
+        uint32_t vrow = vcount, frow = fcount;
+        if (!trivial)
+            str_surface<KEYED>(p, g, w, sc, ntri, lane, st, hasx, seg, vrow, frow, vcount, fcount, over, my_off,
+                               any_faces);
+        if (lane < kStrRows) {
+            w.vt[st * kStrRows + lane] = vrow;
+            w.ft[st * kStrRows + lane] = frow;
+        }
+        uin0 = uin1;
+        uout0 = uout1;
+        __builtin_amdgcn_wave_barrier();
+    };
+    for (int st = 0; st < g.nx; st += 3) {
+        step(st, B0, B1);
+        if (st + 1 < g.nx) step(st + 1, B1, B2);
+        if (st + 2 < g.nx) step(st + 2, B2, B0);
+    }
+}
+
 template <bool KEYED>
 __global__ void __launch_bounds__(256) k_stream(McParams p) {
     __shared__ StrLds sw[kStrWaves];
     __shared__ StrCoords sc;
     __shared__ uint8_t ntri[256];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform per wave
     const int64_t r = str_region(blockIdx.x, p.NR);
     if (r >= p.NR) return;
     const StrWave g = str_wave_geom(p, r, wave, lane);
@@ -2305,43 +2385,15 @@ __global__ void __launch_bounds__(256) k_stream(McParams p) {
     __syncthreads();
     const int64_t wid = r * kStrWaves + wave;
     StrLds &w = sw[wave];
-    const int seg = (int) (wid % kStrArenaSeg);
+    const int seg = (int) (uint32_t) wid;   // arena segment hash input
     uint32_t vcount = 0, fcount = 0;
     bool any_faces = false, over = false;
     int32_t my_off = 0;   // lane s: arena entry of slab s's first vertex
     if (g.ny > 0) {
-        const float level = p.level;
-        StrSlab S1, S2, S3;
-        bool uin0, uout0, uin1 = false, uout1 = false;
-        {
-            StrSlab S0;
-            str_load(p, g.x0, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, S0);
-            if (g.x0 + 1 < p.X) str_load(p, g.x0 + 1, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, S1);
-            if (1 < g.nx && g.x0 + 2 < p.X) str_load(p, g.x0 + 2, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, S2);
-            str_commit(S0, g, level, lane, w, 0, uin0, uout0);
-        }
-        for (int st = 0; st < g.nx; ++st) {
-            const int64_t x = g.x0 + st;
-            const bool hasx = x + 1 < p.X;
-            // slab st + 3 in flight (at most the next region's first slab)
-            if (st + 2 < g.nx && x + 3 < p.X) str_load(p, x + 3, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, S3);
-            if (hasx) str_commit(S1, g, level, lane, w, st + 1, uin1, uout1);
-            __builtin_amdgcn_wave_barrier();
-            const bool trivial = hasx ? ((uin0 && uin1) || (uout0 && uout1)) : (uin0 || uout0);
-            uint32_t vrow = vcount, frow = fcount;
-            if (!trivial)
-                str_surface<KEYED>(p, g, w, sc, ntri, lane, st, hasx, seg, vrow, frow, vcount, fcount, over, my_off,
-                                   any_faces);
-            if (lane < kStrRows) {
-                w.vt[st * kStrRows + lane] = vrow;
-                w.ft[st * kStrRows + lane] = frow;
-            }
-            S1 = S2;
-            S2 = S3;
-            uin0 = uin1;
-            uout0 = uout1;
-            __builtin_amdgcn_wave_barrier();
-        }
+        if (g.nrow == kStrRows + 1 && g.hz)
+            str_sweep<true, KEYED>(p, g, w, sc, ntri, lane, seg, vcount, fcount, over, my_off, any_faces);
+        else
+            str_sweep<false, KEYED>(p, g, w, sc, ntri, lane, seg, vcount, fcount, over, my_off, any_faces);
     }
     // region-major entries of this wave: whole lines
     const int64_t e0 = wid * (int64_t) (kStrRX * kStrRows);
@@ -2399,6 +2451,22 @@ __global__ void __launch_bounds__(kStrScanChunk) k_stream_scan1(McParams p) {
     if (tid == 0) {
         p.csum_v[blockIdx.x] = tv;
         p.csum_f[blockIdx.x] = tf;
+    }
+    // the waves with vertices, for the flush
+    const uint64_t act = __ballot(v > 0);
+    int at = 0;
+    if (lane == 0 && act) at = atomicAdd(p.acount, __popcll(act));
+    at = __shfl(at, 0);
+    if (v > 0) p.alist[at + __popcll(act & ((1ull << lane) - 1ull))] = i;
+}
+
+// Chunk bases into the waves' first vertex / face ids.
+__global__ void __launch_bounds__(kStrScanChunk) k_stream_fin(McParams p) {
+    const int64_t nw = p.NR * kStrWaves;
+    const int64_t i = (int64_t) blockIdx.x * kStrScanChunk + threadIdx.x;
+    if (i < nw) {
+        p.wb_v[i] += p.cofs_v[blockIdx.x];
+        p.wb_f[i] += p.cofs_f[blockIdx.x];
     }
 }
 
@@ -2458,18 +2526,13 @@ __global__ void __launch_bounds__(1024) k_stream_scan2(McParams p) {
 // from the arena to their ids, one per lane (keys, weld-candidate records
 // and the owner rows' candidate flags on the way); a wave whose vertices
 // did not fit the arena reads its rows again and writes them directly.
+// One wave's flush (wid: a wave with vertices).
 template <bool KEYED>
-__global__ void __launch_bounds__(256) k_stream_flush(McParams p) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t r = blockIdx.x, wid = r * kStrWaves + wave;
-    const int64_t ch = wid / kStrScanChunk;
-    const int64_t base = p.wb_v[wid] + p.cofs_v[ch];
-    if (lane == 0) {
-        p.wb_v[wid] = base;
-        p.wb_f[wid] += p.cofs_f[ch];
-    }
+__device__ __forceinline__ void str_flush_wave(const McParams &p, int64_t wid, int lane) {
+    const int64_t r = wid / kStrWaves;
+    const int wave = (int) (wid - r * kStrWaves);
+    const int64_t base = p.wb_v[wid];
     const uint32_t cnt = p.wcnt_v[wid];
-    if (cnt == 0 || (*(volatile int *) p.err & ISO_ERR_TOO_LARGE)) return;
     const StrWave g = str_wave_geom(p, r, wave, lane);
     const int spseg = (int) (r % p.seg.nseg);
     const int64_t e0 = wid * (int64_t) (kStrRX * kStrRows);
@@ -2501,17 +2564,28 @@ __global__ void __launch_bounds__(256) k_stream_flush(McParams p) {
 #pragma unroll
         for (int q = 0; q < kStrRX; ++q) S[q] = (uint32_t) __builtin_amdgcn_readlane((int) s0l, q);
         const StrStage *arena = reinterpret_cast<const StrStage *>(p.arena);
-        for (uint32_t b = 0; b < cnt; b += 64) {
-            const uint32_t i = b + (uint32_t) lane;
-            int st = 0;
+        // four batches of 64 vertices: all loads first, then the stores
+        constexpr int NB = 4;
+        for (uint32_t b = 0; b < cnt; b += 64 * NB) {
+            StrStage e[NB];
+            int sts[NB];
 #pragma unroll
-            for (int q = 1; q < kStrRX; ++q) st += i >= S[q] ? 1 : 0;
-            const uint32_t s0 = (uint32_t) __shfl((int) s0l, st);
-            const int a0 = __shfl(a0l, st);
-            if (i < cnt) {
-                const StrStage e = arena[(int64_t) a0 + (i - s0)];
-                emit(base + i, make_float3(e.x, e.y, e.z), st, (int) ((e.info >> 8) & 7), (int) ((e.info >> 2) & 63),
-                     (int) (e.info & 3), (e.info >> 16) & 1u);
+            for (int k = 0; k < NB; ++k) {
+                const uint32_t i = b + 64 * k + (uint32_t) lane;
+                int st = 0;
+#pragma unroll
+                for (int q = 1; q < kStrRX; ++q) st += i >= S[q] ? 1 : 0;
+                const uint32_t s0 = (uint32_t) __shfl((int) s0l, st);
+                const int a0 = __shfl(a0l, st);
+                sts[k] = st;
+                if (i < cnt) e[k] = arena[(int64_t) a0 + (i - s0)];
+            }
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const uint32_t i = b + 64 * k + (uint32_t) lane;
+                if (i < cnt)
+                    emit(base + i, make_float3(e[k].x, e[k].y, e[k].z), sts[k], (int) ((e[k].info >> 8) & 7),
+                         (int) ((e[k].info >> 2) & 63), (int) (e[k].info & 3), (e[k].info >> 16) & 1u);
             }
         }
         return;
@@ -2521,15 +2595,15 @@ __global__ void __launch_bounds__(256) k_stream_flush(McParams p) {
         const int64_t x = g.x0 + st;
         const bool hasx = x + 1 < p.X;
         StrSlab A, B;
-        str_load(p, x, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, A);
-        if (hasx) str_load(p, x + 1, g.y0, g.nrow, g.z0, lane, g.zin, g.hz, B);
+        str_load<false>(p, x, g, A);
+        if (hasx) str_load<false>(p, x + 1, g, B);
         uint64_t ma[kStrRows + 1], mb[kStrRows + 1];
 #pragma unroll
         for (int j = 0; j <= kStrRows; ++j) {
-            ma[j] = __ballot(g.zin && j < g.nrow && inside(A.v[j], p.level));
-            mb[j] = hasx ? __ballot(g.zin && j < g.nrow && inside(B.v[j], p.level)) : 0ull;
+            ma[j] = j < g.nrow ? __ballot(inside(A.v[j], p.level)) & g.pv : 0ull;
+            mb[j] = (hasx && j < g.nrow) ? __ballot(inside(B.v[j], p.level)) & g.pv : 0ull;
         }
-        const uint32_t ha = (uint32_t) __ballot(g.hz && lane < g.nrow && inside(A.h, p.level));
+        const uint32_t ha = g.hz ? (uint32_t) __ballot(inside(A.h, p.level)) & (uint32_t) low_bits(g.nrow) : 0u;
         const float cx0 = grid_coord(x, p.X, p.amin[0], p.amax[0]);
         const float cx1 = grid_coord(x + 1, p.X, p.amin[0], p.amax[0]);
         const uint64_t below = (1ull << lane) - 1ull;
@@ -2560,6 +2634,23 @@ __global__ void __launch_bounds__(256) k_stream_flush(McParams p) {
                 off += (uint32_t) __popcll(mm);
             }
         }
+    }
+}
+
+// Moves each listed wave's vertices from the arena to their ids (keys,
+// weld-candidate records and the owner rows' candidate flags on the way); a
+// wave whose vertices did not fit the arena reads its rows again and writes
+// them directly.  Persistent: the waves of the grid step through the list.
+template <bool KEYED>
+__global__ void __launch_bounds__(256) k_stream_flush(McParams p) {
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (*(volatile int *) p.err & ISO_ERR_TOO_LARGE) return;
+    const int64_t n = *p.acount;
+    for (int64_t k = (int64_t) blockIdx.x * kStrWaves + wave; k < n; k += (int64_t) gridDim.x * kStrWaves) {
+        const int64_t wid = p.alist[k];
+        const int lo = __builtin_amdgcn_readfirstlane((int) (uint32_t) wid);
+        const int hi = __builtin_amdgcn_readfirstlane((int) (uint32_t) (wid >> 32));
+        str_flush_wave<KEYED>(p, (int64_t) (((uint64_t) (uint32_t) hi << 32) | (uint32_t) lo), lane);
     }
 }
 
@@ -2860,6 +2951,7 @@ __global__ void k_stream_reset(McParams p, SpecialLists init) {
     }
     if (t == 0) {
         if (p.tile_count) *p.tile_count = 0;
+        *p.acount = 0;
         *p.err = 0;
         p.totals[0] = p.totals[1] = 0;
         init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
@@ -2891,10 +2983,12 @@ hipError_t mc_stream_launch(McParams &p, const SpecialLists &sp_init, hipStream_
     const int64_t nchunk = (p.NR * kStrWaves + kStrScanChunk - 1) / kStrScanChunk;
     hipLaunchKernelGGL(k_stream_scan1, dim3((unsigned) nchunk), dim3(kStrScanChunk), 0, stream, p);
     hipLaunchKernelGGL(k_stream_scan2, dim3(1), dim3(1024), 0, stream, p);
+    hipLaunchKernelGGL(k_stream_fin, dim3((unsigned) nchunk), dim3(kStrScanChunk), 0, stream, p);
+    const unsigned fblocks = (unsigned) std::min<int64_t>(2048, std::max<int64_t>(1, (p.NR + 7) / 8));
     if (p.out_key)
-        hipLaunchKernelGGL(k_stream_flush<true>, dim3((unsigned) p.NR), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_stream_flush<true>, dim3(fblocks), dim3(256), 0, stream, p);
     else
-        hipLaunchKernelGGL(k_stream_flush<false>, dim3((unsigned) p.NR), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_stream_flush<false>, dim3(fblocks), dim3(256), 0, stream, p);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(3)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_faces<16, 16, 256>), dim3((unsigned) ntiles), dim3(256), 0, stream, p);
