@@ -78,6 +78,8 @@ struct McLayout {
     bool stream;
     int64_t NRX, NRY, NR;
     size_t o_vofs, o_fofs, o_wcv, o_wcf, o_wbv, o_wbf, o_csum, o_cofs, o_actr, o_aoff, o_wover, o_alist, o_acnt;
+    size_t o_dslots;
+    int64_t dslots;   // candidate position table (a power of two, >= 2 x the candidate capacity)
 };
 
 constexpr size_t kHeader = 4096;   // err, totals, SpecialLists
@@ -130,6 +132,9 @@ bool mc_layout(const int64_t shape[3], int cfg, int64_t cap, McLayout &L) {
     L.o_tflag = take(sizeof(int32_t) * L.NI * L.NJ * L.NK);
     L.o_tlist = take(sizeof(int32_t) * L.NI * L.NJ * L.NK);
     L.o_tcnt = take(sizeof(int32_t));
+    L.dslots = 1;
+    while (L.dslots < 2 * iso::kSpSeg * L.seg_cap[0] + 16) L.dslots <<= 1;
+    L.o_dslots = take(sizeof(int32_t) * L.dslots);
     L.o_vofs = L.o_fofs = L.o_wcv = L.o_wcf = L.o_wbv = L.o_wbf = L.o_csum = L.o_cofs = 0;
     L.o_actr = L.o_aoff = L.o_wover = L.o_alist = L.o_acnt = 0;
     if (L.stream) {
@@ -903,6 +908,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     char *ws = (char *) workspace;
 
     iso::SpecialLists sp_host{};
+    sp_host.dup = -1;
     sp_host.cap_cand = iso::kSpSeg * L.seg_cap[0];
     sp_host.cap_scell = iso::kSpSeg * L.seg_cap[1];
     sp_host.cap_kept = iso::kSpSeg * L.seg_cap[2];
@@ -959,6 +965,8 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     p.tile_flag = (int *) (ws + L.o_tflag);
     p.tile_list = (int *) (ws + L.o_tlist);
     p.tile_count = (int *) (ws + L.o_tcnt);
+    p.dslots = (int32_t *) (ws + L.o_dslots);
+    p.dmask = L.dslots - 1;
     p.tx = L.tx;
     p.ty = L.ty;
     if (spheres) {
@@ -1048,7 +1056,10 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
                     (long long) nv, (long long) nf);
     if (need > special_cap)
         return fail(ISOEXT_E_WORKSPACE, "special-vertex lists need capacity %lld", (long long) need);
-    if (sp_dev.n_cand > 0 || sp_dev.n_hole > 0) {
+    // no dropped face and no two candidates at one position: every vertex
+    // stays referenced and none has a twin, so there is nothing to weld
+    const bool weld_noop = sp_dev.dup == 0 && sp_dev.n_hole == 0;
+    if ((sp_dev.n_cand > 0 || sp_dev.n_hole > 0) && !weld_noop) {
         iso::FixupGrid g{shape[0], shape[1], shape[2], p.blk_v, p.blk_f, p.cnt_v, p.cnt_f, p.S};
         if (read_once) g = iso::FixupGrid{shape[0], shape[1], shape[2]};   // region-major numbering: no skip tables
         // the fixed scratch of the allocation-free variant must hold the

@@ -36,6 +36,7 @@ struct SpecialLists {
     int64_t *kept;         // face slots kept in special cells
     int64_t *hole;         // face slots dropped as degenerate
     int64_t need;          // special capacity a call needs (segment overflow included)
+    int64_t dup;           // two candidates share a position: 1, none: 0, not checked: -1
 };
 
 // The emit kernel appends special entries through nseg (<= kSpSeg)
@@ -44,6 +45,7 @@ struct SpecialLists {
 // do not serialise on one address; a gather pass concatenates the segments
 // into the contiguous SpecialLists.
 constexpr int kSpSeg = 256;
+static_assert((kSpSeg & (kSpSeg - 1)) == 0, "segment counts are powers of two");
 constexpr int kSpStride = 32;
 struct SpecialSegs {
     unsigned long long *ctr;   // [4][nseg * kSpStride]: cand, scell, kept, hole
@@ -128,6 +130,10 @@ struct McParams {
     uint32_t *wcnt_f;    // (NR, 4): faces of each wave
     int64_t *wb_v;       // (NR, 4): first vertex id of each wave (chunk-relative until the flush)
     int64_t *wb_f;       // (NR, 4): first face id of each wave (chunk-relative until the flush)
+    // optional (single grids): open-addressing table of candidate indices by
+    // position, dmask + 1 slots, cleared by the reset kernel
+    int32_t *dslots;
+    int64_t dmask;
     int64_t *alist;      // waves with vertices (k_stream_scan1), *acount of them
     int *acount;
     int64_t *csum_v, *csum_f;   // per chunk of kStrScanChunk waves: vertices / faces
@@ -202,6 +208,7 @@ constexpr int kMaxBatch = 256;        // items per batch
 // configuration measured fastest over a mix of 32^3 to 512^3 grids.
 constexpr int kBatchTileConfig = 2;
 constexpr int kMaxSpSegBatch = 16;    // special-list segments per batch item
+static_assert((kMaxSpSegBatch & (kMaxSpSegBatch - 1)) == 0, "segment counts are powers of two");
 
 int mc_config_count();
 int64_t mc_config_tiles(int cfg, const int64_t shape[3]);

@@ -52,7 +52,7 @@ static __constant__ uint64_t cTris[2][256];
 // Returns the entry index, or -1 when the segment is full (the counter
 // still counts so the host can size a retry).
 __device__ __forceinline__ int64_t seg_push(const McParams &p, int k, int seg) {
-    seg = (int) ((unsigned) (seg + (int) threadIdx.x) % (unsigned) p.seg.nseg);
+    seg = (seg + (int) threadIdx.x) & (p.seg.nseg - 1);   // nseg: a power of two
     const unsigned long long i = atomicAdd(&p.seg.ctr[(k * p.seg.nseg + seg) * kSpStride], 1ull);
     return (int64_t) i < p.seg.cap[k] ? (int64_t) seg * p.seg.cap[k] + (int64_t) i : -1;
 }
@@ -1194,6 +1194,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             const uint64_t wl = s.tris[cs];
             const int ntri = (int) (wl & 15);
             int64_t slot = (int64_t) s.fb[r] + fo;
+            bool dropped = false;   // a face of this cell is degenerate
             for (int q = 0; q < ntri; ++q, ++slot) {
                 const int e0 = (int) ((wl >> (4 + 12 * q)) & 15);
                 const int e1 = (int) ((wl >> (8 + 12 * q)) & 15);
@@ -1208,6 +1209,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                     const bool d02 = xa == xc && ya == yc && za == zc;
                     const bool d12 = xb == xc && yb == yc && zb == zc;
                     keep = !(d01 || d02 || d12);
+                    dropped |= !keep;
                     const int64_t i = seg_push(p, keep ? 2 : 3, spseg);
                     if (i >= 0) (keep ? p.seg.kept : p.seg.hole)[i] = slot;
                 }
@@ -1220,7 +1222,10 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                     }
                 }
             }
-            if (special) {
+            // the crossing edges of a cell that dropped a face: the fix-up
+            // needs them to tell whether a vertex still has a face (a vertex
+            // next to a cell that dropped none is referenced by its faces)
+            if (special && dropped) {
                 uint32_t m = (uint32_t) status;
                 while (m) {
                     const int e = __ffs(m) - 1;
@@ -1267,12 +1272,15 @@ __global__ void mc_dense_reset(McParams p, SpecialLists init) {
         const int64_t ntiles = p.NI * p.NJ * p.NK;
         for (int64_t i = t; i < ntiles; i += (int64_t) gridDim.x * blockDim.x) p.tile_flag[i] = 0;
     }
+    if (p.dslots)
+        for (int64_t i = t; i <= p.dmask; i += (int64_t) gridDim.x * blockDim.x) p.dslots[i] = -1;
     if (t == 0) {
         if (p.tile_count) *p.tile_count = 0;
         *p.err = 0;
         p.totals[0] = p.totals[1] = 0;
         init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
         init.need = 0;
+        init.dup = p.dslots ? 0 : -1;
         *p.sp = init;
     }
 }
@@ -1339,6 +1347,34 @@ __device__ __forceinline__ void sp_gather_seg(const McParams &p, int b) {
 
 __global__ void __launch_bounds__(256) k_sp_gather(McParams p) { sp_gather_seg(p, blockIdx.x); }
 
+// Whether two gathered weld candidates share a position (as the fix-up
+// groups them: equal coordinates): candidate indices go into an
+// open-addressing table keyed by position.  With no shared position and no
+// dropped face the fix-up has nothing to do.
+__global__ void __launch_bounds__(256) k_sp_dupcheck(McParams p) {
+    const SpecialVertex *cv = p.sp->cand;
+    const int64_t n = min((int64_t) p.sp->n_cand, p.sp->cap_cand);
+    const uint64_t mask = (uint64_t) p.dmask;
+    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+        const SpecialVertex v = cv[i];
+        const uint64_t h = f3_hash(v.x, v.y, v.z) & mask;
+        for (uint64_t probe = 0; probe <= mask; ++probe) {
+            const int64_t slot = (int64_t) ((h + probe) & mask);
+            int32_t cur = p.dslots[slot];
+            if (cur < 0) {
+                const int32_t prev = atomicCAS(&p.dslots[slot], -1, (int32_t) i);
+                if (prev < 0) break;
+                cur = prev;
+            }
+            const SpecialVertex o = cv[cur];
+            if (o.x == v.x && o.y == v.y && o.z == v.z) {
+                atomicExch((unsigned long long *) &p.sp->dup, 1ull);
+                break;
+            }
+        }
+    }
+}
+
 // ---- ragged batches ----------------------------------------------------------------
 // Every kernel walks the batch-wide index space of its stage (classify words,
 // count blocks, emit tiles) and finds the owning item by binary search over
@@ -1374,6 +1410,7 @@ __global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, c
             SpecialLists init = items[i].sp_init;
             init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
             init.need = 0;
+            init.dup = -1;
             *p.sp = init;
         }
     }
@@ -1841,6 +1878,7 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(4)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sp_gather, dim3(p.seg.nseg), dim3(256), 0, stream, p);
+    if (p.dslots) hipLaunchKernelGGL(k_sp_dupcheck, dim3(64), dim3(256), 0, stream, p);
     return hipGetLastError();
 }
 
@@ -2891,6 +2929,7 @@ __device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int 
             const uint64_t wl = s.tris[cs];
             const int ntri = (int) (wl & 15);
             int64_t slot = (int64_t) s.fb[rrow] + fo;
+            bool dropped = false;   // a face of this cell is degenerate
             for (int q = 0; q < ntri; ++q, ++slot) {
                 const int e0 = (int) ((wl >> (4 + 12 * q)) & 15);
                 const int e1 = (int) ((wl >> (8 + 12 * q)) & 15);
@@ -2907,6 +2946,7 @@ __device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int 
                     const bool d02 = a.x == c.x && a.y == c.y && a.z == c.z;
                     const bool d12 = b.x == c.x && b.y == c.y && b.z == c.z;
                     keep = !(d01 || d02 || d12);
+                    dropped |= !keep;
                     const int64_t i = seg_push(p, keep ? 2 : 3, spseg);
                     if (i >= 0) (keep ? p.seg.kept : p.seg.hole)[i] = slot;
                 }
@@ -2915,7 +2955,8 @@ __device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int 
                     *reinterpret_cast<int3 *>(dst) = keep ? make_int3(i0, i1, i2) : make_int3(-1, -1, -1);
                 }
             }
-            if (special) {
+            // crossing edges of a cell that dropped a face (see emit_tile)
+            if (special && dropped) {
                 uint32_t m = status;
                 while (m) {
                     const int e = __ffs(m) - 1;
@@ -2949,6 +2990,8 @@ __global__ void k_stream_reset(McParams p, SpecialLists init) {
         const int64_t ntiles = p.NI * p.NJ * p.NK;
         for (int64_t i = t; i < ntiles; i += nt) p.tile_flag[i] = 0;
     }
+    if (p.dslots)
+        for (int64_t i = t; i <= p.dmask; i += nt) p.dslots[i] = -1;
     if (t == 0) {
         if (p.tile_count) *p.tile_count = 0;
         *p.acount = 0;
@@ -2956,6 +2999,7 @@ __global__ void k_stream_reset(McParams p, SpecialLists init) {
         p.totals[0] = p.totals[1] = 0;
         init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
         init.need = 0;
+        init.dup = p.dslots ? 0 : -1;
         *p.sp = init;
     }
 }
@@ -2995,6 +3039,7 @@ hipError_t mc_stream_launch(McParams &p, const SpecialLists &sp_init, hipStream_
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(4)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_sp_gather, dim3(p.seg.nseg), dim3(256), 0, stream, p);
+    if (p.dslots) hipLaunchKernelGGL(k_sp_dupcheck, dim3(64), dim3(256), 0, stream, p);
     return hipGetLastError();
 }
 
