@@ -22,13 +22,13 @@ struct SparseMcIn {
 };
 
 struct SparseMcWork {
+    uint16_t *cst;        // crossing-edge mask of each cell
     int64_t *cell_inst;   // first instance of each active cell
     float *ipos;          // instance positions
     int64_t *isrc;        // slot * 12 + edge of each instance
-    int64_t *islot;       // hash slot of each instance
-    unsigned long long *hkey, *hmin;
-    uint32_t *hused;
-    int64_t hmask;
+    int64_t *rkey;        // slot * 12 + edge of its representative (first instance at its position)
+    uint8_t *used;        // the representative is a corner of a kept face
+    int32_t *vid;         // vertex id of the instance's representative
     int *err;
 };
 

@@ -10,10 +10,13 @@
 //
 // Each sparse cell carries its own 8 corner values, so two cells sharing an
 // edge may disagree about its vertex; the reference welds by position only.
-// Sparse MC therefore keys vertices by position (hash on the float triple,
-// equality on floats), with the representative of a position = the smallest
-// (slot * 12 + edge) that produces it; vertex ids follow representative order
-// and faces follow (slot, LUT slot) order, so output is deterministic.
+// Sparse MC therefore welds instances with equal positions (equality on
+// floats), with the representative of a position = the smallest (slot * 12
+// + edge) that produces it; vertex ids follow representative order and
+// faces follow (slot, LUT slot) order, so output is deterministic.  The
+// instances that can share a position are found on the sorted ids: a column
+// table gives the slots of every (x, y) cell column, and a neighbour is a
+// binary search inside its column (see k_smc_inst).
 // Degenerate triangles (two equal corners) are dropped as in
 // src/mc/nagae.cu:71; a position is emitted only if a kept triangle uses it.
 // The per-vertex representative is exported for the backward pass.
@@ -106,15 +109,18 @@ __device__ __forceinline__ int scross(int cs) {
     return st;
 }
 
-// Pass 1: crossing-edge instances per cell -> block sums.
-__global__ void k_smc_count(SparseMcIn in, int64_t *bsum) {
+// Pass 1: crossing-edge instances per cell -> block sums; optionally each
+// cell's crossing-edge mask.
+__global__ void k_smc_count(SparseMcIn in, int64_t *bsum, uint16_t *cst) {
     int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
     int64_t k = 0;
     if (i < in.n) {
         int cs = 0;
 #pragma unroll
         for (int j = 0; j < 8; ++j) cs |= inside(in.values[8 * i + j], in.level) << j;
-        if (cs != 0 && cs != 255) k = __popc(scross(cs));
+        const int st = (cs != 0 && cs != 255) ? scross(cs) : 0;
+        k = __popc(st);
+        if (cst) cst[i] = (uint16_t) st;
     }
     __shared__ int64_t sh[kB];
     sh[threadIdx.x] = k;
@@ -140,79 +146,324 @@ __device__ __forceinline__ int64_t block_excl(int64_t v, int64_t *sh) {
     return r;
 }
 
-// Pass 2: write each instance's position; instance ids are (slot, edge) ordered.
-__global__ void k_smc_inst(SparseMcIn in, const int64_t *bpre, SparseMcWork w) {
+// ---- neighbour lookup on the sorted ids ------------------------------------------
+// The cells of column (x, y) (ids (x (Y-1) + y) (Z-1) + z, sorted) are the
+// slots [cf[c], cf[c + 1]) of that column c = x (Y-1) + y, cf[c] the first
+// slot whose column is >= c (cf[ncol] = n); a neighbour is found by a binary
+// search inside its column.
+__global__ void k_smc_cols(const int64_t *ids, int64_t n, int64_t cz, int32_t *cf) {
+    const int64_t s = (int64_t) blockIdx.x * kB + threadIdx.x;
+    if (s >= n) return;
+    const int64_t col = div_fast(ids[s], cz);
+    if (s == 0 || div_fast(ids[s - 1], cz) != col) cf[col] = (int32_t) s;
+}
+
+// Suffix minimum over cf (empty columns take the next column's start):
+// per 1024-entry chunk, then over the chunk minima, then applied.
+__global__ void k_cols_chunk(int32_t *cf, int64_t m, int32_t *cmin) {
+    __shared__ int32_t sh[1024];
+    const int64_t i = (int64_t) blockIdx.x * 1024 + (1023 - threadIdx.x);   // reversed within the chunk
+    sh[threadIdx.x] = i < m ? cf[i] : INT32_MAX;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int32_t y = threadIdx.x >= (unsigned) o ? sh[threadIdx.x - o] : INT32_MAX;
+        __syncthreads();
+        sh[threadIdx.x] = min(sh[threadIdx.x], y);
+        __syncthreads();
+    }
+    if (i < m) cf[i] = sh[threadIdx.x];
+    if (threadIdx.x == 1023) cmin[blockIdx.x] = sh[1023];
+}
+// Chunk minima -> minimum of the chunks after each (one workgroup; each
+// thread a contiguous run).
+__global__ void k_cols_top(int32_t *cmin, int64_t nc) {
+    __shared__ int32_t sh[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (nc + 1023) / 1024, lo = min<int64_t>(t * per, nc), hi = min<int64_t>(lo + per, nc);
+    int32_t m = INT32_MAX;
+    for (int64_t c = lo; c < hi; ++c) m = min(m, cmin[c]);
+    sh[t] = m;
+    __syncthreads();
+    // suffix minimum over the threads' runs (reversed inclusive scan)
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int32_t y = t + o < 1024 ? sh[t + o] : INT32_MAX;
+        __syncthreads();
+        sh[t] = min(sh[t], y);
+        __syncthreads();
+    }
+    int32_t run = t + 1 < 1024 ? sh[t + 1] : INT32_MAX;   // minimum of the runs after this one
+    for (int64_t c = hi - 1; c >= lo; --c) {
+        const int32_t v = cmin[c];
+        cmin[c] = run;
+        run = min(run, v);
+    }
+}
+__global__ void k_cols_apply(int32_t *cf, int64_t m, const int32_t *cmin) {
+    const int64_t i = (int64_t) blockIdx.x * 1024 + threadIdx.x;
+    if (i < m) cf[i] = min(cf[i], cmin[blockIdx.x]);
+}
+
+struct SLook {
+    const int64_t *ids;
+    const int32_t *cf;
+    int64_t cx, cy, cz;   // cells per axis
+};
+
+// Slot of cell (x, y, z), or -1.
+__device__ __forceinline__ int64_t find_cell(const SLook &L, int64_t x, int64_t y, int64_t z) {
+    if (x < 0 || y < 0 || z < 0 || x >= L.cx || y >= L.cy || z >= L.cz) return -1;
+    const int64_t col = x * L.cy + y, id = col * L.cz + z;
+    int lo = L.cf[col];
+    const int end = L.cf[col + 1];
+    int hi = end;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (L.ids[mid] < id) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < end && L.ids[lo] == id) ? lo : -1;
+}
+
+// Cell edge from corner c (its bit along axis a clear) along axis a.
+__device__ __forceinline__ int edge_of(int c, int a) {
+    const int k = a * 8 + c;
+    return k < 16 ? (int) ((0xff57ff13ffffab98ull >> (4 * k)) & 15) : (int) ((0xf6f4f2f0u >> (4 * (k - 16))) & 15);
+}
+
+// Position of the vertex cell slot s (origin (x, y, z)) places on its edge e,
+// if that edge crosses the level in the cell (same arithmetic as spos).
+__device__ __forceinline__ bool inst_pos(const SparseMcIn &in, int64_t s, int64_t x, int64_t y, int64_t z, int e,
+                                         float &px, float &py, float &pz) {
+    const int ca = edge_a(e), cb = edge_b(e);
+    const float va = in.values[8 * s + ca], vb = in.values[8 * s + cb];
+    if (inside(va, in.level) == inside(vb, in.level)) return false;
+    const float t = edge_t(va, vb, in.level);
+    px = lerp1(t, grid_coord(x + (ca >> 2), in.g.X, in.g.amin[0], in.g.amax[0]),
+               grid_coord(x + (cb >> 2), in.g.X, in.g.amin[0], in.g.amax[0]));
+    py = lerp1(t, grid_coord(y + ((ca >> 1) & 1), in.g.Y, in.g.amin[1], in.g.amax[1]),
+               grid_coord(y + ((cb >> 1) & 1), in.g.Y, in.g.amin[1], in.g.amax[1]));
+    pz = lerp1(t, grid_coord(z + (ca & 1), in.g.Z, in.g.amin[2], in.g.amax[2]),
+               grid_coord(z + (cb & 1), in.g.Z, in.g.amin[2], in.g.amax[2]));
+    return true;
+}
+
+// The 9 cells sharing an edge with a cell that come before it in id order
+// (offsets with a negative id step): index of offset (dx, dy, dz), or -1.
+__device__ __forceinline__ int lower_nb(int dx, int dy, int dz) {
+    const int k = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);   // 0..26
+    // (0,0,-1) (0,-1,0) (-1,0,0) (0,-1,-1) (0,-1,1) (-1,0,-1) (-1,0,1) (-1,-1,0) (-1,1,0)
+    switch (k) {
+    case 12: return 0;
+    case 10: return 1;
+    case 4: return 2;
+    case 9: return 3;
+    case 11: return 4;
+    case 3: return 5;
+    case 5: return 6;
+    case 1: return 7;
+    case 7: return 8;
+    default: return -1;
+    }
+}
+__device__ __forceinline__ void lower_off(int k, int &dx, int &dy, int &dz) {
+    constexpr int8_t t[9][3] = {{0, 0, -1}, {0, -1, 0}, {-1, 0, 0}, {0, -1, -1}, {0, -1, 1},
+                                {-1, 0, -1}, {-1, 0, 1}, {-1, -1, 0}, {-1, 1, 0}};
+    dx = t[k][0];
+    dy = t[k][1];
+    dz = t[k][2];
+}
+
+// Pass 2: each instance's position and representative key (slot * 12 +
+// edge of the first instance in (slot, edge) order with the same position).
+// A vertex more than 16 ulps from both endpoints along its edge can only
+// share its position with the same grid edge in the (at most 3) other cells
+// around it, and only those before this cell can lower the key: their
+// slots are looked up once per cell (independent searches), and an edge's
+// copy in such a cell is compared by its two end values first.  A vertex
+// nearer an endpoint P can share its position with any edge through P in
+// the 8 cells around P, which are searched directly (rare).
+constexpr int kSmcStage = 2048;   // ids a workgroup stages per neighbour range
+
+// Slot of cell id `id` in the staged sorted range (base, m entries), or -1.
+__device__ __forceinline__ int64_t find_staged(const int64_t *a, int m, int64_t base, int64_t id) {
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < id) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < m && a[lo] == id) ? base + lo : -1;
+}
+
+__global__ void __launch_bounds__(kB) k_smc_inst(SparseMcIn in, SLook L, const int64_t *bpre, SparseMcWork w) {
     __shared__ int64_t sh[kB];
-    int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
+    // the ids of the columns this workgroup's cells can look up: range A =
+    // columns [colF - 1, colL] (same x), range B = columns colF - (Y-1) - 1
+    // .. colL - (Y-1) + 1 (x - 1); each a contiguous run of slots
+    __shared__ int64_t ida[kSmcStage], idb[kSmcStage];
+    __shared__ int64_t rng[6];   // A lo, A count (or -1), B lo, B count (or -1), colF, colL
+    const int64_t first = (int64_t) blockIdx.x * kB, last = min(in.n, first + kB) - 1;
+    if (threadIdx.x == 0) {
+        const int64_t colF = div_fast(in.ids[first], L.cz), colL = div_fast(in.ids[last], L.cz);
+        const int64_t alo = L.cf[colF >= 1 ? colF - 1 : 0], ahi = L.cf[colL + 1];
+        const int64_t b0 = colF - L.cy - 1, b1 = colL - L.cy + 2;
+        const int64_t blo = b1 > 0 ? L.cf[b0 > 0 ? b0 : 0] : 0, bhi = b1 > 0 ? L.cf[b1] : 0;
+        rng[0] = alo;
+        rng[1] = ahi - alo <= kSmcStage ? ahi - alo : -1;
+        rng[2] = blo;
+        rng[3] = bhi - blo <= kSmcStage ? bhi - blo : -1;
+        rng[4] = colF;
+        rng[5] = colL;
+    }
+    __syncthreads();
+    const int64_t alo = rng[0], an = rng[1], blo = rng[2], bn = rng[3], colF = rng[4], colL = rng[5];
+    for (int64_t k = threadIdx.x; k < an; k += kB) ida[k] = in.ids[alo + k];
+    for (int64_t k = threadIdx.x; k < bn; k += kB) idb[k] = in.ids[blo + k];
+    int64_t i = first + threadIdx.x;
     SCell s;
     bool act = i < in.n && load_cell(in, i, s);
     int st = act ? scross(s.cs) : 0;
-    int64_t off = bpre[blockIdx.x] + block_excl(__popc(st), sh);
+    int64_t off = bpre[blockIdx.x] + block_excl(__popc(st), sh);   // (synchronises: the staged ids are in)
     if (!act) return;
+    // a neighbour's slot: staged range if its column is there, else the column table
+    auto lookup = [&](int64_t xx, int64_t yy, int64_t zz) -> int64_t {
+        if (xx < 0 || yy < 0 || zz < 0 || xx >= L.cx || yy >= L.cy || zz >= L.cz) return -1;
+        const int64_t col = xx * L.cy + yy, id = col * L.cz + zz;
+        if (an >= 0 && col >= colF - 1 && col <= colL) return find_staged(ida, (int) an, alo, id);
+        if (bn >= 0 && col >= colF - L.cy - 1 && col <= colL - L.cy + 1) return find_staged(idb, (int) bn, blo, id);
+        return find_cell(L, xx, yy, zz);
+    };
     w.cell_inst[i] = off;
+    int64_t x, y, z;
+    cell_xyz(in.ids[i], in.g.Y, in.g.Z, x, y, z);
+    const int64_t o[3] = {x, y, z};
+    // candidacy, and the lower neighbours the other edges need
+    uint32_t candm = 0, need = 0;
     for (int e = 0; e < 12; ++e) {
         if (!((st >> e) & 1)) continue;
-        float x, y, z;
-        spos(s, in.level, e, x, y, z);
-        w.ipos[3 * off + 0] = x;
-        w.ipos[3 * off + 1] = y;
-        w.ipos[3 * off + 2] = z;
+        float p3[3];
+        spos(s, in.level, e, p3[0], p3[1], p3[2]);
+        const int ca = edge_a(e), cb = edge_b(e), ax = edge_axis(e);
+        const int sa = ax == 0 ? (ca >> 2) : (ax == 1 ? ((ca >> 1) & 1) : (ca & 1));
+        const int sb = ax == 0 ? (cb >> 2) : (ax == 1 ? ((cb >> 1) & 1) : (cb & 1));
+        const float ra = ax == 0 ? p3[0] : (ax == 1 ? p3[1] : p3[2]);
+        if (near_endpoint(ra, s.c[ax][sa], s.c[ax][sb])) {
+            candm |= 1u << e;
+            continue;
+        }
+        const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
+        const int bu = (ca >> (2 - u)) & 1, bv = (ca >> (2 - v)) & 1;
+        for (int d = 0; d < 4; ++d) {
+            int dd[3] = {0, 0, 0};
+            dd[u] = bu - (d & 1);
+            dd[v] = bv - (d >> 1);
+            const int k = lower_nb(dd[0], dd[1], dd[2]);
+            if (k >= 0) need |= 1u << k;
+        }
+    }
+    int64_t nbs[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        int dx, dy, dz;
+        lower_off(k, dx, dy, dz);
+        nbs[k] = ((need >> k) & 1u) ? lookup(x + dx, y + dy, z + dz) : -1;
+    }
+    for (int e = 0; e < 12; ++e) {
+        if (!((st >> e) & 1)) continue;
+        float px, py, pz;
+        spos(s, in.level, e, px, py, pz);
+        int64_t rep = i * 12 + e;
+        const int ca = edge_a(e), cb = edge_b(e), ax = edge_axis(e);
+        if (!((candm >> e) & 1u)) {
+            const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
+            const int bu = (ca >> (2 - u)) & 1, bv = (ca >> (2 - v)) & 1;
+            for (int d = 0; d < 4; ++d) {
+                const int du = d & 1, dv = d >> 1;
+                int dd[3] = {0, 0, 0};
+                dd[u] = bu - du;
+                dd[v] = bv - dv;
+                const int k = lower_nb(dd[0], dd[1], dd[2]);
+                if (k < 0 || nbs[k] < 0) continue;
+                const int64_t s2 = nbs[k];
+                const int cc = (du << (2 - u)) | (dv << (2 - v));
+                const int e2 = edge_of(cc, ax);
+                const float va = in.values[8 * s2 + cc], vb = in.values[8 * s2 + (cc | (4 >> ax))];
+                bool same;
+                if (va == s.cv[ca] && vb == s.cv[cb]) {
+                    same = true;   // the same end values: the same vertex
+                } else {
+                    if (inside(va, in.level) == inside(vb, in.level)) continue;
+                    const float t = edge_t(va, vb, in.level);
+                    const float qx = lerp1(t, s.c[0][ca >> 2], s.c[0][cb >> 2]);
+                    const float qy = lerp1(t, s.c[1][(ca >> 1) & 1], s.c[1][(cb >> 1) & 1]);
+                    const float qz = lerp1(t, s.c[2][ca & 1], s.c[2][cb & 1]);
+                    same = qx == px && qy == py && qz == pz;
+                }
+                if (same) rep = min(rep, s2 * 12 + e2);
+            }
+        } else {
+            // the endpoint it is near, and the 3 edges through it in each cell around it
+            const int sa = ax == 0 ? (ca >> 2) : (ax == 1 ? ((ca >> 1) & 1) : (ca & 1));
+            const int sb = ax == 0 ? (cb >> 2) : (ax == 1 ? ((cb >> 1) & 1) : (cb & 1));
+            const float ra = ax == 0 ? px : (ax == 1 ? py : pz);
+            const int64_t kr = ordered_key(ra), ka = ordered_key(s.c[ax][sa]), kb2 = ordered_key(s.c[ax][sb]);
+            const int pc = (llabs(kr - ka) <= llabs(kr - kb2)) ? ca : cb;
+            const int64_t P[3] = {o[0] + (pc >> 2), o[1] + ((pc >> 1) & 1), o[2] + (pc & 1)};
+            for (int d = 0; d < 8; ++d) {
+                const int64_t c0 = P[0] - (d >> 2), c1 = P[1] - ((d >> 1) & 1), c2 = P[2] - (d & 1);
+                const int64_t s2 = lookup(c0, c1, c2);
+                if (s2 < 0 || s2 > i) continue;
+                for (int a2 = 0; a2 < 3; ++a2) {
+                    const int e2 = edge_of(d & ~(4 >> a2), a2);
+                    if (s2 == i && e2 == e) continue;
+                    float qx, qy, qz;
+                    if (inst_pos(in, s2, c0, c1, c2, e2, qx, qy, qz) && qx == px && qy == py && qz == pz)
+                        rep = min(rep, s2 * 12 + e2);
+                }
+            }
+        }
+        w.ipos[3 * off + 0] = px;
+        w.ipos[3 * off + 1] = py;
+        w.ipos[3 * off + 2] = pz;
         w.isrc[off] = i * 12 + e;
+        w.rkey[off] = rep;
         ++off;
     }
-}
-
-// Pass 3: group instances by position (hash slot holds the first writer; the
-// representative is the smallest instance id with that position).
-__global__ void k_smc_hash(SparseMcWork w, int64_t ninst) {
-    int64_t q = (int64_t) blockIdx.x * kB + threadIdx.x;
-    if (q >= ninst) return;
-    float x = w.ipos[3 * q], y = w.ipos[3 * q + 1], z = w.ipos[3 * q + 2];
-    uint64_t h = f3_hash(x, y, z) & (uint64_t) w.hmask;
-    for (int64_t probe = 0; probe <= w.hmask; ++probe) {
-        int64_t sl = (int64_t) ((h + probe) & (uint64_t) w.hmask);
-        unsigned long long cur = w.hkey[sl];
-        if (cur == ~0ull) {
-            unsigned long long prev = atomicCAS(&w.hkey[sl], ~0ull, (unsigned long long) q);
-            if (prev == ~0ull) cur = (unsigned long long) q;
-            else cur = prev;
-        }
-        if (w.ipos[3 * cur] == x && w.ipos[3 * cur + 1] == y && w.ipos[3 * cur + 2] == z) {
-            w.islot[q] = sl;
-            atomicMin(&w.hmin[sl], (unsigned long long) q);
-            return;
-        }
-    }
-    atomicOr(w.err, 1);
 }
 
 __device__ __forceinline__ int64_t inst_of(int st, int e, int64_t base) {
     return base + __popc(st & ((1 << e) - 1));
 }
 
-// Pass 4: kept faces per cell (a triangle with two corners at one position is
-// dropped); positions used by kept faces are marked.
+// Instance index of a representative key.
+__device__ __forceinline__ int64_t inst_of_key(const SparseMcWork &w, int64_t key) {
+    const int64_t slot = key / 12;
+    return inst_of((int) w.cst[slot], (int) (key - slot * 12), w.cell_inst[slot]);
+}
+
+// Pass 3: kept faces per cell (a triangle with two corners at one position is
+// dropped); the representatives of the kept faces' corners are marked used.
 __global__ void k_smc_faces(SparseMcIn in, SparseMcWork w, int64_t *bsum) {
     int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
     int64_t nf = 0;
     if (i < in.n) {
-        int cs = 0;
+        const int st = (int) w.cst[i];
+        if (st) {
+            int cs = 0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) cs |= inside(in.values[8 * i + j], in.level) << j;
-        if (cs != 0 && cs != 255) {
-            const int st = scross(cs);
+            for (int j = 0; j < 8; ++j) cs |= inside(in.values[8 * i + j], in.level) << j;
             const int64_t base = w.cell_inst[i];
             const uint64_t tw = cSTris[in.method][cs];
             const int nt = (int) (tw & 15);
             for (int q = 0; q < nt; ++q) {
-                int64_t s0 = w.islot[inst_of(st, (int) ((tw >> (4 + 12 * q)) & 15), base)];
-                int64_t s1 = w.islot[inst_of(st, (int) ((tw >> (8 + 12 * q)) & 15), base)];
-                int64_t s2 = w.islot[inst_of(st, (int) ((tw >> (12 + 12 * q)) & 15), base)];
-                if (s0 == s1 || s0 == s2 || s1 == s2) continue;
+                const int64_t r0 = w.rkey[inst_of(st, (int) ((tw >> (4 + 12 * q)) & 15), base)];
+                const int64_t r1 = w.rkey[inst_of(st, (int) ((tw >> (8 + 12 * q)) & 15), base)];
+                const int64_t r2 = w.rkey[inst_of(st, (int) ((tw >> (12 + 12 * q)) & 15), base)];
+                if (r0 == r1 || r0 == r2 || r1 == r2) continue;
                 ++nf;
-                w.hused[s0] = 1u;
-                w.hused[s1] = 1u;
-                w.hused[s2] = 1u;
+                w.used[inst_of_key(w, r0)] = 1;
+                w.used[inst_of_key(w, r1)] = 1;
+                w.used[inst_of_key(w, r2)] = 1;
             }
         }
     }
@@ -226,12 +477,15 @@ __global__ void k_smc_faces(SparseMcIn in, SparseMcWork w, int64_t *bsum) {
     if (threadIdx.x == 0) bsum[blockIdx.x] = sh[0];
 }
 
-// Used positions mark their representative instance in a bitmap.
-__global__ void k_smc_mark(SparseMcWork w, int64_t nslots, uint32_t *bits) {
-    int64_t sl = (int64_t) blockIdx.x * kB + threadIdx.x;
-    if (sl >= nslots || w.hkey[sl] == ~0ull || !w.hused[sl]) return;
-    unsigned long long k = w.hmin[sl];
-    atomicOr(&bits[k >> 5], 1u << (k & 31));
+// Output vertices: used representatives, one bit per instance (64 per wave).
+__global__ void k_smc_vbits(SparseMcWork w, int64_t ninst, uint32_t *bits) {
+    const int64_t q = (int64_t) blockIdx.x * kB + threadIdx.x;
+    const bool is_v = q < ninst && w.used[q] && w.rkey[q] == w.isrc[q];
+    const uint64_t m = __ballot(is_v);
+    const int lane = threadIdx.x & 63;
+    const int64_t w0 = (q - lane) >> 5;
+    if (lane == 0 && w0 < (ninst + 31) / 32) bits[w0] = (uint32_t) m;
+    if (lane == 32 && w0 + 1 < (ninst + 31) / 32) bits[w0 + 1] = (uint32_t) (m >> 32);
 }
 
 __device__ __forceinline__ int64_t rank_of(const uint32_t *bits, const int64_t *pre, int64_t k) {
@@ -240,56 +494,59 @@ __device__ __forceinline__ int64_t rank_of(const uint32_t *bits, const int64_t *
     return pre[k >> 5] + __popc(below);
 }
 
-// Pass 5: faces in (slot, LUT slot) order, representatives write vertices.
+// Vertex id of each instance's representative.
+__global__ void k_smc_vid(SparseMcWork w, int64_t ninst, const uint32_t *bits, const int64_t *wpre) {
+    const int64_t q = (int64_t) blockIdx.x * kB + threadIdx.x;
+    if (q >= ninst) return;
+    const int64_t r = inst_of_key(w, w.rkey[q]);
+    w.vid[q] = (int32_t) rank_of(bits, wpre, r);
+}
+
+// Pass 4: faces in (slot, LUT slot) order, representatives write vertices.
 __global__ void k_smc_emit(SparseMcIn in, SparseMcWork w, const int64_t *fpre, const uint32_t *bits,
                            const int64_t *wpre, SparseMcOut o) {
     __shared__ int64_t sh[kB];
     int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
+    const int st = i < in.n ? (int) w.cst[i] : 0;
     int cs = 0;
-    if (i < in.n) {
+    if (st) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) cs |= inside(in.values[8 * i + j], in.level) << j;
     }
-    const bool act = i < in.n && cs != 0 && cs != 255;
-    const int st = act ? scross(cs) : 0;
-    const int64_t base = act ? w.cell_inst[i] : 0;
-    const uint64_t tw = act ? cSTris[in.method][cs] : 0;
+    const int64_t base = st ? w.cell_inst[i] : 0;
+    const uint64_t tw = st ? cSTris[in.method][cs] : 0;
     const int nt = (int) (tw & 15);
     int64_t nf = 0;
     for (int q = 0; q < nt; ++q) {
-        int64_t s0 = w.islot[inst_of(st, (int) ((tw >> (4 + 12 * q)) & 15), base)];
-        int64_t s1 = w.islot[inst_of(st, (int) ((tw >> (8 + 12 * q)) & 15), base)];
-        int64_t s2 = w.islot[inst_of(st, (int) ((tw >> (12 + 12 * q)) & 15), base)];
-        if (!(s0 == s1 || s0 == s2 || s1 == s2)) ++nf;
+        const int64_t r0 = w.rkey[inst_of(st, (int) ((tw >> (4 + 12 * q)) & 15), base)];
+        const int64_t r1 = w.rkey[inst_of(st, (int) ((tw >> (8 + 12 * q)) & 15), base)];
+        const int64_t r2 = w.rkey[inst_of(st, (int) ((tw >> (12 + 12 * q)) & 15), base)];
+        if (!(r0 == r1 || r0 == r2 || r1 == r2)) ++nf;
     }
     int64_t f = fpre[blockIdx.x] + block_excl(nf, sh);
-    if (!act) return;
+    if (!st) return;
     for (int q = 0; q < nt; ++q) {
-        int64_t s0 = w.islot[inst_of(st, (int) ((tw >> (4 + 12 * q)) & 15), base)];
-        int64_t s1 = w.islot[inst_of(st, (int) ((tw >> (8 + 12 * q)) & 15), base)];
-        int64_t s2 = w.islot[inst_of(st, (int) ((tw >> (12 + 12 * q)) & 15), base)];
-        if (s0 == s1 || s0 == s2 || s1 == s2) continue;
+        const int64_t i0 = inst_of(st, (int) ((tw >> (4 + 12 * q)) & 15), base);
+        const int64_t i1 = inst_of(st, (int) ((tw >> (8 + 12 * q)) & 15), base);
+        const int64_t i2 = inst_of(st, (int) ((tw >> (12 + 12 * q)) & 15), base);
+        const int64_t r0 = w.rkey[i0], r1 = w.rkey[i1], r2 = w.rkey[i2];
+        if (r0 == r1 || r0 == r2 || r1 == r2) continue;
         if (f < o.cap_f) {
-            o.f[3 * f + 0] = (int32_t) rank_of(bits, wpre, (int64_t) w.hmin[s0]);
-            o.f[3 * f + 1] = (int32_t) rank_of(bits, wpre, (int64_t) w.hmin[s1]);
-            o.f[3 * f + 2] = (int32_t) rank_of(bits, wpre, (int64_t) w.hmin[s2]);
+            o.f[3 * f + 0] = w.vid[i0];
+            o.f[3 * f + 1] = w.vid[i1];
+            o.f[3 * f + 2] = w.vid[i2];
         }
         ++f;
     }
-    int64_t q = base;
-    for (int e = 0; e < 12; ++e) {
-        if (!((st >> e) & 1)) continue;
-        int64_t sl = w.islot[q];
-        if ((int64_t) w.hmin[sl] == q && w.hused[sl]) {
-            int64_t v = rank_of(bits, wpre, q);
-            if (v < o.cap_v) {
-                o.v[3 * v + 0] = w.ipos[3 * q + 0];
-                o.v[3 * v + 1] = w.ipos[3 * q + 1];
-                o.v[3 * v + 2] = w.ipos[3 * q + 2];
-                if (o.vsrc) o.vsrc[v] = w.isrc[q];
-            }
+    for (int64_t q = base; q < base + __popc(st); ++q) {
+        if (!((bits[q >> 5] >> (q & 31)) & 1u)) continue;
+        const int64_t v = rank_of(bits, wpre, q);
+        if (v < o.cap_v) {
+            o.v[3 * v + 0] = w.ipos[3 * q + 0];
+            o.v[3 * v + 1] = w.ipos[3 * q + 1];
+            o.v[3 * v + 2] = w.ipos[3 * q + 2];
+            if (o.vsrc) o.vsrc[v] = w.isrc[q];
         }
-        ++q;
     }
 }
 
@@ -344,6 +601,37 @@ __global__ void k_bit_block(const uint32_t *bits, int64_t nw, int64_t *pre, int6
 __global__ void k_bit_add(int64_t *pre, int64_t nw, const int64_t *bsum) {
     int64_t i = (int64_t) blockIdx.x * 1024 + threadIdx.x;
     if (i < nw) pre[i] += bsum[blockIdx.x];
+}
+
+// Sum of n int64 entries (one workgroup, coalesced) to *tot.
+__global__ void k_sum_i64(const int64_t *a, int64_t n, int64_t *tot) {
+    __shared__ int64_t sh[1024];
+    int64_t v = 0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) v += a[i];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned) o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *tot = sh[0];
+}
+
+// In-place exclusive scan of each 1024-entry chunk of a; chunk sums to bsum.
+__global__ void k_scan_chunk(int64_t *a, int64_t n, int64_t *bsum) {
+    __shared__ int64_t sh[1024];
+    int64_t i = (int64_t) blockIdx.x * 1024 + threadIdx.x;
+    int64_t v = i < n ? a[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int64_t y = threadIdx.x >= (unsigned) o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += y;
+        __syncthreads();
+    }
+    if (i < n) a[i] = sh[threadIdx.x] - v;
+    if (threadIdx.x == 1023) bsum[blockIdx.x] = sh[1023];
 }
 
 // ---- population: sorted-key set operations (src/grid/sparse.cu:71-121) ----------
@@ -501,46 +789,68 @@ hipError_t sparse_mc_luts() {
 }
 
 size_t sparse_mc_work_bytes(int64_t n, int64_t ninst) {
-    int64_t hs = 1;
-    while (hs < 2 * ninst + 64) hs <<= 1;
-    int64_t nw = (ninst + 31) / 32 + 1;
+    const int64_t nw = (ninst + 31) / 32 + 1, nblk = nb(n);
     size_t b = 0;
     auto add = [&](size_t x) { b += (x + 255) & ~size_t(255); };
-    add(8 * n);                      // cell_inst
-    add(12 * ninst + 12);            // ipos
-    add(8 * ninst + 8);              // isrc
-    add(8 * ninst + 8);              // islot
-    add(8 * hs);                     // hkey
-    add(8 * hs);                     // hmin
-    add(4 * hs);                     // hused
-    add(8 * (nb(n) + 8));            // block sums (instances)
-    add(8 * (nb(n) + 8));            // block sums (faces)
-    add(4 * nw);                     // bits
-    add(8 * nw);                     // word prefix
+    add(2 * n);                              // cst
+    add(8 * n);                              // cell_inst
+    add(12 * ninst + 12);                    // ipos
+    add(8 * ninst + 8);                      // isrc
+    add(8 * ninst + 8);                      // rkey
+    add(ninst + 8);                          // used
+    add(4 * ninst + 8);                      // vid
+    add(8 * (nblk + 8));                     // block sums (instances)
+    add(8 * (nblk + 8));                     // block sums (faces)
+    add(8 * ((nblk + 1023) / 1024 + 8));     // chunk sums (instances)
+    add(8 * ((nblk + 1023) / 1024 + 8));     // chunk sums (faces)
+    add(4 * nw);                             // bits
+    add(8 * nw);                             // word prefix
     add(8 * ((nw + 1023) / 1024 + 8));
     add(64);
     return b;
+}
+
+// Exclusive scan of m int64 entries in place (chunks of 1024, then the
+// chunk sums in one workgroup); the total to *tot.
+static void scan_i64(int64_t *a, int64_t m, int64_t *chunk, int64_t *tot, hipStream_t st) {
+    const int64_t nc = (m + 1023) / 1024;
+    hipLaunchKernelGGL(k_scan_chunk, dim3((unsigned) nc), dim3(1024), 0, st, a, m, chunk);
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, chunk, nc, tot);
+    hipLaunchKernelGGL(k_bit_add, dim3((unsigned) nc), dim3(1024), 0, st, a, m, (const int64_t *) chunk);
 }
 
 hipError_t sparse_mc_count(const SparseMcIn &in, int64_t *scratch, int64_t &ninst, hipStream_t st) {
     ninst = 0;
     if (in.n == 0) return hipSuccess;
     int64_t nblk = nb(in.n);
-    hipLaunchKernelGGL(k_smc_count, dim3((unsigned) nblk), dim3(kB), 0, st, in, scratch);
-    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, scratch, nblk, scratch + nblk);
+    hipLaunchKernelGGL(k_smc_count, dim3((unsigned) nblk), dim3(kB), 0, st, in, scratch, (uint16_t *) nullptr);
+    hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, st, (const int64_t *) scratch, nblk, scratch + nblk);
     ISO_TRY(hipGetLastError());
     ISO_TRY(hipMemcpyAsync(&ninst, scratch + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     ISO_TRY(hipStreamSynchronize(st));
     return hipSuccess;
 }
 
+namespace {
+// The column table of the calling thread (device memory, grown on demand,
+// freed when the thread ends).
+struct ColTable {
+    int32_t *p = nullptr;
+    size_t entries = 0;
+    int dev = -1;
+    ~ColTable() {
+        if (p) (void) hipFree(p);
+    }
+};
+thread_local ColTable tl_cols;
+}   // namespace
+
 hipError_t sparse_mc(const SparseMcIn &in, int64_t ninst, void *work, size_t work_bytes, SparseMcOut &o,
                      int64_t &nv, int64_t &nf, hipStream_t st) {
     nv = nf = 0;
     if (in.n == 0 || ninst == 0) return hipSuccess;
+    if (in.n >= (int64_t) INT32_MAX || ninst >= (int64_t) INT32_MAX) return hipErrorInvalidValue;
     ISO_TRY(sparse_mc_luts());
-    int64_t hs = 1;
-    while (hs < 2 * ninst + 64) hs <<= 1;
     const int64_t nw = (ninst + 31) / 32 + 1;
     const int64_t nblk = nb(in.n);
     char *p = (char *) work;
@@ -550,16 +860,17 @@ hipError_t sparse_mc(const SparseMcIn &in, int64_t ninst, void *work, size_t wor
         return r;
     };
     SparseMcWork w;
-    w.hmask = hs - 1;
+    w.cst = (uint16_t *) take(2 * in.n);
     w.cell_inst = (int64_t *) take(8 * in.n);
     w.ipos = (float *) take(12 * ninst + 12);
     w.isrc = (int64_t *) take(8 * ninst + 8);
-    w.islot = (int64_t *) take(8 * ninst + 8);
-    w.hkey = (unsigned long long *) take(8 * hs);
-    w.hmin = (unsigned long long *) take(8 * hs);
-    w.hused = (uint32_t *) take(4 * hs);
+    w.rkey = (int64_t *) take(8 * ninst + 8);
+    w.used = (uint8_t *) take(ninst + 8);
+    w.vid = (int32_t *) take(4 * ninst + 8);
     int64_t *ibs = (int64_t *) take(8 * (nblk + 8));
     int64_t *fbs = (int64_t *) take(8 * (nblk + 8));
+    int64_t *ich = (int64_t *) take(8 * ((nblk + 1023) / 1024 + 8));
+    int64_t *fch = (int64_t *) take(8 * ((nblk + 1023) / 1024 + 8));
     uint32_t *bits = (uint32_t *) take(4 * nw);
     int64_t *wpre = (int64_t *) take(8 * nw);
     const int64_t nwb = (nw + 1023) / 1024;
@@ -567,22 +878,41 @@ hipError_t sparse_mc(const SparseMcIn &in, int64_t ninst, void *work, size_t wor
     int64_t *misc = (int64_t *) take(64);
     w.err = (int *) (misc + 4);
     if ((size_t) (p - (char *) work) > work_bytes) return hipErrorInvalidValue;
+    // column table: (X - 1) (Y - 1) columns
+    const int64_t ncol = (in.g.X - 1) * (in.g.Y - 1);
+    int dev = -1;
+    ISO_TRY(hipGetDevice(&dev));
+    const int64_t ncf = ncol + 1, nchunk = (ncf + 1023) / 1024;
+    const size_t entries = (size_t) (ncf + nchunk);
+    if (tl_cols.entries < entries || tl_cols.dev != dev) {
+        if (tl_cols.p) ISO_TRY(hipFree(tl_cols.p));
+        tl_cols.p = nullptr;
+        tl_cols.entries = 0;
+        ISO_TRY(hipMalloc((void **) &tl_cols.p, sizeof(int32_t) * entries));
+        tl_cols.entries = entries;
+        tl_cols.dev = dev;
+    }
+    int32_t *cf = tl_cols.p, *cmin = tl_cols.p + ncf;
+    SLook L{in.ids, cf, in.g.X - 1, in.g.Y - 1, in.g.Z - 1};
 
-    hipLaunchKernelGGL(k_fill_u64, dim3(gs(hs)), dim3(kB), 0, st, w.hkey, hs, ~0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(gs(hs)), dim3(kB), 0, st, w.hmin, hs, ~0ull);
-    hipLaunchKernelGGL(k_fill_u32, dim3(gs(hs)), dim3(kB), 0, st, w.hused, hs, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(gs(nw)), dim3(kB), 0, st, bits, nw, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(gs(ncf)), dim3(kB), 0, st, (uint32_t *) cf, ncf, (uint32_t) in.n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(gs((ninst + 8) / 4 + 1)), dim3(kB), 0, st, (uint32_t *) w.used,
+                       (ninst + 8) / 4 + 1, 0u);
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kB), 0, st, (unsigned long long *) misc, 8, 0ull);
-    hipLaunchKernelGGL(k_smc_count, dim3((unsigned) nblk), dim3(kB), 0, st, in, ibs);
-    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, ibs, nblk, misc + 2);
-    hipLaunchKernelGGL(k_smc_inst, dim3((unsigned) nblk), dim3(kB), 0, st, in, ibs, w);
-    hipLaunchKernelGGL(k_smc_hash, dim3(nb(ninst)), dim3(kB), 0, st, w, ninst);
+    hipLaunchKernelGGL(k_smc_cols, dim3(nb(in.n)), dim3(kB), 0, st, in.ids, in.n, in.g.Z - 1, cf);
+    hipLaunchKernelGGL(k_cols_chunk, dim3((unsigned) nchunk), dim3(1024), 0, st, cf, ncf, cmin);
+    hipLaunchKernelGGL(k_cols_top, dim3(1), dim3(1024), 0, st, cmin, nchunk);
+    hipLaunchKernelGGL(k_cols_apply, dim3((unsigned) nchunk), dim3(1024), 0, st, cf, ncf, (const int32_t *) cmin);
+    hipLaunchKernelGGL(k_smc_count, dim3((unsigned) nblk), dim3(kB), 0, st, in, ibs, w.cst);
+    scan_i64(ibs, nblk, ich, misc + 2, st);
+    hipLaunchKernelGGL(k_smc_inst, dim3((unsigned) nblk), dim3(kB), 0, st, in, L, ibs, w);
     hipLaunchKernelGGL(k_smc_faces, dim3((unsigned) nblk), dim3(kB), 0, st, in, w, fbs);
-    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, fbs, nblk, misc + 0);
-    hipLaunchKernelGGL(k_smc_mark, dim3(nb(hs)), dim3(kB), 0, st, w, hs, bits);
-    hipLaunchKernelGGL(k_bit_block, dim3((unsigned) nwb), dim3(1024), 0, st, bits, nw, wpre, wbs);
+    scan_i64(fbs, nblk, fch, misc + 0, st);
+    hipLaunchKernelGGL(k_smc_vbits, dim3(nb(ninst)), dim3(kB), 0, st, w, ninst, bits);
+    hipLaunchKernelGGL(k_bit_block, dim3((unsigned) nwb), dim3(1024), 0, st, bits, nw - 1, wpre, wbs);
     hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, st, wbs, nwb, misc + 1);
-    hipLaunchKernelGGL(k_bit_add, dim3((unsigned) nwb), dim3(1024), 0, st, wpre, nw, wbs);
+    hipLaunchKernelGGL(k_bit_add, dim3((unsigned) nwb), dim3(1024), 0, st, wpre, nw - 1, wbs);
+    hipLaunchKernelGGL(k_smc_vid, dim3(nb(ninst)), dim3(kB), 0, st, w, ninst, bits, wpre);
     ISO_TRY(hipGetLastError());
     int64_t h[5];
     ISO_TRY(hipMemcpyAsync(h, misc, sizeof(h), hipMemcpyDeviceToHost, st));
