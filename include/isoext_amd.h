@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define ISOEXT_ABI_VERSION 1
+#define ISOEXT_ABI_VERSION 2
 
 enum {
     ISOEXT_OK = 0,
@@ -312,6 +312,8 @@ typedef struct isoext_mc_item {
     int64_t cap_v;
     int32_t *out_f;
     int64_t cap_f;
+    const void *sdf;          /* optional: SDF program (isoext_sdf_upload) giving the field; values unused */
+    int sdf_normals;          /* dual contouring of an sdf item: 1 analytic program normals, 0 trilinear */
 } isoext_mc_item;
 
 /* Workspace for a batch (zero-filled before first use); special_cap is the
@@ -364,6 +366,49 @@ int64_t isoext_dc_batch_scratch_bytes(int n, const isoext_mc_item *items, const 
 int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *counts, float reg, float svd_tol,
                         void *workspace, int64_t workspace_bytes, void *scratch, int64_t scratch_bytes, float *out_v,
                         int64_t cap_v, int32_t *out_f, int64_t cap_f, int64_t *out_counts, void *stream);
+
+/* ---- SDF programs (the step before the path: UniformGrid.set_values(sdf(p)),
+ * src/grid/uniform.cu:37-40, with the SDF built from the primitives and CSG
+ * operators of src/isoext/sdf.py:70-221) ------------------------------------------
+ * A program is n_ops ops of two int32 each -- a code word
+ * op | dst << 8 | src << 12 | acc << 16 and a parameter offset -- over 8 value
+ * slots and 4 point slots (point slot 0 = the evaluation point, result =
+ * value slot 0), with n_params float parameters:
+ *   1 SPHERE r | 2 SPHERE_AT cx cy cz r | 3 TORUS R r | 4 CUBOID hx hy hz |
+ *   5 CYLINDER_Z r   (primitives: value slot dst from point slot src,
+ *                     acc 0 store, 1 min, 2 max into the slot)
+ *   6 VALUE (v[src], acc as above) | 7 SMOOTH_MIN k (v[dst] with v[src]) |
+ *   8 NEG (v[dst]) | 9 TRANSLATE ox oy oz | 10 ROTATE R00..R22 (point slot
+ *   dst = point slot src - o, or @ R).
+ * Formulas keep the op order of the reference's torch code.  At most 256 ops
+ * and 1024 parameters. */
+
+/* Device bytes of an uploaded program. */
+int64_t isoext_sdf_program_bytes(void);
+
+/* Validate a program (ops, params in HOST memory) and write its device form
+ * to `program` (isoext_sdf_program_bytes() bytes of device memory, 16-byte
+ * aligned).  Synchronises `stream`. */
+int isoext_sdf_upload(const int32_t *ops, int32_t n_ops, const float *params, int32_t n_params, void *program,
+                      void *stream);
+
+/* The program's values at the grid points of (shape, aabb): the stored grid
+ * of UniformGrid.set_values(sdf(grid.get_points())). */
+int isoext_sdf_grid(const void *program, const int64_t shape[3], const float aabb_min[3], const float aabb_max[3],
+                    float *out, void *stream);
+
+/* Values (optional) and analytic unit normals (optional: get_sdf_normal,
+ * src/isoext/sdf.py:39-50) at n points (n x 3 floats). */
+int isoext_sdf_points(const void *program, const float *points, int64_t n, float *values, float *normals,
+                      void *stream);
+
+/* Marching cubes of a program's field without storing the grid (same
+ * arguments and results as isoext_mc_dense on the grid isoext_sdf_grid
+ * writes; identical meshes).  No reference counterpart: the reference always
+ * stores the grid first (src/grid/uniform.cu:37-40, src/mc/mc.cu:17-68). */
+int isoext_mc_sdf(const void *program, const int64_t shape[3], const float aabb_min[3], const float aabb_max[3],
+                  float level, int method, int cfg, void *workspace, int64_t workspace_bytes, int64_t special_cap,
+                  float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f, int64_t counts[6], void *stream);
 
 /* ---- OBJ output ------------------------------------------------------------- */
 

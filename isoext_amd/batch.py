@@ -33,7 +33,7 @@ class McItem(ctypes.Structure):
     _fields_ = [("values", ctypes.c_void_p), ("shape", ctypes.c_int64 * 3), ("aabb_min", ctypes.c_float * 3),
                 ("aabb_max", ctypes.c_float * 3), ("level", ctypes.c_float), ("method", ctypes.c_int),
                 ("out_v", ctypes.c_void_p), ("cap_v", ctypes.c_int64), ("out_f", ctypes.c_void_p),
-                ("cap_f", ctypes.c_int64)]
+                ("cap_f", ctypes.c_int64), ("sdf", ctypes.c_void_p), ("sdf_normals", ctypes.c_int)]
 
 
 _declared = False
@@ -71,25 +71,51 @@ def _decl():
 
 @dataclass
 class GridJob:
-    values: torch.Tensor
+    """One extraction of a batch: a stored grid (values), or the field of an
+    SDF program (sdf: a csg.SdfProgram, values None, shape given) evaluated
+    inside the kernels; sdf_normals: dual contouring with the program's
+    analytic normals instead of trilinear ones."""
+
+    values: torch.Tensor | None
     level: float = 0.0
     method: int = 0
     aabb_min: tuple = (-1.0, -1.0, -1.0)
     aabb_max: tuple = (1.0, 1.0, 1.0)
+    sdf: object = None
+    shape: tuple | None = None
+    sdf_normals: bool = False
+
+    @property
+    def grid_shape(self) -> tuple:
+        return tuple(int(s) for s in (self.values.shape if self.sdf is None else self.shape))
+
+    @property
+    def device(self):
+        return self.values.device if self.sdf is None else self.sdf.device
 
 
 BATCH_SPECIAL_CAP = 4096
 
 
 def _items(jobs, who):
-    dev = jobs[0].values.device
+    dev = jobs[0].device
     items = (McItem * len(jobs))()
     for it, j in zip(items, jobs):
-        if j.values.device != dev or j.values.dtype != torch.float32 or not j.values.is_contiguous():
-            raise TypeError(f"{who}: values must be contiguous float32 tensors on one device")
-        it.values = j.values.data_ptr()
+        if j.sdf is not None:
+            if j.sdf.device != dev or j.shape is None or len(j.shape) != 3:
+                raise TypeError(f"{who}: an SDF job needs a program on the batch's device and a 3-entry shape")
+            it.values = None
+            it.sdf = j.sdf.ptr
+            it.sdf_normals = 1 if j.sdf_normals else 0
+        else:
+            if j.values.device != dev or j.values.dtype != torch.float32 or not j.values.is_contiguous():
+                raise TypeError(f"{who}: values must be contiguous float32 tensors on one device")
+            it.values = j.values.data_ptr()
+            it.sdf = None
+            it.sdf_normals = 0
+        shape = j.grid_shape
         for a in range(3):
-            it.shape[a] = int(j.values.shape[a])
+            it.shape[a] = shape[a]
             it.aabb_min[a] = float(j.aabb_min[a])
             it.aabb_max[a] = float(j.aabb_max[a])
         it.level = float(j.level)
@@ -211,8 +237,8 @@ def mc_batch(jobs: list[GridJob], cfg: int | None = BATCH_CFG, ws_kind: str = "m
         if nv < 0:   # special lists overflowed the batch capacity: run this grid alone
             from .mc import mc_dense_raw
             j = jobs[i]
-            v, f, _ = mc_dense_raw(j.values, tuple(j.values.shape), j.aabb_min, j.aabb_max, j.level, j.method,
-                                   cfg if cfg >= 0 else None)
+            v, f, _ = mc_dense_raw(j.values, j.grid_shape, j.aabb_min, j.aabb_max, j.level, j.method,
+                                   cfg if cfg >= 0 else None, program=j.sdf)
             res.append((v, f))
         else:
             res.append((v[:nv], f[:nf]))

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -19,6 +20,7 @@
 #include "sparse.h"
 #include "backward.h"
 #include "mc_dense.h"
+#include "csg.h"
 
 namespace {
 
@@ -359,7 +361,7 @@ constexpr int kBatchSeg = iso::kMaxSpSegBatch;   // special-list segments per ba
 struct ItemLayout {
     bool cells;
     int cfg, nzc;   // the item's emit tile configuration
-    int64_t W, S, NI, NJ, NK, nblk, nctask, ntiles, tile0, seg_cap[4];
+    int64_t W, S, NI, NJ, NK, nblk, nctask, ngtask, ntiles, tile0, seg_cap[4];
     size_t o_hdr, o_bits, o_cnt_v, o_cnt_f, o_bsum_v, o_bsum_f, o_blk_v, o_blk_f;
     size_t o_ctr, o_scand, o_sscell, o_skept, o_shole, o_cand, o_scell, o_kept, o_hole;
 };
@@ -406,9 +408,12 @@ bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_c
         L.NJ = L.cells ? (Y - 1 + ty - 1) / ty : 0;
         L.NK = L.cells ? ((Z - 1 + 63) / 64 + nzc - 1) / nzc : 0;
         L.nblk = L.cells ? (X * Y * L.S + iso::kMcScanBlock - 1) / iso::kMcScanBlock : 0;
-        L.nctask = L.cells ? (X * Y * Z + iso::kFlatPts - 1) / iso::kFlatPts : 0;
+        L.nctask = L.cells && !items[i].sdf ? (X * Y * Z + iso::kFlatPts - 1) / iso::kFlatPts : 0;
+        L.ngtask = L.cells && items[i].sdf ? iso::csg_classify_tasks(X, Y, L.W) : 0;
         L.ntiles = L.NI * L.NJ * L.NK;
         B.h.nctask += L.nctask;
+        B.h.ngtask += L.ngtask;
+        if (items[i].sdf) ++B.h.n_prog;
         B.h.nblk += L.nblk;
         B.h.ntiles += L.ntiles;
         const int64_t R = L.cells ? X * Y : 0;
@@ -484,6 +489,7 @@ struct Pinned {
 };
 thread_local Pinned g_pinned;
 thread_local Pinned g_pinned_rb;   // single-call read-back (counts, list descriptors)
+thread_local Pinned g_pinned_prog;   // SDF program upload
 
 // Stage timing events of the calling thread, created once per device.
 hipEvent_t *timing_events() {
@@ -504,7 +510,7 @@ hipEvent_t *timing_events() {
 }
 
 void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t ctask0, int64_t blk0,
-               int64_t tile0, iso::McBatchItem &out) {
+               int64_t tile0, int64_t gtask0, iso::McBatchItem &out) {
     iso::McParams &p = out.p;
     p = iso::McParams{};
     p.values = in.values;
@@ -560,9 +566,15 @@ void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t 
     sp.scell = (iso::SpecialEdge *) (ws + L.o_scell);
     sp.kept = (int64_t *) (ws + L.o_kept);
     sp.hole = (int64_t *) (ws + L.o_hole);
+    if (in.sdf) {
+        p.values = nullptr;
+        p.vec4 = 0;
+        p.csg = (const iso::CsgProgram *) in.sdf;
+    }
     out.ctask0 = ctask0;
     out.blk0 = blk0;
     out.tile0 = tile0;
+    out.gtask0 = gtask0;
     out.mode = p.vec4 ? (p.Z % 64 == 0 ? 0 : 1) : 2;
 }
 
@@ -584,9 +596,9 @@ int batch_upload(int n, const isoext_mc_item *items, char *ws, const BatchLayout
         h->tile_count = (int *) (ws + B.o_tcnt);
     }
     iso::McBatchItem *it = (iso::McBatchItem *) (host + sizeof(iso::McBatchHdr));
-    int64_t w0 = 0, b0 = 0;
+    int64_t w0 = 0, b0 = 0, g0 = 0;
     for (int i = 0; i < n; ++i) {
-        fill_item(items[i], B.it[i], ws, w0, b0, B.it[i].tile0, it[i]);
+        fill_item(items[i], B.it[i], ws, w0, b0, B.it[i].tile0, g0, it[i]);
         it[i].p.nzc = B.it[i].nzc;
         if (listed) {
             iso::McParams &p = it[i].p;
@@ -612,6 +624,7 @@ int batch_upload(int n, const isoext_mc_item *items, char *ws, const BatchLayout
         }
         w0 += B.it[i].nctask;
         b0 += B.it[i].nblk;
+        g0 += B.it[i].ngtask;
     }
     int32_t *order = (int32_t *) (it + n);
     for (int k = 0; k < n; ++k) order[k] = B.order[k];
@@ -634,7 +647,9 @@ int batch_check(int n, const isoext_mc_item *items, int cfg) {
             return fail(ISOEXT_E_ARG, "Unknown method: %d", items[i].method);
         for (int a = 0; a < 3; ++a)
             if (items[i].shape[a] < 1) return fail(ISOEXT_E_ARG, "grid shape must be positive");
-        if (!items[i].values) return fail(ISOEXT_E_ARG, "values is null");
+        if (!items[i].values && !items[i].sdf) return fail(ISOEXT_E_ARG, "values is null");
+        if (items[i].sdf && ((uintptr_t) items[i].sdf & 15))
+            return fail(ISOEXT_E_ARG, "item %d: SDF program must be 16-byte aligned", i);
         AABB_CHECK(items[i].aabb_min, items[i].aabb_max);
     }
     return ISOEXT_OK;
@@ -855,7 +870,8 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
                          int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream,
-                         const float *spheres = nullptr, int nsph = 0, const iso::FixupScratch *na = nullptr);
+                         const float *spheres = nullptr, int nsph = 0, const iso::FixupScratch *na = nullptr,
+                         const iso::CsgProgram *csg = nullptr);
 
 int isoext_mc_spheres(const float *spheres, int nsph, const int64_t shape[3], const float aabb_min[3],
                       const float aabb_max[3], float level, int method, int cfg, void *workspace,
@@ -864,6 +880,58 @@ int isoext_mc_spheres(const float *spheres, int nsph, const int64_t shape[3], co
     if (!spheres) return fail(ISOEXT_E_ARG, "spheres is null");
     return mc_dense_impl(nullptr, shape, aabb_min, aabb_max, level, method, cfg, workspace, workspace_bytes,
                          special_cap, out_v, cap_v, out_f, cap_f, nullptr, counts, stream, spheres, nsph);
+}
+
+int isoext_mc_sdf(const void *program, const int64_t shape[3], const float aabb_min[3], const float aabb_max[3],
+                  float level, int method, int cfg, void *workspace, int64_t workspace_bytes, int64_t special_cap,
+                  float *out_v, int64_t cap_v, int32_t *out_f, int64_t cap_f, int64_t counts[6], void *stream) {
+    if (!program) return fail(ISOEXT_E_ARG, "program is null");
+    return mc_dense_impl(nullptr, shape, aabb_min, aabb_max, level, method, cfg, workspace, workspace_bytes,
+                         special_cap, out_v, cap_v, out_f, cap_f, nullptr, counts, stream, nullptr, 0, nullptr,
+                         (const iso::CsgProgram *) program);
+}
+
+// ---- SDF programs --------------------------------------------------------------------
+
+int64_t isoext_sdf_program_bytes(void) { return (int64_t) sizeof(iso::CsgProgram); }
+
+int isoext_sdf_upload(const int32_t *ops, int32_t n_ops, const float *params, int32_t n_params, void *program,
+                      void *stream) {
+    if (!program) return fail(ISOEXT_E_ARG, "program is null");
+    if ((uintptr_t) program & 15) return fail(ISOEXT_E_ARG, "program must be 16-byte aligned");
+    // validate first (host only), then stage in pinned memory for the copy
+    thread_local std::unique_ptr<iso::CsgProgram> built(new iso::CsgProgram);
+    if (const char *err = iso::csg_build(ops, n_ops, params, n_params, *built))
+        return fail(ISOEXT_E_ARG, "%s", err);
+    iso::CsgProgram *h = (iso::CsgProgram *) g_pinned_prog.get(sizeof(iso::CsgProgram));
+    if (!h) return fail(ISOEXT_E_DEVICE, "pinned staging allocation failed");
+    std::memcpy(h, built.get(), sizeof(iso::CsgProgram));
+    hipStream_t st = (hipStream_t) stream;
+    HIP_CHECK(hipMemcpyAsync(program, h, sizeof(iso::CsgProgram), hipMemcpyHostToDevice, st), "upload program");
+    HIP_CHECK(hipStreamSynchronize(st), "upload program");   // the staging buffer is reused by the next call
+    return ISOEXT_OK;
+}
+
+int isoext_sdf_grid(const void *program, const int64_t shape[3], const float aabb_min[3], const float aabb_max[3],
+                    float *out, void *stream) {
+    if (!program || !out) return fail(ISOEXT_E_ARG, "program or out is null");
+    for (int a = 0; a < 3; ++a)
+        if (shape[a] < 1) return fail(ISOEXT_E_ARG, "grid shape must be positive");
+    AABB_CHECK(aabb_min, aabb_max);
+    HIP_CHECK(iso::sdf_grid_launch((const iso::CsgProgram *) program, shape, aabb_min, aabb_max, out,
+                                   (hipStream_t) stream),
+              "SDF grid");
+    return ISOEXT_OK;
+}
+
+int isoext_sdf_points(const void *program, const float *points, int64_t n, float *values, float *normals,
+                      void *stream) {
+    if (!program) return fail(ISOEXT_E_ARG, "program is null");
+    if (n < 0 || (n > 0 && !points)) return fail(ISOEXT_E_ARG, "points is null");
+    HIP_CHECK(iso::sdf_points_launch((const iso::CsgProgram *) program, points, n, values, normals,
+                                     (hipStream_t) stream),
+              "SDF points");
+    return ISOEXT_OK;
 }
 
 int isoext_mc_dense(const float *values, const int64_t shape[3], const float aabb_min[3],
@@ -888,7 +956,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
                          int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream,
-                         const float *spheres, int nsph, const iso::FixupScratch *na) {
+                         const float *spheres, int nsph, const iso::FixupScratch *na, const iso::CsgProgram *csg) {
     for (int i = 0; i < 6; ++i) counts[i] = 0;
     if (method != 0 && method != 1) return fail(ISOEXT_E_ARG, "Unknown method: %d", method);
     AABB_CHECK(aabb_min, aabb_max);
@@ -900,7 +968,8 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
     if ((int64_t) L.total > workspace_bytes)
         return fail(ISOEXT_E_WORKSPACE, "workspace too small: %lld < %lld bytes",
                     (long long) workspace_bytes, (long long) L.total);
-    if (!values && !spheres) return fail(ISOEXT_E_ARG, "values is null");
+    if (!values && !spheres && !csg) return fail(ISOEXT_E_ARG, "values is null");
+    if (csg && ((uintptr_t) csg & 15)) return fail(ISOEXT_E_ARG, "SDF program must be 16-byte aligned");
     if (spheres && (nsph < 1 || nsph > iso::kMaxFusedSpheres))
         return fail(ISOEXT_E_ARG, "sphere count %d outside [1, %d]", nsph, iso::kMaxFusedSpheres);
     if (spheres && ((uintptr_t) spheres & 15)) return fail(ISOEXT_E_ARG, "spheres must be 16-byte aligned");
@@ -974,10 +1043,14 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
         p.vec4 = 0;
         p.spheres = (const float4 *) spheres;
         p.nsph = nsph;
+    } else if (csg) {
+        p.values = nullptr;
+        p.vec4 = 0;
+        p.csg = csg;
     }
     hipEvent_t *ev = (g_timing && !na) ? timing_events() : nullptr;
     if (g_timing && !na && !ev) return fail(ISOEXT_E_DEVICE, "timing events unavailable");
-    const bool read_once = L.stream && !spheres && L.NR <= (int64_t) INT32_MAX && shape[2] < (int64_t(1) << 26);
+    const bool read_once = L.stream && !spheres && !csg && L.NR <= (int64_t) INT32_MAX && shape[2] < (int64_t(1) << 26);
     if (read_once) {
         p.NRX = L.NRX;
         p.NRY = L.NRY;
@@ -1624,6 +1697,8 @@ int isoext_dc_batch_run(int n, const isoext_mc_item *items, const int64_t *count
                        {items[i].aabb_min[0], items[i].aabb_min[1], items[i].aabb_min[2]},
                        {items[i].aabb_max[0], items[i].aabb_max[1], items[i].aabb_max[2]}};
         g[i] = to_desc(&gi);
+        g[i].csg = (const iso::CsgProgram *) items[i].sdf;
+        g[i].csg_normals = items[i].sdf ? items[i].sdf_normals : 0;
     }
     const float cond_fp64 = std::min(1e6f, 0.5f / std::max(svd_tol, 1e-30f));
     std::vector<int64_t> nv(n), nq(n);

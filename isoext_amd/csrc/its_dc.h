@@ -7,12 +7,19 @@
 
 namespace iso {
 
+struct CsgProgram;
+
 struct GridDesc {
     const float *values;         // dense (X, Y, Z) or sparse (n, 8)
     const int64_t *sparse_ids;   // sorted dense cell ids of the sparse slots, or nullptr
     int64_t n_sparse;
     int64_t X, Y, Z;             // points per axis
     float amin[3], amax[3];
+    // dense grids in batched dual contouring: the field given by an SDF
+    // program (device form, csg.h) instead of values; csg_normals = 1 takes
+    // the program's analytic unit gradients as normals (else trilinear)
+    const CsgProgram *csg = nullptr;
+    int csg_normals = 0;
 };
 
 struct ItsOut {

@@ -17,8 +17,11 @@
 // values[8 s ..]); `sparse` selects the source.  Counting and emission use a
 // reduce-then-scan over fixed blocks of cells so the output is ordered.
 
+#include <type_traits>
+
 #include "common.h"
 #include "its_dc.h"
+#include "csg.h"
 
 namespace iso {
 
@@ -32,6 +35,8 @@ struct CellSrc {
     int64_t n;            // number of cells (dense: (X-1)(Y-1)(Z-1); sparse: slots)
     int64_t X, Y, Z;
     float amin[3], amax[3];
+    const CsgProgram *csg;   // dense grid-mode dual contouring: program field (values unused)
+    int csg_normals;         // 1: analytic program normals
 };
 
 __device__ __forceinline__ int64_t cell_dense_id(const CellSrc &s, int64_t i) {
@@ -407,6 +412,20 @@ struct GridEdges {
     }
 };
 
+// Entries of one cell of a program field: GridEdges' points with the
+// program's analytic unit gradient at each point as its normal (the
+// normals get_sdf_normal gives at the intersection points,
+// src/isoext/sdf.py:39-50).
+struct CsgEdges : GridEdges {
+    const CsgProgram *prog;
+    __device__ __forceinline__ void point_normal(int i, float p[3], float nv[3]) const {
+        point(i, p);
+        float g[3];
+        csg_value_grad<false>(*prog, p[0], p[1], p[2], g);
+        csg_normalize(g, nv);
+    }
+};
+
 // The QEF solve of one cell over its n >= 1 entries: FP32, centred at the
 // mass point and scaled by 1/h, one FP64-residual refinement step, FP64
 // absolute re-solve when ill-conditioned; clipped to the cell (fix_dual_v_op,
@@ -566,6 +585,10 @@ __global__ void k_dc_qef(const ItsIn in, const CellSrc s1, float reg, float svd_
 // Writes dual_v, used, bnd, qn as k_dc_qef with dense_used, and qmask[a]:
 // bit b (b = 0, 1, 2 for canonical edges 0, 3, 8) when the edge emits a quad,
 // bit b + 3 its orientation (value at the lower corner <= value at the upper).
+// PROG: the cells of grids given by an SDF program (corner values evaluated
+// at the grid coordinates, as the stored grid holds them; normals trilinear
+// or analytic); the other variant takes the stored grids' cells.
+template <bool PROG>
 __global__ void k_dc_qef_grid(const int64_t *cell_dense, int64_t ncells, const CellSrc s1, float level, float reg,
                               float svd_tol, float cond_fp64, float *dual_v, unsigned long long *n_fp64,
                               uint8_t *used, uint8_t *bnd, uint8_t *qn, uint8_t *qmask, DcBatch bt) {
@@ -578,11 +601,24 @@ __global__ void k_dc_qef_grid(const int64_t *cell_dense, int64_t ncells, const C
     if (a >= ncells) return;
     const int item = bt.items ? batch_item_of(bt, starts, a) : 0;
     const CellSrc &s = bt.items ? bt.items[item].s : s1;
+    if ((s.csg != nullptr) != PROG) return;
     if (bt.items) level = bt.items[item].level;
     int64_t x, y, z;
     cell_xyz(s, cell_dense[a], x, y, z);
-    GridEdges en;
-    {
+    std::conditional_t<PROG, CsgEdges, GridEdges> en;
+    en.lo[0] = grid_coord(x, s.X, s.amin[0], s.amax[0]);
+    en.lo[1] = grid_coord(y, s.Y, s.amin[1], s.amax[1]);
+    en.lo[2] = grid_coord(z, s.Z, s.amin[2], s.amax[2]);
+    en.hi[0] = grid_coord(x + 1, s.X, s.amin[0], s.amax[0]);
+    en.hi[1] = grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]);
+    en.hi[2] = grid_coord(z + 1, s.Z, s.amin[2], s.amax[2]);
+    if constexpr (PROG) {
+        en.prog = s.csg;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            en.cv[q] = csg_value<false>(*s.csg, (q >> 2) ? en.hi[0] : en.lo[0], ((q >> 1) & 1) ? en.hi[1] : en.lo[1],
+                                        (q & 1) ? en.hi[2] : en.lo[2]);
+    } else {
         const float *r00 = s.values + (x * s.Y + y) * s.Z + z;
         const float *r01 = r00 + s.Z;
         const float *r10 = r00 + s.Y * s.Z;
@@ -596,12 +632,6 @@ __global__ void k_dc_qef_grid(const int64_t *cell_dense, int64_t ncells, const C
         en.cv[6] = r11[0];
         en.cv[7] = r11[1];
     }
-    en.lo[0] = grid_coord(x, s.X, s.amin[0], s.amax[0]);
-    en.lo[1] = grid_coord(y, s.Y, s.amin[1], s.amax[1]);
-    en.lo[2] = grid_coord(z, s.Z, s.amin[2], s.amax[2]);
-    en.hi[0] = grid_coord(x + 1, s.X, s.amin[0], s.amax[0]);
-    en.hi[1] = grid_coord(y + 1, s.Y, s.amin[1], s.amax[1]);
-    en.hi[2] = grid_coord(z + 1, s.Z, s.amin[2], s.amax[2]);
     en.level = level;
     const int st = crossing12(case_bits(en.cv, level));
     en.n = 0;
@@ -620,8 +650,12 @@ __global__ void k_dc_qef_grid(const int64_t *cell_dense, int64_t ncells, const C
     }
     float out[3];
     bool on_box = false;
-    if (en.n > 0) qef_cell(en, en.lo, en.hi, reg, svd_tol, cond_fp64, n_fp64, out, on_box);
-    else out[0] = out[1] = out[2] = 0.0f;   // not reached: active cells have a crossing edge
+    if (en.n > 0) {
+        if (PROG && s.csg_normals) qef_cell(en, en.lo, en.hi, reg, svd_tol, cond_fp64, n_fp64, out, on_box);
+        else qef_cell(static_cast<const GridEdges &>(en), en.lo, en.hi, reg, svd_tol, cond_fp64, n_fp64, out, on_box);
+    } else {
+        out[0] = out[1] = out[2] = 0.0f;   // not reached: active cells have a crossing edge
+    }
     for (int k = 0; k < 3; ++k) dual_v[3 * a + k] = out[k];
     used[a] = u;
     bnd[a] = on_box ? 1 : 0;
@@ -1280,6 +1314,8 @@ CellSrc make_src(const GridDesc &g) {
         s.amin[a] = g.amin[a];
         s.amax[a] = g.amax[a];
     }
+    s.csg = g.csg;
+    s.csg_normals = g.csg_normals;
     return s;
 }
 
@@ -1401,11 +1437,16 @@ static hipError_t dc_launch(const CellSrc &s, const ItsIn &in, float reg, float 
 // offsets, quads per cell.
 static hipError_t dc_launch_grid(const CellSrc &s, const int64_t *cell_dense, int64_t ncells, const uint64_t *amask,
                                  const int32_t *apre, float level, float reg, float svd_tol, float cond_fp64,
-                                 DcScratch &w, const DcBatch &bt, hipStream_t st) {
+                                 DcScratch &w, const DcBatch &bt, hipStream_t st, bool has_grid = true,
+                                 bool has_prog = false) {
     const unsigned nbv = nblocks(ncells);
     hipLaunchKernelGGL(k_fill_i64, dim3(1), dim3(64), 0, st, (int64_t *) w.n_fp64, (int64_t) 1, (int64_t) 0);
-    hipLaunchKernelGGL(k_dc_qef_grid, dim3(nbv), dim3(kBlock), 0, st, cell_dense, ncells, s, level, reg, svd_tol,
-                       cond_fp64, w.dual_v, w.n_fp64, w.used, w.bnd, w.qn, w.qmask, bt);
+    if (has_grid)
+        hipLaunchKernelGGL(k_dc_qef_grid<false>, dim3(nbv), dim3(kBlock), 0, st, cell_dense, ncells, s, level, reg,
+                           svd_tol, cond_fp64, w.dual_v, w.n_fp64, w.used, w.bnd, w.qn, w.qmask, bt);
+    if (has_prog)
+        hipLaunchKernelGGL(k_dc_qef_grid<true>, dim3(nbv), dim3(kBlock), 0, st, cell_dense, ncells, s, level, reg,
+                           svd_tol, cond_fp64, w.dual_v, w.n_fp64, w.used, w.bnd, w.qn, w.qmask, bt);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.hslots, w.hmask + 1, -1);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(w.hmask + 1)), dim3(kBlock), 0, st, w.slot_min, w.hmask + 1,
                        INT32_MAX);
@@ -1485,7 +1526,7 @@ hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const ui
     }
     DcItem *hi = (DcItem *) ps.p;
     int64_t key_base = 0;
-    bool tables = amask && apre;
+    bool tables = amask && apre, has_grid = false, has_prog = false;
     for (int i = 0; i < n; ++i) {
         hi[i].s = make_src(g[i]);
         hi[i].cell0 = cell0[i];
@@ -1496,14 +1537,17 @@ hipError_t dc_run_batch(int n, const GridDesc *g, const int64_t *cell0, const ui
         hi[i].W = (g[i].Z + 63) / 64;
         hi[i].level = levels ? levels[i] : 0.0f;
         if (g[i].sparse_ids) tables = false;
+        if (g[i].csg) has_prog = true;
+        else has_grid = true;
     }
+    if (has_prog && !levels) return hipErrorInvalidValue;   // program fields have no intersection set
     if (levels && !tables) return hipErrorInvalidValue;   // the grid path needs the tables
     int64_t *rb = (int64_t *) (ps.p + ((up + 255) & ~size_t(255)));
     ISO_TRY(hipMemcpyAsync(d.items, hi, up, hipMemcpyHostToDevice, st));
     const DcBatch bt{(const DcItem *) d.items, n, tables, d.cell_item, d.vbase, d.qbase};
     if (levels)
         ISO_TRY(dc_launch_grid(hi[0].s, in.cell_dense, in.ncells, nullptr, nullptr, 0.0f, reg, svd_tol, cond_fp64, w,
-                               bt, st));
+                               bt, st, has_grid, has_prog));
     else
         ISO_TRY(dc_launch(hi[0].s, in, reg, svd_tol, cond_fp64, w, bt, st));
     ISO_TRY(hipMemcpyAsync(rb, d.vbase, sizeof(int64_t) * (size_t) (n + 1), hipMemcpyDeviceToHost, st));

@@ -8,6 +8,8 @@
 
 namespace iso {
 
+struct CsgProgram;
+
 enum : int {
     ISO_ERR_CAPACITY = 1,    // an output buffer was too small (counts are still exact)
     ISO_ERR_SPIN = 2,        // a bounded inter-workgroup wait gave up
@@ -104,6 +106,9 @@ struct McParams {
     // needed with the sphere-union generator's formula
     const float4 *spheres;
     int nsph;
+    // fused field given by an SDF program (device form, csg.h); values
+    // unused when set
+    const CsgProgram *csg;
     // chunk index -> (row, chunk) -> (x, y) and chunk -> z-segment by
     // multiply-high, valid when narrow (X * Y * W < 2^32); set by
     // mc_params_divisors once W, Y and nzc are final
@@ -158,6 +163,12 @@ constexpr int kStrScanChunk = 256; // waves per first-level scan block
 // cell rows x one word (tile configuration 6).
 constexpr int kStrTileConfig = 6;
 constexpr int kMaxFusedSpheres = 256;
+// Classify task of a program field: 4 x 4 x 4 blocks of 4 x 4 point rows x
+// one 64-point word (16 x 16 point rows x 256 points).
+__host__ __device__ inline int64_t csg_classify_tasks(int64_t X, int64_t Y, int64_t W) {
+    const int64_t nbx = (X + 3) / 4, nby = (Y + 3) / 4;
+    return ((nbx + 3) / 4) * ((nby + 3) / 4) * ((W + 3) / 4);
+}
 
 // Fill McParams' invariant-divisor fields from X, Y, W and nzc.
 void mc_params_divisors(McParams &p);
@@ -176,11 +187,14 @@ struct McBatchItem {
     int64_t ctask0;         // first flat classify task
     int64_t blk0;           // first count block
     int64_t tile0;          // first emit tile
+    int64_t gtask0;         // first program classify task (items with an SDF program)
     int mode;               // classify loads: 0 Z%64 == 0 float4, 1 float4, 2 scalar
 };
 
 struct McBatchHdr {
     int64_t n, nctask, nblk, ntiles;   // items, flat classify tasks, count blocks, emit tiles
+    int64_t ngtask = 0;                // program classify tasks (items with an SDF program)
+    int n_prog = 0;                    // items with an SDF program
     // active-tile list of the batch (null: every tile runs): one flag per
     // batch tile and one list length per tile configuration
     int *tile_flag = nullptr;

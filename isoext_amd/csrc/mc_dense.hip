@@ -39,6 +39,7 @@
 #include "common.h"
 #include "its_dc.h"
 #include "mc_dense.h"
+#include "csg.h"
 #include "mc_luts.h"
 
 namespace iso {
@@ -712,12 +713,118 @@ __global__ void __launch_bounds__(256) k_classify_spheres(McParams p) {
     }
 }
 
+// ---- fused SDF program field ---------------------------------------------------------
+// Inside masks of v(p) = program(p) (csg.h) with exact sign culling: the
+// field is 1-Lipschitz (programs without smooth min), so a box of grid
+// points whose centre value is farther from the level than the box radius
+// (plus a margin far above the FP32 evaluation error) has one sign.  A task
+// holds 64 blocks of 4 x 4 point rows x one word; each lane tests one block
+// at its centre.  An undecided block tests its 64 columns of 4 x 4 points
+// (one z per lane), and the points of the columns still undecided are
+// evaluated one per lane, 64 at a time, and assembled into the rows' words.
+// Every evaluated value is the stored grid's (same program, same grid
+// coordinates), and decided points have the sign their value would give.
+__device__ void classify_csg_task(const McParams &p, const CsgProgram &P, int64_t t, double cscale) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nbx = (p.X + 3) / 4, nby = (p.Y + 3) / 4;
+    const int64_t tx4 = (nbx + 3) / 4, ty4 = (nby + 3) / 4, tc4 = (p.W + 3) / 4;
+    const int64_t gc = t % tc4, r0 = t / tc4, gy = r0 % ty4, gx = r0 / ty4;
+    (void) tx4;
+    // this lane's block
+    const int64_t xa = 4 * (4 * gx + (lane >> 4)), ya = 4 * (4 * gy + ((lane >> 2) & 3)), c = 4 * gc + (lane & 3);
+    const int64_t za = 64 * c;
+    const bool valid = xa < p.X && ya < p.Y && c < p.W;
+    const int nxr = valid ? (int) min<int64_t>(4, p.X - xa) : 1, nyr = valid ? (int) min<int64_t>(4, p.Y - ya) : 1;
+    const int nzp = valid ? (int) min<int64_t>(64, p.Z - za) : 1;
+    int sign = 1;
+    {
+        const int64_t xs = valid ? xa : 0, ys = valid ? ya : 0, zs = valid ? za : 0;
+        const float x0 = grid_coord(xs, p.X, p.amin[0], p.amax[0]), x1 = grid_coord(xs + nxr - 1, p.X, p.amin[0], p.amax[0]);
+        const float y0 = grid_coord(ys, p.Y, p.amin[1], p.amax[1]), y1 = grid_coord(ys + nyr - 1, p.Y, p.amin[1], p.amax[1]);
+        const float z0 = grid_coord(zs, p.Z, p.amin[2], p.amax[2]), z1 = grid_coord(zs + nzp - 1, p.Z, p.amin[2], p.amax[2]);
+        const float cx = 0.5f * (x0 + x1), cy = 0.5f * (y0 + y1), cz = 0.5f * (z0 + z1);
+        const double ex = fmax(fabs((double) cx - x0), fabs((double) x1 - cx));
+        const double ey = fmax(fabs((double) cy - y0), fabs((double) y1 - cy));
+        const double ez = fmax(fabs((double) cz - z0), fabs((double) z1 - cz));
+        const float vc = csg_value<true>(P, cx, cy, cz);
+        sign = valid ? csg_box_sign(P, vc, sqrt(ex * ex + ey * ey + ez * ez), cscale, p.level) : 1;
+    }
+    if (valid && sign != 0) {
+        const uint64_t w = sign < 0 ? low_bits(nzp) : 0ull;
+        for (int r = 0; r < 16; ++r)
+            if ((r >> 2) < nxr && (r & 3) < nyr) p.bits[((xa + (r >> 2)) * p.Y + (ya + (r & 3))) * p.W + c] = w;
+    }
+    uint64_t und = __ballot(valid && sign == 0);
+    while (und) {
+        const int b = __ffsll((unsigned long long) und) - 1;
+        und &= und - 1;
+        const int64_t bxa = 4 * (4 * gx + (b >> 4)), bya = 4 * (4 * gy + ((b >> 2) & 3)), bc = 4 * gc + (b & 3);
+        const int64_t bza = 64 * bc;
+        const int bnx = (int) min<int64_t>(4, p.X - bxa), bny = (int) min<int64_t>(4, p.Y - bya);
+        const int bnz = (int) min<int64_t>(64, p.Z - bza);
+        // columns: 4 x 4 rows at one z per lane
+        const float x0 = grid_coord(bxa, p.X, p.amin[0], p.amax[0]), x1 = grid_coord(bxa + bnx - 1, p.X, p.amin[0], p.amax[0]);
+        const float y0 = grid_coord(bya, p.Y, p.amin[1], p.amax[1]), y1 = grid_coord(bya + bny - 1, p.Y, p.amin[1], p.amax[1]);
+        const float cx = 0.5f * (x0 + x1), cy = 0.5f * (y0 + y1);
+        const double ex = fmax(fabs((double) cx - x0), fabs((double) x1 - cx));
+        const double ey = fmax(fabs((double) cy - y0), fabs((double) y1 - cy));
+        const bool zok = lane < bnz;
+        const float pz = grid_coord(bza + (zok ? lane : 0), p.Z, p.amin[2], p.amax[2]);
+        const float vs = csg_value<true>(P, cx, cy, pz);
+        const int s2 = zok ? csg_box_sign(P, vs, sqrt(ex * ex + ey * ey), cscale, p.level) : 1;
+        const uint64_t known_in = __ballot(zok && s2 < 0);
+        const uint64_t need = __ballot(zok && s2 == 0);
+        const int nn = __popcll(need);
+        uint64_t word = known_in;   // lane r < 16: the word of row r
+        for (int j0 = 0; j0 < 16 * nn; j0 += 64) {
+            const int j = j0 + lane, r = lane & 15;
+            const bool act = j < 16 * nn && (r >> 2) < bnx && (r & 3) < bny;
+            const int L = j < 16 * nn ? nth_set_bit(need, j >> 4) : 0;
+            const float px = grid_coord(bxa + (act ? (r >> 2) : 0), p.X, p.amin[0], p.amax[0]);
+            const float py = grid_coord(bya + (act ? (r & 3) : 0), p.Y, p.amin[1], p.amax[1]);
+            const float qz = grid_coord(bza + L, p.Z, p.amin[2], p.amax[2]);
+            const int in = act && inside(csg_value<true>(P, px, py, qz), p.level) ? 1 : 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int bq = __shfl(in, r + 16 * q), lq = __shfl(L, r + 16 * q);
+                if (bq) word |= 1ull << lq;
+            }
+        }
+        if (lane < 16 && (lane >> 2) < bnx && (lane & 3) < bny)
+            p.bits[((bxa + (lane >> 2)) * p.Y + (bya + (lane & 3))) * p.W + bc] = word & low_bits(bnz);
+    }
+}
+
+__device__ __forceinline__ double csg_coord_scale(const McParams &p) {
+    double m = 0.0;
+    for (int a = 0; a < 3; ++a) m = fmax(m, fmax(fabs((double) p.amin[a]), fabs((double) p.amax[a])));
+    return m;
+}
+
+// One wave per workgroup, tasks grid-strided.
+__global__ void __launch_bounds__(64) k_classify_csg(McParams p) {
+    __shared__ CsgProgram P;
+    csg_stage(P, p.csg);
+    __syncthreads();
+    const int64_t ntask = csg_classify_tasks(p.X, p.Y, p.W);
+    const double cs = csg_coord_scale(p);
+    for (int64_t t = blockIdx.x; t < ntask; t += gridDim.x) classify_csg_task(p, P, t, cs);
+}
+
 // LDS of the fused emit: per 64-point z chunk of the tile, the spheres that
 // can be the minimum somewhere in the chunk's box (about 2 KB in all).
-template <bool F, int NZC>
+// Field sources of the emit kernels: a stored grid, the fused sphere union,
+// a fused SDF program.
+enum : int { kSrcGrid = 0, kSrcSpheres = 1, kSrcProgram = 2 };
+
+template <int SRC, int NZC>
 struct FusedLds {};
 template <int NZC>
-struct FusedLds<true, NZC> {
+struct FusedLds<kSrcProgram, NZC> {
+    CsgProgram prog;
+};
+template <int NZC>
+struct FusedLds<kSrcSpheres, NZC> {
     static constexpr int CAP = 128 / NZC > 8 ? 128 / NZC : 8;   // spheres per chunk
     float4 sph[NZC][CAP];
     int n[NZC];   // culled spheres; > CAP: evaluate all of p.spheres instead
@@ -862,11 +969,11 @@ __device__ __forceinline__ bool edge_maybe_candidate(const CellCtx &cc, float ep
     return !(fabsf(cc.level - cc.v[ca]) > d && fabsf(cc.v[cb] - cc.level) > d);
 }
 
-template <int TX, int TY, int NZC, int NT, bool FUSED = false>
+template <int TX, int TY, int NZC, int NT, int SRC = kSrcGrid>
 __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int spseg) {
     using L = EmitLds<TX, TY, NZC, NT>;
     __shared__ L s;
-    __shared__ FusedLds<FUSED, NZC> fl;
+    __shared__ FusedLds<SRC, NZC> fl;
     const int tid = threadIdx.x;
     if (*(volatile int *) p.err & ISO_ERR_TOO_LARGE) return;
 
@@ -920,8 +1027,9 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             s.cz[c - TX - TY - 2] = grid_coord(z0 + (c - TX - TY - 2), p.Z, p.amin[2], p.amax[2]);
         }
     }
-    if constexpr (FUSED) {
-        using F = FusedLds<true, NZC>;
+    if constexpr (SRC == kSrcProgram) csg_stage(fl.prog, p.csg);
+    if constexpr (SRC == kSrcSpheres) {
+        using F = FusedLds<kSrcSpheres, NZC>;
         for (int j = 0; j < NZC; ++j) {
             const int64_t za = min(z0 + 64 * j, p.Z - 1);
             cull_spheres<NT, F::CAP>(p, point_box(p, x0, min(x0 + TX, p.X - 1), y0, min(y0 + TY, p.Y - 1), za,
@@ -998,7 +1106,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             u = s.unit[tid];
             k = nth_set_bit(s.amask[u], jj - (int) s.na[u]);
             const int jc = u % NZC, r = u / NZC, cyy = r % TY, cxx = r / TY;
-            if constexpr (!FUSED) {
+            if constexpr (SRC == kSrcGrid) {
                 const float *r00 = p.values + ((x0 + cxx) * p.Y + (y0 + cyy)) * p.Z + (z0 + 64 * jc + k);
                 const float *r01 = r00 + p.Z;
                 const float *r10 = r00 + p.Y * p.Z;
@@ -1045,10 +1153,15 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             const int zl = 64 * jc + k;
             const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + zl;
             CellCtx cc;
-            if constexpr (FUSED) {
+            if constexpr (SRC == kSrcProgram) {
                 const float xs[2] = {s.cx[cx], s.cx[cx + 1]}, ys[2] = {s.cy[cy], s.cy[cy + 1]};
                 const float zs[2] = {s.cz[zl], s.cz[zl + 1]};
-                using F = FusedLds<true, NZC>;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) cc.v[q] = csg_value<true>(fl.prog, xs[q >> 2], ys[(q >> 1) & 1], zs[q & 1]);
+            } else if constexpr (SRC == kSrcSpheres) {
+                const float xs[2] = {s.cx[cx], s.cx[cx + 1]}, ys[2] = {s.cy[cy], s.cy[cy + 1]};
+                const float zs[2] = {s.cz[zl], s.cz[zl + 1]};
+                using F = FusedLds<kSrcSpheres, NZC>;
                 const int ns = fl.n[jc];
                 float best[8];
 #pragma unroll
@@ -1260,7 +1373,13 @@ __global__ void __launch_bounds__(NT) k_emit(McParams p) {
 template <int TX, int TY, int NZC, int NT>
 __global__ void __launch_bounds__(NT) k_emit_spheres(McParams p) {
     const int64_t tile = emit_block_tile(p);
-    if (tile >= 0) emit_tile<TX, TY, NZC, NT, true>(p, tile, (int) (blockIdx.x % p.seg.nseg));
+    if (tile >= 0) emit_tile<TX, TY, NZC, NT, kSrcSpheres>(p, tile, (int) (blockIdx.x % p.seg.nseg));
+}
+
+template <int TX, int TY, int NZC, int NT>
+__global__ void __launch_bounds__(NT) k_emit_program(McParams p) {
+    const int64_t tile = emit_block_tile(p);
+    if (tile >= 0) emit_tile<TX, TY, NZC, NT, kSrcProgram>(p, tile, (int) (blockIdx.x % p.seg.nseg));
 }
 
 // Reset of the per-call counters (and the active-tile flags when listed);
@@ -1435,6 +1554,33 @@ __global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *__res
     }
 }
 
+// Program items of a batch: tasks over the items' program task ranges
+// (gtask0 is non-decreasing in item order; items without a program own no
+// task); the program is staged again when the item changes.
+__global__ void __launch_bounds__(64) k_classify_csg_batch(const McBatchItem *__restrict__ items,
+                                                           const McBatchHdr *__restrict__ hdr) {
+    __shared__ CsgProgram P;
+    __shared__ int64_t starts[kMaxBatch];
+    const int n = (int) hdr->n;
+    for (int i = threadIdx.x; i < n; i += 64) starts[i] = items[i].gtask0;
+    __syncthreads();
+    int cur = -1;
+    double cs = 0.0;
+    for (int64_t t = blockIdx.x; t < hdr->ngtask; t += gridDim.x) {
+        // program task ranges are consecutive in item order (other items own none)
+        const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, t));
+        const McParams &p = items[i].p;
+        if (i != cur) {
+            __syncthreads();
+            csg_stage(P, p.csg);
+            __syncthreads();
+            cur = i;
+            cs = csg_coord_scale(p);
+        }
+        classify_csg_task(p, P, t - items[i].gtask0, cs);
+    }
+}
+
 // One workgroup per count block (grid = the batch's block count); the
 // item comes from a search over the item starts staged in LDS.
 template <int NT>
@@ -1461,7 +1607,9 @@ __global__ void __launch_bounds__(1024) k_scan_top_batch(const McBatchItem *__re
 
 // One workgroup per emit tile of the items using this configuration (tiles
 // [t_begin, t_begin + grid)); items are found in tile order.
-template <int TX, int TY, int NZC, int NT>
+// SRC kSrcGrid takes the stored-grid items, kSrcProgram the program items;
+// each skips the other kind's tiles.
+template <int TX, int TY, int NZC, int NT, int SRC>
 __global__ void __launch_bounds__(NT) k_emit_batch(const McBatchItem *__restrict__ items,
                                                    const int32_t *__restrict__ order,
                                                    const McBatchHdr *__restrict__ hdr, int64_t t_begin,
@@ -1477,8 +1625,9 @@ __global__ void __launch_bounds__(NT) k_emit_batch(const McBatchItem *__restrict
     for (int k = threadIdx.x; k < n; k += NT) starts[k] = items[order[k]].tile0;
     __syncthreads();
     const int i = order[__builtin_amdgcn_readfirstlane(find_start(starts, n, gt))];
+    if ((items[i].p.csg != nullptr) != (SRC == kSrcProgram)) return;
     const int64_t t = gt - items[i].tile0;
-    emit_tile<TX, TY, NZC, NT>(items[i].p, t, (int) (t % items[i].p.seg.nseg));
+    emit_tile<TX, TY, NZC, NT, SRC>(items[i].p, t, (int) (t % items[i].p.seg.nseg));
 }
 
 // grid (segments, items)
@@ -1802,7 +1951,11 @@ hipError_t mc_upload_luts() {
 // words, the flat kernel otherwise.
 static void launch_classify(const McParams &p, hipStream_t stream) {
     const int64_t nwords = p.X * p.Y * p.W;
-    if (p.nsph > 0) {
+    if (p.csg) {
+        const int64_t ntask = csg_classify_tasks(p.X, p.Y, p.W);
+        const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>(ntask, 16384));
+        hipLaunchKernelGGL(k_classify_csg, dim3(blocks), dim3(64), 0, stream, p);
+    } else if (p.nsph > 0) {
         const int64_t ntask = ((p.X + 3) / 4) * ((p.Y + 3) / 4) * p.W;
         const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((ntask + 3) / 4, 8192));
         hipLaunchKernelGGL(k_classify_spheres, dim3(blocks), dim3(256), 0, stream, p);
@@ -1865,7 +2018,9 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     switch (cfg) {
 #define X(id, tx, ty, nzc, nt)                                                                     \
     case id:                                                                                       \
-        if (p.nsph > 0)                                                                            \
+        if (p.csg)                                                                                 \
+            hipLaunchKernelGGL((k_emit_program<tx, ty, nzc, nt>), dim3(tiles), dim3(nt), 0, stream, p); \
+        else if (p.nsph > 0)                                                                       \
             hipLaunchKernelGGL((k_emit_spheres<tx, ty, nzc, nt>), dim3(tiles), dim3(nt), 0, stream, p); \
         else                                                                                       \
             hipLaunchKernelGGL((k_emit<tx, ty, nzc, nt>), dim3(tiles), dim3(nt), 0, stream, p);    \
@@ -1924,6 +2079,11 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     const unsigned cb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.nctask + 3) / 4, 8192));
     hipLaunchKernelGGL(k_classify_batch, dim3(cb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (h.ngtask > 0) {
+        const unsigned gb = (unsigned) std::min<int64_t>(h.ngtask, 16384);
+        hipLaunchKernelGGL(k_classify_csg_batch, dim3(gb), dim3(64), 0, stream, items, hdr);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     const unsigned nb = (unsigned) std::max<int64_t>(1, h.nblk);
     if (h.nblk < kCountWideBelow)
         hipLaunchKernelGGL(k_count_batch<1024>, dim3(nb), dim3(1024), 0, stream, items, hdr);
@@ -1973,9 +2133,16 @@ hipError_t mc_batch_emit_launch(const int64_t *tile_begin, const int64_t *tile_e
         switch (c) {
 #define X(id, tx, ty, nzc, nth)                                                                              \
     case id:                                                                                                 \
-        hipLaunchKernelGGL((k_emit_batch<tx, ty, nzc, nth>), dim3((unsigned) nt), dim3(nth), 0, stream, items, \
-                           order, hdr, tile_begin[c], tile_list ? tile_list + tile_begin[c] : nullptr,        \
-                           tile_list ? tile_count + c : nullptr);                                            \
+        if (h.n_prog < h.n)                                                                                  \
+            hipLaunchKernelGGL((k_emit_batch<tx, ty, nzc, nth, kSrcGrid>), dim3((unsigned) nt), dim3(nth), 0,  \
+                               stream, items, order, hdr, tile_begin[c],                                     \
+                               tile_list ? tile_list + tile_begin[c] : nullptr,                               \
+                               tile_list ? tile_count + c : nullptr);                                        \
+        if (h.n_prog > 0)                                                                                    \
+            hipLaunchKernelGGL((k_emit_batch<tx, ty, nzc, nth, kSrcProgram>), dim3((unsigned) nt), dim3(nth), 0, \
+                               stream, items, order, hdr, tile_begin[c],                                     \
+                               tile_list ? tile_list + tile_begin[c] : nullptr,                               \
+                               tile_list ? tile_count + c : nullptr);                                        \
         break;
             ISO_MC_CONFIGS(X)
 #undef X

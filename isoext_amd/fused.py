@@ -1,4 +1,4 @@
-"""Marching cubes of a sphere-union field that is never stored.
+"""Extraction from fields that are never stored: sphere unions and SDF programs.
 
 The reference's users build a grid with ``UniformGrid.set_values(sdf(p))``
 and then extract (the step before the path).  For the union of spheres --
@@ -9,6 +9,17 @@ emit kernel evaluates the corners of its active cells over the spheres that
 can be the minimum in its tile.  The mesh is identical (same vertices, same
 order) to ``marching_cubes`` on the grid that ``workloads.spheres_field``
 writes for the same spheres, without the 4 bytes per point written and read.
+
+``marching_cubes_sdf`` and ``dual_contouring_sdf`` do the same for any SDF
+tree of the reference's primitives and operators (src/isoext/sdf.py:70-221)
+compiled into a program (csg.py): classify evaluates the program at the
+centres of point blocks and decides whole blocks by the field's Lipschitz
+bound, then evaluates the points of the undecided ones; emit and the QEF
+pass evaluate the corners of their active cells.  The meshes equal
+``marching_cubes`` / ``dual_contouring`` on the grid ``SdfProgram.grid``
+writes.  Dual contouring can take the program's analytic normals (the
+normals ``get_sdf_normal`` gives at the intersection points) in place of
+trilinear ones.
 """
 
 from __future__ import annotations
@@ -16,6 +27,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from .csg import as_program
 from .mc import METHODS, mc_dense_raw
 
 
@@ -46,3 +58,37 @@ def marching_cubes_spheres(shape, centers, radii, level: float = 0.0, method: st
     if f.shape[0] == 0:
         return None, None
     return v, f
+
+
+def marching_cubes_sdf(sdf, shape, level: float = 0.0, method: str = "nagae", aabb_min=(-1.0, -1.0, -1.0),
+                       aabb_max=(1.0, 1.0, 1.0), device="cuda", cfg: int | None = None):
+    """Marching cubes of an SDF tree (or csg.SdfProgram) on the grid points
+    of (shape, aabb) without storing the grid.  Returns (vertices, faces) or
+    (None, None)."""
+    if method not in METHODS:
+        raise RuntimeError(f"Unknown method: {method}")
+    shape = tuple(int(s) for s in shape)
+    if len(shape) != 3:
+        raise ValueError("shape must have 3 entries")
+    prog = as_program(sdf, device)
+    v, f, _ = mc_dense_raw(None, shape, aabb_min, aabb_max, level, METHODS[method], cfg, program=prog)
+    if f.shape[0] == 0:
+        return None, None
+    return v, f
+
+
+def dual_contouring_sdf(sdf, shape, level: float = 0.0, reg: float = 1e-2, svd_tol: float = 1e-6,
+                        normals: str = "analytic", aabb_min=(-1.0, -1.0, -1.0), aabb_max=(1.0, 1.0, 1.0),
+                        device="cuda"):
+    """Dual contouring of an SDF tree (or csg.SdfProgram) without storing
+    the grid.  normals: "analytic" (the program's unit gradients at the
+    intersection points) or "trilinear" (from the corner values, as
+    dual_contouring on the stored grid).  Returns (vertices, faces) or
+    (None, None)."""
+    if normals not in ("analytic", "trilinear"):
+        raise ValueError('normals must be "analytic" or "trilinear"')
+    from .batch import GridJob, dc_batch
+    prog = as_program(sdf, device)
+    job = GridJob(None, float(level), 0, tuple(aabb_min), tuple(aabb_max), sdf=prog,
+                  shape=tuple(int(s) for s in shape), sdf_normals=normals == "analytic")
+    return dc_batch([job], reg=reg, svd_tol=svd_tol, ws_kind="dc_grid")[0]

@@ -52,16 +52,23 @@ _ws_bytes_cache: dict = {}
 
 
 def mc_dense_raw(values: torch.Tensor | None, shape, aabb_min, aabb_max, level: float, method: int,
-                 cfg: int | None = None, want_keys: bool = False, spheres: torch.Tensor | None = None):
+                 cfg: int | None = None, want_keys: bool = False, spheres: torch.Tensor | None = None,
+                 program=None):
     """Run dense marching cubes on a (X, Y, Z) float32 CUDA tensor, or, with
     `spheres` ((K, 4) float32 (cx, cy, cz, r) on the device; values None), on
-    the sphere-union field evaluated inside the kernels (never stored).
+    the sphere-union field evaluated inside the kernels (never stored), or,
+    with `program` (a csg.SdfProgram; values None), on the program's field.
 
     Returns (v, f, info) with v (V, 3) float32 and f (F, 3) int32 tensors
     (possibly zero rows) and an info dict (provisional counts, tile config;
     with want_keys, info["keys"] holds each vertex's grid edge).
     """
-    dev = values.device if values is not None else spheres.device
+    if program is not None:
+        from .csg import _decl as _csg_decl
+        _csg_decl()
+        dev = program.device
+    else:
+        dev = values.device if values is not None else spheres.device
     shape = tuple(int(s) for s in shape)
     if cfg is None:
         cfg = autotune.mc_config(shape)
@@ -84,7 +91,11 @@ def mc_dense_raw(values: torch.Tensor | None, shape, aabb_min, aabb_max, level: 
         out_f = torch.empty((cap_f, 3), dtype=torch.int32, device=dev)
         keys = torch.empty(cap_v, dtype=torch.int64, device=dev) if want_keys else None
         with _lib.on_device(dev):
-            if spheres is not None:
+            if program is not None:
+                rc = L.isoext_mc_sdf(program.ptr, shape_c, amin_c, amax_c, float(level), int(method), int(cfg),
+                                     ws.data_ptr(), ws.numel(), special_cap, out_v.data_ptr(), cap_v,
+                                     out_f.data_ptr(), cap_f, counts, stream)
+            elif spheres is not None:
                 rc = L.isoext_mc_spheres(spheres.data_ptr(), int(spheres.shape[0]), shape_c, amin_c, amax_c,
                                          float(level), int(method), int(cfg), ws.data_ptr(), ws.numel(), special_cap,
                                          out_v.data_ptr(), cap_v, out_f.data_ptr(), cap_f, counts, stream)

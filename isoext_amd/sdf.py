@@ -72,6 +72,17 @@ class CuboidSDF(SDF):
 
 
 @dataclass
+class CylinderSDF(SDF):
+    """Infinite cylinder around the z axis, |p_xy| - radius (not a class of
+    the reference; the field of the dual-contouring benchmark carves one)."""
+
+    radius: float
+
+    def __call__(self, p):
+        return p[..., :2].norm(dim=-1) - self.radius
+
+
+@dataclass
 class UnionOp(SDF):
     sdf_list: list
 

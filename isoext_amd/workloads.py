@@ -79,10 +79,25 @@ def multi_sphere(shape, n_spheres=64, seed=0, rmin=0.05, rmax=0.25, device="cuda
     return out
 
 
+def csg_box_cylinder_sdf(n_spheres=16, seed=0):
+    """The dual-contouring workload as an isoext.sdf tree: box (half-extent
+    0.6) minus a z-cylinder (r 0.3), union seeded spheres r in [0.05, 0.15]
+    (the same floats, in the same op order, as csg_eval)."""
+    from . import sdf as S
+    c, r = sphere_params(n_spheres, seed, 0.05, 0.15)
+    body = S.IntersectionOp([S.CuboidSDF([1.2, 1.2, 1.2]), S.NegationOp(S.CylinderSDF(0.3))])
+    return S.UnionOp([body] + [S.TranslationOp(S.SphereSDF(float(r[k])), [float(x) for x in c[k]])
+                               for k in range(n_spheres)])
+
+
 @torch.no_grad()
-def csg_box_cylinder(shape, n_spheres=16, seed=0, device="cuda"):
-    """Box (half-extent 0.6) minus a z-cylinder (r 0.3), union seeded spheres
-    r in [0.05, 0.15].  Returns the (X, Y, Z) field."""
+def csg_box_cylinder(shape, n_spheres=16, seed=0, device="cuda", program: bool = True):
+    """(X, Y, Z) field of the dual-contouring workload, written by the SDF
+    program evaluator (program=True; the grid the fused extractions
+    reproduce) or by the torch ops of csg_eval."""
+    if program:
+        from .csg import SdfProgram
+        return SdfProgram(csg_box_cylinder_sdf(n_spheres, seed), device).grid(shape)
     ax, ay, az = (axis_coords(n, device=device) for n in shape)
     p = torch.stack(torch.meshgrid(ax, ay, az, indexing="ij"), -1)
     return csg_eval(p, n_spheres, seed)

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_error_without_gpu():
     lib = ctypes.CDLL(str(ROOT / "isoext_amd/libisoext_amd.so"))
     lib.isoext_version.restype = ctypes.c_int
-    assert lib.isoext_version() == 1
+    assert lib.isoext_version() == 2
     lib.isoext_last_error.restype = ctypes.c_char_p
     assert lib.isoext_last_error() == b""
 

@@ -167,6 +167,24 @@ def test_dc_csg_analytic_normals_256(gpu):
     compare_dc(vals, its_np, user_normals=nrm)
 
 
+def test_dc_csg_program_normals_256(gpu):
+    """The same workload with the SDF program's stored grid and analytic
+    normals (the normals fused dual contouring uses): dual vertices within
+    1e-4 h of FP64."""
+    import torch
+
+    from isoext_amd import workloads
+    from isoext_amd.csg import SdfProgram
+    shape = (256, 256, 256)
+    prog = SdfProgram(workloads.csg_box_cylinder_sdf(16, 0))
+    vals = prog.grid(shape).cpu().numpy()
+    its_np = oracle.intersection(vals, 0.0)
+    _, nrm = prog.points(torch.from_numpy(its_np["points"]).cuda(), normals=True)
+    nrm = nrm.cpu().numpy()
+    its_np["normals"] = nrm
+    compare_dc(vals, its_np, user_normals=nrm)
+
+
 def test_dc_fp64_resolve_with_truncation(gpu):
     """reg = 0: flat regions are singular, the reference truncates singular
     values and returns the origin-centred minimum-norm solution."""
