@@ -46,9 +46,12 @@ def parse():
     ap.add_argument("--no-sparse", action="store_true", help="skip the 2048^3 sparse summary")
     ap.add_argument("--serving", type=int, default=10000,
                     help="requests in the serving comparison (0 = skip)")
-    ap.add_argument("--workload", default="mc1024", choices=["mc1024", "dc256", "sparse2048"],
+    ap.add_argument("--serving-sdf", type=int, default=2000,
+                    help="requests in the serving comparison with SDF-program fields (0 = skip)")
+    ap.add_argument("--workload", default="mc1024", choices=["mc1024", "dc256", "sparse2048", "csg"],
                     help="mc1024: plain MC at 1024^3 (default); dc256: dual contouring of the 256^3 CSG "
-                         "field; sparse2048: differentiable sparse MC on the 2048^3 narrow band")
+                         "field; sparse2048: differentiable sparse MC on the 2048^3 narrow band; csg: the CSG "
+                         "field's fused (never stored) extractions against generate-then-extract")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run a plain 4-byte elementwise copy (torch.neg) over MIB "
                          "MiB three times, so that one counter run also sees a kernel of known traffic")
@@ -143,6 +146,7 @@ def bench_dc256(args):
         dc_raw(g, it, normals, 1e-2, 1e-6)
     ms = timed_steps(step, args.steps, args.warmup) * 1e3
     v, f, info, dual = dc_raw(g, its, normals, 1e-2, 1e-6, want_dual=True)
+    fused = csg_dc_fused(shape, args.steps, args.warmup)
     # accuracy against the FP64 oracle on the same intersections and normals
     vals_np = vals.cpu().numpy()
     its_np = oracle.intersection(vals_np, 0.0)
@@ -167,12 +171,70 @@ def bench_dc256(args):
                    "faces": nf, "fp64_resolves": info["fp64_resolves"],
                    "dual_error_h": {"max": float(err.max()), "p99": float(np.percentile(err, 99)),
                                     "mean": float(err.mean())},
-                   "achieved_GBs": alg / (ms * 1e-3) / 1e9},
+                   "achieved_GBs": alg / (ms * 1e-3) / 1e9, "fused": fused},
         "roofline": {"bound": "hbm", "achieved": alg / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": {"value": npts / t_cpu / 1e9, "unit": "Gvoxels/s", "cores": 1, "kind": "port",
                          "sample": f"oracle FP64 QEF dual contouring on the same intersections, {t_cpu:.2f} s"},
     }
+
+
+def csg_dc_fused(shape, steps, warmup):
+    """Dual contouring of the CSG workload end to end, field included: fused
+    (the SDF program evaluated inside the kernels, analytic normals, no grid)
+    against the stored-grid recipe (program grid, intersections, program
+    normals at the points, DC) and a torch evaluation of the grid."""
+    import torch
+
+    import isoext_amd as iso
+    from isoext_amd import workloads
+    from isoext_amd.csg import SdfProgram
+    from isoext_amd.dc import dc_raw, get_intersection
+    from isoext_amd.fused import dual_contouring_sdf
+    prog = SdfProgram(workloads.csg_box_cylinder_sdf(16, 0))
+    vals = torch.empty(shape, dtype=torch.float32, device="cuda")
+    g = iso.UniformGrid.wrap(vals)
+
+    def stored():
+        prog.grid(shape, out=vals)
+        it = get_intersection(g, 0.0)
+        _, nrm = prog.points(it.points, normals=True)
+        dc_raw(g, it, nrm, 1e-2, 1e-6)
+    ms_stored = timed_steps(stored, steps, warmup) * 1e3
+    ms_fused = timed_steps(lambda: dual_contouring_sdf(prog, shape), steps, warmup) * 1e3
+    ms_gen = timed_steps(lambda: prog.grid(shape, out=vals), steps, warmup) * 1e3
+    ms_torch = timed_steps(lambda: workloads.csg_box_cylinder(shape, program=False), 2, 1) * 1e3
+    v, f = dual_contouring_sdf(prog, shape)
+    return {"fused_ms": ms_fused, "stored_recipe_ms": ms_stored, "program_grid_ms": ms_gen,
+            "torch_grid_ms": ms_torch, "vertices": int(v.shape[0]), "faces": int(f.shape[0]),
+            "note": "fused = dual_contouring_sdf (analytic normals, no grid); stored recipe = program grid + "
+                    "get_intersection + program normals + DC; identical meshes (tests/test_csg_gpu.py)"}
+
+
+def csg_mc_fused(sizes=(256, 1024), steps=5, warmup=2):
+    """Marching cubes of the CSG workload's field, fused (never stored)
+    against generate-then-extract, per grid size."""
+    import torch
+
+    import isoext_amd as iso
+    from isoext_amd import workloads
+    from isoext_amd.csg import SdfProgram
+    from isoext_amd.fused import marching_cubes_sdf
+    from isoext_amd.mc import mc_dense_raw
+    prog = SdfProgram(workloads.csg_box_cylinder_sdf(16, 0))
+    out = {}
+    for n in sizes:
+        shape = (n, n, n)
+        ms_fused = timed_steps(lambda: marching_cubes_sdf(prog, shape), steps, warmup) * 1e3
+        vals = torch.empty(shape, dtype=torch.float32, device="cuda")
+        ms_gen = timed_steps(lambda: prog.grid(shape, out=vals), steps, warmup) * 1e3
+        ms_mc = timed_steps(lambda: mc_dense_raw(vals, shape, (-1.0,) * 3, (1.0,) * 3, 0.0, 0), steps, warmup) * 1e3
+        v, f = marching_cubes_sdf(prog, shape)
+        out[f"{n}^3"] = {"fused_ms": ms_fused, "grid_ms": ms_gen, "mc_on_grid_ms": ms_mc,
+                         "stored_total_ms": ms_gen + ms_mc, "vertices": int(v.shape[0]), "faces": int(f.shape[0])}
+        del vals
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_sparse2048(args):
@@ -316,7 +378,7 @@ def sparse_fd_check(g, vals0, src, w, grad, n_check, step):
             "entries_step_over_0.1_dv": near}
 
 
-def serving_bench(n_requests: int, seed: int = 0):
+def serving_bench(n_requests: int, seed: int = 0, sdf: bool = False):
     """Continuous-batching server vs the plain per-call path on the same
     seeded request stream: dense grids with cubic sides log-uniform in
     [32, 512], SDF = union of 1-64 random spheres, 80% marching cubes / 20%
@@ -324,14 +386,17 @@ def serving_bench(n_requests: int, seed: int = 0):
     of 320 distinct grids generated on the GPU before timing, taken in a
     fixed permutation, so that the 8 workers x 32 requests in flight never
     hold one grid twice (no batch sees a grid that another request of the
-    batch already pulled into the caches)."""
+    batch already pulled into the caches).  sdf=True: the same distribution
+    of fields as SDF programs (union-of-spheres trees) that the batched
+    kernels evaluate without storing a grid; the plain path writes each
+    request's grid (set_values) and then extracts."""
     import numpy as np
     import torch
 
     from isoext_amd import serving
-    pool = serving.make_pool(POOL_GRIDS, seed=seed)
+    pool = serving.make_sdf_pool(POOL_GRIDS, seed=seed) if sdf else serving.make_pool(POOL_GRIDS, seed=seed)
     torch.cuda.synchronize()
-    pts = float(np.mean([g.numel() for g in pool]))
+    pts = float(np.mean([(e[1][0] ** 3 if isinstance(e, tuple) else e.numel()) for e in pool]))
 
     def stream(rate, s):
         return serving.make_stream(pool, n_requests, rate=rate, seed=s, dc_frac=0.2, cycle=True)
@@ -384,6 +449,7 @@ def serving_bench(n_requests: int, seed: int = 0):
     q = lambda a, x: float(np.percentile(a, x) * 1e3)
     return {
         "requests": n_requests, "dc_fraction": 0.2, "mean_points": pts, "pool_grids": len(pool),
+        "fields": "SDF programs evaluated in the kernels (no stored grid)" if sdf else "stored grids",
         "meshes_per_s_batched": thr_b, "meshes_per_s_plain": thr_p, "throughput_ratio": thr_b / thr_p,
         "saturation_passes_ms": {"batched": [w * 1e3 for w in walls_b], "plain": [w * 1e3 for w in walls_p]},
         "poisson_rate_per_s": rate,
@@ -517,7 +583,9 @@ def main():
     if args.workload != "mc1024":
         import torch
         torch.cuda.set_device(0)
-        fn = {"dc256": bench_dc256, "sparse2048": bench_sparse2048}[args.workload]
+        fn = {"dc256": bench_dc256, "sparse2048": bench_sparse2048,
+              "csg": lambda a: {"csg_mc": csg_mc_fused(steps=a.steps, warmup=a.warmup),
+                                "csg_dc256": csg_dc_fused((256, 256, 256), a.steps, a.warmup)}}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     import torch
@@ -646,7 +714,9 @@ def main():
                                   "fp64_resolves": r["config"]["fp64_resolves"],
                                   "active_cells": r["config"]["active_cells"],
                                   "intersections": r["config"]["intersections"], "roofline": r["roofline"],
-                                  "cpu_baseline": r["cpu_baseline"]}
+                                  "cpu_baseline": r["cpu_baseline"], "fused": r["config"]["fused"]}
+        torch.cuda.empty_cache()
+        out["config"]["csg_mc"] = csg_mc_fused()
         torch.cuda.empty_cache()
     if not args.no_sparse:
         r = bench_sparse2048(sub)
@@ -659,6 +729,9 @@ def main():
         torch.cuda.empty_cache()
     if args.serving > 0:
         out["config"]["serving"] = serving_bench(args.serving, args.seed)
+        torch.cuda.empty_cache()
+    if args.serving_sdf > 0:
+        out["config"]["serving_sdf"] = serving_bench(args.serving_sdf, args.seed, sdf=True)
     print(json.dumps(out), flush=True)
 
 

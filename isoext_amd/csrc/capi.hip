@@ -409,7 +409,8 @@ bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_c
         L.NK = L.cells ? ((Z - 1 + 63) / 64 + nzc - 1) / nzc : 0;
         L.nblk = L.cells ? (X * Y * L.S + iso::kMcScanBlock - 1) / iso::kMcScanBlock : 0;
         L.nctask = L.cells && !items[i].sdf ? (X * Y * Z + iso::kFlatPts - 1) / iso::kFlatPts : 0;
-        L.ngtask = L.cells && items[i].sdf ? iso::csg_classify_tasks(X, Y, L.W) : 0;
+        // program blocks, padded to whole 256-block chunks (one item per workgroup)
+        L.ngtask = L.cells && items[i].sdf ? (iso::csg_classify_tasks(X, Y, L.W) + 255) / 256 * 256 : 0;
         L.ntiles = L.NI * L.NJ * L.NK;
         B.h.nctask += L.nctask;
         B.h.ngtask += L.ngtask;
@@ -570,6 +571,7 @@ void fill_item(const isoext_mc_item &in, const ItemLayout &L, char *ws, int64_t 
         p.values = nullptr;
         p.vec4 = 0;
         p.csg = (const iso::CsgProgram *) in.sdf;
+        p.csg_list = iso::csg_list_fits(p.X, p.Y, p.W, p.S) ? 1 : 0;
     }
     out.ctask0 = ctask0;
     out.blk0 = blk0;
@@ -1047,6 +1049,7 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
         p.values = nullptr;
         p.vec4 = 0;
         p.csg = csg;
+        p.csg_list = iso::csg_list_fits(p.X, p.Y, p.W, p.S) ? 1 : 0;
     }
     hipEvent_t *ev = (g_timing && !na) ? timing_events() : nullptr;
     if (g_timing && !na && !ev) return fail(ISOEXT_E_DEVICE, "timing events unavailable");

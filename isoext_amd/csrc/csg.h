@@ -165,8 +165,38 @@ __device__ __forceinline__ CsgOpWord csg_decode(const CsgProgram &P, int i) {
     return o;
 }
 
+// Value of primitive `op` (SPHERE .. CYLINDER_Z) at (x, y, z).
+__device__ __forceinline__ float csg_prim(int op, const float *a, float x, float y, float z) {
+    switch (op) {
+    case CSG_SPHERE: return csg_norm3(x, y, z) - a[0];
+    case CSG_SPHERE_AT: return csg_norm3(x - a[0], y - a[1], z - a[2]) - a[3];
+    case CSG_TORUS: {
+        const float t = csg_norm2(x, y) - a[0];
+        return csg_norm2(t, z) - a[1];
+    }
+    case CSG_CUBOID: {
+        const float ux = fabsf(x) - a[0], uy = fabsf(y) - a[1], uz = fabsf(z) - a[2];
+        float m = ux;
+        if (csg_take_max(m, uy)) m = uy;
+        if (csg_take_max(m, uz)) m = uz;
+        const float ox = ux != ux ? ux : fmaxf(ux, 0.0f), oy = uy != uy ? uy : fmaxf(uy, 0.0f),
+                    oz = uz != uz ? uz : fmaxf(uz, 0.0f);
+        const float in = m != m ? m : fminf(m, 0.0f);
+        return csg_norm3(ox, oy, oz) + in;
+    }
+    default: return csg_norm2(x, y) - a[0];   // CYLINDER_Z
+    }
+}
+
+__device__ __forceinline__ float csg_combine(int acc, float cur, float r) {
+    if (acc == CSG_ACC_MIN) return csg_take_min(cur, r) ? r : cur;
+    if (acc == CSG_ACC_MAX) return csg_take_max(cur, r) ? r : cur;
+    return r;
+}
+
 // Field value at (px, py, pz).  UNI: every lane of the wave runs the same
-// program (ops decoded into scalar registers).
+// program (ops decoded into scalar registers).  Primitives on point slot 0
+// into value slot 0 (most ops of a union of shapes) skip the slot selects.
 template <bool UNI>
 __device__ float csg_value(const CsgProgram &P, float px, float py, float pz) {
     CsgSlots8 v;
@@ -183,6 +213,10 @@ __device__ float csg_value(const CsgProgram &P, float px, float py, float pz) {
     for (int i = 0; i < n; ++i) {
         const CsgOpWord o = csg_decode<UNI>(P, i);
         const float *a = P.prm + o.prm;
+        if (o.op >= CSG_SPHERE && o.op <= CSG_CYLINDER_Z && o.dst == 0 && o.src == 0) {
+            v.a.s0 = csg_combine(o.acc, v.a.s0, csg_prim(o.op, a, qx.a.x, qy.a.x, qz.a.x));
+            continue;
+        }
         const int ps = o.src & (kCsgPts - 1);
         const float x = qx.get(ps), y = qy.get(ps), z = qz.get(ps);
         if (o.op == CSG_TRANSLATE || o.op == CSG_ROTATE) {
@@ -207,39 +241,19 @@ __device__ float csg_value(const CsgProgram &P, float px, float py, float pz) {
         }
         const float cur = v.get(o.dst);
         float r;
-        switch (o.op) {
-        case CSG_SPHERE: r = csg_norm3(x, y, z) - a[0]; break;
-        case CSG_SPHERE_AT: r = csg_norm3(x - a[0], y - a[1], z - a[2]) - a[3]; break;
-        case CSG_TORUS: {
-            const float t = csg_norm2(x, y) - a[0];
-            r = csg_norm2(t, z) - a[1];
-            break;
-        }
-        case CSG_CUBOID: {
-            const float ux = fabsf(x) - a[0], uy = fabsf(y) - a[1], uz = fabsf(z) - a[2];
-            float m = ux;
-            if (csg_take_max(m, uy)) m = uy;
-            if (csg_take_max(m, uz)) m = uz;
-            const float ox = ux != ux ? ux : fmaxf(ux, 0.0f), oy = uy != uy ? uy : fmaxf(uy, 0.0f),
-                        oz = uz != uz ? uz : fmaxf(uz, 0.0f);
-            const float in = m != m ? m : fminf(m, 0.0f);
-            r = csg_norm3(ox, oy, oz) + in;
-            break;
-        }
-        case CSG_CYLINDER_Z: r = csg_norm2(x, y) - a[0]; break;
-        case CSG_VALUE: r = v.get(o.src); break;
-        case CSG_SMOOTH_MIN: {
+        if (o.op >= CSG_SPHERE && o.op <= CSG_CYLINDER_Z) {
+            r = csg_prim(o.op, a, x, y, z);
+        } else if (o.op == CSG_VALUE) {
+            r = v.get(o.src);
+        } else if (o.op == CSG_SMOOTH_MIN) {
             const float k = a[0], b = v.get(o.src);
             r = (-k) * logf(expf(-cur / k) + expf(-b / k));
-            break;
+        } else if (o.op == CSG_NEG) {
+            r = -cur;
+        } else {
+            r = cur;
         }
-        case CSG_NEG: r = -cur; break;
-        default: r = cur; break;
-        }
-        float res = r;
-        if (o.acc == CSG_ACC_MIN) res = csg_take_min(cur, r) ? r : cur;
-        else if (o.acc == CSG_ACC_MAX) res = csg_take_max(cur, r) ? r : cur;
-        v.set(o.dst, res);
+        v.set(o.dst, csg_combine(o.acc, cur, r));
     }
     return v.a.s0;
 }

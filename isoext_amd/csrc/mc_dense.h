@@ -109,6 +109,7 @@ struct McParams {
     // fused field given by an SDF program (device form, csg.h); values
     // unused when set
     const CsgProgram *csg;
+    int csg_list;      // the classify's undecided-block list fits in cnt_v (see csg_list_fits)
     // chunk index -> (row, chunk) -> (x, y) and chunk -> z-segment by
     // multiply-high, valid when narrow (X * Y * W < 2^32); set by
     // mc_params_divisors once W, Y and nzc are final
@@ -163,12 +164,16 @@ constexpr int kStrScanChunk = 256; // waves per first-level scan block
 // cell rows x one word (tile configuration 6).
 constexpr int kStrTileConfig = 6;
 constexpr int kMaxFusedSpheres = 256;
-// Classify task of a program field: 4 x 4 x 4 blocks of 4 x 4 point rows x
-// one 64-point word (16 x 16 point rows x 256 points).
+// Classify blocks of a program field: 4 x 4 point rows x one 64-point word.
 __host__ __device__ inline int64_t csg_classify_tasks(int64_t X, int64_t Y, int64_t W) {
-    const int64_t nbx = (X + 3) / 4, nby = (Y + 3) / 4;
-    return ((nbx + 3) / 4) * ((nby + 3) / 4) * ((W + 3) / 4);
+    return ((X + 3) / 4) * ((Y + 3) / 4) * W;
 }
+// The undecided-block list (uint32 block ids in cnt_v, X*Y*S entries) fits.
+inline bool csg_list_fits(int64_t X, int64_t Y, int64_t W, int64_t S) {
+    const int64_t nb = csg_classify_tasks(X, Y, W);
+    return nb <= X * Y * S && nb <= (int64_t) UINT32_MAX;
+}
+constexpr int kCsgFillWaves = 4096;   // phase-2 workgroups (one wave each)
 
 // Fill McParams' invariant-divisor fields from X, Y, W and nzc.
 void mc_params_divisors(McParams &p);

@@ -717,82 +717,34 @@ __global__ void __launch_bounds__(256) k_classify_spheres(McParams p) {
 // Inside masks of v(p) = program(p) (csg.h) with exact sign culling: the
 // field is 1-Lipschitz (programs without smooth min), so a box of grid
 // points whose centre value is farther from the level than the box radius
-// (plus a margin far above the FP32 evaluation error) has one sign.  A task
-// holds 64 blocks of 4 x 4 point rows x one word; each lane tests one block
-// at its centre.  An undecided block tests its 64 columns of 4 x 4 points
-// (one z per lane), and the points of the columns still undecided are
-// evaluated one per lane, 64 at a time, and assembled into the rows' words.
+// (plus a margin far above the FP32 evaluation error) has one sign.
+// Phase 1 (one lane per block of 4 x 4 point rows x one 64-point word)
+// tests each block at its centre, writes the words of decided blocks and
+// lists the others; phase 2 (one wave per listed block) tests the block's
+// 64 columns of 4 x 4 points (one z per lane) and evaluates the points of
+// the columns still undecided, 64 at a time, assembling the rows' words.
 // Every evaluated value is the stored grid's (same program, same grid
 // coordinates), and decided points have the sign their value would give.
-__device__ void classify_csg_task(const McParams &p, const CsgProgram &P, int64_t t, double cscale) {
-    const int lane = threadIdx.x & 63;
-    const int64_t nbx = (p.X + 3) / 4, nby = (p.Y + 3) / 4;
-    const int64_t tx4 = (nbx + 3) / 4, ty4 = (nby + 3) / 4, tc4 = (p.W + 3) / 4;
-    const int64_t gc = t % tc4, r0 = t / tc4, gy = r0 % ty4, gx = r0 / ty4;
-    (void) tx4;
-    // this lane's block
-    const int64_t xa = 4 * (4 * gx + (lane >> 4)), ya = 4 * (4 * gy + ((lane >> 2) & 3)), c = 4 * gc + (lane & 3);
-    const int64_t za = 64 * c;
-    const bool valid = xa < p.X && ya < p.Y && c < p.W;
-    const int nxr = valid ? (int) min<int64_t>(4, p.X - xa) : 1, nyr = valid ? (int) min<int64_t>(4, p.Y - ya) : 1;
-    const int nzp = valid ? (int) min<int64_t>(64, p.Z - za) : 1;
-    int sign = 1;
-    {
-        const int64_t xs = valid ? xa : 0, ys = valid ? ya : 0, zs = valid ? za : 0;
-        const float x0 = grid_coord(xs, p.X, p.amin[0], p.amax[0]), x1 = grid_coord(xs + nxr - 1, p.X, p.amin[0], p.amax[0]);
-        const float y0 = grid_coord(ys, p.Y, p.amin[1], p.amax[1]), y1 = grid_coord(ys + nyr - 1, p.Y, p.amin[1], p.amax[1]);
-        const float z0 = grid_coord(zs, p.Z, p.amin[2], p.amax[2]), z1 = grid_coord(zs + nzp - 1, p.Z, p.amin[2], p.amax[2]);
-        const float cx = 0.5f * (x0 + x1), cy = 0.5f * (y0 + y1), cz = 0.5f * (z0 + z1);
-        const double ex = fmax(fabs((double) cx - x0), fabs((double) x1 - cx));
-        const double ey = fmax(fabs((double) cy - y0), fabs((double) y1 - cy));
-        const double ez = fmax(fabs((double) cz - z0), fabs((double) z1 - cz));
-        const float vc = csg_value<true>(P, cx, cy, cz);
-        sign = valid ? csg_box_sign(P, vc, sqrt(ex * ex + ey * ey + ez * ez), cscale, p.level) : 1;
-    }
-    if (valid && sign != 0) {
-        const uint64_t w = sign < 0 ? low_bits(nzp) : 0ull;
-        for (int r = 0; r < 16; ++r)
-            if ((r >> 2) < nxr && (r & 3) < nyr) p.bits[((xa + (r >> 2)) * p.Y + (ya + (r & 3))) * p.W + c] = w;
-    }
-    uint64_t und = __ballot(valid && sign == 0);
-    while (und) {
-        const int b = __ffsll((unsigned long long) und) - 1;
-        und &= und - 1;
-        const int64_t bxa = 4 * (4 * gx + (b >> 4)), bya = 4 * (4 * gy + ((b >> 2) & 3)), bc = 4 * gc + (b & 3);
-        const int64_t bza = 64 * bc;
-        const int bnx = (int) min<int64_t>(4, p.X - bxa), bny = (int) min<int64_t>(4, p.Y - bya);
-        const int bnz = (int) min<int64_t>(64, p.Z - bza);
-        // columns: 4 x 4 rows at one z per lane
-        const float x0 = grid_coord(bxa, p.X, p.amin[0], p.amax[0]), x1 = grid_coord(bxa + bnx - 1, p.X, p.amin[0], p.amax[0]);
-        const float y0 = grid_coord(bya, p.Y, p.amin[1], p.amax[1]), y1 = grid_coord(bya + bny - 1, p.Y, p.amin[1], p.amax[1]);
-        const float cx = 0.5f * (x0 + x1), cy = 0.5f * (y0 + y1);
-        const double ex = fmax(fabs((double) cx - x0), fabs((double) x1 - cx));
-        const double ey = fmax(fabs((double) cy - y0), fabs((double) y1 - cy));
-        const bool zok = lane < bnz;
-        const float pz = grid_coord(bza + (zok ? lane : 0), p.Z, p.amin[2], p.amax[2]);
-        const float vs = csg_value<true>(P, cx, cy, pz);
-        const int s2 = zok ? csg_box_sign(P, vs, sqrt(ex * ex + ey * ey), cscale, p.level) : 1;
-        const uint64_t known_in = __ballot(zok && s2 < 0);
-        const uint64_t need = __ballot(zok && s2 == 0);
-        const int nn = __popcll(need);
-        uint64_t word = known_in;   // lane r < 16: the word of row r
-        for (int j0 = 0; j0 < 16 * nn; j0 += 64) {
-            const int j = j0 + lane, r = lane & 15;
-            const bool act = j < 16 * nn && (r >> 2) < bnx && (r & 3) < bny;
-            const int L = j < 16 * nn ? nth_set_bit(need, j >> 4) : 0;
-            const float px = grid_coord(bxa + (act ? (r >> 2) : 0), p.X, p.amin[0], p.amax[0]);
-            const float py = grid_coord(bya + (act ? (r & 3) : 0), p.Y, p.amin[1], p.amax[1]);
-            const float qz = grid_coord(bza + L, p.Z, p.amin[2], p.amax[2]);
-            const int in = act && inside(csg_value<true>(P, px, py, qz), p.level) ? 1 : 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int bq = __shfl(in, r + 16 * q), lq = __shfl(L, r + 16 * q);
-                if (bq) word |= 1ull << lq;
-            }
-        }
-        if (lane < 16 && (lane >> 2) < bnx && (lane & 3) < bny)
-            p.bits[((bxa + (lane >> 2)) * p.Y + (bya + (lane & 3))) * p.W + bc] = word & low_bits(bnz);
-    }
+// The list lives in cnt_v (its count in cnt_f[0]), free until the count
+// pass; when it does not fit (fewer than 8 point rows along x or y), phase
+// 2 walks every block and repeats the centre test (p.csg_list = 0).
+
+struct CsgBlock {
+    int64_t xa, ya, c, za;
+    int nx, ny, nz;
+};
+__device__ __forceinline__ CsgBlock csg_block(const McParams &p, int64_t b) {
+    const int64_t nby = (p.Y + 3) / 4;
+    CsgBlock k;
+    k.c = b % p.W;
+    const int64_t r = b / p.W, by = r % nby, bx = r / nby;
+    k.xa = 4 * bx;
+    k.ya = 4 * by;
+    k.za = 64 * k.c;
+    k.nx = (int) min<int64_t>(4, p.X - k.xa);
+    k.ny = (int) min<int64_t>(4, p.Y - k.ya);
+    k.nz = (int) min<int64_t>(64, p.Z - k.za);
+    return k;
 }
 
 __device__ __forceinline__ double csg_coord_scale(const McParams &p) {
@@ -801,14 +753,103 @@ __device__ __forceinline__ double csg_coord_scale(const McParams &p) {
     return m;
 }
 
-// One wave per workgroup, tasks grid-strided.
-__global__ void __launch_bounds__(64) k_classify_csg(McParams p) {
+// Sign of block k from its centre value (every lane of the wave evaluates
+// its own block, or all the same one).
+__device__ __forceinline__ int csg_block_sign(const McParams &p, const CsgProgram &P, const CsgBlock &k, double cs) {
+    const float x0 = grid_coord(k.xa, p.X, p.amin[0], p.amax[0]), x1 = grid_coord(k.xa + k.nx - 1, p.X, p.amin[0], p.amax[0]);
+    const float y0 = grid_coord(k.ya, p.Y, p.amin[1], p.amax[1]), y1 = grid_coord(k.ya + k.ny - 1, p.Y, p.amin[1], p.amax[1]);
+    const float z0 = grid_coord(k.za, p.Z, p.amin[2], p.amax[2]), z1 = grid_coord(k.za + k.nz - 1, p.Z, p.amin[2], p.amax[2]);
+    const float cx = 0.5f * (x0 + x1), cy = 0.5f * (y0 + y1), cz = 0.5f * (z0 + z1);
+    const double ex = fmax(fabs((double) cx - x0), fabs((double) x1 - cx));
+    const double ey = fmax(fabs((double) cy - y0), fabs((double) y1 - cy));
+    const double ez = fmax(fabs((double) cz - z0), fabs((double) z1 - cz));
+    const float vc = csg_value<true>(P, cx, cy, cz);
+    return csg_box_sign(P, vc, sqrt(ex * ex + ey * ey + ez * ez), cs, p.level);
+}
+
+__device__ __forceinline__ void csg_write_block(const McParams &p, const CsgBlock &k, int sign) {
+    const uint64_t w = sign < 0 ? low_bits(k.nz) : 0ull;
+    for (int r = 0; r < 16; ++r)
+        if ((r >> 2) < k.nx && (r & 3) < k.ny) p.bits[((k.xa + (r >> 2)) * p.Y + (k.ya + (r & 3))) * p.W + k.c] = w;
+}
+
+// Phase 1 for block b (valid: b is a block of the grid); all lanes call it.
+__device__ void csg_test_block(const McParams &p, const CsgProgram &P, int64_t b, bool valid, double cs) {
+    const CsgBlock k = csg_block(p, valid ? b : 0);
+    const int sign = csg_block_sign(p, P, k, cs);
+    if (valid && sign != 0) csg_write_block(p, k, sign);
+    const uint64_t und = __ballot(valid && sign == 0);
+    if (!p.csg_list || !und) return;
+    const int lane = threadIdx.x & 63;
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(p.cnt_f, (unsigned) __popcll(und));
+    base = __shfl(base, 0);
+    if (valid && sign == 0) p.cnt_v[base + __popcll(und & ((1ull << lane) - 1ull))] = (uint32_t) b;
+}
+
+// Phase 2 for an undecided block b (one wave).
+__device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b, double cs) {
+    const int lane = threadIdx.x & 63;
+    const CsgBlock k = csg_block(p, b);
+    // columns: 4 x 4 rows at one z per lane
+    const float x0 = grid_coord(k.xa, p.X, p.amin[0], p.amax[0]), x1 = grid_coord(k.xa + k.nx - 1, p.X, p.amin[0], p.amax[0]);
+    const float y0 = grid_coord(k.ya, p.Y, p.amin[1], p.amax[1]), y1 = grid_coord(k.ya + k.ny - 1, p.Y, p.amin[1], p.amax[1]);
+    const float cx = 0.5f * (x0 + x1), cy = 0.5f * (y0 + y1);
+    const double ex = fmax(fabs((double) cx - x0), fabs((double) x1 - cx));
+    const double ey = fmax(fabs((double) cy - y0), fabs((double) y1 - cy));
+    const bool zok = lane < k.nz;
+    const float pz = grid_coord(k.za + (zok ? lane : 0), p.Z, p.amin[2], p.amax[2]);
+    const float vs = csg_value<true>(P, cx, cy, pz);
+    const int s2 = zok ? csg_box_sign(P, vs, sqrt(ex * ex + ey * ey), cs, p.level) : 1;
+    const uint64_t known_in = __ballot(zok && s2 < 0);
+    const uint64_t need = __ballot(zok && s2 == 0);
+    const int nn = __popcll(need);
+    uint64_t word = known_in;   // lane r < 16: the word of row r
+    for (int j0 = 0; j0 < 16 * nn; j0 += 64) {
+        const int j = j0 + lane, r = lane & 15;
+        const bool act = j < 16 * nn && (r >> 2) < k.nx && (r & 3) < k.ny;
+        const int L = j < 16 * nn ? nth_set_bit(need, j >> 4) : 0;
+        const float px = grid_coord(k.xa + (act ? (r >> 2) : 0), p.X, p.amin[0], p.amax[0]);
+        const float py = grid_coord(k.ya + (act ? (r & 3) : 0), p.Y, p.amin[1], p.amax[1]);
+        const float qz = grid_coord(k.za + L, p.Z, p.amin[2], p.amax[2]);
+        const int in = act && inside(csg_value<true>(P, px, py, qz), p.level) ? 1 : 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int bq = __shfl(in, r + 16 * q), lq = __shfl(L, r + 16 * q);
+            if (bq) word |= 1ull << lq;
+        }
+    }
+    if (lane < 16 && (lane >> 2) < k.nx && (lane & 3) < k.ny)
+        p.bits[((k.xa + (lane >> 2)) * p.Y + (k.ya + (lane & 3))) * p.W + k.c] = word & low_bits(k.nz);
+}
+
+// Phase 2 over the list, or over every block (repeating the test).
+__device__ void csg_fill(const McParams &p, const CsgProgram &P, double cs, int64_t first, int64_t stride) {
+    if (p.csg_list) {
+        const int64_t n = __builtin_amdgcn_readfirstlane((int) *(volatile uint32_t *) p.cnt_f);
+        for (int64_t i = first; i < n; i += stride) csg_fill_block(p, P, (int64_t) p.cnt_v[i], cs);
+    } else {
+        const int64_t nb = csg_classify_tasks(p.X, p.Y, p.W);
+        for (int64_t b = first; b < nb; b += stride)
+            if (csg_block_sign(p, P, csg_block(p, b), cs) == 0) csg_fill_block(p, P, b, cs);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_csg_blocks(McParams p) {
     __shared__ CsgProgram P;
     csg_stage(P, p.csg);
     __syncthreads();
-    const int64_t ntask = csg_classify_tasks(p.X, p.Y, p.W);
+    const int64_t nb = csg_classify_tasks(p.X, p.Y, p.W);
     const double cs = csg_coord_scale(p);
-    for (int64_t t = blockIdx.x; t < ntask; t += gridDim.x) classify_csg_task(p, P, t, cs);
+    for (int64_t base = (int64_t) blockIdx.x * 256; base < nb; base += (int64_t) gridDim.x * 256)
+        csg_test_block(p, P, base + threadIdx.x, base + threadIdx.x < nb, cs);
+}
+
+__global__ void __launch_bounds__(64) k_csg_fill(McParams p) {
+    __shared__ CsgProgram P;
+    csg_stage(P, p.csg);
+    __syncthreads();
+    csg_fill(p, P, csg_coord_scale(p), blockIdx.x, gridDim.x);
 }
 
 // LDS of the fused emit: per 64-point z chunk of the tile, the spheres that
@@ -1395,6 +1436,7 @@ __global__ void mc_dense_reset(McParams p, SpecialLists init) {
         for (int64_t i = t; i <= p.dmask; i += (int64_t) gridDim.x * blockDim.x) p.dslots[i] = -1;
     if (t == 0) {
         if (p.tile_count) *p.tile_count = 0;
+        if (p.csg) p.cnt_f[0] = 0;   // the program classify's block list count
         *p.err = 0;
         p.totals[0] = p.totals[1] = 0;
         init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
@@ -1526,6 +1568,7 @@ __global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, c
         if (c == 0) {
             *p.err = 0;
             p.totals[0] = p.totals[1] = 0;
+            if (p.csg && p.X >= 2 && p.Y >= 2 && p.Z >= 2) p.cnt_f[0] = 0;   // the program classify's list count
             SpecialLists init = items[i].sp_init;
             init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
             init.need = 0;
@@ -1554,21 +1597,21 @@ __global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *__res
     }
 }
 
-// Program items of a batch: tasks over the items' program task ranges
-// (gtask0 is non-decreasing in item order; items without a program own no
-// task); the program is staged again when the item changes.
-__global__ void __launch_bounds__(64) k_classify_csg_batch(const McBatchItem *__restrict__ items,
-                                                           const McBatchHdr *__restrict__ hdr) {
+// Program items of a batch, phase 1: the items' block ranges are
+// consecutive and padded to 256 blocks (gtask0 non-decreasing in item
+// order; items without a program own none), so a workgroup's 256 blocks
+// belong to one item; the program is staged again when the item changes.
+__global__ void __launch_bounds__(256) k_csg_blocks_batch(const McBatchItem *__restrict__ items,
+                                                         const McBatchHdr *__restrict__ hdr) {
     __shared__ CsgProgram P;
     __shared__ int64_t starts[kMaxBatch];
     const int n = (int) hdr->n;
-    for (int i = threadIdx.x; i < n; i += 64) starts[i] = items[i].gtask0;
+    for (int i = threadIdx.x; i < n; i += 256) starts[i] = items[i].gtask0;
     __syncthreads();
     int cur = -1;
     double cs = 0.0;
-    for (int64_t t = blockIdx.x; t < hdr->ngtask; t += gridDim.x) {
-        // program task ranges are consecutive in item order (other items own none)
-        const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, t));
+    for (int64_t base = (int64_t) blockIdx.x * 256; base < hdr->ngtask; base += (int64_t) gridDim.x * 256) {
+        const int i = __builtin_amdgcn_readfirstlane(find_start(starts, n, base));
         const McParams &p = items[i].p;
         if (i != cur) {
             __syncthreads();
@@ -1577,7 +1620,24 @@ __global__ void __launch_bounds__(64) k_classify_csg_batch(const McBatchItem *__
             cur = i;
             cs = csg_coord_scale(p);
         }
-        classify_csg_task(p, P, t - items[i].gtask0, cs);
+        const int64_t b = base - items[i].gtask0 + threadIdx.x;
+        csg_test_block(p, P, b, b < csg_classify_tasks(p.X, p.Y, p.W), cs);
+    }
+}
+
+// Phase 2 of every program item of a batch (one wave per workgroup).
+__global__ void __launch_bounds__(64) k_csg_fill_batch(const McBatchItem *__restrict__ items,
+                                                       const McBatchHdr *__restrict__ hdr) {
+    __shared__ CsgProgram P;
+    for (int i = 0; i < (int) hdr->n; ++i) {
+        const McParams &p = items[i].p;
+        if (!p.csg || p.X < 2 || p.Y < 2 || p.Z < 2) continue;
+        const int64_t work = p.csg_list ? (int64_t) *(volatile uint32_t *) p.cnt_f : csg_classify_tasks(p.X, p.Y, p.W);
+        if ((int64_t) blockIdx.x >= work) continue;
+        __syncthreads();
+        csg_stage(P, p.csg);
+        __syncthreads();
+        csg_fill(p, P, csg_coord_scale(p), blockIdx.x, gridDim.x);
     }
 }
 
@@ -1952,9 +2012,10 @@ hipError_t mc_upload_luts() {
 static void launch_classify(const McParams &p, hipStream_t stream) {
     const int64_t nwords = p.X * p.Y * p.W;
     if (p.csg) {
-        const int64_t ntask = csg_classify_tasks(p.X, p.Y, p.W);
-        const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>(ntask, 16384));
-        hipLaunchKernelGGL(k_classify_csg, dim3(blocks), dim3(64), 0, stream, p);
+        const int64_t nb = csg_classify_tasks(p.X, p.Y, p.W);
+        const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nb + 255) / 256, 8192));
+        hipLaunchKernelGGL(k_csg_blocks, dim3(blocks), dim3(256), 0, stream, p);
+        hipLaunchKernelGGL(k_csg_fill, dim3(kCsgFillWaves), dim3(64), 0, stream, p);
     } else if (p.nsph > 0) {
         const int64_t ntask = ((p.X + 3) / 4) * ((p.Y + 3) / 4) * p.W;
         const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((ntask + 3) / 4, 8192));
@@ -2080,8 +2141,9 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     hipLaunchKernelGGL(k_classify_batch, dim3(cb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (h.ngtask > 0) {
-        const unsigned gb = (unsigned) std::min<int64_t>(h.ngtask, 16384);
-        hipLaunchKernelGGL(k_classify_csg_batch, dim3(gb), dim3(64), 0, stream, items, hdr);
+        const unsigned gb = (unsigned) std::min<int64_t>((h.ngtask + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_csg_blocks_batch, dim3(gb), dim3(256), 0, stream, items, hdr);
+        hipLaunchKernelGGL(k_csg_fill_batch, dim3(kCsgFillWaves), dim3(64), 0, stream, items, hdr);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const unsigned nb = (unsigned) std::max<int64_t>(1, h.nblk);

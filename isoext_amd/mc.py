@@ -24,8 +24,10 @@ from .grid import Grid, UniformGrid
 
 METHODS = {"nagae": 0, "lorensen": 1}
 
-# provisional output sizes seen last time for a grid shape (sizing hint)
+# provisional output sizes seen last time for a grid shape (sizing hint), and
+# the weld-candidate capacity a shape needed when above the default
 _size_hint: dict = {}
+_special_hint: dict = {}
 _hint_lock = threading.Lock()
 DEFAULT_SPECIAL_CAP = 1 << 16
 
@@ -73,7 +75,8 @@ def mc_dense_raw(values: torch.Tensor | None, shape, aabb_min, aabb_max, level: 
     if cfg is None:
         cfg = autotune.mc_config(shape)
     L = _lib.lib()
-    special_cap = DEFAULT_SPECIAL_CAP
+    with _hint_lock:
+        special_cap = max(DEFAULT_SPECIAL_CAP, _special_hint.get(shape, 0))
     cap_v, cap_f = _initial_caps(shape)
     counts = _lib.I64x6()
     shape_c, amin_c, amax_c = _lib.i64x3(shape), _lib.f32x3(aabb_min), _lib.f32x3(aabb_max)
@@ -119,6 +122,8 @@ def mc_dense_raw(values: torch.Tensor | None, shape, aabb_min, aabb_max, level: 
         raise _lib.IsoextError("marching_cubes: output sizing did not converge")
     with _hint_lock:
         _size_hint[tuple(shape)] = (max(int(counts[4]), 1), max(int(counts[5]), 1))
+        if special_cap > DEFAULT_SPECIAL_CAP:   # fields with many corner-exact crossings keep their need
+            _special_hint[tuple(shape)] = special_cap
     nv, nf = int(counts[0]), int(counts[1])
     v, f = out_v[:nv], out_f[:nf]
     # do not pin a much larger allocation behind a small mesh

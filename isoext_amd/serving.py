@@ -1,7 +1,9 @@
 """Continuous-batching extraction server on one GPU.
 
-Requests (marching cubes or dual contouring on a dense grid already in HBM)
-arrive at any time.  A request is *admitted* when its estimated device
+Requests (marching cubes or dual contouring on a dense grid already in HBM,
+or on the field of an SDF program -- csg.SdfProgram -- that the kernels
+evaluate where they need it, so the request's grid is never stored) arrive
+at any time.  A request is *admitted* when its estimated device
 footprint (mask workspace, count arrays, outputs) fits in the HBM budget
 that in-flight requests leave free; admitted marching-cubes requests are
 grouped by whichever worker becomes free next into one ragged batch
@@ -28,7 +30,7 @@ import torch
 @dataclass
 class Request:
     rid: int
-    values: torch.Tensor            # (X, Y, Z) float32 on the server's device
+    values: torch.Tensor | None     # (X, Y, Z) float32 on the server's device (None with sdf)
     kind: str = "mc"                # "mc" or "dc"
     level: float = 0.0
     method: int = 0                 # marching cubes: 0 nagae, 1 lorensen
@@ -39,16 +41,23 @@ class Request:
     result: tuple | None = None
     error: str | None = None
     done: threading.Event = field(default_factory=threading.Event)
+    sdf: object = None              # csg.SdfProgram on the server's device (values None)
+    shape: tuple | None = None      # grid points per axis of an sdf request
+
+    @property
+    def grid_shape(self) -> tuple:
+        return tuple(int(s) for s in (self.values.shape if self.sdf is None else self.shape))
 
     @property
     def points(self) -> int:
-        return int(self.values.numel())
+        X, Y, Z = self.grid_shape
+        return X * Y * Z
 
 
 def work_bytes(req: Request) -> int:
     """Device bytes of a request's intermediate state: the inside masks and
     count arrays (about N/8 + N/16 bytes) plus a fixed 1 MiB."""
-    n = int(req.values.numel())
+    n = req.points
     return n // 8 + n // 16 + (1 << 20)
 
 
@@ -56,7 +65,7 @@ def out_estimate(req: Request) -> int:
     """Output bytes assumed at admission, before the request's counts are
     known: a surface crossing every x-plane (12 B per vertex + 24 B per face,
     at most 3 vertices per cross-section cell; twice that for DC)."""
-    X, Y, Z = (int(s) for s in req.values.shape)
+    X, Y, Z = req.grid_shape
     out = 3 * max(X * Y, Y * Z, X * Z) * (12 + 24)
     return 2 * out if req.kind == "dc" else out
 
@@ -72,11 +81,19 @@ def validate(req: Request) -> str | None:
     """Why a request cannot run (None when it can): the checks of the
     per-call API, done at submission so that a bad request never joins a
     batch."""
-    v = req.values
-    if not isinstance(v, torch.Tensor) or v.dtype != torch.float32 or v.dim() != 3 or not v.is_cuda:
-        return "values must be a 3-D float32 CUDA tensor"
-    if not v.is_contiguous():
-        return "values must be C-contiguous"
+    if req.sdf is not None:
+        if req.values is not None:
+            return "a request holds values or an SDF program, not both"
+        if req.shape is None or len(req.shape) != 3 or any(int(s) < 1 for s in req.shape):
+            return "an SDF request needs a 3-entry positive shape"
+        if not hasattr(req.sdf, "ptr") or getattr(req.sdf, "device", None) is None:
+            return "sdf must be an isoext_amd.csg.SdfProgram"
+    else:
+        v = req.values
+        if not isinstance(v, torch.Tensor) or v.dtype != torch.float32 or v.dim() != 3 or not v.is_cuda:
+            return "values must be a 3-D float32 CUDA tensor"
+        if not v.is_contiguous():
+            return "values must be C-contiguous"
     if req.kind not in ("mc", "dc"):
         return f"unknown request kind {req.kind!r}"
     if req.kind == "mc" and req.method not in (0, 1):
@@ -251,7 +268,7 @@ class Server:
 
     def _run_mc(self, reqs):
         from .batch import BATCH_CFG, GridJob, mc_batch
-        jobs = [GridJob(r.values, r.level, r.method) for r in reqs]
+        jobs = [GridJob(r.values, r.level, r.method, sdf=r.sdf, shape=r.shape) for r in reqs]
         out = mc_batch(jobs, cfg=self.cfg if self.cfg is not None else BATCH_CFG, ws_kind="serving",
                        on_sizes=lambda b: self._sized(reqs, b))
         for r, (v, f) in zip(reqs, out):
@@ -262,7 +279,8 @@ class Server:
 
     def _run_dc(self, reqs):
         from .batch import GridJob, dc_batch
-        out = dc_batch([GridJob(r.values, r.level) for r in reqs], ws_kind="serving_dc",
+        # program requests take trilinear normals, as dual_contouring on their stored grid
+        out = dc_batch([GridJob(r.values, r.level, sdf=r.sdf, shape=r.shape) for r in reqs], ws_kind="serving_dc",
                        on_sizes=lambda b: self._sized(reqs, b))
         for r, res in zip(reqs, out):
             r.result = res
@@ -278,7 +296,8 @@ def run_plain(r: Request):
     from .dc import dual_contouring
     from .grid import UniformGrid
     from .mc import marching_cubes
-    g = UniformGrid.wrap(r.values)
+    # a program request: set_values(sdf(points)) first, as the reference's users do
+    g = UniformGrid.wrap(r.values if r.sdf is None else r.sdf.grid(r.shape))
     if r.kind == "dc":
         return dual_contouring(g, level=r.level)
     return marching_cubes(g, r.level, ["nagae", "lorensen"][r.method])
@@ -305,11 +324,30 @@ def make_pool(n_grids: int, seed: int = 0, side_lo: int = 32, side_hi: int = 512
     return pool
 
 
+def make_sdf_pool(n_programs: int, seed: int = 0, side_lo: int = 32, side_hi: int = 512, device="cuda"):
+    """Request fields as SDF programs, never stored: (program, shape) pairs
+    with the make_pool distribution (cubic sides log-uniform in [side_lo,
+    side_hi], unions of 1-64 random spheres) compiled from isoext.sdf trees."""
+    from . import sdf as S
+    from .csg import SdfProgram
+    rng = np.random.default_rng(seed)
+    pool = []
+    for _ in range(n_programs):
+        side = int(round(np.exp(rng.uniform(np.log(side_lo), np.log(side_hi)))))
+        k = int(rng.integers(1, 65))
+        c = rng.uniform(-0.8, 0.8, size=(k, 3)).astype(np.float32)
+        r = rng.uniform(0.05, 0.25, size=k).astype(np.float32)
+        tree = S.UnionOp([S.TranslationOp(S.SphereSDF(float(r[j])), [float(x) for x in c[j]]) for j in range(k)])
+        pool.append((SdfProgram(tree, device), (side, side, side)))
+    return pool
+
+
 def make_stream(pool, n: int, rate: float, seed: int = 0, dc_frac: float = 0.2, cycle: bool = False):
     """n requests, Poisson arrivals at `rate` per second (rate <= 0: all at
     t = 0), dc_frac of them dual contouring.  Grids are drawn from the pool
     at random, or with cycle=True in a fixed seeded permutation repeated, so
-    that any len(pool) consecutive requests use distinct grids."""
+    that any len(pool) consecutive requests use distinct grids.  Pool entries
+    are grids or (SdfProgram, shape) pairs (make_sdf_pool)."""
     rng = np.random.default_rng(seed + 1)
     t = np.cumsum(rng.exponential(1.0 / rate, size=n)) if rate > 0 else np.zeros(n)
     perm = np.random.default_rng(seed + 2).permutation(len(pool))
@@ -317,8 +355,12 @@ def make_stream(pool, n: int, rate: float, seed: int = 0, dc_frac: float = 0.2, 
     for i in range(n):
         kind = "dc" if rng.random() < dc_frac else "mc"
         g = int(perm[i % len(pool)]) if cycle else int(rng.integers(len(pool)))
-        reqs.append(Request(rid=i, values=pool[g], kind=kind,
-                            method=int(rng.integers(0, 2)) if kind == "mc" else 0, arrival=float(t[i])))
+        method = int(rng.integers(0, 2)) if kind == "mc" else 0
+        if isinstance(pool[g], tuple):
+            reqs.append(Request(rid=i, values=None, kind=kind, method=method, arrival=float(t[i]),
+                                sdf=pool[g][0], shape=pool[g][1]))
+        else:
+            reqs.append(Request(rid=i, values=pool[g], kind=kind, method=method, arrival=float(t[i])))
     return reqs
 
 

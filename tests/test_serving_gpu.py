@@ -49,6 +49,29 @@ def test_mix_equals_alone(gpu):
     assert np.all(lat >= 0)
 
 
+def test_program_requests_mix_equals_alone(gpu):
+    """Requests whose fields are SDF programs (never stored as grids) mixed
+    with stored-grid requests: each result equals the per-call path, which
+    writes the program's grid and extracts from it."""
+    import torch
+
+    from isoext_amd import serving
+    grids = serving.make_pool(6, seed=5, side_lo=16, side_hi=96)
+    progs = serving.make_sdf_pool(6, seed=6, side_lo=16, side_hi=120)
+    torch.cuda.synchronize()
+    reqs = serving.make_stream(grids + progs, 48, rate=3000.0, seed=8, dc_frac=0.25)
+    assert any(r.sdf is not None for r in reqs) and any(r.sdf is None for r in reqs)
+    assert any(r.sdf is not None and r.kind == "dc" for r in reqs)
+    srv = serving.Server(hbm_budget=200e9, max_batch=8, workers=2)
+    try:
+        serving.drive(srv, reqs)
+    finally:
+        srv.close()
+    assert all(r.error is None for r in reqs), [r.error for r in reqs if r.error][:3]
+    for r in reqs:
+        assert same(canon(r.result), canon(serving.run_plain(r))), (r.rid, r.kind, r.sdf is not None)
+
+
 def test_admission_under_pressure(gpu):
     from isoext_amd import serving
     # a budget that holds about one large request at a time
