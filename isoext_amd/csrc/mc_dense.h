@@ -168,10 +168,11 @@ constexpr int kMaxFusedSpheres = 256;
 __host__ __device__ inline int64_t csg_classify_tasks(int64_t X, int64_t Y, int64_t W) {
     return ((X + 3) / 4) * ((Y + 3) / 4) * W;
 }
-// The undecided-block list (uint32 block ids in cnt_v, X*Y*S entries) fits.
+// The undecided-block list fits: uint32 block ids in cnt_v, the count and
+// the blocks' quarter states in cnt_f (X*Y*S entries each).
 inline bool csg_list_fits(int64_t X, int64_t Y, int64_t W, int64_t S) {
     const int64_t nb = csg_classify_tasks(X, Y, W);
-    return nb <= X * Y * S && nb <= (int64_t) UINT32_MAX;
+    return nb + 1 <= X * Y * S && nb <= (int64_t) UINT32_MAX;
 }
 constexpr int kCsgFillWaves = 4096;   // phase-2 workgroups (one wave each)
 

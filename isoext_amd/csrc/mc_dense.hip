@@ -753,18 +753,24 @@ __device__ __forceinline__ double csg_coord_scale(const McParams &p) {
     return m;
 }
 
-// Sign of block k from its centre value (every lane of the wave evaluates
-// its own block, or all the same one).
-__device__ __forceinline__ int csg_block_sign(const McParams &p, const CsgProgram &P, const CsgBlock &k, double cs) {
-    const float x0 = grid_coord(k.xa, p.X, p.amin[0], p.amax[0]), x1 = grid_coord(k.xa + k.nx - 1, p.X, p.amin[0], p.amax[0]);
-    const float y0 = grid_coord(k.ya, p.Y, p.amin[1], p.amax[1]), y1 = grid_coord(k.ya + k.ny - 1, p.Y, p.amin[1], p.amax[1]);
-    const float z0 = grid_coord(k.za, p.Z, p.amin[2], p.amax[2]), z1 = grid_coord(k.za + k.nz - 1, p.Z, p.amin[2], p.amax[2]);
+// Sign of the box of grid points [xa, xa + nx) x [ya, ya + ny) x [za, za + nz)
+// from its centre value (every lane of the wave evaluates its own box, or
+// all the same one).
+__device__ __forceinline__ int csg_points_sign(const McParams &p, const CsgProgram &P, int64_t xa, int nx, int64_t ya,
+                                               int ny, int64_t za, int nz, double cs) {
+    const float x0 = grid_coord(xa, p.X, p.amin[0], p.amax[0]), x1 = grid_coord(xa + nx - 1, p.X, p.amin[0], p.amax[0]);
+    const float y0 = grid_coord(ya, p.Y, p.amin[1], p.amax[1]), y1 = grid_coord(ya + ny - 1, p.Y, p.amin[1], p.amax[1]);
+    const float z0 = grid_coord(za, p.Z, p.amin[2], p.amax[2]), z1 = grid_coord(za + nz - 1, p.Z, p.amin[2], p.amax[2]);
     const float cx = 0.5f * (x0 + x1), cy = 0.5f * (y0 + y1), cz = 0.5f * (z0 + z1);
     const double ex = fmax(fabs((double) cx - x0), fabs((double) x1 - cx));
     const double ey = fmax(fabs((double) cy - y0), fabs((double) y1 - cy));
     const double ez = fmax(fabs((double) cz - z0), fabs((double) z1 - cz));
     const float vc = csg_value<true>(P, cx, cy, cz);
     return csg_box_sign(P, vc, sqrt(ex * ex + ey * ey + ez * ez), cs, p.level);
+}
+
+__device__ __forceinline__ int csg_block_sign(const McParams &p, const CsgProgram &P, const CsgBlock &k, double cs) {
+    return csg_points_sign(p, P, k.xa, k.nx, k.ya, k.ny, k.za, k.nz, cs);
 }
 
 __device__ __forceinline__ void csg_write_block(const McParams &p, const CsgBlock &k, int sign) {
@@ -774,21 +780,53 @@ __device__ __forceinline__ void csg_write_block(const McParams &p, const CsgBloc
 }
 
 // Phase 1 for block b (valid: b is a block of the grid); all lanes call it.
+// A block undecided at its centre tests its four 16-point quarters (radius
+// about a quarter of the block's); if all four are decided its words are
+// written, else it is listed with the quarters' states (2 bits each: 0
+// undecided, 1 inside, 2 outside or empty) in cnt_f[1 + list index].
 __device__ void csg_test_block(const McParams &p, const CsgProgram &P, int64_t b, bool valid, double cs) {
     const CsgBlock k = csg_block(p, valid ? b : 0);
     const int sign = csg_block_sign(p, P, k, cs);
     if (valid && sign != 0) csg_write_block(p, k, sign);
-    const uint64_t und = __ballot(valid && sign == 0);
-    if (!p.csg_list || !und) return;
+    bool und = valid && sign == 0;
+    if (!__ballot(und)) return;
+    uint32_t qs = 0;
+    if (und) {
+        for (int q = 0; q < 4; ++q) {
+            const int nzq = min(16, k.nz - 16 * q);
+            const int sq = nzq > 0 ? csg_points_sign(p, P, k.xa, k.nx, k.ya, k.ny, k.za + 16 * q, nzq, cs) : 1;
+            qs |= (uint32_t) (sq < 0 ? 1 : (sq > 0 ? 2 : 0)) << (2 * q);
+        }
+        bool all = true;
+        uint64_t w = 0;
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t st = (qs >> (2 * q)) & 3u;
+            all = all && st != 0;
+            if (st == 1) w |= 0xffffull << (16 * q);
+        }
+        if (all) {
+            w &= low_bits(k.nz);
+            for (int r = 0; r < 16; ++r)
+                if ((r >> 2) < k.nx && (r & 3) < k.ny) p.bits[((k.xa + (r >> 2)) * p.Y + (k.ya + (r & 3))) * p.W + k.c] = w;
+            und = false;
+        }
+    }
+    const uint64_t lst = __ballot(und);
+    if (!p.csg_list || !lst) return;
     const int lane = threadIdx.x & 63;
     unsigned base = 0;
-    if (lane == 0) base = atomicAdd(p.cnt_f, (unsigned) __popcll(und));
+    if (lane == 0) base = atomicAdd(p.cnt_f, (unsigned) __popcll(lst));
     base = __shfl(base, 0);
-    if (valid && sign == 0) p.cnt_v[base + __popcll(und & ((1ull << lane) - 1ull))] = (uint32_t) b;
+    if (und) {
+        const unsigned idx = base + (unsigned) __popcll(lst & ((1ull << lane) - 1ull));
+        p.cnt_v[idx] = (uint32_t) b;
+        p.cnt_f[1 + idx] = qs;
+    }
 }
 
-// Phase 2 for an undecided block b (one wave).
-__device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b, double cs) {
+// Phase 2 for an undecided block b (one wave); qs: its quarters' states
+// from phase 1 (0: test every column).
+__device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b, uint32_t qs, double cs) {
     const int lane = threadIdx.x & 63;
     const CsgBlock k = csg_block(p, b);
     // columns: 4 x 4 rows at one z per lane
@@ -798,9 +836,14 @@ __device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b
     const double ex = fmax(fabs((double) cx - x0), fabs((double) x1 - cx));
     const double ey = fmax(fabs((double) cy - y0), fabs((double) y1 - cy));
     const bool zok = lane < k.nz;
-    const float pz = grid_coord(k.za + (zok ? lane : 0), p.Z, p.amin[2], p.amax[2]);
-    const float vs = csg_value<true>(P, cx, cy, pz);
-    const int s2 = zok ? csg_box_sign(P, vs, sqrt(ex * ex + ey * ey), cs, p.level) : 1;
+    const uint32_t qst = (qs >> (2 * (lane >> 4))) & 3u;   // this lane's quarter, decided in phase 1
+    int s2 = 1;
+    if (zok && qst != 0) {
+        s2 = qst == 1 ? -1 : 1;
+    } else if (zok) {
+        const float pz = grid_coord(k.za + lane, p.Z, p.amin[2], p.amax[2]);
+        s2 = csg_box_sign(P, csg_value<true>(P, cx, cy, pz), sqrt(ex * ex + ey * ey), cs, p.level);
+    }
     const uint64_t known_in = __ballot(zok && s2 < 0);
     const uint64_t need = __ballot(zok && s2 == 0);
     const int nn = __popcll(need);
@@ -827,11 +870,11 @@ __device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b
 __device__ void csg_fill(const McParams &p, const CsgProgram &P, double cs, int64_t first, int64_t stride) {
     if (p.csg_list) {
         const int64_t n = __builtin_amdgcn_readfirstlane((int) *(volatile uint32_t *) p.cnt_f);
-        for (int64_t i = first; i < n; i += stride) csg_fill_block(p, P, (int64_t) p.cnt_v[i], cs);
+        for (int64_t i = first; i < n; i += stride) csg_fill_block(p, P, (int64_t) p.cnt_v[i], p.cnt_f[1 + i], cs);
     } else {
         const int64_t nb = csg_classify_tasks(p.X, p.Y, p.W);
         for (int64_t b = first; b < nb; b += stride)
-            if (csg_block_sign(p, P, csg_block(p, b), cs) == 0) csg_fill_block(p, P, b, cs);
+            if (csg_block_sign(p, P, csg_block(p, b), cs) == 0) csg_fill_block(p, P, b, 0u, cs);
     }
 }
 
@@ -860,6 +903,21 @@ enum : int { kSrcGrid = 0, kSrcSpheres = 1, kSrcProgram = 2 };
 
 template <int SRC, int NZC>
 struct FusedLds {};
+
+// Program fields: the distinct corner points of an emit tile's active cells,
+// evaluated once each.  need[w]: bit b = point (px, py, 64 c + b) of point
+// word w = (px * (TY + 1) + py) * (NZC + 1) + c is a corner of an active
+// cell; pre: exclusive offsets of the words' points into val.
+template <int TX, int TY, int NZC, bool ON>
+struct CsgTileCache {};
+template <int TX, int TY, int NZC>
+struct CsgTileCache<TX, TY, NZC, true> {
+    static constexpr int NPW = (TX + 1) * (TY + 1) * (NZC + 1);
+    static constexpr int CAP = 4096;   // more distinct corners: each cell evaluates its own 8
+    uint64_t need[NPW];
+    uint32_t pre[NPW + 1];
+    float val[CAP];
+};
 template <int NZC>
 struct FusedLds<kSrcProgram, NZC> {
     CsgProgram prog;
@@ -1015,6 +1073,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
     using L = EmitLds<TX, TY, NZC, NT>;
     __shared__ L s;
     __shared__ FusedLds<SRC, NZC> fl;
+    __shared__ CsgTileCache<TX, TY, NZC, SRC == kSrcProgram> cache;
     const int tid = threadIdx.x;
     if (*(volatile int *) p.err & ISO_ERR_TOO_LARGE) return;
 
@@ -1099,6 +1158,47 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
     block_exscan_lds<L::NUNITS, NT>(s.na, s.tsum);
     const int nactive = (int) s.na[L::NUNITS];
     if (nactive == 0) return;
+
+    // ---- program fields: every corner point of the active cells evaluated once
+    bool cached = false;
+    if constexpr (SRC == kSrcProgram) {
+        using C = CsgTileCache<TX, TY, NZC, true>;
+        for (int pw = tid; pw < C::NPW; pw += NT) {
+            const int c = pw % (NZC + 1), r = pw / (NZC + 1), py = r % (TY + 1), px = r / (TY + 1);
+            uint64_t m = 0;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const int cx = px - (d >> 1), cy = py - (d & 1);
+                if (cx < 0 || cy < 0 || cx >= TX || cy >= TY) continue;
+                const int ub = (cx * TY + cy) * NZC;
+                if (c < NZC) {
+                    const uint64_t a = s.amask[ub + c];
+                    m |= a | (a << 1);
+                }
+                if (c > 0) m |= s.amask[ub + c - 1] >> 63;
+            }
+            cache.need[pw] = m;
+            cache.pre[pw] = (uint32_t) __popcll(m);
+        }
+        __syncthreads();
+        block_exscan_lds<C::NPW, NT>(cache.pre, s.tsum);
+        const int U = (int) cache.pre[C::NPW];
+        cached = U <= C::CAP;
+        if (cached) {
+            for (int j = tid; j < U; j += NT) {
+                int lo = 0, hi = C::NPW - 1;   // the word holding point j: first w with pre[w + 1] > j
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (cache.pre[mid + 1] > (uint32_t) j) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int c = lo % (NZC + 1), r = lo / (NZC + 1), py = r % (TY + 1), px = r / (TY + 1);
+                const int zl = 64 * c + nth_set_bit(cache.need[lo], j - (int) cache.pre[lo]);
+                cache.val[j] = csg_value<true>(fl.prog, s.cx[px], s.cy[py], s.cz[zl]);
+            }
+            __syncthreads();
+        }
+    }
 
     // ---- global bases: faces per cell row, vertices per (point row, chunk) ------
 #pragma unroll
@@ -1195,10 +1295,21 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             const int64_t gx = x0 + cx, gy = y0 + cy, gz = z0 + zl;
             CellCtx cc;
             if constexpr (SRC == kSrcProgram) {
-                const float xs[2] = {s.cx[cx], s.cx[cx + 1]}, ys[2] = {s.cy[cy], s.cy[cy + 1]};
-                const float zs[2] = {s.cz[zl], s.cz[zl + 1]};
+                if (cached) {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) cc.v[q] = csg_value<true>(fl.prog, xs[q >> 2], ys[(q >> 1) & 1], zs[q & 1]);
+                    for (int q = 0; q < 8; ++q) {
+                        const int px = cx + (q >> 2), py = cy + ((q >> 1) & 1), zq = zl + (q & 1);
+                        const int w = (px * (TY + 1) + py) * (NZC + 1) + (zq >> 6);
+                        const uint64_t below = (1ull << (zq & 63)) - 1ull;
+                        cc.v[q] = cache.val[cache.pre[w] + (uint32_t) __popcll(cache.need[w] & below)];
+                    }
+                } else {
+                    const float xs[2] = {s.cx[cx], s.cx[cx + 1]}, ys[2] = {s.cy[cy], s.cy[cy + 1]};
+                    const float zs[2] = {s.cz[zl], s.cz[zl + 1]};
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        cc.v[q] = csg_value<true>(fl.prog, xs[q >> 2], ys[(q >> 1) & 1], zs[q & 1]);
+                }
             } else if constexpr (SRC == kSrcSpheres) {
                 const float xs[2] = {s.cx[cx], s.cx[cx + 1]}, ys[2] = {s.cy[cy], s.cy[cy + 1]};
                 const float zs[2] = {s.cz[zl], s.cz[zl + 1]};

@@ -30,6 +30,7 @@ _size_hint: dict = {}
 _special_hint: dict = {}
 _hint_lock = threading.Lock()
 DEFAULT_SPECIAL_CAP = 1 << 16
+PROGRAM_CFG = 5   # (8, 8, 1) emit tiles for SDF-program fields
 
 
 def _method_id(method: str) -> int:
@@ -73,7 +74,9 @@ def mc_dense_raw(values: torch.Tensor | None, shape, aabb_min, aabb_max, level: 
         dev = values.device if values is not None else spheres.device
     shape = tuple(int(s) for s in shape)
     if cfg is None:
-        cfg = autotune.mc_config(shape)
+        # program fields: small tiles, so each tile's distinct corner points
+        # fit its LDS cache (every corner evaluated once)
+        cfg = PROGRAM_CFG if program is not None else autotune.mc_config(shape)
     L = _lib.lib()
     with _hint_lock:
         special_cap = max(DEFAULT_SPECIAL_CAP, _special_hint.get(shape, 0))
