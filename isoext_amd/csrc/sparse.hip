@@ -282,57 +282,17 @@ __device__ __forceinline__ void lower_off(int k, int &dx, int &dy, int &dz) {
 // copy in such a cell is compared by its two end values first.  A vertex
 // nearer an endpoint P can share its position with any edge through P in
 // the 8 cells around P, which are searched directly (rare).
-constexpr int kSmcStage = 2048;   // ids a workgroup stages per neighbour range
 
-// Slot of cell id `id` in the staged sorted range (base, m entries), or -1.
-__device__ __forceinline__ int64_t find_staged(const int64_t *a, int m, int64_t base, int64_t id) {
-    int lo = 0, hi = m;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < id) lo = mid + 1;
-        else hi = mid;
-    }
-    return (lo < m && a[lo] == id) ? base + lo : -1;
-}
 
 __global__ void __launch_bounds__(kB) k_smc_inst(SparseMcIn in, SLook L, const int64_t *bpre, SparseMcWork w) {
     __shared__ int64_t sh[kB];
-    // the ids of the columns this workgroup's cells can look up: range A =
-    // columns [colF - 1, colL] (same x), range B = columns colF - (Y-1) - 1
-    // .. colL - (Y-1) + 1 (x - 1); each a contiguous run of slots
-    __shared__ int64_t ida[kSmcStage], idb[kSmcStage];
-    __shared__ int64_t rng[6];   // A lo, A count (or -1), B lo, B count (or -1), colF, colL
-    const int64_t first = (int64_t) blockIdx.x * kB, last = min(in.n, first + kB) - 1;
-    if (threadIdx.x == 0) {
-        const int64_t colF = div_fast(in.ids[first], L.cz), colL = div_fast(in.ids[last], L.cz);
-        const int64_t alo = L.cf[colF >= 1 ? colF - 1 : 0], ahi = L.cf[colL + 1];
-        const int64_t b0 = colF - L.cy - 1, b1 = colL - L.cy + 2;
-        const int64_t blo = b1 > 0 ? L.cf[b0 > 0 ? b0 : 0] : 0, bhi = b1 > 0 ? L.cf[b1] : 0;
-        rng[0] = alo;
-        rng[1] = ahi - alo <= kSmcStage ? ahi - alo : -1;
-        rng[2] = blo;
-        rng[3] = bhi - blo <= kSmcStage ? bhi - blo : -1;
-        rng[4] = colF;
-        rng[5] = colL;
-    }
-    __syncthreads();
-    const int64_t alo = rng[0], an = rng[1], blo = rng[2], bn = rng[3], colF = rng[4], colL = rng[5];
-    for (int64_t k = threadIdx.x; k < an; k += kB) ida[k] = in.ids[alo + k];
-    for (int64_t k = threadIdx.x; k < bn; k += kB) idb[k] = in.ids[blo + k];
-    int64_t i = first + threadIdx.x;
+    const int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
     SCell s;
     bool act = i < in.n && load_cell(in, i, s);
     int st = act ? scross(s.cs) : 0;
-    int64_t off = bpre[blockIdx.x] + block_excl(__popc(st), sh);   // (synchronises: the staged ids are in)
+    int64_t off = bpre[blockIdx.x] + block_excl(__popc(st), sh);
     if (!act) return;
-    // a neighbour's slot: staged range if its column is there, else the column table
-    auto lookup = [&](int64_t xx, int64_t yy, int64_t zz) -> int64_t {
-        if (xx < 0 || yy < 0 || zz < 0 || xx >= L.cx || yy >= L.cy || zz >= L.cz) return -1;
-        const int64_t col = xx * L.cy + yy, id = col * L.cz + zz;
-        if (an >= 0 && col >= colF - 1 && col <= colL) return find_staged(ida, (int) an, alo, id);
-        if (bn >= 0 && col >= colF - L.cy - 1 && col <= colL - L.cy + 1) return find_staged(idb, (int) bn, blo, id);
-        return find_cell(L, xx, yy, zz);
-    };
+    auto lookup = [&](int64_t xx, int64_t yy, int64_t zz) -> int64_t { return find_cell(L, xx, yy, zz); };
     w.cell_inst[i] = off;
     int64_t x, y, z;
     cell_xyz(in.ids[i], in.g.Y, in.g.Z, x, y, z);
@@ -361,12 +321,42 @@ __global__ void __launch_bounds__(kB) k_smc_inst(SparseMcIn in, SLook L, const i
             if (k >= 0) need |= 1u << k;
         }
     }
+    // the lower neighbours: (0, 0, -1) is the previous slot if present; the
+    // others lie in four columns (y - 1; x - 1; x - 1, y -/+ 1), whose slot
+    // ranges come from the column table (short in a narrow band)
     int64_t nbs[9];
+    {
+        const int64_t col = x * L.cy + y;
+        const int64_t cc[4] = {col - 1, col - L.cy, col - L.cy - 1, col - L.cy + 1};
+        const bool ok[4] = {y > 0, x > 0, x > 0 && y > 0, x > 0 && y + 1 < L.cy};
+        int64_t lo[4], hi[4];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        int dx, dy, dz;
-        lower_off(k, dx, dy, dz);
-        nbs[k] = ((need >> k) & 1u) ? lookup(x + dx, y + dy, z + dz) : -1;
+        for (int r = 0; r < 4; ++r) {
+            const bool want = ok[r] && (need & (r == 0 ? 0x1au : (r == 1 ? 0x64u : (r == 2 ? 0x80u : 0x100u))));
+            lo[r] = want ? (int64_t) L.cf[cc[r]] : 0;
+            hi[r] = want ? (int64_t) L.cf[cc[r] + 1] : 0;
+        }
+        const int64_t id = col * L.cz + z;
+        nbs[0] = ((need & 1u) && z > 0 && i > 0 && L.ids[i - 1] == id - 1) ? i - 1 : -1;
+#pragma unroll
+        for (int k = 1; k < 9; ++k) {
+            int dx, dy, dz;
+            lower_off(k, dx, dy, dz);
+            const int r = (k == 1 || k == 3 || k == 4) ? 0 : ((k == 2 || k == 5 || k == 6) ? 1 : (k == 7 ? 2 : 3));
+            const int64_t zz = z + dz;
+            int64_t slot = -1;
+            if (((need >> k) & 1u) && zz >= 0 && zz < L.cz) {
+                const int64_t key = cc[r] * L.cz + zz;
+                int64_t a = lo[r], b = hi[r];
+                while (a < b) {
+                    const int64_t m = (a + b) >> 1;
+                    if (L.ids[m] < key) a = m + 1;
+                    else b = m;
+                }
+                slot = (a < hi[r] && L.ids[a] == key) ? a : -1;
+            }
+            nbs[k] = slot;
+        }
     }
     for (int e = 0; e < 12; ++e) {
         if (!((st >> e) & 1)) continue;
