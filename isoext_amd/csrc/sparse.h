@@ -22,7 +22,7 @@ struct SparseMcIn {
 };
 
 struct SparseMcWork {
-    uint16_t *cst;        // crossing-edge mask of each cell
+    uint32_t *cst;        // per cell: crossing-edge mask (bits 0-11), case index (bits 16-23)
     int64_t *cell_inst;   // first instance of each active cell
     float *ipos;          // instance positions
     int64_t *isrc;        // slot * 12 + edge of each instance

@@ -111,7 +111,7 @@ __device__ __forceinline__ int scross(int cs) {
 
 // Pass 1: crossing-edge instances per cell -> block sums; optionally each
 // cell's crossing-edge mask.
-__global__ void k_smc_count(SparseMcIn in, int64_t *bsum, uint16_t *cst) {
+__global__ void k_smc_count(SparseMcIn in, int64_t *bsum, uint32_t *cst) {
     int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
     int64_t k = 0;
     if (i < in.n) {
@@ -120,7 +120,7 @@ __global__ void k_smc_count(SparseMcIn in, int64_t *bsum, uint16_t *cst) {
         for (int j = 0; j < 8; ++j) cs |= inside(in.values[8 * i + j], in.level) << j;
         const int st = (cs != 0 && cs != 255) ? scross(cs) : 0;
         k = __popc(st);
-        if (cst) cst[i] = (uint16_t) st;
+        if (cst) cst[i] = (uint32_t) st | ((uint32_t) cs << 16);
     }
     __shared__ int64_t sh[kB];
     sh[threadIdx.x] = k;
@@ -284,13 +284,20 @@ __device__ __forceinline__ void lower_off(int k, int &dx, int &dy, int &dz) {
 // the 8 cells around P, which are searched directly (rare).
 
 
-__global__ void __launch_bounds__(kB) k_smc_inst(SparseMcIn in, SLook L, const int64_t *bpre, SparseMcWork w) {
+// Workgroups are dispatched round-robin over the 8 XCDs; block b of the
+// launch takes cell block (b % 8) * per + b / 8, so that each XCD walks one
+// contiguous eighth of the slots and finds the previous x-slab's cells (the
+// x - 1 neighbours, about one slab of slots back) in its own L2.
+__global__ void __launch_bounds__(kB) k_smc_inst(SparseMcIn in, SLook L, const int64_t *bpre, SparseMcWork w,
+                                                 int64_t nblk) {
     __shared__ int64_t sh[kB];
-    const int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
+    const int64_t per = (nblk + 7) / 8, blk = (int64_t) (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (blk >= nblk) return;
+    const int64_t i = blk * kB + threadIdx.x;
     SCell s;
     bool act = i < in.n && load_cell(in, i, s);
     int st = act ? scross(s.cs) : 0;
-    int64_t off = bpre[blockIdx.x] + block_excl(__popc(st), sh);
+    int64_t off = bpre[blk] + block_excl(__popc(st), sh);
     if (!act) return;
     auto lookup = [&](int64_t xx, int64_t yy, int64_t zz) -> int64_t { return find_cell(L, xx, yy, zz); };
     w.cell_inst[i] = off;
@@ -428,7 +435,7 @@ __device__ __forceinline__ int64_t inst_of(int st, int e, int64_t base) {
 // Instance index of a representative key.
 __device__ __forceinline__ int64_t inst_of_key(const SparseMcWork &w, int64_t key) {
     const int64_t slot = key / 12;
-    return inst_of((int) w.cst[slot], (int) (key - slot * 12), w.cell_inst[slot]);
+    return inst_of((int) (w.cst[slot] & 0xfffu), (int) (key - slot * 12), w.cell_inst[slot]);
 }
 
 // Pass 3: kept faces per cell (a triangle with two corners at one position is
@@ -437,11 +444,10 @@ __global__ void k_smc_faces(SparseMcIn in, SparseMcWork w, int64_t *bsum) {
     int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
     int64_t nf = 0;
     if (i < in.n) {
-        const int st = (int) w.cst[i];
+        const uint32_t cw = w.cst[i];
+        const int st = (int) (cw & 0xfffu);
         if (st) {
-            int cs = 0;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) cs |= inside(in.values[8 * i + j], in.level) << j;
+            const int cs = (int) (cw >> 16);
             const int64_t base = w.cell_inst[i];
             const uint64_t tw = cSTris[in.method][cs];
             const int nt = (int) (tw & 15);
@@ -497,12 +503,8 @@ __global__ void k_smc_emit(SparseMcIn in, SparseMcWork w, const int64_t *fpre, c
                            const int64_t *wpre, SparseMcOut o) {
     __shared__ int64_t sh[kB];
     int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
-    const int st = i < in.n ? (int) w.cst[i] : 0;
-    int cs = 0;
-    if (st) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) cs |= inside(in.values[8 * i + j], in.level) << j;
-    }
+    const uint32_t cw = i < in.n ? w.cst[i] : 0u;
+    const int st = (int) (cw & 0xfffu), cs = (int) (cw >> 16);
     const int64_t base = st ? w.cell_inst[i] : 0;
     const uint64_t tw = st ? cSTris[in.method][cs] : 0;
     const int nt = (int) (tw & 15);
@@ -782,7 +784,7 @@ size_t sparse_mc_work_bytes(int64_t n, int64_t ninst) {
     const int64_t nw = (ninst + 31) / 32 + 1, nblk = nb(n);
     size_t b = 0;
     auto add = [&](size_t x) { b += (x + 255) & ~size_t(255); };
-    add(2 * n);                              // cst
+    add(4 * n);                              // cst
     add(8 * n);                              // cell_inst
     add(12 * ninst + 12);                    // ipos
     add(8 * ninst + 8);                      // isrc
@@ -813,7 +815,7 @@ hipError_t sparse_mc_count(const SparseMcIn &in, int64_t *scratch, int64_t &nins
     ninst = 0;
     if (in.n == 0) return hipSuccess;
     int64_t nblk = nb(in.n);
-    hipLaunchKernelGGL(k_smc_count, dim3((unsigned) nblk), dim3(kB), 0, st, in, scratch, (uint16_t *) nullptr);
+    hipLaunchKernelGGL(k_smc_count, dim3((unsigned) nblk), dim3(kB), 0, st, in, scratch, (uint32_t *) nullptr);
     hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, st, (const int64_t *) scratch, nblk, scratch + nblk);
     ISO_TRY(hipGetLastError());
     ISO_TRY(hipMemcpyAsync(&ninst, scratch + nblk, sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -850,7 +852,7 @@ hipError_t sparse_mc(const SparseMcIn &in, int64_t ninst, void *work, size_t wor
         return r;
     };
     SparseMcWork w;
-    w.cst = (uint16_t *) take(2 * in.n);
+    w.cst = (uint32_t *) take(4 * in.n);
     w.cell_inst = (int64_t *) take(8 * in.n);
     w.ipos = (float *) take(12 * ninst + 12);
     w.isrc = (int64_t *) take(8 * ninst + 8);
@@ -895,7 +897,8 @@ hipError_t sparse_mc(const SparseMcIn &in, int64_t ninst, void *work, size_t wor
     hipLaunchKernelGGL(k_cols_apply, dim3((unsigned) nchunk), dim3(1024), 0, st, cf, ncf, (const int32_t *) cmin);
     hipLaunchKernelGGL(k_smc_count, dim3((unsigned) nblk), dim3(kB), 0, st, in, ibs, w.cst);
     scan_i64(ibs, nblk, ich, misc + 2, st);
-    hipLaunchKernelGGL(k_smc_inst, dim3((unsigned) nblk), dim3(kB), 0, st, in, L, ibs, w);
+    hipLaunchKernelGGL(k_smc_inst, dim3((unsigned) (8 * ((nblk + 7) / 8))), dim3(kB), 0, st, in, L, ibs, w,
+                       (int64_t) nblk);
     hipLaunchKernelGGL(k_smc_faces, dim3((unsigned) nblk), dim3(kB), 0, st, in, w, fbs);
     scan_i64(fbs, nblk, fch, misc + 0, st);
     hipLaunchKernelGGL(k_smc_vbits, dim3(nb(ninst)), dim3(kB), 0, st, w, ninst, bits);
