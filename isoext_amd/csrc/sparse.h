@@ -24,6 +24,7 @@ struct SparseMcIn {
 struct SparseMcWork {
     uint32_t *cst;        // per cell: crossing-edge mask (bits 0-11), case index (bits 16-23)
     int64_t *cell_inst;   // first instance of each active cell
+    int32_t *inst_cell;   // cell slot of each instance
     float *ipos;          // instance positions
     int64_t *isrc;        // slot * 12 + edge of each instance
     int64_t *rkey;        // slot * 12 + edge of its representative (first instance at its position)

@@ -284,148 +284,121 @@ __device__ __forceinline__ void lower_off(int k, int &dx, int &dy, int &dz) {
 // the 8 cells around P, which are searched directly (rare).
 
 
-// Workgroups are dispatched round-robin over the 8 XCDs; block b of the
-// launch takes cell block (b % 8) * per + b / 8, so that each XCD walks one
-// contiguous eighth of the slots and finds the previous x-slab's cells (the
-// x - 1 neighbours, about one slab of slots back) in its own L2.
-__global__ void __launch_bounds__(kB) k_smc_inst(SparseMcIn in, SLook L, const int64_t *bpre, SparseMcWork w,
-                                                 int64_t nblk) {
+// Pass 2a: each cell's first instance (block scan of its crossing-edge
+// count) and, per instance, its cell.
+__global__ void __launch_bounds__(kB) k_smc_cells(int64_t n, const int64_t *bpre, SparseMcWork w) {
     __shared__ int64_t sh[kB];
-    const int64_t per = (nblk + 7) / 8, blk = (int64_t) (blockIdx.x % 8) * per + blockIdx.x / 8;
-    if (blk >= nblk) return;
-    const int64_t i = blk * kB + threadIdx.x;
-    SCell s;
-    bool act = i < in.n && load_cell(in, i, s);
-    int st = act ? scross(s.cs) : 0;
-    int64_t off = bpre[blk] + block_excl(__popc(st), sh);
-    if (!act) return;
-    auto lookup = [&](int64_t xx, int64_t yy, int64_t zz) -> int64_t { return find_cell(L, xx, yy, zz); };
+    const int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
+    const int st = i < n ? (int) (w.cst[i] & 0xfffu) : 0;
+    const int64_t off = bpre[blockIdx.x] + block_excl(__popc(st), sh);
+    if (!st) return;
     w.cell_inst[i] = off;
+    for (int j = 0; j < __popc(st); ++j) w.inst_cell[off + j] = (int32_t) i;
+}
+
+// Slot of the cell with id `key` among the slots [lo, hi) of its column, or -1.
+__device__ __forceinline__ int64_t col_find(const int64_t *ids, int64_t lo, int64_t hi, int64_t key) {
+    int64_t a = lo, b = hi;
+    while (a < b) {
+        const int64_t m = (a + b) >> 1;
+        if (ids[m] < key) a = m + 1;
+        else b = m;
+    }
+    return (a < hi && ids[a] == key) ? a : -1;
+}
+
+// Pass 2b: one lane per instance (cell slot i, its j-th crossing edge e):
+// position and representative key (slot * 12 + edge of the first instance
+// in (slot, edge) order with the same position).  A vertex more than 16 ulps
+// from both endpoints along its edge can only share its position with the
+// same grid edge in the (at most 3) other cells around that edge, and only
+// those before this cell can lower the key; their slots come from the
+// column table (the previous slot for z - 1, a short search in a
+// neighbouring column otherwise), and an edge's copy is compared by its two
+// end values first.  A vertex nearer an endpoint P can share its position
+// with any edge through P in the 8 cells around P (rare).
+__global__ void __launch_bounds__(kB) k_smc_inst(SparseMcIn in, SLook L, SparseMcWork w, int64_t ninst) {
+    const int64_t q = (int64_t) blockIdx.x * kB + threadIdx.x;
+    if (q >= ninst) return;
+    const int64_t i = w.inst_cell[q];
+    const int st = (int) (w.cst[i] & 0xfffu);
+    int e = 0;
+    {
+        int r = (int) (q - w.cell_inst[i]);
+        uint32_t m = (uint32_t) st;
+        while (r-- > 0) m &= m - 1;
+        e = __ffs(m) - 1;
+    }
+    SCell s;
+    load_cell(in, i, s);
+    float px, py, pz;
+    spos(s, in.level, e, px, py, pz);
     int64_t x, y, z;
     cell_xyz(in.ids[i], in.g.Y, in.g.Z, x, y, z);
-    const int64_t o[3] = {x, y, z};
-    // candidacy, and the lower neighbours the other edges need
-    uint32_t candm = 0, need = 0;
-    for (int e = 0; e < 12; ++e) {
-        if (!((st >> e) & 1)) continue;
-        float p3[3];
-        spos(s, in.level, e, p3[0], p3[1], p3[2]);
-        const int ca = edge_a(e), cb = edge_b(e), ax = edge_axis(e);
-        const int sa = ax == 0 ? (ca >> 2) : (ax == 1 ? ((ca >> 1) & 1) : (ca & 1));
-        const int sb = ax == 0 ? (cb >> 2) : (ax == 1 ? ((cb >> 1) & 1) : (cb & 1));
-        const float ra = ax == 0 ? p3[0] : (ax == 1 ? p3[1] : p3[2]);
-        if (near_endpoint(ra, s.c[ax][sa], s.c[ax][sb])) {
-            candm |= 1u << e;
-            continue;
-        }
+    const int ca = edge_a(e), cb = edge_b(e), ax = edge_axis(e);
+    const int sa = ax == 0 ? (ca >> 2) : (ax == 1 ? ((ca >> 1) & 1) : (ca & 1));
+    const int sb = ax == 0 ? (cb >> 2) : (ax == 1 ? ((cb >> 1) & 1) : (cb & 1));
+    const float ra = ax == 0 ? px : (ax == 1 ? py : pz);
+    int64_t rep = i * 12 + e;
+    if (!near_endpoint(ra, s.c[ax][sa], s.c[ax][sb])) {
         const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
         const int bu = (ca >> (2 - u)) & 1, bv = (ca >> (2 - v)) & 1;
-        for (int d = 0; d < 4; ++d) {
-            int dd[3] = {0, 0, 0};
-            dd[u] = bu - (d & 1);
-            dd[v] = bv - (d >> 1);
-            const int k = lower_nb(dd[0], dd[1], dd[2]);
-            if (k >= 0) need |= 1u << k;
-        }
-    }
-    // the lower neighbours: (0, 0, -1) is the previous slot if present; the
-    // others lie in four columns (y - 1; x - 1; x - 1, y -/+ 1), whose slot
-    // ranges come from the column table (short in a narrow band)
-    int64_t nbs[9];
-    {
         const int64_t col = x * L.cy + y;
-        const int64_t cc[4] = {col - 1, col - L.cy, col - L.cy - 1, col - L.cy + 1};
-        const bool ok[4] = {y > 0, x > 0, x > 0 && y > 0, x > 0 && y + 1 < L.cy};
-        int64_t lo[4], hi[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const bool want = ok[r] && (need & (r == 0 ? 0x1au : (r == 1 ? 0x64u : (r == 2 ? 0x80u : 0x100u))));
-            lo[r] = want ? (int64_t) L.cf[cc[r]] : 0;
-            hi[r] = want ? (int64_t) L.cf[cc[r] + 1] : 0;
-        }
-        const int64_t id = col * L.cz + z;
-        nbs[0] = ((need & 1u) && z > 0 && i > 0 && L.ids[i - 1] == id - 1) ? i - 1 : -1;
-#pragma unroll
-        for (int k = 1; k < 9; ++k) {
-            int dx, dy, dz;
-            lower_off(k, dx, dy, dz);
-            const int r = (k == 1 || k == 3 || k == 4) ? 0 : ((k == 2 || k == 5 || k == 6) ? 1 : (k == 7 ? 2 : 3));
-            const int64_t zz = z + dz;
-            int64_t slot = -1;
-            if (((need >> k) & 1u) && zz >= 0 && zz < L.cz) {
-                const int64_t key = cc[r] * L.cz + zz;
-                int64_t a = lo[r], b = hi[r];
-                while (a < b) {
-                    const int64_t m = (a + b) >> 1;
-                    if (L.ids[m] < key) a = m + 1;
-                    else b = m;
-                }
-                slot = (a < hi[r] && L.ids[a] == key) ? a : -1;
+        for (int d = 0; d < 4; ++d) {
+            const int du = d & 1, dv = d >> 1;
+            int dd[3] = {0, 0, 0};
+            dd[u] = bu - du;
+            dd[v] = bv - dv;
+            if (lower_nb(dd[0], dd[1], dd[2]) < 0) continue;   // this cell, or a later one
+            const int64_t xx = x + dd[0], yy = y + dd[1], zz = z + dd[2];
+            if (xx < 0 || yy < 0 || zz < 0 || yy >= L.cy || zz >= L.cz) continue;
+            int64_t s2;
+            if (dd[0] == 0 && dd[1] == 0) {   // (0, 0, -1): the previous slot
+                s2 = (i > 0 && in.ids[i - 1] == in.ids[i] - 1) ? i - 1 : -1;
+            } else {
+                const int64_t c2 = col + dd[0] * L.cy + dd[1];
+                s2 = col_find(L.ids, L.cf[c2], L.cf[c2 + 1], c2 * L.cz + zz);
             }
-            nbs[k] = slot;
+            if (s2 < 0) continue;
+            const int cc = (du << (2 - u)) | (dv << (2 - v));
+            const int e2 = edge_of(cc, ax);
+            const float va = in.values[8 * s2 + cc], vb = in.values[8 * s2 + (cc | (4 >> ax))];
+            bool same;
+            if (va == s.cv[ca] && vb == s.cv[cb]) {
+                same = true;   // the same end values: the same vertex
+            } else {
+                if (inside(va, in.level) == inside(vb, in.level)) continue;
+                const float t = edge_t(va, vb, in.level);
+                const float qx = lerp1(t, s.c[0][ca >> 2], s.c[0][cb >> 2]);
+                const float qy = lerp1(t, s.c[1][(ca >> 1) & 1], s.c[1][(cb >> 1) & 1]);
+                const float qz = lerp1(t, s.c[2][ca & 1], s.c[2][cb & 1]);
+                same = qx == px && qy == py && qz == pz;
+            }
+            if (same) rep = min(rep, s2 * 12 + e2);
+        }
+    } else {
+        // the endpoint it is near, and the 3 edges through it in each cell around it
+        const int64_t kr = ordered_key(ra), ka = ordered_key(s.c[ax][sa]), kb2 = ordered_key(s.c[ax][sb]);
+        const int pc = (llabs(kr - ka) <= llabs(kr - kb2)) ? ca : cb;
+        const int64_t P[3] = {x + (pc >> 2), y + ((pc >> 1) & 1), z + (pc & 1)};
+        for (int d = 0; d < 8; ++d) {
+            const int64_t c0 = P[0] - (d >> 2), c1 = P[1] - ((d >> 1) & 1), c2 = P[2] - (d & 1);
+            const int64_t s2 = find_cell(L, c0, c1, c2);
+            if (s2 < 0 || s2 > i) continue;
+            for (int a2 = 0; a2 < 3; ++a2) {
+                const int e2 = edge_of(d & ~(4 >> a2), a2);
+                if (s2 == i && e2 == e) continue;
+                float qx, qy, qz;
+                if (inst_pos(in, s2, c0, c1, c2, e2, qx, qy, qz) && qx == px && qy == py && qz == pz)
+                    rep = min(rep, s2 * 12 + e2);
+            }
         }
     }
-    for (int e = 0; e < 12; ++e) {
-        if (!((st >> e) & 1)) continue;
-        float px, py, pz;
-        spos(s, in.level, e, px, py, pz);
-        int64_t rep = i * 12 + e;
-        const int ca = edge_a(e), cb = edge_b(e), ax = edge_axis(e);
-        if (!((candm >> e) & 1u)) {
-            const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
-            const int bu = (ca >> (2 - u)) & 1, bv = (ca >> (2 - v)) & 1;
-            for (int d = 0; d < 4; ++d) {
-                const int du = d & 1, dv = d >> 1;
-                int dd[3] = {0, 0, 0};
-                dd[u] = bu - du;
-                dd[v] = bv - dv;
-                const int k = lower_nb(dd[0], dd[1], dd[2]);
-                if (k < 0 || nbs[k] < 0) continue;
-                const int64_t s2 = nbs[k];
-                const int cc = (du << (2 - u)) | (dv << (2 - v));
-                const int e2 = edge_of(cc, ax);
-                const float va = in.values[8 * s2 + cc], vb = in.values[8 * s2 + (cc | (4 >> ax))];
-                bool same;
-                if (va == s.cv[ca] && vb == s.cv[cb]) {
-                    same = true;   // the same end values: the same vertex
-                } else {
-                    if (inside(va, in.level) == inside(vb, in.level)) continue;
-                    const float t = edge_t(va, vb, in.level);
-                    const float qx = lerp1(t, s.c[0][ca >> 2], s.c[0][cb >> 2]);
-                    const float qy = lerp1(t, s.c[1][(ca >> 1) & 1], s.c[1][(cb >> 1) & 1]);
-                    const float qz = lerp1(t, s.c[2][ca & 1], s.c[2][cb & 1]);
-                    same = qx == px && qy == py && qz == pz;
-                }
-                if (same) rep = min(rep, s2 * 12 + e2);
-            }
-        } else {
-            // the endpoint it is near, and the 3 edges through it in each cell around it
-            const int sa = ax == 0 ? (ca >> 2) : (ax == 1 ? ((ca >> 1) & 1) : (ca & 1));
-            const int sb = ax == 0 ? (cb >> 2) : (ax == 1 ? ((cb >> 1) & 1) : (cb & 1));
-            const float ra = ax == 0 ? px : (ax == 1 ? py : pz);
-            const int64_t kr = ordered_key(ra), ka = ordered_key(s.c[ax][sa]), kb2 = ordered_key(s.c[ax][sb]);
-            const int pc = (llabs(kr - ka) <= llabs(kr - kb2)) ? ca : cb;
-            const int64_t P[3] = {o[0] + (pc >> 2), o[1] + ((pc >> 1) & 1), o[2] + (pc & 1)};
-            for (int d = 0; d < 8; ++d) {
-                const int64_t c0 = P[0] - (d >> 2), c1 = P[1] - ((d >> 1) & 1), c2 = P[2] - (d & 1);
-                const int64_t s2 = lookup(c0, c1, c2);
-                if (s2 < 0 || s2 > i) continue;
-                for (int a2 = 0; a2 < 3; ++a2) {
-                    const int e2 = edge_of(d & ~(4 >> a2), a2);
-                    if (s2 == i && e2 == e) continue;
-                    float qx, qy, qz;
-                    if (inst_pos(in, s2, c0, c1, c2, e2, qx, qy, qz) && qx == px && qy == py && qz == pz)
-                        rep = min(rep, s2 * 12 + e2);
-                }
-            }
-        }
-        w.ipos[3 * off + 0] = px;
-        w.ipos[3 * off + 1] = py;
-        w.ipos[3 * off + 2] = pz;
-        w.isrc[off] = i * 12 + e;
-        w.rkey[off] = rep;
-        ++off;
-    }
+    w.ipos[3 * q + 0] = px;
+    w.ipos[3 * q + 1] = py;
+    w.ipos[3 * q + 2] = pz;
+    w.isrc[q] = i * 12 + e;
+    w.rkey[q] = rep;
 }
 
 __device__ __forceinline__ int64_t inst_of(int st, int e, int64_t base) {
@@ -786,6 +759,7 @@ size_t sparse_mc_work_bytes(int64_t n, int64_t ninst) {
     auto add = [&](size_t x) { b += (x + 255) & ~size_t(255); };
     add(4 * n);                              // cst
     add(8 * n);                              // cell_inst
+    add(4 * ninst + 8);                      // inst_cell
     add(12 * ninst + 12);                    // ipos
     add(8 * ninst + 8);                      // isrc
     add(8 * ninst + 8);                      // rkey
@@ -854,6 +828,7 @@ hipError_t sparse_mc(const SparseMcIn &in, int64_t ninst, void *work, size_t wor
     SparseMcWork w;
     w.cst = (uint32_t *) take(4 * in.n);
     w.cell_inst = (int64_t *) take(8 * in.n);
+    w.inst_cell = (int32_t *) take(4 * ninst + 8);
     w.ipos = (float *) take(12 * ninst + 12);
     w.isrc = (int64_t *) take(8 * ninst + 8);
     w.rkey = (int64_t *) take(8 * ninst + 8);
@@ -897,8 +872,8 @@ hipError_t sparse_mc(const SparseMcIn &in, int64_t ninst, void *work, size_t wor
     hipLaunchKernelGGL(k_cols_apply, dim3((unsigned) nchunk), dim3(1024), 0, st, cf, ncf, (const int32_t *) cmin);
     hipLaunchKernelGGL(k_smc_count, dim3((unsigned) nblk), dim3(kB), 0, st, in, ibs, w.cst);
     scan_i64(ibs, nblk, ich, misc + 2, st);
-    hipLaunchKernelGGL(k_smc_inst, dim3((unsigned) (8 * ((nblk + 7) / 8))), dim3(kB), 0, st, in, L, ibs, w,
-                       (int64_t) nblk);
+    hipLaunchKernelGGL(k_smc_cells, dim3((unsigned) nblk), dim3(kB), 0, st, in.n, (const int64_t *) ibs, w);
+    hipLaunchKernelGGL(k_smc_inst, dim3(nb(ninst)), dim3(kB), 0, st, in, L, w, ninst);
     hipLaunchKernelGGL(k_smc_faces, dim3((unsigned) nblk), dim3(kB), 0, st, in, w, fbs);
     scan_i64(fbs, nblk, fch, misc + 0, st);
     hipLaunchKernelGGL(k_smc_vbits, dim3(nb(ninst)), dim3(kB), 0, st, w, ninst, bits);
