@@ -420,20 +420,23 @@ __global__ void k_smc_faces(SparseMcIn in, SparseMcWork w, int64_t *bsum) {
         const uint32_t cw = w.cst[i];
         const int st = (int) (cw & 0xfffu);
         if (st) {
-            const int cs = (int) (cw >> 16);
+            const int cs = (int) ((cw >> 16) & 0xffu);
             const int64_t base = w.cell_inst[i];
             const uint64_t tw = cSTris[in.method][cs];
             const int nt = (int) (tw & 15);
+            uint32_t kept = 0;
             for (int q = 0; q < nt; ++q) {
                 const int64_t r0 = w.rkey[inst_of(st, (int) ((tw >> (4 + 12 * q)) & 15), base)];
                 const int64_t r1 = w.rkey[inst_of(st, (int) ((tw >> (8 + 12 * q)) & 15), base)];
                 const int64_t r2 = w.rkey[inst_of(st, (int) ((tw >> (12 + 12 * q)) & 15), base)];
                 if (r0 == r1 || r0 == r2 || r1 == r2) continue;
                 ++nf;
+                kept |= 1u << q;
                 w.used[inst_of_key(w, r0)] = 1;
                 w.used[inst_of_key(w, r1)] = 1;
                 w.used[inst_of_key(w, r2)] = 1;
             }
+            w.cst[i] = cw | (kept << 24);   // kept triangles for the emit pass
         }
     }
     __shared__ int64_t sh[kB];
@@ -471,31 +474,23 @@ __global__ void k_smc_vid(SparseMcWork w, int64_t ninst, const uint32_t *bits, c
     w.vid[q] = (int32_t) rank_of(bits, wpre, r);
 }
 
-// Pass 4: faces in (slot, LUT slot) order, representatives write vertices.
-__global__ void k_smc_emit(SparseMcIn in, SparseMcWork w, const int64_t *fpre, const uint32_t *bits,
-                           const int64_t *wpre, SparseMcOut o) {
+// Pass 4: faces in (slot, LUT slot) order (the kept triangles recorded by
+// the faces pass), corners as vertex ids of their representatives.
+__global__ void k_smc_emit(SparseMcIn in, SparseMcWork w, const int64_t *fpre, SparseMcOut o) {
     __shared__ int64_t sh[kB];
     int64_t i = (int64_t) blockIdx.x * kB + threadIdx.x;
     const uint32_t cw = i < in.n ? w.cst[i] : 0u;
-    const int st = (int) (cw & 0xfffu), cs = (int) (cw >> 16);
-    const int64_t base = st ? w.cell_inst[i] : 0;
-    const uint64_t tw = st ? cSTris[in.method][cs] : 0;
-    const int nt = (int) (tw & 15);
-    int64_t nf = 0;
-    for (int q = 0; q < nt; ++q) {
-        const int64_t r0 = w.rkey[inst_of(st, (int) ((tw >> (4 + 12 * q)) & 15), base)];
-        const int64_t r1 = w.rkey[inst_of(st, (int) ((tw >> (8 + 12 * q)) & 15), base)];
-        const int64_t r2 = w.rkey[inst_of(st, (int) ((tw >> (12 + 12 * q)) & 15), base)];
-        if (!(r0 == r1 || r0 == r2 || r1 == r2)) ++nf;
-    }
-    int64_t f = fpre[blockIdx.x] + block_excl(nf, sh);
-    if (!st) return;
-    for (int q = 0; q < nt; ++q) {
+    const int st = (int) (cw & 0xfffu), cs = (int) ((cw >> 16) & 0xffu);
+    const uint32_t kept = st ? (cw >> 24) & 31u : 0u;
+    int64_t f = fpre[blockIdx.x] + block_excl(__popc(kept), sh);
+    if (!kept) return;
+    const int64_t base = w.cell_inst[i];
+    const uint64_t tw = cSTris[in.method][cs];
+    for (uint32_t m = kept; m; m &= m - 1) {
+        const int q = __ffs(m) - 1;
         const int64_t i0 = inst_of(st, (int) ((tw >> (4 + 12 * q)) & 15), base);
         const int64_t i1 = inst_of(st, (int) ((tw >> (8 + 12 * q)) & 15), base);
         const int64_t i2 = inst_of(st, (int) ((tw >> (12 + 12 * q)) & 15), base);
-        const int64_t r0 = w.rkey[i0], r1 = w.rkey[i1], r2 = w.rkey[i2];
-        if (r0 == r1 || r0 == r2 || r1 == r2) continue;
         if (f < o.cap_f) {
             o.f[3 * f + 0] = w.vid[i0];
             o.f[3 * f + 1] = w.vid[i1];
@@ -503,16 +498,20 @@ __global__ void k_smc_emit(SparseMcIn in, SparseMcWork w, const int64_t *fpre, c
         }
         ++f;
     }
-    for (int64_t q = base; q < base + __popc(st); ++q) {
-        if (!((bits[q >> 5] >> (q & 31)) & 1u)) continue;
-        const int64_t v = rank_of(bits, wpre, q);
-        if (v < o.cap_v) {
-            o.v[3 * v + 0] = w.ipos[3 * q + 0];
-            o.v[3 * v + 1] = w.ipos[3 * q + 1];
-            o.v[3 * v + 2] = w.ipos[3 * q + 2];
-            if (o.vsrc) o.vsrc[v] = w.isrc[q];
-        }
-    }
+}
+
+// Output vertices: each representative that a kept face uses writes its
+// position (and its source edge) at its rank.
+__global__ void k_smc_vwrite(SparseMcWork w, int64_t ninst, const uint32_t *bits, const int64_t *wpre,
+                             SparseMcOut o) {
+    const int64_t q = (int64_t) blockIdx.x * kB + threadIdx.x;
+    if (q >= ninst || !((bits[q >> 5] >> (q & 31)) & 1u)) return;
+    const int64_t v = rank_of(bits, wpre, q);
+    if (v >= o.cap_v) return;
+    o.v[3 * v + 0] = w.ipos[3 * q + 0];
+    o.v[3 * v + 1] = w.ipos[3 * q + 1];
+    o.v[3 * v + 2] = w.ipos[3 * q + 2];
+    if (o.vsrc) o.vsrc[v] = w.isrc[q];
 }
 
 __global__ void k_fill_u64(unsigned long long *p, int64_t n, unsigned long long v) {
@@ -889,7 +888,9 @@ hipError_t sparse_mc(const SparseMcIn &in, int64_t ninst, void *work, size_t wor
     nf = h[0];
     nv = h[1];
     if (nv > o.cap_v || nf > o.cap_f) return hipSuccess;   // caller re-allocates and retries
-    hipLaunchKernelGGL(k_smc_emit, dim3((unsigned) nblk), dim3(kB), 0, st, in, w, fbs, bits, wpre, o);
+    hipLaunchKernelGGL(k_smc_emit, dim3((unsigned) nblk), dim3(kB), 0, st, in, w, (const int64_t *) fbs, o);
+    hipLaunchKernelGGL(k_smc_vwrite, dim3(nb(ninst)), dim3(kB), 0, st, w, ninst, (const uint32_t *) bits,
+                       (const int64_t *) wpre, o);
     ISO_TRY(hipGetLastError());
     ISO_TRY(hipStreamSynchronize(st));
     return hipSuccess;
