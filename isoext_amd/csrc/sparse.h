@@ -27,7 +27,7 @@ struct SparseMcWork {
     int32_t *inst_cell;   // cell slot of each instance
     float *ipos;          // instance positions
     int64_t *isrc;        // slot * 12 + edge of each instance
-    int64_t *rkey;        // slot * 12 + edge of its representative (first instance at its position)
+    int64_t *rkey;        // instance index of its representative (first instance at its position)
     uint8_t *used;        // the representative is a corner of a kept face
     int32_t *vid;         // vertex id of the instance's representative
     int *err;

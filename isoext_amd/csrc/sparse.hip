@@ -398,18 +398,21 @@ __global__ void __launch_bounds__(kB) k_smc_inst(SparseMcIn in, SLook L, SparseM
     w.ipos[3 * q + 1] = py;
     w.ipos[3 * q + 2] = pz;
     w.isrc[q] = i * 12 + e;
-    w.rkey[q] = rep;
+    // the representative as an instance index (its cell's first instance +
+    // its rank among the cell's crossing edges)
+    if (rep != i * 12 + e) {
+        const int64_t s2 = rep / 12;
+        const int e2 = (int) (rep - s2 * 12);
+        w.rkey[q] = w.cell_inst[s2] + __popc((int) (w.cst[s2] & 0xfffu) & ((1 << e2) - 1));
+    } else {
+        w.rkey[q] = q;
+    }
 }
 
 __device__ __forceinline__ int64_t inst_of(int st, int e, int64_t base) {
     return base + __popc(st & ((1 << e) - 1));
 }
 
-// Instance index of a representative key.
-__device__ __forceinline__ int64_t inst_of_key(const SparseMcWork &w, int64_t key) {
-    const int64_t slot = key / 12;
-    return inst_of((int) (w.cst[slot] & 0xfffu), (int) (key - slot * 12), w.cell_inst[slot]);
-}
 
 // Pass 3: kept faces per cell (a triangle with two corners at one position is
 // dropped); the representatives of the kept faces' corners are marked used.
@@ -432,9 +435,9 @@ __global__ void k_smc_faces(SparseMcIn in, SparseMcWork w, int64_t *bsum) {
                 if (r0 == r1 || r0 == r2 || r1 == r2) continue;
                 ++nf;
                 kept |= 1u << q;
-                w.used[inst_of_key(w, r0)] = 1;
-                w.used[inst_of_key(w, r1)] = 1;
-                w.used[inst_of_key(w, r2)] = 1;
+                w.used[r0] = 1;
+                w.used[r1] = 1;
+                w.used[r2] = 1;
             }
             w.cst[i] = cw | (kept << 24);   // kept triangles for the emit pass
         }
@@ -452,7 +455,7 @@ __global__ void k_smc_faces(SparseMcIn in, SparseMcWork w, int64_t *bsum) {
 // Output vertices: used representatives, one bit per instance (64 per wave).
 __global__ void k_smc_vbits(SparseMcWork w, int64_t ninst, uint32_t *bits) {
     const int64_t q = (int64_t) blockIdx.x * kB + threadIdx.x;
-    const bool is_v = q < ninst && w.used[q] && w.rkey[q] == w.isrc[q];
+    const bool is_v = q < ninst && w.used[q] && w.rkey[q] == q;
     const uint64_t m = __ballot(is_v);
     const int lane = threadIdx.x & 63;
     const int64_t w0 = (q - lane) >> 5;
@@ -470,7 +473,7 @@ __device__ __forceinline__ int64_t rank_of(const uint32_t *bits, const int64_t *
 __global__ void k_smc_vid(SparseMcWork w, int64_t ninst, const uint32_t *bits, const int64_t *wpre) {
     const int64_t q = (int64_t) blockIdx.x * kB + threadIdx.x;
     if (q >= ninst) return;
-    const int64_t r = inst_of_key(w, w.rkey[q]);
+    const int64_t r = w.rkey[q];
     w.vid[q] = (int32_t) rank_of(bits, wpre, r);
 }
 
