@@ -319,7 +319,8 @@ typedef struct isoext_mc_item {
 /* Workspace for a batch (zero-filled before first use); special_cap is the
  * weld-candidate capacity per item.  cfg >= 0: every item uses that emit tile
  * configuration; cfg < 0: each item its own (isoext_mc_auto_config of its
- * shape).  The same cfg must be passed to both phases. */
+ * shape).  Items with an SDF program always use configuration 5 (8 x 8 x 1
+ * cells).  The same cfg must be passed to both phases. */
 int64_t isoext_mc_batch_workspace_bytes(int n, const isoext_mc_item *items, int cfg, int64_t special_cap);
 
 /* Phase 1: classify, count and scan every item (one launch per stage for the

@@ -398,7 +398,7 @@ bool batch_layout(int n, const isoext_mc_item *items, int cfg, int64_t special_c
     for (int i = 0; i < n; ++i) {
         ItemLayout &L = B.it[i];
         const int64_t X = items[i].shape[0], Y = items[i].shape[1], Z = items[i].shape[2];
-        L.cfg = cfg >= 0 ? cfg : iso::mc_auto_config(items[i].shape);
+        L.cfg = items[i].sdf ? iso::kProgramTileConfig : (cfg >= 0 ? cfg : iso::mc_auto_config(items[i].shape));
         iso::mc_config_geom(L.cfg, tx, ty, nzc);
         L.nzc = nzc;
         L.cells = X >= 2 && Y >= 2 && Z >= 2;

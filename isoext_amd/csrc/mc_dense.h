@@ -175,6 +175,9 @@ inline bool csg_list_fits(int64_t X, int64_t Y, int64_t W, int64_t S) {
     return nb + 1 <= X * Y * S && nb <= (int64_t) UINT32_MAX;
 }
 constexpr int kCsgFillWaves = 4096;   // phase-2 workgroups (one wave each)
+// Emit tile configuration of SDF-program fields, (8, 8, 1): small tiles, so
+// that a tile's distinct corner points fit its LDS cache.
+constexpr int kProgramTileConfig = 5;
 
 // Fill McParams' invariant-divisor fields from X, Y, W and nzc.
 void mc_params_divisors(McParams &p);
