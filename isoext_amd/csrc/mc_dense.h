@@ -174,7 +174,8 @@ inline bool csg_list_fits(int64_t X, int64_t Y, int64_t W, int64_t S) {
     const int64_t nb = csg_classify_tasks(X, Y, W);
     return nb + 1 <= X * Y * S && nb <= (int64_t) UINT32_MAX;
 }
-constexpr int kCsgFillWaves = 4096;   // phase-2 workgroups (one wave each)
+constexpr int kCsgFillGroups = 2048;   // phase-2 workgroups
+constexpr int kCsgFillWaves = 4;       // waves per phase-2 workgroup (they share a block's evaluations)
 // Emit tile configuration of SDF-program fields, (8, 8, 1): small tiles, so
 // that a tile's distinct corner points fit its LDS cache.
 constexpr int kProgramTileConfig = 5;

@@ -824,10 +824,14 @@ __device__ void csg_test_block(const McParams &p, const CsgProgram &P, int64_t b
     }
 }
 
-// Phase 2 for an undecided block b (one wave); qs: its quarters' states
-// from phase 1 (0: test every column).
-__device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b, uint32_t qs, double cs) {
-    const int lane = threadIdx.x & 63;
+// Phase 2 for an undecided block b by the kCsgFillWaves waves of a
+// workgroup (all of them call it); qs: its quarters' states from phase 1
+// (0: test every column).  Wave 0 tests the columns and shares the masks
+// through LDS (wacc[16], wacc[17]); the rounds of point evaluations are dealt
+// to the waves, whose partial words meet in LDS (wacc[0, 16)).
+__device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b, uint32_t qs, double cs,
+                               uint64_t *wacc) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const CsgBlock k = csg_block(p, b);
     // columns: 4 x 4 rows at one z per lane
     const float x0 = grid_coord(k.xa, p.X, p.amin[0], p.amax[0]), x1 = grid_coord(k.xa + k.nx - 1, p.X, p.amin[0], p.amax[0]);
@@ -835,20 +839,28 @@ __device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b
     const float cx = 0.5f * (x0 + x1), cy = 0.5f * (y0 + y1);
     const double ex = fmax(fabs((double) cx - x0), fabs((double) x1 - cx));
     const double ey = fmax(fabs((double) cy - y0), fabs((double) y1 - cy));
-    const bool zok = lane < k.nz;
-    const uint32_t qst = (qs >> (2 * (lane >> 4))) & 3u;   // this lane's quarter, decided in phase 1
-    int s2 = 1;
-    if (zok && qst != 0) {
-        s2 = qst == 1 ? -1 : 1;
-    } else if (zok) {
-        const float pz = grid_coord(k.za + lane, p.Z, p.amin[2], p.amax[2]);
-        s2 = csg_box_sign(P, csg_value<true>(P, cx, cy, pz), sqrt(ex * ex + ey * ey), cs, p.level);
+    if (wave == 0) {
+        const bool zok = lane < k.nz;
+        const uint32_t qst = (qs >> (2 * (lane >> 4))) & 3u;   // this lane's quarter, decided in phase 1
+        int s2 = 1;
+        if (zok && qst != 0) {
+            s2 = qst == 1 ? -1 : 1;
+        } else if (zok) {
+            const float pz = grid_coord(k.za + lane, p.Z, p.amin[2], p.amax[2]);
+            s2 = csg_box_sign(P, csg_value<true>(P, cx, cy, pz), sqrt(ex * ex + ey * ey), cs, p.level);
+        }
+        const uint64_t kin = __ballot(zok && s2 < 0), nd = __ballot(zok && s2 == 0);
+        if (lane < 16) wacc[lane] = 0ull;
+        if (lane == 0) {
+            wacc[16] = kin;
+            wacc[17] = nd;
+        }
     }
-    const uint64_t known_in = __ballot(zok && s2 < 0);
-    const uint64_t need = __ballot(zok && s2 == 0);
+    __syncthreads();
+    const uint64_t known_in = wacc[16], need = wacc[17];
     const int nn = __popcll(need);
-    uint64_t word = known_in;   // lane r < 16: the word of row r
-    for (int j0 = 0; j0 < 16 * nn; j0 += 64) {
+    uint64_t word = 0;   // lane r < 16: this wave's bits of row r
+    for (int j0 = 64 * wave; j0 < 16 * nn; j0 += 64 * kCsgFillWaves) {
         const int j = j0 + lane, r = lane & 15;
         const bool act = j < 16 * nn && (r >> 2) < k.nx && (r & 3) < k.ny;
         const int L = j < 16 * nn ? nth_set_bit(need, j >> 4) : 0;
@@ -862,19 +874,26 @@ __device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b
             if (bq) word |= 1ull << lq;
         }
     }
-    if (lane < 16 && (lane >> 2) < k.nx && (lane & 3) < k.ny)
-        p.bits[((k.xa + (lane >> 2)) * p.Y + (k.ya + (lane & 3))) * p.W + k.c] = word & low_bits(k.nz);
+    if (lane < 16 && word) atomicOr((unsigned long long *) &wacc[lane], (unsigned long long) word);
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < 16 && (t >> 2) < k.nx && (t & 3) < k.ny)
+        p.bits[((k.xa + (t >> 2)) * p.Y + (k.ya + (t & 3))) * p.W + k.c] = (known_in | wacc[t]) & low_bits(k.nz);
+    __syncthreads();   // wacc is reused by the next block
 }
 
-// Phase 2 over the list, or over every block (repeating the test).
-__device__ void csg_fill(const McParams &p, const CsgProgram &P, double cs, int64_t first, int64_t stride) {
+// Phase 2 over the list, or over every block (repeating the test); one
+// block per workgroup at a time.
+__device__ void csg_fill(const McParams &p, const CsgProgram &P, double cs, int64_t first, int64_t stride,
+                         uint64_t *wacc) {
     if (p.csg_list) {
         const int64_t n = __builtin_amdgcn_readfirstlane((int) *(volatile uint32_t *) p.cnt_f);
-        for (int64_t i = first; i < n; i += stride) csg_fill_block(p, P, (int64_t) p.cnt_v[i], p.cnt_f[1 + i], cs);
+        for (int64_t i = first; i < n; i += stride)
+            csg_fill_block(p, P, (int64_t) p.cnt_v[i], p.cnt_f[1 + i], cs, wacc);
     } else {
         const int64_t nb = csg_classify_tasks(p.X, p.Y, p.W);
         for (int64_t b = first; b < nb; b += stride)
-            if (csg_block_sign(p, P, csg_block(p, b), cs) == 0) csg_fill_block(p, P, b, 0u, cs);
+            if (csg_block_sign(p, P, csg_block(p, b), cs) == 0) csg_fill_block(p, P, b, 0u, cs, wacc);
     }
 }
 
@@ -888,11 +907,12 @@ __global__ void __launch_bounds__(256) k_csg_blocks(McParams p) {
         csg_test_block(p, P, base + threadIdx.x, base + threadIdx.x < nb, cs);
 }
 
-__global__ void __launch_bounds__(64) k_csg_fill(McParams p) {
+__global__ void __launch_bounds__(64 * kCsgFillWaves) k_csg_fill(McParams p) {
     __shared__ CsgProgram P;
+    __shared__ uint64_t wacc[18];
     csg_stage(P, p.csg);
     __syncthreads();
-    csg_fill(p, P, csg_coord_scale(p), blockIdx.x, gridDim.x);
+    csg_fill(p, P, csg_coord_scale(p), blockIdx.x, gridDim.x, wacc);
 }
 
 // LDS of the fused emit: per 64-point z chunk of the tile, the spheres that
@@ -1736,10 +1756,11 @@ __global__ void __launch_bounds__(256) k_csg_blocks_batch(const McBatchItem *__r
     }
 }
 
-// Phase 2 of every program item of a batch (one wave per workgroup).
-__global__ void __launch_bounds__(64) k_csg_fill_batch(const McBatchItem *__restrict__ items,
-                                                       const McBatchHdr *__restrict__ hdr) {
+// Phase 2 of every program item of a batch.
+__global__ void __launch_bounds__(64 * kCsgFillWaves) k_csg_fill_batch(const McBatchItem *__restrict__ items,
+                                                                       const McBatchHdr *__restrict__ hdr) {
     __shared__ CsgProgram P;
+    __shared__ uint64_t wacc[18];
     for (int i = 0; i < (int) hdr->n; ++i) {
         const McParams &p = items[i].p;
         if (!p.csg || p.X < 2 || p.Y < 2 || p.Z < 2) continue;
@@ -1748,7 +1769,7 @@ __global__ void __launch_bounds__(64) k_csg_fill_batch(const McBatchItem *__rest
         __syncthreads();
         csg_stage(P, p.csg);
         __syncthreads();
-        csg_fill(p, P, csg_coord_scale(p), blockIdx.x, gridDim.x);
+        csg_fill(p, P, csg_coord_scale(p), blockIdx.x, gridDim.x, wacc);
     }
 }
 
@@ -2126,7 +2147,7 @@ static void launch_classify(const McParams &p, hipStream_t stream) {
         const int64_t nb = csg_classify_tasks(p.X, p.Y, p.W);
         const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nb + 255) / 256, 8192));
         hipLaunchKernelGGL(k_csg_blocks, dim3(blocks), dim3(256), 0, stream, p);
-        hipLaunchKernelGGL(k_csg_fill, dim3(kCsgFillWaves), dim3(64), 0, stream, p);
+        hipLaunchKernelGGL(k_csg_fill, dim3(kCsgFillGroups), dim3(64 * kCsgFillWaves), 0, stream, p);
     } else if (p.nsph > 0) {
         const int64_t ntask = ((p.X + 3) / 4) * ((p.Y + 3) / 4) * p.W;
         const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((ntask + 3) / 4, 8192));
@@ -2254,7 +2275,7 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     if (h.ngtask > 0) {
         const unsigned gb = (unsigned) std::min<int64_t>((h.ngtask + 255) / 256, 8192);
         hipLaunchKernelGGL(k_csg_blocks_batch, dim3(gb), dim3(256), 0, stream, items, hdr);
-        hipLaunchKernelGGL(k_csg_fill_batch, dim3(kCsgFillWaves), dim3(64), 0, stream, items, hdr);
+        hipLaunchKernelGGL(k_csg_fill_batch, dim3(kCsgFillGroups), dim3(64 * kCsgFillWaves), 0, stream, items, hdr);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const unsigned nb = (unsigned) std::max<int64_t>(1, h.nblk);
