@@ -164,3 +164,39 @@ def test_batches_mix_grids_and_programs(gpu):
     for j, (v, f) in zip(jobs, got):
         alone = dc_batch([GridJob(j.values, j.level, 0, sdf=j.sdf, shape=j.shape, sdf_normals=j.sdf_normals)])[0]
         assert torch.equal(v, alone[0]) and torch.equal(f, alone[1])
+
+
+@pytest.mark.parametrize("shape,cfg", [((2, 2, 1200), 4), ((3, 5, 700), 4), ((9, 2, 300), 0)])
+def test_mc_sdf_thin_grids(gpu, shape, cfg):
+    """Grids too thin for the undecided-block list (fewer than 8 point rows
+    along x or y with deep emit segments): the second classify phase walks
+    every block and repeats the centre test."""
+    from isoext_amd import sdf as S
+    from isoext_amd.mc import mc_dense_raw
+    tree = S.UnionOp([S.TorusSDF(0.5, 0.2), S.TranslationOp(S.SphereSDF(0.3), [0.0, 0.0, 0.5])])
+    prog = program(tree)
+    amin, amax = (-0.05, -0.05, -1.0), (0.05, 0.05, 1.0)
+    v_ref, f_ref, _ = mc_dense_raw(prog.grid(shape, amin, amax), shape, amin, amax, 0.0, 0, cfg)
+    v, f, _ = mc_dense_raw(None, shape, amin, amax, 0.0, 0, cfg, program=prog)
+    assert f_ref.shape[0] > 0
+    assert torch.equal(v, v_ref) and torch.equal(f, f_ref)
+
+
+def test_sdf_single_primitives_and_empty(gpu):
+    from isoext_amd import marching_cubes
+    from isoext_amd import sdf as S
+    from isoext_amd.fused import dual_contouring_sdf, marching_cubes_sdf
+    from isoext_amd.grid import UniformGrid
+    for tree in (S.SphereSDF(0.6), S.TorusSDF(0.5, 0.2), S.CuboidSDF([1.0, 0.6, 1.4]), S.CylinderSDF(0.4),
+                 S.NegationOp(S.SphereSDF(0.6))):
+        prog = program(tree)
+        shape = (40, 44, 48)
+        v_ref, f_ref = marching_cubes(UniformGrid.wrap(prog.grid(shape)), 0.0)
+        v, f = marching_cubes_sdf(prog, shape)
+        assert torch.equal(v, v_ref) and torch.equal(f, f_ref), type(tree).__name__
+    # no crossing: every block is decided in the first phase
+    far = program(S.TranslationOp(S.SphereSDF(0.1), [5.0, 5.0, 5.0]))
+    assert marching_cubes_sdf(far, (64, 64, 64)) == (None, None)
+    assert dual_contouring_sdf(far, (64, 64, 64)) == (None, None)
+    inside_all = program(S.SphereSDF(10.0))
+    assert marching_cubes_sdf(inside_all, (64, 64, 64), level=0.0) == (None, None)
