@@ -719,15 +719,18 @@ __global__ void __launch_bounds__(256) k_classify_spheres(McParams p) {
 // points whose centre value is farther from the level than the box radius
 // (plus a margin far above the FP32 evaluation error) has one sign.
 // Phase 1 (one lane per block of 4 x 4 point rows x one 64-point word)
-// tests each block at its centre, writes the words of decided blocks and
-// lists the others; phase 2 (one wave per listed block) tests the block's
-// 64 columns of 4 x 4 points (one z per lane) and evaluates the points of
-// the columns still undecided, 64 at a time, assembling the rows' words.
-// Every evaluated value is the stored grid's (same program, same grid
-// coordinates), and decided points have the sign their value would give.
-// The list lives in cnt_v (its count in cnt_f[0]), free until the count
-// pass; when it does not fit (fewer than 8 point rows along x or y), phase
-// 2 walks every block and repeats the centre test (p.csg_list = 0).
+// tests each block at its centre, then an undecided block at its four
+// 16-point quarters, writes the words of decided blocks and lists the
+// others with their quarters' states; phase 2 (four waves per listed block)
+// tests the block's 64 columns of 4 x 4 points (one z per lane) and
+// evaluates the points of the columns still undecided, 64 at a time,
+// assembling the rows' words.  Every evaluated value is the stored grid's
+// (same program, same grid coordinates), and decided points have the sign
+// their value would give.  The list lives in cnt_v (block ids) and cnt_f
+// (its count in cnt_f[0], the quarter states after it), free until the
+// count pass; when it does not fit (thin grids with deep emit segments, see
+// csg_list_fits), phase 2 walks every block and repeats the centre test
+// (p.csg_list = 0).
 
 struct CsgBlock {
     int64_t xa, ya, c, za;
