@@ -297,6 +297,24 @@ int isoext_mc_sparse(const isoext_grid *g, float level, int method, int64_t nins
                      int64_t workspace_bytes, float *out_v, int64_t cap_v, int32_t *out_f,
                      int64_t cap_f, int64_t *vsrc, int64_t counts[2], void *stream);
 
+/* ---- captured plans (no reference counterpart: repeated extractions of one
+ * grid buffer, e.g. an optimisation loop that updates the values in place) ----
+ * isoext_mc_plan_create records isoext_mc_dense's kernel launches for these
+ * exact arguments (values pointer, shape, level, method, tile configuration
+ * other than the read-once one, workspace, output buffers and capacities)
+ * into a HIP graph; isoext_mc_plan_run replays it on `stream`, reads the
+ * counts back and runs the weld fix-up when needed, with isoext_mc_dense's
+ * results and error codes (ISOEXT_E_CAPACITY / _WORKSPACE: create a plan
+ * with larger buffers).  The buffers must stay allocated while the plan
+ * lives; a plan's runs must not overlap. */
+typedef struct isoext_mc_plan isoext_mc_plan;
+int isoext_mc_plan_create(const float *values, const int64_t shape[3], const float aabb_min[3],
+                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
+                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v, int32_t *out_f,
+                          int64_t cap_f, isoext_mc_plan **plan);
+int isoext_mc_plan_run(isoext_mc_plan *plan, int64_t counts[6], void *stream);
+void isoext_mc_plan_destroy(isoext_mc_plan *plan);
+
 /* ---- ragged batches of dense marching cubes (no reference counterpart:
  * the reference extracts one grid per call, src/mc/mc.cu:17-68) ---------- */
 

@@ -868,12 +868,56 @@ int64_t isoext_mc_dense_workspace_bytes(const int64_t shape[3], int cfg, int64_t
     return (int64_t) L.total;
 }
 
+// A captured marching-cubes pipeline: the arguments of every run and the
+// instantiated graph of its kernel launches (reset, classify, count, scan,
+// emit, special-list gather and dup-check).  The read-back of the counts and
+// the weld fix-up run outside the graph, after each replay.
+struct isoext_mc_plan {
+    const float *values;
+    int64_t shape[3];
+    float amin[3], amax[3];
+    float level;
+    int method, cfg;
+    void *workspace;
+    int64_t workspace_bytes, special_cap;
+    float *out_v;
+    int64_t cap_v;
+    int32_t *out_f;
+    int64_t cap_f;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+enum { kPlanNone = 0, kPlanCapture = 1, kPlanRun = 2 };
+
+// Capture on a private stream (the legacy default stream cannot capture).
+static hipError_t plan_capture(isoext_mc_plan *plan, int cfg, iso::McParams &p, const iso::SpecialLists &sp) {
+    hipStream_t cs = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        const hipError_t le = iso::mc_dense_launch(cfg, p, sp, cs, nullptr);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(cs, &g);
+        e = le != hipSuccess ? le : ce;
+        if (e == hipSuccess) {
+            plan->graph = g;
+            e = hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0);
+        } else if (g) {
+            (void) hipGraphDestroy(g);
+        }
+    }
+    (void) hipStreamDestroy(cs);
+    return e;
+}
+
 static int mc_dense_impl(const float *values, const int64_t shape[3], const float aabb_min[3],
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
                          int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream,
                          const float *spheres = nullptr, int nsph = 0, const iso::FixupScratch *na = nullptr,
-                         const iso::CsgProgram *csg = nullptr);
+                         const iso::CsgProgram *csg = nullptr, isoext_mc_plan *plan = nullptr, int plan_mode = 0);
 
 int isoext_mc_spheres(const float *spheres, int nsph, const int64_t shape[3], const float aabb_min[3],
                       const float aabb_max[3], float level, int method, int cfg, void *workspace,
@@ -891,6 +935,57 @@ int isoext_mc_sdf(const void *program, const int64_t shape[3], const float aabb_
     return mc_dense_impl(nullptr, shape, aabb_min, aabb_max, level, method, cfg, workspace, workspace_bytes,
                          special_cap, out_v, cap_v, out_f, cap_f, nullptr, counts, stream, nullptr, 0, nullptr,
                          (const iso::CsgProgram *) program);
+}
+
+// ---- captured marching-cubes plans ----------------------------------------------------
+
+int isoext_mc_plan_create(const float *values, const int64_t shape[3], const float aabb_min[3],
+                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
+                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v, int32_t *out_f,
+                          int64_t cap_f, isoext_mc_plan **plan) {
+    if (!plan) return fail(ISOEXT_E_ARG, "plan is null");
+    *plan = nullptr;
+    if (!values) return fail(ISOEXT_E_ARG, "values is null");
+    std::unique_ptr<isoext_mc_plan> pl(new isoext_mc_plan);
+    pl->values = values;
+    for (int a = 0; a < 3; ++a) {
+        pl->shape[a] = shape[a];
+        pl->amin[a] = aabb_min[a];
+        pl->amax[a] = aabb_max[a];
+    }
+    pl->level = level;
+    pl->method = method;
+    pl->cfg = cfg;
+    pl->workspace = workspace;
+    pl->workspace_bytes = workspace_bytes;
+    pl->special_cap = special_cap;
+    pl->out_v = out_v;
+    pl->cap_v = cap_v;
+    pl->out_f = out_f;
+    pl->cap_f = cap_f;
+    int64_t counts[6];
+    const int rc = mc_dense_impl(values, shape, aabb_min, aabb_max, level, method, cfg, workspace, workspace_bytes,
+                                 special_cap, out_v, cap_v, out_f, cap_f, nullptr, counts, nullptr, nullptr, 0,
+                                 nullptr, nullptr, pl.get(), kPlanCapture);
+    if (rc != ISOEXT_OK) return rc;
+    *plan = pl.release();
+    return ISOEXT_OK;
+}
+
+int isoext_mc_plan_run(isoext_mc_plan *pl, int64_t counts[6], void *stream) {
+    if (!pl) return fail(ISOEXT_E_ARG, "plan is null");
+    for (int i = 0; i < 6; ++i) counts[i] = 0;
+    if (!pl->exec) return ISOEXT_OK;   // a grid without cells
+    return mc_dense_impl(pl->values, pl->shape, pl->amin, pl->amax, pl->level, pl->method, pl->cfg, pl->workspace,
+                         pl->workspace_bytes, pl->special_cap, pl->out_v, pl->cap_v, pl->out_f, pl->cap_f, nullptr,
+                         counts, stream, nullptr, 0, nullptr, nullptr, pl, kPlanRun);
+}
+
+void isoext_mc_plan_destroy(isoext_mc_plan *pl) {
+    if (!pl) return;
+    if (pl->exec) (void) hipGraphExecDestroy(pl->exec);
+    if (pl->graph) (void) hipGraphDestroy(pl->graph);
+    delete pl;
 }
 
 // ---- SDF programs --------------------------------------------------------------------
@@ -958,7 +1053,8 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
                          const float aabb_max[3], float level, int method, int cfg, void *workspace,
                          int64_t workspace_bytes, int64_t special_cap, float *out_v, int64_t cap_v,
                          int32_t *out_f, int64_t cap_f, int64_t *out_key, int64_t counts[6], void *stream,
-                         const float *spheres, int nsph, const iso::FixupScratch *na, const iso::CsgProgram *csg) {
+                         const float *spheres, int nsph, const iso::FixupScratch *na, const iso::CsgProgram *csg,
+                         isoext_mc_plan *plan, int plan_mode) {
     for (int i = 0; i < 6; ++i) counts[i] = 0;
     if (method != 0 && method != 1) return fail(ISOEXT_E_ARG, "Unknown method: %d", method);
     AABB_CHECK(aabb_min, aabb_max);
@@ -1051,8 +1147,14 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
         p.csg = csg;
         p.csg_list = iso::csg_list_fits(p.X, p.Y, p.W, p.S) ? 1 : 0;
     }
-    hipEvent_t *ev = (g_timing && !na) ? timing_events() : nullptr;
-    if (g_timing && !na && !ev) return fail(ISOEXT_E_DEVICE, "timing events unavailable");
+    hipEvent_t *ev = (g_timing && !na && !plan) ? timing_events() : nullptr;
+    if (g_timing && !na && !plan && !ev) return fail(ISOEXT_E_DEVICE, "timing events unavailable");
+    if (plan && L.stream) return fail(ISOEXT_E_ARG, "plans take the four-stage tile configurations, not %d", cfg);
+    if (plan_mode == kPlanCapture) {   // record the pipeline's launches into the plan's graph; no run
+        HIP_CHECK(iso::mc_upload_luts(), "upload tables");
+        HIP_CHECK(plan_capture(plan, cfg, p, sp_host), "capture marching cubes");
+        return ISOEXT_OK;
+    }
     const bool read_once = L.stream && !spheres && !csg && L.NR <= (int64_t) INT32_MAX && shape[2] < (int64_t(1) << 26);
     if (read_once) {
         p.NRX = L.NRX;
@@ -1086,6 +1188,8 @@ static int mc_dense_impl(const float *values, const int64_t shape[3], const floa
             p.arena_seg_cap = entries / iso::kStrArenaSeg;
         }
         HIP_CHECK(iso::mc_stream_launch(p, sp_host, st, ev), "launch marching cubes");
+    } else if (plan_mode == kPlanRun) {
+        HIP_CHECK(hipGraphLaunch(plan->exec, st), "launch the plan's graph");
     } else {
         HIP_CHECK(iso::mc_dense_launch(cfg, p, sp_host, st, ev), "launch marching cubes");
     }

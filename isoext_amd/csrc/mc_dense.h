@@ -238,6 +238,9 @@ int mc_config_count();
 int64_t mc_config_tiles(int cfg, const int64_t shape[3]);
 int mc_auto_config(const int64_t shape[3]);
 bool mc_config_geom(int cfg, int &tx, int &ty, int &nzc);
+// Uploads the triangle tables once per process (a synchronous copy: call it
+// before capturing launches into a graph).
+hipError_t mc_upload_luts();
 hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hipStream_t stream,
                            hipEvent_t *ev = nullptr);
 // The read-once pipeline: reset, k_stream (masks, counts, vertices into the

@@ -163,3 +163,108 @@ def marching_cubes(grid: Grid, level: float = 0.0, method: str = "nagae"):
     if v is None or f is None or f.shape[0] == 0:
         return None, None
     return v, f
+
+
+class MarchingCubesPlan:
+    """Repeated marching cubes of one grid tensor whose values change in
+    place (an optimisation loop, an animation): the pipeline's kernel
+    launches are captured once into a HIP graph (isoext_mc_plan_create) and
+    each run() replays it, reads the counts back and welds when needed.  The
+    plan owns its workspace and output buffers, sized from a first
+    extraction with headroom; a run that outgrows them rebuilds the plan.
+    Results equal marching_cubes on the tensor's current values."""
+
+    def __init__(self, values: torch.Tensor, level: float = 0.0, method: str = "nagae",
+                 aabb_min=(-1.0, -1.0, -1.0), aabb_max=(1.0, 1.0, 1.0), cfg: int | None = None,
+                 headroom: float = 1.25):
+        import ctypes
+        from .grid import check_cuda_f32
+        check_cuda_f32(values, 3, name="values")
+        if not values.is_contiguous():
+            raise TypeError("values must be C-contiguous")
+        self.values = values
+        self.level = float(level)
+        self.method = _method_id(method)
+        self.aabb_min, self.aabb_max = tuple(aabb_min), tuple(aabb_max)
+        self.shape = tuple(int(s) for s in values.shape)
+        self.cfg = autotune.mc_config(self.shape) if cfg is None else int(cfg)
+        if self.cfg == 6:
+            self.cfg = 3   # plans take the four-stage configurations
+        self.headroom = float(headroom)
+        self._h = None
+        self._L = _lib.lib()
+        L = self._L
+        L.isoext_mc_plan_create.restype = ctypes.c_int
+        L.isoext_mc_plan_create.argtypes = [ctypes.c_void_p, _lib.I64x3, _lib.F32x3, _lib.F32x3, ctypes.c_float,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int64,
+                                            ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                            ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]
+        L.isoext_mc_plan_run.restype = ctypes.c_int
+        L.isoext_mc_plan_run.argtypes = [ctypes.c_void_p, _lib.I64x6, ctypes.c_void_p]
+        L.isoext_mc_plan_destroy.restype = None
+        L.isoext_mc_plan_destroy.argtypes = [ctypes.c_void_p]
+        v, f, info = mc_dense_raw(values, self.shape, self.aabb_min, self.aabb_max, self.level, self.method,
+                                  self.cfg)
+        with _hint_lock:
+            special = max(DEFAULT_SPECIAL_CAP, _special_hint.get(self.shape, 0), 2 * int(info["special"]))
+        self._build(int(v.shape[0]), int(f.shape[0]), special)
+
+    def _build(self, nv: int, nf: int, special: int):
+        import ctypes
+        self.close()
+        dev = self.values.device
+        self.special = int(special)
+        self.cap_v = max(int(nv * self.headroom) + 1024, 1 << 12)
+        self.cap_f = max(int(nf * self.headroom) + 2048, 1 << 13)
+        shape_c = _lib.i64x3(self.shape)
+        ws_bytes = int(self._L.isoext_mc_dense_workspace_bytes(shape_c, self.cfg, self.special))
+        if ws_bytes < 0:
+            raise ValueError(f"invalid tile config {self.cfg}")
+        self.ws = torch.zeros(ws_bytes, dtype=torch.uint8, device=dev)   # this plan's alone: the graph holds it
+        self.out_v = torch.empty((self.cap_v, 3), dtype=torch.float32, device=dev)
+        self.out_f = torch.empty((self.cap_f, 3), dtype=torch.int32, device=dev)
+        h = ctypes.c_void_p()
+        with _lib.on_device(dev):
+            _lib.check(self._L.isoext_mc_plan_create(self.values.data_ptr(), shape_c, _lib.f32x3(self.aabb_min),
+                                                     _lib.f32x3(self.aabb_max), self.level, self.method, self.cfg,
+                                                     self.ws.data_ptr(), ws_bytes, self.special,
+                                                     self.out_v.data_ptr(), self.cap_v, self.out_f.data_ptr(),
+                                                     self.cap_f, ctypes.byref(h)), "marching cubes plan")
+        self._h = h
+
+    def run(self, copy: bool = True):
+        """Extract from the tensor's current values.  Returns (vertices,
+        faces) or (None, None); with copy=False, views of the plan's buffers
+        that the next run overwrites."""
+        dev = self.values.device
+        for _ in range(3):
+            counts = _lib.I64x6()
+            with _lib.on_device(dev):
+                rc = self._L.isoext_mc_plan_run(self._h, counts, _lib.stream_ptr(dev))
+            if rc == _lib.E_CAPACITY:
+                self._build(int(counts[0]), int(counts[1]), self.special)
+                continue
+            if rc == _lib.E_WORKSPACE and counts[2] > self.special:
+                nv, nf = self.cap_v, self.cap_f
+                self._build(int(nv / self.headroom), int(nf / self.headroom), int(counts[2]) + 1024)
+                continue
+            _lib.check(rc, "marching cubes plan")
+            break
+        else:
+            raise _lib.IsoextError("marching cubes plan: output sizing did not converge")
+        nv, nf = int(counts[0]), int(counts[1])
+        if nf == 0:
+            return None, None
+        v, f = self.out_v[:nv], self.out_f[:nf]
+        return (v.clone(), f.clone()) if copy else (v, f)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.isoext_mc_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
