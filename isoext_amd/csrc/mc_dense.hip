@@ -45,6 +45,7 @@
 namespace iso {
 
 static __constant__ uint64_t cTris[2][256];
+static __constant__ uint8_t cNtri[2][256];   // triangles per case (the low 4 bits of cTris)
 
 
 // Append to special list k (0 cand, 1 scell, 2 kept, 3 hole) in segment
@@ -989,17 +990,22 @@ struct EmitLds {
     static constexpr int PX = TX + 2, PY = TY + 2, PC = NZC + 1;   // mask patch
     static constexpr int QY = TY + 1, QC = NZC + 1;                // vertex bases
     static constexpr int NUNITS = TX * TY * NZC;                   // (cell row, chunk), chunk fastest
+    // active-cell offsets fit 16 bits when the tile has fewer than 2^16
+    // cells; slot counts (at most 5 faces per slot) and unit indices always
+    // do.  The narrow types keep the (16, 16, 2) tile under 32 KB, five
+    // workgroups per CU.
+    using NaT = std::conditional_t<(TX * TY * 64 * NZC < 65536), uint16_t, uint32_t>;
+    static_assert(NUNITS < 32768 && 5 * NT < 65536, "narrow slot types");
     uint64_t w[PX * PY * PC];
     uint64_t amask[NUNITS];           // active cells of each unit
     int32_t vb[(TX + 1) * QY * QC];   // first vertex id per (point row, chunk)
     int32_t fb[TX * TY];              // first face id per cell row (this z-segment)
-    uint32_t na[NUNITS + 1];          // active cells per unit -> exclusive offsets
-    int32_t unit[NT];                 // batch: unit of each slot
-    uint32_t fsc[NT + 1];             // batch: faces per slot -> exclusive offsets
+    NaT na[NUNITS + 1];               // active cells per unit -> exclusive offsets
+    int16_t unit[NT];                 // batch: unit of each slot
+    uint16_t fsc[NT + 1];             // batch: faces per slot -> exclusive offsets
     uint32_t tsum[NT / 64];
     int32_t carry[2];                 // faces of the straddling row's cells before a batch
     int32_t ids[12][NT];              // batch: vertex id of each edge of each slot's cell
-    uint64_t tris[256];               // the method's packed triangle table
     float cx[TX + 1], cy[TY + 1], cz[64 * NZC + 1];   // grid coordinates of the tile's points
 };
 
@@ -1036,8 +1042,8 @@ __device__ __forceinline__ int vertex_id(const EmitLds<TX, TY, NZC, NT> &s, int 
     return s.vb[(opx * L::QY + opy) * L::QC + j] + r;
 }
 
-template <int N, int NT>
-__device__ __forceinline__ void block_exscan_lds(uint32_t *a, uint32_t *tsum) {
+template <int N, int NT, typename T>
+__device__ __forceinline__ void block_exscan_lds(T *a, uint32_t *tsum) {
     constexpr int NW = NT / 64, PER = (N + NT - 1) / NT;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lo = tid * PER, hi = min(lo + PER, N);
@@ -1054,26 +1060,26 @@ __device__ __forceinline__ void block_exscan_lds(uint32_t *a, uint32_t *tsum) {
     }
     run += inc - sum;
     for (int i = lo; i < hi; ++i) {
-        uint32_t v = a[i];
-        a[i] = run;
+        const uint32_t v = a[i];
+        a[i] = (T) run;
         run += v;
     }
-    if (tid == 0) a[N] = tot;
+    if (tid == 0) a[N] = (T) tot;
     __syncthreads();
 }
 
 // Inclusive running maximum over a[0, NT) (one element per thread).
-template <int NT>
-__device__ __forceinline__ void block_maxscan_lds(int32_t *a, uint32_t *tsum) {
+template <int NT, typename T>
+__device__ __forceinline__ void block_maxscan_lds(T *a, uint32_t *tsum) {
     constexpr int NW = NT / 64;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int32_t m = wave_incl_max(a[tid]);
+    int32_t m = wave_incl_max((int32_t) a[tid]);
     if (lane == 63) tsum[wave] = (uint32_t) m;
     __syncthreads();
 #pragma unroll
     for (int w = 0; w < NW; ++w)
         if (w < wave) m = max(m, (int32_t) tsum[w]);
-    a[tid] = m;
+    a[tid] = (T) m;
     __syncthreads();
 }
 
@@ -1140,7 +1146,6 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             if (tk + 1 < p.S) vnext_pf[k] = p.blk_v[(e + 1) / kMcScanBlock] + (int64_t) p.cnt_v[e + 1];
         }
     }
-    for (int c = tid; c < 256; c += NT) s.tris[c] = cTris[p.method][c];
     for (int c = tid; c < TX + TY + 64 * NZC + 3; c += NT) {
         if (c <= TX) {
             s.cx[c] = grid_coord(x0 + c, p.X, p.amin[0], p.amax[0]);
@@ -1175,7 +1180,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
         if (cx < nx && cy < ny && zlo < p.Z - 1)
             act = cell_active(unit_cells(s, cx, cy, j)) & low_bits(p.Z - 1 - zlo);
         s.amask[u] = act;
-        s.na[u] = (uint32_t) __popcll(act);
+        s.na[u] = (typename L::NaT) __popcll(act);
     }
     __syncthreads();
     block_exscan_lds<L::NUNITS, NT>(s.na, s.tsum);
@@ -1300,7 +1305,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                         (((w10 >> k) & 1) << 4) | (((up >> 2) & 1) << 5) | (((w11 >> k) & 1) << 6) |
                         (((up >> 3) & 1) << 7));
         }
-        s.fsc[tid] = live ? (uint32_t) (s.tris[cs] & 15) : 0u;
+        s.fsc[tid] = live ? (uint16_t) cNtri[p.method][cs] : (uint16_t) 0;
         __syncthreads();
         block_exscan_lds<NT, NT>(s.fsc, s.tsum);
         // faces of the straddling row's cells before the next batch
@@ -1479,7 +1484,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
                 });
             }
             // faces
-            const uint64_t wl = s.tris[cs];
+            const uint64_t wl = cTris[p.method][cs];
             const int ntri = (int) (wl & 15);
             int64_t slot = (int64_t) s.fb[r] + fo;
             bool dropped = false;   // a face of this cell is degenerate
@@ -1539,8 +1544,10 @@ __device__ __forceinline__ int64_t emit_block_tile(const McParams &p) {
     return (int) blockIdx.x < *p.tile_count ? (int64_t) p.tile_list[blockIdx.x] : -1;
 }
 
+// Five waves per SIMD: 96 VGPRs, and the narrow LDS slot types keep the
+// (16, 16, 2) and smaller tiles under 32 KB (five workgroups per CU).
 template <int TX, int TY, int NZC, int NT>
-__global__ void __launch_bounds__(NT) k_emit(McParams p) {
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5, 8))) k_emit(McParams p) {
     const int64_t tile = emit_block_tile(p);
     if (tile >= 0) emit_tile<TX, TY, NZC, NT>(p, tile, (int) (blockIdx.x % p.seg.nseg));
 }
@@ -2133,11 +2140,15 @@ hipError_t mc_upload_luts() {
     static bool done = false;
     if (done) return hipSuccess;
     uint64_t h[2][256];
+    uint8_t n[2][256];
     for (int i = 0; i < 256; ++i) {
         h[0][i] = kNagaeTris[i];
         h[1][i] = kLorensenTris[i];
+        n[0][i] = (uint8_t) (h[0][i] & 15);
+        n[1][i] = (uint8_t) (h[1][i] & 15);
     }
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cTris), h, sizeof(h));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cNtri), n, sizeof(n));
     if (e == hipSuccess) done = true;
     return e;
 }
@@ -3122,7 +3133,6 @@ __device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int 
                           ((int64_t) (o >> 31) << 62);
         }
     }
-    for (int c = tid; c < 256; c += NT) s.tris[c] = cTris[p.method][c];
     for (int c = tid; c < TX + TY + 64 + 3; c += NT) {
         if (c <= TX) {
             s.cx[c] = grid_coord(x0 + c, p.X, p.amin[0], p.amax[0]);
@@ -3144,7 +3154,7 @@ __device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int 
         uint64_t act = 0;
         if (cx < nx && cy < ny && zlo < p.Z - 1) act = cell_active(unit_cells(s, cx, cy, 0)) & low_bits(p.Z - 1 - zlo);
         s.amask[u] = act;
-        s.na[u] = (uint32_t) __popcll(act);
+        s.na[u] = (typename L::NaT) __popcll(act);
     }
     __syncthreads();
     block_exscan_lds<L::NUNITS, NT>(s.na, s.tsum);
@@ -3196,7 +3206,7 @@ __device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int 
                         (((w10 >> k) & 1) << 4) | (((up >> 2) & 1) << 5) | (((w11 >> k) & 1) << 6) |
                         (((up >> 3) & 1) << 7));
         }
-        s.fsc[tid] = live ? (uint32_t) (s.tris[cs] & 15) : 0u;
+        s.fsc[tid] = live ? (uint16_t) cNtri[p.method][cs] : (uint16_t) 0;
         __syncthreads();
         block_exscan_lds<NT, NT>(s.fsc, s.tsum);
         if (tid == NT - 1 && j0 + NT < nactive) {
@@ -3290,7 +3300,7 @@ __device__ __forceinline__ void faces_tile(const McParams &p, int64_t tile, int 
                     if (near_endpoint(ra, a, b)) special = true;
                 }
             }
-            const uint64_t wl = s.tris[cs];
+            const uint64_t wl = cTris[p.method][cs];
             const int ntri = (int) (wl & 15);
             int64_t slot = (int64_t) s.fb[rrow] + fo;
             bool dropped = false;   // a face of this cell is degenerate
