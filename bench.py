@@ -605,7 +605,6 @@ def main():
     m = METHODS[args.method]
 
     import ctypes
-    L.isoext_set_timing(1)
     kernel_ms = []
     stages = []
     st = (ctypes.c_float * 5)()
@@ -613,15 +612,21 @@ def main():
     cfg = None
     for _ in range(args.warmup):
         v, f, info = mc_dense_raw(grid.values, shape, grid.aabb_min, grid.aabb_max, 0.0, m, args.cfg)
+    # the timed steps run the path as a caller does: no stage events
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        v, f, info = mc_dense_raw(grid.values, shape, grid.aabb_min, grid.aabb_max, 0.0, m, args.cfg)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    # stage durations from HIP events on the launch stream, in steps of their own
+    L.isoext_set_timing(1)
+    for _ in range(max(5, args.steps)):
         v, f, info = mc_dense_raw(grid.values, shape, grid.aabb_min, grid.aabb_max, 0.0, m, args.cfg)
         kernel_ms.append(L.isoext_last_kernel_ms())
         L.isoext_mc_last_stage_ms(st)
         stages.append(list(st))
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
     L.isoext_set_timing(0)
     if args.calibrate_copy:
         src = torch.ones(args.calibrate_copy * 2 ** 18, dtype=torch.float32, device=dev)
