@@ -176,6 +176,7 @@ inline bool csg_list_fits(int64_t X, int64_t Y, int64_t W, int64_t S) {
 }
 constexpr int kCsgFillGroups = 2048;   // phase-2 workgroups
 constexpr int kCsgFillWaves = 4;       // waves per phase-2 workgroup (they share a block's evaluations)
+constexpr int kCsgFillBatchGroups = 4096;   // phase-2 workgroups of a batch (one wave each)
 // Emit tile configuration of SDF-program fields, (8, 8, 1): small tiles, so
 // that a tile's distinct corner points fit its LDS cache.
 constexpr int kProgramTileConfig = 5;

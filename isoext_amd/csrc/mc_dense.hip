@@ -828,11 +828,12 @@ __device__ void csg_test_block(const McParams &p, const CsgProgram &P, int64_t b
     }
 }
 
-// Phase 2 for an undecided block b by the kCsgFillWaves waves of a
-// workgroup (all of them call it); qs: its quarters' states from phase 1
-// (0: test every column).  Wave 0 tests the columns and shares the masks
-// through LDS (wacc[16], wacc[17]); the rounds of point evaluations are dealt
-// to the waves, whose partial words meet in LDS (wacc[0, 16)).
+// Phase 2 for an undecided block b by the NW waves of a workgroup (all of
+// them call it); qs: its quarters' states from phase 1 (0: test every
+// column).  Wave 0 tests the columns and shares the masks through LDS
+// (wacc[16], wacc[17]); the rounds of point evaluations are dealt to the
+// waves, whose partial words meet in LDS (wacc[0, 16)).
+template <int NW>
 __device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b, uint32_t qs, double cs,
                                uint64_t *wacc) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -864,7 +865,7 @@ __device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b
     const uint64_t known_in = wacc[16], need = wacc[17];
     const int nn = __popcll(need);
     uint64_t word = 0;   // lane r < 16: this wave's bits of row r
-    for (int j0 = 64 * wave; j0 < 16 * nn; j0 += 64 * kCsgFillWaves) {
+    for (int j0 = 64 * wave; j0 < 16 * nn; j0 += 64 * NW) {
         const int j = j0 + lane, r = lane & 15;
         const bool act = j < 16 * nn && (r >> 2) < k.nx && (r & 3) < k.ny;
         const int L = j < 16 * nn ? nth_set_bit(need, j >> 4) : 0;
@@ -888,16 +889,17 @@ __device__ void csg_fill_block(const McParams &p, const CsgProgram &P, int64_t b
 
 // Phase 2 over the list, or over every block (repeating the test); one
 // block per workgroup at a time.
+template <int NW>
 __device__ void csg_fill(const McParams &p, const CsgProgram &P, double cs, int64_t first, int64_t stride,
                          uint64_t *wacc) {
     if (p.csg_list) {
         const int64_t n = __builtin_amdgcn_readfirstlane((int) *(volatile uint32_t *) p.cnt_f);
         for (int64_t i = first; i < n; i += stride)
-            csg_fill_block(p, P, (int64_t) p.cnt_v[i], p.cnt_f[1 + i], cs, wacc);
+            csg_fill_block<NW>(p, P, (int64_t) p.cnt_v[i], p.cnt_f[1 + i], cs, wacc);
     } else {
         const int64_t nb = csg_classify_tasks(p.X, p.Y, p.W);
         for (int64_t b = first; b < nb; b += stride)
-            if (csg_block_sign(p, P, csg_block(p, b), cs) == 0) csg_fill_block(p, P, b, 0u, cs, wacc);
+            if (csg_block_sign(p, P, csg_block(p, b), cs) == 0) csg_fill_block<NW>(p, P, b, 0u, cs, wacc);
     }
 }
 
@@ -916,7 +918,7 @@ __global__ void __launch_bounds__(64 * kCsgFillWaves) k_csg_fill(McParams p) {
     __shared__ uint64_t wacc[18];
     csg_stage(P, p.csg);
     __syncthreads();
-    csg_fill(p, P, csg_coord_scale(p), blockIdx.x, gridDim.x, wacc);
+    csg_fill<kCsgFillWaves>(p, P, csg_coord_scale(p), blockIdx.x, gridDim.x, wacc);
 }
 
 // LDS of the fused emit: per 64-point z chunk of the tile, the spheres that
@@ -1766,20 +1768,55 @@ __global__ void __launch_bounds__(256) k_csg_blocks_batch(const McBatchItem *__r
     }
 }
 
-// Phase 2 of every program item of a batch.
-__global__ void __launch_bounds__(64 * kCsgFillWaves) k_csg_fill_batch(const McBatchItem *__restrict__ items,
-                                                                       const McBatchHdr *__restrict__ hdr) {
+// Phase 2 of every program item of a batch, one wave per workgroup: the
+// blocks of small grids leave few point rounds to share, and four-wave
+// groups measured slower here (serving with program requests: 538 vs
+// 363 ms per pass).  The items' phase-2 blocks (listed undecided blocks, or
+// every block of an item without a list) form one concatenated range that
+// is cut into a contiguous slice per workgroup; a workgroup stages a
+// program again only when its slice crosses into the next item.
+__global__ void __launch_bounds__(64) k_csg_fill_batch(const McBatchItem *__restrict__ items,
+                                                      const McBatchHdr *__restrict__ hdr) {
+    constexpr int NT = 64;
     __shared__ CsgProgram P;
     __shared__ uint64_t wacc[18];
-    for (int i = 0; i < (int) hdr->n; ++i) {
+    __shared__ uint32_t work[kMaxBatch + 1];
+    __shared__ int64_t wstart[kMaxBatch + 1];
+    __shared__ uint32_t tsum[NT / 64];
+    const int n = (int) hdr->n;
+    for (int i = threadIdx.x; i < kMaxBatch; i += NT) {
+        uint32_t w = 0;
+        if (i < n) {
+            const McParams &p = items[i].p;
+            if (p.csg && p.X >= 2 && p.Y >= 2 && p.Z >= 2)
+                w = p.csg_list ? *(volatile uint32_t *) p.cnt_f : (uint32_t) csg_classify_tasks(p.X, p.Y, p.W);
+        }
+        work[i] = w;
+    }
+    __syncthreads();
+    block_exscan_lds<kMaxBatch, NT>(work, tsum);
+    for (int i = threadIdx.x; i <= n; i += NT) wstart[i] = (int64_t) work[i];
+    __syncthreads();
+    const int64_t total = wstart[n];
+    const int64_t chunk = (total + gridDim.x - 1) / gridDim.x;
+    const int64_t g0 = (int64_t) blockIdx.x * chunk, g1 = min(total, g0 + chunk);
+    int cur = -1;
+    double cs = 0.0;
+    for (int64_t g = g0; g < g1; ++g) {
+        const int i = __builtin_amdgcn_readfirstlane(find_start(wstart, n, g));
         const McParams &p = items[i].p;
-        if (!p.csg || p.X < 2 || p.Y < 2 || p.Z < 2) continue;
-        const int64_t work = p.csg_list ? (int64_t) *(volatile uint32_t *) p.cnt_f : csg_classify_tasks(p.X, p.Y, p.W);
-        if ((int64_t) blockIdx.x >= work) continue;
-        __syncthreads();
-        csg_stage(P, p.csg);
-        __syncthreads();
-        csg_fill(p, P, csg_coord_scale(p), blockIdx.x, gridDim.x, wacc);
+        if (i != cur) {
+            __syncthreads();
+            csg_stage(P, p.csg);
+            __syncthreads();
+            cur = i;
+            cs = csg_coord_scale(p);
+        }
+        const int64_t k = g - wstart[i];
+        if (p.csg_list)
+            csg_fill_block<1>(p, P, (int64_t) p.cnt_v[k], p.cnt_f[1 + k], cs, wacc);
+        else if (csg_block_sign(p, P, csg_block(p, k), cs) == 0)
+            csg_fill_block<1>(p, P, k, 0u, cs, wacc);
     }
 }
 
@@ -2289,7 +2326,7 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     if (h.ngtask > 0) {
         const unsigned gb = (unsigned) std::min<int64_t>((h.ngtask + 255) / 256, 8192);
         hipLaunchKernelGGL(k_csg_blocks_batch, dim3(gb), dim3(256), 0, stream, items, hdr);
-        hipLaunchKernelGGL(k_csg_fill_batch, dim3(kCsgFillGroups), dim3(64 * kCsgFillWaves), 0, stream, items, hdr);
+        hipLaunchKernelGGL(k_csg_fill_batch, dim3(kCsgFillBatchGroups), dim3(64), 0, stream, items, hdr);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const unsigned nb = (unsigned) std::max<int64_t>(1, h.nblk);
