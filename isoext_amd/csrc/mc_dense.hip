@@ -1768,16 +1768,18 @@ __global__ void __launch_bounds__(256) k_csg_blocks_batch(const McBatchItem *__r
     }
 }
 
-// Phase 2 of every program item of a batch, one wave per workgroup: the
-// blocks of small grids leave few point rounds to share, and four-wave
-// groups measured slower here (serving with program requests: 538 vs
-// 363 ms per pass).  The items' phase-2 blocks (listed undecided blocks, or
-// every block of an item without a list) form one concatenated range that
-// is cut into a contiguous slice per workgroup; a workgroup stages a
-// program again only when its slice crosses into the next item.
-__global__ void __launch_bounds__(64) k_csg_fill_batch(const McBatchItem *__restrict__ items,
-                                                      const McBatchHdr *__restrict__ hdr) {
-    constexpr int NT = 64;
+// Phase 2 of every program item of a batch by NW-wave workgroups.  The
+// items' phase-2 blocks (listed undecided blocks, or every block of an item
+// without a list) form one concatenated range that is cut into a contiguous
+// slice per workgroup; a workgroup stages a program again only when its
+// slice crosses into the next item.  A batch with one program item takes
+// four-wave groups (fused DC at 256^3: 0.78 vs 0.89 ms with one wave); a
+// batch of several takes one-wave groups, which measured faster under the
+// server's concurrent batches (program requests: 363 vs 538 ms per pass).
+template <int NW>
+__global__ void __launch_bounds__(64 * NW) k_csg_fill_batch(const McBatchItem *__restrict__ items,
+                                                           const McBatchHdr *__restrict__ hdr) {
+    constexpr int NT = 64 * NW;
     __shared__ CsgProgram P;
     __shared__ uint64_t wacc[18];
     __shared__ uint32_t work[kMaxBatch + 1];
@@ -1814,9 +1816,9 @@ __global__ void __launch_bounds__(64) k_csg_fill_batch(const McBatchItem *__rest
         }
         const int64_t k = g - wstart[i];
         if (p.csg_list)
-            csg_fill_block<1>(p, P, (int64_t) p.cnt_v[k], p.cnt_f[1 + k], cs, wacc);
+            csg_fill_block<NW>(p, P, (int64_t) p.cnt_v[k], p.cnt_f[1 + k], cs, wacc);
         else if (csg_block_sign(p, P, csg_block(p, k), cs) == 0)
-            csg_fill_block<1>(p, P, k, 0u, cs, wacc);
+            csg_fill_block<NW>(p, P, k, 0u, cs, wacc);
     }
 }
 
@@ -2326,7 +2328,11 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
     if (h.ngtask > 0) {
         const unsigned gb = (unsigned) std::min<int64_t>((h.ngtask + 255) / 256, 8192);
         hipLaunchKernelGGL(k_csg_blocks_batch, dim3(gb), dim3(256), 0, stream, items, hdr);
-        hipLaunchKernelGGL(k_csg_fill_batch, dim3(kCsgFillBatchGroups), dim3(64), 0, stream, items, hdr);
+        if (h.n_prog == 1)
+            hipLaunchKernelGGL(k_csg_fill_batch<kCsgFillWaves>, dim3(kCsgFillGroups), dim3(64 * kCsgFillWaves), 0,
+                               stream, items, hdr);
+        else
+            hipLaunchKernelGGL(k_csg_fill_batch<1>, dim3(kCsgFillBatchGroups), dim3(64), 0, stream, items, hdr);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const unsigned nb = (unsigned) std::max<int64_t>(1, h.nblk);
