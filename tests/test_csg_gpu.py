@@ -200,3 +200,28 @@ def test_sdf_single_primitives_and_empty(gpu):
     assert dual_contouring_sdf(far, (64, 64, 64)) == (None, None)
     inside_all = program(S.SphereSDF(10.0))
     assert marching_cubes_sdf(inside_all, (64, 64, 64), level=0.0) == (None, None)
+
+
+@pytest.mark.parametrize("waves", ["one item", "several items"])
+def test_batches_with_thin_program_grids(gpu, waves):
+    """Program items too thin for the undecided-block list (every block is
+    tested again in the second classify phase) batched alone and beside
+    listed items: each mesh equals the stored-grid extraction."""
+    from isoext_amd import sdf as S
+    from isoext_amd.batch import GridJob, mc_batch
+    tree = S.UnionOp([S.TorusSDF(0.5, 0.2), S.TranslationOp(S.SphereSDF(0.3), [0.0, 0.0, 0.5])])
+    prog = program(tree)
+    thin = [(2, 2, 1200), (3, 5, 700), (9, 2, 300)]
+    box = dict(aabb_min=(-0.05, -0.05, -1.0), aabb_max=(0.05, 0.05, 1.0))   # along the z axis: crossings
+    jobs = [GridJob(None, 0.0, 0, sdf=prog, shape=s, **box) for s in thin]
+    if waves == "several items":
+        jobs += [GridJob(None, 0.0, 0, sdf=program(trees()["torus_box"]), shape=(48, 40, 64)),
+                 GridJob(None, 0.01, 1, sdf=program(trees()["carved"]), shape=(40, 44, 36))]
+    else:
+        jobs = jobs[:1]
+    got = mc_batch(jobs)
+    for j, (v, f) in zip(jobs, got):
+        vals = j.sdf.grid(j.shape, j.aabb_min, j.aabb_max)
+        ref = mc_batch([GridJob(vals, j.level, j.method, aabb_min=j.aabb_min, aabb_max=j.aabb_max)])[0]
+        assert f is not None and f.shape[0] > 0, j.shape
+        assert torch.equal(v, ref[0]) and torch.equal(f, ref[1]), j.shape
