@@ -56,6 +56,7 @@ constexpr int64_t kFixSmallSlots = 2048;   // candidate + special-edge entries t
 // final[2] of an item whose removed vertices did not fit the small path: its
 // outputs are untouched and the multi-kernel fix-up must run instead
 constexpr int64_t kFixSmallOverflow = 2;
+constexpr int kFixMaxDevices = 64;        // per-thread scratch caches are kept per device up to this id
 bool fixup_small_ok(const SpecialLists &sp);
 // Scratch bytes of one small fix-up item and the item's table sizes.
 size_t fixup_small_scratch(const SpecialLists &sp, int64_t &nslots, int64_t &npslots);

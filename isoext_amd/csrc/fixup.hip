@@ -704,7 +704,13 @@ hipError_t weld_fixup_small_batch(int n, const SpecialLists *sp, float *const *v
         dev = ext->dev;
         host = ext->host;
     } else {
-        thread_local Cache c;
+        // one cache per (thread, device): a thread that moves between GPUs
+        // never hands one GPU's scratch to another
+        int devid = 0;
+        ISO_TRY(hipGetDevice(&devid));
+        if (devid < 0 || devid >= kFixMaxDevices) return hipErrorInvalidDevice;
+        thread_local Cache caches[kFixMaxDevices];
+        Cache &c = caches[devid];
         if (c.dev_bytes < bytes) {
             if (c.dev) {
                 ISO_TRY(hipStreamSynchronize(st));
@@ -858,7 +864,11 @@ hipError_t weld_fixup_impl(const SpecialLists &sp, float *v, int32_t *f, int64_t
                 if (p) (void) hipFree(p);
             }
         };
-        static thread_local DevCache cache;
+        int devid = 0;   // one cache per (thread, device)
+        ISO_TRY(hipGetDevice(&devid));
+        if (devid < 0 || devid >= kFixMaxDevices) return hipErrorInvalidDevice;
+        static thread_local DevCache caches[kFixMaxDevices];
+        DevCache &cache = caches[devid];
         if (cache.bytes < bytes) {
             if (cache.p) {
                 ISO_TRY(hipStreamSynchronize(st));

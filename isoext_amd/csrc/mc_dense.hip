@@ -33,6 +33,8 @@
 //     pass (the reference welds by position equality and drops degenerate
 //     triangles, src/utils.cu:49-55, src/mc/nagae.cu:71).
 
+#include <atomic>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -2176,8 +2178,18 @@ int mc_auto_config(const int64_t shape[3]) {
 }
 
 hipError_t mc_upload_luts() {
-    static bool done = false;
-    if (done) return hipSuccess;
+    // once per device (each GPU holds its own copy of the constant tables);
+    // host threads that call first together wait for one upload
+    constexpr int kMaxDevices = 64;
+    static std::atomic<bool> done[kMaxDevices];
+    static std::mutex mu;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (done[dev].load(std::memory_order_acquire)) return hipSuccess;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[dev].load(std::memory_order_relaxed)) return hipSuccess;
     uint64_t h[2][256];
     uint8_t n[2][256];
     for (int i = 0; i < 256; ++i) {
@@ -2186,9 +2198,9 @@ hipError_t mc_upload_luts() {
         n[0][i] = (uint8_t) (h[0][i] & 15);
         n[1][i] = (uint8_t) (h[1][i] & 15);
     }
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cTris), h, sizeof(h));
+    e = hipMemcpyToSymbol(HIP_SYMBOL(cTris), h, sizeof(h));
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(cNtri), n, sizeof(n));
-    if (e == hipSuccess) done = true;
+    if (e == hipSuccess) done[dev].store(true, std::memory_order_release);
     return e;
 }
 

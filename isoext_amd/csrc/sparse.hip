@@ -21,6 +21,8 @@
 // src/mc/nagae.cu:71; a position is emitted only if a kept triangle uses it.
 // The per-vertex representative is exported for the backward pass.
 
+#include <atomic>
+#include <mutex>
 #include "common.h"
 #include "mc_luts.h"
 #include "sparse.h"
@@ -743,15 +745,25 @@ hipError_t sparse_crossing_flags(const float *values, int64_t n, float level, ui
 }
 
 hipError_t sparse_mc_luts() {
-    static bool done = false;
-    if (done) return hipSuccess;
+    // once per device (each GPU holds its own copy of the constant table);
+    // host threads that call first together wait for one upload
+    constexpr int kMaxDevices = 64;
+    static std::atomic<bool> done[kMaxDevices];
+    static std::mutex mu;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (done[dev].load(std::memory_order_acquire)) return hipSuccess;
+    std::lock_guard<std::mutex> lock(mu);
+    if (done[dev].load(std::memory_order_relaxed)) return hipSuccess;
     uint64_t h[2][256];
     for (int i = 0; i < 256; ++i) {
         h[0][i] = kNagaeTris[i];
         h[1][i] = kLorensenTris[i];
     }
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(cSTris), h, sizeof(h));
-    if (e == hipSuccess) done = true;
+    e = hipMemcpyToSymbol(HIP_SYMBOL(cSTris), h, sizeof(h));
+    if (e == hipSuccess) done[dev].store(true, std::memory_order_release);
     return e;
 }
 
