@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--cfg", type=int, default=None, help="tile configuration (default: autotuned)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fused", action="store_true", help="skip the fused sphere-field extraction")
-    ap.add_argument("--cpu-planes", type=int, default=129, help="x-planes in the CPU sample")
+    ap.add_argument("--cpu-planes", type=int, default=257, help="x-planes in the CPU sample")
     ap.add_argument("--no-dc", action="store_true", help="skip the 256^3 dual-contouring summary")
     ap.add_argument("--no-sparse", action="store_true", help="skip the 2048^3 sparse summary")
     ap.add_argument("--serving", type=int, default=10000,
