@@ -1384,10 +1384,25 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
             if (p.all_special) {
                 candm = status;
             } else {
+                // a cell whose every corner lies further from the level than
+                // cand_eps * (max - min) of its corners has no edge that passes
+                // the prefilter (|vb - va| <= max - min, and rounding is
+                // monotone); NaN or infinite corners fall through to it
+                float vlo = cc.v[0], vhi = cc.v[0];
+#pragma unroll
+                for (int q = 1; q < 8; ++q) {
+                    vlo = fminf(vlo, cc.v[q]);
+                    vhi = fmaxf(vhi, cc.v[q]);
+                }
+                const float dcell = (vhi - vlo) * p.cand_eps;
+                bool near = false;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) near |= !(fabsf(cc.level - cc.v[q]) > dcell);
                 uint32_t maybe = 0;
-                for_edges([&](auto E) {
-                    if (((status >> E) & 1) && edge_maybe_candidate<E>(cc, p.cand_eps)) maybe |= 1u << E;
-                });
+                if (near)
+                    for_edges([&](auto E) {
+                        if (((status >> E) & 1) && edge_maybe_candidate<E>(cc, p.cand_eps)) maybe |= 1u << E;
+                    });
                 if (maybe) {
                     for_edges([&](auto E) {
                         if (((maybe >> E) & 1) && edge_is_candidate<E>(cc)) candm |= 1u << E;
