@@ -59,3 +59,47 @@ def test_python_grid_rejects_non_finite_aabb():
     with pytest.raises(ValueError):
         _vec3([0.0, float("nan"), 0.0], "aabb_min")
     assert torch is not None
+
+
+def test_plan_entry_points_check_arguments_without_gpu():
+    """isoext_mc_plan_*: null handles and bad arguments are argument errors
+    returned before any HIP call; destroying a null plan is a no-op."""
+    import math
+    lib = ctypes.CDLL(str(ROOT / "isoext_amd/libisoext_amd.so"))
+    lib.isoext_last_error.restype = ctypes.c_char_p
+    vp, i64 = ctypes.c_void_p, ctypes.c_int64
+    I3, F3, I6 = i64 * 3, ctypes.c_float * 3, i64 * 6
+    lib.isoext_mc_plan_create.restype = ctypes.c_int
+    lib.isoext_mc_plan_create.argtypes = [vp, I3, F3, F3, ctypes.c_float, ctypes.c_int, ctypes.c_int, vp, i64, i64,
+                                          vp, i64, vp, i64, ctypes.POINTER(vp)]
+    lib.isoext_mc_plan_run.restype = ctypes.c_int
+    lib.isoext_mc_plan_run.argtypes = [vp, I6, vp]
+    lib.isoext_mc_plan_destroy.restype = None
+    lib.isoext_mc_plan_destroy.argtypes = [vp]
+    shape, amin, amax = I3(8, 8, 8), F3(-1, -1, -1), F3(1, 1, 1)
+    dummy = vp(256)   # never dereferenced: every call below fails its checks first
+
+    def create(values, method=0, lo=amin, out=True):
+        h = vp(12345)
+        rc = lib.isoext_mc_plan_create(values, shape, lo, amax, 0.0, method, 0, dummy, 1 << 20, 65536, dummy,
+                                       1024, dummy, 1024, ctypes.byref(h) if out else None)
+        return rc, h
+
+    rc, _ = create(dummy, out=False)
+    assert rc == -1 and b"plan" in lib.isoext_last_error()
+    rc, h = create(None)
+    assert rc == -1 and h.value is None and b"values" in lib.isoext_last_error()
+    rc, h = create(dummy, method=7)
+    assert rc == -1 and h.value is None and b"method" in lib.isoext_last_error()
+    rc, h = create(dummy, lo=F3(-1, math.nan, -1))
+    assert rc == -1 and h.value is None
+    assert lib.isoext_mc_plan_run(None, I6(), None) == -1
+    lib.isoext_mc_plan_destroy(None)
+
+
+def test_plan_wrapper_takes_device_tensors_only():
+    import pytest
+    import torch
+    from isoext_amd.mc import MarchingCubesPlan
+    with pytest.raises(TypeError):
+        MarchingCubesPlan(torch.zeros((4, 4, 4)))
