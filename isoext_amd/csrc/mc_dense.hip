@@ -1787,9 +1787,10 @@ __global__ void __launch_bounds__(256) k_csg_blocks_batch(const McBatchItem *__r
 
 // Phase 2 of every program item of a batch by NW-wave workgroups.  The
 // items' phase-2 blocks (listed undecided blocks, or every block of an item
-// without a list) form one concatenated range that is cut into a contiguous
-// slice per workgroup; a workgroup stages a program again only when its
-// slice crosses into the next item.  A batch with one program item takes
+// without a list) form one concatenated range that the workgroups walk with
+// a grid stride (neighbouring blocks, often of similar cost, go to different
+// workgroups); the item only grows along a workgroup's walk, so it stages
+// each program at most once.  A batch with one program item takes
 // four-wave groups (fused DC at 256^3: 0.78 vs 0.89 ms with one wave); a
 // batch of several takes one-wave groups, which measured faster under the
 // server's concurrent batches (program requests: 363 vs 538 ms per pass).
@@ -1817,11 +1818,9 @@ __global__ void __launch_bounds__(64 * NW) k_csg_fill_batch(const McBatchItem *_
     for (int i = threadIdx.x; i <= n; i += NT) wstart[i] = (int64_t) work[i];
     __syncthreads();
     const int64_t total = wstart[n];
-    const int64_t chunk = (total + gridDim.x - 1) / gridDim.x;
-    const int64_t g0 = (int64_t) blockIdx.x * chunk, g1 = min(total, g0 + chunk);
     int cur = -1;
     double cs = 0.0;
-    for (int64_t g = g0; g < g1; ++g) {
+    for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
         const int i = __builtin_amdgcn_readfirstlane(find_start(wstart, n, g));
         const McParams &p = items[i].p;
         if (i != cur) {
