@@ -1612,57 +1612,72 @@ __global__ void mc_dense_reset(McParams p, SpecialLists init) {
 // (clamped) counts of the segments before it.
 static_assert(kSpSeg == 256, "sp_gather_seg maps one thread to each segment");
 __device__ __forceinline__ void sp_gather_seg(const McParams &p, int b) {
-    __shared__ int64_t part[4][4];
-    __shared__ int64_t off[4];
+    // 256 threads, nseg <= 256: thread t reads segment t's four counters in
+    // one round of loads; the offsets (clamped counts of the segments before
+    // b), block 0's totals (unclamped) and the capacity a call needs come
+    // from the same wave reductions, and the four lists are copied in one
+    // interleaved round per 256 entries
+    constexpr int NW = 4;
+    __shared__ int64_t part[4][NW], own[4];
+    __shared__ unsigned long long tot[4][NW], needw[NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // thread t reads segment t of each list
-    int64_t below[4];
+    const int nseg = p.seg.nseg;
+    unsigned long long c[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) c[k] = tid < nseg ? p.seg.ctr[(k * nseg + tid) * kSpStride] : 0ull;
+    unsigned long long need = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int64_t n = tid < b ? (int64_t) p.seg.ctr[(k * p.seg.nseg + tid) * kSpStride] : 0;
-        below[k] = min(n, p.seg.cap[k]);
+        const int64_t cl = min((int64_t) c[k], p.seg.cap[k]);
+        if (tid == b) own[k] = cl;
+        int64_t v = tid < b ? cl : 0;
+        unsigned long long t = c[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            v += __shfl_xor(v, o);
+            t += __shfl_xor(t, o);
+        }
+        if (lane == 0) {
+            part[k][wave] = v;
+            tot[k][wave] = t;
+        }
+        const unsigned long long mult = k == 1 ? 12 : (k >= 2 ? 5 : 1);
+        if (c[k] > 0) need = max(need, ((c[k] + mult - 1) / mult) * (unsigned long long) nseg);
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        int64_t v = below[k];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if (lane == 0) part[k][wave] = v;
-    }
+    for (int o = 32; o > 0; o >>= 1) need = max(need, (unsigned long long) __shfl_xor(need, o));
+    if (lane == 0) needw[wave] = need;
     __syncthreads();
-    if (tid < 4) off[tid] = part[tid][0] + part[tid][1] + part[tid][2] + part[tid][3];
-    __syncthreads();
+    int64_t off[4], n[4], nmax = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int64_t n = min((int64_t) p.seg.ctr[(k * p.seg.nseg + b) * kSpStride], p.seg.cap[k]);
-        const int64_t src = (int64_t) b * p.seg.cap[k], dst = off[k];
-        for (int64_t i = tid; i < n; i += 256) {
-            if (k == 0) p.sp->cand[dst + i] = p.seg.cand[src + i];
-            else if (k == 1) p.sp->scell[dst + i] = p.seg.scell[src + i];
-            else if (k == 2) p.sp->kept[dst + i] = p.seg.kept[src + i];
-            else p.sp->hole[dst + i] = p.seg.hole[src + i];
-        }
+        off[k] = part[k][0] + part[k][1] + part[k][2] + part[k][3];
+        n[k] = own[k];
+        nmax = max(nmax, n[k]);
     }
-    if (b == 0) {
-        // totals (unclamped) and the capacity a call needs
-        __shared__ unsigned long long raw[4], need;
-        if (tid < 4) raw[tid] = 0;
-        if (tid == 0) need = 0;
-        __syncthreads();
+    const int64_t b64 = b;
+    for (int64_t i = tid; i < nmax; i += 256) {
+        SpecialVertex cv{};
+        SpecialEdge se{};
+        int64_t kv = 0, hv = 0;
+        if (i < n[0]) cv = p.seg.cand[b64 * p.seg.cap[0] + i];
+        if (i < n[1]) se = p.seg.scell[b64 * p.seg.cap[1] + i];
+        if (i < n[2]) kv = p.seg.kept[b64 * p.seg.cap[2] + i];
+        if (i < n[3]) hv = p.seg.hole[b64 * p.seg.cap[3] + i];
+        if (i < n[0]) p.sp->cand[off[0] + i] = cv;
+        if (i < n[1]) p.sp->scell[off[1] + i] = se;
+        if (i < n[2]) p.sp->kept[off[2] + i] = kv;
+        if (i < n[3]) p.sp->hole[off[3] + i] = hv;
+    }
+    if (b == 0 && tid == 0) {
+        unsigned long long raw[4], nd = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const unsigned long long n = tid < p.seg.nseg ? p.seg.ctr[(k * p.seg.nseg + tid) * kSpStride] : 0ull;
-            atomicAdd(&raw[k], n);
-            const int mult = k == 1 ? 12 : (k >= 2 ? 5 : 1);
-            if (n > 0) atomicMax(&need, ((n + mult - 1) / mult) * (unsigned long long) p.seg.nseg);
-        }
-        __syncthreads();
-        if (tid == 0) {
-            p.sp->n_cand = raw[0];
-            p.sp->n_scell = raw[1];
-            p.sp->n_kept = raw[2];
-            p.sp->n_hole = raw[3];
-            p.sp->need = (int64_t) need;
-        }
+        for (int k = 0; k < 4; ++k) raw[k] = tot[k][0] + tot[k][1] + tot[k][2] + tot[k][3];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) nd = max(nd, needw[w]);
+        p.sp->n_cand = raw[0];
+        p.sp->n_scell = raw[1];
+        p.sp->n_kept = raw[2];
+        p.sp->n_hole = raw[3];
+        p.sp->need = (int64_t) nd;
     }
 }
 
