@@ -176,7 +176,10 @@ class MarchingCubesPlan:
 
     def __init__(self, values: torch.Tensor, level: float = 0.0, method: str = "nagae",
                  aabb_min=(-1.0, -1.0, -1.0), aabb_max=(1.0, 1.0, 1.0), cfg: int | None = None,
-                 headroom: float = 1.25):
+                 headroom: float = 1.25, special_cap: int | None = None):
+        """special_cap: capacity of the special-vertex lists (weld candidates
+        and their cells) the plan starts with; by default the larger of the
+        library default and twice the first extraction's need."""
         import ctypes
         from .grid import check_cuda_f32
         check_cuda_f32(values, 3, name="values")
@@ -205,8 +208,11 @@ class MarchingCubesPlan:
         L.isoext_mc_plan_destroy.argtypes = [ctypes.c_void_p]
         v, f, info = mc_dense_raw(values, self.shape, self.aabb_min, self.aabb_max, self.level, self.method,
                                   self.cfg)
-        with _hint_lock:
-            special = max(DEFAULT_SPECIAL_CAP, _special_hint.get(self.shape, 0), 2 * int(info["special"]))
+        if special_cap is None:
+            with _hint_lock:
+                special = max(DEFAULT_SPECIAL_CAP, _special_hint.get(self.shape, 0), 2 * int(info["special"]))
+        else:
+            special = max(int(special_cap), 1)
         self._build(int(v.shape[0]), int(f.shape[0]), special)
 
     def _build(self, nv: int, nf: int, special: int):

@@ -75,3 +75,26 @@ def test_plan_empty_surface(gpu):
         assert torch.equal(v, v_ref) and torch.equal(f, f_ref)
     finally:
         plan.close()
+
+
+def test_plan_grows_its_special_lists(gpu):
+    """A plan started with tiny special-vertex lists on a field whose values
+    sit on a coarse lattice (many corners exactly at the level): the run
+    reports the lists' need, the plan rebuilds with it, and the mesh equals
+    the plain call's."""
+    import numpy as np
+    from isoext_amd.mc import MarchingCubesPlan
+    n = 64
+    ax = np.linspace(-1.0, 1.0, n, dtype=np.float32)
+    x, y, z = np.meshgrid(ax, ax, ax, indexing="ij")
+    field = np.round((np.sqrt(x * x + y * y + z * z) - 0.6) * 8) / 8
+    values = torch.from_numpy(field.astype(np.float32)).cuda()
+    plan = MarchingCubesPlan(values, 0.0, special_cap=16)
+    try:
+        assert plan.special == 16
+        v, f = plan.run()
+        v_ref, f_ref = _plain(values, 0.0, 0, plan.cfg)
+        assert plan.special > 16
+        assert torch.equal(v, v_ref) and torch.equal(f, f_ref)
+    finally:
+        plan.close()
