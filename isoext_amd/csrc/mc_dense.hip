@@ -63,6 +63,29 @@ __device__ __forceinline__ int64_t seg_push(const McParams &p, int k, int seg) {
 
 __device__ __forceinline__ void set_err(const McParams &p, int code) { atomicOr(p.err, code); }
 
+// Clears what an extraction's later kernels count into (special-list
+// counters, tile flags, the duplicate-check slots, the header); thread t of
+// a grid of `stride` threads.
+__device__ __forceinline__ void reset_state(const McParams &p, SpecialLists init, int64_t t, int64_t stride) {
+    for (int64_t i = t; i < 4 * (int64_t) p.seg.nseg; i += stride) p.seg.ctr[i * kSpStride] = 0;
+    if (p.tile_flag) {
+        const int64_t ntiles = p.NI * p.NJ * p.NK;
+        for (int64_t i = t; i < ntiles; i += stride) p.tile_flag[i] = 0;
+    }
+    if (p.dslots)
+        for (int64_t i = t; i <= p.dmask; i += stride) p.dslots[i] = -1;
+    if (t == 0) {
+        if (p.tile_count) *p.tile_count = 0;
+        if (p.csg) p.cnt_f[0] = 0;   // the program classify's block list count
+        *p.err = 0;
+        p.totals[0] = p.totals[1] = 0;
+        init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
+        init.need = 0;
+        init.dup = p.dslots ? 0 : -1;
+        *p.sp = init;
+    }
+}
+
 // Bits [0, n), n clamped to [0, 64].
 __device__ __forceinline__ uint64_t low_bits(int64_t n) {
     return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1));
@@ -219,8 +242,14 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 // k_classify_v4 (Z % 64 == 0, float4-aligned rows): word w covers values
 // [64w, 64w + 64); 16 lanes load it with dwordx4 loads and merge their
 // nibbles with three in-group shuffles.
+// RESET: the grid first clears the extraction's state (reset_state), which
+// nothing in classify reads, so the stored-grid pipeline needs no separate
+// reset launch.
+template <bool RESET>
 __global__ void __launch_bounds__(256) k_classify_v4(const float *__restrict__ values, int64_t nwords, float level,
-                                                     uint64_t *__restrict__ bits) {
+                                                     uint64_t *__restrict__ bits, McParams rp, SpecialLists init) {
+    if constexpr (RESET)
+        reset_state(rp, init, (int64_t) blockIdx.x * blockDim.x + threadIdx.x, (int64_t) gridDim.x * blockDim.x);
     constexpr int U = 4;   // words in flight per 16-lane group
     const int lane = threadIdx.x & 63;
     const int grp = lane >> 4, q = lane & 15;
@@ -302,8 +331,12 @@ __device__ __forceinline__ void classify_flat_task(const float *__restrict__ val
     __builtin_amdgcn_wave_barrier();
 }
 
+template <bool RESET>
 __global__ void __launch_bounds__(256) k_classify(const float *__restrict__ values, int64_t N, int64_t Z, int64_t W,
-                                                  float level, uint64_t *__restrict__ bits) {
+                                                  float level, uint64_t *__restrict__ bits, McParams rp,
+                                                  SpecialLists init) {
+    if constexpr (RESET)
+        reset_state(rp, init, (int64_t) blockIdx.x * blockDim.x + threadIdx.x, (int64_t) gridDim.x * blockDim.x);
     __shared__ uint64_t ballots[4][kFlatChunks + 1];
     const int64_t wave = __builtin_amdgcn_readfirstlane((int) (blockIdx.x * 4 + (threadIdx.x >> 6)));
     const int64_t nwave = (int64_t) gridDim.x * 4;
@@ -1586,24 +1619,7 @@ __global__ void __launch_bounds__(NT) k_emit_program(McParams p) {
 // Reset of the per-call counters (and the active-tile flags when listed);
 // grid-stride over max(4 * nseg, tiles) threads.
 __global__ void mc_dense_reset(McParams p, SpecialLists init) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < 4 * p.seg.nseg) p.seg.ctr[t * kSpStride] = 0;
-    if (p.tile_flag) {
-        const int64_t ntiles = p.NI * p.NJ * p.NK;
-        for (int64_t i = t; i < ntiles; i += (int64_t) gridDim.x * blockDim.x) p.tile_flag[i] = 0;
-    }
-    if (p.dslots)
-        for (int64_t i = t; i <= p.dmask; i += (int64_t) gridDim.x * blockDim.x) p.dslots[i] = -1;
-    if (t == 0) {
-        if (p.tile_count) *p.tile_count = 0;
-        if (p.csg) p.cnt_f[0] = 0;   // the program classify's block list count
-        *p.err = 0;
-        p.totals[0] = p.totals[1] = 0;
-        init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
-        init.need = 0;
-        init.dup = p.dslots ? 0 : -1;
-        *p.sp = init;
-    }
+    reset_state(p, init, (int64_t) blockIdx.x * blockDim.x + threadIdx.x, (int64_t) gridDim.x * blockDim.x);
 }
 
 // Concatenate the special-list segments (in segment order) into the
@@ -2235,7 +2251,9 @@ hipError_t mc_upload_luts() {
 
 // Inside masks of a whole grid: dwordx4 loads when rows are whole aligned
 // words, the flat kernel otherwise.
-static void launch_classify(const McParams &p, hipStream_t stream) {
+// reset: the stored-grid classify clears the extraction's state itself
+// (returns true); program and sphere fields keep the separate reset kernel.
+static bool launch_classify(const McParams &p, hipStream_t stream, const SpecialLists *reset = nullptr) {
     const int64_t nwords = p.X * p.Y * p.W;
     if (p.csg) {
         const int64_t nb = csg_classify_tasks(p.X, p.Y, p.W);
@@ -2248,12 +2266,25 @@ static void launch_classify(const McParams &p, hipStream_t stream) {
         hipLaunchKernelGGL(k_classify_spheres, dim3(blocks), dim3(256), 0, stream, p);
     } else if (p.vec4 && p.Z % 64 == 0) {
         const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((nwords + 63) / 64, 8192));
-        hipLaunchKernelGGL(k_classify_v4, dim3(blocks), dim3(256), 0, stream, p.values, nwords, p.level, p.bits);
+        if (reset)
+            hipLaunchKernelGGL(k_classify_v4<true>, dim3(blocks), dim3(256), 0, stream, p.values, nwords, p.level,
+                               p.bits, p, *reset);
+        else
+            hipLaunchKernelGGL(k_classify_v4<false>, dim3(blocks), dim3(256), 0, stream, p.values, nwords, p.level,
+                               p.bits, p, SpecialLists{});
+        return reset != nullptr;
     } else {
         const int64_t N = p.X * p.Y * p.Z, ntask = (N + kFlatPts - 1) / kFlatPts;
         const unsigned blocks = (unsigned) std::max<int64_t>(1, std::min<int64_t>((ntask + 3) / 4, 8192));
-        hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, stream, p.values, N, p.Z, p.W, p.level, p.bits);
+        if (reset)
+            hipLaunchKernelGGL(k_classify<true>, dim3(blocks), dim3(256), 0, stream, p.values, N, p.Z, p.W, p.level,
+                               p.bits, p, *reset);
+        else
+            hipLaunchKernelGGL(k_classify<false>, dim3(blocks), dim3(256), 0, stream, p.values, N, p.Z, p.W, p.level,
+                               p.bits, p, SpecialLists{});
+        return reset != nullptr;
     }
+    return false;
 }
 
 void mc_params_divisors(McParams &p) {
@@ -2281,13 +2312,18 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     if (e != hipSuccess) return e;
     auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], stream) : hipSuccess; };
     if ((e = mark(0)) != hipSuccess) return e;
-    const int64_t ntiles = p.NI * p.NJ * p.NK;
-    const unsigned rblocks = (unsigned) std::max<int64_t>(
-        (4 * kSpSeg + 255) / 256, p.tile_flag ? std::min<int64_t>((ntiles + 255) / 256, 1024) : 0);
-    hipLaunchKernelGGL(mc_dense_reset, dim3(rblocks), dim3(256), 0, stream, p, sp_init);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // a stored grid's classify kernel clears the state itself; program and
+    // sphere fields, whose classify counts into it, take the reset kernel
+    const bool grid_field = !p.csg && p.nsph <= 0;
+    if (!grid_field) {
+        const int64_t ntiles = p.NI * p.NJ * p.NK;
+        const unsigned rblocks = (unsigned) std::max<int64_t>(
+            (4 * kSpSeg + 255) / 256, p.tile_flag ? std::min<int64_t>((ntiles + 255) / 256, 1024) : 0);
+        hipLaunchKernelGGL(mc_dense_reset, dim3(rblocks), dim3(256), 0, stream, p, sp_init);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
 
-    launch_classify(p, stream);
+    launch_classify(p, stream, grid_field ? &sp_init : nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(1)) != hipSuccess) return e;
 
