@@ -1703,27 +1703,45 @@ __global__ void __launch_bounds__(256) k_sp_gather(McParams p) { sp_gather_seg(p
 // groups them: equal coordinates): candidate indices go into an
 // open-addressing table keyed by position.  With no shared position and no
 // dropped face the fix-up has nothing to do.
-__global__ void __launch_bounds__(256) k_sp_dupcheck(McParams p) {
-    const SpecialVertex *cv = p.sp->cand;
-    const int64_t n = min((int64_t) p.sp->n_cand, p.sp->cap_cand);
+// Duplicate check of the weld candidates: every candidate's position goes
+// into the dslots hash; two candidates at one position set sp->dup (the weld
+// fix-up is skipped when none do).  Candidates are read from their segments
+// (index s * cap[0] + j), so the check does not wait for the gather: both
+// run in one launch.
+__device__ __forceinline__ void sp_dup_insert(const McParams &p, int32_t i) {
+    const SpecialVertex *cv = p.seg.cand;
+    const SpecialVertex v = cv[i];
     const uint64_t mask = (uint64_t) p.dmask;
-    for (int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
-        const SpecialVertex v = cv[i];
-        const uint64_t h = f3_hash(v.x, v.y, v.z) & mask;
-        for (uint64_t probe = 0; probe <= mask; ++probe) {
-            const int64_t slot = (int64_t) ((h + probe) & mask);
-            int32_t cur = p.dslots[slot];
-            if (cur < 0) {
-                const int32_t prev = atomicCAS(&p.dslots[slot], -1, (int32_t) i);
-                if (prev < 0) break;
-                cur = prev;
-            }
-            const SpecialVertex o = cv[cur];
-            if (o.x == v.x && o.y == v.y && o.z == v.z) {
-                atomicExch((unsigned long long *) &p.sp->dup, 1ull);
-                break;
-            }
+    const uint64_t h = f3_hash(v.x, v.y, v.z) & mask;
+    for (uint64_t probe = 0; probe <= mask; ++probe) {
+        const int64_t slot = (int64_t) ((h + probe) & mask);
+        int32_t cur = p.dslots[slot];
+        if (cur < 0) {
+            const int32_t prev = atomicCAS(&p.dslots[slot], -1, i);
+            if (prev < 0) break;
+            cur = prev;
         }
+        const SpecialVertex o = cv[cur];
+        if (o.x == v.x && o.y == v.y && o.z == v.z) {
+            atomicExch((unsigned long long *) &p.sp->dup, 1ull);
+            break;
+        }
+    }
+}
+
+// Workgroups [0, nseg) gather segment b of the special lists; the others
+// check the candidates for duplicate positions, segment by segment.
+__global__ void __launch_bounds__(256) k_sp_gather_dup(McParams p) {
+    const int nseg = p.seg.nseg;
+    if ((int) blockIdx.x < nseg) {
+        sp_gather_seg(p, blockIdx.x);
+        return;
+    }
+    const int nb = (int) gridDim.x - nseg, b = (int) blockIdx.x - nseg;
+    const int64_t cap0 = p.seg.cap[0];
+    for (int sg = b; sg < nseg; sg += nb) {
+        const int64_t n = min((int64_t) p.seg.ctr[(int64_t) sg * kSpStride], cap0);
+        for (int64_t j = threadIdx.x; j < n; j += 256) sp_dup_insert(p, (int32_t) ((int64_t) sg * cap0 + j));
     }
 }
 
@@ -2355,8 +2373,8 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(4)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sp_gather, dim3(p.seg.nseg), dim3(256), 0, stream, p);
-    if (p.dslots) hipLaunchKernelGGL(k_sp_dupcheck, dim3(64), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_sp_gather_dup, dim3((unsigned) p.seg.nseg + (p.dslots ? 64u : 0u)), dim3(256), 0, stream,
+                       p);
     return hipGetLastError();
 }
 
@@ -3532,8 +3550,8 @@ hipError_t mc_stream_launch(McParams &p, const SpecialLists &sp_init, hipStream_
     hipLaunchKernelGGL((k_faces<16, 16, 256>), dim3((unsigned) ntiles), dim3(256), 0, stream, p);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(4)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sp_gather, dim3(p.seg.nseg), dim3(256), 0, stream, p);
-    if (p.dslots) hipLaunchKernelGGL(k_sp_dupcheck, dim3(64), dim3(256), 0, stream, p);
+    hipLaunchKernelGGL(k_sp_gather_dup, dim3((unsigned) p.seg.nseg + (p.dslots ? 64u : 0u)), dim3(256), 0, stream,
+                       p);
     return hipGetLastError();
 }
 
