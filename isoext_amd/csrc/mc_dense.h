@@ -110,6 +110,10 @@ struct McParams {
     // unused when set
     const CsgProgram *csg;
     int csg_list;      // the classify's undecided-block list fits in cnt_v (see csg_list_fits)
+    // nblk <= 256 (single grids): every emit workgroup scans the block totals
+    // itself, and workgroup 0 writes blk_v / blk_f and the mesh size (no
+    // separate scan launch)
+    int scan_in_emit;
     // chunk index -> (row, chunk) -> (x, y) and chunk -> z-segment by
     // multiply-high, valid when narrow (X * Y * W < 2^32); set by
     // mc_params_divisors once W, Y and nzc are final

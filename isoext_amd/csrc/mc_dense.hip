@@ -1143,6 +1143,58 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
     const int tid = threadIdx.x;
     if (*(volatile int *) p.err & ISO_ERR_TOO_LARGE) return;
 
+    // small grids (nblk <= NT): the block totals are scanned here, into the
+    // LDS of the id table (used only by the batches below, after the bases
+    // are read); workgroup 0 also publishes the bases and the mesh size
+    const int64_t *bv_tab = p.blk_v, *bf_tab = p.blk_f;
+    if (p.scan_in_emit) {
+        static_assert(12 * NT * sizeof(int32_t) >= 2 * NT * sizeof(int64_t), "scan table in the id table");
+        __shared__ int64_t wsa[NT / 64], wsb[NT / 64];
+        int64_t *tab = reinterpret_cast<int64_t *>(&s.ids[0][0]);
+        const int lane = tid & 63, wave = tid >> 6;
+        const int64_t a = tid < p.nblk ? (int64_t) p.bsum_v[tid] : 0, b = tid < p.nblk ? (int64_t) p.bsum_f[tid] : 0;
+        int64_t ia = a, ib = b;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t ya = __shfl_up(ia, o), yb = __shfl_up(ib, o);
+            if (lane >= o) {
+                ia += ya;
+                ib += yb;
+            }
+        }
+        if (lane == 63) {
+            wsa[wave] = ia;
+            wsb[wave] = ib;
+        }
+        __syncthreads();
+        int64_t ra = ia - a, rb = ib - b, ta = 0, tb = 0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) {
+            if (w < wave) {
+                ra += wsa[w];
+                rb += wsb[w];
+            }
+            ta += wsa[w];
+            tb += wsb[w];
+        }
+        tab[tid] = ra;
+        tab[NT + tid] = rb;
+        if (blockIdx.x == 0) {
+            if (tid < p.nblk) {
+                p.blk_v[tid] = ra;
+                p.blk_f[tid] = rb;
+            }
+            if (tid == 0) {
+                p.totals[0] = ta;
+                p.totals[1] = tb;
+                if (ta > 0x7fffffffll || tb > 0x7fffffffll) set_err(p, ISO_ERR_TOO_LARGE);
+            }
+        }
+        __syncthreads();
+        bv_tab = tab;
+        bf_tab = tab + NT;
+    }
+
     const int64_t tk = tile % p.NK, tj = (tile / p.NK) % p.NJ, ti = tile / (p.NK * p.NJ);
     const int64_t x0 = ti * TX, y0 = tj * TY, c0 = tk * NZC, z0 = 64 * c0;
     const int nx = (int) min<int64_t>(TX, p.X - 1 - x0);
@@ -1170,7 +1222,7 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
         fbase_pf[k] = 0;
         if (i < TX * TY && cx < nx && cy < ny) {
             const int64_t e = ((x0 + cx) * p.Y + (y0 + cy)) * p.S + tk;
-            fbase_pf[k] = p.blk_f[e / kMcScanBlock] + (int64_t) p.cnt_f[e];
+            fbase_pf[k] = bf_tab[e / kMcScanBlock] + (int64_t) p.cnt_f[e];
         }
     }
 #pragma unroll
@@ -1179,8 +1231,8 @@ __device__ __forceinline__ void emit_tile(const McParams &p, int64_t tile, int s
         vbase_pf[k] = vnext_pf[k] = 0;
         if (i < (nx + 1) * (ny + 1)) {
             const int64_t e = ((x0 + px) * p.Y + (y0 + py)) * p.S + tk;
-            vbase_pf[k] = p.blk_v[e / kMcScanBlock] + (int64_t) p.cnt_v[e];
-            if (tk + 1 < p.S) vnext_pf[k] = p.blk_v[(e + 1) / kMcScanBlock] + (int64_t) p.cnt_v[e + 1];
+            vbase_pf[k] = bv_tab[e / kMcScanBlock] + (int64_t) p.cnt_v[e];
+            if (tk + 1 < p.S) vnext_pf[k] = bv_tab[(e + 1) / kMcScanBlock] + (int64_t) p.cnt_v[e + 1];
         }
     }
     for (int c = tid; c < TX + TY + 64 * NZC + 3; c += NT) {
@@ -2351,8 +2403,13 @@ hipError_t mc_dense_launch(int cfg, McParams &p, const SpecialLists &sp_init, hi
         hipLaunchKernelGGL(k_count<256>, dim3((unsigned) p.nblk), dim3(256), 0, stream, p);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = mark(2)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, p);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // up to 256 count blocks the emit workgroups scan the block totals
+    // themselves: one launch fewer on small grids
+    p.scan_in_emit = p.nblk <= 256 ? 1 : 0;
+    if (!p.scan_in_emit) {
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, p);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     if ((e = mark(3)) != hipSuccess) return e;
 
     const unsigned tiles = (unsigned) (p.NI * p.NJ * p.NK);
