@@ -19,6 +19,7 @@ _HERE = Path(__file__).resolve().parent
 TABLE = _HERE / "autotune.json"
 _lock = threading.Lock()
 _table = None
+_by_shape: dict = {}   # shape -> configuration (cleared when the table changes)
 
 # (TX, TY, NZC) of each compiled configuration, in library order
 # (configuration 6 is the read-once pipeline: k_stream regions + 16 x 16 face tiles)
@@ -53,14 +54,22 @@ def _load():
 
 
 def mc_config(shape) -> int:
+    key = tuple(int(s) for s in shape)
+    cfg = _by_shape.get(key)
+    if cfg is not None:
+        return cfg
     with _lock:
         t = _load()
-        hit = t.get(size_class(shape))
+        hit = t.get(size_class(key))
     if hit is not None:
-        return int(hit)
-    from . import _lib
-    L = _lib.lib()
-    return int(L.isoext_mc_auto_config(_lib.i64x3([int(s) for s in shape])))
+        cfg = int(hit)
+    else:
+        from . import _lib
+        cfg = int(_lib.lib().isoext_mc_auto_config(_lib.i64x3(key)))
+    if len(_by_shape) > 4096:
+        _by_shape.clear()
+    _by_shape[key] = cfg
+    return cfg
 
 
 def tune(shape, values=None, level=0.0, reps=5, write=True):
@@ -92,4 +101,5 @@ def tune(shape, values=None, level=0.0, reps=5, write=True):
             t = _load()
             t[size_class(shape)] = best
             TABLE.write_text(json.dumps(t, indent=1, sort_keys=True) + "\n")
+            _by_shape.clear()
     return best, results

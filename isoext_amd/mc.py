@@ -41,8 +41,7 @@ def _method_id(method: str) -> int:
 
 def _initial_caps(shape):
     X, Y, Z = shape
-    with _hint_lock:
-        hint = _size_hint.get(tuple(shape))
+    hint = _size_hint.get(tuple(shape))   # a single dict read needs no lock
     if hint is not None:
         return hint
     # surface-like estimate: a few times the largest cross-section, at least 64K
@@ -52,6 +51,17 @@ def _initial_caps(shape):
 
 
 _ws_bytes_cache: dict = {}
+_abi_args: dict = {}   # (shape, aabb_min, aabb_max) -> the ctypes arrays of a call
+
+
+def _abi_arrays(shape, aabb_min, aabb_max):
+    key = (shape, tuple(aabb_min), tuple(aabb_max))
+    a = _abi_args.get(key)
+    if a is None:
+        if len(_abi_args) > 4096:
+            _abi_args.clear()
+        a = _abi_args[key] = (_lib.i64x3(shape), _lib.f32x3(aabb_min), _lib.f32x3(aabb_max))
+    return a
 
 
 def mc_dense_raw(values: torch.Tensor | None, shape, aabb_min, aabb_max, level: float, method: int,
@@ -78,11 +88,10 @@ def mc_dense_raw(values: torch.Tensor | None, shape, aabb_min, aabb_max, level: 
         # fit its LDS cache (every corner evaluated once)
         cfg = PROGRAM_CFG if program is not None else autotune.mc_config(shape)
     L = _lib.lib()
-    with _hint_lock:
-        special_cap = max(DEFAULT_SPECIAL_CAP, _special_hint.get(shape, 0))
+    special_cap = max(DEFAULT_SPECIAL_CAP, _special_hint.get(shape, 0))
     cap_v, cap_f = _initial_caps(shape)
     counts = _lib.I64x6()
-    shape_c, amin_c, amax_c = _lib.i64x3(shape), _lib.f32x3(aabb_min), _lib.f32x3(aabb_max)
+    shape_c, amin_c, amax_c = _abi_arrays(shape, aabb_min, aabb_max)
     stream = _lib.stream_ptr(dev)
     for attempt in range(4):
         key = (shape, cfg, special_cap)
@@ -92,7 +101,7 @@ def mc_dense_raw(values: torch.Tensor | None, shape, aabb_min, aabb_max, level: 
             _ws_bytes_cache[key] = ws_bytes
         if ws_bytes < 0:
             raise ValueError(f"invalid tile config {cfg}")
-        ws = workspace.get("mc_dense", ws_bytes, dev)
+        ws = workspace.get("mc_dense", ws_bytes, dev, stream)
         out_v = torch.empty((cap_v, 3), dtype=torch.float32, device=dev)
         out_f = torch.empty((cap_f, 3), dtype=torch.int32, device=dev)
         keys = torch.empty(cap_v, dtype=torch.int64, device=dev) if want_keys else None
@@ -123,10 +132,12 @@ def mc_dense_raw(values: torch.Tensor | None, shape, aabb_min, aabb_max, level: 
         break
     else:
         raise _lib.IsoextError("marching_cubes: output sizing did not converge")
-    with _hint_lock:
-        _size_hint[tuple(shape)] = (max(int(counts[4]), 1), max(int(counts[5]), 1))
-        if special_cap > DEFAULT_SPECIAL_CAP:   # fields with many corner-exact crossings keep their need
-            _special_hint[tuple(shape)] = special_cap
+    hint = (max(int(counts[4]), 1), max(int(counts[5]), 1))
+    if _size_hint.get(shape) != hint or special_cap > DEFAULT_SPECIAL_CAP:
+        with _hint_lock:
+            _size_hint[shape] = hint
+            if special_cap > DEFAULT_SPECIAL_CAP:   # fields with many corner-exact crossings keep their need
+                _special_hint[shape] = max(special_cap, _special_hint.get(shape, 0))
     nv, nf = int(counts[0]), int(counts[1])
     v, f = out_v[:nv], out_f[:nf]
     # do not pin a much larger allocation behind a small mesh

@@ -26,8 +26,11 @@ def _buffers() -> dict:
     return d
 
 
-def get(kind: str, nbytes: int, device: torch.device) -> torch.Tensor:
-    stream = torch.cuda.current_stream(device).cuda_stream
+def get(kind: str, nbytes: int, device: torch.device, stream: int | None = None) -> torch.Tensor:
+    """stream: the current stream's handle on `device`, when the caller has
+    it already."""
+    if stream is None:
+        stream = torch.cuda.current_stream(device).cuda_stream
     key = (kind, device.index, stream)
     bufs = _buffers()
     buf = bufs.get(key)
