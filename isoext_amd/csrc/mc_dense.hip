@@ -82,7 +82,7 @@ __device__ __forceinline__ void reset_state(const McParams &p, SpecialLists init
         init.n_cand = init.n_scell = init.n_kept = init.n_hole = 0;
         init.need = 0;
         init.dup = p.dslots ? 0 : -1;
-        *p.sp = init;
+        if (p.sp) *p.sp = init;   // the intersection pipeline keeps no special lists
     }
 }
 
@@ -2439,10 +2439,17 @@ hipError_t its_dense_count_launch(McParams &p, hipStream_t stream) {
     mc_params_divisors(p);
     hipError_t e = mc_upload_luts();
     if (e != hipSuccess) return e;
-    const unsigned rblocks =
-        p.tile_flag ? (unsigned) std::max<int64_t>(1, std::min<int64_t>((p.NI * p.NJ * p.NK + 255) / 256, 1024)) : 1;
-    hipLaunchKernelGGL(k_its_reset, dim3(rblocks), dim3(256), 0, stream, p);
-    launch_classify(p, stream);
+    // a stored grid's classify clears the state itself (reset_state covers
+    // what k_its_reset clears)
+    const bool grid_field = !p.csg && p.nsph <= 0;
+    if (!grid_field) {
+        const unsigned rblocks = p.tile_flag ? (unsigned) std::max<int64_t>(
+                                                   1, std::min<int64_t>((p.NI * p.NJ * p.NK + 255) / 256, 1024))
+                                             : 1;
+        hipLaunchKernelGGL(k_its_reset, dim3(rblocks), dim3(256), 0, stream, p);
+    }
+    const SpecialLists none{};
+    launch_classify(p, stream, grid_field ? &none : nullptr);
     if (p.nblk < kCountWideBelow)
         hipLaunchKernelGGL(k_count<1024>, dim3((unsigned) p.nblk), dim3(1024), 0, stream, p);
     else
