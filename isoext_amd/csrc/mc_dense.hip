@@ -1814,15 +1814,18 @@ __device__ __forceinline__ int find_start(const int64_t *st, int n, int64_t x) {
     return lo;
 }
 
-__global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, const McBatchHdr *hdr) {
+// Clears a batch's state (tile flags and counts, every item's special-list
+// counters, error flag, totals, program block-list count and list
+// descriptor); thread t0 of a grid of `stride` threads.  Nothing the batch
+// classify does reads it, so the classify kernel runs this first.
+__device__ __forceinline__ void reset_batch_state(const McBatchItem *items, const McBatchHdr *hdr, int64_t t0,
+                                                  int64_t stride) {
     const int n = (int) hdr->n;
     if (hdr->tile_flag) {
-        for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < hdr->ntiles; t += (int64_t) gridDim.x * 256)
-            hdr->tile_flag[t] = 0;
-        if (blockIdx.x == 0 && threadIdx.x < kMaxTileConfigs) hdr->tile_count[threadIdx.x] = 0;
+        for (int64_t t = t0; t < hdr->ntiles; t += stride) hdr->tile_flag[t] = 0;
+        if (t0 < kMaxTileConfigs) hdr->tile_count[t0] = 0;
     }
-    for (int64_t t = (int64_t) blockIdx.x * 256 + threadIdx.x; t < (int64_t) n * 4 * kSpSeg;
-         t += (int64_t) gridDim.x * 256) {
+    for (int64_t t = t0; t < (int64_t) n * 4 * kSpSeg; t += stride) {
         const int i = (int) (t / (4 * kSpSeg)), c = (int) (t % (4 * kSpSeg));
         const McParams &p = items[i].p;
         if (c < 4 * p.seg.nseg) p.seg.ctr[c * kSpStride] = 0;
@@ -1839,11 +1842,13 @@ __global__ void __launch_bounds__(256) k_reset_batch(const McBatchItem *items, c
     }
 }
 
-// Flat classify tasks over the batch (each grid owns a range of whole tasks).
+// Flat classify tasks over the batch (each grid owns a range of whole
+// tasks), after the batch's state is cleared.
 __global__ void __launch_bounds__(256) k_classify_batch(const McBatchItem *__restrict__ items,
                                                         const McBatchHdr *__restrict__ hdr) {
     __shared__ int64_t starts[kMaxBatch];
     __shared__ uint64_t ballots[4][kFlatChunks + 1];
+    reset_batch_state(items, hdr, (int64_t) blockIdx.x * 256 + threadIdx.x, (int64_t) gridDim.x * 256);
     const int n = (int) hdr->n;
     const int64_t ntask = hdr->nctask;
     for (int i = threadIdx.x; i < n; i += 256) starts[i] = items[i].ctask0;
@@ -2478,9 +2483,7 @@ hipError_t mc_batch_count_launch(const McBatchItem *items, const McBatchHdr *hdr
                                  McBatchSummary *summary, hipStream_t stream) {
     hipError_t e = mc_upload_luts();
     if (e != hipSuccess) return e;
-    const unsigned rb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.n * 4 * kSpSeg + 255) / 256, 1024));
-    hipLaunchKernelGGL(k_reset_batch, dim3(rb), dim3(256), 0, stream, items, hdr);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // the classify kernel clears the batch's state first (no reset launch)
     const unsigned cb = (unsigned) std::max<int64_t>(1, std::min<int64_t>((h.nctask + 3) / 4, 8192));
     hipLaunchKernelGGL(k_classify_batch, dim3(cb), dim3(256), 0, stream, items, hdr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
